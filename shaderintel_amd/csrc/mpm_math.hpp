@@ -1,0 +1,181 @@
+// Device math for the MLS-MPM kernels (gfx950).
+//
+// Every expression is evaluated in a fixed, documented order so that the GPU result is a pure
+// function of the input bits: no contraction into FMA (the library is compiled with
+// -ffp-contract=off), IEEE division and square root (HIP's default correctly rounded fp32
+// div/sqrt), mat*vec as ((m0*v.x + m1*v.y) + m2*v.z), dot as ((a.x*b.x + a.y*b.y) + a.z*b.z),
+// WGSL clamp(e, lo, hi) = min(max(e, lo), hi), WGSL round = round-half-to-even (rint), and
+// inverseSqrt read as 1/sqrt. exp and log are the self-contained Cody-Waite routines below, so
+// no vendor math library decides any bit of the result.
+//
+// The WGSL being restated is src/physics/mpm.js:64-78 (module `mpm`) and
+// src/shaders/linalg.js:8-69 (module `linalg`).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mpm {
+
+#define MPM_HD __host__ __device__ __forceinline__
+
+struct f3 { float x, y, z; };
+struct m3 { f3 c[3]; };  // column-major like WGSL mat3x3f: c[col], component = row
+
+MPM_HD f3 mk(float x, float y, float z) { f3 r; r.x = x; r.y = y; r.z = z; return r; }
+MPM_HD f3 operator+(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+MPM_HD f3 operator-(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+MPM_HD f3 operator*(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+MPM_HD f3 operator*(float s, f3 a) { return mk(s * a.x, s * a.y, s * a.z); }
+MPM_HD f3 operator/(f3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+MPM_HD float dot3(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+MPM_HD float len3(f3 a) { return sqrtf(dot3(a, a)); }
+MPM_HD f3 norm3(f3 a) { return a / len3(a); }
+MPM_HD f3 cross3(f3 a, f3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+MPM_HD float clampf(float e, float lo, float hi) { return fminf(fmaxf(e, lo), hi); }
+
+MPM_HD m3 mat(f3 a, f3 b, f3 c) { m3 m; m.c[0] = a; m.c[1] = b; m.c[2] = c; return m; }
+MPM_HD m3 ident() { return mat(mk(1, 0, 0), mk(0, 1, 0), mk(0, 0, 1)); }
+MPM_HD m3 zero3() { return mat(mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)); }
+MPM_HD f3 mul(const m3& m, f3 v) { return (m.c[0] * v.x + m.c[1] * v.y) + m.c[2] * v.z; }
+MPM_HD m3 mul(const m3& a, const m3& b) { return mat(mul(a, b.c[0]), mul(a, b.c[1]), mul(a, b.c[2])); }
+MPM_HD m3 operator+(const m3& a, const m3& b) { return mat(a.c[0] + b.c[0], a.c[1] + b.c[1], a.c[2] + b.c[2]); }
+MPM_HD m3 operator-(const m3& a, const m3& b) { return mat(a.c[0] - b.c[0], a.c[1] - b.c[1], a.c[2] - b.c[2]); }
+MPM_HD m3 operator*(float s, const m3& a) { return mat(s * a.c[0], s * a.c[1], s * a.c[2]); }
+MPM_HD m3 tr(const m3& m) {
+  return mat(mk(m.c[0].x, m.c[1].x, m.c[2].x), mk(m.c[0].y, m.c[1].y, m.c[2].y), mk(m.c[0].z, m.c[1].z, m.c[2].z));
+}
+MPM_HD float trace(const m3& m) { return (m.c[0].x + m.c[1].y) + m.c[2].z; }
+MPM_HD m3 diag(f3 d) { return mat(mk(d.x, 0, 0), mk(0, d.y, 0), mk(0, 0, d.z)); }
+MPM_HD m3 outer(f3 a, f3 b) { return mat(a * b.x, a * b.y, a * b.z); }
+MPM_HD float det3(const m3& m) { return dot3(m.c[0], cross3(m.c[1], m.c[2])); }
+
+// ---- elementary functions (Cody & Waite 1980 reduction + Taylor / atanh series in float) ----
+MPM_HD float bits2f(uint32_t u) { return __builtin_bit_cast(float, u); }
+MPM_HD uint32_t f2bits(float f) { return __builtin_bit_cast(uint32_t, f); }
+
+MPM_HD float exp_cw(float x) {
+  const float kLn2Hi = 0.693145751953125f;         // 16 significant bits: k * kLn2Hi exact
+  const float kLn2Lo = 1.428606765330187045e-06f;
+  const float kLog2e = 1.44269504088896340736f;
+  if (x != x) return x;
+  if (x > 88.7f) return bits2f(0x7f800000u);
+  if (x < -103.9f) return 0.0f;
+  float kf = rintf(x * kLog2e);
+  int k = (int)kf;
+  float r = (x - kf * kLn2Hi) - kf * kLn2Lo;
+  float p = 1.0f / 5040.0f;
+  p = p * r + 1.0f / 720.0f;
+  p = p * r + 1.0f / 120.0f;
+  p = p * r + 1.0f / 24.0f;
+  p = p * r + 1.0f / 6.0f;
+  p = p * r + 0.5f;
+  p = p * r + 1.0f;
+  p = p * r + 1.0f;
+  int k1 = k / 2, k2 = k - k1;
+  return (p * bits2f((uint32_t)(k1 + 127) << 23)) * bits2f((uint32_t)(k2 + 127) << 23);
+}
+
+MPM_HD float log_cw(float x) {
+  const float kLn2Hi = 0.693145751953125f;
+  const float kLn2Lo = 1.428606765330187045e-06f;
+  if (x != x) return x;
+  if (x < 0.0f) return bits2f(0x7fc00000u);
+  if (x == 0.0f) return bits2f(0xff800000u);
+  if (x == bits2f(0x7f800000u)) return x;
+  int e = 0;
+  uint32_t u = f2bits(x);
+  if ((u >> 23) == 0) { x = x * 16777216.0f; u = f2bits(x); e = -24; }
+  e += (int)(u >> 23) - 127;
+  float m = bits2f((u & 0x007fffffu) | 0x3f800000u);
+  if (m > 1.41421356f) { m = m * 0.5f; e += 1; }
+  float f = m - 1.0f;
+  float s = f / (2.0f + f);
+  float s2 = s * s;
+  float p = 1.0f / 11.0f;
+  p = p * s2 + 1.0f / 9.0f;
+  p = p * s2 + 1.0f / 7.0f;
+  p = p * s2 + 1.0f / 5.0f;
+  p = p * s2 + 1.0f / 3.0f;
+  p = p * s2 + 1.0f;
+  float lm = (2.0f * s) * p;
+  float ef = (float)e;
+  return ef * kLn2Hi + (lm + ef * kLn2Lo);
+}
+
+// ---- mpm module: fixed point, B-spline, Morton ----
+constexpr float kFP = 4194304.0f;           // 2^22
+constexpr float kInvFP = 1.0f / 4194304.0f;
+
+MPM_HD int32_t to_fixed(float x) { return (int32_t)rintf(clampf(x * kFP, -2.0e9f, 2.0e9f)); }
+
+MPM_HD void bspline(f3 fx, f3 w[3]) {
+  f3 t0 = mk(1.5f, 1.5f, 1.5f) - fx;
+  f3 t1 = fx - mk(1.0f, 1.0f, 1.0f);
+  f3 t2 = fx - mk(0.5f, 0.5f, 0.5f);
+  w[0] = mk((0.5f * t0.x) * t0.x, (0.5f * t0.y) * t0.y, (0.5f * t0.z) * t0.z);
+  w[1] = mk(0.75f - t1.x * t1.x, 0.75f - t1.y * t1.y, 0.75f - t1.z * t1.z);
+  w[2] = mk((0.5f * t2.x) * t2.x, (0.5f * t2.y) * t2.y, (0.5f * t2.z) * t2.z);
+}
+
+MPM_HD uint32_t spread3(uint32_t x) {  // 10-bit interleave helper of morton3
+  uint32_t v = x & 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+MPM_HD uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) { return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2); }
+
+// ---- 3x3 symmetric eigen-decomposition and SVD (cyclic Jacobi, 4 sweeps) ----
+MPM_HD float& el(m3& m, int c, int r) { return r == 0 ? m.c[c].x : (r == 1 ? m.c[c].y : m.c[c].z); }
+MPM_HD float el(const m3& m, int c, int r) { return r == 0 ? m.c[c].x : (r == 1 ? m.c[c].y : m.c[c].z); }
+
+template <int P, int Q>
+MPM_HD void jacobi(m3& A, m3& V) {
+  float apq = el(A, Q, P);
+  if (fabsf(apq) < 1e-30f) return;
+  float theta = (el(A, Q, Q) - el(A, P, P)) / (2.0f * apq);
+  float sgn = theta >= 0.0f ? 1.0f : -1.0f;
+  float t = sgn / (fabsf(theta) + sqrtf(theta * theta + 1.0f));
+  float c = 1.0f / sqrtf(t * t + 1.0f);
+  float s = t * c;
+  m3 J = ident();
+  el(J, P, P) = c; el(J, Q, Q) = c; el(J, Q, P) = s; el(J, P, Q) = -s;
+  A = mul(mul(tr(J), A), J);
+  V = mul(V, J);
+}
+
+MPM_HD void svd3(const m3& F, m3& U, f3& s, m3& V) {
+  m3 A = mul(tr(F), F);
+  V = ident();
+#pragma unroll 1
+  for (int sweep = 0; sweep < 4; sweep++) {
+    jacobi<0, 1>(A, V);
+    jacobi<0, 2>(A, V);
+    jacobi<1, 2>(A, V);
+  }
+  f3 e = mk(A.c[0].x, A.c[1].y, A.c[2].z);
+  if (e.x < e.y) { e = mk(e.y, e.x, e.z); V = mat(V.c[1], V.c[0], -1.0f * V.c[2]); }
+  if (e.x < e.z) { e = mk(e.z, e.y, e.x); V = mat(V.c[2], -1.0f * V.c[1], V.c[0]); }
+  if (e.y < e.z) { e = mk(e.x, e.z, e.y); V = mat(-1.0f * V.c[0], V.c[2], V.c[1]); }
+  if (det3(V) < 0.0f) V.c[2] = -1.0f * V.c[2];
+  s = mk(sqrtf(fmaxf(e.x, 0.0f)), sqrtf(fmaxf(e.y, 0.0f)), sqrtf(fmaxf(e.z, 0.0f)));
+  f3 u0 = mul(F, V.c[0]);
+  float l0 = len3(u0);
+  u0 = l0 > 1e-12f ? u0 / l0 : mk(1, 0, 0);
+  f3 u1 = mul(F, V.c[1]);
+  u1 = u1 - dot3(u1, u0) * u0;
+  float l1 = len3(u1);
+  if (l1 > 1e-12f) {
+    u1 = u1 / l1;
+  } else {
+    u1 = fabsf(u0.x) < 0.9f ? mk(1, 0, 0) : mk(0, 1, 0);
+    u1 = norm3(u1 - dot3(u1, u0) * u0);
+  }
+  f3 u2 = cross3(u0, u1);
+  if (dot3(mul(F, V.c[2]), u2) < 0.0f) s.z = -s.z;
+  U = mat(u0, u1, u2);
+}
+
+}  // namespace mpm

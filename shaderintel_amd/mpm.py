@@ -1,0 +1,449 @@
+"""Python mirror of ShaderIntel's ``MPM`` world API over the C ABI of ``libmpm_hip.so``.
+
+It keeps the reference's names, argument meanings and error behaviour (src/physics/mpm.js:82-680,
+types/shaderintel.d.ts:813-900): ``MPM(ctx, maxParticles=…, grid=…, size=…, …)``, ``add``,
+``clear``, ``setColliders``, ``step``, ``p2g``, ``_sort``, ``readPositions`` and the fields
+``count, dims, dx, origin, bound, numNodes, materials, ppcAxis, substeps, timeScale, sortEvery,
+gravity, substepDt, stepCount``. Errors are raised as :class:`SIError` with the reference's
+``code`` strings. Every method runs on the GPU through the HIP library; there is no CPU path and
+a missing library raises at import of the first context.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import math
+import os
+from typing import Iterable, Sequence
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "libmpm_hip.so")
+
+FLUID, JELLY, SNOW, SAND = 0, 1, 2, 3
+MPM_KINDS = {"fluid": FLUID, "jelly": JELLY, "elastic": JELLY, "snow": SNOW, "sand": SAND}
+MAX_DENSITY = 50  # mpm.js:34
+
+# mpm.js:38-46
+MATERIALS = {
+    "water": {"kind": FLUID, "density": 1, "stiffness": 1.0, "viscosity": 0.0, "color": [0.12, 0.42, 0.95]},
+    "oil": {"kind": FLUID, "density": 0.9, "stiffness": 0.9, "viscosity": 0.3, "color": [0.9, 0.62, 0.12]},
+    "honey": {"kind": FLUID, "density": 1.4, "stiffness": 0.9, "viscosity": 1.0, "color": [0.95, 0.6, 0.05]},
+    "jelly": {"kind": JELLY, "density": 1, "stiffness": 0.35, "poisson": 0.3, "color": [0.95, 0.3, 0.35]},
+    "rubber": {"kind": JELLY, "density": 1.2, "stiffness": 0.7, "poisson": 0.45, "color": [0.25, 0.25, 0.28]},
+    "snow": {"kind": SNOW, "density": 0.4, "stiffness": 0.6, "poisson": 0.2, "hardening": 10, "compression": 0.025,
+             "stretch": 0.0075, "color": [0.95, 0.96, 1.0]},
+    "sand": {"kind": SAND, "density": 1.6, "stiffness": 0.6, "poisson": 0.3, "friction": 33,
+             "color": [0.86, 0.68, 0.42]},
+}
+
+_CODES = {0: "ok", 1: "invalid-argument", 2: "unsupported", 3: "no-device", 4: "error", 5: "out-of-memory",
+          6: "destroyed"}
+
+
+class SIError(Exception):
+    """The reference's error class (src/core/errors.js:5-11): ``code`` is one of 'invalid-argument',
+    'unsupported', 'no-device', 'error', plus 'out-of-memory' and 'destroyed'."""
+
+    def __init__(self, msg: str, code: str = "error"):
+        super().__init__(msg if msg.startswith("ShaderIntel") else f"ShaderIntel: {msg}")
+        self.code = code
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int32 * 3), ("max_particles", ctypes.c_uint32), ("ppc_axis", ctypes.c_int32),
+                ("bound", ctypes.c_float), ("sort_every", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libmpm_hip.so and declare its signatures (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise SIError(f"libmpm_hip.so is not built ({path}); run `python -m shaderintel_amd.build`", "unsupported")
+    L = ctypes.CDLL(path)
+    vp, u32, i32, u64, f32, f64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64, ctypes.c_float, ctypes.c_double
+    P = ctypes.POINTER
+    sig = {
+        "mpm_ctx_create": (i32, [ctypes.c_int, P(vp)]),
+        "mpm_ctx_destroy": (None, [vp]),
+        "mpm_ctx_info": (i32, [vp, ctypes.c_char_p, ctypes.c_size_t, P(i32)]),
+        "mpm_sim_create": (i32, [vp, P(_Config), P(vp)]),
+        "mpm_sim_destroy": (None, [vp]),
+        "mpm_set_materials": (i32, [vp, vp, u32]),
+        "mpm_set_colliders": (i32, [vp, vp, u32]),
+        "mpm_write_particles": (i32, [vp, vp, u32, u32]),
+        "mpm_set_count": (i32, [vp, u32]),
+        "mpm_get_count": (i32, [vp, P(u32)]),
+        "mpm_set_step_count": (i32, [vp, u64]),
+        "mpm_get_step_count": (i32, [vp, P(u64)]),
+        "mpm_step": (i32, [vp, u32, vp, f32, f32]),
+        "mpm_p2g": (i32, [vp, i32]),
+        "mpm_sort": (i32, [vp]),
+        "mpm_rescale": (i32, [vp, f32]),
+        "mpm_read_particles": (i32, [vp, vp, u32, u32]),
+        "mpm_read_grid": (i32, [vp, vp]),
+        "mpm_sync": (i32, [vp]),
+        "mpm_set_profiling": (i32, [vp, i32]),
+        "mpm_phase_times": (i32, [vp, vp]),
+        "mpm_scan_u32": (i32, [vp, vp, vp, u64]),
+        "mpm_counting_sort": (i32, [vp, vp, u64, u32, vp, vp]),
+        "mpm_material_row": (None, [i32, f64, f64, f64, f64, f64, f64, f64, f64, i32, vp]),
+        "mpm_sample_shape": (ctypes.c_int64, [i32, vp, f64, f64, u32, vp, ctypes.c_int64]),
+        "mpm_pack_particles": (None, [vp, ctypes.c_int64, vp, f64, vp, f64, u32, vp, f64, f64, u32, vp]),
+        "mpm_status_string": (ctypes.c_char_p, [i32]),
+        "mpm_last_error_message": (ctypes.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(st: int, what: str = "") -> None:
+    if st != 0:
+        L = load_library()
+        msg = L.mpm_last_error_message().decode(errors="replace")
+        raise SIError(f"{what}: {msg}" if what else msg, _CODES.get(st, "error"))
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """A HIP device (stands in for the reference's WebGPU Context from SI.init, context.js:32-100)."""
+
+    def __init__(self, device: int = 0):
+        L = load_library()
+        h = ctypes.c_void_p()
+        _check(L.mpm_ctx_create(device, ctypes.byref(h)), "init")
+        self._h = h
+        buf = ctypes.create_string_buffer(256)
+        cus = ctypes.c_int32()
+        _check(L.mpm_ctx_info(h, buf, 256, ctypes.byref(cus)))
+        self.name = buf.value.decode()
+        self.cus = cus.value
+
+    def mpm(self, **o) -> "MPM":  # gpu.mpm(o), src/index.js:87
+        return MPM(self, **o)
+
+    def scan(self, a: np.ndarray) -> np.ndarray:
+        """Scan.run (src/prims/scan.js:292-306): exclusive prefix sum of u32 (wrapping)."""
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        out = np.empty_like(a)
+        _check(load_library().mpm_scan_u32(self._h, _ptr(a), _ptr(out), a.size), "Scan.run")
+        return out
+
+    def counting_sort(self, keys: np.ndarray, num_buckets: int) -> tuple[np.ndarray, np.ndarray]:
+        """SpatialGrid.build (src/prims/grid.js:70-79): returns (order, start)."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint32)
+        order = np.empty(keys.size, np.uint32)
+        start = np.empty(num_buckets + 1, np.uint32)
+        _check(load_library().mpm_counting_sort(self._h, _ptr(keys), keys.size, num_buckets, _ptr(order), _ptr(start)),
+               "SpatialGrid.build")
+        return order, start
+
+    def destroy(self) -> None:
+        if getattr(self, "_h", None):
+            load_library().mpm_ctx_destroy(self._h)
+            self._h = None
+
+
+def material_row(m: dict, ppc_axis: int) -> np.ndarray:
+    """_uploadMaterials row (mpm.js:170-196) computed by the library's host helper."""
+    out = np.zeros(12, np.float32)
+    load_library().mpm_material_row(
+        int(m["kind"]), float(m.get("density", 1) if m.get("density") is not None else 1),
+        float(m["stiffness"]) if m.get("stiffness") is not None else 1.0,
+        float(m["poisson"]) if m.get("poisson") is not None else 0.3,
+        float(m["viscosity"]) if m.get("viscosity") is not None else 0.0,
+        float(m["hardening"]) if m.get("hardening") is not None else 10.0,
+        float(m["compression"]) if m.get("compression") is not None else 0.025,
+        float(m["stretch"]) if m.get("stretch") is not None else 0.0075,
+        float(m["friction"]) if m.get("friction") is not None else 33.0, int(ppc_axis), _ptr(out))
+    return out
+
+
+def sample_shape(shape: dict, spacing: float, jitter: float, seed: int) -> np.ndarray:
+    """sampleShape (mpm.js:683-714) for box / sphere / cylinder; float32 xyz points (world)."""
+    if "box" in shape:
+        lo, hi = shape["box"]
+        kind, prm = 0, [*lo, *hi]
+    elif "sphere" in shape:
+        c, r = shape["sphere"]
+        kind, prm = 1, [*c, r]
+    elif "cylinder" in shape:
+        c, r, h = shape["cylinder"]
+        kind, prm = 2, [*c, r, h]
+    else:
+        raise SIError("MPM.add: shape needs box | sphere | cylinder | points", "invalid-argument")
+    p = np.asarray(prm + [0.0] * (6 - len(prm)), np.float64)
+    L = load_library()
+    n = L.mpm_sample_shape(kind, _ptr(p), spacing, jitter, seed & 0xFFFFFFFF, None, 0)
+    out = np.empty((max(n, 0), 3), np.float32)
+    if n > 0:
+        L.mpm_sample_shape(kind, _ptr(p), spacing, jitter, seed & 0xFFFFFFFF, _ptr(out), n)
+    return out
+
+
+class MPM:
+    """MLS-MPM simulation on one MI355X (src/physics/mpm.js:82-680)."""
+
+    def __init__(self, ctx: Context, maxParticles: int = 262144, grid: int = 64, size=1, origin=None, gravity=None,
+                 substeps: int = 8, timeScale: float = 1, sortEvery: int = 4, ppcAxis: int = 2, tileCap=None):
+        self.ctx = ctx
+        self.maxParticles = int(maxParticles or 262144)
+        size = list(size) if isinstance(size, (list, tuple)) else [size or 1] * 3
+        res = grid or 64
+        self.dx = max(size) / res
+        self.dims = [max(8, round(s / self.dx)) for s in size]
+        self.size = [d * self.dx for d in self.dims]
+        self.origin = list(origin or [0, 0, 0])
+        self.gravity = list(gravity or [0, -9.81, 0])
+        self._substeps = max(1, round(substeps or 8))
+        self._timeScale = 1 if timeScale is None else timeScale
+        self.sortEvery = sortEvery or 4
+        self.ppcAxis = ppcAxis or 2
+        self.bound = 3
+        self.materials: list[dict] = []
+        self._material_index: dict[str, int] = {}
+        self.colliders: list[dict] = []
+        self.destroyed = False
+        self.numNodes = self.dims[0] * self.dims[1] * self.dims[2]
+        cfg = _Config((ctypes.c_int32 * 3)(*self.dims), self.maxParticles, self.ppcAxis, float(self.bound),
+                      self.sortEvery, 0)
+        h = ctypes.c_void_p()
+        _check(load_library().mpm_sim_create(ctx._h, ctypes.byref(cfg), ctypes.byref(h)), "MPM")
+        self._h = h
+
+    # ---- fields ----
+    @property
+    def count(self) -> int:
+        n = ctypes.c_uint32()
+        _check(load_library().mpm_get_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    @property
+    def stepCount(self) -> int:
+        n = ctypes.c_uint64()
+        _check(load_library().mpm_get_step_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    @stepCount.setter
+    def stepCount(self, n: int) -> None:
+        _check(load_library().mpm_set_step_count(self._h, int(n)))
+
+    @property
+    def substepDt(self) -> float:  # mpm.js:130
+        return (1 / 60) * self.timeScale / self.substeps
+
+    @property
+    def substeps(self) -> int:
+        return self._substeps
+
+    @substeps.setter
+    def substeps(self, n) -> None:  # mpm.js:137
+        self._retime(lambda: setattr(self, "_substeps", max(1, round(n))))
+
+    @property
+    def timeScale(self) -> float:
+        return self._timeScale
+
+    @timeScale.setter
+    def timeScale(self, x) -> None:
+        self._retime(lambda: setattr(self, "_timeScale", max(1e-6, float(x))))
+
+    def _retime(self, change) -> None:  # mpm.js:140-145
+        before = self.substepDt
+        change()
+        k = self.substepDt / before
+        if self.count and abs(k - 1) > 1e-12:
+            _check(load_library().mpm_rescale(self._h, k), "MPM.retime")
+
+    # ---- materials (mpm.js:148-199) ----
+    def _material(self, name_or_def) -> int:
+        if isinstance(name_or_def, str):
+            if name_or_def not in MATERIALS:
+                raise SIError(f"MPM: unknown material {json.dumps(name_or_def)} (use {', '.join(MATERIALS)} or a "
+                              "definition with a kind)", "invalid-argument")
+            d = {**MATERIALS[name_or_def], "name": name_or_def}
+        else:
+            d = dict(name_or_def or {})
+        if d.get("kind") is None:
+            raise SIError(f"MPM: unknown material {json.dumps(name_or_def)}", "invalid-argument")
+        if isinstance(d["kind"], str):
+            if d["kind"] not in MPM_KINDS:
+                raise SIError(f'MPM: unknown material kind "{d["kind"]}" (use {", ".join(MPM_KINDS)})', "invalid-argument")
+            d["kind"] = MPM_KINDS[d["kind"]]
+        if d["kind"] not in (FLUID, JELLY, SNOW, SAND):
+            raise SIError(f"MPM: material kind must be 0–3 (got {d['kind']})", "invalid-argument")
+        density = d.get("density") if d.get("density") is not None else 1
+        if not (density > 0 and density <= MAX_DENSITY):
+            raise SIError(f"MPM: density {density} is outside (0, {MAX_DENSITY}] — densities are relative to water (1), "
+                          "not kg/m³", "invalid-argument")
+        key = json.dumps(d)
+        if key in self._material_index:
+            return self._material_index[key]
+        if len(self.materials) >= 8:
+            raise SIError("MPM: at most 8 distinct materials at once (clear() frees them)", "invalid-argument")
+        idx = len(self.materials)
+        self.materials.append(d)
+        self._material_index[key] = idx
+        self._upload_materials()
+        return idx
+
+    def material_table(self) -> np.ndarray:
+        tbl = np.zeros((8, 12), np.float32)
+        for k, m in enumerate(self.materials):
+            tbl[k] = material_row(m, self.ppcAxis)
+        return tbl
+
+    def _upload_materials(self) -> None:
+        tbl = self.material_table()
+        _check(load_library().mpm_set_materials(self._h, _ptr(tbl), 8), "MPM materials")
+
+    # ---- emission (mpm.js:207-232) ----
+    def add(self, shape: dict, material=None, velocity=None, jitter=None, seed=None, noise: float = 0.0,
+            noiseSeed=None) -> int:
+        """Fill a shape with ppcAxis^3 particles per cell. noise: std-dev of Gaussian velocity noise
+        (world units/s) — an extension the reference does not have."""
+        self._alive("MPM.add")
+        mat = self._material(material or shape.get("material") or "water")
+        vel = velocity or shape.get("velocity") or [0, 0, 0]
+        count = self.count
+        if "points" in shape:
+            pts = np.asarray(shape["points"], np.float32).reshape(-1, 3)
+        else:
+            pts = sample_shape(shape, self.dx / self.ppcAxis, 0.6 if jitter is None else jitter,
+                               count + 1 if seed is None else seed)
+        n = min(len(pts), self.maxParticles - count)
+        if n <= 0:
+            return 0
+        pts = np.ascontiguousarray(pts[:n])
+        rec = np.zeros((n, 32), np.float32)
+        s = self.substepDt / self.dx
+        load_library().mpm_pack_particles(
+            _ptr(pts), n, _ptr(np.asarray(self.origin, np.float64)), self.dx, _ptr(np.asarray(self.dims, np.int32)),
+            float(self.bound), mat, _ptr(np.asarray(vel, np.float64)), s, float(noise),
+            (noiseSeed if noiseSeed is not None else (count + 1) * 2654435761) & 0xFFFFFFFF, _ptr(rec))
+        self.write_particles(rec, count)
+        _check(load_library().mpm_set_count(self._h, count + n))
+        return n
+
+    def clear(self) -> None:  # mpm.js:238-242
+        _check(load_library().mpm_set_count(self._h, 0))
+        self.materials = []
+        self._material_index.clear()
+
+    def setColliders(self, lst: list[dict]) -> None:  # mpm.js:274
+        self.colliders = list(lst)
+
+    def collider_table(self) -> np.ndarray:  # mpm.js:276-293
+        data = np.zeros(16 * 12, np.float32)
+        u = data.view(np.uint32)
+        s = self.substepDt / self.dx
+        for k, c in enumerate(self.colliders[:16]):
+            o = k * 12
+            typ = c.get("type") or "sphere"
+            u[o] = {"sphere": 0, "box": 1, "plane": 2}[typ]
+            ctr = c.get("center") or [0, 0, 0]
+            for a in range(3):
+                data[o + 1 + a] = (ctr[a] - self.origin[a]) / self.dx
+            if typ == "plane":
+                nrm = c.get("normal") or [0, 1, 0]
+                data[o + 4:o + 7] = nrm
+            elif typ == "box":
+                h = c.get("half") or [0.1, 0.1, 0.1]
+                data[o + 4:o + 7] = [h[0] / self.dx, h[1] / self.dx, h[2] / self.dx]
+            else:
+                data[o + 4] = (c.get("radius") if c.get("radius") is not None else 0.1) / self.dx
+            data[o + 7] = c.get("friction") if c.get("friction") is not None else 0.3
+            v = c.get("velocity") or [0, 0, 0]
+            data[o + 8:o + 11] = [v[0] * s, v[1] * s, v[2] * s]
+        return data
+
+    def lattice_gravity(self) -> np.ndarray:  # mpm.js:588, f64 -> f32
+        dt = self.substepDt
+        return np.array([(x * dt * dt) / self.dx for x in self.gravity], np.float32)
+
+    # ---- stepping (mpm.js:582-621) ----
+    def step(self) -> None:
+        if self.destroyed:
+            raise SIError("MPM.step: it was destroyed", "destroyed")
+        if self.count == 0:
+            return
+        L = load_library()
+        tbl = self.collider_table()
+        _check(L.mpm_set_colliders(self._h, _ptr(tbl), min(16, len(self.colliders))), "MPM colliders")
+        g = self.lattice_gravity()
+        _check(L.mpm_step(self._h, self.substeps, _ptr(g), 0.4, 0.9), "MPM.step")
+
+    def p2g(self, force_global: bool = False) -> None:
+        self._alive("MPM.p2g")
+        _check(load_library().mpm_p2g(self._h, 1 if force_global else 0), "MPM.p2g")
+
+    def _sort(self) -> None:
+        self._alive("MPM._sort")
+        _check(load_library().mpm_sort(self._h), "MPM._sort")
+
+    # ---- data ----
+    def write_particles(self, rec: np.ndarray, offset: int = 0) -> None:
+        """P.write (records of 32 float32, reference layout)."""
+        self._alive("MPM.P.write")
+        rec = np.ascontiguousarray(rec, np.float32).reshape(-1, 32)
+        _check(load_library().mpm_write_particles(self._h, _ptr(rec), offset, rec.shape[0]), "MPM.P.write")
+
+    def read_particles(self, n: int | None = None, offset: int = 0) -> np.ndarray:
+        """P.read (blocking): (n, 32) float32 records in the reference layout."""
+        self._alive("MPM.P.read")
+        n = self.count if n is None else n
+        out = np.empty((n, 32), np.float32)
+        _check(load_library().mpm_read_particles(self._h, _ptr(out), offset, n), "MPM.P.read")
+        return out
+
+    def read_grid(self) -> np.ndarray:
+        """grid.read (blocking): (numNodes, 4) int32 words in the reference layout."""
+        self._alive("MPM.grid.read")
+        out = np.empty((self.numNodes, 4), np.int32)
+        _check(load_library().mpm_read_grid(self._h, _ptr(out)), "MPM.grid.read")
+        return out
+
+    def readPositions(self) -> np.ndarray:  # mpm.js:624-630
+        raw = self.read_particles()
+        return (raw[:, :3].astype(np.float64) * self.dx + np.asarray(self.origin)).astype(np.float32)
+
+    def sync(self) -> None:
+        _check(load_library().mpm_sync(self._h))
+
+    def set_profiling(self, on: bool) -> None:
+        _check(load_library().mpm_set_profiling(self._h, 1 if on else 0))
+
+    def phase_times(self) -> dict:
+        ms = np.zeros(4, np.float32)
+        _check(load_library().mpm_phase_times(self._h, _ptr(ms)))
+        return dict(zip(["mpm sort", "mpm p2g", "mpm grid", "mpm g2p"], ms.tolist()))
+
+    def palette(self) -> list:
+        return [m.get("color") or [0.7, 0.7, 0.7] for m in self.materials]
+
+    def fluidMask(self) -> int:
+        return sum(1 << k for k, d in enumerate(self.materials) if d["kind"] == FLUID)
+
+    def _alive(self, what: str) -> None:
+        if self.destroyed:
+            raise SIError(f"{what}: it was destroyed", "destroyed")
+
+    def destroy(self) -> None:  # mpm.js:262-267 (idempotent)
+        if self.destroyed:
+            return
+        self.destroyed = True
+        load_library().mpm_sim_destroy(self._h)
+        self._h = None

@@ -1,0 +1,99 @@
+"""Synthetic MLS-MPM workloads, stated through the reference's world API.
+
+Every scene uses the reference's default regime: ppcAxis 2 (8 particles per cell), bound 3,
+sortEvery 4, 8 substeps per 1/60 s frame (dt = 1/480 s), no colliders, and a world size of
+grid/64 m so that dx = 1/64 m and the lattice gravity is -9.81*dt^2/dx for every scene. Boxes are
+given in cells as [lo, hi). Seeds follow MPM.add's default (count + 1, mpm.js:210) with jitter 1.
+
+- ``dam64``: 64^3 grid, water box x in [3,11), y in [3,35), z in [3,35): 65,536 particles.
+- ``mixed128``: 128^3 grid, water [3,67)x[3,35)x[3,35) and a jelly box [72,104)x[40,72)x[40,104)
+  falling at 2 cells/s: 1,048,576 particles.
+- ``mixed256``: 256^3 grid, four 64^3-cell blocks at y in [3,67): water, jelly, snow and
+  Drucker-Prager sand (friction 30 deg), Gaussian velocity noise 0.1 cells/s: 8,388,608 particles.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import mpm as _m
+
+SAND30 = {"kind": "sand", "density": 1.6, "stiffness": 0.6, "poisson": 0.3, "friction": 30}
+
+SCENES = {
+    "dam64": {
+        "grid": 64,
+        "blocks": [("water", (3, 3, 3), (11, 35, 35), (0, 0, 0), 0.0)],
+    },
+    "mixed128": {
+        "grid": 128,
+        "blocks": [("water", (3, 3, 3), (67, 35, 35), (0, 0, 0), 0.0),
+                   ("jelly", (72, 40, 40), (104, 72, 104), (0, -2, 0), 0.0)],
+    },
+    "mixed256": {
+        "grid": 256,
+        "blocks": [("water", (3, 3, 3), (67, 67, 67), (0, 0, 0), 0.1),
+                   ("jelly", (128, 3, 3), (192, 67, 67), (0, 0, 0), 0.1),
+                   ("snow", (3, 3, 128), (67, 67, 192), (0, 0, 0), 0.1),
+                   (SAND30, (128, 3, 128), (192, 67, 192), (0, 0, 0), 0.1)],
+    },
+}
+
+SUBSTEPS = 8
+DX = 1.0 / 64.0
+
+
+def options(name: str) -> dict:
+    sc = SCENES[name]
+    n = expected_count(name)
+    return {"grid": sc["grid"], "size": sc["grid"] / 64.0, "maxParticles": n, "substeps": SUBSTEPS, "sortEvery": 4,
+            "ppcAxis": 2}
+
+
+def expected_count(name: str) -> int:
+    return sum(8 * int(np.prod(np.subtract(hi, lo))) for _, lo, hi, _, _ in SCENES[name]["blocks"])
+
+
+def populate(sim: "_m.MPM", name: str) -> int:
+    """Add the scene's blocks to a simulation through MPM.add; returns the particle count."""
+    for mat, lo, hi, vel, noise in SCENES[name]["blocks"]:
+        box = [[l * DX for l in lo], [h * DX for h in hi]]
+        sim.add({"box": box}, material=mat, velocity=[v * DX for v in vel], jitter=1.0, noise=noise * DX)
+    return sim.count
+
+
+def host_build(name: str):
+    """The same scene built on the host only (no device): returns (dims, material table (8, 12),
+    records (N, 32) float32, lattice gravity (3,) float32). Mirrors MPM.add step by step."""
+    sc = SCENES[name]
+    grid = sc["grid"]
+    dims = [grid] * 3
+    dx = (grid / 64.0) / grid
+    ppc = 2
+    dt = (1 / 60) * 1 / SUBSTEPS
+    materials, index, recs = [], {}, []
+    count = 0
+    for mat, lo, hi, vel, noise in sc["blocks"]:
+        d = {**_m.MATERIALS[mat], "name": mat} if isinstance(mat, str) else dict(mat)
+        if isinstance(d["kind"], str):
+            d["kind"] = _m.MPM_KINDS[d["kind"]]
+        import json
+        key = json.dumps(d)
+        if key not in index:
+            index[key] = len(materials)
+            materials.append(d)
+        mid = index[key]
+        box = [[l * DX for l in lo], [h * DX for h in hi]]
+        pts = _m.sample_shape({"box": box}, dx / ppc, 1.0, count + 1)
+        rec = np.zeros((len(pts), 32), np.float32)
+        L = _m.load_library()
+        L.mpm_pack_particles(_m._ptr(np.ascontiguousarray(pts)), len(pts), _m._ptr(np.zeros(3)), dx,
+                             _m._ptr(np.asarray(dims, np.int32)), 3.0, mid,
+                             _m._ptr(np.asarray([v * DX for v in vel], np.float64)), dt / dx, float(noise * DX),
+                             ((count + 1) * 2654435761) & 0xFFFFFFFF, _m._ptr(rec))
+        recs.append(rec)
+        count += len(pts)
+    tbl = np.zeros((8, 12), np.float32)
+    for k, m in enumerate(materials):
+        tbl[k] = _m.material_row(m, ppc)
+    g = np.array([(x * dt * dt) / dx for x in (0, -9.81, 0)], np.float32)
+    return dims, tbl, np.concatenate(recs), g

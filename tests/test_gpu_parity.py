@@ -1,0 +1,238 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle, bit for bit.
+
+The oracle fixes one op order for every float expression; the kernels follow the same order with
+no contraction, so particles and grids must match exactly (not just within a tolerance).
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_mpm as O
+from helpers import bits, randomize_fast
+from shaderintel_amd import scenes
+from shaderintel_amd.mpm import MPM, SIError
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg: str) -> None:
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _mirror(sim: MPM, rec: np.ndarray, sort_every: int = 4) -> O.Scene:
+    sc = O.Scene(sim.dims, max(len(rec), 1), sim.ppcAxis, sort_every)
+    sc.set_materials(sim.material_table())
+    sc.write(rec)
+    sc.set_count(len(rec))
+    return sc
+
+
+def _first_diff(a: np.ndarray, b: np.ndarray) -> str:
+    ab, bb = bits(a).reshape(a.shape[0], -1), bits(b).reshape(b.shape[0], -1)
+    bad = np.argwhere(ab != bb)
+    if len(bad) == 0:
+        return "identical"
+    i, j = bad[0]
+    return f"{len(bad)} words differ; first at row {i} word {j}: {a.reshape(a.shape[0], -1)[i, j]!r} vs {b.reshape(b.shape[0], -1)[i, j]!r}"
+
+
+def test_add_matches_host_build(ctx):
+    _say("[gpu] add() records vs host build, dam64")
+    sim = MPM(ctx, **scenes.options("dam64"))
+    n = scenes.populate(sim, "dam64")
+    assert n == 65536
+    dims, tbl, rec, g = scenes.host_build("dam64")
+    got = sim.read_particles()
+    assert np.array_equal(bits(got), bits(rec)), _first_diff(got, rec)
+    assert np.array_equal(bits(sim.material_table()), bits(tbl))
+    sim.destroy()
+
+
+@pytest.mark.parametrize("force_global", [False, True])
+def test_p2g_grid_bit_exact(ctx, force_global):
+    _say(f"[gpu] p2g bit-exact, dam64 randomized, force_global={force_global}")
+    sim = MPM(ctx, **scenes.options("dam64"))
+    scenes.populate(sim, "dam64")
+    raw = randomize_fast(sim.read_particles(), seed=3)
+    sim.write_particles(raw)
+    sim.p2g(force_global=force_global)
+    got = sim.read_grid()
+    sc = _mirror(sim, raw)
+    sc.p2g()
+    want = sc.grid()
+    assert np.array_equal(got, want), _first_diff(got, want)
+    assert want[:, 3].sum() > 0
+    sim.destroy()
+
+
+def test_tiled_equals_global_after_steps(ctx):
+    """physics.test.js:33-42: 3 frames, _sort, then tiled and global P2G give identical words."""
+    _say("[gpu] tiled == global after 3 frames (grid 48)")
+    sim = MPM(ctx, grid=48, maxParticles=80000, substeps=4)
+    sim.add({"box": [[0.1, 0.1, 0.1], [0.9, 0.4, 0.9]]}, material="water")
+    for _ in range(3):
+        sim.step()
+    sim._sort()
+    sim.p2g(force_global=True)
+    a = sim.read_grid()
+    sim.p2g(force_global=False)
+    b = sim.read_grid()
+    assert np.array_equal(a, b)
+    assert a[:, 3].sum() > 0
+    sim.destroy()
+
+
+def test_shuffle_invariance(ctx):
+    _say("[gpu] shuffled particle order gives the same grid")
+    sim = MPM(ctx, **scenes.options("dam64"))
+    scenes.populate(sim, "dam64")
+    raw = randomize_fast(sim.read_particles(), seed=5)
+    sim.write_particles(raw)
+    sim.p2g()
+    a = sim.read_grid()
+    perm = np.random.default_rng(11).permutation(len(raw))
+    sim.write_particles(raw[perm])
+    sim.p2g(force_global=True)
+    b = sim.read_grid()
+    assert np.array_equal(a, b)
+    sim.destroy()
+
+
+@pytest.mark.parametrize("substeps", [1, 10])
+def test_dam_step_bit_exact(ctx, substeps):
+    _say(f"[gpu] dam64 {substeps} substeps vs oracle")
+    sim = MPM(ctx, **scenes.options("dam64"))
+    scenes.populate(sim, "dam64")
+    rec = sim.read_particles()
+    sc = _mirror(sim, rec)
+    g = sim.lattice_gravity()
+    from shaderintel_amd.mpm import load_library, _ptr
+    L = load_library()
+    assert L.mpm_step(sim._h, substeps, _ptr(g), 0.4, 0.9) == 0
+    sc.step(substeps, g)
+    got, want = sim.read_particles(), sc.read(len(rec))
+    assert np.array_equal(bits(got), bits(want)), _first_diff(got, want)
+    gg, gw = sim.read_grid(), sc.grid()
+    assert np.array_equal(gg, gw), _first_diff(gg, gw)
+    sim.destroy()
+
+
+MATERIAL_SCENES = {
+    "water": "water", "jelly": "jelly", "snow": "snow", "sand": scenes.SAND30, "honey": "honey", "rubber": "rubber",
+}
+
+
+@pytest.mark.parametrize("mat", list(MATERIAL_SCENES))
+@pytest.mark.parametrize("substeps", [1, 10])
+def test_material_step_bit_exact(ctx, mat, substeps):
+    """Each material, randomized v/C/tau and a sheared F, 1 and 10 substeps: particles and grid
+    identical to the oracle."""
+    _say(f"[gpu] material {mat}, {substeps} substeps vs oracle")
+    sim = MPM(ctx, grid=32, maxParticles=40000, substeps=8)
+    sim.add({"box": [[0.2, 0.15, 0.2], [0.6, 0.55, 0.6]]}, material=MATERIAL_SCENES[mat], velocity=[0.3, -0.2, 0.1],
+            jitter=1.0, noise=0.2)
+    raw = sim.read_particles()
+    rng = np.random.default_rng(7)
+    n = len(raw)
+    raw[:, 8:11] = (rng.random((n, 3)) - 0.5) * 0.05
+    raw[:, 12:15] = (rng.random((n, 3)) - 0.5) * 0.05
+    raw[:, 16:19] = (rng.random((n, 3)) - 0.5) * 0.05
+    if MATERIAL_SCENES[mat] not in ("water", "honey"):
+        F = np.eye(3)[None] + (rng.random((n, 3, 3)) - 0.5) * 0.3
+        raw[:, 20:23], raw[:, 24:27], raw[:, 28:31] = F[:, :, 0], F[:, :, 1], F[:, :, 2]
+    sim.write_particles(raw)
+    sc = _mirror(sim, raw)
+    g = sim.lattice_gravity()
+    from shaderintel_amd.mpm import load_library, _ptr
+    assert load_library().mpm_step(sim._h, substeps, _ptr(g), 0.4, 0.9) == 0
+    sc.step(substeps, g)
+    got, want = sim.read_particles(), sc.read(n)
+    assert np.all(np.isfinite(got))
+    assert np.array_equal(bits(got), bits(want)), _first_diff(got, want)
+    sim.destroy()
+
+
+def test_sort_matches_oracle(ctx):
+    _say("[gpu] _sort order vs oracle stable sort")
+    sim = MPM(ctx, **scenes.options("dam64"))
+    scenes.populate(sim, "dam64")
+    raw = sim.read_particles()
+    perm = np.random.default_rng(2).permutation(len(raw))
+    raw = raw[perm]
+    sim.write_particles(raw)
+    sim._sort()
+    got = sim.read_particles()
+    sc = _mirror(sim, raw)
+    sc.sort()
+    want = sc.read(len(raw))
+    assert np.array_equal(bits(got), bits(want)), _first_diff(got, want)
+    keys = sc.keys(len(raw))
+    assert np.all(np.diff(keys.astype(np.int64)) >= 0)
+    sim.destroy()
+
+
+SCAN_SIZES = [1, 3, 4, 17, 4096, 4097, 65537, 1 << 20, (1 << 24) + 13]
+
+
+@pytest.mark.parametrize("n", SCAN_SIZES)
+def test_scan(ctx, n):
+    """core.test.js:72-110 sizes: exclusive scan against a running sum (u32 wrapping)."""
+    _say(f"[gpu] scan n={n}")
+    a = np.random.default_rng(n).integers(0, 1 << 20, n, dtype=np.uint32)
+    want = np.concatenate([[0], np.cumsum(a.astype(np.uint64))[:-1]]).astype(np.uint64) & 0xFFFFFFFF
+    got = ctx.scan(a)
+    assert np.array_equal(got.astype(np.uint64), want)
+
+
+@pytest.mark.parametrize("n,nb", [(0, 7), (1, 1), (1000, 1), (65536, 4096), (300001, 262144), (2_000_003, 1 << 24)])
+def test_counting_sort(ctx, n, nb):
+    """grid.js:70-79 semantics with a stable rank: order is the stable argsort, start the bucket offsets."""
+    _say(f"[gpu] counting sort n={n} buckets={nb}")
+    keys = np.random.default_rng(n + nb).integers(0, nb, n, dtype=np.uint32)
+    order, start = ctx.counting_sort(keys, nb)
+    assert np.array_equal(order, np.argsort(keys, kind="stable").astype(np.uint32))
+    assert start[nb] == n
+    assert np.array_equal(start, np.searchsorted(np.sort(keys), np.arange(nb + 1)).astype(np.uint32))
+
+
+def test_conservation_law(ctx):
+    """verify/index.js:322-329 (law mpm-conservation) with checkMPM's tolerance (:175-193)."""
+    _say("[gpu] mpm-conservation law")
+    sim = MPM(ctx, grid=32, maxParticles=50000)
+    sim.add({"box": [[0.2, 0.2, 0.2], [0.6, 0.6, 0.6]]}, material="water", velocity=[0.3, 0.1, -0.2])
+    for _ in range(10):
+        sim.step()
+    raw = sim.read_particles()
+    sim.p2g()
+    g = sim.read_grid().astype(np.float64)
+    FP = 2.0 ** 22
+    m = 1 / 8
+    n = len(raw)
+    pm = m * n
+    pv = m * raw[:, 4:7].astype(np.float64).sum(0)
+    gm = g[:, 3].sum() / FP
+    gp = g[:, :3].sum(0) / FP
+    mass_err = abs(gm - pm) / pm
+    mom_err = np.linalg.norm(gp - pv) / pm
+    assert mass_err < 1e-4 and mom_err < 1e-4 + n * 27 * 4 / FP / pm, (mass_err, mom_err)
+    sim.destroy()
+
+
+def test_errors(ctx):
+    sim = MPM(ctx, grid=16, maxParticles=1024)
+    rec = np.zeros((1, 32), np.float32)
+    rec[0, :3] = [100, 5, 5]
+    with pytest.raises(SIError) as e:
+        sim.write_particles(rec)
+    assert e.value.code == "invalid-argument"
+    with pytest.raises(SIError) as e:
+        sim.add({"box": [[0.1, 0.1, 0.1], [0.2, 0.2, 0.2]]}, material={"kind": "fluid", "density": 1000})
+    assert e.value.code == "invalid-argument"
+    sim.destroy()
+    sim.destroy()
+    with pytest.raises(SIError):
+        sim.step()
