@@ -52,7 +52,7 @@ typedef struct mpm_config {
   int32_t dims[3];        /* grid cells per axis, 8..1024 (mpm.js:95) */
   uint32_t max_particles; /* capacity (mpm.js:91) */
   int32_t ppc_axis;       /* particles per cell per axis, default 2 (mpm.js:102) */
-  float bound;            /* boundary cells, 3 in the reference (mpm.js:103) */
+  float bound;            /* boundary cells, 3..16; 3 in the reference (mpm.js:103) */
   uint32_t sort_every;    /* sort every this many substeps, default 4 (mpm.js:101) */
   uint32_t flags;         /* MPM_FLAG_* */
 } mpm_config;

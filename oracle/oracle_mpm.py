@@ -57,7 +57,8 @@ def lib() -> ctypes.CDLL:
 
 
 def _p(a: np.ndarray) -> ctypes.c_void_p:
-    return ctypes.c_void_p(a.ctypes.data)
+    # data_as keeps a reference to the array, so temporaries outlive the call they are passed to
+    return a.ctypes.data_as(ctypes.c_void_p)
 
 
 def expf(x: float) -> float:

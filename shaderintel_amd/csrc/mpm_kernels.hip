@@ -20,14 +20,6 @@ __device__ __forceinline__ uint32_t node_addr(int gx, int gy, int gz, const Geo&
   return (b << 6) | (uint32_t)((gx & 3) | ((gy & 3) << 2) | ((gz & 3) << 4));
 }
 
-// First touch of block b in this P2G appends it to the active list.
-__device__ __forceinline__ void stamp_block(uint32_t b, uint32_t epoch, uint32_t* stamp, uint32_t* list,
-                                            uint32_t* count) {
-  if (__hip_atomic_load(&stamp[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-    if (atomicExch(&stamp[b], epoch) != epoch) list[atomicAdd(count, 1u)] = b;
-  }
-}
-
 __device__ __forceinline__ void load_record(const float4* __restrict__ P, uint32_t cap, uint32_t i, float4 r[8]) {
 #pragma unroll
   for (int k = 0; k < 8; k++) r[k] = P[(size_t)k * cap + i];
@@ -72,97 +64,6 @@ __device__ __forceinline__ int wave_max(int v) {
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
   return v;
-}
-
-// P2G: one workgroup scatters `ppg` consecutive particles. When the AABB of their stencils fits
-// tile_cap nodes, contributions are summed in an LDS tile with LDS integer atomics and only
-// non-zero nodes are flushed with global atomics (mpm.js:351-400); otherwise every contribution is
-// a global atomic (the reference's forceGlobal path, mpm.js:342-345).
-__global__ __launch_bounds__(kThreads) void k_p2g(P2GArgs a, Geo geo, const float* __restrict__ mats, int tile_cap,
-                                                  int ppg) {
-  extern __shared__ int tile[];
-  __shared__ int box[6];
-  const int tid = threadIdx.x;
-  const uint32_t first = blockIdx.x * (uint32_t)ppg;
-  const uint32_t last = min(first + (uint32_t)ppg, a.n);
-  if (first >= last) return;  // uniform across the workgroup
-
-  if (tid < 3) { box[tid] = INT_MAX; box[3 + tid] = INT_MIN; }
-  __syncthreads();
-  int lo0 = INT_MAX, lo1 = INT_MAX, lo2 = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hi2 = INT_MIN;
-  for (uint32_t i = first + tid; i < last; i += kThreads) {
-    const float4 x = a.P[i];
-    const int b0 = (int)floorf(x.x - 0.5f), b1 = (int)floorf(x.y - 0.5f), b2 = (int)floorf(x.z - 0.5f);
-    lo0 = min(lo0, b0); lo1 = min(lo1, b1); lo2 = min(lo2, b2);
-    hi0 = max(hi0, b0); hi1 = max(hi1, b1); hi2 = max(hi2, b2);
-  }
-  lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
-  hi0 = wave_max(hi0); hi1 = wave_max(hi1); hi2 = wave_max(hi2);
-  if ((tid & 63) == 0) {
-    atomicMin(&box[0], lo0); atomicMin(&box[1], lo1); atomicMin(&box[2], lo2);
-    atomicMax(&box[3], hi0); atomicMax(&box[4], hi1); atomicMax(&box[5], hi2);
-  }
-  __syncthreads();
-  const int ox = box[0], oy = box[1], oz = box[2];
-  const int ex = box[3] - ox + 3, ey = box[4] - oy + 3, ez = box[5] - oz + 3;
-  const long long nodes = (long long)ex * ey * ez;
-  const bool use_tile = tile_cap > 0 && nodes <= (long long)tile_cap;
-
-  if (use_tile) {
-    // stamp every block the tile overlaps
-    const int b0x = ox >> 2, b0y = oy >> 2, b0z = oz >> 2;
-    const int nbxk = ((ox + ex - 1) >> 2) - b0x + 1, nbyk = ((oy + ey - 1) >> 2) - b0y + 1,
-              nbzk = ((oz + ez - 1) >> 2) - b0z + 1;
-    const int nbk = nbxk * nbyk * nbzk;
-    for (int t = tid; t < nbk; t += kThreads) {
-      const int kx = t % nbxk, ky = (t / nbxk) % nbyk, kz = t / (nbxk * nbyk);
-      const uint32_t b = (uint32_t)(b0x + kx) + (uint32_t)geo.nbx * ((uint32_t)(b0y + ky) + (uint32_t)geo.nby * (uint32_t)(b0z + kz));
-      stamp_block(b, a.epoch, a.stamp, a.list, a.list_count);
-    }
-    for (int t = tid; t < (int)nodes * 4; t += kThreads) tile[t] = 0;
-    __syncthreads();
-    for (uint32_t i = first + tid; i < last; i += kThreads) {
-      float4 r[8];
-      load_record(a.P, a.cap, i, r);
-      p2g_particle(r, mats, a.vol4, [&](int gx, int gy, int gz, int q0, int q1, int q2, int q3) {
-        const int ti = (((gz - oz) * ey + (gy - oy)) * ex + (gx - ox)) * 4;
-        atomicAdd(&tile[ti], q0);
-        atomicAdd(&tile[ti + 1], q1);
-        atomicAdd(&tile[ti + 2], q2);
-        atomicAdd(&tile[ti + 3], q3);
-      });
-    }
-    __syncthreads();
-    for (int t = tid; t < (int)nodes; t += kThreads) {
-      const int q0 = tile[4 * t], q1 = tile[4 * t + 1], q2 = tile[4 * t + 2], q3 = tile[4 * t + 3];
-      if ((q0 | q1 | q2 | q3) != 0) {
-        const int lx = t % ex, ly = (t / ex) % ey, lz = t / (ex * ey);
-        int* g = reinterpret_cast<int*>(a.A + node_addr(ox + lx, oy + ly, oz + lz, geo));
-        atomicAdd(g, q0);
-        atomicAdd(g + 1, q1);
-        atomicAdd(g + 2, q2);
-        atomicAdd(g + 3, q3);
-      }
-    }
-  } else {
-    for (uint32_t i = first + tid; i < last; i += kThreads) {
-      float4 r[8];
-      load_record(a.P, a.cap, i, r);
-      const int bx = (int)floorf(r[0].x - 0.5f), by = (int)floorf(r[0].y - 0.5f), bz = (int)floorf(r[0].z - 0.5f);
-      for (int kz = bz >> 2; kz <= (bz + 2) >> 2; kz++)
-        for (int ky = by >> 2; ky <= (by + 2) >> 2; ky++)
-          for (int kx = bx >> 2; kx <= (bx + 2) >> 2; kx++)
-            stamp_block((uint32_t)kx + (uint32_t)geo.nbx * ((uint32_t)ky + (uint32_t)geo.nby * (uint32_t)kz), a.epoch,
-                        a.stamp, a.list, a.list_count);
-      p2g_particle(r, mats, a.vol4, [&](int gx, int gy, int gz, int q0, int q1, int q2, int q3) {
-        int* g = reinterpret_cast<int*>(a.A + node_addr(gx, gy, gz, geo));
-        atomicAdd(g, q0);
-        atomicAdd(g + 1, q1);
-        atomicAdd(g + 2, q2);
-        atomicAdd(g + 3, q3);
-      });
-    }
-  }
 }
 
 __device__ __forceinline__ float collider_sdf(uint32_t kind, f3 d, const float* s) {  // mpm.js:419-426
@@ -223,30 +124,6 @@ __device__ __forceinline__ float4 grid_node(int4 q, int ix, int iy, int iz, cons
     }
   }
   return make_float4(v.x, v.y, v.z, m);
-}
-
-__global__ __launch_bounds__(kThreads) void k_grid_update(const uint32_t* __restrict__ list,
-                                                          const uint32_t* __restrict__ count, int4* __restrict__ A,
-                                                          float4* __restrict__ V, Geo geo, GridArgs ga,
-                                                          const float* __restrict__ cols) {
-  const uint32_t cnt = *count;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (uint32_t j = blockIdx.x * 4 + w; j < cnt; j += gridDim.x * 4) {
-    const uint32_t b = list[j];
-    const int bx = (int)(b % (uint32_t)geo.nbx), by = (int)((b / (uint32_t)geo.nbx) % (uint32_t)geo.nby),
-              bz = (int)(b / ((uint32_t)geo.nbx * (uint32_t)geo.nby));
-    const uint32_t addr = (b << 6) | (uint32_t)lane;
-    const int4 q = A[addr];
-    V[addr] = grid_node(q, bx * 4 + (lane & 3), by * 4 + ((lane >> 2) & 3), bz * 4 + (lane >> 4), geo, ga, cols);
-    A[addr] = make_int4(0, 0, 0, 0);
-  }
-}
-
-__global__ __launch_bounds__(kThreads) void k_clear_listed(const uint32_t* __restrict__ list,
-                                                           const uint32_t* __restrict__ count, int4* __restrict__ A) {
-  const uint32_t cnt = *count;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (uint32_t j = blockIdx.x * 4 + w; j < cnt; j += gridDim.x * 4) A[(list[j] << 6) | (uint32_t)lane] = make_int4(0, 0, 0, 0);
 }
 
 // G2P (mpm.js:479-559): gather velocity and APIC C, advect, update F, plasticity, stress.
@@ -397,23 +274,271 @@ __global__ void k_permute(const float4* __restrict__ src, float4* __restrict__ d
   for (int k = 0; k < 8; k++) dst[(size_t)k * cap + j] = src[(size_t)k * cap + p];
 }
 
-__global__ void k_grid_export(const int4* __restrict__ A, const float4* __restrict__ V, const uint32_t* __restrict__ stamp,
-                              uint32_t epoch, int mode, Geo geo, int4* __restrict__ out) {
+// ---- binning ----
+__device__ __forceinline__ uint32_t compact3(uint32_t x) {  // inverse of spread3
+  x &= 0x09249249u;
+  x = (x ^ (x >> 2)) & 0x030c30c3u;
+  x = (x ^ (x >> 4)) & 0x0300f00fu;
+  x = (x ^ (x >> 8)) & 0xff0000ffu;
+  x = (x ^ (x >> 16)) & 0x000003ffu;
+  return x;
+}
+
+__global__ void k_bin_flags(const uint32_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ flags) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  flags[i] = (i < n && (i == 0 || (keys[i] >> 6) != (keys[i - 1] >> 6))) ? 1u : 0u;
+}
+
+__global__ void k_bin_emit(const uint32_t* __restrict__ keys, uint32_t n, const uint32_t* __restrict__ scanned,
+                           uint32_t* __restrict__ wl_start, uint32_t* __restrict__ wl_block, uint32_t* __restrict__ map,
+                           uint32_t* __restrict__ item_count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t blk = keys[i] >> 6;
+  if (i == 0 || (keys[i - 1] >> 6) != blk) {
+    const uint32_t k = scanned[i];
+    wl_start[k] = i;
+    wl_block[k] = blk;
+    map[blk] = k;
+  }
+  if (i == n - 1) {
+    wl_start[scanned[n]] = n;
+    *item_count = scanned[n];
+  }
+}
+
+// First touch of node block b in this binning appends it to S.
+__device__ __forceinline__ void add_to_set(uint32_t b, uint32_t epoch, uint32_t* stamp, uint32_t* list, uint32_t* count) {
+  if (__hip_atomic_load(&stamp[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch)
+    if (atomicExch(&stamp[b], epoch) != epoch) list[atomicAdd(count, 1u)] = b;
+}
+
+__global__ __launch_bounds__(kThreads) void k_bin_dilate(const uint32_t* __restrict__ wl_block,
+                                                         const uint32_t* __restrict__ item_count, Geo geo,
+                                                         uint32_t* __restrict__ sstamp, uint32_t* __restrict__ slist,
+                                                         uint32_t* __restrict__ scount, uint32_t bin_epoch) {
+  const uint32_t cnt = *item_count;
+  // 64 threads per item: one per node block in [B-1, B+2]^3
+  for (uint32_t t = blockIdx.x * kThreads + threadIdx.x; t < cnt * 64; t += gridDim.x * kThreads) {
+    const uint32_t k = t >> 6, d = t & 63;
+    const uint32_t mb = wl_block[k];
+    const int gx = (int)compact3(mb) - 1 + (int)(d & 3), gy = (int)compact3(mb >> 1) - 1 + (int)((d >> 2) & 3),
+              gz = (int)compact3(mb >> 2) - 1 + (int)(d >> 4);
+    if (gx < 0 || gy < 0 || gz < 0 || gx >= geo.nbx || gy >= geo.nby || gz >= geo.nbz) continue;
+    add_to_set((uint32_t)gx + (uint32_t)geo.nbx * ((uint32_t)gy + (uint32_t)geo.nby * (uint32_t)gz), bin_epoch, sstamp,
+               slist, scount);
+  }
+}
+
+// A fallback write to node block b: mark it written in this P2G (so the grid update reads and clears
+// A there) and list it in D unless S already covers it.
+__device__ __forceinline__ void stamp_fallback(uint32_t b, const FallbackView& fb, const uint32_t* sstamp,
+                                               uint32_t bin_epoch) {
+  if (__hip_atomic_load(&fb.stamp[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != fb.epoch)
+    if (atomicExch(&fb.stamp[b], fb.epoch) != fb.epoch)
+      if (sstamp == nullptr || sstamp[b] != bin_epoch) fb.list[atomicAdd(fb.count, 1u)] = b;
+}
+
+__device__ __forceinline__ void fallback_particle(const float4 r[8], const float* __restrict__ mats, float vol4, int4* A,
+                                                  const FallbackView& fb, const uint32_t* sstamp, uint32_t bin_epoch,
+                                                  const Geo& geo) {
+  const int bx = (int)floorf(r[0].x - 0.5f), by = (int)floorf(r[0].y - 0.5f), bz = (int)floorf(r[0].z - 0.5f);
+  for (int kz = bz >> 2; kz <= (bz + 2) >> 2; kz++)
+    for (int ky = by >> 2; ky <= (by + 2) >> 2; ky++)
+      for (int kx = bx >> 2; kx <= (bx + 2) >> 2; kx++)
+        stamp_fallback((uint32_t)kx + (uint32_t)geo.nbx * ((uint32_t)ky + (uint32_t)geo.nby * (uint32_t)kz), fb, sstamp,
+                       bin_epoch);
+  p2g_particle(r, mats, vol4, [&](int gx, int gy, int gz, int q0, int q1, int q2, int q3) {
+    int* g = reinterpret_cast<int*>(A + node_addr(gx, gy, gz, geo));
+    atomicAdd(g, q0);
+    atomicAdd(g + 1, q1);
+    atomicAdd(g + 2, q2);
+    atomicAdd(g + 3, q3);
+  });
+}
+
+// P2G over the binning's work items (mpm.js:306-401 restated for aligned particle blocks). One
+// 256-thread workgroup per item at a time (persistent loop). The LDS tile covers the bounding box of
+// the stencils of the item's particles whose base lies within kDrift cells of the block; lanes of one
+// instruction take particles `stride` apart in the sorted order, so they sit in different cells and
+// their LDS atomics hit different nodes.
+__global__ __launch_bounds__(kThreads) void k_p2g_binned(const float4* __restrict__ P, uint32_t cap,
+                                                         const float* __restrict__ mats, float vol4, BinView bv, int4* A,
+                                                         FallbackView fb, Geo geo) {
+  __shared__ int tile[4 * kTileNodes];
+  __shared__ int box[6];
+  const int tid = threadIdx.x;
+  const uint32_t items = *bv.item_count;
+  for (uint32_t k = blockIdx.x; k < items; k += gridDim.x) {
+    const uint32_t start = bv.wl_start[k], cnt = bv.wl_start[k + 1] - start;
+    const uint32_t mb = bv.wl_block[k];
+    const int wx = 4 * (int)compact3(mb) - kDrift, wy = 4 * (int)compact3(mb >> 1) - kDrift,
+              wz = 4 * (int)compact3(mb >> 2) - kDrift;  // lowest in-window base
+    const bool has_slot = k < bv.tile_slots;
+    __syncthreads();  // the previous item's write-out has finished reading the tile
+    if (tid < 3) { box[tid] = INT_MAX; box[3 + tid] = INT_MIN; }
+    __syncthreads();
+    int lo0 = INT_MAX, lo1 = INT_MAX, lo2 = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hi2 = INT_MIN;
+    for (uint32_t v = tid; v < cnt; v += kThreads) {
+      const uint32_t j = start + v;
+      const float4 x = P[bv.perm ? bv.perm[j] : j];
+      const int b0 = (int)floorf(x.x - 0.5f), b1 = (int)floorf(x.y - 0.5f), b2 = (int)floorf(x.z - 0.5f);
+      if ((unsigned)(b0 - wx) < (unsigned)(4 + 2 * kDrift) && (unsigned)(b1 - wy) < (unsigned)(4 + 2 * kDrift) &&
+          (unsigned)(b2 - wz) < (unsigned)(4 + 2 * kDrift)) {
+        lo0 = min(lo0, b0); lo1 = min(lo1, b1); lo2 = min(lo2, b2);
+        hi0 = max(hi0, b0); hi1 = max(hi1, b1); hi2 = max(hi2, b2);
+      }
+    }
+    lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
+    hi0 = wave_max(hi0); hi1 = wave_max(hi1); hi2 = wave_max(hi2);
+    if ((tid & 63) == 0) {
+      atomicMin(&box[0], lo0); atomicMin(&box[1], lo1); atomicMin(&box[2], lo2);
+      atomicMax(&box[3], hi0); atomicMax(&box[4], hi1); atomicMax(&box[5], hi2);
+    }
+    __syncthreads();
+    const bool any = box[0] != INT_MAX;
+    const int ox = box[0], oy = box[1], oz = box[2];
+    const int ex = any ? box[3] - ox + 3 : 0, ey = any ? box[4] - oy + 3 : 0, ez = any ? box[5] - oz + 3 : 0;
+    const int nodes = ex * ey * ez;  // <= kTileNodes by the window test
+    if (has_slot)
+      for (int t = tid; t < nodes; t += kThreads) {
+        tile[t] = 0;
+        tile[kTileNodes + t] = 0;
+        tile[2 * kTileNodes + t] = 0;
+        tile[3 * kTileNodes + t] = 0;
+      }
+    __syncthreads();
+    const uint32_t stride = cnt >= 512u ? 8u : (cnt >= 64u ? cnt >> 6 : 1u);
+    const uint32_t chunk = 64u * stride;
+    const uint32_t span = ((cnt + chunk - 1) / chunk) * chunk;
+    for (uint32_t v = tid; v < span; v += kThreads) {
+      const uint32_t li = (v / chunk) * chunk + (v & 63u) * stride + (v >> 6) % stride;
+      if (li >= cnt) continue;
+      const uint32_t j = start + li;
+      float4 r[8];
+      load_record(P, cap, bv.perm ? bv.perm[j] : j, r);
+      const int b0 = (int)floorf(r[0].x - 0.5f), b1 = (int)floorf(r[0].y - 0.5f), b2 = (int)floorf(r[0].z - 0.5f);
+      const bool inwin = (unsigned)(b0 - wx) < (unsigned)(4 + 2 * kDrift) &&
+                         (unsigned)(b1 - wy) < (unsigned)(4 + 2 * kDrift) && (unsigned)(b2 - wz) < (unsigned)(4 + 2 * kDrift);
+      if (inwin && has_slot) {
+        const int tb = ((b2 - oz) * ey + (b1 - oy)) * ex + (b0 - ox);
+        p2g_particle(r, mats, vol4, [&](int gx, int gy, int gz, int q0, int q1, int q2, int q3) {
+          const int t = tb + ((gz - b2) * ey + (gy - b1)) * ex + (gx - b0);
+          atomicAdd(&tile[t], q0);
+          atomicAdd(&tile[kTileNodes + t], q1);
+          atomicAdd(&tile[2 * kTileNodes + t], q2);
+          atomicAdd(&tile[3 * kTileNodes + t], q3);
+        });
+      } else {
+        fallback_particle(r, mats, vol4, A, fb, bv.sstamp, bv.bin_epoch, geo);
+      }
+    }
+    __syncthreads();
+    if (has_slot) {
+      if (tid == 0) bv.tile_hdr[k] = make_int4(ox, oy, oz, ex | (ey << 8) | (ez << 16));
+      int4* out = bv.tiles + (size_t)k * kTileNodes;
+      for (int t = tid; t < nodes; t += kThreads)
+        out[t] = make_int4(tile[t], tile[kTileNodes + t], tile[2 * kTileNodes + t], tile[3 * kTileNodes + t]);
+    }
+  }
+}
+
+// Global-atomic P2G (the reference's forceGlobal path, mpm.js:342-345), any particle order.
+__global__ __launch_bounds__(kThreads) void k_p2g_global(const float4* __restrict__ P, uint32_t cap, uint32_t n,
+                                                         const float* __restrict__ mats, float vol4, int4* A,
+                                                         FallbackView fb, const uint32_t* sstamp, uint32_t bin_epoch,
+                                                         Geo geo) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  float4 r[8];
+  load_record(P, cap, i, r);
+  fallback_particle(r, mats, vol4, A, fb, sstamp, bin_epoch, geo);
+}
+
+// Sum of the P2G result at node (n0, n1, n2) of node block (g0, g1, g2): the tiles of particle blocks
+// G-2..G+1 per axis that cover the node. All lanes of the wave share the node block, so the map and
+// header reads are uniform.
+__device__ __forceinline__ int4 pull_tiles(const BinView& bv, const Geo& geo, int g0, int g1, int g2, int n0, int n1,
+                                           int n2, int4 acc) {
+  const int pb = 1 << geo.mbits;  // particle blocks per axis covered by the Morton code
+  for (int dz = -2; dz <= 1; dz++) {
+    const int bz = g2 + dz;
+    if (bz < 0 || bz >= pb) continue;
+    for (int dy = -2; dy <= 1; dy++) {
+      const int by = g1 + dy;
+      if (by < 0 || by >= pb) continue;
+      for (int dx = -2; dx <= 1; dx++) {
+        const int bx = g0 + dx;
+        if (bx < 0 || bx >= pb) continue;
+        const uint32_t k = bv.map[morton3((uint32_t)bx, (uint32_t)by, (uint32_t)bz)];
+        if (k == 0xffffffffu || k >= bv.tile_slots) continue;
+        const int4 h = bv.tile_hdr[k];
+        const int ex = h.w & 255, ey = (h.w >> 8) & 255, ez = (h.w >> 16) & 255;
+        const int lx = n0 - h.x, ly = n1 - h.y, lz = n2 - h.z;
+        if ((unsigned)lx < (unsigned)ex && (unsigned)ly < (unsigned)ey && (unsigned)lz < (unsigned)ez) {
+          const int4 q = bv.tiles[(size_t)k * kTileNodes + (lz * ey + ly) * ex + lx];
+          acc.x = (int)((uint32_t)acc.x + (uint32_t)q.x);
+          acc.y = (int)((uint32_t)acc.y + (uint32_t)q.y);
+          acc.z = (int)((uint32_t)acc.z + (uint32_t)q.z);
+          acc.w = (int)((uint32_t)acc.w + (uint32_t)q.w);
+        }
+      }
+    }
+  }
+  return acc;
+}
+
+// Grid update (VELOCITY = true, mpm.js:428-470) or tile resolve into A (VELOCITY = false) over S then D.
+template <bool VELOCITY>
+__global__ __launch_bounds__(kThreads) void k_grid_pull(BinView bv, FallbackView fb, int4* __restrict__ A,
+                                                        float4* __restrict__ V, Geo geo, GridArgs ga,
+                                                        const float* __restrict__ cols) {
+  const uint32_t ns = bv.scount ? *bv.scount : 0u, nd = *fb.count;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (uint32_t j = blockIdx.x * 4 + w; j < ns + nd; j += gridDim.x * 4) {
+    const uint32_t b = __builtin_amdgcn_readfirstlane(j < ns ? bv.slist[j] : fb.list[j - ns]);
+    const int g0 = (int)(b % (uint32_t)geo.nbx), g1 = (int)((b / (uint32_t)geo.nbx) % (uint32_t)geo.nby),
+              g2 = (int)(b / ((uint32_t)geo.nbx * (uint32_t)geo.nby));
+    const int n0 = 4 * g0 + (lane & 3), n1 = 4 * g1 + ((lane >> 2) & 3), n2 = 4 * g2 + (lane >> 4);
+    const uint32_t addr = (b << 6) | (uint32_t)lane;
+    int4 acc = make_int4(0, 0, 0, 0);
+    const bool written = fb.stamp[b] == fb.epoch;
+    if (written) acc = A[addr];
+    if (j < ns) acc = pull_tiles(bv, geo, g0, g1, g2, n0, n1, n2, acc);
+    if (VELOCITY) {
+      V[addr] = grid_node(acc, n0, n1, n2, geo, ga, cols);
+      if (written) A[addr] = make_int4(0, 0, 0, 0);
+    } else {
+      A[addr] = acc;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_clear_blocks(const uint32_t* __restrict__ list,
+                                                           const uint32_t* __restrict__ count, int4* __restrict__ A) {
+  const uint32_t cnt = *count;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (uint32_t j = blockIdx.x * 4 + w; j < cnt; j += gridDim.x * 4) A[(list[j] << 6) | (uint32_t)lane] = make_int4(0, 0, 0, 0);
+}
+
+__global__ void k_grid_export(const int4* __restrict__ A, const float4* __restrict__ V, const uint32_t* __restrict__ sstamp,
+                              uint32_t bin_epoch, const uint32_t* __restrict__ stamp, uint32_t epoch, int mode, Geo geo,
+                              int4* __restrict__ out) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t nodes = (size_t)geo.X * geo.Y * geo.Z;
   if (i >= nodes) return;
   const int gx = (int)(i % (size_t)geo.X), gy = (int)((i / (size_t)geo.X) % (size_t)geo.Y),
             gz = (int)(i / ((size_t)geo.X * geo.Y));
   const uint32_t addr = node_addr(gx, gy, gz, geo);
+  const uint32_t b = addr >> 6;
   if (mode == 0) {
     out[i] = A[addr];
+  } else if ((sstamp && sstamp[b] == bin_epoch) || stamp[b] == epoch) {
+    const float4 v = V[addr];
+    out[i] = make_int4(__float_as_int(v.x), __float_as_int(v.y), __float_as_int(v.z), __float_as_int(v.w));
   } else {
-    if (stamp[addr >> 6] == epoch) {
-      const float4 v = V[addr];
-      out[i] = make_int4(__float_as_int(v.x), __float_as_int(v.y), __float_as_int(v.z), __float_as_int(v.w));
-    } else {
-      out[i] = make_int4(0, 0, 0, 0);
-    }
+    out[i] = make_int4(0, 0, 0, 0);
   }
 }
 
@@ -446,43 +571,75 @@ hipError_t launch_permute(const float4* src, float4* dst, uint32_t cap, uint32_t
   hipLaunchKernelGGL(k_permute, dim3(blocks_for(n, 256)), dim3(256), 0, s, src, dst, cap, n, perm);
   return hipGetLastError();
 }
-}  // namespace mpm
 
-// Device-pointer variants used by the runtime (mats / cols in device memory).
-namespace mpm {
-hipError_t launch_p2g_dev(const P2GArgs& a, const Geo& geo, const float* mats, int tile_cap, int ppg, hipStream_t s) {
-  if (!a.n) return hipSuccess;
-  if (ppg <= 0) ppg = 512;
-  const size_t lds = tile_cap > 0 ? (size_t)tile_cap * 16 : 0;
-  hipLaunchKernelGGL(k_p2g, dim3(blocks_for(a.n, (unsigned)ppg)), dim3(kThreads), lds, s, a, geo, mats, tile_cap, ppg);
+namespace {
+unsigned persistent_grid(uint32_t max_work, unsigned per_block, unsigned cap) {
+  unsigned g = (unsigned)((max_work + per_block - 1) / per_block);
+  if (g > cap) g = cap;
+  return g ? g : 1u;
+}
+}  // namespace
+
+hipError_t launch_bin_flags(const uint32_t* keys, uint32_t n, uint32_t* flags, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_flags, dim3(blocks_for((size_t)n + 1, 256)), dim3(256), 0, s, keys, n, flags);
   return hipGetLastError();
 }
-hipError_t launch_grid_update_dev(const uint32_t* list, const uint32_t* list_count, int4* A, float4* V, const Geo& geo,
-                                  const GridArgs& ga, const float* cols, uint32_t max_blocks, hipStream_t s) {
-  unsigned g = blocks_for(max_blocks, 4);
-  if (g > 8192) g = 8192;
-  if (g == 0) g = 1;
-  hipLaunchKernelGGL(k_grid_update, dim3(g), dim3(kThreads), 0, s, list, list_count, A, V, geo, ga, cols);
+hipError_t launch_bin_emit(const uint32_t* keys, uint32_t n, const uint32_t* scanned, uint32_t* wl_start,
+                           uint32_t* wl_block, uint32_t* map, uint32_t* item_count, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_bin_emit, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, scanned, wl_start, wl_block, map,
+                     item_count);
   return hipGetLastError();
 }
-hipError_t launch_g2p_dev(float4* P, uint32_t cap, uint32_t n, const float4* V, const Geo& geo, const float* mats,
-                          float bound, float vmax, hipStream_t s) {
+hipError_t launch_bin_dilate(const uint32_t* wl_block, const uint32_t* item_count, const Geo& geo, uint32_t* sstamp,
+                             uint32_t* slist, uint32_t* scount, uint32_t bin_epoch, uint32_t max_items, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_dilate, dim3(persistent_grid(max_items, 4, 4096)), dim3(kThreads), 0, s, wl_block, item_count,
+                     geo, sstamp, slist, scount, bin_epoch);
+  return hipGetLastError();
+}
+hipError_t launch_p2g_binned(const float4* P, uint32_t cap, const float* mats, float vol4, const BinView& bv, int4* A,
+                             const FallbackView& fb, const Geo& geo, uint32_t max_items, hipStream_t s) {
+  hipLaunchKernelGGL(k_p2g_binned, dim3(persistent_grid(max_items, 1, 2048)), dim3(kThreads), 0, s, P, cap, mats, vol4, bv,
+                     A, fb, geo);
+  return hipGetLastError();
+}
+hipError_t launch_p2g_global(const float4* P, uint32_t cap, uint32_t n, const float* mats, float vol4, int4* A,
+                             const FallbackView& fb, const uint32_t* sstamp, uint32_t bin_epoch, const Geo& geo,
+                             hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_p2g_global, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, P, cap, n, mats, vol4, A, fb,
+                     sstamp, bin_epoch, geo);
+  return hipGetLastError();
+}
+hipError_t launch_grid_update(const BinView& bv, const FallbackView& fb, int4* A, float4* V, const Geo& geo,
+                              const GridArgs& ga, const float* cols, uint32_t max_blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_grid_pull<true>, dim3(persistent_grid(max_blocks, 4, 8192)), dim3(kThreads), 0, s, bv, fb, A, V,
+                     geo, ga, cols);
+  return hipGetLastError();
+}
+hipError_t launch_resolve_tiles(const BinView& bv, const FallbackView& fb, int4* A, const Geo& geo, uint32_t max_blocks,
+                                hipStream_t s) {
+  GridArgs ga{};
+  hipLaunchKernelGGL(k_grid_pull<false>, dim3(persistent_grid(max_blocks, 4, 8192)), dim3(kThreads), 0, s, bv, fb, A,
+                     (float4*)nullptr, geo, ga, (const float*)nullptr);
+  return hipGetLastError();
+}
+hipError_t launch_clear_blocks(const uint32_t* list, const uint32_t* count, int4* A, uint32_t max_blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_clear_blocks, dim3(persistent_grid(max_blocks, 4, 8192)), dim3(kThreads), 0, s, list, count, A);
+  return hipGetLastError();
+}
+hipError_t launch_g2p(float4* P, uint32_t cap, uint32_t n, const float4* V, const Geo& geo, const float* mats, float bound,
+                      float vmax, hipStream_t s) {
   if (!n) return hipSuccess;
   hipLaunchKernelGGL(k_g2p, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, P, cap, n, V, geo, mats, bound, vmax);
   return hipGetLastError();
 }
-hipError_t launch_clear_listed(const uint32_t* list, const uint32_t* list_count, int4* A, uint32_t max_blocks,
-                               hipStream_t s) {
-  unsigned g = blocks_for(max_blocks, 4);
-  if (g > 8192) g = 8192;
-  if (g == 0) g = 1;
-  hipLaunchKernelGGL(k_clear_listed, dim3(g), dim3(kThreads), 0, s, list, list_count, A);
-  return hipGetLastError();
-}
-hipError_t launch_grid_export(const int4* A, const float4* V, const uint32_t* stamp, uint32_t epoch, int mode,
-                              const Geo& geo, int4* out, hipStream_t s) {
+hipError_t launch_grid_export(const int4* A, const float4* V, const uint32_t* sstamp, uint32_t bin_epoch,
+                              const uint32_t* stamp, uint32_t epoch, int mode, const Geo& geo, int4* out, hipStream_t s) {
   const size_t nodes = (size_t)geo.X * geo.Y * geo.Z;
-  hipLaunchKernelGGL(k_grid_export, dim3(blocks_for(nodes, 256)), dim3(256), 0, s, A, V, stamp, epoch, mode, geo, out);
+  hipLaunchKernelGGL(k_grid_export, dim3(blocks_for(nodes, 256)), dim3(256), 0, s, A, V, sstamp, bin_epoch, stamp, epoch,
+                     mode, geo, out);
   return hipGetLastError();
 }
+
 }  // namespace mpm

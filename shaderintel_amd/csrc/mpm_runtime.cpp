@@ -48,7 +48,11 @@ mpm_status from_hip(hipError_t e, const char* where) {
     if (_e != hipSuccess) return from_hip(_e, #expr);  \
   } while (0)
 
-enum GridState { kGridClean = 0, kGridAccumulated = 1, kGridVelocity = 2 };
+// What the grid buffers hold after the last call: nothing, the i32 P2G result in A, the P2G result
+// split over tiles and A (inside mpm_step only), or velocities in V.
+enum GridState { kGridClean = 0, kGridA = 1, kGridTiles = 2, kGridVelocity = 3 };
+
+constexpr size_t kTileBudgetBytes = size_t(8) << 30;  // tile slots beyond this use the fallback path
 
 // Small tables are written by a one-thread-per-float kernel whose argument holds the data: the copy
 // is ordered with the work already on the stream and needs no host buffer to outlive the call (the
@@ -74,31 +78,57 @@ struct mpm_sim {
   Geo geo{};
   uint32_t cap = 0, nblocks = 0;
   int key_bits = 0;
+  uint32_t nbm = 0;         // Morton particle blocks (map entries)
+  uint32_t max_items = 0;   // work items a binning can produce
+  uint32_t tile_slots = 0;
   hipStream_t stream = nullptr;
   float4* P[2] = {nullptr, nullptr};
   int cur = 0;
   int4* A = nullptr;
   float4* V = nullptr;
+  // D (fallback writes of the current P2G)
   uint32_t* stamp = nullptr;
-  uint32_t* list[2] = {nullptr, nullptr};
-  uint32_t* counts = nullptr;  // counts[2]
+  uint32_t* dlist = nullptr;
+  uint32_t* dcount = nullptr;
+  uint32_t epoch = 0;
+  // binning
+  uint32_t* wl_start = nullptr;
+  uint32_t* wl_block = nullptr;
+  uint32_t* item_count = nullptr;
+  uint32_t* map = nullptr;
+  uint32_t* flags = nullptr;
+  uint32_t* slist = nullptr;
+  uint32_t* scount = nullptr;
+  uint32_t* sstamp = nullptr;
+  int4* tiles = nullptr;
+  int4* tile_hdr = nullptr;
+  const uint32_t* perm = nullptr;
+  uint32_t bin_epoch = 0;
+  bool bin_valid = false;
+  // materials / colliders
   float* mats_d = nullptr;
   float* cols_d = nullptr;
   uint32_t ncol = 0;
+  // sort
   uint32_t* keys[2] = {nullptr, nullptr};
   uint32_t* vals[2] = {nullptr, nullptr};
   uint32_t* sort_scratch = nullptr;
+  // transfers
   float4* staging = nullptr;
   uint32_t staging_records = 0;
   int4* export_buf = nullptr;
   uint32_t count = 0;
   uint64_t step_count = 0;
-  uint32_t epoch = 0;
   int grid_state = kGridClean;
-  int tile_cap = 1024;
-  int ppg = 512;
+  bool a_dirty_s = false, a_dirty_d = false;  // A holds data over S / over D
+  uint32_t v_bin_epoch = 0, v_epoch = 0;      // binning and P2G the velocities in V belong to
   bool profiling = false;
+  // Profiling: five events per profiled substep (sort, p2g, grid, g2p start; end), recorded on the
+  // stream without host synchronization and resolved by mpm_phase_times.
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
   float phase_ms[4] = {0, 0, 0, 0};
+  uint32_t phase_substeps = 0;
 };
 
 namespace {
@@ -117,8 +147,11 @@ void free_sim(mpm_sim* s) {
   if (!s) return;
   if (s->ctx) hipSetDevice(s->ctx->device);
   if (s->stream) hipStreamSynchronize(s->stream);
-  void* ptrs[] = {s->P[0], s->P[1], s->A, s->V, s->stamp, s->list[0], s->list[1], s->counts, s->mats_d, s->cols_d,
-                  s->keys[0], s->keys[1], s->vals[0], s->vals[1], s->sort_scratch, s->staging, s->export_buf};
+  for (hipEvent_t e : s->ev_pool) hipEventDestroy(e);
+  void* ptrs[] = {s->P[0],     s->P[1],     s->A,        s->V,          s->stamp,    s->dlist,    s->dcount,
+                  s->wl_start, s->wl_block, s->item_count, s->map,    s->flags,    s->slist,    s->scount,
+                  s->sstamp,   s->tiles,    s->tile_hdr, s->mats_d,     s->cols_d,   s->keys[0],  s->keys[1],
+                  s->vals[0],  s->vals[1],  s->sort_scratch, s->staging, s->export_buf};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (s->stream) hipStreamDestroy(s->stream);
@@ -134,52 +167,116 @@ mpm_status write_table(mpm_sim* s, float* dst, const float* src, int n) {
   return MPM_OK;
 }
 
-mpm_status do_sort(mpm_sim* s) {  // mpm.js:614-621
-  if (s->count == 0) return MPM_OK;
+BinView bin_view(const mpm_sim* s) {
+  BinView bv;
+  bv.wl_start = s->wl_start;
+  bv.wl_block = s->wl_block;
+  bv.item_count = s->item_count;
+  bv.map = s->map;
+  bv.perm = s->perm;
+  bv.tiles = s->tiles;
+  bv.tile_hdr = s->tile_hdr;
+  bv.tile_slots = s->tile_slots;
+  bv.slist = s->slist;
+  bv.scount = s->scount;
+  bv.sstamp = s->sstamp;
+  bv.bin_epoch = s->bin_epoch;
+  return bv;
+}
+
+FallbackView fallback_view(const mpm_sim* s) {
+  FallbackView fb;
+  fb.stamp = s->stamp;
+  fb.list = s->dlist;
+  fb.count = s->dcount;
+  fb.epoch = s->epoch;
+  return fb;
+}
+
+// Return A to all-zero where the last standalone P2G left data (before S or D change).
+mpm_status clean_a(mpm_sim* s) {
+  if (s->a_dirty_d) HIP_TRY(launch_clear_blocks(s->dlist, s->dcount, s->A, s->nblocks, s->stream));
+  if (s->a_dirty_s) HIP_TRY(launch_clear_blocks(s->slist, s->scount, s->A, s->nblocks, s->stream));
+  s->a_dirty_d = s->a_dirty_s = false;
+  return MPM_OK;
+}
+
+// Sort keys, then either reorder the particles (MPM._sort, mpm.js:614-621) or only record the sorted
+// order as a permutation, and rebuild the work items, the block map and S from the sorted keys.
+mpm_status do_bin(mpm_sim* s, bool reorder) {
+  mpm_status st = clean_a(s);
+  if (st) return st;
+  s->bin_epoch++;
+  HIP_TRY(hipMemsetAsync(s->scount, 0, sizeof(uint32_t), s->stream));
+  HIP_TRY(hipMemsetAsync(s->item_count, 0, sizeof(uint32_t), s->stream));
+  s->perm = nullptr;
+  if (s->count == 0) {
+    s->bin_valid = true;
+    return MPM_OK;
+  }
   HIP_TRY(launch_keys(s->P[s->cur], s->count, s->keys[0], s->vals[0], s->stream));
   uint32_t *rk = nullptr, *rv = nullptr;
   HIP_TRY(radix_sort_pairs(s->keys[0], s->vals[0], s->keys[1], s->vals[1], s->count, s->key_bits, s->sort_scratch,
                            s->stream, &rk, &rv));
-  HIP_TRY(launch_permute(s->P[s->cur], s->P[s->cur ^ 1], s->cap, s->count, rv, s->stream));
-  s->cur ^= 1;
+  if (reorder) {
+    HIP_TRY(launch_permute(s->P[s->cur], s->P[s->cur ^ 1], s->cap, s->count, rv, s->stream));
+    s->cur ^= 1;
+  } else {
+    s->perm = rv;
+  }
+  HIP_TRY(hipMemsetAsync(s->map, 0xff, (size_t)s->nbm * sizeof(uint32_t), s->stream));
+  HIP_TRY(launch_bin_flags(rk, s->count, s->flags, s->stream));
+  HIP_TRY(scan_exclusive(s->flags, (size_t)s->count + 1, s->sort_scratch, s->stream));
+  HIP_TRY(launch_bin_emit(rk, s->count, s->flags, s->wl_start, s->wl_block, s->map, s->item_count, s->stream));
+  HIP_TRY(launch_bin_dilate(s->wl_block, s->item_count, s->geo, s->sstamp, s->slist, s->scount, s->bin_epoch,
+                            s->max_items, s->stream));
+  s->bin_valid = true;
   return MPM_OK;
 }
 
-mpm_status do_p2g(mpm_sim* s, bool force_global) {  // mpm.js:607-612
-  if (s->grid_state == kGridAccumulated)
-    HIP_TRY(launch_clear_listed(s->list[s->epoch & 1], s->counts + (s->epoch & 1), s->A, s->nblocks, s->stream));
+float vol4_of(const mpm_sim* s) { return (float)(4.0 * (1.0 / std::pow((double)s->cfg.ppc_axis, 3))); }  // mpm.js:610
+
+// P2G (mpm.js:607-612). Binned: into tiles (+ A for stragglers), grid state kGridTiles. Global: every
+// contribution a global atomic on A, grid state kGridA.
+mpm_status do_p2g(mpm_sim* s, bool force_global) {
+  mpm_status st = clean_a(s);
+  if (st) return st;
+  if (!force_global && !s->bin_valid && (st = do_bin(s, false))) return st;
   s->epoch++;
-  const uint32_t e = s->epoch & 1;
-  HIP_TRY(hipMemsetAsync(s->counts + e, 0, sizeof(uint32_t), s->stream));
-  P2GArgs a;
-  a.P = s->P[s->cur];
-  a.cap = s->cap;
-  a.n = s->count;
-  a.vol4 = (float)(4.0 * (1.0 / std::pow((double)s->cfg.ppc_axis, 3)));  // mpm.js:610
-  a.A = s->A;
-  a.stamp = s->stamp;
-  a.list = s->list[e];
-  a.list_count = s->counts + e;
-  a.epoch = s->epoch;
-  HIP_TRY(launch_p2g_dev(a, s->geo, s->mats_d, force_global ? 0 : s->tile_cap, s->ppg, s->stream));
-  s->grid_state = kGridAccumulated;
+  HIP_TRY(hipMemsetAsync(s->dcount, 0, sizeof(uint32_t), s->stream));
+  const FallbackView fb = fallback_view(s);
+  if (force_global) {
+    HIP_TRY(launch_p2g_global(s->P[s->cur], s->cap, s->count, s->mats_d, vol4_of(s), s->A, fb, nullptr, 0, s->geo,
+                              s->stream));
+    s->grid_state = kGridA;
+  } else {
+    HIP_TRY(launch_p2g_binned(s->P[s->cur], s->cap, s->mats_d, vol4_of(s), bin_view(s), s->A, fb, s->geo, s->max_items,
+                              s->stream));
+    s->grid_state = kGridTiles;
+  }
+  s->a_dirty_d = true;
   return MPM_OK;
 }
 
-mpm_status do_grid(mpm_sim* s, const float g[3], float ff) {  // mpm.js:593-595
+// Grid update (mpm.js:593-595) from tiles + A into V; A returns to zero.
+mpm_status do_grid(mpm_sim* s, const float g[3], float ff) {
   GridArgs ga;
   ga.g[0] = g[0]; ga.g[1] = g[1]; ga.g[2] = g[2];
   ga.bound = s->cfg.bound;
   ga.floor_friction = ff;
   ga.ncol = s->ncol;
-  const uint32_t e = s->epoch & 1;
-  HIP_TRY(launch_grid_update_dev(s->list[e], s->counts + e, s->A, s->V, s->geo, ga, s->cols_d, s->nblocks, s->stream));
+  BinView bv = bin_view(s);
+  if (s->grid_state == kGridA) bv.scount = nullptr;  // global-atomic P2G: only D holds data
+  HIP_TRY(launch_grid_update(bv, fallback_view(s), s->A, s->V, s->geo, ga, s->cols_d, 2 * s->nblocks, s->stream));
+  s->a_dirty_d = false;
   s->grid_state = kGridVelocity;
+  s->v_bin_epoch = bv.scount ? s->bin_epoch : 0xffffffffu;
+  s->v_epoch = s->epoch;
   return MPM_OK;
 }
 
 mpm_status do_g2p(mpm_sim* s, float vmax) {  // mpm.js:598-600
-  HIP_TRY(launch_g2p_dev(s->P[s->cur], s->cap, s->count, s->V, s->geo, s->mats_d, s->cfg.bound, vmax, s->stream));
+  HIP_TRY(launch_g2p(s->P[s->cur], s->cap, s->count, s->V, s->geo, s->mats_d, s->cfg.bound, vmax, s->stream));
   return MPM_OK;
 }
 
@@ -266,8 +363,10 @@ mpm_status mpm_sim_create(mpm_ctx* c, const mpm_config* cfg, mpm_sim** out) {
     return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_sim_create: max_particles = %u outside [1, 2^30]", cfg->max_particles);
   if (cfg->ppc_axis < 1 || cfg->ppc_axis > 8)
     return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_sim_create: ppc_axis = %d outside [1, 8]", cfg->ppc_axis);
-  if (!(cfg->bound >= 2.0f && cfg->bound <= 16.0f))
-    return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_sim_create: bound = %g outside [2, 16]", (double)cfg->bound);
+  // bound >= 3 keeps every stencil inside the grid: G2P clamps x to [bound-1, dims-bound+1]
+  // (mpm.js:502), so the base cell floor(x - 1/2) stays in [1, dims-3].
+  if (!(cfg->bound >= 3.0f && cfg->bound <= 16.0f))
+    return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_sim_create: bound = %g outside [3, 16]", (double)cfg->bound);
   if (cfg->sort_every == 0) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_sim_create: sort_every = 0");
   mpm_status st = set_dev(c);
   if (st != MPM_OK) return st;
@@ -283,6 +382,10 @@ mpm_status mpm_sim_create(mpm_ctx* c, const mpm_config* cfg, mpm_sim** out) {
   int bits = 0;
   while ((1 << bits) < maxd) bits++;
   s->key_bits = 3 * bits;
+  s->geo.mbits = std::max(1, bits - 2);
+  s->nbm = 1u << (3 * s->geo.mbits);
+  s->max_items = std::min<uint32_t>(s->nbm, s->cap);
+  s->tile_slots = (uint32_t)std::min<size_t>(s->max_items, kTileBudgetBytes / (sizeof(int4) * kTileNodes));
   const size_t nodes = (size_t)s->nblocks * 64;
   hipError_t e = hipStreamCreate(&s->stream);
   if (e == hipSuccess) e = dmalloc(&s->P[0], (size_t)s->cap * 8);
@@ -290,16 +393,26 @@ mpm_status mpm_sim_create(mpm_ctx* c, const mpm_config* cfg, mpm_sim** out) {
   if (e == hipSuccess) e = dmalloc(&s->A, nodes);
   if (e == hipSuccess) e = dmalloc(&s->V, nodes);
   if (e == hipSuccess) e = dmalloc(&s->stamp, s->nblocks);
-  if (e == hipSuccess) e = dmalloc(&s->list[0], s->nblocks);
-  if (e == hipSuccess) e = dmalloc(&s->list[1], s->nblocks);
-  if (e == hipSuccess) e = dmalloc(&s->counts, 2);
+  if (e == hipSuccess) e = dmalloc(&s->dlist, s->nblocks);
+  if (e == hipSuccess) e = dmalloc(&s->dcount, 1);
+  if (e == hipSuccess) e = dmalloc(&s->wl_start, (size_t)s->max_items + 1);
+  if (e == hipSuccess) e = dmalloc(&s->wl_block, s->max_items);
+  if (e == hipSuccess) e = dmalloc(&s->item_count, 1);
+  if (e == hipSuccess) e = dmalloc(&s->map, s->nbm);
+  if (e == hipSuccess) e = dmalloc(&s->flags, (size_t)s->cap + 1);
+  if (e == hipSuccess) e = dmalloc(&s->slist, s->nblocks);
+  if (e == hipSuccess) e = dmalloc(&s->scount, 1);
+  if (e == hipSuccess) e = dmalloc(&s->sstamp, s->nblocks);
+  if (e == hipSuccess) e = dmalloc(&s->tiles, (size_t)s->tile_slots * kTileNodes);
+  if (e == hipSuccess) e = dmalloc(&s->tile_hdr, s->tile_slots);
   if (e == hipSuccess) e = dmalloc(&s->mats_d, 192);
   if (e == hipSuccess) e = dmalloc(&s->cols_d, 192);
   if (e == hipSuccess) e = dmalloc(&s->keys[0], s->cap);
   if (e == hipSuccess) e = dmalloc(&s->keys[1], s->cap);
   if (e == hipSuccess) e = dmalloc(&s->vals[0], s->cap);
   if (e == hipSuccess) e = dmalloc(&s->vals[1], s->cap);
-  if (e == hipSuccess) e = dmalloc(&s->sort_scratch, sort_scratch_words(s->cap));
+  if (e == hipSuccess)
+    e = dmalloc(&s->sort_scratch, std::max(sort_scratch_words(s->cap), scan_scratch_words((size_t)s->cap + 1)));
   if (e == hipSuccess) {
     s->staging_records = std::min<uint32_t>(s->cap, 1u << 20);
     e = dmalloc(&s->staging, (size_t)s->staging_records * 8);
@@ -307,7 +420,10 @@ mpm_status mpm_sim_create(mpm_ctx* c, const mpm_config* cfg, mpm_sim** out) {
   if (e == hipSuccess) e = hipMemsetAsync(s->A, 0, nodes * sizeof(int4), s->stream);
   if (e == hipSuccess) e = hipMemsetAsync(s->V, 0, nodes * sizeof(float4), s->stream);
   if (e == hipSuccess) e = hipMemsetAsync(s->stamp, 0xff, (size_t)s->nblocks * 4, s->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(s->counts, 0, 8, s->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(s->sstamp, 0xff, (size_t)s->nblocks * 4, s->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(s->dcount, 0, 4, s->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(s->scount, 0, 4, s->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(s->item_count, 0, 4, s->stream);
   if (e == hipSuccess) e = hipMemsetAsync(s->mats_d, 0, 192 * 4, s->stream);
   if (e == hipSuccess) e = hipMemsetAsync(s->cols_d, 0, 192 * 4, s->stream);
   if (e == hipSuccess) e = hipMemsetAsync(s->P[0], 0, (size_t)s->cap * 128, s->stream);
@@ -371,6 +487,7 @@ mpm_status mpm_write_particles(mpm_sim* s, const float* rec, uint32_t offset, ui
     HIP_TRY(launch_records_in(s->staging, s->P[s->cur], s->cap, offset + done, k, s->stream));
     done += k;
   }
+  s->bin_valid = false;
   HIP_TRY(hipStreamSynchronize(s->stream));
   return MPM_OK;
 }
@@ -394,6 +511,7 @@ mpm_status mpm_read_particles(mpm_sim* s, float* rec, uint32_t offset, uint32_t 
 mpm_status mpm_set_count(mpm_sim* s, uint32_t n) {
   CHECK_SIM(s);
   if (n > s->cap) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_count: %u exceeds capacity %u", n, s->cap);
+  if (n != s->count) s->bin_valid = false;
   s->count = n;
   return MPM_OK;
 }
@@ -416,54 +534,69 @@ mpm_status mpm_get_step_count(mpm_sim* s, uint64_t* n) {
   return MPM_OK;
 }
 
+}  // extern "C"
+
+namespace {
+mpm_status next_event(mpm_sim* s, hipEvent_t* out) {
+  if (s->ev_used == s->ev_pool.size()) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    s->ev_pool.push_back(e);
+  }
+  *out = s->ev_pool[s->ev_used++];
+  return MPM_OK;
+}
+mpm_status mark(mpm_sim* s) {
+  if (!s->profiling) return MPM_OK;
+  hipEvent_t e;
+  mpm_status st = next_event(s, &e);
+  if (st) return st;
+  HIP_TRY(hipEventRecord(e, s->stream));
+  return MPM_OK;
+}
+}  // namespace
+
+extern "C" {
+
 mpm_status mpm_step(mpm_sim* s, uint32_t substeps, const float g[3], float ff, float vmax) {
   CHECK_SIM(s);
   if (!g) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_step: gravity is NULL");
   if (s->count == 0) return MPM_OK;  // mpm.js:584
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  if (s->profiling) {
-    for (auto& x : ev) HIP_TRY(hipEventCreate(&x));
-    for (float& m : s->phase_ms) m = 0.0f;
-  }
   for (uint32_t k = 0; k < substeps; k++) {
-    if (s->profiling) HIP_TRY(hipEventRecord(ev[0], s->stream));
+    mpm_status st;
+    if ((st = mark(s))) return st;
     if (s->step_count % s->cfg.sort_every == 0) {
-      mpm_status st = do_sort(s);
-      if (st) return st;
+      if ((st = do_bin(s, true))) return st;
+    } else if (!s->bin_valid && (st = do_bin(s, false))) {
+      return st;
     }
-    if (s->profiling) HIP_TRY(hipEventRecord(ev[1], s->stream));
-    mpm_status st = do_p2g(s, false);
-    if (st) return st;
-    if (s->profiling) HIP_TRY(hipEventRecord(ev[2], s->stream));
-    st = do_grid(s, g, ff);
-    if (st) return st;
-    if (s->profiling) HIP_TRY(hipEventRecord(ev[3], s->stream));
-    st = do_g2p(s, vmax);
-    if (st) return st;
-    if (s->profiling) {
-      HIP_TRY(hipEventRecord(ev[4], s->stream));
-      HIP_TRY(hipEventSynchronize(ev[4]));
-      for (int p = 0; p < 4; p++) {
-        float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, ev[p], ev[p + 1]));
-        s->phase_ms[p] += ms;
-      }
-    }
+    if ((st = mark(s))) return st;
+    if ((st = do_p2g(s, false))) return st;
+    if ((st = mark(s))) return st;
+    if ((st = do_grid(s, g, ff))) return st;
+    if ((st = mark(s))) return st;
+    if ((st = do_g2p(s, vmax))) return st;
+    if ((st = mark(s))) return st;
     s->step_count++;
   }
-  if (s->profiling)
-    for (auto& x : ev) hipEventDestroy(x);
   return MPM_OK;
 }
 
 mpm_status mpm_p2g(mpm_sim* s, int32_t force_global) {
   CHECK_SIM(s);
-  return do_p2g(s, force_global != 0);
+  mpm_status st = do_p2g(s, force_global != 0);
+  if (st) return st;
+  if (s->grid_state == kGridTiles) {  // fold the tiles into A so the grid reads as the i32 P2G result
+    HIP_TRY(launch_resolve_tiles(bin_view(s), fallback_view(s), s->A, s->geo, 2 * s->nblocks, s->stream));
+    s->grid_state = kGridA;
+    s->a_dirty_s = true;
+  }
+  return MPM_OK;
 }
 
 mpm_status mpm_sort(mpm_sim* s) {
   CHECK_SIM(s);
-  return do_sort(s);
+  return do_bin(s, true);
 }
 
 mpm_status mpm_rescale(mpm_sim* s, float k) {
@@ -477,12 +610,12 @@ mpm_status mpm_read_grid(mpm_sim* s, int32_t* words) {
   if (!words) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_read_grid: words NULL");
   const size_t nodes = (size_t)s->geo.X * s->geo.Y * s->geo.Z;
   if (!s->export_buf) HIP_TRY(dmalloc(&s->export_buf, nodes));
-  if (s->grid_state == kGridClean) {
+  if (s->grid_state != kGridA && s->grid_state != kGridVelocity) {
     std::memset(words, 0, nodes * 16);
     return MPM_OK;
   }
-  HIP_TRY(launch_grid_export(s->A, s->V, s->stamp, s->epoch, s->grid_state == kGridAccumulated ? 0 : 1, s->geo,
-                             s->export_buf, s->stream));
+  HIP_TRY(launch_grid_export(s->A, s->V, s->sstamp, s->v_bin_epoch, s->stamp, s->v_epoch,
+                             s->grid_state == kGridA ? 0 : 1, s->geo, s->export_buf, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   HIP_TRY(hipMemcpy(words, s->export_buf, nodes * 16, hipMemcpyDeviceToHost));
   return MPM_OK;
@@ -497,11 +630,26 @@ mpm_status mpm_sync(mpm_sim* s) {
 mpm_status mpm_set_profiling(mpm_sim* s, int32_t enable) {
   if (!s) return fail(MPM_ERR_INVALID_ARGUMENT, "simulation is NULL");
   s->profiling = enable != 0;
+  s->ev_used = 0;
+  for (float& m : s->phase_ms) m = 0.0f;
+  s->phase_substeps = 0;
   return MPM_OK;
 }
 
 mpm_status mpm_phase_times(mpm_sim* s, float ms[4]) {
   if (!s || !ms) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_phase_times: NULL argument");
+  mpm_status st = set_dev(s->ctx);
+  if (st) return st;
+  const size_t groups = s->ev_used / 5;
+  if (groups) HIP_TRY(hipEventSynchronize(s->ev_pool[groups * 5 - 1]));
+  for (size_t g = 0; g < groups; g++)
+    for (int p = 0; p < 4; p++) {
+      float t = 0;
+      HIP_TRY(hipEventElapsedTime(&t, s->ev_pool[g * 5 + p], s->ev_pool[g * 5 + p + 1]));
+      s->phase_ms[p] += t;
+    }
+  s->phase_substeps += (uint32_t)groups;
+  s->ev_used = 0;
   for (int p = 0; p < 4; p++) ms[p] = s->phase_ms[p];
   return MPM_OK;
 }

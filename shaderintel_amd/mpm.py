@@ -115,7 +115,9 @@ def _check(st: int, what: str = "") -> None:
 
 
 def _ptr(a: np.ndarray) -> ctypes.c_void_p:
-    return ctypes.c_void_p(a.ctypes.data)
+    """Pointer to an array's data that keeps the array alive while the pointer object lives (so a
+    temporary passed straight into a ctypes call is not freed before the call runs)."""
+    return a.ctypes.data_as(ctypes.c_void_p)
 
 
 class Context:
