@@ -142,7 +142,11 @@ inline M3 outer3(V3 a, V3 b) { return {{scale(a, b.x), scale(a, b.y), scale(a, b
 inline float determinant(const M3& m) { return dot(m.c[0], cross(m.c[1], m.c[2])); }
 
 // ---- linalg (src/shaders/linalg.js:14-69) ----
-void jacobi_rotate(M3& A, M3& V, int p, int q) {  // linalg.js:14-25
+// One cyclic-Jacobi rotation (linalg.js:14-25): A <- J^T A J, V <- V J with J the Givens rotation
+// in the (p, q) plane. Only rows / columns p and q change; each changed entry is the two-term sum
+// the full 3x3 products form, in the same order (the remaining terms of those products are
+// multiples of zero, which leave a nonzero sum unchanged).
+void jacobi_rotate(M3& A, M3& V, int p, int q) {
   float apq = at(A, q, p);
   if (fabsf(apq) < 1e-30f) return;
   float theta = (at(A, q, q) - at(A, p, p)) / (2.0f * apq);
@@ -150,10 +154,22 @@ void jacobi_rotate(M3& A, M3& V, int p, int q) {  // linalg.js:14-25
   float t = sgn / (fabsf(theta) + sqrtf(theta * theta + 1.0f));
   float c = 1.0f / sqrtf(t * t + 1.0f);
   float s = t * c;
-  M3 J = I3();
-  at(J, p, p) = c; at(J, q, q) = c; at(J, q, p) = s; at(J, p, q) = -s;
-  A = mulmm(mulmm(transpose(J), A), J);
-  V = mulmm(V, J);
+  float ms = -s;
+  // M = J^T A: rows p and q of every column
+  for (int j = 0; j < 3; j++) {
+    float ap = at(A, j, p), aq = at(A, j, q);
+    at(A, j, p) = c * ap + ms * aq;
+    at(A, j, q) = s * ap + c * aq;
+  }
+  // A = M J and V = V J: columns p and q
+  for (int i = 0; i < 3; i++) {
+    float mp = at(A, p, i), mq = at(A, q, i);
+    at(A, p, i) = mp * c + mq * ms;
+    at(A, q, i) = mp * s + mq * c;
+    float vp = at(V, p, i), vq = at(V, q, i);
+    at(V, p, i) = vp * c + vq * ms;
+    at(V, q, i) = vp * s + vq * c;
+  }
 }
 
 void eig_sym3(const M3& A0, M3& V, V3& e) {  // linalg.js:29-38
@@ -463,13 +479,33 @@ inline uint32_t particle_key(const float* p) {
   return morton3(b[0], b[1], b[2]);
 }
 
-void sort_particles(Scene& S) {  // mpm.js:614-621 (key, scan, reorder, swap)
-  std::vector<uint64_t> kv(S.count);
-  for (uint32_t i = 0; i < S.count; i++) kv[i] = ((uint64_t)particle_key(&S.P[32 * i]) << 32) | i;
-  std::sort(kv.begin(), kv.end());  // (key, index) pairs: a stable order by key
+// MPM._sort (mpm.js:614-621). Particles end up grouped by 4^3-cell block in Morton order, which is
+// the reference's order (its key is the block's Morton code); inside a block they are ordered by
+// (rank within their base cell, base cell), where the rank counts earlier particles of the same
+// cell in the input order. Consecutive particles therefore sit in different cells of the block.
+void sort_particles(Scene& S) {
+  const uint32_t n = S.count;
+  std::vector<uint64_t> kv(n);
+  for (uint32_t i = 0; i < n; i++) kv[i] = ((uint64_t)particle_key(&S.P[32 * (size_t)i]) << 32) | i;
+  std::sort(kv.begin(), kv.end());  // by cell key, ties in input order
+  std::vector<uint32_t> order(n);
+  for (uint32_t b0 = 0; b0 < n;) {  // one block at a time
+    const uint32_t blk = (uint32_t)(kv[b0] >> 38);
+    uint32_t b1 = b0;
+    while (b1 < n && (uint32_t)(kv[b1] >> 38) == blk) b1++;
+    std::vector<std::pair<uint64_t, uint32_t>> loc;  // ((rank << 6) | cell, sorted position)
+    uint32_t rank = 0;
+    for (uint32_t j = b0; j < b1; j++) {
+      rank = (j > b0 && (kv[j] >> 32) == (kv[j - 1] >> 32)) ? rank + 1 : 0;
+      loc.push_back({((uint64_t)rank << 6) | ((kv[j] >> 32) & 63u), j});
+    }
+    std::sort(loc.begin(), loc.end());
+    for (uint32_t j = b0; j < b1; j++) order[j] = (uint32_t)(kv[loc[j - b0].second] & 0xffffffffu);
+    b0 = b1;
+  }
   std::vector<float> P2(S.P.size());
-  for (uint32_t j = 0; j < S.count; j++) std::memcpy(&P2[32 * j], &S.P[32 * (kv[j] & 0xffffffffu)], 128);
-  std::copy(S.P.begin() + 32 * (size_t)S.count, S.P.end(), P2.begin() + 32 * (size_t)S.count);
+  for (uint32_t j = 0; j < n; j++) std::memcpy(&P2[32 * (size_t)j], &S.P[32 * (size_t)order[j]], 128);
+  std::copy(S.P.begin() + 32 * (size_t)n, S.P.end(), P2.begin() + 32 * (size_t)n);
   S.P.swap(P2);
 }
 

@@ -126,12 +126,11 @@ __device__ __forceinline__ float4 grid_node(int4 q, int ix, int iy, int iz, cons
   return make_float4(v.x, v.y, v.z, m);
 }
 
-// G2P (mpm.js:479-559): gather velocity and APIC C, advect, update F, plasticity, stress.
-__global__ __launch_bounds__(kThreads) void k_g2p(float4* __restrict__ P, uint32_t cap, uint32_t n,
-                                                  const float4* __restrict__ V, Geo geo,
-                                                  const float* __restrict__ mats, float bound, float vmax) {
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= n) return;
+// G2P for particle i (mpm.js:479-559): gather velocity and APIC C, advect, update F, plasticity,
+// stress. Writes the new record to the 8 streams and returns it in r.
+__device__ __forceinline__ void g2p_particle(float4* __restrict__ P, uint32_t cap, uint32_t i,
+                                             const float4* __restrict__ V, const Geo& geo,
+                                             const float* __restrict__ mats, float bound, float vmax, float4 r[8]) {
   const float4 x4 = P[i];
   const float vw0 = P[(size_t)cap + i].w;
   const f3 xp = mk(x4.x, x4.y, x4.z);
@@ -216,14 +215,25 @@ __global__ __launch_bounds__(kThreads) void k_g2p(float4* __restrict__ P, uint32
       tau = mul(mul(U, diag(dd)), tr(U));
     }
   }
-  P[i] = make_float4(x.x, x.y, x.z, x4.w);
-  P[(size_t)cap + i] = make_float4(v.x, v.y, v.z, vw);
-  P[(size_t)2 * cap + i] = make_float4(C.c[0].x, C.c[0].y, C.c[0].z, tau.c[0].x);
-  P[(size_t)3 * cap + i] = make_float4(C.c[1].x, C.c[1].y, C.c[1].z, tau.c[1].y);
-  P[(size_t)4 * cap + i] = make_float4(C.c[2].x, C.c[2].y, C.c[2].z, tau.c[2].z);
-  P[(size_t)5 * cap + i] = make_float4(F.c[0].x, F.c[0].y, F.c[0].z, tau.c[1].x);
-  P[(size_t)6 * cap + i] = make_float4(F.c[1].x, F.c[1].y, F.c[1].z, tau.c[2].x);
-  P[(size_t)7 * cap + i] = make_float4(F.c[2].x, F.c[2].y, F.c[2].z, tau.c[2].y);
+  r[0] = make_float4(x.x, x.y, x.z, x4.w);
+  r[1] = make_float4(v.x, v.y, v.z, vw);
+  r[2] = make_float4(C.c[0].x, C.c[0].y, C.c[0].z, tau.c[0].x);
+  r[3] = make_float4(C.c[1].x, C.c[1].y, C.c[1].z, tau.c[1].y);
+  r[4] = make_float4(C.c[2].x, C.c[2].y, C.c[2].z, tau.c[2].z);
+  r[5] = make_float4(F.c[0].x, F.c[0].y, F.c[0].z, tau.c[1].x);
+  r[6] = make_float4(F.c[1].x, F.c[1].y, F.c[1].z, tau.c[2].x);
+  r[7] = make_float4(F.c[2].x, F.c[2].y, F.c[2].z, tau.c[2].y);
+#pragma unroll
+  for (int k = 0; k < 8; k++) P[(size_t)k * cap + i] = r[k];
+}
+
+__global__ __launch_bounds__(kThreads) void k_g2p(float4* __restrict__ P, uint32_t cap, uint32_t n,
+                                                  const float4* __restrict__ V, Geo geo,
+                                                  const float* __restrict__ mats, float bound, float vmax) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  float4 r[8];
+  g2p_particle(P, cap, i, V, geo, mats, bound, vmax, r);
 }
 
 __global__ void k_records_in(const float4* __restrict__ aos, float4* __restrict__ P, uint32_t cap, uint32_t offset,
@@ -308,6 +318,51 @@ __global__ void k_bin_emit(const uint32_t* __restrict__ keys, uint32_t n, const 
   }
 }
 
+
+// Within-block transposition of the sorted order (run once per binning): the particles of block k,
+// sorted by cell, are reordered by (rank within cell, cell). pos(r, c) = sum_c' min(n_c', r) +
+// #{c' < c : n_c' > r} over the 64 cells c' of the block with counts n_c'. Writes out[start + pos] =
+// in[start + j].
+__global__ __launch_bounds__(kThreads) void k_bin_transpose(const uint32_t* __restrict__ keys,
+                                                            const uint32_t* __restrict__ in,
+                                                            const uint32_t* __restrict__ wl_start,
+                                                            const uint32_t* __restrict__ item_count,
+                                                            uint32_t* __restrict__ out) {
+  __shared__ uint32_t ncell[64];
+  __shared__ uint32_t cstart[64];
+  const int tid = threadIdx.x;
+  const uint32_t items = *item_count;
+  for (uint32_t k = blockIdx.x; k < items; k += gridDim.x) {
+    const uint32_t start = wl_start[k], cnt = wl_start[k + 1] - start;
+    __syncthreads();
+    if (tid < 64) ncell[tid] = 0;
+    __syncthreads();
+    for (uint32_t j = tid; j < cnt; j += kThreads) atomicAdd(&ncell[keys[start + j] & 63u], 1u);
+    __syncthreads();
+    if (tid < 64) {  // exclusive scan of the cell counts (one wave)
+      const uint32_t c = ncell[tid];
+      uint32_t inc = c;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(inc, d, 64);
+        if (tid >= d) inc += o;
+      }
+      cstart[tid] = inc - c;
+    }
+    __syncthreads();
+    for (uint32_t j = tid; j < cnt; j += kThreads) {
+      const uint32_t c = keys[start + j] & 63u;
+      const uint32_t r = j - cstart[c];
+      uint32_t pos = 0;
+      for (uint32_t q = 0; q < 64; q++) {
+        const uint32_t nq = ncell[q];
+        pos += min(nq, r) + ((q < c && nq > r) ? 1u : 0u);
+      }
+      out[start + pos] = in[start + j];
+    }
+  }
+}
+
 // First touch of node block b in this binning appends it to S.
 __device__ __forceinline__ void add_to_set(uint32_t b, uint32_t epoch, uint32_t* stamp, uint32_t* list, uint32_t* count) {
   if (__hip_atomic_load(&stamp[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch)
@@ -360,9 +415,9 @@ __device__ __forceinline__ void fallback_particle(const float4 r[8], const float
 
 // P2G over the binning's work items (mpm.js:306-401 restated for aligned particle blocks). One
 // 256-thread workgroup per item at a time (persistent loop). The LDS tile covers the bounding box of
-// the stencils of the item's particles whose base lies within kDrift cells of the block; lanes of one
-// instruction take particles `stride` apart in the sorted order, so they sit in different cells and
-// their LDS atomics hit different nodes.
+// the stencils of the item's particles whose base lies within kDrift cells of the block. Inside a
+// block the sort orders particles by (rank in cell, cell), so the 64 lanes of one instruction read
+// consecutive records (coalesced) that sit in different cells (their LDS atomics hit different nodes).
 __global__ __launch_bounds__(kThreads) void k_p2g_binned(const float4* __restrict__ P, uint32_t cap,
                                                          const float* __restrict__ mats, float vol4, BinView bv, int4* A,
                                                          FallbackView fb, Geo geo) {
@@ -409,12 +464,7 @@ __global__ __launch_bounds__(kThreads) void k_p2g_binned(const float4* __restric
         tile[3 * kTileNodes + t] = 0;
       }
     __syncthreads();
-    const uint32_t stride = cnt >= 512u ? 8u : (cnt >= 64u ? cnt >> 6 : 1u);
-    const uint32_t chunk = 64u * stride;
-    const uint32_t span = ((cnt + chunk - 1) / chunk) * chunk;
-    for (uint32_t v = tid; v < span; v += kThreads) {
-      const uint32_t li = (v / chunk) * chunk + (v & 63u) * stride + (v >> 6) % stride;
-      if (li >= cnt) continue;
+    for (uint32_t li = tid; li < cnt; li += kThreads) {
       const uint32_t j = start + li;
       float4 r[8];
       load_record(P, cap, bv.perm ? bv.perm[j] : j, r);
@@ -440,6 +490,95 @@ __global__ __launch_bounds__(kThreads) void k_p2g_binned(const float4* __restric
       int4* out = bv.tiles + (size_t)k * kTileNodes;
       for (int t = tid; t < nodes; t += kThreads)
         out[t] = make_int4(tile[t], tile[kTileNodes + t], tile[2 * kTileNodes + t], tile[3 * kTileNodes + t]);
+    }
+  }
+}
+
+
+// G2P of substep s fused with P2G of substep s+1 over the binning's work items. Each particle's new
+// record is scattered from registers right after it is written, so P2G never re-reads it. The LDS
+// tile spans the item's previous tile box grown by one cell (a particle's base moves at most one cell
+// per substep while |v| <= vmax < 1), clipped to the item's window; particles outside it take the
+// global-atomic fallback. The stored tile is trimmed to the box of the bases actually scattered.
+__global__ __launch_bounds__(kThreads) void k_g2p_p2g(float4* __restrict__ P, uint32_t cap,
+                                                      const float4* __restrict__ V, Geo geo,
+                                                      const float* __restrict__ mats, float bound, float vmax,
+                                                      float vol4, BinView bv, int4* A, FallbackView fb) {
+  __shared__ int tile[4 * kTileNodes];
+  __shared__ int box[6];
+  const int tid = threadIdx.x;
+  const uint32_t items = *bv.item_count;
+  for (uint32_t k = blockIdx.x; k < items; k += gridDim.x) {
+    const uint32_t start = bv.wl_start[k], cnt = bv.wl_start[k + 1] - start;
+    const uint32_t mb = bv.wl_block[k];
+    const int wx = 4 * (int)compact3(mb) - kDrift, wy = 4 * (int)compact3(mb >> 1) - kDrift,
+              wz = 4 * (int)compact3(mb >> 2) - kDrift;
+    const bool has_slot = k < bv.tile_slots;
+    // node box D of the LDS tile: previous box +- 1, clipped to the window's nodes [w, w + kTileSpan)
+    int dx0 = wx, dy0 = wy, dz0 = wz, dx1 = wx + kTileSpan, dy1 = wy + kTileSpan, dz1 = wz + kTileSpan;
+    if (has_slot) {
+      const int4 h = bv.tile_hdr[k];
+      const int ex = h.w & 255, ey = (h.w >> 8) & 255, ez = (h.w >> 16) & 255;
+      if (ex > 0) {
+        dx0 = max(wx, h.x - 1); dy0 = max(wy, h.y - 1); dz0 = max(wz, h.z - 1);
+        dx1 = min(wx + kTileSpan, h.x + ex + 1); dy1 = min(wy + kTileSpan, h.y + ey + 1);
+        dz1 = min(wz + kTileSpan, h.z + ez + 1);
+      }
+    }
+    const int ex = dx1 - dx0, ey = dy1 - dy0, ez = dz1 - dz0;
+    const int nodes = ex * ey * ez;
+    __syncthreads();  // the previous item's write-out has finished reading the tile
+    if (tid < 3) { box[tid] = INT_MAX; box[3 + tid] = INT_MIN; }
+    if (has_slot)
+      for (int t = tid; t < nodes; t += kThreads) {
+        tile[t] = 0;
+        tile[kTileNodes + t] = 0;
+        tile[2 * kTileNodes + t] = 0;
+        tile[3 * kTileNodes + t] = 0;
+      }
+    __syncthreads();
+    int lo0 = INT_MAX, lo1 = INT_MAX, lo2 = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hi2 = INT_MIN;
+    for (uint32_t li = tid; li < cnt; li += kThreads) {
+      const uint32_t j = start + li;
+      float4 r[8];
+      g2p_particle(P, cap, bv.perm ? bv.perm[j] : j, V, geo, mats, bound, vmax, r);
+      const int b0 = (int)floorf(r[0].x - 0.5f), b1 = (int)floorf(r[0].y - 0.5f), b2 = (int)floorf(r[0].z - 0.5f);
+      const bool inwin = (unsigned)(b0 - wx) < (unsigned)(4 + 2 * kDrift) &&
+                         (unsigned)(b1 - wy) < (unsigned)(4 + 2 * kDrift) && (unsigned)(b2 - wz) < (unsigned)(4 + 2 * kDrift);
+      const bool inbox = b0 >= dx0 && b1 >= dy0 && b2 >= dz0 && b0 + 3 <= dx1 && b1 + 3 <= dy1 && b2 + 3 <= dz1;
+      if (inwin && inbox && has_slot) {
+        lo0 = min(lo0, b0); lo1 = min(lo1, b1); lo2 = min(lo2, b2);
+        hi0 = max(hi0, b0); hi1 = max(hi1, b1); hi2 = max(hi2, b2);
+        const int tb = ((b2 - dz0) * ey + (b1 - dy0)) * ex + (b0 - dx0);
+        p2g_particle(r, mats, vol4, [&](int gx, int gy, int gz, int q0, int q1, int q2, int q3) {
+          const int t = tb + ((gz - b2) * ey + (gy - b1)) * ex + (gx - b0);
+          atomicAdd(&tile[t], q0);
+          atomicAdd(&tile[kTileNodes + t], q1);
+          atomicAdd(&tile[2 * kTileNodes + t], q2);
+          atomicAdd(&tile[3 * kTileNodes + t], q3);
+        });
+      } else {
+        fallback_particle(r, mats, vol4, A, fb, bv.sstamp, bv.bin_epoch, geo);
+      }
+    }
+    lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
+    hi0 = wave_max(hi0); hi1 = wave_max(hi1); hi2 = wave_max(hi2);
+    if ((tid & 63) == 0) {
+      atomicMin(&box[0], lo0); atomicMin(&box[1], lo1); atomicMin(&box[2], lo2);
+      atomicMax(&box[3], hi0); atomicMax(&box[4], hi1); atomicMax(&box[5], hi2);
+    }
+    __syncthreads();
+    if (has_slot) {
+      const bool any = box[0] != INT_MAX;
+      const int ox = box[0], oy = box[1], oz = box[2];
+      const int fx = any ? box[3] - ox + 3 : 0, fy = any ? box[4] - oy + 3 : 0, fz = any ? box[5] - oz + 3 : 0;
+      if (tid == 0) bv.tile_hdr[k] = make_int4(ox, oy, oz, fx | (fy << 8) | (fz << 16));
+      int4* out = bv.tiles + (size_t)k * kTileNodes;
+      for (int t = tid; t < fx * fy * fz; t += kThreads) {
+        const int lx = t % fx, ly = (t / fx) % fy, lz = t / (fx * fy);
+        const int s = ((oz + lz - dz0) * ey + (oy + ly - dy0)) * ex + (ox + lx - dx0);
+        out[t] = make_int4(tile[s], tile[kTileNodes + s], tile[2 * kTileNodes + s], tile[3 * kTileNodes + s]);
+      }
     }
   }
 }
@@ -591,6 +730,12 @@ hipError_t launch_bin_emit(const uint32_t* keys, uint32_t n, const uint32_t* sca
                      item_count);
   return hipGetLastError();
 }
+hipError_t launch_bin_transpose(const uint32_t* keys_sorted, const uint32_t* in, const uint32_t* wl_start,
+                                const uint32_t* item_count, uint32_t* out, uint32_t max_items, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_transpose, dim3(persistent_grid(max_items, 1, 4096)), dim3(kThreads), 0, s, keys_sorted, in,
+                     wl_start, item_count, out);
+  return hipGetLastError();
+}
 hipError_t launch_bin_dilate(const uint32_t* wl_block, const uint32_t* item_count, const Geo& geo, uint32_t* sstamp,
                              uint32_t* slist, uint32_t* scount, uint32_t bin_epoch, uint32_t max_items, hipStream_t s) {
   hipLaunchKernelGGL(k_bin_dilate, dim3(persistent_grid(max_items, 4, 4096)), dim3(kThreads), 0, s, wl_block, item_count,
@@ -626,6 +771,13 @@ hipError_t launch_resolve_tiles(const BinView& bv, const FallbackView& fb, int4*
 }
 hipError_t launch_clear_blocks(const uint32_t* list, const uint32_t* count, int4* A, uint32_t max_blocks, hipStream_t s) {
   hipLaunchKernelGGL(k_clear_blocks, dim3(persistent_grid(max_blocks, 4, 8192)), dim3(kThreads), 0, s, list, count, A);
+  return hipGetLastError();
+}
+hipError_t launch_g2p_p2g(float4* P, uint32_t cap, const float4* V, const Geo& geo, const float* mats, float bound,
+                          float vmax, float vol4, const BinView& bv, int4* A, const FallbackView& fb, uint32_t max_items,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_g2p_p2g, dim3(persistent_grid(max_items, 1, 2048)), dim3(kThreads), 0, s, P, cap, V, geo, mats,
+                     bound, vmax, vol4, bv, A, fb);
   return hipGetLastError();
 }
 hipError_t launch_g2p(float4* P, uint32_t cap, uint32_t n, const float4* V, const Geo& geo, const float* mats, float bound,

@@ -78,6 +78,9 @@ hipError_t launch_permute(const float4* src, float4* dst, uint32_t cap, uint32_t
 hipError_t launch_bin_flags(const uint32_t* keys_sorted, uint32_t n, uint32_t* flags, hipStream_t s);
 hipError_t launch_bin_emit(const uint32_t* keys_sorted, uint32_t n, const uint32_t* scanned, uint32_t* wl_start,
                            uint32_t* wl_block, uint32_t* map, uint32_t* item_count, hipStream_t s);
+// Reorders each block's sorted indices by (rank within cell, cell): out[start + pos] = in[start + j].
+hipError_t launch_bin_transpose(const uint32_t* keys_sorted, const uint32_t* in, const uint32_t* wl_start,
+                                const uint32_t* item_count, uint32_t* out, uint32_t max_items, hipStream_t s);
 hipError_t launch_bin_dilate(const uint32_t* wl_block, const uint32_t* item_count, const Geo& geo, uint32_t* sstamp,
                              uint32_t* slist, uint32_t* scount, uint32_t bin_epoch, uint32_t max_items, hipStream_t s);
 
@@ -97,6 +100,10 @@ hipError_t launch_resolve_tiles(const BinView& bv, const FallbackView& fb, int4*
 hipError_t launch_clear_blocks(const uint32_t* list, const uint32_t* count, int4* A, uint32_t max_blocks, hipStream_t s);
 hipError_t launch_g2p(float4* P, uint32_t cap, uint32_t n, const float4* V, const Geo& geo, const float* mats, float bound,
                       float vmax, hipStream_t s);
+// G2P of this substep fused with the next substep's P2G (tiles + A for the next grid update).
+hipError_t launch_g2p_p2g(float4* P, uint32_t cap, const float4* V, const Geo& geo, const float* mats, float bound,
+                          float vmax, float vol4, const BinView& bv, int4* A, const FallbackView& fb, uint32_t max_items,
+                          hipStream_t s);
 // Reference-layout export: mode 0 = A (i32 words), mode 1 = V (f32 bits, 0 outside S and D).
 hipError_t launch_grid_export(const int4* A, const float4* V, const uint32_t* sstamp, uint32_t bin_epoch,
                               const uint32_t* stamp, uint32_t epoch, int mode, const Geo& geo, int4* out, hipStream_t s);

@@ -131,6 +131,9 @@ MPM_HD uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) { return spread3(x) 
 MPM_HD float& el(m3& m, int c, int r) { return r == 0 ? m.c[c].x : (r == 1 ? m.c[c].y : m.c[c].z); }
 MPM_HD float el(const m3& m, int c, int r) { return r == 0 ? m.c[c].x : (r == 1 ? m.c[c].y : m.c[c].z); }
 
+// One cyclic-Jacobi rotation in the (P, Q) plane: A <- J^T A J, V <- V J. Only rows / columns P and
+// Q change, each as the two-term sum the full 3x3 products would form, in the same order (their other
+// terms are multiples of zero).
 template <int P, int Q>
 MPM_HD void jacobi(m3& A, m3& V) {
   float apq = el(A, Q, P);
@@ -140,10 +143,22 @@ MPM_HD void jacobi(m3& A, m3& V) {
   float t = sgn / (fabsf(theta) + sqrtf(theta * theta + 1.0f));
   float c = 1.0f / sqrtf(t * t + 1.0f);
   float s = t * c;
-  m3 J = ident();
-  el(J, P, P) = c; el(J, Q, Q) = c; el(J, Q, P) = s; el(J, P, Q) = -s;
-  A = mul(mul(tr(J), A), J);
-  V = mul(V, J);
+  float ms = -s;
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    const float ap = el(A, j, P), aq = el(A, j, Q);
+    el(A, j, P) = c * ap + ms * aq;
+    el(A, j, Q) = s * ap + c * aq;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const float mp = el(A, P, i), mq = el(A, Q, i);
+    el(A, P, i) = mp * c + mq * ms;
+    el(A, Q, i) = mp * s + mq * c;
+    const float vp = el(V, P, i), vq = el(V, Q, i);
+    el(V, P, i) = vp * c + vq * ms;
+    el(V, Q, i) = vp * s + vq * c;
+  }
 }
 
 MPM_HD void svd3(const m3& F, m3& U, f3& s, m3& V) {

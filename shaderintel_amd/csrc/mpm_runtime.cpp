@@ -218,16 +218,19 @@ mpm_status do_bin(mpm_sim* s, bool reorder) {
   uint32_t *rk = nullptr, *rv = nullptr;
   HIP_TRY(radix_sort_pairs(s->keys[0], s->vals[0], s->keys[1], s->vals[1], s->count, s->key_bits, s->sort_scratch,
                            s->stream, &rk, &rv));
-  if (reorder) {
-    HIP_TRY(launch_permute(s->P[s->cur], s->P[s->cur ^ 1], s->cap, s->count, rv, s->stream));
-    s->cur ^= 1;
-  } else {
-    s->perm = rv;
-  }
   HIP_TRY(hipMemsetAsync(s->map, 0xff, (size_t)s->nbm * sizeof(uint32_t), s->stream));
   HIP_TRY(launch_bin_flags(rk, s->count, s->flags, s->stream));
   HIP_TRY(scan_exclusive(s->flags, (size_t)s->count + 1, s->sort_scratch, s->stream));
   HIP_TRY(launch_bin_emit(rk, s->count, s->flags, s->wl_start, s->wl_block, s->map, s->item_count, s->stream));
+  // final order inside each block: (rank in cell, cell); written to whichever vals buffer rv is not
+  uint32_t* order = rv == s->vals[0] ? s->vals[1] : s->vals[0];
+  HIP_TRY(launch_bin_transpose(rk, rv, s->wl_start, s->item_count, order, s->max_items, s->stream));
+  if (reorder) {
+    HIP_TRY(launch_permute(s->P[s->cur], s->P[s->cur ^ 1], s->cap, s->count, order, s->stream));
+    s->cur ^= 1;
+  } else {
+    s->perm = order;
+  }
   HIP_TRY(launch_bin_dilate(s->wl_block, s->item_count, s->geo, s->sstamp, s->slist, s->scount, s->bin_epoch,
                             s->max_items, s->stream));
   s->bin_valid = true;
@@ -272,6 +275,18 @@ mpm_status do_grid(mpm_sim* s, const float g[3], float ff) {
   s->grid_state = kGridVelocity;
   s->v_bin_epoch = bv.scount ? s->bin_epoch : 0xffffffffu;
   s->v_epoch = s->epoch;
+  return MPM_OK;
+}
+
+// G2P of this substep fused with the next substep's P2G (grid state kGridTiles for the next grid
+// update). Valid only when the last P2G used the current binning (its tile boxes seed the new ones).
+mpm_status do_g2p_p2g(mpm_sim* s, float vmax) {
+  s->epoch++;
+  HIP_TRY(hipMemsetAsync(s->dcount, 0, sizeof(uint32_t), s->stream));
+  HIP_TRY(launch_g2p_p2g(s->P[s->cur], s->cap, s->V, s->geo, s->mats_d, s->cfg.bound, vmax, vol4_of(s), bin_view(s),
+                         s->A, fallback_view(s), s->max_items, s->stream));
+  s->grid_state = kGridTiles;
+  s->a_dirty_d = true;
   return MPM_OK;
 }
 
@@ -562,20 +577,26 @@ mpm_status mpm_step(mpm_sim* s, uint32_t substeps, const float g[3], float ff, f
   CHECK_SIM(s);
   if (!g) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_step: gravity is NULL");
   if (s->count == 0) return MPM_OK;  // mpm.js:584
+  const bool fuse = !(s->cfg.flags & MPM_FLAG_NO_FUSE);
+  bool p2g_ready = false;  // this substep's P2G was already done by the previous fused kernel
   for (uint32_t k = 0; k < substeps; k++) {
     mpm_status st;
     if ((st = mark(s))) return st;
-    if (s->step_count % s->cfg.sort_every == 0) {
-      if ((st = do_bin(s, true))) return st;
-    } else if (!s->bin_valid && (st = do_bin(s, false))) {
-      return st;
+    if (!p2g_ready) {
+      if (s->step_count % s->cfg.sort_every == 0) {
+        if ((st = do_bin(s, true))) return st;
+      } else if (!s->bin_valid && (st = do_bin(s, false))) {
+        return st;
+      }
     }
     if ((st = mark(s))) return st;
-    if ((st = do_p2g(s, false))) return st;
+    if (!p2g_ready && (st = do_p2g(s, false))) return st;
     if ((st = mark(s))) return st;
     if ((st = do_grid(s, g, ff))) return st;
     if ((st = mark(s))) return st;
-    if ((st = do_g2p(s, vmax))) return st;
+    // fuse this G2P with the next P2G unless the call ends here or the next substep sorts
+    p2g_ready = fuse && k + 1 < substeps && (s->step_count + 1) % s->cfg.sort_every != 0;
+    if ((st = p2g_ready ? do_g2p_p2g(s, vmax) : do_g2p(s, vmax))) return st;
     if ((st = mark(s))) return st;
     s->step_count++;
   }

@@ -200,7 +200,7 @@ class MPM:
     """MLS-MPM simulation on one MI355X (src/physics/mpm.js:82-680)."""
 
     def __init__(self, ctx: Context, maxParticles: int = 262144, grid: int = 64, size=1, origin=None, gravity=None,
-                 substeps: int = 8, timeScale: float = 1, sortEvery: int = 4, ppcAxis: int = 2, tileCap=None):
+                 substeps: int = 8, timeScale: float = 1, sortEvery: int = 4, ppcAxis: int = 2, fuse: bool = True):
         self.ctx = ctx
         self.maxParticles = int(maxParticles or 262144)
         size = list(size) if isinstance(size, (list, tuple)) else [size or 1] * 3
@@ -220,8 +220,9 @@ class MPM:
         self.colliders: list[dict] = []
         self.destroyed = False
         self.numNodes = self.dims[0] * self.dims[1] * self.dims[2]
+        # fuse=False runs every G2P and P2G as separate kernels (the reference's dispatch shape)
         cfg = _Config((ctypes.c_int32 * 3)(*self.dims), self.maxParticles, self.ppcAxis, float(self.bound),
-                      self.sortEvery, 0)
+                      self.sortEvery, 0 if fuse else 1)
         h = ctypes.c_void_p()
         _check(load_library().mpm_sim_create(ctx._h, ctypes.byref(cfg), ctypes.byref(h)), "MPM")
         self._h = h
