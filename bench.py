@@ -65,6 +65,7 @@ def main() -> int:
     ap.add_argument("--scene", default="mixed256")
     ap.add_argument("--cpu-substeps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sweep", action="store_true", help="time each tuning knob setting and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
@@ -89,6 +90,19 @@ def main() -> int:
     run(args.warmup)
     sim.sync()
     log(f"warmup {args.warmup} substeps done")
+    if args.sweep:
+        combos = [dict(g2p_waves=w, p2g_waves=p) for w in (3, 4, 5, 6) for p in (4, 5, 6)]
+        for c in combos:
+            sim.set_tuning(**c)
+            run(8)
+            sim.sync()
+            t0 = time.perf_counter()
+            run(args.steps)
+            sim.sync()
+            dt = time.perf_counter() - t0
+            print(json.dumps({"knobs": c, "ms_per_substep": round(dt * 1e3 / args.steps, 4),
+                              "particle_substeps_per_s": round(n * args.steps / dt, 1)}), flush=True)
+        return 0
     sim.set_profiling(True)
     sim.sync()
     t0 = time.perf_counter()

@@ -46,6 +46,7 @@ enum {
 
 typedef struct mpm_ctx mpm_ctx;
 typedef struct mpm_sim mpm_sim;
+typedef struct mpm_pool mpm_pool;
 
 /* Geometry and capacity of one simulation (mpm.js:89-127). */
 typedef struct mpm_config {
@@ -54,11 +55,8 @@ typedef struct mpm_config {
   int32_t ppc_axis;       /* particles per cell per axis, default 2 (mpm.js:102) */
   float bound;            /* boundary cells, 3..16; 3 in the reference (mpm.js:103) */
   uint32_t sort_every;    /* sort every this many substeps, default 4 (mpm.js:101) */
-  uint32_t flags;         /* MPM_FLAG_* */
+  uint32_t flags;         /* reserved, 0 */
 } mpm_config;
-
-/* flags */
-#define MPM_FLAG_NO_FUSE 1u /* run G2P and the next P2G as separate kernels (reference shape) */
 
 /* ---- device context (replaces SI.init / Context: src/core/context.js:32-100) ---- */
 mpm_status mpm_ctx_create(int device, mpm_ctx** out);
@@ -104,6 +102,61 @@ mpm_status mpm_sync(mpm_sim* sim);
  * {sort, p2g, grid, g2p}, measured with HIP events when profiling is enabled. */
 mpm_status mpm_set_profiling(mpm_sim* sim, int32_t enable);
 mpm_status mpm_phase_times(mpm_sim* sim, float ms[4]);
+
+/* Tuning knobs (no counterpart in the reference; results do not depend on them — every variant
+ * computes the same integer grid sums and the same per-particle float sequence). */
+#define MPM_TUNE_G2P_WAVES 2 /* occupancy target of the G2P kernel, 3..6 waves per SIMD (default 3) */
+#define MPM_TUNE_P2G_WAVES 4 /* occupancy target of the P2G kernel, 4..6 (default 4) */
+mpm_status mpm_set_tuning(mpm_sim* sim, int32_t knob, int32_t value);
+
+/* ---- scene pools: many independent simulations advanced by the same launches ----
+ * No single reference counterpart: the reference runs one MPM object per scene and steps each with
+ * its own dispatches (mpm.js:582-604). A pool holds many scenes (each with its own grid, gravity,
+ * materials and particles) in one particle pool and one node-block pool, and every substep phase is
+ * one launch over all of them; results per scene are the same as for a separate simulation of that
+ * scene. Used for batched serving. Records at this boundary use the reference layout with
+ * scene-local material ids (0..7). */
+typedef struct mpm_pool_config {
+  uint32_t max_particles;   /* particle capacity over all scenes */
+  uint32_t max_scenes;      /* scene slots, 1..4096 */
+  uint32_t max_node_blocks; /* 4x4x4-node blocks over all scenes; a scene needs ceil(X/4)*ceil(Y/4)*ceil(Z/4) */
+  int32_t max_dim;          /* largest grid edge of any scene, 8..1024 */
+  uint32_t sort_every;      /* 0: default 4 */
+  uint32_t flags;           /* reserved, 0 */
+} mpm_pool_config;
+
+typedef struct mpm_scene_desc {
+  int32_t dims[3];         /* grid cells per axis, 8..max_dim */
+  int32_t ppc_axis;        /* particles per cell per axis (particle volume, mpm.js:102, 610) */
+  float bound;             /* boundary cells, 3..16 */
+  float gravity[3];        /* lattice gravity g*dt^2/dx (mpm.js:588) */
+  uint32_t num_materials;  /* rows of `materials`, <= 8 */
+  const float* materials;  /* 12 floats per row (mpm_material_row) */
+  const float* records;    /* count records of 32 floats, reference layout */
+  uint32_t count;
+} mpm_scene_desc;
+
+/* Device bytes mpm_pool_create allocates for this configuration (for admission against an HBM budget). */
+size_t mpm_pool_bytes(const mpm_pool_config* cfg);
+mpm_status mpm_pool_create(mpm_ctx* ctx, const mpm_pool_config* cfg, mpm_pool** out);
+void mpm_pool_destroy(mpm_pool* pool);
+/* Adds a scene with its particles; *scene receives its id. MPM_ERR_OUT_OF_MEMORY when particles,
+ * node blocks or scene slots do not fit (nothing is changed then). */
+mpm_status mpm_pool_add_scene(mpm_pool* pool, const mpm_scene_desc* desc, uint32_t* scene);
+/* Removes a scene; its particles and node blocks are released at the start of the next step. */
+mpm_status mpm_pool_remove_scene(mpm_pool* pool, uint32_t scene);
+/* substeps substeps of every scene (MPM.step's loop, mpm.js:590-602, with each scene's own gravity). */
+mpm_status mpm_pool_step(mpm_pool* pool, uint32_t substeps, float floor_friction, float vmax);
+mpm_status mpm_pool_scene_count(mpm_pool* pool, uint32_t scene, uint32_t* n);
+/* Blocking read of min(capacity, count) records of one scene (reference layout, scene-local
+ * material ids; order is the pool's current particle order). *n receives the scene's count. */
+mpm_status mpm_pool_read_scene(mpm_pool* pool, uint32_t scene, float* rec, uint32_t capacity, uint32_t* n);
+mpm_status mpm_pool_info(mpm_pool* pool, uint32_t* particles, uint32_t* scenes, uint32_t* free_node_blocks);
+mpm_status mpm_pool_sync(mpm_pool* pool);
+/* 1: run whole sort periods (sort_every substeps) as one captured HIP graph, replayed while the
+ * pool's particle count and vmax do not change. */
+mpm_status mpm_pool_use_graphs(mpm_pool* pool, int32_t enable);
+mpm_status mpm_pool_set_tuning(mpm_pool* pool, int32_t knob, int32_t value);
 
 /* ---- primitives ---- */
 /* Scan.run(src, dst, n) — src/prims/scan.js:292-306: exclusive prefix sum of n u32 (wrapping).

@@ -1,9 +1,9 @@
 // MLS-MPM substep kernels for gfx950 (wave64, 160 KiB LDS per CU).
 //
-// Restates src/physics/mpm.js:306-577 (P2G, grid update, G2P, sort key, reorder) for the layouts
-// described in mpm_kernels.hpp. Each float expression follows the op order fixed in mpm_math.hpp,
-// so results are bit-for-bit reproducible; P2G accumulates in 32-bit fixed point with wrapping
-// integer atomics, so its sums do not depend on thread order (mpm.js:64-66, 340-345).
+// Restates src/physics/mpm.js:306-577 (P2G, grid update, G2P, sort key, reorder) for the pooled
+// layouts described in mpm_kernels.hpp. Each float expression follows the op order fixed in
+// mpm_math.hpp, so results are bit-for-bit reproducible; P2G accumulates in 32-bit fixed point with
+// wrapping integer additions, so its sums do not depend on thread order (mpm.js:64-66, 340-345).
 #include <climits>
 
 #include "mpm_kernels.hpp"
@@ -15,14 +15,28 @@ namespace {
 
 constexpr int kThreads = 256;
 
-__device__ __forceinline__ uint32_t node_addr(int gx, int gy, int gz, const Geo& g) {
-  uint32_t b = (uint32_t)(gx >> 2) + (uint32_t)g.nbx * ((uint32_t)(gy >> 2) + (uint32_t)g.nby * (uint32_t)(gz >> 2));
+__device__ __forceinline__ uint32_t node_addr(int gx, int gy, int gz, const SceneGeo& g) {
+  const uint32_t b = g.block_base + (uint32_t)(gx >> 2) +
+                     (uint32_t)g.nbx * ((uint32_t)(gy >> 2) + (uint32_t)g.nby * (uint32_t)(gz >> 2));
   return (b << 6) | (uint32_t)((gx & 3) | ((gy & 3) << 2) | ((gz & 3) << 4));
+}
+
+__device__ __forceinline__ const SceneGeo& scene_of_row(const Pool& pool, uint32_t row) {
+  return pool.scenes[__float_as_uint(pool.mats[12 * row + 9])];
 }
 
 __device__ __forceinline__ void load_record(const float4* __restrict__ P, uint32_t cap, uint32_t i, float4 r[8]) {
 #pragma unroll
   for (int k = 0; k < 8; k++) r[k] = P[(size_t)k * cap + i];
+}
+
+__device__ __forceinline__ uint32_t compact3(uint32_t x) {  // inverse of spread3
+  x &= 0x09249249u;
+  x = (x ^ (x >> 2)) & 0x030c30c3u;
+  x = (x ^ (x >> 4)) & 0x0300f00fu;
+  x = (x ^ (x >> 8)) & 0xff0000ffu;
+  x = (x ^ (x >> 16)) & 0x000003ffu;
+  return x;
 }
 
 // The 27 fixed-point contributions of one particle (mpm.js:318-349). add(gx, gy, gz, q0..q3).
@@ -33,8 +47,7 @@ __device__ __forceinline__ void p2g_particle(const float4 r[8], const float* __r
   const f3 fx = xp - mk((float)bx, (float)by, (float)bz);
   f3 w[3];
   bspline(fx, w);
-  const uint32_t mid = __float_as_uint(r[0].w);
-  const float mass = mats[mid * 12 + 1];
+  const float mass = mats[12 * __float_as_uint(r[0].w) + 1];
   const m3 tau = mat(mk(r[2].w, r[5].w, r[6].w), mk(r[5].w, r[3].w, r[7].w), mk(r[6].w, r[7].w, r[4].w));
   const m3 C = mat(mk(r[2].x, r[2].y, r[2].z), mk(r[3].x, r[3].y, r[3].z), mk(r[4].x, r[4].y, r[4].z));
   const m3 affine = (-vol4) * tau + mass * C;
@@ -77,21 +90,22 @@ __device__ __forceinline__ float collider_sdf(uint32_t kind, f3 d, const float* 
 }
 
 // One node of the grid update (mpm.js:428-470): fixed point -> velocity, gravity, slip walls,
-// Coulomb floor, SDF colliders.
-__device__ __forceinline__ float4 grid_node(int4 q, int ix, int iy, int iz, const Geo& geo, const GridArgs& ga,
+// Coulomb floor, SDF colliders of the node's scene.
+__device__ __forceinline__ float4 grid_node(int4 q, int ix, int iy, int iz, const SceneGeo& geo,
                                             const float* __restrict__ cols) {
   if (!(q.w > 0)) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const float m = (float)q.w * kInvFP;
-  f3 v = mk(((float)q.x * kInvFP) / m + ga.g[0], ((float)q.y * kInvFP) / m + ga.g[1], ((float)q.z * kInvFP) / m + ga.g[2]);
+  f3 v = mk(((float)q.x * kInvFP) / m + geo.g[0], ((float)q.y * kInvFP) / m + geo.g[1],
+            ((float)q.z * kInvFP) / m + geo.g[2]);
   const f3 c = mk((float)ix, (float)iy, (float)iz);
-  const float bound = ga.bound;
+  const float bound = geo.bound;
   const f3 hi = mk((float)geo.X - bound, (float)geo.Y - bound, (float)geo.Z - bound);
   if (c.y < bound && v.y < 0.0f) {
     const float vn = -v.y;
     v.y = 0.0f;
     const float vt = sqrtf(v.x * v.x + v.z * v.z);
     if (vt > 1e-9f) {
-      const float f = fmaxf(0.0f, 1.0f - (ga.floor_friction * vn) / vt);
+      const float f = fmaxf(0.0f, 1.0f - (geo.floor_friction * vn) / vt);
       v = mk(v.x * f, 0.0f, v.z * f);
     }
   }
@@ -100,7 +114,7 @@ __device__ __forceinline__ float4 grid_node(int4 q, int ix, int iy, int iz, cons
   if (c.x > hi.x && v.x > 0.0f) v.x = 0.0f;
   if (c.y > hi.y && v.y > 0.0f) v.y = 0.0f;
   if (c.z > hi.z && v.z > 0.0f) v.z = 0.0f;
-  for (uint32_t k = 0; k < ga.ncol; k++) {
+  for (uint32_t k = 0; k < geo.ncol; k++) {
     const float* h = cols + 12 * k;
     const float* s = h + 4;
     const f3 cv = mk(h[8], h[9], h[10]);
@@ -128,10 +142,12 @@ __device__ __forceinline__ float4 grid_node(int4 q, int ix, int iy, int iz, cons
 
 // G2P for particle i (mpm.js:479-559): gather velocity and APIC C, advect, update F, plasticity,
 // stress. Writes the new record to the 8 streams and returns it in r.
-__device__ __forceinline__ void g2p_particle(float4* __restrict__ P, uint32_t cap, uint32_t i,
-                                             const float4* __restrict__ V, const Geo& geo,
-                                             const float* __restrict__ mats, float bound, float vmax, float4 r[8]) {
+__device__ __forceinline__ void g2p_particle(const Pool& pool, float4* __restrict__ P, uint32_t cap, uint32_t i,
+                                             float vmax, float4 r[8]) {
   const float4 x4 = P[i];
+  const float* m0 = pool.mats + 12 * __float_as_uint(x4.w);
+  const SceneGeo& geo = pool.scenes[__float_as_uint(m0[9])];
+  const float bound = geo.bound;
   const float vw0 = P[(size_t)cap + i].w;
   const f3 xp = mk(x4.x, x4.y, x4.z);
   const int bx = (int)floorf(xp.x - 0.5f), by = (int)floorf(xp.y - 0.5f), bz = (int)floorf(xp.z - 0.5f);
@@ -151,7 +167,7 @@ __device__ __forceinline__ void g2p_particle(float4* __restrict__ P, uint32_t ca
         const float wy = b == 0 ? w[0].y : (b == 1 ? w[1].y : w[2].y);
         const float wz = c == 0 ? w[0].z : (c == 1 ? w[1].z : w[2].z);
         const float wt = (wx * wy) * wz;
-        const float4 g4 = V[node_addr(bx + a, by + b, bz + c, geo)];
+        const float4 g4 = pool.V[node_addr(bx + a, by + b, bz + c, geo)];
         const f3 gv = mk(g4.x, g4.y, g4.z);
         v = v + wt * gv;
         B = B + outer(wt * gv, dpos);
@@ -162,8 +178,6 @@ __device__ __forceinline__ void g2p_particle(float4* __restrict__ P, uint32_t ca
   const float lo = bound - 1.0f;
   const f3 x = mk(clampf(xp.x + v.x, lo, ((float)geo.X - bound) + 1.0f), clampf(xp.y + v.y, lo, ((float)geo.Y - bound) + 1.0f),
                   clampf(xp.z + v.z, lo, ((float)geo.Z - bound) + 1.0f));
-  const uint32_t mid = __float_as_uint(x4.w);
-  const float* m0 = mats + 12 * mid;
   const uint32_t kind = __float_as_uint(m0[0]);
   m3 F, tau;
   float vw = vw0;
@@ -227,51 +241,94 @@ __device__ __forceinline__ void g2p_particle(float4* __restrict__ P, uint32_t ca
   for (int k = 0; k < 8; k++) P[(size_t)k * cap + i] = r[k];
 }
 
-__global__ __launch_bounds__(kThreads) void k_g2p(float4* __restrict__ P, uint32_t cap, uint32_t n,
-                                                  const float4* __restrict__ V, Geo geo,
-                                                  const float* __restrict__ mats, float bound, float vmax) {
+template <int WAVES>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_g2p(
+    Pool pool, float4* __restrict__ P, uint32_t n, float vmax) {
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
   float4 r[8];
-  g2p_particle(P, cap, i, V, geo, mats, bound, vmax, r);
+  g2p_particle(pool, P, pool.cap, i, vmax, r);
 }
 
 __global__ void k_records_in(const float4* __restrict__ aos, float4* __restrict__ P, uint32_t cap, uint32_t offset,
-                             uint32_t n) {
+                             uint32_t n, uint32_t row_base) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // one float4 per thread
   if (t >= (size_t)n * 8) return;
   const uint32_t i = (uint32_t)(t >> 3), k = (uint32_t)(t & 7);
-  P[(size_t)k * cap + offset + i] = aos[t];
+  float4 q = aos[t];
+  if (k == 0) q.w = __uint_as_float(__float_as_uint(q.w) + row_base);  // scene-local -> pool material row
+  P[(size_t)k * cap + offset + i] = q;
 }
 
 __global__ void k_records_out(const float4* __restrict__ P, uint32_t cap, uint32_t offset, uint32_t n,
-                              float4* __restrict__ aos) {
+                              uint32_t row_base, float4* __restrict__ aos) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)n * 8) return;
   const uint32_t i = (uint32_t)(t >> 3), k = (uint32_t)(t & 7);
-  aos[t] = P[(size_t)k * cap + offset + i];
+  float4 q = P[(size_t)k * cap + offset + i];
+  if (k == 0) q.w = __uint_as_float(__float_as_uint(q.w) - row_base);
+  aos[t] = q;
 }
 
-__global__ void k_rescale(float4* __restrict__ P, uint32_t cap, uint32_t n, float k) {  // mpm.js:405-413
+__global__ void k_rescale(float4* __restrict__ P, uint32_t cap, uint32_t offset, uint32_t n, float k) {  // mpm.js:405-413
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
 #pragma unroll
   for (int s = 1; s <= 4; s++) {
-    float4 q = P[(size_t)s * cap + i];
+    float4 q = P[(size_t)s * cap + offset + i];
     q.x = q.x * k; q.y = q.y * k; q.z = q.z * k;
-    P[(size_t)s * cap + i] = q;
+    P[(size_t)s * cap + offset + i] = q;
   }
 }
 
-// Sort key: Morton code of the stencil base cell (a refinement of the reference's 4^3-block key,
-// mpm.js:567-568); vals = current index, so the radix sort's stable order breaks ties by it.
-__global__ void k_keys(const float4* __restrict__ P, uint32_t n, uint32_t* __restrict__ keys,
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t value, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = value;
+}
+
+__global__ void k_begin_p2g(uint32_t* ctr) {
+  ctr[kCtrEpoch] += 1u;
+  ctr[kCtrD] = 0u;
+  ctr[kCtrWork] = 0u;
+}
+
+__global__ void k_begin_bin(uint32_t* ctr) {
+  ctr[kCtrBinEpoch] += 1u;
+  ctr[kCtrS] = 0u;
+  ctr[kCtrItems] = 0u;
+}
+
+__global__ void k_mark_v(uint32_t* ctr, int use_s) {
+  ctr[kCtrVEpoch] = ctr[kCtrEpoch];
+  ctr[kCtrVBinEpoch] = use_s ? ctr[kCtrBinEpoch] : 0xffffffffu;
+}
+
+// Sort key: scene in the top bits, then the Morton code of the stencil base cell (a refinement of
+// the reference's 4^3-block key, mpm.js:567-568); vals = current index, so the radix sort's
+// stable order breaks ties by it.
+__global__ void k_keys(Pool pool, const float4* __restrict__ P, uint32_t n, uint32_t* __restrict__ keys,
                        uint32_t* __restrict__ vals) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 x = P[i];
-  keys[i] = morton3((uint32_t)fmaxf(floorf(x.x - 0.5f), 0.0f), (uint32_t)fmaxf(floorf(x.y - 0.5f), 0.0f),
+  const uint32_t scene = __float_as_uint(pool.mats[12 * __float_as_uint(x.w) + 9]);
+  if (!pool.scenes[scene].active) {
+    keys[i] = 0xffffffffu;
+    vals[i] = i;
+    return;
+  }
+  keys[i] = (scene << pool.key_bits) |
+            morton3((uint32_t)fmaxf(floorf(x.x - 0.5f), 0.0f), (uint32_t)fmaxf(floorf(x.y - 0.5f), 0.0f),
                     (uint32_t)fmaxf(floorf(x.z - 0.5f), 0.0f));
+  vals[i] = i;
+}
+
+__global__ void k_live_keys(Pool pool, const float4* __restrict__ P, uint32_t n, uint32_t* __restrict__ keys,
+                            uint32_t* __restrict__ vals) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t scene = __float_as_uint(pool.mats[12 * __float_as_uint(P[i].w) + 9]);
+  keys[i] = pool.scenes[scene].active ? 0u : 1u;
   vals[i] = i;
 }
 
@@ -285,55 +342,43 @@ __global__ void k_permute(const float4* __restrict__ src, float4* __restrict__ d
 }
 
 // ---- binning ----
-__device__ __forceinline__ uint32_t compact3(uint32_t x) {  // inverse of spread3
-  x &= 0x09249249u;
-  x = (x ^ (x >> 2)) & 0x030c30c3u;
-  x = (x ^ (x >> 4)) & 0x0300f00fu;
-  x = (x ^ (x >> 8)) & 0xff0000ffu;
-  x = (x ^ (x >> 16)) & 0x000003ffu;
-  return x;
-}
-
 __global__ void k_bin_flags(const uint32_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ flags) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n) return;
   flags[i] = (i < n && (i == 0 || (keys[i] >> 6) != (keys[i - 1] >> 6))) ? 1u : 0u;
 }
 
-__global__ void k_bin_emit(const uint32_t* __restrict__ keys, uint32_t n, const uint32_t* __restrict__ scanned,
-                           uint32_t* __restrict__ wl_start, uint32_t* __restrict__ wl_block, uint32_t* __restrict__ map,
-                           uint32_t* __restrict__ item_count) {
+__global__ void k_bin_emit(Pool pool, const uint32_t* __restrict__ keys, uint32_t n,
+                           const uint32_t* __restrict__ scanned) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t blk = keys[i] >> 6;
   if (i == 0 || (keys[i - 1] >> 6) != blk) {
     const uint32_t k = scanned[i];
-    wl_start[k] = i;
-    wl_block[k] = blk;
-    map[blk] = k;
+    const uint32_t scene = keys[i] >> pool.key_bits;
+    const uint32_t mblock = blk & ((1u << (pool.key_bits - 6)) - 1u);
+    pool.wl_start[k] = i;
+    pool.wl_block[k] = blk;
+    pool.map[pool.scenes[scene].map_base + mblock] = k;
   }
   if (i == n - 1) {
-    wl_start[scanned[n]] = n;
-    *item_count = scanned[n];
+    pool.wl_start[scanned[n]] = n;
+    pool.ctr[kCtrItems] = scanned[n];
   }
 }
 
-
-// Within-block transposition of the sorted order (run once per binning): the particles of block k,
-// sorted by cell, are reordered by (rank within cell, cell). pos(r, c) = sum_c' min(n_c', r) +
-// #{c' < c : n_c' > r} over the 64 cells c' of the block with counts n_c'. Writes out[start + pos] =
-// in[start + j].
-__global__ __launch_bounds__(kThreads) void k_bin_transpose(const uint32_t* __restrict__ keys,
+// Within-block transposition of the sorted order: the particles of item k, sorted by cell, are
+// reordered by (rank within cell, cell). pos(r, c) = sum_c' min(n_c', r) + #{c' < c : n_c' > r} over
+// the 64 cells c' of the block with counts n_c'. out[start + pos] = in[start + j].
+__global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uint32_t* __restrict__ keys,
                                                             const uint32_t* __restrict__ in,
-                                                            const uint32_t* __restrict__ wl_start,
-                                                            const uint32_t* __restrict__ item_count,
                                                             uint32_t* __restrict__ out) {
   __shared__ uint32_t ncell[64];
   __shared__ uint32_t cstart[64];
   const int tid = threadIdx.x;
-  const uint32_t items = *item_count;
+  const uint32_t items = pool.ctr[kCtrItems];
   for (uint32_t k = blockIdx.x; k < items; k += gridDim.x) {
-    const uint32_t start = wl_start[k], cnt = wl_start[k + 1] - start;
+    const uint32_t start = pool.wl_start[k], cnt = pool.wl_start[k + 1] - start;
     __syncthreads();
     if (tid < 64) ncell[tid] = 0;
     __syncthreads();
@@ -363,49 +408,51 @@ __global__ __launch_bounds__(kThreads) void k_bin_transpose(const uint32_t* __re
   }
 }
 
-// First touch of node block b in this binning appends it to S.
+// First touch of node block b under `epoch` appends it to `list`.
 __device__ __forceinline__ void add_to_set(uint32_t b, uint32_t epoch, uint32_t* stamp, uint32_t* list, uint32_t* count) {
   if (__hip_atomic_load(&stamp[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch)
     if (atomicExch(&stamp[b], epoch) != epoch) list[atomicAdd(count, 1u)] = b;
 }
 
-__global__ __launch_bounds__(kThreads) void k_bin_dilate(const uint32_t* __restrict__ wl_block,
-                                                         const uint32_t* __restrict__ item_count, Geo geo,
-                                                         uint32_t* __restrict__ sstamp, uint32_t* __restrict__ slist,
-                                                         uint32_t* __restrict__ scount, uint32_t bin_epoch) {
-  const uint32_t cnt = *item_count;
-  // 64 threads per item: one per node block in [B-1, B+2]^3
+// S: the node blocks [B-1, B+2]^3 of every work item (everything its tile can reach).
+__global__ __launch_bounds__(kThreads) void k_bin_dilate(Pool pool) {
+  const uint32_t cnt = pool.ctr[kCtrItems];
+  const uint32_t epoch = pool.ctr[kCtrBinEpoch];
+  const uint32_t mshift = pool.key_bits - 6;
   for (uint32_t t = blockIdx.x * kThreads + threadIdx.x; t < cnt * 64; t += gridDim.x * kThreads) {
     const uint32_t k = t >> 6, d = t & 63;
-    const uint32_t mb = wl_block[k];
+    const uint32_t wb = pool.wl_block[k];
+    const SceneGeo& geo = pool.scenes[wb >> mshift];
+    const uint32_t mb = wb & ((1u << mshift) - 1u);
     const int gx = (int)compact3(mb) - 1 + (int)(d & 3), gy = (int)compact3(mb >> 1) - 1 + (int)((d >> 2) & 3),
               gz = (int)compact3(mb >> 2) - 1 + (int)(d >> 4);
     if (gx < 0 || gy < 0 || gz < 0 || gx >= geo.nbx || gy >= geo.nby || gz >= geo.nbz) continue;
-    add_to_set((uint32_t)gx + (uint32_t)geo.nbx * ((uint32_t)gy + (uint32_t)geo.nby * (uint32_t)gz), bin_epoch, sstamp,
-               slist, scount);
+    add_to_set(geo.block_base + (uint32_t)gx + (uint32_t)geo.nbx * ((uint32_t)gy + (uint32_t)geo.nby * (uint32_t)gz),
+               epoch, pool.sstamp, pool.slist, pool.ctr + kCtrS);
   }
 }
 
-// A fallback write to node block b: mark it written in this P2G (so the grid update reads and clears
-// A there) and list it in D unless S already covers it.
-__device__ __forceinline__ void stamp_fallback(uint32_t b, const FallbackView& fb, const uint32_t* sstamp,
+// A fallback write to node block b: mark it written in this P2G (the grid update reads and clears A
+// there) and list it in D unless S already covers it (use_s).
+__device__ __forceinline__ void stamp_fallback(const Pool& pool, uint32_t b, uint32_t epoch, bool use_s,
                                                uint32_t bin_epoch) {
-  if (__hip_atomic_load(&fb.stamp[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != fb.epoch)
-    if (atomicExch(&fb.stamp[b], fb.epoch) != fb.epoch)
-      if (sstamp == nullptr || sstamp[b] != bin_epoch) fb.list[atomicAdd(fb.count, 1u)] = b;
+  if (__hip_atomic_load(&pool.stamp[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch)
+    if (atomicExch(&pool.stamp[b], epoch) != epoch)
+      if (!use_s || pool.sstamp[b] != bin_epoch) pool.dlist[atomicAdd(pool.ctr + kCtrD, 1u)] = b;
 }
 
-__device__ __forceinline__ void fallback_particle(const float4 r[8], const float* __restrict__ mats, float vol4, int4* A,
-                                                  const FallbackView& fb, const uint32_t* sstamp, uint32_t bin_epoch,
-                                                  const Geo& geo) {
+__device__ __forceinline__ void fallback_particle(const Pool& pool, const float4 r[8], uint32_t epoch, bool use_s,
+                                                  uint32_t bin_epoch) {
+  const SceneGeo& geo = scene_of_row(pool, __float_as_uint(r[0].w));
+  const float vol4 = geo.vol4;
   const int bx = (int)floorf(r[0].x - 0.5f), by = (int)floorf(r[0].y - 0.5f), bz = (int)floorf(r[0].z - 0.5f);
   for (int kz = bz >> 2; kz <= (bz + 2) >> 2; kz++)
     for (int ky = by >> 2; ky <= (by + 2) >> 2; ky++)
       for (int kx = bx >> 2; kx <= (bx + 2) >> 2; kx++)
-        stamp_fallback((uint32_t)kx + (uint32_t)geo.nbx * ((uint32_t)ky + (uint32_t)geo.nby * (uint32_t)kz), fb, sstamp,
-                       bin_epoch);
-  p2g_particle(r, mats, vol4, [&](int gx, int gy, int gz, int q0, int q1, int q2, int q3) {
-    int* g = reinterpret_cast<int*>(A + node_addr(gx, gy, gz, geo));
+        stamp_fallback(pool, geo.block_base + (uint32_t)kx + (uint32_t)geo.nbx * ((uint32_t)ky + (uint32_t)geo.nby * (uint32_t)kz),
+                       epoch, use_s, bin_epoch);
+  p2g_particle(r, pool.mats, vol4, [&](int gx, int gy, int gz, int q0, int q1, int q2, int q3) {
+    int* g = reinterpret_cast<int*>(pool.A + node_addr(gx, gy, gz, geo));
     atomicAdd(g, q0);
     atomicAdd(g + 1, q1);
     atomicAdd(g + 2, q2);
@@ -414,30 +461,39 @@ __device__ __forceinline__ void fallback_particle(const float4 r[8], const float
 }
 
 // P2G over the binning's work items (mpm.js:306-401 restated for aligned particle blocks). One
-// 256-thread workgroup per item at a time (persistent loop). The LDS tile covers the bounding box of
-// the stencils of the item's particles whose base lies within kDrift cells of the block. Inside a
-// block the sort orders particles by (rank in cell, cell), so the 64 lanes of one instruction read
-// consecutive records (coalesced) that sit in different cells (their LDS atomics hit different nodes).
-__global__ __launch_bounds__(kThreads) void k_p2g_binned(const float4* __restrict__ P, uint32_t cap,
-                                                         const float* __restrict__ mats, float vol4, BinView bv, int4* A,
-                                                         FallbackView fb, Geo geo) {
+// 256-thread workgroup per item at a time, items handed out by a device counter. The LDS tile
+// covers the bounding box of the stencils of the item's particles whose base lies within kDrift
+// cells of the block. Inside a block the sort orders particles by (rank in cell, cell), so the 64
+// lanes of one instruction read consecutive records (coalesced) that sit in different cells (their
+// LDS atomics hit different nodes).
+template <int WAVES>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_p2g_binned(
+    Pool pool, const float4* __restrict__ P, const uint32_t* __restrict__ perm, float4* __restrict__ dst) {
   __shared__ int tile[4 * kTileNodes];
   __shared__ int box[6];
+  __shared__ uint32_t s_item;
   const int tid = threadIdx.x;
-  const uint32_t items = *bv.item_count;
-  for (uint32_t k = blockIdx.x; k < items; k += gridDim.x) {
-    const uint32_t start = bv.wl_start[k], cnt = bv.wl_start[k + 1] - start;
-    const uint32_t mb = bv.wl_block[k];
+  const uint32_t items = pool.ctr[kCtrItems];
+  const uint32_t epoch = pool.ctr[kCtrEpoch], bin_epoch = pool.ctr[kCtrBinEpoch];
+  const uint32_t mshift = pool.key_bits - 6;
+  for (;;) {
+    __syncthreads();  // the previous item is finished with the tile and with s_item
+    if (tid == 0) s_item = atomicAdd(pool.ctr + kCtrWork, 1u);
+    __syncthreads();
+    const uint32_t k = s_item;
+    if (k >= items) break;
+    const uint32_t start = pool.wl_start[k], cnt = pool.wl_start[k + 1] - start;
+    const uint32_t mb = pool.wl_block[k] & ((1u << mshift) - 1u);
+    const float vol4 = pool.scenes[pool.wl_block[k] >> mshift].vol4;
     const int wx = 4 * (int)compact3(mb) - kDrift, wy = 4 * (int)compact3(mb >> 1) - kDrift,
               wz = 4 * (int)compact3(mb >> 2) - kDrift;  // lowest in-window base
-    const bool has_slot = k < bv.tile_slots;
-    __syncthreads();  // the previous item's write-out has finished reading the tile
+    const bool has_slot = k < pool.tile_slots;
     if (tid < 3) { box[tid] = INT_MAX; box[3 + tid] = INT_MIN; }
     __syncthreads();
     int lo0 = INT_MAX, lo1 = INT_MAX, lo2 = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hi2 = INT_MIN;
     for (uint32_t v = tid; v < cnt; v += kThreads) {
       const uint32_t j = start + v;
-      const float4 x = P[bv.perm ? bv.perm[j] : j];
+      const float4 x = P[perm ? perm[j] : j];
       const int b0 = (int)floorf(x.x - 0.5f), b1 = (int)floorf(x.y - 0.5f), b2 = (int)floorf(x.z - 0.5f);
       if ((unsigned)(b0 - wx) < (unsigned)(4 + 2 * kDrift) && (unsigned)(b1 - wy) < (unsigned)(4 + 2 * kDrift) &&
           (unsigned)(b2 - wz) < (unsigned)(4 + 2 * kDrift)) {
@@ -467,13 +523,16 @@ __global__ __launch_bounds__(kThreads) void k_p2g_binned(const float4* __restric
     for (uint32_t li = tid; li < cnt; li += kThreads) {
       const uint32_t j = start + li;
       float4 r[8];
-      load_record(P, cap, bv.perm ? bv.perm[j] : j, r);
+      load_record(P, pool.cap, perm ? perm[j] : j, r);
+      if (dst)  // the sort's reorder folded into this pass: record j of the sorted order goes to slot j
+#pragma unroll
+        for (int q = 0; q < 8; q++) dst[(size_t)q * pool.cap + j] = r[q];
       const int b0 = (int)floorf(r[0].x - 0.5f), b1 = (int)floorf(r[0].y - 0.5f), b2 = (int)floorf(r[0].z - 0.5f);
       const bool inwin = (unsigned)(b0 - wx) < (unsigned)(4 + 2 * kDrift) &&
                          (unsigned)(b1 - wy) < (unsigned)(4 + 2 * kDrift) && (unsigned)(b2 - wz) < (unsigned)(4 + 2 * kDrift);
       if (inwin && has_slot) {
         const int tb = ((b2 - oz) * ey + (b1 - oy)) * ex + (b0 - ox);
-        p2g_particle(r, mats, vol4, [&](int gx, int gy, int gz, int q0, int q1, int q2, int q3) {
+        p2g_particle(r, pool.mats, vol4, [&](int gx, int gy, int gz, int q0, int q1, int q2, int q3) {
           const int t = tb + ((gz - b2) * ey + (gy - b1)) * ex + (gx - b0);
           atomicAdd(&tile[t], q0);
           atomicAdd(&tile[kTileNodes + t], q1);
@@ -481,126 +540,41 @@ __global__ __launch_bounds__(kThreads) void k_p2g_binned(const float4* __restric
           atomicAdd(&tile[3 * kTileNodes + t], q3);
         });
       } else {
-        fallback_particle(r, mats, vol4, A, fb, bv.sstamp, bv.bin_epoch, geo);
+        fallback_particle(pool, r, epoch, true, bin_epoch);
       }
     }
     __syncthreads();
     if (has_slot) {
-      if (tid == 0) bv.tile_hdr[k] = make_int4(ox, oy, oz, ex | (ey << 8) | (ez << 16));
-      int4* out = bv.tiles + (size_t)k * kTileNodes;
+      if (tid == 0) pool.tile_hdr[k] = make_int4(ox, oy, oz, ex | (ey << 8) | (ez << 16));
+      int4* out = pool.tiles + (size_t)k * kTileNodes;
       for (int t = tid; t < nodes; t += kThreads)
         out[t] = make_int4(tile[t], tile[kTileNodes + t], tile[2 * kTileNodes + t], tile[3 * kTileNodes + t]);
     }
   }
 }
 
-
-// G2P of substep s fused with P2G of substep s+1 over the binning's work items. Each particle's new
-// record is scattered from registers right after it is written, so P2G never re-reads it. The LDS
-// tile spans the item's previous tile box grown by one cell (a particle's base moves at most one cell
-// per substep while |v| <= vmax < 1), clipped to the item's window; particles outside it take the
-// global-atomic fallback. The stored tile is trimmed to the box of the bases actually scattered.
-__global__ __launch_bounds__(kThreads) void k_g2p_p2g(float4* __restrict__ P, uint32_t cap,
-                                                      const float4* __restrict__ V, Geo geo,
-                                                      const float* __restrict__ mats, float bound, float vmax,
-                                                      float vol4, BinView bv, int4* A, FallbackView fb) {
-  __shared__ int tile[4 * kTileNodes];
-  __shared__ int box[6];
-  const int tid = threadIdx.x;
-  const uint32_t items = *bv.item_count;
-  for (uint32_t k = blockIdx.x; k < items; k += gridDim.x) {
-    const uint32_t start = bv.wl_start[k], cnt = bv.wl_start[k + 1] - start;
-    const uint32_t mb = bv.wl_block[k];
-    const int wx = 4 * (int)compact3(mb) - kDrift, wy = 4 * (int)compact3(mb >> 1) - kDrift,
-              wz = 4 * (int)compact3(mb >> 2) - kDrift;
-    const bool has_slot = k < bv.tile_slots;
-    // node box D of the LDS tile: previous box +- 1, clipped to the window's nodes [w, w + kTileSpan)
-    int dx0 = wx, dy0 = wy, dz0 = wz, dx1 = wx + kTileSpan, dy1 = wy + kTileSpan, dz1 = wz + kTileSpan;
-    if (has_slot) {
-      const int4 h = bv.tile_hdr[k];
-      const int ex = h.w & 255, ey = (h.w >> 8) & 255, ez = (h.w >> 16) & 255;
-      if (ex > 0) {
-        dx0 = max(wx, h.x - 1); dy0 = max(wy, h.y - 1); dz0 = max(wz, h.z - 1);
-        dx1 = min(wx + kTileSpan, h.x + ex + 1); dy1 = min(wy + kTileSpan, h.y + ey + 1);
-        dz1 = min(wz + kTileSpan, h.z + ez + 1);
-      }
-    }
-    const int ex = dx1 - dx0, ey = dy1 - dy0, ez = dz1 - dz0;
-    const int nodes = ex * ey * ez;
-    __syncthreads();  // the previous item's write-out has finished reading the tile
-    if (tid < 3) { box[tid] = INT_MAX; box[3 + tid] = INT_MIN; }
-    if (has_slot)
-      for (int t = tid; t < nodes; t += kThreads) {
-        tile[t] = 0;
-        tile[kTileNodes + t] = 0;
-        tile[2 * kTileNodes + t] = 0;
-        tile[3 * kTileNodes + t] = 0;
-      }
-    __syncthreads();
-    int lo0 = INT_MAX, lo1 = INT_MAX, lo2 = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hi2 = INT_MIN;
-    for (uint32_t li = tid; li < cnt; li += kThreads) {
-      const uint32_t j = start + li;
-      float4 r[8];
-      g2p_particle(P, cap, bv.perm ? bv.perm[j] : j, V, geo, mats, bound, vmax, r);
-      const int b0 = (int)floorf(r[0].x - 0.5f), b1 = (int)floorf(r[0].y - 0.5f), b2 = (int)floorf(r[0].z - 0.5f);
-      const bool inwin = (unsigned)(b0 - wx) < (unsigned)(4 + 2 * kDrift) &&
-                         (unsigned)(b1 - wy) < (unsigned)(4 + 2 * kDrift) && (unsigned)(b2 - wz) < (unsigned)(4 + 2 * kDrift);
-      const bool inbox = b0 >= dx0 && b1 >= dy0 && b2 >= dz0 && b0 + 3 <= dx1 && b1 + 3 <= dy1 && b2 + 3 <= dz1;
-      if (inwin && inbox && has_slot) {
-        lo0 = min(lo0, b0); lo1 = min(lo1, b1); lo2 = min(lo2, b2);
-        hi0 = max(hi0, b0); hi1 = max(hi1, b1); hi2 = max(hi2, b2);
-        const int tb = ((b2 - dz0) * ey + (b1 - dy0)) * ex + (b0 - dx0);
-        p2g_particle(r, mats, vol4, [&](int gx, int gy, int gz, int q0, int q1, int q2, int q3) {
-          const int t = tb + ((gz - b2) * ey + (gy - b1)) * ex + (gx - b0);
-          atomicAdd(&tile[t], q0);
-          atomicAdd(&tile[kTileNodes + t], q1);
-          atomicAdd(&tile[2 * kTileNodes + t], q2);
-          atomicAdd(&tile[3 * kTileNodes + t], q3);
-        });
-      } else {
-        fallback_particle(r, mats, vol4, A, fb, bv.sstamp, bv.bin_epoch, geo);
-      }
-    }
-    lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
-    hi0 = wave_max(hi0); hi1 = wave_max(hi1); hi2 = wave_max(hi2);
-    if ((tid & 63) == 0) {
-      atomicMin(&box[0], lo0); atomicMin(&box[1], lo1); atomicMin(&box[2], lo2);
-      atomicMax(&box[3], hi0); atomicMax(&box[4], hi1); atomicMax(&box[5], hi2);
-    }
-    __syncthreads();
-    if (has_slot) {
-      const bool any = box[0] != INT_MAX;
-      const int ox = box[0], oy = box[1], oz = box[2];
-      const int fx = any ? box[3] - ox + 3 : 0, fy = any ? box[4] - oy + 3 : 0, fz = any ? box[5] - oz + 3 : 0;
-      if (tid == 0) bv.tile_hdr[k] = make_int4(ox, oy, oz, fx | (fy << 8) | (fz << 16));
-      int4* out = bv.tiles + (size_t)k * kTileNodes;
-      for (int t = tid; t < fx * fy * fz; t += kThreads) {
-        const int lx = t % fx, ly = (t / fx) % fy, lz = t / (fx * fy);
-        const int s = ((oz + lz - dz0) * ey + (oy + ly - dy0)) * ex + (ox + lx - dx0);
-        out[t] = make_int4(tile[s], tile[kTileNodes + s], tile[2 * kTileNodes + s], tile[3 * kTileNodes + s]);
-      }
-    }
-  }
-}
-
-// Global-atomic P2G (the reference's forceGlobal path, mpm.js:342-345), any particle order.
-__global__ __launch_bounds__(kThreads) void k_p2g_global(const float4* __restrict__ P, uint32_t cap, uint32_t n,
-                                                         const float* __restrict__ mats, float vol4, int4* A,
-                                                         FallbackView fb, const uint32_t* sstamp, uint32_t bin_epoch,
-                                                         Geo geo) {
+// Global-atomic P2G (the reference's forceGlobal path, mpm.js:342-345), any particle order; every
+// written node block goes to D.
+__global__ __launch_bounds__(kThreads) void k_p2g_global(Pool pool, const float4* __restrict__ P, uint32_t n) {
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
   float4 r[8];
-  load_record(P, cap, i, r);
-  fallback_particle(r, mats, vol4, A, fb, sstamp, bin_epoch, geo);
+  load_record(P, pool.cap, i, r);
+  fallback_particle(pool, r, pool.ctr[kCtrEpoch], false, 0);
 }
 
-// Sum of the P2G result at node (n0, n1, n2) of node block (g0, g1, g2): the tiles of particle blocks
-// G-2..G+1 per axis that cover the node. All lanes of the wave share the node block, so the map and
-// header reads are uniform.
-__device__ __forceinline__ int4 pull_tiles(const BinView& bv, const Geo& geo, int g0, int g1, int g2, int n0, int n1,
-                                           int n2, int4 acc) {
-  const int pb = 1 << geo.mbits;  // particle blocks per axis covered by the Morton code
+struct Words64 { uint32_t w[64]; };
+__global__ void k_write_words(Words64 src, uint32_t* dst, uint32_t n) {
+  const uint32_t i = threadIdx.x;
+  if (i < n) dst[i] = src.w[i];
+}
+
+// Sum of the P2G result at scene-local node (n0, n1, n2) of node block (g0, g1, g2): the tiles of
+// particle blocks G-2..G+1 per axis that cover the node. All lanes of the wave share the node
+// block, so the map and header reads are uniform.
+__device__ __forceinline__ int4 pull_tiles(const Pool& pool, const SceneGeo& geo, int g0, int g1, int g2, int n0,
+                                           int n1, int n2, int4 acc) {
+  const int pb = 1 << geo.mbits;  // particle blocks per axis covered by the scene's Morton code
   for (int dz = -2; dz <= 1; dz++) {
     const int bz = g2 + dz;
     if (bz < 0 || bz >= pb) continue;
@@ -610,13 +584,13 @@ __device__ __forceinline__ int4 pull_tiles(const BinView& bv, const Geo& geo, in
       for (int dx = -2; dx <= 1; dx++) {
         const int bx = g0 + dx;
         if (bx < 0 || bx >= pb) continue;
-        const uint32_t k = bv.map[morton3((uint32_t)bx, (uint32_t)by, (uint32_t)bz)];
-        if (k == 0xffffffffu || k >= bv.tile_slots) continue;
-        const int4 h = bv.tile_hdr[k];
+        const uint32_t k = pool.map[geo.map_base + morton3((uint32_t)bx, (uint32_t)by, (uint32_t)bz)];
+        if (k == 0xffffffffu || k >= pool.tile_slots) continue;
+        const int4 h = pool.tile_hdr[k];
         const int ex = h.w & 255, ey = (h.w >> 8) & 255, ez = (h.w >> 16) & 255;
         const int lx = n0 - h.x, ly = n1 - h.y, lz = n2 - h.z;
         if ((unsigned)lx < (unsigned)ex && (unsigned)ly < (unsigned)ey && (unsigned)lz < (unsigned)ez) {
-          const int4 q = bv.tiles[(size_t)k * kTileNodes + (lz * ey + ly) * ex + lx];
+          const int4 q = pool.tiles[(size_t)k * kTileNodes + (lz * ey + ly) * ex + lx];
           acc.x = (int)((uint32_t)acc.x + (uint32_t)q.x);
           acc.y = (int)((uint32_t)acc.y + (uint32_t)q.y);
           acc.z = (int)((uint32_t)acc.z + (uint32_t)q.z);
@@ -628,42 +602,45 @@ __device__ __forceinline__ int4 pull_tiles(const BinView& bv, const Geo& geo, in
   return acc;
 }
 
-// Grid update (VELOCITY = true, mpm.js:428-470) or tile resolve into A (VELOCITY = false) over S then D.
+// Grid update (VELOCITY = true, mpm.js:428-470) or tile resolve into A (VELOCITY = false) over S
+// (when use_s) then D, one wavefront per node block.
 template <bool VELOCITY>
-__global__ __launch_bounds__(kThreads) void k_grid_pull(BinView bv, FallbackView fb, int4* __restrict__ A,
-                                                        float4* __restrict__ V, Geo geo, GridArgs ga,
-                                                        const float* __restrict__ cols) {
-  const uint32_t ns = bv.scount ? *bv.scount : 0u, nd = *fb.count;
+__global__ __launch_bounds__(kThreads) void k_grid_pull(Pool pool, int use_s) {
+  const uint32_t ns = use_s ? pool.ctr[kCtrS] : 0u, nd = pool.ctr[kCtrD];
+  const uint32_t epoch = pool.ctr[kCtrEpoch];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (uint32_t j = blockIdx.x * 4 + w; j < ns + nd; j += gridDim.x * 4) {
-    const uint32_t b = __builtin_amdgcn_readfirstlane(j < ns ? bv.slist[j] : fb.list[j - ns]);
-    const int g0 = (int)(b % (uint32_t)geo.nbx), g1 = (int)((b / (uint32_t)geo.nbx) % (uint32_t)geo.nby),
-              g2 = (int)(b / ((uint32_t)geo.nbx * (uint32_t)geo.nby));
+    const uint32_t b = __builtin_amdgcn_readfirstlane(j < ns ? pool.slist[j] : pool.dlist[j - ns]);
+    const uint32_t scene = __builtin_amdgcn_readfirstlane(pool.block_scene[b]);
+    const SceneGeo& geo = pool.scenes[scene];
+    const uint32_t lb = b - geo.block_base;
+    const int g0 = (int)(lb % (uint32_t)geo.nbx), g1 = (int)((lb / (uint32_t)geo.nbx) % (uint32_t)geo.nby),
+              g2 = (int)(lb / ((uint32_t)geo.nbx * (uint32_t)geo.nby));
     const int n0 = 4 * g0 + (lane & 3), n1 = 4 * g1 + ((lane >> 2) & 3), n2 = 4 * g2 + (lane >> 4);
     const uint32_t addr = (b << 6) | (uint32_t)lane;
     int4 acc = make_int4(0, 0, 0, 0);
-    const bool written = fb.stamp[b] == fb.epoch;
-    if (written) acc = A[addr];
-    if (j < ns) acc = pull_tiles(bv, geo, g0, g1, g2, n0, n1, n2, acc);
+    const bool written = pool.stamp[b] == epoch;
+    if (written) acc = pool.A[addr];
+    if (j < ns) acc = pull_tiles(pool, geo, g0, g1, g2, n0, n1, n2, acc);
     if (VELOCITY) {
-      V[addr] = grid_node(acc, n0, n1, n2, geo, ga, cols);
-      if (written) A[addr] = make_int4(0, 0, 0, 0);
+      pool.V[addr] = grid_node(acc, n0, n1, n2, geo, pool.cols + 12 * kColRows * scene);
+      if (written) pool.A[addr] = make_int4(0, 0, 0, 0);
     } else {
-      A[addr] = acc;
+      pool.A[addr] = acc;
     }
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_clear_blocks(const uint32_t* __restrict__ list,
-                                                           const uint32_t* __restrict__ count, int4* __restrict__ A) {
-  const uint32_t cnt = *count;
+__global__ __launch_bounds__(kThreads) void k_clear_a(Pool pool, int use_s) {
+  const uint32_t ns = use_s ? pool.ctr[kCtrS] : 0u, nd = pool.ctr[kCtrD];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (uint32_t j = blockIdx.x * 4 + w; j < cnt; j += gridDim.x * 4) A[(list[j] << 6) | (uint32_t)lane] = make_int4(0, 0, 0, 0);
+  for (uint32_t j = blockIdx.x * 4 + w; j < ns + nd; j += gridDim.x * 4) {
+    const uint32_t b = j < ns ? pool.slist[j] : pool.dlist[j - ns];
+    pool.A[(b << 6) | (uint32_t)lane] = make_int4(0, 0, 0, 0);
+  }
 }
 
-__global__ void k_grid_export(const int4* __restrict__ A, const float4* __restrict__ V, const uint32_t* __restrict__ sstamp,
-                              uint32_t bin_epoch, const uint32_t* __restrict__ stamp, uint32_t epoch, int mode, Geo geo,
-                              int4* __restrict__ out) {
+__global__ void k_grid_export(Pool pool, SceneGeo geo, int mode, int4* __restrict__ out) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t nodes = (size_t)geo.X * geo.Y * geo.Z;
   if (i >= nodes) return;
@@ -672,9 +649,9 @@ __global__ void k_grid_export(const int4* __restrict__ A, const float4* __restri
   const uint32_t addr = node_addr(gx, gy, gz, geo);
   const uint32_t b = addr >> 6;
   if (mode == 0) {
-    out[i] = A[addr];
-  } else if ((sstamp && sstamp[b] == bin_epoch) || stamp[b] == epoch) {
-    const float4 v = V[addr];
+    out[i] = pool.A[addr];
+  } else if (pool.sstamp[b] == pool.ctr[kCtrVBinEpoch] || pool.stamp[b] == pool.ctr[kCtrVEpoch]) {
+    const float4 v = pool.V[addr];
     out[i] = make_int4(__float_as_int(v.x), __float_as_int(v.y), __float_as_int(v.z), __float_as_int(v.w));
   } else {
     out[i] = make_int4(0, 0, 0, 0);
@@ -682,27 +659,53 @@ __global__ void k_grid_export(const int4* __restrict__ A, const float4* __restri
 }
 
 inline unsigned blocks_for(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+inline unsigned persistent_grid(uint32_t max_work, unsigned per_block, unsigned cap) {
+  unsigned g = (unsigned)((max_work + per_block - 1) / per_block);
+  if (g > cap) g = cap;
+  return g ? g : 1u;
+}
 
 }  // namespace
 
-hipError_t launch_records_in(const float4* aos, float4* P, uint32_t cap, uint32_t offset, uint32_t n, hipStream_t s) {
+hipError_t launch_records_in(const float4* aos, float4* P, uint32_t cap, uint32_t offset, uint32_t n, uint32_t row_base,
+                             hipStream_t s) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_records_in, dim3(blocks_for((size_t)n * 8, 256)), dim3(256), 0, s, aos, P, cap, offset, n);
+  hipLaunchKernelGGL(k_records_in, dim3(blocks_for((size_t)n * 8, 256)), dim3(256), 0, s, aos, P, cap, offset, n, row_base);
   return hipGetLastError();
 }
-hipError_t launch_records_out(const float4* P, uint32_t cap, uint32_t offset, uint32_t n, float4* aos, hipStream_t s) {
+hipError_t launch_records_out(const float4* P, uint32_t cap, uint32_t offset, uint32_t n, uint32_t row_base, float4* aos,
+                              hipStream_t s) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_records_out, dim3(blocks_for((size_t)n * 8, 256)), dim3(256), 0, s, P, cap, offset, n, aos);
+  hipLaunchKernelGGL(k_records_out, dim3(blocks_for((size_t)n * 8, 256)), dim3(256), 0, s, P, cap, offset, n, row_base,
+                     aos);
   return hipGetLastError();
 }
-hipError_t launch_rescale(float4* P, uint32_t cap, uint32_t n, float k, hipStream_t s) {
+hipError_t launch_rescale(float4* P, uint32_t cap, uint32_t offset, uint32_t n, float k, hipStream_t s) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_rescale, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, cap, n, k);
+  hipLaunchKernelGGL(k_rescale, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, cap, offset, n, k);
   return hipGetLastError();
 }
-hipError_t launch_keys(const float4* P, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s) {
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t s) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, n, keys, vals);
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, value, n);
+  return hipGetLastError();
+}
+hipError_t launch_begin_p2g(const Pool& pool, hipStream_t s) {
+  hipLaunchKernelGGL(k_begin_p2g, dim3(1), dim3(1), 0, s, pool.ctr);
+  return hipGetLastError();
+}
+hipError_t launch_begin_bin(const Pool& pool, hipStream_t s) {
+  hipLaunchKernelGGL(k_begin_bin, dim3(1), dim3(1), 0, s, pool.ctr);
+  return hipGetLastError();
+}
+hipError_t launch_keys(const Pool& pool, const float4* P, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, P, n, keys, vals);
+  return hipGetLastError();
+}
+hipError_t launch_live_keys(const Pool& pool, const float4* P, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_live_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, P, n, keys, vals);
   return hipGetLastError();
 }
 hipError_t launch_permute(const float4* src, float4* dst, uint32_t cap, uint32_t n, const uint32_t* perm, hipStream_t s) {
@@ -710,87 +713,77 @@ hipError_t launch_permute(const float4* src, float4* dst, uint32_t cap, uint32_t
   hipLaunchKernelGGL(k_permute, dim3(blocks_for(n, 256)), dim3(256), 0, s, src, dst, cap, n, perm);
   return hipGetLastError();
 }
-
-namespace {
-unsigned persistent_grid(uint32_t max_work, unsigned per_block, unsigned cap) {
-  unsigned g = (unsigned)((max_work + per_block - 1) / per_block);
-  if (g > cap) g = cap;
-  return g ? g : 1u;
-}
-}  // namespace
-
 hipError_t launch_bin_flags(const uint32_t* keys, uint32_t n, uint32_t* flags, hipStream_t s) {
   hipLaunchKernelGGL(k_bin_flags, dim3(blocks_for((size_t)n + 1, 256)), dim3(256), 0, s, keys, n, flags);
   return hipGetLastError();
 }
-hipError_t launch_bin_emit(const uint32_t* keys, uint32_t n, const uint32_t* scanned, uint32_t* wl_start,
-                           uint32_t* wl_block, uint32_t* map, uint32_t* item_count, hipStream_t s) {
+hipError_t launch_bin_emit(const Pool& pool, const uint32_t* keys, uint32_t n, const uint32_t* scanned, hipStream_t s) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_bin_emit, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, scanned, wl_start, wl_block, map,
-                     item_count);
+  hipLaunchKernelGGL(k_bin_emit, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, keys, n, scanned);
   return hipGetLastError();
 }
-hipError_t launch_bin_transpose(const uint32_t* keys_sorted, const uint32_t* in, const uint32_t* wl_start,
-                                const uint32_t* item_count, uint32_t* out, uint32_t max_items, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_transpose, dim3(persistent_grid(max_items, 1, 4096)), dim3(kThreads), 0, s, keys_sorted, in,
-                     wl_start, item_count, out);
+hipError_t launch_bin_transpose(const Pool& pool, const uint32_t* keys, const uint32_t* in, uint32_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_transpose, dim3(persistent_grid(pool.max_items, 1, 4096)), dim3(kThreads), 0, s, pool, keys,
+                     in, out);
   return hipGetLastError();
 }
-hipError_t launch_bin_dilate(const uint32_t* wl_block, const uint32_t* item_count, const Geo& geo, uint32_t* sstamp,
-                             uint32_t* slist, uint32_t* scount, uint32_t bin_epoch, uint32_t max_items, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_dilate, dim3(persistent_grid(max_items, 4, 4096)), dim3(kThreads), 0, s, wl_block, item_count,
-                     geo, sstamp, slist, scount, bin_epoch);
+hipError_t launch_bin_dilate(const Pool& pool, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_dilate, dim3(persistent_grid(pool.max_items, 4, 4096)), dim3(kThreads), 0, s, pool);
   return hipGetLastError();
 }
-hipError_t launch_p2g_binned(const float4* P, uint32_t cap, const float* mats, float vol4, const BinView& bv, int4* A,
-                             const FallbackView& fb, const Geo& geo, uint32_t max_items, hipStream_t s) {
-  hipLaunchKernelGGL(k_p2g_binned, dim3(persistent_grid(max_items, 1, 2048)), dim3(kThreads), 0, s, P, cap, mats, vol4, bv,
-                     A, fb, geo);
-  return hipGetLastError();
-}
-hipError_t launch_p2g_global(const float4* P, uint32_t cap, uint32_t n, const float* mats, float vol4, int4* A,
-                             const FallbackView& fb, const uint32_t* sstamp, uint32_t bin_epoch, const Geo& geo,
+hipError_t launch_p2g_binned(const Pool& pool, const float4* P, const uint32_t* perm, float4* dst, int waves,
                              hipStream_t s) {
+  const dim3 grid(persistent_grid(pool.max_items, 1, 2048));
+  switch (waves) {
+    case 5: hipLaunchKernelGGL(k_p2g_binned<5>, grid, dim3(kThreads), 0, s, pool, P, perm, dst); break;
+    case 6: hipLaunchKernelGGL(k_p2g_binned<6>, grid, dim3(kThreads), 0, s, pool, P, perm, dst); break;
+    default: hipLaunchKernelGGL(k_p2g_binned<4>, grid, dim3(kThreads), 0, s, pool, P, perm, dst); break;
+  }
+  return hipGetLastError();
+}
+hipError_t launch_p2g_global(const Pool& pool, const float4* P, uint32_t n, hipStream_t s) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_p2g_global, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, P, cap, n, mats, vol4, A, fb,
-                     sstamp, bin_epoch, geo);
+  hipLaunchKernelGGL(k_p2g_global, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, pool, P, n);
   return hipGetLastError();
 }
-hipError_t launch_grid_update(const BinView& bv, const FallbackView& fb, int4* A, float4* V, const Geo& geo,
-                              const GridArgs& ga, const float* cols, uint32_t max_blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_grid_pull<true>, dim3(persistent_grid(max_blocks, 4, 8192)), dim3(kThreads), 0, s, bv, fb, A, V,
-                     geo, ga, cols);
+hipError_t launch_write_words(uint32_t* dst, const uint32_t* src, uint32_t n, hipStream_t s) {
+  if (n > 64) return hipErrorInvalidValue;
+  Words64 w;
+  for (uint32_t i = 0; i < 64; i++) w.w[i] = i < n ? src[i] : 0u;
+  hipLaunchKernelGGL(k_write_words, dim3(1), dim3(64), 0, s, w, dst, n);
   return hipGetLastError();
 }
-hipError_t launch_resolve_tiles(const BinView& bv, const FallbackView& fb, int4* A, const Geo& geo, uint32_t max_blocks,
-                                hipStream_t s) {
-  GridArgs ga{};
-  hipLaunchKernelGGL(k_grid_pull<false>, dim3(persistent_grid(max_blocks, 4, 8192)), dim3(kThreads), 0, s, bv, fb, A,
-                     (float4*)nullptr, geo, ga, (const float*)nullptr);
+hipError_t launch_grid_update(const Pool& pool, bool use_s, hipStream_t s) {
+  hipLaunchKernelGGL(k_grid_pull<true>, dim3(persistent_grid(2 * pool.nblocks, 4, 8192)), dim3(kThreads), 0, s, pool,
+                     use_s ? 1 : 0);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_mark_v, dim3(1), dim3(1), 0, s, pool.ctr, use_s ? 1 : 0);
   return hipGetLastError();
 }
-hipError_t launch_clear_blocks(const uint32_t* list, const uint32_t* count, int4* A, uint32_t max_blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_clear_blocks, dim3(persistent_grid(max_blocks, 4, 8192)), dim3(kThreads), 0, s, list, count, A);
+hipError_t launch_resolve_tiles(const Pool& pool, hipStream_t s) {
+  hipLaunchKernelGGL(k_grid_pull<false>, dim3(persistent_grid(2 * pool.nblocks, 4, 8192)), dim3(kThreads), 0, s, pool, 1);
   return hipGetLastError();
 }
-hipError_t launch_g2p_p2g(float4* P, uint32_t cap, const float4* V, const Geo& geo, const float* mats, float bound,
-                          float vmax, float vol4, const BinView& bv, int4* A, const FallbackView& fb, uint32_t max_items,
-                          hipStream_t s) {
-  hipLaunchKernelGGL(k_g2p_p2g, dim3(persistent_grid(max_items, 1, 2048)), dim3(kThreads), 0, s, P, cap, V, geo, mats,
-                     bound, vmax, vol4, bv, A, fb);
+hipError_t launch_clear_a(const Pool& pool, bool use_s, hipStream_t s) {
+  hipLaunchKernelGGL(k_clear_a, dim3(persistent_grid(2 * pool.nblocks, 4, 8192)), dim3(kThreads), 0, s, pool,
+                     use_s ? 1 : 0);
   return hipGetLastError();
 }
-hipError_t launch_g2p(float4* P, uint32_t cap, uint32_t n, const float4* V, const Geo& geo, const float* mats, float bound,
-                      float vmax, hipStream_t s) {
+hipError_t launch_g2p(const Pool& pool, float4* P, uint32_t n, float vmax, int waves, hipStream_t s) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_g2p, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, P, cap, n, V, geo, mats, bound, vmax);
+  const dim3 grid(blocks_for(n, kThreads));
+  switch (waves) {
+    case 4: hipLaunchKernelGGL(k_g2p<4>, grid, dim3(kThreads), 0, s, pool, P, n, vmax); break;
+    case 5: hipLaunchKernelGGL(k_g2p<5>, grid, dim3(kThreads), 0, s, pool, P, n, vmax); break;
+    case 6: hipLaunchKernelGGL(k_g2p<6>, grid, dim3(kThreads), 0, s, pool, P, n, vmax); break;
+    default: hipLaunchKernelGGL(k_g2p<3>, grid, dim3(kThreads), 0, s, pool, P, n, vmax); break;
+  }
   return hipGetLastError();
 }
-hipError_t launch_grid_export(const int4* A, const float4* V, const uint32_t* sstamp, uint32_t bin_epoch,
-                              const uint32_t* stamp, uint32_t epoch, int mode, const Geo& geo, int4* out, hipStream_t s) {
+hipError_t launch_grid_export(const Pool& pool, const SceneGeo& geo, int mode, int4* out, hipStream_t s) {
   const size_t nodes = (size_t)geo.X * geo.Y * geo.Z;
-  hipLaunchKernelGGL(k_grid_export, dim3(blocks_for(nodes, 256)), dim3(256), 0, s, A, V, sstamp, bin_epoch, stamp, epoch,
-                     mode, geo, out);
+  hipLaunchKernelGGL(k_grid_export, dim3(blocks_for(nodes, 256)), dim3(256), 0, s, pool, geo, mode, out);
   return hipGetLastError();
 }
 

@@ -1,111 +1,143 @@
 // MLS-MPM substep kernels for gfx950 (P2G, grid update, G2P, binning, layout conversion).
 //
+// One set of kernels advances any number of independent scenes at once ("a pool"): a plain
+// simulation is a pool with one scene. Every phase is one launch over all scenes.
+//
 // Device data layout
-//  - Particles: structure of float4 streams. Stream k (k = 0..7) of a buffer holds vec4 k of every
-//    particle record (x|mat, v|J, c0|t00, c1|t11, c2|t22, f0|t01, f1|t02, f2|t12), so a wavefront
-//    reading one stream touches 64 consecutive 16-byte elements (1 KiB, fully coalesced).
-//  - Grid: block-major. The domain is cut into 4x4x4-node blocks; block b = bx + NBX*(by + NBY*bz)
-//    holds its 64 nodes contiguously (local = lx + 4*(ly + 4*lz)): 1 KiB of int4 (A, the i32
-//    fixed-point grid that only out-of-window particles and the global-atomic P2G write) or of
-//    float4 (V, velocity and mass after the grid update).
-//  - Binning (rebuilt whenever the particles are sorted): particles sorted by the Morton code of
-//    their stencil base cell form one contiguous run per 4^3-cell "particle block". Each occupied
-//    particle block is a work item (start offset, Morton block id) and map[Morton block] = item.
-//  - Tiles: P2G gives every work item one workgroup, which sums the 27 fixed-point contributions of
-//    its particles in an LDS tile with LDS integer atomics and stores the tile's bounding box
-//    (origin, extent) and nodes to tile slot `item` with plain stores. The grid update of node block
-//    G pulls from the (at most 64) tiles of particle blocks G-2..G+1 per axis, so no global atomic
-//    is needed on this path. A particle whose base drifted more than kDrift cells outside its
-//    block since the binning (or any particle of an item beyond the tile slots) falls back to
-//    global atomics on A and stamps the node blocks it touched.
-//  - Active node blocks: S = node blocks within reach of any work item (from the binning);
-//    D = node blocks stamped by fallback atomics in the current P2G (stamp[b] = epoch). The grid
-//    update visits S and D and returns A to zero where it was written.
+//  - Particles: one pool of structure-of-float4 streams. Stream k (0..7) holds vec4 k of every
+//    record (x|row, v|J, c0|t00, c1|t11, c2|t22, f0|t01, f1|t02, f2|t12), so a wavefront reading one
+//    stream touches 64 consecutive 16-byte elements. Positions are in the scene's own lattice
+//    coordinates; x.w holds the particle's global material row (scene * 8 + material), and row
+//    word 9 holds the scene id, so a particle finds its scene through its material.
+//  - Grid: a pool of 4x4x4-node blocks. Scene s owns blocks [block_base, block_base + NBX*NBY*NBZ),
+//    block-major inside (block = bx + NBX*(by + NBY*bz), node = lx + 4*(ly + 4*lz)): 1 KiB of int4
+//    (A, the i32 fixed-point grid written only by the global-atomic paths) or of float4 (V,
+//    velocity and mass after the grid update).
+//  - Binning (rebuilt at every sort): particles sorted by (scene, Morton code of their stencil base
+//    cell) form one contiguous run per 4^3-cell particle block. Each occupied block is a work item
+//    (start offset, key >> 6); map[map_base(scene) + Morton block] = item. Inside a block the order
+//    is (rank within cell, cell), so consecutive particles sit in different cells.
+//  - Tiles: P2G gives every work item one workgroup that sums the 27 fixed-point contributions of
+//    its particles in an LDS tile (LDS integer atomics) and stores the tile box and nodes to tile slot
+//    `item` with plain stores. The grid update of a node block pulls from the tiles of particle
+//    blocks G-2..G+1 per axis, so the tiled path needs no global atomic. A particle whose base
+//    drifted more than kDrift cells outside its block since the binning (or of an item beyond the
+//    tile slots) falls back to global atomics on A and stamps the node blocks it wrote.
+//  - Active node blocks: S = node blocks within reach of any work item (rebuilt at each binning);
+//    D = node blocks written through A in the current P2G. The grid update visits S and D and
+//    returns A to zero where it was written.
+//  - Counters (device words, so captured graphs can be replayed): P2G epoch, binning epoch, item
+//    count, |D|, |S|, work counter, and the epochs the velocities in V belong to.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace mpm {
 
-struct Geo {
+constexpr int kDrift = 3;                                      // cells a base may drift outside its block
+constexpr int kTileSpan = 4 + 2 + 2 * kDrift;                  // nodes per axis a work item can touch (12)
+constexpr int kTileNodes = kTileSpan * kTileSpan * kTileSpan;  // 1728 nodes = 27 KiB of int4
+constexpr int kMatRows = 8;                                    // material rows per scene
+constexpr int kColRows = 16;                                   // collider rows per scene
+
+enum Counter : int {
+  kCtrEpoch = 0,     // P2G epoch (stamps of D)
+  kCtrBinEpoch = 1,  // binning epoch (stamps of S)
+  kCtrItems = 2,     // work items of the current binning
+  kCtrD = 3,         // |D|
+  kCtrS = 4,         // |S|
+  kCtrWork = 5,      // next work item of an item-persistent kernel
+  kCtrVEpoch = 6,    // P2G epoch whose grid update wrote V
+  kCtrVBinEpoch = 7, // binning epoch of that grid update (0xffffffff: S not used)
+  kCounters = 8
+};
+
+// Per-scene geometry and step parameters (device table, one entry per scene).
+struct SceneGeo {
   int32_t X, Y, Z;        // cells (= nodes) per axis
   int32_t nbx, nby, nbz;  // 4^3 node blocks per axis
-  int32_t mbits;          // bits per axis of a Morton particle-block coordinate
+  int32_t mbits;          // bits per axis of a Morton particle-block coordinate (map has 2^(3*mbits))
+  uint32_t block_base;    // first node block of the scene in the pool
+  uint32_t map_base;      // first map entry of the scene
+  float bound;            // boundary cells (mpm.js:103)
+  float g[3];             // lattice gravity (mpm.js:588)
+  float floor_friction;   // mpm.js:594
+  uint32_t ncol;          // colliders (rows scene * kColRows + k)
+  float vol4;             // 4 * particle volume = 4 / ppc_axis^3 (mpm.js:610)
+  uint32_t active;        // 0: the scene's particles are dropped at the next sort
 };
 
-struct GridArgs {
-  float g[3];
-  float bound;
-  float floor_friction;
-  uint32_t ncol;
-};
-
-constexpr int kDrift = 3;                       // cells a base may drift outside its block
-constexpr int kTileSpan = 4 + 2 + 2 * kDrift;   // nodes per axis a work item can touch (12)
-constexpr int kTileNodes = kTileSpan * kTileSpan * kTileSpan;  // 1728 nodes = 27 KiB of int4
-
-// Everything P2G, the grid update and the export need to know about the current binning.
-struct BinView {
-  const uint32_t* wl_start;   // [items + 1] particle offsets (in sorted order)
-  const uint32_t* wl_block;   // [items] Morton block id
-  const uint32_t* item_count; // device scalar
-  const uint32_t* map;        // [Morton blocks] -> item or 0xffffffff
-  const uint32_t* perm;       // sorted position -> particle index (nullptr: identity)
-  int4* tiles;                // [tile_slots * kTileNodes]
-  int4* tile_hdr;             // [tile_slots] (ox, oy, oz, ex | ey << 8 | ez << 16)
+// Pointers and sizes fixed for the pool's lifetime, passed by value to every kernel.
+struct Pool {
+  uint32_t cap;            // particle capacity of each P buffer
+  uint32_t key_bits;       // bits of the per-scene cell key (3 * per-axis bits)
+  uint32_t nblocks;        // node blocks in the pool
+  uint32_t max_items;      // work items a binning can produce
   uint32_t tile_slots;
-  const uint32_t* slist;      // S list and its count
-  const uint32_t* scount;
-  const uint32_t* sstamp;     // per node block: binning epoch that put it in S
-  uint32_t bin_epoch;
+  int4* A;
+  float4* V;
+  const uint32_t* block_scene;  // node block -> scene
+  uint32_t* stamp;         // D stamps (P2G epoch)
+  uint32_t* dlist;
+  uint32_t* sstamp;        // S stamps (binning epoch)
+  uint32_t* slist;
+  uint32_t* map;
+  uint32_t* wl_start;      // [max_items + 1]
+  uint32_t* wl_block;      // [max_items] key >> 6 (scene bits on top)
+  int4* tiles;             // [tile_slots * kTileNodes]
+  int4* tile_hdr;          // [tile_slots] (ox, oy, oz, ex | ey << 8 | ez << 16)
+  const float* mats;       // [scenes * kMatRows * 12]
+  const float* cols;       // [scenes * kColRows * 12]
+  const SceneGeo* scenes;
+  uint32_t* ctr;           // [kCounters]
 };
 
-struct FallbackView {         // D: node blocks written through A in this P2G
-  uint32_t* stamp;
-  uint32_t* list;
-  uint32_t* count;
-  uint32_t epoch;
-};
-
-// Launch helpers (all enqueue on `s`; none synchronizes).
-hipError_t launch_records_in(const float4* aos, float4* P, uint32_t cap, uint32_t offset, uint32_t n, hipStream_t s);
-hipError_t launch_records_out(const float4* P, uint32_t cap, uint32_t offset, uint32_t n, float4* aos, hipStream_t s);
-hipError_t launch_rescale(float4* P, uint32_t cap, uint32_t n, float k, hipStream_t s);
-hipError_t launch_keys(const float4* P, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s);
-hipError_t launch_permute(const float4* src, float4* dst, uint32_t cap, uint32_t n, const uint32_t* perm, hipStream_t s);
-
-// Binning from sorted keys: flags/scan scratch of n + 1 words (scan_scratch separate).
-hipError_t launch_bin_flags(const uint32_t* keys_sorted, uint32_t n, uint32_t* flags, hipStream_t s);
-hipError_t launch_bin_emit(const uint32_t* keys_sorted, uint32_t n, const uint32_t* scanned, uint32_t* wl_start,
-                           uint32_t* wl_block, uint32_t* map, uint32_t* item_count, hipStream_t s);
-// Reorders each block's sorted indices by (rank within cell, cell): out[start + pos] = in[start + j].
-hipError_t launch_bin_transpose(const uint32_t* keys_sorted, const uint32_t* in, const uint32_t* wl_start,
-                                const uint32_t* item_count, uint32_t* out, uint32_t max_items, hipStream_t s);
-hipError_t launch_bin_dilate(const uint32_t* wl_block, const uint32_t* item_count, const Geo& geo, uint32_t* sstamp,
-                             uint32_t* slist, uint32_t* scount, uint32_t bin_epoch, uint32_t max_items, hipStream_t s);
-
-// P2G over the work items into tiles (+ fallback atomics on A).
-hipError_t launch_p2g_binned(const float4* P, uint32_t cap, const float* mats, float vol4, const BinView& bv,
-                             int4* A, const FallbackView& fb, const Geo& geo, uint32_t max_items, hipStream_t s);
-// P2G with one global atomic per contribution (the reference's forceGlobal baseline).
-hipError_t launch_p2g_global(const float4* P, uint32_t cap, uint32_t n, const float* mats, float vol4, int4* A,
-                             const FallbackView& fb, const uint32_t* sstamp, uint32_t bin_epoch, const Geo& geo,
+// Launch helpers; all enqueue on `s` and none synchronizes. P is the current particle buffer.
+hipError_t launch_records_in(const float4* aos, float4* P, uint32_t cap, uint32_t offset, uint32_t n, uint32_t row_base,
                              hipStream_t s);
-// Grid update over S and D: pulls tiles + A, writes V and zeroes A where it was written.
-hipError_t launch_grid_update(const BinView& bv, const FallbackView& fb, int4* A, float4* V, const Geo& geo,
-                              const GridArgs& ga, const float* cols, uint32_t max_blocks, hipStream_t s);
-// Folds the tiles into A over S (A then holds the whole P2G result, for reads and checkMPM).
-hipError_t launch_resolve_tiles(const BinView& bv, const FallbackView& fb, int4* A, const Geo& geo,
-                                uint32_t max_blocks, hipStream_t s);
-hipError_t launch_clear_blocks(const uint32_t* list, const uint32_t* count, int4* A, uint32_t max_blocks, hipStream_t s);
-hipError_t launch_g2p(float4* P, uint32_t cap, uint32_t n, const float4* V, const Geo& geo, const float* mats, float bound,
-                      float vmax, hipStream_t s);
-// G2P of this substep fused with the next substep's P2G (tiles + A for the next grid update).
-hipError_t launch_g2p_p2g(float4* P, uint32_t cap, const float4* V, const Geo& geo, const float* mats, float bound,
-                          float vmax, float vol4, const BinView& bv, int4* A, const FallbackView& fb, uint32_t max_items,
-                          hipStream_t s);
-// Reference-layout export: mode 0 = A (i32 words), mode 1 = V (f32 bits, 0 outside S and D).
-hipError_t launch_grid_export(const int4* A, const float4* V, const uint32_t* sstamp, uint32_t bin_epoch,
-                              const uint32_t* stamp, uint32_t epoch, int mode, const Geo& geo, int4* out, hipStream_t s);
+hipError_t launch_records_out(const float4* P, uint32_t cap, uint32_t offset, uint32_t n, uint32_t row_base, float4* aos,
+                              hipStream_t s);
+hipError_t launch_rescale(float4* P, uint32_t cap, uint32_t offset, uint32_t n, float k, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t s);
+// ctr[kCtrEpoch]++, |D| = 0, work = 0 (start of a P2G)
+hipError_t launch_begin_p2g(const Pool& pool, hipStream_t s);
+// ctr[kCtrBinEpoch]++, |S| = 0, items = 0 (start of a binning)
+hipError_t launch_begin_bin(const Pool& pool, hipStream_t s);
+
+// Sort keys (scene << key_bits | Morton(base cell)) and identity values; particles of inactive
+// scenes get key 0xffffffff (sorted behind every live particle).
+hipError_t launch_keys(const Pool& pool, const float4* P, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s);
+// Keys 0 for particles of active scenes, 1 for removed ones (a stable sort on them is a stable
+// partition); vals = index.
+hipError_t launch_live_keys(const Pool& pool, const float4* P, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s);
+hipError_t launch_permute(const float4* src, float4* dst, uint32_t cap, uint32_t n, const uint32_t* perm, hipStream_t s);
+// Binning from the sorted keys: head flags (n + 1 words), work items + map, within-block
+// transposition of the sorted indices into `out`, and S.
+hipError_t launch_bin_flags(const uint32_t* keys_sorted, uint32_t n, uint32_t* flags, hipStream_t s);
+hipError_t launch_bin_emit(const Pool& pool, const uint32_t* keys_sorted, uint32_t n, const uint32_t* scanned,
+                           hipStream_t s);
+hipError_t launch_bin_transpose(const Pool& pool, const uint32_t* keys_sorted, const uint32_t* in, uint32_t* out,
+                                hipStream_t s);
+hipError_t launch_bin_dilate(const Pool& pool, hipStream_t s);
+
+// P2G over the work items into tiles (+ fallback atomics on A). perm: sorted position -> particle
+// (nullptr: identity). dst != nullptr also stores every record at its sorted position in dst (the
+// sort's reorder folded into this pass).
+hipError_t launch_p2g_binned(const Pool& pool, const float4* P, const uint32_t* perm, float4* dst, int waves,
+                             hipStream_t s);
+// P2G with one global atomic per contribution (the reference's forceGlobal path), any order.
+hipError_t launch_p2g_global(const Pool& pool, const float4* P, uint32_t n, hipStream_t s);
+// Writes n (<= 64) 32-bit words to dst from a kernel argument (ordered with the stream's work).
+hipError_t launch_write_words(uint32_t* dst, const uint32_t* src, uint32_t n, hipStream_t s);
+// Grid update over S and D (tiles + A -> V, A back to zero). use_s = false: only D holds data.
+hipError_t launch_grid_update(const Pool& pool, bool use_s, hipStream_t s);
+// Folds the tiles into A over S (A then holds the whole P2G result, for reads).
+hipError_t launch_resolve_tiles(const Pool& pool, hipStream_t s);
+// Zeroes A over S (use_s) and D.
+hipError_t launch_clear_a(const Pool& pool, bool use_s, hipStream_t s);
+hipError_t launch_g2p(const Pool& pool, float4* P, uint32_t n, float vmax, int waves, hipStream_t s);
+// Reference-layout export of one scene: mode 0 = A (i32 words), mode 1 = V (f32 bits, 0 outside the
+// node blocks the last grid update wrote).
+hipError_t launch_grid_export(const Pool& pool, const SceneGeo& geo, int mode, int4* out, hipStream_t s);
 
 }  // namespace mpm

@@ -1,8 +1,10 @@
-// Runtime behind include/mpm.h: device context, simulation state, substep orchestration.
+// Runtime behind include/mpm.h: device context, particle/grid pools, substep orchestration.
 //
 // Replaces the WebGPU Context/Kernel machinery the reference path runs on (src/core/context.js,
-// src/core/kernel.js) with one HIP stream per simulation: every call enqueues, reads block. The
-// substep loop is MPM.step (src/physics/mpm.js:582-604); see mpm_kernels.hpp for the device layout.
+// src/core/kernel.js) with one HIP stream per pool: every call enqueues, reads block. A pool holds
+// any number of independent scenes that are advanced by the same launches (mpm_kernels.hpp); the
+// reference's single MPM object (an mpm_sim) is a pool with one scene. The substep loop is
+// MPM.step (src/physics/mpm.js:582-604).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -12,6 +14,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/mpm.h"
@@ -48,20 +51,37 @@ mpm_status from_hip(hipError_t e, const char* where) {
     if (_e != hipSuccess) return from_hip(_e, #expr);  \
   } while (0)
 
+#define ST_TRY(expr)              \
+  do {                            \
+    mpm_status _s = (expr);       \
+    if (_s != MPM_OK) return _s;  \
+  } while (0)
+
 // What the grid buffers hold after the last call: nothing, the i32 P2G result in A, the P2G result
-// split over tiles and A (inside mpm_step only), or velocities in V.
+// split over tiles and A (inside a step only), or velocities in V.
 enum GridState { kGridClean = 0, kGridA = 1, kGridTiles = 2, kGridVelocity = 3 };
+enum BinMode { kBinOnly = 0, kBinReorder = 1, kBinReorderInP2G = 2 };
 
 constexpr size_t kTileBudgetBytes = size_t(8) << 30;  // tile slots beyond this use the fallback path
 
-// Small tables are written by a one-thread-per-float kernel whose argument holds the data: the copy
-// is ordered with the work already on the stream and needs no host buffer to outlive the call (the
-// reference's writeOrdered, src/core/context.js:228-241).
-struct Table192 { float v[192]; };
-__global__ void k_write_table(Table192 t, float* dst, int n) {
-  int i = threadIdx.x;
-  if (i < n) dst[i] = t.v[i];
+int ceil_log2(uint64_t v) {
+  int b = 0;
+  while ((1ull << b) < v) b++;
+  return b;
 }
+
+template <class T>
+hipError_t dmalloc(T** p, size_t count) {
+  return hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T));
+}
+
+struct SceneHost {
+  bool used = false;
+  SceneGeo geo{};
+  uint32_t count = 0;   // particles of the scene
+  uint32_t offset = 0;  // first particle in the pool (scenes are contiguous)
+  uint32_t nblocks = 0;
+};
 
 }  // namespace
 
@@ -72,63 +92,55 @@ struct mpm_ctx {
   hipStream_t stream = nullptr;
 };
 
-struct mpm_sim {
+struct mpm_pool {
   mpm_ctx* ctx = nullptr;
-  mpm_config cfg{};
-  Geo geo{};
-  uint32_t cap = 0, nblocks = 0;
-  int key_bits = 0;
-  uint32_t nbm = 0;         // Morton particle blocks (map entries)
-  uint32_t max_items = 0;   // work items a binning can produce
-  uint32_t tile_slots = 0;
   hipStream_t stream = nullptr;
+  // configuration
+  uint32_t cap = 0, max_scenes = 0, max_blocks = 0, sort_every = 4;
+  int max_dim = 0;
+  uint32_t key_bits = 0, sort_bits = 0, map_per_scene = 0, max_items = 0, tile_slots = 0;
+  // device memory
   float4* P[2] = {nullptr, nullptr};
   int cur = 0;
   int4* A = nullptr;
   float4* V = nullptr;
-  // D (fallback writes of the current P2G)
-  uint32_t* stamp = nullptr;
-  uint32_t* dlist = nullptr;
-  uint32_t* dcount = nullptr;
-  uint32_t epoch = 0;
-  // binning
-  uint32_t* wl_start = nullptr;
-  uint32_t* wl_block = nullptr;
-  uint32_t* item_count = nullptr;
-  uint32_t* map = nullptr;
-  uint32_t* flags = nullptr;
-  uint32_t* slist = nullptr;
-  uint32_t* scount = nullptr;
-  uint32_t* sstamp = nullptr;
-  int4* tiles = nullptr;
-  int4* tile_hdr = nullptr;
-  const uint32_t* perm = nullptr;
-  uint32_t bin_epoch = 0;
-  bool bin_valid = false;
-  // materials / colliders
-  float* mats_d = nullptr;
-  float* cols_d = nullptr;
-  uint32_t ncol = 0;
-  // sort
-  uint32_t* keys[2] = {nullptr, nullptr};
-  uint32_t* vals[2] = {nullptr, nullptr};
+  uint32_t *block_scene = nullptr, *stamp = nullptr, *dlist = nullptr, *sstamp = nullptr, *slist = nullptr;
+  uint32_t *map = nullptr, *wl_start = nullptr, *wl_block = nullptr;
+  int4 *tiles = nullptr, *tile_hdr = nullptr;
+  float *mats = nullptr, *cols = nullptr;
+  SceneGeo* scenes_d = nullptr;
+  uint32_t* ctr = nullptr;
+  uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr}, *order = nullptr, *flags = nullptr;
   uint32_t* sort_scratch = nullptr;
-  // transfers
   float4* staging = nullptr;
   uint32_t staging_records = 0;
   int4* export_buf = nullptr;
+  size_t export_nodes = 0;
+  // host state
+  std::vector<SceneHost> scenes;
+  std::vector<uint32_t> mem_order;  // scenes in the order their particle runs sit in the pool
+  std::vector<std::pair<uint32_t, uint32_t>> free_blocks;  // (start, length), sorted by start
   uint32_t count = 0;
   uint64_t step_count = 0;
+  bool bin_valid = false;
+  const uint32_t* perm = nullptr;  // sorted position -> particle (binning without reorder)
+  bool reorder_pending = false;    // perm is the sort's new order, applied by the next P2G
+  bool needs_repack = false;       // a removed scene's particles are still in the pool
   int grid_state = kGridClean;
   bool a_dirty_s = false, a_dirty_d = false;  // A holds data over S / over D
-  uint32_t v_bin_epoch = 0, v_epoch = 0;      // binning and P2G the velocities in V belong to
+  int g2p_waves = 3, p2g_waves = 4;
+  bool use_graphs = false;
+  hipGraphExec_t period_graph[2] = {nullptr, nullptr};  // one sort period, by parity of cur
+  uint64_t graph_key = 0;                                // (count, scenes, vmax) the graphs were captured for
+  // profiling: five events per profiled substep (sort, p2g, grid, g2p start; end)
   bool profiling = false;
-  // Profiling: five events per profiled substep (sort, p2g, grid, g2p start; end), recorded on the
-  // stream without host synchronization and resolved by mpm_phase_times.
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   float phase_ms[4] = {0, 0, 0, 0};
-  uint32_t phase_substeps = 0;
+};
+
+struct mpm_sim {
+  mpm_pool* pool = nullptr;  // a pool with exactly one scene (id 0)
 };
 
 namespace {
@@ -138,168 +150,565 @@ mpm_status set_dev(mpm_ctx* c) {
   return MPM_OK;
 }
 
-template <class T>
-hipError_t dmalloc(T** p, size_t count) {
-  return hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T));
+Pool view(const mpm_pool* p) {
+  Pool v;
+  v.cap = p->cap;
+  v.key_bits = p->key_bits;
+  v.nblocks = p->max_blocks;
+  v.max_items = p->max_items;
+  v.tile_slots = p->tile_slots;
+  v.A = p->A;
+  v.V = p->V;
+  v.block_scene = p->block_scene;
+  v.stamp = p->stamp;
+  v.dlist = p->dlist;
+  v.sstamp = p->sstamp;
+  v.slist = p->slist;
+  v.map = p->map;
+  v.wl_start = p->wl_start;
+  v.wl_block = p->wl_block;
+  v.tiles = p->tiles;
+  v.tile_hdr = p->tile_hdr;
+  v.mats = p->mats;
+  v.cols = p->cols;
+  v.scenes = p->scenes_d;
+  v.ctr = p->ctr;
+  return v;
 }
 
-void free_sim(mpm_sim* s) {
-  if (!s) return;
-  if (s->ctx) hipSetDevice(s->ctx->device);
-  if (s->stream) hipStreamSynchronize(s->stream);
-  for (hipEvent_t e : s->ev_pool) hipEventDestroy(e);
-  void* ptrs[] = {s->P[0],     s->P[1],     s->A,        s->V,          s->stamp,    s->dlist,    s->dcount,
-                  s->wl_start, s->wl_block, s->item_count, s->map,    s->flags,    s->slist,    s->scount,
-                  s->sstamp,   s->tiles,    s->tile_hdr, s->mats_d,     s->cols_d,   s->keys[0],  s->keys[1],
-                  s->vals[0],  s->vals[1],  s->sort_scratch, s->staging, s->export_buf};
-  for (void* p : ptrs)
-    if (p) hipFree(p);
-  if (s->stream) hipStreamDestroy(s->stream);
-  delete s;
+void drop_graphs(mpm_pool* p) {
+  for (auto& g : p->period_graph)
+    if (g) {
+      hipGraphExecDestroy(g);
+      g = nullptr;
+    }
 }
 
-mpm_status write_table(mpm_sim* s, float* dst, const float* src, int n) {
-  Table192 t;
-  std::memset(&t, 0, sizeof t);
-  std::memcpy(t.v, src, sizeof(float) * (size_t)n);
-  hipLaunchKernelGGL(k_write_table, dim3(1), dim3(192), 0, s->stream, t, dst, 192);
-  HIP_TRY(hipGetLastError());
+void free_pool(mpm_pool* p) {
+  if (!p) return;
+  if (p->ctx) hipSetDevice(p->ctx->device);
+  if (p->stream) hipStreamSynchronize(p->stream);
+  drop_graphs(p);
+  for (hipEvent_t e : p->ev_pool) hipEventDestroy(e);
+  void* ptrs[] = {p->P[0],   p->P[1],     p->A,        p->V,        p->block_scene, p->stamp,   p->dlist,
+                  p->sstamp, p->slist,    p->map,      p->wl_start, p->wl_block,    p->tiles,   p->tile_hdr,
+                  p->mats,   p->cols,     p->scenes_d, p->ctr,      p->keys[0],     p->keys[1], p->vals[0],
+                  p->vals[1], p->order,   p->flags,    p->sort_scratch, p->staging, p->export_buf};
+  for (void* q : ptrs)
+    if (q) hipFree(q);
+  if (p->stream) hipStreamDestroy(p->stream);
+  delete p;
+}
+
+// Device bytes a pool of this shape allocates (used for admission against an HBM budget).
+size_t pool_bytes(uint32_t cap, uint32_t max_scenes, uint32_t max_blocks, int max_dim, uint32_t* map_per_scene,
+                  uint32_t* max_items, uint32_t* tile_slots) {
+  const int bits = std::max(3, ceil_log2((uint64_t)max_dim));
+  const uint32_t mps = 1u << (3 * (bits - 2));
+  const uint64_t items = std::min<uint64_t>((uint64_t)mps * max_scenes, cap);
+  // a work item holds 512 particles when its 4^3 cells are full (8 per cell); one slot per 16
+  // particles covers sparse surface blocks too, and items beyond the slots take the fallback path
+  const uint64_t slots = std::min<uint64_t>(
+      std::min<uint64_t>(items, std::max<uint64_t>(4096, cap / 16)), kTileBudgetBytes / (sizeof(int4) * kTileNodes));
+  if (map_per_scene) *map_per_scene = mps;
+  if (max_items) *max_items = (uint32_t)items;
+  if (tile_slots) *tile_slots = (uint32_t)slots;
+  size_t b = 0;
+  b += 2 * (size_t)cap * 128;                            // P[2]
+  b += (size_t)max_blocks * 64 * (16 + 16);              // A, V
+  b += (size_t)max_blocks * 4 * 5;                       // block_scene, stamp, dlist, sstamp, slist
+  b += (size_t)mps * max_scenes * 4;                     // map
+  b += (items + 1) * 4 * 2;                              // wl_start, wl_block
+  b += slots * (sizeof(int4) * kTileNodes + 16);         // tiles, headers
+  b += (size_t)max_scenes * (kMatRows + kColRows) * 48;  // mats, cols
+  b += (size_t)cap * 4 * 6 + 4;                          // keys, vals, order, flags
+  b += sort_scratch_words(cap) * 4;
+  b += (size_t)std::min<uint32_t>(cap, 1u << 20) * 128;  // staging
+  return b;
+}
+
+mpm_status upload_scene(mpm_pool* p, uint32_t s) {
+  static_assert(sizeof(SceneGeo) % 4 == 0 && sizeof(SceneGeo) <= 256, "SceneGeo words");
+  HIP_TRY(launch_write_words(reinterpret_cast<uint32_t*>(p->scenes_d + s),
+                             reinterpret_cast<const uint32_t*>(&p->scenes[s].geo), sizeof(SceneGeo) / 4, p->stream));
+  drop_graphs(p);
   return MPM_OK;
 }
 
-BinView bin_view(const mpm_sim* s) {
-  BinView bv;
-  bv.wl_start = s->wl_start;
-  bv.wl_block = s->wl_block;
-  bv.item_count = s->item_count;
-  bv.map = s->map;
-  bv.perm = s->perm;
-  bv.tiles = s->tiles;
-  bv.tile_hdr = s->tile_hdr;
-  bv.tile_slots = s->tile_slots;
-  bv.slist = s->slist;
-  bv.scount = s->scount;
-  bv.sstamp = s->sstamp;
-  bv.bin_epoch = s->bin_epoch;
-  return bv;
-}
-
-FallbackView fallback_view(const mpm_sim* s) {
-  FallbackView fb;
-  fb.stamp = s->stamp;
-  fb.list = s->dlist;
-  fb.count = s->dcount;
-  fb.epoch = s->epoch;
-  return fb;
-}
-
-// Return A to all-zero where the last standalone P2G left data (before S or D change).
-mpm_status clean_a(mpm_sim* s) {
-  if (s->a_dirty_d) HIP_TRY(launch_clear_blocks(s->dlist, s->dcount, s->A, s->nblocks, s->stream));
-  if (s->a_dirty_s) HIP_TRY(launch_clear_blocks(s->slist, s->scount, s->A, s->nblocks, s->stream));
-  s->a_dirty_d = s->a_dirty_s = false;
+// Material rows of scene s (n <= 8 rows of 12 floats); word 9 of every row names the scene.
+mpm_status upload_materials(mpm_pool* p, uint32_t s, const float* tbl, uint32_t n) {
+  float rows[kMatRows * 12];
+  std::memset(rows, 0, sizeof rows);
+  if (n) std::memcpy(rows, tbl, 48 * (size_t)n);
+  for (int k = 0; k < kMatRows; k++) std::memcpy(&rows[12 * k + 9], &s, 4);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(p->mats + (size_t)s * kMatRows * 12);
+  HIP_TRY(launch_write_words(dst, reinterpret_cast<const uint32_t*>(rows), 48, p->stream));
+  HIP_TRY(launch_write_words(dst + 48, reinterpret_cast<const uint32_t*>(rows) + 48, 48, p->stream));
   return MPM_OK;
 }
 
-// Sort keys, then either reorder the particles (MPM._sort, mpm.js:614-621) or only record the sorted
-// order as a permutation, and rebuild the work items, the block map and S from the sorted keys.
-mpm_status do_bin(mpm_sim* s, bool reorder) {
-  mpm_status st = clean_a(s);
-  if (st) return st;
-  s->bin_epoch++;
-  HIP_TRY(hipMemsetAsync(s->scount, 0, sizeof(uint32_t), s->stream));
-  HIP_TRY(hipMemsetAsync(s->item_count, 0, sizeof(uint32_t), s->stream));
-  s->perm = nullptr;
-  if (s->count == 0) {
-    s->bin_valid = true;
+mpm_status upload_colliders(mpm_pool* p, uint32_t s, const float* tbl, uint32_t n) {
+  float rows[kColRows * 12];
+  std::memset(rows, 0, sizeof rows);
+  if (n) std::memcpy(rows, tbl, 48 * (size_t)n);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(p->cols + (size_t)s * kColRows * 12);
+  for (int part = 0; part < 3; part++)
+    HIP_TRY(launch_write_words(dst + 64 * part, reinterpret_cast<const uint32_t*>(rows) + 64 * part, 64, p->stream));
+  return MPM_OK;
+}
+
+bool alloc_blocks(mpm_pool* p, uint32_t n, uint32_t* base) {
+  for (size_t i = 0; i < p->free_blocks.size(); i++) {
+    auto& f = p->free_blocks[i];
+    if (f.second >= n) {
+      *base = f.first;
+      f.first += n;
+      f.second -= n;
+      if (f.second == 0) p->free_blocks.erase(p->free_blocks.begin() + (long)i);
+      return true;
+    }
+  }
+  return false;
+}
+
+void release_blocks(mpm_pool* p, uint32_t base, uint32_t n) {
+  auto& fb = p->free_blocks;
+  auto it = std::lower_bound(fb.begin(), fb.end(), std::make_pair(base, 0u));
+  it = fb.insert(it, {base, n});
+  size_t i = (size_t)(it - fb.begin());
+  if (i + 1 < fb.size() && fb[i].first + fb[i].second == fb[i + 1].first) {
+    fb[i].second += fb[i + 1].second;
+    fb.erase(fb.begin() + (long)i + 1);
+  }
+  if (i > 0 && fb[i - 1].first + fb[i - 1].second == fb[i].first) {
+    fb[i - 1].second += fb[i].second;
+    fb.erase(fb.begin() + (long)i);
+  }
+}
+
+uint32_t free_block_total(const mpm_pool* p) {
+  uint32_t t = 0;
+  for (auto& f : p->free_blocks) t += f.second;
+  return t;
+}
+
+// Scene offsets from the order the scenes' particle runs sit in.
+void layout_offsets(mpm_pool* p) {
+  uint32_t off = 0;
+  for (uint32_t s : p->mem_order) {
+    p->scenes[s].offset = off;
+    off += p->scenes[s].count;
+  }
+}
+
+// After a sort's reorder the particles are grouped by scene in id order (the scene is the top of
+// the sort key).
+void recompute_offsets(mpm_pool* p) {
+  p->mem_order.clear();
+  for (uint32_t s = 0; s < p->scenes.size(); s++)
+    if (p->scenes[s].used) p->mem_order.push_back(s);
+  layout_offsets(p);
+}
+
+// Return A to all-zero where the last P2G left data (before S or D change).
+mpm_status clean_a(mpm_pool* p) {
+  if (p->a_dirty_s || p->a_dirty_d) HIP_TRY(launch_clear_a(view(p), p->a_dirty_s, p->stream));
+  p->a_dirty_d = p->a_dirty_s = false;
+  return MPM_OK;
+}
+
+mpm_status apply_pending_reorder(mpm_pool* p) {
+  if (!p->reorder_pending) return MPM_OK;
+  HIP_TRY(launch_permute(p->P[p->cur], p->P[p->cur ^ 1], p->cap, p->count, p->perm, p->stream));
+  p->cur ^= 1;
+  p->perm = nullptr;
+  p->reorder_pending = false;
+  recompute_offsets(p);
+  return MPM_OK;
+}
+
+// Sort keys, then reorder the particles now (MPM._sort, mpm.js:614-621), at the next P2G, or not at
+// all (the sorted order is kept as a permutation), and rebuild work items, map and S.
+mpm_status repack(mpm_pool* p);
+
+mpm_status do_bin(mpm_pool* p, int mode) {
+  if (p->needs_repack) ST_TRY(repack(p));  // binning assumes every particle belongs to a live scene
+  ST_TRY(clean_a(p));
+  const Pool pv = view(p);
+  HIP_TRY(launch_begin_bin(pv, p->stream));
+  p->perm = nullptr;
+  p->reorder_pending = false;
+  if (p->count == 0) {
+    p->bin_valid = true;
     return MPM_OK;
   }
-  HIP_TRY(launch_keys(s->P[s->cur], s->count, s->keys[0], s->vals[0], s->stream));
+  HIP_TRY(launch_keys(pv, p->P[p->cur], p->count, p->keys[0], p->vals[0], p->stream));
   uint32_t *rk = nullptr, *rv = nullptr;
-  HIP_TRY(radix_sort_pairs(s->keys[0], s->vals[0], s->keys[1], s->vals[1], s->count, s->key_bits, s->sort_scratch,
-                           s->stream, &rk, &rv));
-  HIP_TRY(hipMemsetAsync(s->map, 0xff, (size_t)s->nbm * sizeof(uint32_t), s->stream));
-  HIP_TRY(launch_bin_flags(rk, s->count, s->flags, s->stream));
-  HIP_TRY(scan_exclusive(s->flags, (size_t)s->count + 1, s->sort_scratch, s->stream));
-  HIP_TRY(launch_bin_emit(rk, s->count, s->flags, s->wl_start, s->wl_block, s->map, s->item_count, s->stream));
-  // final order inside each block: (rank in cell, cell); written to whichever vals buffer rv is not
-  uint32_t* order = rv == s->vals[0] ? s->vals[1] : s->vals[0];
-  HIP_TRY(launch_bin_transpose(rk, rv, s->wl_start, s->item_count, order, s->max_items, s->stream));
-  if (reorder) {
-    HIP_TRY(launch_permute(s->P[s->cur], s->P[s->cur ^ 1], s->cap, s->count, order, s->stream));
-    s->cur ^= 1;
+  HIP_TRY(radix_sort_pairs(p->keys[0], p->vals[0], p->keys[1], p->vals[1], p->count, (int)p->sort_bits,
+                           p->sort_scratch, p->stream, &rk, &rv));
+  HIP_TRY(launch_fill_u32(p->map, 0xffffffffu, p->map_per_scene * p->max_scenes, p->stream));
+  HIP_TRY(launch_bin_flags(rk, p->count, p->flags, p->stream));
+  HIP_TRY(scan_exclusive(p->flags, (size_t)p->count + 1, p->sort_scratch, p->stream));
+  HIP_TRY(launch_bin_emit(pv, rk, p->count, p->flags, p->stream));
+  HIP_TRY(launch_bin_transpose(pv, rk, rv, p->order, p->stream));
+  if (mode == kBinReorder) {
+    HIP_TRY(launch_permute(p->P[p->cur], p->P[p->cur ^ 1], p->cap, p->count, p->order, p->stream));
+    p->cur ^= 1;
+    recompute_offsets(p);
   } else {
-    s->perm = order;
+    p->perm = p->order;
+    p->reorder_pending = mode == kBinReorderInP2G;
   }
-  HIP_TRY(launch_bin_dilate(s->wl_block, s->item_count, s->geo, s->sstamp, s->slist, s->scount, s->bin_epoch,
-                            s->max_items, s->stream));
-  s->bin_valid = true;
+  HIP_TRY(launch_bin_dilate(pv, p->stream));
+  p->bin_valid = true;
   return MPM_OK;
 }
 
-float vol4_of(const mpm_sim* s) { return (float)(4.0 * (1.0 / std::pow((double)s->cfg.ppc_axis, 3))); }  // mpm.js:610
+// Drops the particles of removed scenes with a stable partition (live before dead), so the live
+// particles keep their order and the other scenes are not perturbed.
+mpm_status repack(mpm_pool* p) {
+  ST_TRY(clean_a(p));
+  ST_TRY(apply_pending_reorder(p));
+  const Pool pv = view(p);
+  if (p->count) {
+    HIP_TRY(launch_live_keys(pv, p->P[p->cur], p->count, p->keys[0], p->vals[0], p->stream));
+    uint32_t *rk = nullptr, *rv = nullptr;
+    HIP_TRY(radix_sort_pairs(p->keys[0], p->vals[0], p->keys[1], p->vals[1], p->count, 1, p->sort_scratch, p->stream,
+                             &rk, &rv));
+    uint32_t live = 0;
+    for (auto& sc : p->scenes)
+      if (sc.used && sc.geo.active) live += sc.count;
+    HIP_TRY(launch_permute(p->P[p->cur], p->P[p->cur ^ 1], p->cap, live, rv, p->stream));
+    p->cur ^= 1;
+    p->count = live;
+  }
+  for (uint32_t s = 0; s < p->scenes.size(); s++) {
+    SceneHost& sc = p->scenes[s];
+    if (sc.used && !sc.geo.active) {
+      release_blocks(p, sc.geo.block_base, sc.nblocks);
+      sc = SceneHost();
+      p->mem_order.erase(std::find(p->mem_order.begin(), p->mem_order.end(), s));
+    }
+  }
+  layout_offsets(p);
+  p->needs_repack = false;
+  p->bin_valid = false;
+  p->grid_state = kGridClean;
+  drop_graphs(p);
+  return MPM_OK;
+}
 
 // P2G (mpm.js:607-612). Binned: into tiles (+ A for stragglers), grid state kGridTiles. Global: every
 // contribution a global atomic on A, grid state kGridA.
-mpm_status do_p2g(mpm_sim* s, bool force_global) {
-  mpm_status st = clean_a(s);
-  if (st) return st;
-  if (!force_global && !s->bin_valid && (st = do_bin(s, false))) return st;
-  s->epoch++;
-  HIP_TRY(hipMemsetAsync(s->dcount, 0, sizeof(uint32_t), s->stream));
-  const FallbackView fb = fallback_view(s);
+mpm_status do_p2g(mpm_pool* p, bool force_global) {
+  ST_TRY(clean_a(p));
+  if (!force_global && !p->bin_valid) ST_TRY(do_bin(p, kBinOnly));
+  if (force_global) ST_TRY(apply_pending_reorder(p));
+  const Pool pv = view(p);
+  HIP_TRY(launch_begin_p2g(pv, p->stream));
   if (force_global) {
-    HIP_TRY(launch_p2g_global(s->P[s->cur], s->cap, s->count, s->mats_d, vol4_of(s), s->A, fb, nullptr, 0, s->geo,
-                              s->stream));
-    s->grid_state = kGridA;
+    HIP_TRY(launch_p2g_global(pv, p->P[p->cur], p->count, p->stream));
+    p->grid_state = kGridA;
   } else {
-    HIP_TRY(launch_p2g_binned(s->P[s->cur], s->cap, s->mats_d, vol4_of(s), bin_view(s), s->A, fb, s->geo, s->max_items,
-                              s->stream));
-    s->grid_state = kGridTiles;
+    float4* dst = p->reorder_pending ? p->P[p->cur ^ 1] : nullptr;
+    HIP_TRY(launch_p2g_binned(pv, p->P[p->cur], p->perm, dst, p->p2g_waves, p->stream));
+    if (p->reorder_pending) {
+      p->cur ^= 1;
+      p->perm = nullptr;
+      p->reorder_pending = false;
+      recompute_offsets(p);
+    }
+    p->grid_state = kGridTiles;
   }
-  s->a_dirty_d = true;
+  p->a_dirty_d = true;
   return MPM_OK;
 }
 
 // Grid update (mpm.js:593-595) from tiles + A into V; A returns to zero.
-mpm_status do_grid(mpm_sim* s, const float g[3], float ff) {
-  GridArgs ga;
-  ga.g[0] = g[0]; ga.g[1] = g[1]; ga.g[2] = g[2];
-  ga.bound = s->cfg.bound;
-  ga.floor_friction = ff;
-  ga.ncol = s->ncol;
-  BinView bv = bin_view(s);
-  if (s->grid_state == kGridA) bv.scount = nullptr;  // global-atomic P2G: only D holds data
-  HIP_TRY(launch_grid_update(bv, fallback_view(s), s->A, s->V, s->geo, ga, s->cols_d, 2 * s->nblocks, s->stream));
-  s->a_dirty_d = false;
-  s->grid_state = kGridVelocity;
-  s->v_bin_epoch = bv.scount ? s->bin_epoch : 0xffffffffu;
-  s->v_epoch = s->epoch;
+mpm_status do_grid(mpm_pool* p) {
+  HIP_TRY(launch_grid_update(view(p), p->grid_state == kGridTiles, p->stream));
+  p->a_dirty_d = false;
+  p->grid_state = kGridVelocity;
   return MPM_OK;
 }
 
-// G2P of this substep fused with the next substep's P2G (grid state kGridTiles for the next grid
-// update). Valid only when the last P2G used the current binning (its tile boxes seed the new ones).
-mpm_status do_g2p_p2g(mpm_sim* s, float vmax) {
-  s->epoch++;
-  HIP_TRY(hipMemsetAsync(s->dcount, 0, sizeof(uint32_t), s->stream));
-  HIP_TRY(launch_g2p_p2g(s->P[s->cur], s->cap, s->V, s->geo, s->mats_d, s->cfg.bound, vmax, vol4_of(s), bin_view(s),
-                         s->A, fallback_view(s), s->max_items, s->stream));
-  s->grid_state = kGridTiles;
-  s->a_dirty_d = true;
+mpm_status do_g2p(mpm_pool* p, float vmax) {  // mpm.js:598-600
+  HIP_TRY(launch_g2p(view(p), p->P[p->cur], p->count, vmax, p->g2p_waves, p->stream));
   return MPM_OK;
 }
 
-mpm_status do_g2p(mpm_sim* s, float vmax) {  // mpm.js:598-600
-  HIP_TRY(launch_g2p(s->P[s->cur], s->cap, s->count, s->V, s->geo, s->mats_d, s->cfg.bound, vmax, s->stream));
+mpm_status mark(mpm_pool* p) {
+  if (!p->profiling) return MPM_OK;
+  if (p->ev_used == p->ev_pool.size()) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    p->ev_pool.push_back(e);
+  }
+  HIP_TRY(hipEventRecord(p->ev_pool[p->ev_used++], p->stream));
   return MPM_OK;
 }
 
-#define CHECK_SIM(s)                                                            \
-  do {                                                                          \
-    if (!(s)) return fail(MPM_ERR_INVALID_ARGUMENT, "simulation is NULL");      \
-    mpm_status _st = set_dev((s)->ctx);                                         \
-    if (_st != MPM_OK) return _st;                                              \
+// One substep on the stream (MPM.step's loop body, mpm.js:591-601).
+mpm_status substep(mpm_pool* p, float vmax) {
+  ST_TRY(mark(p));
+  if (p->step_count % p->sort_every == 0) {
+    ST_TRY(do_bin(p, kBinReorderInP2G));
+  } else if (!p->bin_valid) {
+    ST_TRY(do_bin(p, kBinOnly));
+  }
+  ST_TRY(mark(p));
+  ST_TRY(do_p2g(p, false));
+  ST_TRY(mark(p));
+  ST_TRY(do_grid(p));
+  ST_TRY(mark(p));
+  ST_TRY(do_g2p(p, vmax));
+  ST_TRY(mark(p));
+  p->step_count++;
+  return MPM_OK;
+}
+
+// One sort period (sort_every substeps starting at a sort) captured once per parity of the current
+// particle buffer and replayed; valid while the pool's particle count, scenes and vmax do not change.
+mpm_status period_graph(mpm_pool* p, float vmax, hipGraphExec_t* out) {
+  uint32_t vbits;
+  std::memcpy(&vbits, &vmax, 4);
+  const uint64_t key = ((uint64_t)p->count << 32) ^ (uint64_t)vbits;
+  if (key != p->graph_key) {
+    drop_graphs(p);
+    p->graph_key = key;
+  }
+  const int parity = p->cur;
+  if (p->period_graph[parity]) {
+    *out = p->period_graph[parity];
+    return MPM_OK;
+  }
+  // capture one period from the current host-side state, then restore that state: the launch below
+  // replays exactly what was captured
+  const uint64_t step0 = p->step_count;
+  HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+  mpm_status st = MPM_OK;
+  for (uint32_t k = 0; k < p->sort_every && st == MPM_OK; k++) st = substep(p, vmax);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(p->stream, &g);
+  p->step_count = step0;
+  p->cur = parity;
+  if (st != MPM_OK) {
+    if (g) hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) return from_hip(e, "hipStreamEndCapture");
+  e = hipGraphInstantiate(&p->period_graph[parity], g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) return from_hip(e, "hipGraphInstantiate");
+  *out = p->period_graph[parity];
+  return MPM_OK;
+}
+
+mpm_status pool_step(mpm_pool* p, uint32_t substeps, float vmax) {
+  if (p->needs_repack) ST_TRY(repack(p));
+  if (p->count == 0) return MPM_OK;  // mpm.js:584
+  uint32_t k = 0;
+  while (k < substeps) {
+    const bool at_sort = p->step_count % p->sort_every == 0;
+    if (p->use_graphs && !p->profiling && at_sort && substeps - k >= p->sort_every) {
+      ST_TRY(clean_a(p));
+      ST_TRY(apply_pending_reorder(p));
+      hipGraphExec_t ge = nullptr;
+      ST_TRY(period_graph(p, vmax, &ge));
+      HIP_TRY(hipGraphLaunch(ge, p->stream));
+      // host-side effects of one period: the sort's reorder swapped the buffers once
+      p->cur ^= 1;
+      p->perm = nullptr;
+      p->reorder_pending = false;
+      p->bin_valid = true;
+      p->grid_state = kGridVelocity;
+      p->a_dirty_d = p->a_dirty_s = false;
+      recompute_offsets(p);
+      p->step_count += p->sort_every;
+      k += p->sort_every;
+    } else {
+      ST_TRY(substep(p, vmax));
+      k++;
+    }
+  }
+  return MPM_OK;
+}
+
+mpm_status create_pool(mpm_ctx* c, uint32_t cap, uint32_t max_scenes, uint32_t max_blocks, int max_dim,
+                       uint32_t sort_every, mpm_pool** out) {
+  *out = nullptr;
+  if (cap == 0 || cap > (1u << 30)) return fail(MPM_ERR_INVALID_ARGUMENT, "pool: max_particles = %u outside [1, 2^30]", cap);
+  if (max_scenes == 0 || max_scenes > 4096) return fail(MPM_ERR_INVALID_ARGUMENT, "pool: max_scenes = %u outside [1, 4096]", max_scenes);
+  if (max_dim < 8 || max_dim > 1024) return fail(MPM_ERR_INVALID_ARGUMENT, "pool: max_dim = %d outside [8, 1024]", max_dim);
+  if (sort_every == 0) return fail(MPM_ERR_INVALID_ARGUMENT, "pool: sort_every = 0");
+  const int bits = std::max(3, ceil_log2((uint64_t)max_dim));
+  const int scene_bits = ceil_log2(max_scenes);
+  if (3 * bits + scene_bits > 32)
+    return fail(MPM_ERR_INVALID_ARGUMENT, "pool: %u scenes of edge %d do not fit a 32-bit sort key", max_scenes, max_dim);
+  ST_TRY(set_dev(c));
+  mpm_pool* p = new (std::nothrow) mpm_pool;
+  if (!p) return fail(MPM_ERR_OUT_OF_MEMORY, "pool: host allocation");
+  p->ctx = c;
+  p->cap = cap;
+  p->max_scenes = max_scenes;
+  p->max_blocks = max_blocks;
+  p->max_dim = max_dim;
+  p->sort_every = sort_every;
+  p->key_bits = (uint32_t)(3 * bits);
+  p->sort_bits = p->key_bits + (uint32_t)scene_bits;
+  pool_bytes(cap, max_scenes, max_blocks, max_dim, &p->map_per_scene, &p->max_items, &p->tile_slots);
+  p->scenes.resize(max_scenes);
+  p->free_blocks.push_back({0u, max_blocks});
+  const size_t nodes = (size_t)max_blocks * 64;
+  hipError_t e = hipStreamCreate(&p->stream);
+  if (e == hipSuccess) e = dmalloc(&p->P[0], (size_t)cap * 8);
+  if (e == hipSuccess) e = dmalloc(&p->P[1], (size_t)cap * 8);
+  if (e == hipSuccess) e = dmalloc(&p->A, nodes);
+  if (e == hipSuccess) e = dmalloc(&p->V, nodes);
+  if (e == hipSuccess) e = dmalloc(&p->block_scene, max_blocks);
+  if (e == hipSuccess) e = dmalloc(&p->stamp, max_blocks);
+  if (e == hipSuccess) e = dmalloc(&p->dlist, max_blocks);
+  if (e == hipSuccess) e = dmalloc(&p->sstamp, max_blocks);
+  if (e == hipSuccess) e = dmalloc(&p->slist, max_blocks);
+  if (e == hipSuccess) e = dmalloc(&p->map, (size_t)p->map_per_scene * max_scenes);
+  if (e == hipSuccess) e = dmalloc(&p->wl_start, (size_t)p->max_items + 1);
+  if (e == hipSuccess) e = dmalloc(&p->wl_block, p->max_items);
+  if (e == hipSuccess) e = dmalloc(&p->tiles, (size_t)p->tile_slots * kTileNodes);
+  if (e == hipSuccess) e = dmalloc(&p->tile_hdr, p->tile_slots);
+  if (e == hipSuccess) e = dmalloc(&p->mats, (size_t)max_scenes * kMatRows * 12);
+  if (e == hipSuccess) e = dmalloc(&p->cols, (size_t)max_scenes * kColRows * 12);
+  if (e == hipSuccess) e = dmalloc(&p->scenes_d, max_scenes);
+  if (e == hipSuccess) e = dmalloc(&p->ctr, kCounters);
+  if (e == hipSuccess) e = dmalloc(&p->keys[0], cap);
+  if (e == hipSuccess) e = dmalloc(&p->keys[1], cap);
+  if (e == hipSuccess) e = dmalloc(&p->vals[0], cap);
+  if (e == hipSuccess) e = dmalloc(&p->vals[1], cap);
+  if (e == hipSuccess) e = dmalloc(&p->order, cap);
+  if (e == hipSuccess) e = dmalloc(&p->flags, (size_t)cap + 1);
+  if (e == hipSuccess)
+    e = dmalloc(&p->sort_scratch, std::max(sort_scratch_words(cap), scan_scratch_words((size_t)cap + 1)));
+  if (e == hipSuccess) {
+    p->staging_records = std::min<uint32_t>(cap, 1u << 20);
+    e = dmalloc(&p->staging, (size_t)p->staging_records * 8);
+  }
+  if (e == hipSuccess) e = hipMemsetAsync(p->A, 0, nodes * sizeof(int4), p->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(p->V, 0, nodes * sizeof(float4), p->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(p->block_scene, 0xff, (size_t)max_blocks * 4, p->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(p->stamp, 0xff, (size_t)max_blocks * 4, p->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(p->sstamp, 0xff, (size_t)max_blocks * 4, p->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(p->ctr, 0, kCounters * 4, p->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(p->mats, 0, (size_t)max_scenes * kMatRows * 48, p->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(p->cols, 0, (size_t)max_scenes * kColRows * 48, p->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(p->scenes_d, 0, (size_t)max_scenes * sizeof(SceneGeo), p->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(p->P[0], 0, (size_t)cap * 128, p->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+  if (e != hipSuccess) {
+    mpm_status code = from_hip(e, "pool allocation");
+    free_pool(p);
+    return code;
+  }
+  *out = p;
+  return MPM_OK;
+}
+
+// Adds a scene: geometry, node blocks, material rows (particles are written by the caller).
+mpm_status add_scene(mpm_pool* p, const int32_t dims[3], int32_t ppc_axis, float bound, const float g[3],
+                     const float* mats, uint32_t nmat, uint32_t* id) {
+  for (int a = 0; a < 3; a++)
+    if (dims[a] < 8 || dims[a] > p->max_dim)
+      return fail(MPM_ERR_INVALID_ARGUMENT, "scene: dims[%d] = %d outside [8, %d]", a, dims[a], p->max_dim);
+  if (ppc_axis < 1 || ppc_axis > 8) return fail(MPM_ERR_INVALID_ARGUMENT, "scene: ppc_axis = %d outside [1, 8]", ppc_axis);
+  // bound >= 3 keeps every stencil inside the grid: G2P clamps x to [bound-1, dims-bound+1]
+  // (mpm.js:502), so the base cell floor(x - 1/2) stays in [1, dims-3].
+  if (!(bound >= 3.0f && bound <= 16.0f)) return fail(MPM_ERR_INVALID_ARGUMENT, "scene: bound = %g outside [3, 16]", (double)bound);
+  if (nmat > (uint32_t)kMatRows) return fail(MPM_ERR_INVALID_ARGUMENT, "scene: at most 8 materials (got %u)", nmat);
+  uint32_t s = 0;
+  while (s < p->scenes.size() && p->scenes[s].used) s++;
+  if (s == p->scenes.size()) return fail(MPM_ERR_OUT_OF_MEMORY, "pool: all %u scene slots are in use", p->max_scenes);
+  SceneHost sc;
+  SceneGeo& geo = sc.geo;
+  geo.X = dims[0]; geo.Y = dims[1]; geo.Z = dims[2];
+  geo.nbx = (dims[0] + 3) / 4; geo.nby = (dims[1] + 3) / 4; geo.nbz = (dims[2] + 3) / 4;
+  geo.mbits = std::max(1, ceil_log2((uint64_t)std::max(dims[0], std::max(dims[1], dims[2]))) - 2);
+  sc.nblocks = (uint32_t)geo.nbx * geo.nby * geo.nbz;
+  if (!alloc_blocks(p, sc.nblocks, &geo.block_base))
+    return fail(MPM_ERR_OUT_OF_MEMORY, "pool: no run of %u free node blocks (%u free in total)", sc.nblocks,
+                free_block_total(p));
+  geo.map_base = s * p->map_per_scene;
+  geo.bound = bound;
+  geo.g[0] = g[0]; geo.g[1] = g[1]; geo.g[2] = g[2];
+  geo.floor_friction = 0.4f;
+  geo.ncol = 0;
+  geo.vol4 = (float)(4.0 * (1.0 / std::pow((double)ppc_axis, 3)));  // mpm.js:610
+  geo.active = 1;
+  sc.used = true;
+  sc.count = 0;
+  sc.offset = p->count;
+  p->scenes[s] = sc;
+  p->mem_order.push_back(s);
+  ST_TRY(upload_scene(p, s));
+  ST_TRY(upload_materials(p, s, mats, nmat));
+  ST_TRY(upload_colliders(p, s, nullptr, 0));
+  HIP_TRY(launch_fill_u32(p->block_scene + geo.block_base, s, sc.nblocks, p->stream));
+  HIP_TRY(launch_fill_u32(p->stamp + geo.block_base, 0xffffffffu, sc.nblocks, p->stream));
+  HIP_TRY(launch_fill_u32(p->sstamp + geo.block_base, 0xffffffffu, sc.nblocks, p->stream));
+  *id = s;
+  return MPM_OK;
+}
+
+mpm_status validate_records(const mpm_pool* p, uint32_t s, const float* rec, uint32_t n, uint32_t first) {
+  const SceneGeo& geo = p->scenes[s].geo;
+  const int dims[3] = {geo.X, geo.Y, geo.Z};
+  for (uint32_t i = 0; i < n; i++) {
+    const float* r = rec + 32 * (size_t)i;
+    for (int a = 0; a < 3; a++)
+      if (!(r[a] >= 0.5f && r[a] < (float)dims[a] - 1.5f))
+        return fail(MPM_ERR_INVALID_ARGUMENT, "record %u position[%d] = %g outside [0.5, %d)", first + i, a,
+                    (double)r[a], dims[a] - 1);
+    uint32_t mid;
+    std::memcpy(&mid, r + 3, 4);
+    if (mid >= (uint32_t)kMatRows) return fail(MPM_ERR_INVALID_ARGUMENT, "record %u material %u >= 8", first + i, mid);
+  }
+  return MPM_OK;
+}
+
+// Writes n records (reference layout, scene-local material ids) to pool slots [at, at + n).
+mpm_status write_records(mpm_pool* p, uint32_t s, const float* rec, uint32_t at, uint32_t n) {
+  ST_TRY(apply_pending_reorder(p));
+  for (uint32_t done = 0; done < n;) {
+    const uint32_t k = std::min(n - done, p->staging_records);
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    HIP_TRY(hipMemcpy(p->staging, rec + 32 * (size_t)done, (size_t)k * 128, hipMemcpyHostToDevice));
+    HIP_TRY(launch_records_in(p->staging, p->P[p->cur], p->cap, at + done, k, s * kMatRows, p->stream));
+    done += k;
+  }
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  p->bin_valid = false;
+  return MPM_OK;
+}
+
+mpm_status read_records(mpm_pool* p, uint32_t s, float* rec, uint32_t at, uint32_t n) {
+  ST_TRY(apply_pending_reorder(p));
+  for (uint32_t done = 0; done < n;) {
+    const uint32_t k = std::min(n - done, p->staging_records);
+    HIP_TRY(launch_records_out(p->P[p->cur], p->cap, at + done, k, s * kMatRows, p->staging, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    HIP_TRY(hipMemcpy(rec + 32 * (size_t)done, p->staging, (size_t)k * 128, hipMemcpyDeviceToHost));
+    done += k;
+  }
+  return MPM_OK;
+}
+
+#define CHECK_SIM(s)                                                        \
+  do {                                                                      \
+    if (!(s)) return fail(MPM_ERR_INVALID_ARGUMENT, "simulation is NULL");  \
+    ST_TRY(set_dev((s)->pool->ctx));                                        \
+  } while (0)
+
+#define CHECK_POOL(p)                                                 \
+  do {                                                                \
+    if (!(p)) return fail(MPM_ERR_INVALID_ARGUMENT, "pool is NULL");  \
+    ST_TRY(set_dev((p)->ctx));                                        \
   } while (0)
 
 }  // namespace
@@ -369,90 +778,41 @@ mpm_status mpm_ctx_info(mpm_ctx* c, char* name, size_t name_len, int32_t* num_cu
 }
 
 mpm_status mpm_sim_create(mpm_ctx* c, const mpm_config* cfg, mpm_sim** out) {
+  mpm_pool* p = nullptr;
   if (!c || !cfg || !out) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_sim_create: NULL argument");
   *out = nullptr;
   for (int a = 0; a < 3; a++)
     if (cfg->dims[a] < 8 || cfg->dims[a] > 1024)
       return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_sim_create: dims[%d] = %d outside [8, 1024]", a, cfg->dims[a]);
-  if (cfg->max_particles == 0 || cfg->max_particles > (1u << 30))
-    return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_sim_create: max_particles = %u outside [1, 2^30]", cfg->max_particles);
   if (cfg->ppc_axis < 1 || cfg->ppc_axis > 8)
     return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_sim_create: ppc_axis = %d outside [1, 8]", cfg->ppc_axis);
-  // bound >= 3 keeps every stencil inside the grid: G2P clamps x to [bound-1, dims-bound+1]
-  // (mpm.js:502), so the base cell floor(x - 1/2) stays in [1, dims-3].
   if (!(cfg->bound >= 3.0f && cfg->bound <= 16.0f))
     return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_sim_create: bound = %g outside [3, 16]", (double)cfg->bound);
-  if (cfg->sort_every == 0) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_sim_create: sort_every = 0");
-  mpm_status st = set_dev(c);
-  if (st != MPM_OK) return st;
+  const int maxd = std::max(cfg->dims[0], std::max(cfg->dims[1], cfg->dims[2]));
+  const uint32_t blocks = (uint32_t)((cfg->dims[0] + 3) / 4) * ((cfg->dims[1] + 3) / 4) * ((cfg->dims[2] + 3) / 4);
+  ST_TRY(create_pool(c, cfg->max_particles, 1, blocks, maxd, cfg->sort_every, &p));
+  const float g0[3] = {0, 0, 0};
+  uint32_t id = 0;
+  mpm_status st = add_scene(p, cfg->dims, cfg->ppc_axis, cfg->bound, g0, nullptr, 0, &id);
+  if (st != MPM_OK) {
+    free_pool(p);
+    return st;
+  }
   mpm_sim* s = new (std::nothrow) mpm_sim;
-  if (!s) return fail(MPM_ERR_OUT_OF_MEMORY, "mpm_sim_create: host allocation");
-  s->ctx = c;
-  s->cfg = *cfg;
-  s->cap = cfg->max_particles;
-  s->geo.X = cfg->dims[0]; s->geo.Y = cfg->dims[1]; s->geo.Z = cfg->dims[2];
-  s->geo.nbx = (cfg->dims[0] + 3) / 4; s->geo.nby = (cfg->dims[1] + 3) / 4; s->geo.nbz = (cfg->dims[2] + 3) / 4;
-  s->nblocks = (uint32_t)s->geo.nbx * s->geo.nby * s->geo.nbz;
-  int maxd = std::max(cfg->dims[0], std::max(cfg->dims[1], cfg->dims[2]));
-  int bits = 0;
-  while ((1 << bits) < maxd) bits++;
-  s->key_bits = 3 * bits;
-  s->geo.mbits = std::max(1, bits - 2);
-  s->nbm = 1u << (3 * s->geo.mbits);
-  s->max_items = std::min<uint32_t>(s->nbm, s->cap);
-  s->tile_slots = (uint32_t)std::min<size_t>(s->max_items, kTileBudgetBytes / (sizeof(int4) * kTileNodes));
-  const size_t nodes = (size_t)s->nblocks * 64;
-  hipError_t e = hipStreamCreate(&s->stream);
-  if (e == hipSuccess) e = dmalloc(&s->P[0], (size_t)s->cap * 8);
-  if (e == hipSuccess) e = dmalloc(&s->P[1], (size_t)s->cap * 8);
-  if (e == hipSuccess) e = dmalloc(&s->A, nodes);
-  if (e == hipSuccess) e = dmalloc(&s->V, nodes);
-  if (e == hipSuccess) e = dmalloc(&s->stamp, s->nblocks);
-  if (e == hipSuccess) e = dmalloc(&s->dlist, s->nblocks);
-  if (e == hipSuccess) e = dmalloc(&s->dcount, 1);
-  if (e == hipSuccess) e = dmalloc(&s->wl_start, (size_t)s->max_items + 1);
-  if (e == hipSuccess) e = dmalloc(&s->wl_block, s->max_items);
-  if (e == hipSuccess) e = dmalloc(&s->item_count, 1);
-  if (e == hipSuccess) e = dmalloc(&s->map, s->nbm);
-  if (e == hipSuccess) e = dmalloc(&s->flags, (size_t)s->cap + 1);
-  if (e == hipSuccess) e = dmalloc(&s->slist, s->nblocks);
-  if (e == hipSuccess) e = dmalloc(&s->scount, 1);
-  if (e == hipSuccess) e = dmalloc(&s->sstamp, s->nblocks);
-  if (e == hipSuccess) e = dmalloc(&s->tiles, (size_t)s->tile_slots * kTileNodes);
-  if (e == hipSuccess) e = dmalloc(&s->tile_hdr, s->tile_slots);
-  if (e == hipSuccess) e = dmalloc(&s->mats_d, 192);
-  if (e == hipSuccess) e = dmalloc(&s->cols_d, 192);
-  if (e == hipSuccess) e = dmalloc(&s->keys[0], s->cap);
-  if (e == hipSuccess) e = dmalloc(&s->keys[1], s->cap);
-  if (e == hipSuccess) e = dmalloc(&s->vals[0], s->cap);
-  if (e == hipSuccess) e = dmalloc(&s->vals[1], s->cap);
-  if (e == hipSuccess)
-    e = dmalloc(&s->sort_scratch, std::max(sort_scratch_words(s->cap), scan_scratch_words((size_t)s->cap + 1)));
-  if (e == hipSuccess) {
-    s->staging_records = std::min<uint32_t>(s->cap, 1u << 20);
-    e = dmalloc(&s->staging, (size_t)s->staging_records * 8);
+  if (!s) {
+    free_pool(p);
+    return fail(MPM_ERR_OUT_OF_MEMORY, "mpm_sim_create: host allocation");
   }
-  if (e == hipSuccess) e = hipMemsetAsync(s->A, 0, nodes * sizeof(int4), s->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(s->V, 0, nodes * sizeof(float4), s->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(s->stamp, 0xff, (size_t)s->nblocks * 4, s->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(s->sstamp, 0xff, (size_t)s->nblocks * 4, s->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(s->dcount, 0, 4, s->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(s->scount, 0, 4, s->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(s->item_count, 0, 4, s->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(s->mats_d, 0, 192 * 4, s->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(s->cols_d, 0, 192 * 4, s->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(s->P[0], 0, (size_t)s->cap * 128, s->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-  if (e != hipSuccess) {
-    mpm_status code = from_hip(e, "mpm_sim_create");
-    free_sim(s);
-    return code;
-  }
+  s->pool = p;
   *out = s;
   return MPM_OK;
 }
 
-void mpm_sim_destroy(mpm_sim* s) { free_sim(s); }
+void mpm_sim_destroy(mpm_sim* s) {
+  if (!s) return;
+  free_pool(s->pool);
+  delete s;
+}
 
 mpm_status mpm_set_materials(mpm_sim* s, const float* tbl, uint32_t n) {
   CHECK_SIM(s);
@@ -463,216 +823,304 @@ mpm_status mpm_set_materials(mpm_sim* s, const float* tbl, uint32_t n) {
     std::memcpy(&kind, tbl + 12 * k, 4);
     if (kind > 3) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_materials: row %u has kind %u (0-3)", k, kind);
   }
-  float t[192] = {0};
-  if (n) std::memcpy(t, tbl, 48 * (size_t)n);
-  return write_table(s, s->mats_d, t, 96);
+  return upload_materials(s->pool, 0, tbl, n);
 }
 
 mpm_status mpm_set_colliders(mpm_sim* s, const float* tbl, uint32_t n) {
   CHECK_SIM(s);
   if (n > 16) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_colliders: at most 16 colliders (got %u)", n);
   if (n && !tbl) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_colliders: table is NULL");
-  float t[192] = {0};
-  if (n) std::memcpy(t, tbl, 48 * (size_t)n);
-  s->ncol = n;
-  return write_table(s, s->cols_d, t, 192);
+  mpm_pool* p = s->pool;
+  ST_TRY(upload_colliders(p, 0, tbl, n));
+  if (p->scenes[0].geo.ncol != n) {
+    p->scenes[0].geo.ncol = n;
+    ST_TRY(upload_scene(p, 0));
+  }
+  return MPM_OK;
 }
 
 mpm_status mpm_write_particles(mpm_sim* s, const float* rec, uint32_t offset, uint32_t n) {
   CHECK_SIM(s);
   if (n == 0) return MPM_OK;
   if (!rec) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_write_particles: records NULL");
-  if ((uint64_t)offset + n > s->cap)
-    return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_write_particles: offset %u + n %u exceeds capacity %u", offset, n, s->cap);
-  const int dims[3] = {s->geo.X, s->geo.Y, s->geo.Z};
-  for (uint32_t i = 0; i < n; i++) {
-    const float* r = rec + 32 * (size_t)i;
-    for (int a = 0; a < 3; a++)
-      if (!(r[a] >= 0.5f && r[a] < (float)dims[a] - 1.5f))
-        return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_write_particles: record %u position[%d] = %g outside [0.5, %d)",
-                    offset + i, a, (double)r[a], dims[a] - 1);
-    uint32_t mid;
-    std::memcpy(&mid, r + 3, 4);
-    if (mid >= 8) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_write_particles: record %u material %u >= 8", offset + i, mid);
-  }
-  for (uint32_t done = 0; done < n;) {
-    const uint32_t k = std::min(n - done, s->staging_records);
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    HIP_TRY(hipMemcpy(s->staging, rec + 32 * (size_t)done, (size_t)k * 128, hipMemcpyHostToDevice));
-    HIP_TRY(launch_records_in(s->staging, s->P[s->cur], s->cap, offset + done, k, s->stream));
-    done += k;
-  }
-  s->bin_valid = false;
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  return MPM_OK;
+  mpm_pool* p = s->pool;
+  if ((uint64_t)offset + n > p->cap)
+    return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_write_particles: offset %u + n %u exceeds capacity %u", offset, n, p->cap);
+  ST_TRY(validate_records(p, 0, rec, n, offset));
+  return write_records(p, 0, rec, offset, n);
 }
 
 mpm_status mpm_read_particles(mpm_sim* s, float* rec, uint32_t offset, uint32_t n) {
   CHECK_SIM(s);
   if (n == 0) return MPM_OK;
   if (!rec) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_read_particles: records NULL");
-  if ((uint64_t)offset + n > s->cap)
-    return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_read_particles: offset %u + n %u exceeds capacity %u", offset, n, s->cap);
-  for (uint32_t done = 0; done < n;) {
-    const uint32_t k = std::min(n - done, s->staging_records);
-    HIP_TRY(launch_records_out(s->P[s->cur], s->cap, offset + done, k, s->staging, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    HIP_TRY(hipMemcpy(rec + 32 * (size_t)done, s->staging, (size_t)k * 128, hipMemcpyDeviceToHost));
-    done += k;
-  }
-  return MPM_OK;
+  if ((uint64_t)offset + n > s->pool->cap)
+    return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_read_particles: offset %u + n %u exceeds capacity %u", offset, n,
+                s->pool->cap);
+  return read_records(s->pool, 0, rec, offset, n);
 }
 
 mpm_status mpm_set_count(mpm_sim* s, uint32_t n) {
   CHECK_SIM(s);
-  if (n > s->cap) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_count: %u exceeds capacity %u", n, s->cap);
-  if (n != s->count) s->bin_valid = false;
-  s->count = n;
+  mpm_pool* p = s->pool;
+  if (n > p->cap) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_count: %u exceeds capacity %u", n, p->cap);
+  if (n != p->count) {
+    ST_TRY(apply_pending_reorder(p));
+    p->bin_valid = false;
+    drop_graphs(p);
+  }
+  p->count = n;
+  p->scenes[0].count = n;
   return MPM_OK;
 }
 
 mpm_status mpm_get_count(mpm_sim* s, uint32_t* n) {
   if (!s || !n) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_get_count: NULL argument");
-  *n = s->count;
+  *n = s->pool->count;
   return MPM_OK;
 }
 
 mpm_status mpm_set_step_count(mpm_sim* s, uint64_t n) {
   if (!s) return fail(MPM_ERR_INVALID_ARGUMENT, "simulation is NULL");
-  s->step_count = n;
+  s->pool->step_count = n;
   return MPM_OK;
 }
 
 mpm_status mpm_get_step_count(mpm_sim* s, uint64_t* n) {
   if (!s || !n) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_get_step_count: NULL argument");
-  *n = s->step_count;
+  *n = s->pool->step_count;
   return MPM_OK;
 }
-
-}  // extern "C"
-
-namespace {
-mpm_status next_event(mpm_sim* s, hipEvent_t* out) {
-  if (s->ev_used == s->ev_pool.size()) {
-    hipEvent_t e;
-    HIP_TRY(hipEventCreate(&e));
-    s->ev_pool.push_back(e);
-  }
-  *out = s->ev_pool[s->ev_used++];
-  return MPM_OK;
-}
-mpm_status mark(mpm_sim* s) {
-  if (!s->profiling) return MPM_OK;
-  hipEvent_t e;
-  mpm_status st = next_event(s, &e);
-  if (st) return st;
-  HIP_TRY(hipEventRecord(e, s->stream));
-  return MPM_OK;
-}
-}  // namespace
-
-extern "C" {
 
 mpm_status mpm_step(mpm_sim* s, uint32_t substeps, const float g[3], float ff, float vmax) {
   CHECK_SIM(s);
   if (!g) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_step: gravity is NULL");
-  if (s->count == 0) return MPM_OK;  // mpm.js:584
-  const bool fuse = !(s->cfg.flags & MPM_FLAG_NO_FUSE);
-  bool p2g_ready = false;  // this substep's P2G was already done by the previous fused kernel
-  for (uint32_t k = 0; k < substeps; k++) {
-    mpm_status st;
-    if ((st = mark(s))) return st;
-    if (!p2g_ready) {
-      if (s->step_count % s->cfg.sort_every == 0) {
-        if ((st = do_bin(s, true))) return st;
-      } else if (!s->bin_valid && (st = do_bin(s, false))) {
-        return st;
-      }
-    }
-    if ((st = mark(s))) return st;
-    if (!p2g_ready && (st = do_p2g(s, false))) return st;
-    if ((st = mark(s))) return st;
-    if ((st = do_grid(s, g, ff))) return st;
-    if ((st = mark(s))) return st;
-    // fuse this G2P with the next P2G unless the call ends here or the next substep sorts
-    p2g_ready = fuse && k + 1 < substeps && (s->step_count + 1) % s->cfg.sort_every != 0;
-    if ((st = p2g_ready ? do_g2p_p2g(s, vmax) : do_g2p(s, vmax))) return st;
-    if ((st = mark(s))) return st;
-    s->step_count++;
+  mpm_pool* p = s->pool;
+  SceneGeo& geo = p->scenes[0].geo;
+  if (geo.g[0] != g[0] || geo.g[1] != g[1] || geo.g[2] != g[2] || geo.floor_friction != ff) {
+    geo.g[0] = g[0]; geo.g[1] = g[1]; geo.g[2] = g[2];
+    geo.floor_friction = ff;
+    ST_TRY(upload_scene(p, 0));
   }
-  return MPM_OK;
+  return pool_step(p, substeps, vmax);
 }
 
 mpm_status mpm_p2g(mpm_sim* s, int32_t force_global) {
   CHECK_SIM(s);
-  mpm_status st = do_p2g(s, force_global != 0);
-  if (st) return st;
-  if (s->grid_state == kGridTiles) {  // fold the tiles into A so the grid reads as the i32 P2G result
-    HIP_TRY(launch_resolve_tiles(bin_view(s), fallback_view(s), s->A, s->geo, 2 * s->nblocks, s->stream));
-    s->grid_state = kGridA;
-    s->a_dirty_s = true;
+  mpm_pool* p = s->pool;
+  ST_TRY(do_p2g(p, force_global != 0));
+  if (p->grid_state == kGridTiles) {  // fold the tiles into A so the grid reads as the i32 P2G result
+    HIP_TRY(launch_resolve_tiles(view(p), p->stream));
+    p->grid_state = kGridA;
+    p->a_dirty_s = true;
   }
   return MPM_OK;
 }
 
 mpm_status mpm_sort(mpm_sim* s) {
   CHECK_SIM(s);
-  return do_bin(s, true);
+  return do_bin(s->pool, kBinReorder);
 }
 
 mpm_status mpm_rescale(mpm_sim* s, float k) {
   CHECK_SIM(s);
-  HIP_TRY(launch_rescale(s->P[s->cur], s->cap, s->count, k, s->stream));
+  mpm_pool* p = s->pool;
+  ST_TRY(apply_pending_reorder(p));
+  HIP_TRY(launch_rescale(p->P[p->cur], p->cap, 0, p->count, k, p->stream));
   return MPM_OK;
 }
 
 mpm_status mpm_read_grid(mpm_sim* s, int32_t* words) {
   CHECK_SIM(s);
   if (!words) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_read_grid: words NULL");
-  const size_t nodes = (size_t)s->geo.X * s->geo.Y * s->geo.Z;
-  if (!s->export_buf) HIP_TRY(dmalloc(&s->export_buf, nodes));
-  if (s->grid_state != kGridA && s->grid_state != kGridVelocity) {
+  mpm_pool* p = s->pool;
+  const SceneGeo& geo = p->scenes[0].geo;
+  const size_t nodes = (size_t)geo.X * geo.Y * geo.Z;
+  if (p->grid_state != kGridA && p->grid_state != kGridVelocity) {
     std::memset(words, 0, nodes * 16);
     return MPM_OK;
   }
-  HIP_TRY(launch_grid_export(s->A, s->V, s->sstamp, s->v_bin_epoch, s->stamp, s->v_epoch,
-                             s->grid_state == kGridA ? 0 : 1, s->geo, s->export_buf, s->stream));
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  HIP_TRY(hipMemcpy(words, s->export_buf, nodes * 16, hipMemcpyDeviceToHost));
+  if (p->export_nodes < nodes) {
+    if (p->export_buf) hipFree(p->export_buf);
+    p->export_buf = nullptr;
+    HIP_TRY(dmalloc(&p->export_buf, nodes));
+    p->export_nodes = nodes;
+  }
+  HIP_TRY(launch_grid_export(view(p), geo, p->grid_state == kGridA ? 0 : 1, p->export_buf, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  HIP_TRY(hipMemcpy(words, p->export_buf, nodes * 16, hipMemcpyDeviceToHost));
   return MPM_OK;
 }
 
 mpm_status mpm_sync(mpm_sim* s) {
   CHECK_SIM(s);
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipStreamSynchronize(s->pool->stream));
   return MPM_OK;
+}
+
+mpm_status mpm_set_tuning(mpm_sim* s, int32_t knob, int32_t value) {
+  if (!s) return fail(MPM_ERR_INVALID_ARGUMENT, "simulation is NULL");
+  return mpm_pool_set_tuning(s->pool, knob, value);
 }
 
 mpm_status mpm_set_profiling(mpm_sim* s, int32_t enable) {
   if (!s) return fail(MPM_ERR_INVALID_ARGUMENT, "simulation is NULL");
-  s->profiling = enable != 0;
-  s->ev_used = 0;
-  for (float& m : s->phase_ms) m = 0.0f;
-  s->phase_substeps = 0;
+  mpm_pool* p = s->pool;
+  p->profiling = enable != 0;
+  p->ev_used = 0;
+  for (float& m : p->phase_ms) m = 0.0f;
   return MPM_OK;
 }
 
 mpm_status mpm_phase_times(mpm_sim* s, float ms[4]) {
   if (!s || !ms) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_phase_times: NULL argument");
-  mpm_status st = set_dev(s->ctx);
-  if (st) return st;
-  const size_t groups = s->ev_used / 5;
-  if (groups) HIP_TRY(hipEventSynchronize(s->ev_pool[groups * 5 - 1]));
+  mpm_pool* p = s->pool;
+  ST_TRY(set_dev(p->ctx));
+  const size_t groups = p->ev_used / 5;
+  if (groups) HIP_TRY(hipEventSynchronize(p->ev_pool[groups * 5 - 1]));
   for (size_t g = 0; g < groups; g++)
-    for (int p = 0; p < 4; p++) {
+    for (int ph = 0; ph < 4; ph++) {
       float t = 0;
-      HIP_TRY(hipEventElapsedTime(&t, s->ev_pool[g * 5 + p], s->ev_pool[g * 5 + p + 1]));
-      s->phase_ms[p] += t;
+      HIP_TRY(hipEventElapsedTime(&t, p->ev_pool[g * 5 + ph], p->ev_pool[g * 5 + ph + 1]));
+      p->phase_ms[ph] += t;
     }
-  s->phase_substeps += (uint32_t)groups;
-  s->ev_used = 0;
-  for (int p = 0; p < 4; p++) ms[p] = s->phase_ms[p];
+  p->ev_used = 0;
+  for (int ph = 0; ph < 4; ph++) ms[ph] = p->phase_ms[ph];
   return MPM_OK;
+}
+
+// ---- pools ----
+size_t mpm_pool_bytes(const mpm_pool_config* cfg) {
+  if (!cfg) return 0;
+  return pool_bytes(cfg->max_particles, cfg->max_scenes, cfg->max_node_blocks, cfg->max_dim, nullptr, nullptr, nullptr);
+}
+
+mpm_status mpm_pool_create(mpm_ctx* c, const mpm_pool_config* cfg, mpm_pool** out) {
+  if (!c || !cfg || !out) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_create: NULL argument");
+  if (cfg->max_node_blocks == 0) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_create: max_node_blocks = 0");
+  return create_pool(c, cfg->max_particles, cfg->max_scenes, cfg->max_node_blocks, cfg->max_dim,
+                     cfg->sort_every ? cfg->sort_every : 4, out);
+}
+
+void mpm_pool_destroy(mpm_pool* p) { free_pool(p); }
+
+mpm_status mpm_pool_add_scene(mpm_pool* p, const mpm_scene_desc* d, uint32_t* scene) {
+  CHECK_POOL(p);
+  if (!d || !scene) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_add_scene: NULL argument");
+  if (d->count && !d->records) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_add_scene: records NULL");
+  if (d->num_materials && !d->materials) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_add_scene: materials NULL");
+  for (uint32_t k = 0; k < d->num_materials && k < (uint32_t)kMatRows; k++) {
+    uint32_t kind;
+    std::memcpy(&kind, d->materials + 12 * k, 4);
+    if (kind > 3) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_add_scene: material %u has kind %u (0-3)", k, kind);
+  }
+  if (p->needs_repack) ST_TRY(repack(p));
+  if ((uint64_t)p->count + d->count > p->cap)
+    return fail(MPM_ERR_OUT_OF_MEMORY, "pool: %u particles do not fit (%u of %u used)", d->count, p->count, p->cap);
+  uint32_t id = 0;
+  ST_TRY(add_scene(p, d->dims, d->ppc_axis, d->bound, d->gravity, d->materials, d->num_materials, &id));
+  mpm_status st = validate_records(p, id, d->records, d->count, 0);
+  if (st == MPM_OK && d->count) st = write_records(p, id, d->records, p->count, d->count);
+  if (st != MPM_OK) {
+    release_blocks(p, p->scenes[id].geo.block_base, p->scenes[id].nblocks);
+    p->scenes[id] = SceneHost();
+    p->mem_order.pop_back();
+    return st;
+  }
+  p->scenes[id].offset = p->count;
+  p->scenes[id].count = d->count;
+  p->count += d->count;
+  p->grid_state = kGridClean;
+  drop_graphs(p);
+  *scene = id;
+  return MPM_OK;
+}
+
+mpm_status mpm_pool_remove_scene(mpm_pool* p, uint32_t scene) {
+  CHECK_POOL(p);
+  if (scene >= p->scenes.size() || !p->scenes[scene].used || !p->scenes[scene].geo.active)
+    return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_remove_scene: no scene %u", scene);
+  p->scenes[scene].geo.active = 0;
+  ST_TRY(upload_scene(p, scene));
+  p->needs_repack = true;
+  return MPM_OK;
+}
+
+mpm_status mpm_pool_step(mpm_pool* p, uint32_t substeps, float floor_friction, float vmax) {
+  CHECK_POOL(p);
+  for (uint32_t s = 0; s < p->scenes.size(); s++) {
+    SceneHost& sc = p->scenes[s];
+    if (sc.used && sc.geo.active && sc.geo.floor_friction != floor_friction) {
+      sc.geo.floor_friction = floor_friction;
+      ST_TRY(upload_scene(p, s));
+    }
+  }
+  return pool_step(p, substeps, vmax);
+}
+
+mpm_status mpm_pool_scene_count(mpm_pool* p, uint32_t scene, uint32_t* n) {
+  if (!p || !n) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_scene_count: NULL argument");
+  if (scene >= p->scenes.size() || !p->scenes[scene].used || !p->scenes[scene].geo.active)
+    return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_scene_count: no scene %u", scene);
+  *n = p->scenes[scene].count;
+  return MPM_OK;
+}
+
+mpm_status mpm_pool_read_scene(mpm_pool* p, uint32_t scene, float* rec, uint32_t capacity, uint32_t* n) {
+  CHECK_POOL(p);
+  if (scene >= p->scenes.size() || !p->scenes[scene].used || !p->scenes[scene].geo.active)
+    return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_read_scene: no scene %u", scene);
+  ST_TRY(apply_pending_reorder(p));
+  const SceneHost& sc = p->scenes[scene];
+  if (n) *n = sc.count;
+  if (!rec) return MPM_OK;
+  return read_records(p, scene, rec, sc.offset, std::min(capacity, sc.count));
+}
+
+mpm_status mpm_pool_info(mpm_pool* p, uint32_t* particles, uint32_t* scenes, uint32_t* free_node_blocks) {
+  if (!p) return fail(MPM_ERR_INVALID_ARGUMENT, "pool is NULL");
+  uint32_t live = 0, parts = 0;
+  for (auto& sc : p->scenes)
+    if (sc.used && sc.geo.active) {
+      live++;
+      parts += sc.count;
+    }
+  if (particles) *particles = parts;
+  if (scenes) *scenes = live;
+  if (free_node_blocks) *free_node_blocks = free_block_total(p);
+  return MPM_OK;
+}
+
+mpm_status mpm_pool_sync(mpm_pool* p) {
+  CHECK_POOL(p);
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return MPM_OK;
+}
+
+mpm_status mpm_pool_use_graphs(mpm_pool* p, int32_t enable) {
+  if (!p) return fail(MPM_ERR_INVALID_ARGUMENT, "pool is NULL");
+  p->use_graphs = enable != 0;
+  if (!p->use_graphs) drop_graphs(p);
+  return MPM_OK;
+}
+
+mpm_status mpm_pool_set_tuning(mpm_pool* p, int32_t knob, int32_t value) {
+  if (!p) return fail(MPM_ERR_INVALID_ARGUMENT, "pool is NULL");
+  switch (knob) {
+    case MPM_TUNE_G2P_WAVES:
+      if (value < 3 || value > 6) break;
+      p->g2p_waves = value;
+      drop_graphs(p);
+      return MPM_OK;
+    case MPM_TUNE_P2G_WAVES:
+      if (value < 4 || value > 6) break;
+      p->p2g_waves = value;
+      drop_graphs(p);
+      return MPM_OK;
+    default:
+      return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_tuning: unknown knob %d", knob);
+  }
+  return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_tuning: value %d out of range for knob %d", value, knob);
 }
 
 mpm_status mpm_scan_u32(mpm_ctx* c, const uint32_t* in, uint32_t* out, uint64_t n) {

@@ -56,6 +56,18 @@ class _Config(ctypes.Structure):
                 ("bound", ctypes.c_float), ("sort_every", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
 
 
+class _PoolConfig(ctypes.Structure):
+    _fields_ = [("max_particles", ctypes.c_uint32), ("max_scenes", ctypes.c_uint32),
+                ("max_node_blocks", ctypes.c_uint32), ("max_dim", ctypes.c_int32), ("sort_every", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32)]
+
+
+class _SceneDesc(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int32 * 3), ("ppc_axis", ctypes.c_int32), ("bound", ctypes.c_float),
+                ("gravity", ctypes.c_float * 3), ("num_materials", ctypes.c_uint32), ("materials", ctypes.c_void_p),
+                ("records", ctypes.c_void_p), ("count", ctypes.c_uint32)]
+
+
 _lib = None
 
 
@@ -90,12 +102,25 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "mpm_read_grid": (i32, [vp, vp]),
         "mpm_sync": (i32, [vp]),
         "mpm_set_profiling": (i32, [vp, i32]),
+        "mpm_set_tuning": (i32, [vp, i32, i32]),
         "mpm_phase_times": (i32, [vp, vp]),
         "mpm_scan_u32": (i32, [vp, vp, vp, u64]),
         "mpm_counting_sort": (i32, [vp, vp, u64, u32, vp, vp]),
         "mpm_material_row": (None, [i32, f64, f64, f64, f64, f64, f64, f64, f64, i32, vp]),
         "mpm_sample_shape": (ctypes.c_int64, [i32, vp, f64, f64, u32, vp, ctypes.c_int64]),
         "mpm_pack_particles": (None, [vp, ctypes.c_int64, vp, f64, vp, f64, u32, vp, f64, f64, u32, vp]),
+        "mpm_pool_bytes": (ctypes.c_size_t, [P(_PoolConfig)]),
+        "mpm_pool_create": (i32, [vp, P(_PoolConfig), P(vp)]),
+        "mpm_pool_destroy": (None, [vp]),
+        "mpm_pool_add_scene": (i32, [vp, P(_SceneDesc), P(u32)]),
+        "mpm_pool_remove_scene": (i32, [vp, u32]),
+        "mpm_pool_step": (i32, [vp, u32, f32, f32]),
+        "mpm_pool_scene_count": (i32, [vp, u32, P(u32)]),
+        "mpm_pool_read_scene": (i32, [vp, u32, vp, u32, P(u32)]),
+        "mpm_pool_info": (i32, [vp, P(u32), P(u32), P(u32)]),
+        "mpm_pool_sync": (i32, [vp]),
+        "mpm_pool_use_graphs": (i32, [vp, i32]),
+        "mpm_pool_set_tuning": (i32, [vp, i32, i32]),
         "mpm_status_string": (ctypes.c_char_p, [i32]),
         "mpm_last_error_message": (ctypes.c_char_p, []),
     }
@@ -200,7 +225,7 @@ class MPM:
     """MLS-MPM simulation on one MI355X (src/physics/mpm.js:82-680)."""
 
     def __init__(self, ctx: Context, maxParticles: int = 262144, grid: int = 64, size=1, origin=None, gravity=None,
-                 substeps: int = 8, timeScale: float = 1, sortEvery: int = 4, ppcAxis: int = 2, fuse: bool = True):
+                 substeps: int = 8, timeScale: float = 1, sortEvery: int = 4, ppcAxis: int = 2):
         self.ctx = ctx
         self.maxParticles = int(maxParticles or 262144)
         size = list(size) if isinstance(size, (list, tuple)) else [size or 1] * 3
@@ -220,9 +245,8 @@ class MPM:
         self.colliders: list[dict] = []
         self.destroyed = False
         self.numNodes = self.dims[0] * self.dims[1] * self.dims[2]
-        # fuse=False runs every G2P and P2G as separate kernels (the reference's dispatch shape)
         cfg = _Config((ctypes.c_int32 * 3)(*self.dims), self.maxParticles, self.ppcAxis, float(self.bound),
-                      self.sortEvery, 0 if fuse else 1)
+                      self.sortEvery, 0)
         h = ctypes.c_void_p()
         _check(load_library().mpm_sim_create(ctx._h, ctypes.byref(cfg), ctypes.byref(h)), "MPM")
         self._h = h
@@ -426,6 +450,13 @@ class MPM:
     def sync(self) -> None:
         _check(load_library().mpm_sync(self._h))
 
+    TUNE = {"g2p_waves": 2, "p2g_waves": 4}
+
+    def set_tuning(self, **knobs) -> None:
+        """Kernel variant knobs (result-neutral): g2p_waves, p2g_waves."""
+        for k, v in knobs.items():
+            _check(load_library().mpm_set_tuning(self._h, self.TUNE[k], int(v)), "MPM.set_tuning")
+
     def set_profiling(self, on: bool) -> None:
         _check(load_library().mpm_set_profiling(self._h, 1 if on else 0))
 
@@ -450,3 +481,89 @@ class MPM:
         self.destroyed = True
         load_library().mpm_sim_destroy(self._h)
         self._h = None
+
+
+def node_blocks(dims) -> int:
+    """4x4x4-node blocks a scene of these dims occupies in a pool."""
+    return ((dims[0] + 3) // 4) * ((dims[1] + 3) // 4) * ((dims[2] + 3) // 4)
+
+
+class ScenePool:
+    """Many independent MLS-MPM scenes advanced by the same launches (include/mpm.h, mpm_pool_*).
+
+    Each scene keeps its own grid, gravity, materials and particles; its results are those of a
+    separate MPM simulation of that scene (src/physics/mpm.js:582-604 per scene)."""
+
+    def __init__(self, ctx: Context, max_particles: int, max_scenes: int, max_node_blocks: int, max_dim: int,
+                 sort_every: int = 4):
+        self.ctx = ctx
+        self.cfg = _PoolConfig(int(max_particles), int(max_scenes), int(max_node_blocks), int(max_dim),
+                               int(sort_every), 0)
+        h = ctypes.c_void_p()
+        _check(load_library().mpm_pool_create(ctx._h, ctypes.byref(self.cfg), ctypes.byref(h)), "ScenePool")
+        self._h = h
+
+    @staticmethod
+    def bytes_for(max_particles: int, max_scenes: int, max_node_blocks: int, max_dim: int) -> int:
+        cfg = _PoolConfig(int(max_particles), int(max_scenes), int(max_node_blocks), int(max_dim), 4, 0)
+        return int(load_library().mpm_pool_bytes(ctypes.byref(cfg)))
+
+    def _alive(self, what: str) -> None:
+        if not self._h:
+            raise SIError(f"{what}: pool destroyed", "destroyed")
+
+    def add_scene(self, dims, materials: np.ndarray, records: np.ndarray, gravity, ppc_axis: int = 2,
+                  bound: float = 3.0) -> int:
+        """Adds a scene (records in the reference layout with scene-local material ids) and returns its id."""
+        self._alive("ScenePool.add_scene")
+        mats = np.ascontiguousarray(materials, np.float32).reshape(-1, 12)
+        rec = np.ascontiguousarray(records, np.float32).reshape(-1, 32)
+        d = _SceneDesc((ctypes.c_int32 * 3)(*[int(x) for x in dims]), int(ppc_axis), float(bound),
+                       (ctypes.c_float * 3)(*[float(x) for x in gravity]), len(mats),
+                       mats.ctypes.data if len(mats) else None, rec.ctypes.data if len(rec) else None, len(rec))
+        sid = ctypes.c_uint32()
+        _check(load_library().mpm_pool_add_scene(self._h, ctypes.byref(d), ctypes.byref(sid)), "ScenePool.add_scene")
+        return sid.value
+
+    def remove_scene(self, scene: int) -> None:
+        self._alive("ScenePool.remove_scene")
+        _check(load_library().mpm_pool_remove_scene(self._h, int(scene)), "ScenePool.remove_scene")
+
+    def step(self, substeps: int, floor_friction: float = 0.4, vmax: float = 0.9) -> None:
+        self._alive("ScenePool.step")
+        _check(load_library().mpm_pool_step(self._h, int(substeps), float(floor_friction), float(vmax)),
+               "ScenePool.step")
+
+    def scene_count(self, scene: int) -> int:
+        n = ctypes.c_uint32()
+        _check(load_library().mpm_pool_scene_count(self._h, int(scene), ctypes.byref(n)), "ScenePool.scene_count")
+        return n.value
+
+    def read_scene(self, scene: int) -> np.ndarray:
+        self._alive("ScenePool.read_scene")
+        n = self.scene_count(scene)
+        out = np.zeros((n, 32), np.float32)
+        got = ctypes.c_uint32()
+        _check(load_library().mpm_pool_read_scene(self._h, int(scene), _ptr(out), n, ctypes.byref(got)),
+               "ScenePool.read_scene")
+        return out
+
+    def info(self) -> dict:
+        a, b, c = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(load_library().mpm_pool_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"particles": a.value, "scenes": b.value, "free_node_blocks": c.value}
+
+    def sync(self) -> None:
+        _check(load_library().mpm_pool_sync(self._h))
+
+    def use_graphs(self, on: bool) -> None:
+        _check(load_library().mpm_pool_use_graphs(self._h, 1 if on else 0))
+
+    def set_tuning(self, **knobs) -> None:
+        for k, v in knobs.items():
+            _check(load_library().mpm_pool_set_tuning(self._h, MPM.TUNE[k], int(v)), "ScenePool.set_tuning")
+
+    def destroy(self) -> None:
+        if self._h:
+            load_library().mpm_pool_destroy(self._h)
+            self._h = None

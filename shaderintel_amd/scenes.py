@@ -61,22 +61,22 @@ def populate(sim: "_m.MPM", name: str) -> int:
     return sim.count
 
 
-def host_build(name: str):
-    """The same scene built on the host only (no device): returns (dims, material table (8, 12),
-    records (N, 32) float32, lattice gravity (3,) float32). Mirrors MPM.add step by step."""
-    sc = SCENES[name]
-    grid = sc["grid"]
-    dims = [grid] * 3
+def host_blocks(grid: int, blocks, ppc: int = 2, gravity=(0.0, -9.81, 0.0), substeps: int = SUBSTEPS, dims=None):
+    """A scene of material boxes built on the host only (no device), as MPM.add would build it on a
+    grid^3 lattice (or `dims`) with dx = 1/64 m. blocks: (material, lo cells, hi cells, velocity cells/s, noise
+    cells/s). Returns (dims, material table (8, 12), records (N, 32) float32, lattice gravity (3,)
+    float32)."""
+    import json
+    dims = list(dims) if dims is not None else [grid] * 3
     dx = (grid / 64.0) / grid
-    ppc = 2
-    dt = (1 / 60) * 1 / SUBSTEPS
+    dt = (1 / 60) * 1 / substeps
     materials, index, recs = [], {}, []
     count = 0
-    for mat, lo, hi, vel, noise in sc["blocks"]:
+    L = _m.load_library()
+    for mat, lo, hi, vel, noise in blocks:
         d = {**_m.MATERIALS[mat], "name": mat} if isinstance(mat, str) else dict(mat)
         if isinstance(d["kind"], str):
             d["kind"] = _m.MPM_KINDS[d["kind"]]
-        import json
         key = json.dumps(d)
         if key not in index:
             index[key] = len(materials)
@@ -85,7 +85,6 @@ def host_build(name: str):
         box = [[l * DX for l in lo], [h * DX for h in hi]]
         pts = _m.sample_shape({"box": box}, dx / ppc, 1.0, count + 1)
         rec = np.zeros((len(pts), 32), np.float32)
-        L = _m.load_library()
         L.mpm_pack_particles(_m._ptr(np.ascontiguousarray(pts)), len(pts), _m._ptr(np.zeros(3)), dx,
                              _m._ptr(np.asarray(dims, np.int32)), 3.0, mid,
                              _m._ptr(np.asarray([v * DX for v in vel], np.float64)), dt / dx, float(noise * DX),
@@ -95,5 +94,29 @@ def host_build(name: str):
     tbl = np.zeros((8, 12), np.float32)
     for k, m in enumerate(materials):
         tbl[k] = _m.material_row(m, ppc)
-    g = np.array([(x * dt * dt) / dx for x in (0, -9.81, 0)], np.float32)
-    return dims, tbl, np.concatenate(recs), g
+    g = np.array([(x * dt * dt) / dx for x in gravity], np.float32)
+    recs = np.concatenate(recs) if recs else np.zeros((0, 32), np.float32)
+    return dims, tbl, recs, g
+
+
+def host_build(name: str):
+    """The named scene built on the host only (see host_blocks)."""
+    sc = SCENES[name]
+    return host_blocks(sc["grid"], sc["blocks"])
+
+
+MATERIAL_CHOICES = ["water", "jelly", "snow", SAND30, "honey", "rubber"]
+
+
+def random_blocks(rng: np.random.Generator, grid: int, max_blocks: int = 3, max_edge: int = 24):
+    """1..max_blocks random material boxes inside a grid^3 lattice (cells, clear of the 3-cell
+    boundary), with random velocities and noise: the per-request scenes of the batched workload."""
+    blocks = []
+    for _ in range(int(rng.integers(1, max_blocks + 1))):
+        edge = rng.integers(4, min(max_edge, grid - 8) + 1, size=3)
+        lo = [int(rng.integers(3, grid - 3 - e + 1)) for e in edge]
+        hi = [l + int(e) for l, e in zip(lo, edge)]
+        mat = MATERIAL_CHOICES[int(rng.integers(len(MATERIAL_CHOICES)))]
+        vel = [float(v) for v in rng.uniform(-8, 8, size=3)]
+        blocks.append((mat, lo, hi, vel, float(rng.uniform(0, 0.2))))
+    return blocks

@@ -127,12 +127,12 @@ MATERIAL_SCENES = {
 
 
 @pytest.mark.parametrize("mat", list(MATERIAL_SCENES))
-@pytest.mark.parametrize("substeps,fuse", [(1, True), (10, True), (10, False)])
-def test_material_step_bit_exact(ctx, mat, substeps, fuse):
+@pytest.mark.parametrize("substeps,sort_every", [(1, 4), (10, 4), (10, 3)])
+def test_material_step_bit_exact(ctx, mat, substeps, sort_every):
     """Each material, randomized v/C/tau and a sheared F, 1 and 10 substeps: particles and grid
     identical to the oracle."""
-    _say(f"[gpu] material {mat}, {substeps} substeps (fuse={fuse}) vs oracle")
-    sim = MPM(ctx, grid=32, maxParticles=40000, substeps=8, fuse=fuse)
+    _say(f"[gpu] material {mat}, {substeps} substeps (sort every {sort_every}) vs oracle")
+    sim = MPM(ctx, grid=32, maxParticles=40000, substeps=8, sortEvery=sort_every)
     sim.add({"box": [[0.2, 0.15, 0.2], [0.6, 0.55, 0.6]]}, material=MATERIAL_SCENES[mat], velocity=[0.3, -0.2, 0.1],
             jitter=1.0, noise=0.2)
     raw = sim.read_particles()
@@ -145,7 +145,7 @@ def test_material_step_bit_exact(ctx, mat, substeps, fuse):
         F = np.eye(3)[None] + (rng.random((n, 3, 3)) - 0.5) * 0.3
         raw[:, 20:23], raw[:, 24:27], raw[:, 28:31] = F[:, :, 0], F[:, :, 1], F[:, :, 2]
     sim.write_particles(raw)
-    sc = _mirror(sim, raw)
+    sc = _mirror(sim, raw, sort_every)
     g = sim.lattice_gravity()
     from shaderintel_amd.mpm import load_library, _ptr
     assert load_library().mpm_step(sim._h, substeps, _ptr(g), 0.4, 0.9) == 0

@@ -1,0 +1,150 @@
+"""Scene pools (mpm_pool_*): many scenes advanced by shared launches give, scene by scene, exactly
+the particles a separate simulation of that scene gives (checked against the oracle, bit for bit and
+in the same order), under scene removal and re-admission, with and without captured graphs."""
+import numpy as np
+import pytest
+
+import oracle_mpm as O
+from helpers import bits
+from shaderintel_amd import scenes
+from shaderintel_amd.mpm import SIError, ScenePool, node_blocks
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg: str) -> None:
+    print(msg, flush=True)
+
+
+def _scene(seed: int, grid: int, dims=None, gravity=(0.0, -9.81, 0.0)):
+    rng = np.random.default_rng(seed)
+    blocks = scenes.random_blocks(rng, min(dims) if dims else grid, max_blocks=3, max_edge=14)
+    return scenes.host_blocks(grid, blocks, gravity=gravity, dims=dims)
+
+
+def _oracle(dims, tbl, rec, sort_every=4):
+    sc = O.Scene(dims, max(len(rec), 1), 2, sort_every)
+    sc.set_materials(tbl)
+    sc.write(rec)
+    sc.set_count(len(rec))
+    return sc
+
+
+def _same(got, want, what):
+    assert got.shape == want.shape, f"{what}: shape {got.shape} vs {want.shape}"
+    bad = np.argwhere(bits(got) != bits(want))
+    assert len(bad) == 0, f"{what}: {len(bad)} words differ, first at {tuple(bad[0])}"
+
+
+SPECS = [
+    dict(seed=11, grid=32),
+    dict(seed=12, grid=48, gravity=(0.0, -9.81, 3.0)),
+    dict(seed=13, grid=40, dims=(40, 32, 48)),
+    dict(seed=14, grid=64, gravity=(1.0, -4.0, 0.0)),
+    dict(seed=15, grid=24),
+]
+
+
+def _build_specs():
+    out = []
+    for sp in SPECS:
+        dims, tbl, rec, g = _scene(sp["seed"], sp["grid"], sp.get("dims"), sp.get("gravity", (0.0, -9.81, 0.0)))
+        out.append((dims, tbl, rec, g))
+    return out
+
+
+def test_pool_scenes_match_separate_simulations(ctx):
+    """Five scenes of different sizes, shapes, gravities and materials in one pool, 10 substeps:
+    each equals the oracle run of that scene alone."""
+    _say("[gpu] pool of 5 scenes vs oracle per scene, 10 substeps")
+    built = _build_specs()
+    total = sum(len(b[2]) for b in built)
+    pool = ScenePool(ctx, total + 100, 8, sum(node_blocks(b[0]) for b in built) + 64, 64)
+    ids = [pool.add_scene(d, t, r, g) for d, t, r, g in built]
+    assert pool.info()["particles"] == total
+    pool.step(10)
+    for sid, (d, t, r, g) in zip(ids, built):
+        sc = _oracle(d, t, r)
+        sc.step(10, g)
+        _same(pool.read_scene(sid), sc.read(len(r)), f"scene {sid} {d}")
+    pool.destroy()
+
+
+def test_pool_remove_and_readmit(ctx):
+    """Remove a scene between steps, admit a new one into the freed slot and blocks: the survivors
+    continue exactly, the newcomer starts exactly."""
+    _say("[gpu] pool remove / re-admit")
+    built = _build_specs()[:3]
+    total = sum(len(b[2]) for b in built)
+    nb = sum(node_blocks(b[0]) for b in built)
+    pool = ScenePool(ctx, total + 20000, 4, nb + 512, 64)
+    ids = [pool.add_scene(d, t, r, g) for d, t, r, g in built]
+    oracles = [_oracle(d, t, r) for d, t, r, _ in built]
+    pool.step(6)
+    for sc, (_, _, _, g) in zip(oracles, built):
+        sc.step(6, g)
+    pool.remove_scene(ids[0])
+    with pytest.raises(SIError):
+        pool.read_scene(ids[0])
+    d2, t2, r2, g2 = _scene(99, 32)
+    assert len(r2) <= 20000 + len(built[0][2])
+    new_id = pool.add_scene(d2, t2, r2, g2)
+    new_oracle = _oracle(d2, t2, r2)
+    new_oracle.step_count = 6  # the pool's sort phase (results do not depend on it; particle order does)
+    pool.step(7)
+    new_oracle.step(7, g2)
+    for k in (1, 2):
+        oracles[k].step(7, built[k][3])
+        _same(pool.read_scene(ids[k]), oracles[k].read(len(built[k][2])), f"survivor {ids[k]}")
+    _same(pool.read_scene(new_id), new_oracle.read(len(r2)), "newcomer")
+    info = pool.info()
+    assert info["scenes"] == 3 and info["particles"] == len(r2) + len(built[1][2]) + len(built[2][2])
+    pool.destroy()
+
+
+def test_pool_graphs_match_eager(ctx):
+    """Sort periods replayed from captured graphs give the eager launches' results (several replays,
+    both buffer parities, with eager reads in between)."""
+    _say("[gpu] pool graphs vs eager, 24 substeps")
+    built = _build_specs()
+    total = sum(len(b[2]) for b in built)
+    nb = sum(node_blocks(b[0]) for b in built)
+    pools = []
+    for graphs in (False, True):
+        pool = ScenePool(ctx, total, 8, nb, 64)
+        pool.use_graphs(graphs)
+        ids = [pool.add_scene(d, t, r, g) for d, t, r, g in built]
+        pools.append((pool, ids))
+    for rep in range(3):
+        outs = []
+        for pool, ids in pools:
+            pool.step(8)
+            outs.append([pool.read_scene(i) for i in ids])
+        for a, b in zip(*outs):
+            _same(b, a, f"graph replay {rep}")
+    for pool, _ in pools:
+        pool.destroy()
+
+
+def test_pool_admission_limits(ctx):
+    """A scene that does not fit (particles, node blocks or slots) is refused with out-of-memory and
+    the pool is unchanged."""
+    _say("[gpu] pool admission limits")
+    d, t, r, g = _scene(21, 32)
+    pool = ScenePool(ctx, len(r) + 10, 2, node_blocks(d) + 1, 32)
+    a = pool.add_scene(d, t, r, g)
+    with pytest.raises(SIError) as e:
+        pool.add_scene(d, t, r, g)  # particles
+    assert e.value.code == "out-of-memory"
+    with pytest.raises(SIError) as e:
+        pool.add_scene(d, t, r[:5], g)  # node blocks
+    assert e.value.code == "out-of-memory"
+    with pytest.raises(SIError) as e:
+        pool.add_scene((64, 64, 64), t, r[:5], g)  # larger than max_dim
+    assert e.value.code == "invalid-argument"
+    assert pool.info() == {"particles": len(r), "scenes": 1, "free_node_blocks": 1}
+    pool.step(4)
+    sc = _oracle(d, t, r)
+    sc.step(4, g)
+    _same(pool.read_scene(a), sc.read(len(r)), "after refusals")
+    pool.destroy()
