@@ -21,9 +21,22 @@ __device__ __forceinline__ uint32_t node_addr(int gx, int gy, int gz, const Scen
   return (b << 6) | (uint32_t)((gx & 3) | ((gy & 3) << 2) | ((gz & 3) << 4));
 }
 
-__device__ __forceinline__ const SceneGeo& scene_of_row(const Pool& pool, uint32_t row) {
-  return pool.scenes[__float_as_uint(pool.mats[12 * row + 9])];
+static_assert(kMatRows == 8, "row >> 3 is the scene of a material row");
+
+// Wave-uniform read of a scene's geometry through the constant address space, so it is fetched with
+// scalar loads into scalar registers (the table is never written by a kernel).
+typedef const __attribute__((address_space(4))) uint32_t* ConstWords;
+__device__ __forceinline__ SceneGeo uniform_geo(const Pool& pool, uint32_t scene) {
+  static_assert(sizeof(SceneGeo) % 4 == 0, "SceneGeo is whole words");
+  const ConstWords src = (ConstWords)(pool.scenes + scene);
+  SceneGeo g;
+  uint32_t* dst = reinterpret_cast<uint32_t*>(&g);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(SceneGeo) / 4); k++) dst[k] = src[k];
+  return g;
 }
+
+__device__ __forceinline__ const SceneGeo& scene_of_row(const Pool& pool, uint32_t row) { return pool.scenes[row >> 3]; }
 
 __device__ __forceinline__ void load_record(const float4* __restrict__ P, uint32_t cap, uint32_t i, float4 r[8]) {
 #pragma unroll
@@ -142,11 +155,11 @@ __device__ __forceinline__ float4 grid_node(int4 q, int ix, int iy, int iz, cons
 
 // G2P for particle i (mpm.js:479-559): gather velocity and APIC C, advect, update F, plasticity,
 // stress. Writes the new record to the 8 streams and returns it in r.
-__device__ __forceinline__ void g2p_particle(const Pool& pool, float4* __restrict__ P, uint32_t cap, uint32_t i,
+__device__ __forceinline__ void g2p_particle(const float* __restrict__ mats, const float4* __restrict__ V,
+                                             const SceneGeo& geo, float4* __restrict__ P, uint32_t cap, uint32_t i,
                                              float vmax, float4 r[8]) {
   const float4 x4 = P[i];
-  const float* m0 = pool.mats + 12 * __float_as_uint(x4.w);
-  const SceneGeo& geo = pool.scenes[__float_as_uint(m0[9])];
+  const float* m0 = mats + 12 * __float_as_uint(x4.w);
   const float bound = geo.bound;
   const float vw0 = P[(size_t)cap + i].w;
   const f3 xp = mk(x4.x, x4.y, x4.z);
@@ -167,7 +180,7 @@ __device__ __forceinline__ void g2p_particle(const Pool& pool, float4* __restric
         const float wy = b == 0 ? w[0].y : (b == 1 ? w[1].y : w[2].y);
         const float wz = c == 0 ? w[0].z : (c == 1 ? w[1].z : w[2].z);
         const float wt = (wx * wy) * wz;
-        const float4 g4 = pool.V[node_addr(bx + a, by + b, bz + c, geo)];
+        const float4 g4 = V[node_addr(bx + a, by + b, bz + c, geo)];
         const f3 gv = mk(g4.x, g4.y, g4.z);
         v = v + wt * gv;
         B = B + outer(wt * gv, dpos);
@@ -247,7 +260,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
   float4 r[8];
-  g2p_particle(pool, P, pool.cap, i, vmax, r);
+  g2p_particle(pool.mats, pool.V, pool.scenes[__float_as_uint(P[i].w) >> 3], P, pool.cap, i, vmax, r);
 }
 
 __global__ void k_records_in(const float4* __restrict__ aos, float4* __restrict__ P, uint32_t cap, uint32_t offset,
@@ -286,6 +299,14 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t value, uint32_t n)
   if (i < n) p[i] = value;
 }
 
+__global__ void k_block_info(uint2* __restrict__ info, uint32_t base, uint32_t n, uint32_t scene, uint32_t nbx,
+                             uint32_t nby) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t bx = i % nbx, by = (i / nbx) % nby, bz = i / (nbx * nby);
+  info[base + i] = make_uint2(scene, bx | (by << 10) | (bz << 20));
+}
+
 __global__ void k_begin_p2g(uint32_t* ctr) {
   ctr[kCtrEpoch] += 1u;
   ctr[kCtrD] = 0u;
@@ -311,7 +332,7 @@ __global__ void k_keys(Pool pool, const float4* __restrict__ P, uint32_t n, uint
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 x = P[i];
-  const uint32_t scene = __float_as_uint(pool.mats[12 * __float_as_uint(x.w) + 9]);
+  const uint32_t scene = __float_as_uint(x.w) >> 3;
   if (!pool.scenes[scene].active) {
     keys[i] = 0xffffffffu;
     vals[i] = i;
@@ -327,7 +348,7 @@ __global__ void k_live_keys(Pool pool, const float4* __restrict__ P, uint32_t n,
                             uint32_t* __restrict__ vals) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint32_t scene = __float_as_uint(pool.mats[12 * __float_as_uint(P[i].w) + 9]);
+  const uint32_t scene = __float_as_uint(P[i].w) >> 3;
   keys[i] = pool.scenes[scene].active ? 0u : 1u;
   vals[i] = i;
 }
@@ -420,9 +441,9 @@ __global__ __launch_bounds__(kThreads) void k_bin_dilate(Pool pool) {
   const uint32_t epoch = pool.ctr[kCtrBinEpoch];
   const uint32_t mshift = pool.key_bits - 6;
   for (uint32_t t = blockIdx.x * kThreads + threadIdx.x; t < cnt * 64; t += gridDim.x * kThreads) {
-    const uint32_t k = t >> 6, d = t & 63;
-    const uint32_t wb = pool.wl_block[k];
-    const SceneGeo& geo = pool.scenes[wb >> mshift];
+    const uint32_t k = t >> 6, d = t & 63;  // one item per wave
+    const uint32_t wb = __builtin_amdgcn_readfirstlane(pool.wl_block[k]);
+    const SceneGeo geo = uniform_geo(pool, wb >> mshift);
     const uint32_t mb = wb & ((1u << mshift) - 1u);
     const int gx = (int)compact3(mb) - 1 + (int)(d & 3), gy = (int)compact3(mb >> 1) - 1 + (int)((d >> 2) & 3),
               gz = (int)compact3(mb >> 2) - 1 + (int)(d >> 4);
@@ -570,33 +591,46 @@ __global__ void k_write_words(Words64 src, uint32_t* dst, uint32_t n) {
 }
 
 // Sum of the P2G result at scene-local node (n0, n1, n2) of node block (g0, g1, g2): the tiles of
-// particle blocks G-2..G+1 per axis that cover the node. All lanes of the wave share the node
-// block, so the map and header reads are uniform.
-__device__ __forceinline__ int4 pull_tiles(const Pool& pool, const SceneGeo& geo, int g0, int g1, int g2, int n0,
+// particle blocks G-2..G+1 per axis that cover the node. Lane L looks up candidate particle block L
+// (its map entry and tile header: one load each for the whole wave); the candidates whose tile box
+// overlaps the node block are then visited with wave-uniform tile indices, up to four tile loads in
+// flight at a time. Integer sums, so the visiting order does not matter.
+__device__ __forceinline__ int4 pull_tiles(const Pool& pool, uint32_t scene, int g0, int g1, int g2, int lane, int n0,
                                            int n1, int n2, int4 acc) {
-  const int pb = 1 << geo.mbits;  // particle blocks per axis covered by the scene's Morton code
-  for (int dz = -2; dz <= 1; dz++) {
-    const int bz = g2 + dz;
-    if (bz < 0 || bz >= pb) continue;
-    for (int dy = -2; dy <= 1; dy++) {
-      const int by = g1 + dy;
-      if (by < 0 || by >= pb) continue;
-      for (int dx = -2; dx <= 1; dx++) {
-        const int bx = g0 + dx;
-        if (bx < 0 || bx >= pb) continue;
-        const uint32_t k = pool.map[geo.map_base + morton3((uint32_t)bx, (uint32_t)by, (uint32_t)bz)];
-        if (k == 0xffffffffu || k >= pool.tile_slots) continue;
-        const int4 h = pool.tile_hdr[k];
-        const int ex = h.w & 255, ey = (h.w >> 8) & 255, ez = (h.w >> 16) & 255;
-        const int lx = n0 - h.x, ly = n1 - h.y, lz = n2 - h.z;
-        if ((unsigned)lx < (unsigned)ex && (unsigned)ly < (unsigned)ey && (unsigned)lz < (unsigned)ez) {
-          const int4 q = pool.tiles[(size_t)k * kTileNodes + (lz * ey + ly) * ex + lx];
-          acc.x = (int)((uint32_t)acc.x + (uint32_t)q.x);
-          acc.y = (int)((uint32_t)acc.y + (uint32_t)q.y);
-          acc.z = (int)((uint32_t)acc.z + (uint32_t)q.z);
-          acc.w = (int)((uint32_t)acc.w + (uint32_t)q.w);
-        }
+  const int pb = 1 << (pool.key_bits / 3 - 2);  // particle blocks per axis a scene's map addresses
+  const int cx = g0 - 2 + (lane & 3), cy = g1 - 2 + ((lane >> 2) & 3), cz = g2 - 2 + (lane >> 4);
+  uint32_t k = 0xffffffffu;
+  if ((unsigned)cx < (unsigned)pb && (unsigned)cy < (unsigned)pb && (unsigned)cz < (unsigned)pb)
+    k = pool.map[(scene << (pool.key_bits - 6)) + morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz)];
+  int4 h = make_int4(0, 0, 0, 0);
+  if (k < pool.tile_slots) h = pool.tile_hdr[k];
+  const int ex = h.w & 255, ey = (h.w >> 8) & 255, ez = (h.w >> 16) & 255;
+  const bool hit = ex > 0 && h.x < 4 * g0 + 4 && h.x + ex > 4 * g0 && h.y < 4 * g1 + 4 && h.y + ey > 4 * g1 &&
+                   h.z < 4 * g2 + 4 && h.z + ez > 4 * g2;
+  uint64_t mask = __builtin_amdgcn_ballot_w64(hit);
+  while (mask) {
+    int4 q[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      q[u] = make_int4(0, 0, 0, 0);
+      if (mask) {
+        const int c = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const uint32_t kc = (uint32_t)__builtin_amdgcn_readlane((int)k, c);
+        const int hx = __builtin_amdgcn_readlane(h.x, c), hy = __builtin_amdgcn_readlane(h.y, c),
+                  hz = __builtin_amdgcn_readlane(h.z, c), hw = __builtin_amdgcn_readlane(h.w, c);
+        const int tx = hw & 255, ty = (hw >> 8) & 255, tz = (hw >> 16) & 255;
+        const int lx = n0 - hx, ly = n1 - hy, lz = n2 - hz;
+        if ((unsigned)lx < (unsigned)tx && (unsigned)ly < (unsigned)ty && (unsigned)lz < (unsigned)tz)
+          q[u] = pool.tiles[(size_t)kc * kTileNodes + (lz * ty + ly) * tx + lx];
       }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      acc.x = (int)((uint32_t)acc.x + (uint32_t)q[u].x);
+      acc.y = (int)((uint32_t)acc.y + (uint32_t)q[u].y);
+      acc.z = (int)((uint32_t)acc.z + (uint32_t)q[u].z);
+      acc.w = (int)((uint32_t)acc.w + (uint32_t)q[u].w);
     }
   }
   return acc;
@@ -611,17 +645,16 @@ __global__ __launch_bounds__(kThreads) void k_grid_pull(Pool pool, int use_s) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (uint32_t j = blockIdx.x * 4 + w; j < ns + nd; j += gridDim.x * 4) {
     const uint32_t b = __builtin_amdgcn_readfirstlane(j < ns ? pool.slist[j] : pool.dlist[j - ns]);
-    const uint32_t scene = __builtin_amdgcn_readfirstlane(pool.block_scene[b]);
-    const SceneGeo& geo = pool.scenes[scene];
-    const uint32_t lb = b - geo.block_base;
-    const int g0 = (int)(lb % (uint32_t)geo.nbx), g1 = (int)((lb / (uint32_t)geo.nbx) % (uint32_t)geo.nby),
-              g2 = (int)(lb / ((uint32_t)geo.nbx * (uint32_t)geo.nby));
+    const uint2 bi = pool.block_info[b];
+    const uint32_t scene = __builtin_amdgcn_readfirstlane(bi.x), bc = __builtin_amdgcn_readfirstlane(bi.y);
+    const SceneGeo geo = uniform_geo(pool, scene);
+    const int g0 = (int)(bc & 1023u), g1 = (int)((bc >> 10) & 1023u), g2 = (int)(bc >> 20);
     const int n0 = 4 * g0 + (lane & 3), n1 = 4 * g1 + ((lane >> 2) & 3), n2 = 4 * g2 + (lane >> 4);
     const uint32_t addr = (b << 6) | (uint32_t)lane;
     int4 acc = make_int4(0, 0, 0, 0);
     const bool written = pool.stamp[b] == epoch;
     if (written) acc = pool.A[addr];
-    if (j < ns) acc = pull_tiles(pool, geo, g0, g1, g2, n0, n1, n2, acc);
+    if (j < ns) acc = pull_tiles(pool, scene, g0, g1, g2, lane, n0, n1, n2, acc);
     if (VELOCITY) {
       pool.V[addr] = grid_node(acc, n0, n1, n2, geo, pool.cols + 12 * kColRows * scene);
       if (written) pool.A[addr] = make_int4(0, 0, 0, 0);
@@ -688,6 +721,12 @@ hipError_t launch_rescale(float4* P, uint32_t cap, uint32_t offset, uint32_t n, 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t s) {
   if (!n) return hipSuccess;
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, value, n);
+  return hipGetLastError();
+}
+hipError_t launch_block_info(uint2* info, uint32_t base, uint32_t n, uint32_t scene, uint32_t nbx, uint32_t nby,
+                             hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_block_info, dim3(blocks_for(n, 256)), dim3(256), 0, s, info, base, n, scene, nbx, nby);
   return hipGetLastError();
 }
 hipError_t launch_begin_p2g(const Pool& pool, hipStream_t s) {

@@ -7,8 +7,8 @@
 //  - Particles: one pool of structure-of-float4 streams. Stream k (0..7) holds vec4 k of every
 //    record (x|row, v|J, c0|t00, c1|t11, c2|t22, f0|t01, f1|t02, f2|t12), so a wavefront reading one
 //    stream touches 64 consecutive 16-byte elements. Positions are in the scene's own lattice
-//    coordinates; x.w holds the particle's global material row (scene * 8 + material), and row
-//    word 9 holds the scene id, so a particle finds its scene through its material.
+//    coordinates; x.w holds the particle's global material row (scene * 8 + material), so
+//    row >> 3 is the particle's scene.
 //  - Grid: a pool of 4x4x4-node blocks. Scene s owns blocks [block_base, block_base + NBX*NBY*NBZ),
 //    block-major inside (block = bx + NBX*(by + NBY*bz), node = lx + 4*(ly + 4*lz)): 1 KiB of int4
 //    (A, the i32 fixed-point grid written only by the global-atomic paths) or of float4 (V,
@@ -37,7 +37,7 @@ namespace mpm {
 constexpr int kDrift = 3;                                      // cells a base may drift outside its block
 constexpr int kTileSpan = 4 + 2 + 2 * kDrift;                  // nodes per axis a work item can touch (12)
 constexpr int kTileNodes = kTileSpan * kTileSpan * kTileSpan;  // 1728 nodes = 27 KiB of int4
-constexpr int kMatRows = 8;                                    // material rows per scene
+constexpr int kMatRows = 8;                                    // material rows per scene (row >> 3 = scene)
 constexpr int kColRows = 16;                                   // collider rows per scene
 
 enum Counter : int {
@@ -76,7 +76,7 @@ struct Pool {
   uint32_t tile_slots;
   int4* A;
   float4* V;
-  const uint32_t* block_scene;  // node block -> scene
+  const uint2* block_info; // node block -> (scene, bx | by << 10 | bz << 20) in the scene's blocks
   uint32_t* stamp;         // D stamps (P2G epoch)
   uint32_t* dlist;
   uint32_t* sstamp;        // S stamps (binning epoch)
@@ -99,6 +99,9 @@ hipError_t launch_records_out(const float4* P, uint32_t cap, uint32_t offset, ui
                               hipStream_t s);
 hipError_t launch_rescale(float4* P, uint32_t cap, uint32_t offset, uint32_t n, float k, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t s);
+// block_info of the n = nbx*nby*nbz node blocks of a scene starting at block `base`.
+hipError_t launch_block_info(uint2* info, uint32_t base, uint32_t n, uint32_t scene, uint32_t nbx, uint32_t nby,
+                             hipStream_t s);
 // ctr[kCtrEpoch]++, |D| = 0, work = 0 (start of a P2G)
 hipError_t launch_begin_p2g(const Pool& pool, hipStream_t s);
 // ctr[kCtrBinEpoch]++, |S| = 0, items = 0 (start of a binning)

@@ -104,7 +104,8 @@ struct mpm_pool {
   int cur = 0;
   int4* A = nullptr;
   float4* V = nullptr;
-  uint32_t *block_scene = nullptr, *stamp = nullptr, *dlist = nullptr, *sstamp = nullptr, *slist = nullptr;
+  uint2* block_info = nullptr;
+  uint32_t *stamp = nullptr, *dlist = nullptr, *sstamp = nullptr, *slist = nullptr;
   uint32_t *map = nullptr, *wl_start = nullptr, *wl_block = nullptr;
   int4 *tiles = nullptr, *tile_hdr = nullptr;
   float *mats = nullptr, *cols = nullptr;
@@ -159,7 +160,7 @@ Pool view(const mpm_pool* p) {
   v.tile_slots = p->tile_slots;
   v.A = p->A;
   v.V = p->V;
-  v.block_scene = p->block_scene;
+  v.block_info = p->block_info;
   v.stamp = p->stamp;
   v.dlist = p->dlist;
   v.sstamp = p->sstamp;
@@ -190,7 +191,7 @@ void free_pool(mpm_pool* p) {
   if (p->stream) hipStreamSynchronize(p->stream);
   drop_graphs(p);
   for (hipEvent_t e : p->ev_pool) hipEventDestroy(e);
-  void* ptrs[] = {p->P[0],   p->P[1],     p->A,        p->V,        p->block_scene, p->stamp,   p->dlist,
+  void* ptrs[] = {p->P[0],   p->P[1],     p->A,        p->V,        p->block_info, p->stamp,   p->dlist,
                   p->sstamp, p->slist,    p->map,      p->wl_start, p->wl_block,    p->tiles,   p->tile_hdr,
                   p->mats,   p->cols,     p->scenes_d, p->ctr,      p->keys[0],     p->keys[1], p->vals[0],
                   p->vals[1], p->order,   p->flags,    p->sort_scratch, p->staging, p->export_buf};
@@ -216,7 +217,7 @@ size_t pool_bytes(uint32_t cap, uint32_t max_scenes, uint32_t max_blocks, int ma
   size_t b = 0;
   b += 2 * (size_t)cap * 128;                            // P[2]
   b += (size_t)max_blocks * 64 * (16 + 16);              // A, V
-  b += (size_t)max_blocks * 4 * 5;                       // block_scene, stamp, dlist, sstamp, slist
+  b += (size_t)max_blocks * 4 * 6;                       // block_info, stamp, dlist, sstamp, slist
   b += (size_t)mps * max_scenes * 4;                     // map
   b += (items + 1) * 4 * 2;                              // wl_start, wl_block
   b += slots * (sizeof(int4) * kTileNodes + 16);         // tiles, headers
@@ -235,12 +236,11 @@ mpm_status upload_scene(mpm_pool* p, uint32_t s) {
   return MPM_OK;
 }
 
-// Material rows of scene s (n <= 8 rows of 12 floats); word 9 of every row names the scene.
+// Material rows of scene s (n <= 8 rows of 12 floats) at rows [8s, 8s + 8).
 mpm_status upload_materials(mpm_pool* p, uint32_t s, const float* tbl, uint32_t n) {
   float rows[kMatRows * 12];
   std::memset(rows, 0, sizeof rows);
   if (n) std::memcpy(rows, tbl, 48 * (size_t)n);
-  for (int k = 0; k < kMatRows; k++) std::memcpy(&rows[12 * k + 9], &s, 4);
   uint32_t* dst = reinterpret_cast<uint32_t*>(p->mats + (size_t)s * kMatRows * 12);
   HIP_TRY(launch_write_words(dst, reinterpret_cast<const uint32_t*>(rows), 48, p->stream));
   HIP_TRY(launch_write_words(dst + 48, reinterpret_cast<const uint32_t*>(rows) + 48, 48, p->stream));
@@ -565,7 +565,7 @@ mpm_status create_pool(mpm_ctx* c, uint32_t cap, uint32_t max_scenes, uint32_t m
   if (e == hipSuccess) e = dmalloc(&p->P[1], (size_t)cap * 8);
   if (e == hipSuccess) e = dmalloc(&p->A, nodes);
   if (e == hipSuccess) e = dmalloc(&p->V, nodes);
-  if (e == hipSuccess) e = dmalloc(&p->block_scene, max_blocks);
+  if (e == hipSuccess) e = dmalloc(&p->block_info, max_blocks);
   if (e == hipSuccess) e = dmalloc(&p->stamp, max_blocks);
   if (e == hipSuccess) e = dmalloc(&p->dlist, max_blocks);
   if (e == hipSuccess) e = dmalloc(&p->sstamp, max_blocks);
@@ -593,7 +593,7 @@ mpm_status create_pool(mpm_ctx* c, uint32_t cap, uint32_t max_scenes, uint32_t m
   }
   if (e == hipSuccess) e = hipMemsetAsync(p->A, 0, nodes * sizeof(int4), p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->V, 0, nodes * sizeof(float4), p->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(p->block_scene, 0xff, (size_t)max_blocks * 4, p->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(p->block_info, 0xff, (size_t)max_blocks * 8, p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->stamp, 0xff, (size_t)max_blocks * 4, p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->sstamp, 0xff, (size_t)max_blocks * 4, p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->ctr, 0, kCounters * 4, p->stream);
@@ -649,7 +649,7 @@ mpm_status add_scene(mpm_pool* p, const int32_t dims[3], int32_t ppc_axis, float
   ST_TRY(upload_scene(p, s));
   ST_TRY(upload_materials(p, s, mats, nmat));
   ST_TRY(upload_colliders(p, s, nullptr, 0));
-  HIP_TRY(launch_fill_u32(p->block_scene + geo.block_base, s, sc.nblocks, p->stream));
+  HIP_TRY(launch_block_info(p->block_info, geo.block_base, sc.nblocks, s, (uint32_t)geo.nbx, (uint32_t)geo.nby, p->stream));
   HIP_TRY(launch_fill_u32(p->stamp + geo.block_base, 0xffffffffu, sc.nblocks, p->stream));
   HIP_TRY(launch_fill_u32(p->sstamp + geo.block_base, 0xffffffffu, sc.nblocks, p->stream));
   *id = s;
