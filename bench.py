@@ -57,6 +57,23 @@ def cpu_baseline(scene: str, substeps: int) -> dict:
                       "CPU restatement of the reference algorithm, not the reference's WebGPU path"}
 
 
+def copy_calibration(nbytes: int) -> None:
+    """Three device-to-device copies of nbytes (the HIP runtime's copy kernel): a dispatch whose HBM
+    traffic is known (nbytes read, nbytes written), to scale byte counters by."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: 3 copies of {nbytes} bytes (each reads and writes {nbytes} bytes)")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,10 +83,13 @@ def main() -> int:
     ap.add_argument("--cpu-substeps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", action="store_true", help="time each tuning knob setting and exit")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the timed region, copy a MIB-MiB device buffer 3 times (a kernel of known traffic "
+                         "for scaling HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
-              "described in DESIGN.md and not built", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); a multi-GPU run (slabs "
+              "along z with a ghost-node exchange per substep) is designed but not built", file=sys.stderr)
         return 2
 
     from shaderintel_amd import scenes
@@ -126,6 +146,14 @@ def main() -> int:
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom.replace(" ", "_"),
                 "algo_bytes_per_launch": ALGO_BYTES[dom] * n,
                 "avg_launch_ms": round(per_launch[dom], 4)}
+    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3 counter passes
+    # (FETCH_SIZE / WRITE_SIZE scaled by a copy kernel of known size) on the same workload
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.scene}.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath))
+        if t.get("kernel") == "k_" + dom.split()[-1]:
+            roofline["traffic"] = round(t["traffic_bytes"])
+            roofline["traffic_source"] = t["source"]
     substep_bytes = 384 * n
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "particle-substeps/s", "n_gpus": 1,
@@ -142,6 +170,8 @@ def main() -> int:
         "finite": finite,
     }
     sim.destroy()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration << 20)
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(args.scene, args.cpu_substeps)

@@ -12,8 +12,8 @@
 // WGSL clamp(e,lo,hi) = min(max(e,lo),hi) with NaN-discarding min/max, WGSL round = rint (half to
 // even). inverseSqrt(x) is read as 1/sqrt(x); log/exp are the self-contained routines below.
 //
-// Parity status of each function is listed in DESIGN.md's oracle table; every function here has a
-// pinning test under tests/ (closed forms, invariants, brute force or numpy references).
+// Every function here has a pinning test under tests/ (closed forms, invariants, brute force or
+// numpy references); tests/test_oracle_*.py name the function each test pins.
 //
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off -fopenmp).
 

@@ -1,0 +1,259 @@
+"""Pins the oracle's host-side math (seeding, material table), its sort / scan primitives and its
+whole substep against known answers, brute force, physical identities and the reference's own
+tests (physics.test.js:43-60, lifecycle.test.js:274-306, verify/index.js:175-193, core.test.js:74).
+CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_mpm as O
+
+FP = float(1 << 22)
+
+
+# ---- seeding (mpm.js:207-232, 683-714) ------------------------------------------------------------
+
+def _xorshift32(seed, n):
+    """Marsaglia's xorshift32 with the (13, 17, 5) triple ("Xorshift RNGs", J. Stat. Softw. 2003)."""
+    x, out = seed & 0xFFFFFFFF, []
+    for _ in range(n):
+        x ^= (x << 13) & 0xFFFFFFFF
+        x ^= x >> 17
+        x ^= (x << 5) & 0xFFFFFFFF
+        out.append(x)
+    return out
+
+
+def test_xorshift_known_answer():
+    assert _xorshift32(1, 3) == [270369, 67634689, 2647435461]
+
+
+@pytest.mark.parametrize("seed", [1, 7, 123456789])
+def test_sample_shape_uses_xorshift_stream(seed):
+    """A one-point box at the origin with jitter 1 puts the point at (r1, r2, r3) * spacing, where r are
+    the first three xorshift32 outputs / 2^32 (x, y, z order)."""
+    sp = 0.25
+    pts = O.sample_shape(0, [0, 0, 0, sp, sp, sp], sp, 1.0, seed)
+    assert pts.shape == (1, 3)
+    r = np.array(_xorshift32(seed, 3), np.float64) / 2.0 ** 32
+    np.testing.assert_allclose(pts[0], (r * sp).astype(np.float32), rtol=0, atol=0)
+
+
+def test_sample_shape_counts():
+    """Box: floor(extent / spacing) points per axis (dam break: 16 x 64 x 64 = 65,536); sphere:
+    exactly the jittered lattice points inside (brute force)."""
+    dx = 1 / 64
+    box = O.sample_shape(0, [3 * dx, 3 * dx, 3 * dx, 11 * dx, 35 * dx, 35 * dx], dx / 2, 1.0, 1)
+    assert len(box) == 65536
+    assert box.min(0).min() >= 3 * dx and box.max(0).max() <= 35 * dx
+    # unjittered sphere: count lattice points within r by brute force
+    c, r, sp = np.array([0.5, 0.5, 0.5]), 0.25, 1 / 32  # 2r / sp = 16 exactly; no point lies on the sphere
+    pts = O.sample_shape(1, [*c, r], sp, 0.0, 1)
+    n = int(math.floor(2 * r / sp))
+    g = (c - r)[None] + (np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).reshape(-1, 3) + 0.5) * sp
+    assert len(pts) == int(((g - c) ** 2).sum(1).__le__(r * r).sum())
+    assert len(pts) == pytest.approx(4 / 3 * math.pi * r ** 3 / sp ** 3, rel=0.05)
+
+
+def test_pack_particles_lattice_coordinates():
+    """MPM.add packing: x = clamp((p - origin) / dx, bound + 1e-3, dims - bound - 1e-3), v = vel * dt/dx,
+    J = 1, F = I, C = 0, material id in x.w (mpm.js:213-227)."""
+    pts = np.array([[0.1, 0.2, 0.3], [0.0, 0.99, 0.5]], np.float32)
+    rec = O.pack_particles(pts, [0, 0, 0], 1 / 32, [32, 32, 32], 3.0, 5, [1.0, -2.0, 0.5], 0.01)
+    np.testing.assert_allclose(rec[0, 0:3], [3.2, 6.4, 9.6], rtol=1e-6)
+    assert rec[1, 0] == pytest.approx(3.001) and rec[1, 1] == pytest.approx(28.999)
+    assert rec[0, 3:4].view(np.uint32)[0] == 5
+    np.testing.assert_allclose(rec[0, 4:7], [0.01, -0.02, 0.005], rtol=1e-6)
+    assert rec[0, 7] == 1.0
+    np.testing.assert_array_equal(rec[0, 20:23], [1, 0, 0])
+    np.testing.assert_array_equal(rec[0, 24:27], [0, 1, 0])
+    np.testing.assert_array_equal(rec[0, 28:31], [0, 0, 1])
+    assert not rec[0, 8:20].any()
+
+
+def test_pack_noise_is_gaussian():
+    pts = np.tile(np.array([[0.5, 0.5, 0.5]], np.float32), (20000, 1))
+    rec = O.pack_particles(pts, [0, 0, 0], 1 / 64, [64, 64, 64], 3.0, 0, [0, 0, 0], 1.0, noise_sigma=0.3,
+                           noise_seed=99)
+    v = rec[:, 4:7].astype(np.float64) / 1.0  # lattice velocity = world velocity * (dt / dx) with dt / dx = 1
+    assert abs(v.mean()) < 0.01 and v.std() == pytest.approx(0.3, rel=0.02)
+    kurt = ((v - v.mean()) ** 4).mean() / v.var() ** 2
+    assert kurt == pytest.approx(3.0, abs=0.1)
+
+
+# ---- material table (mpm.js:170-199) --------------------------------------------------------------
+
+@pytest.mark.parametrize("nu", [0.0, 0.2, 0.3, 0.45, 0.5])
+def test_material_row_lame_identities(nu):
+    """Solids: lambda / mu = 2 nu / (1 - 2 nu) and lambda + 2 mu = rho c^2 (the P-wave modulus), with
+    c = 0.4 stiffness and nu clamped to 0.49; mass = density / ppc^3."""
+    row = O.material_row({"kind": 1, "density": 2.0, "stiffness": 0.5, "poisson": nu}, 2).astype(np.float64)
+    nuc = min(nu, 0.49)
+    mu, lam = row[2], row[3]
+    assert row[0:1].astype(np.float32).view(np.uint32)[0] == 1
+    assert row[1] == pytest.approx(2.0 / 8)
+    assert lam + 2 * mu == pytest.approx(2.0 * 0.2 ** 2, rel=1e-6)
+    if nuc > 0:
+        assert lam / mu == pytest.approx(2 * nuc / (1 - 2 * nuc), rel=1e-5)
+    else:
+        assert lam == 0
+
+
+def test_material_row_drucker_prager_alpha():
+    """alpha = sqrt(2/3) 2 sin(phi) / (3 - sin(phi)) (the Drucker-Prager cone inscribed in the
+    Mohr-Coulomb pyramid): phi = 30 deg gives sqrt(2/3) / 2.5."""
+    row = O.material_row({"kind": 3, "density": 1.6, "stiffness": 0.6, "poisson": 0.3, "friction": 30}, 2)
+    assert float(row[8]) == pytest.approx(math.sqrt(2 / 3) / 2.5, rel=1e-6)
+    fl = O.material_row({"kind": 0, "density": 1.0, "stiffness": 0.5}, 2)
+    assert float(fl[2]) == pytest.approx(1.0 * 0.2 ** 2, rel=1e-6)  # fluid bulk modulus K = rho c^2
+
+
+# ---- primitives (scan.js:292-320, grid.js:70-79, mpm.js:563-577) ---------------------------------
+
+@pytest.mark.parametrize("n", [0, 1, 255, 256, 257, 4097, 100003])
+def test_scan_exclusive_matches_numpy(n):
+    """core.test.js:74 sizes; exclusive, wrapping u32."""
+    a = np.random.default_rng(n).integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    out = O.scan(a)
+    want = (np.concatenate([[0], np.cumsum(a.astype(np.uint64))[:-1]]) % 2 ** 32).astype(np.uint32) if n else a
+    assert np.array_equal(out, want)
+
+
+@pytest.mark.parametrize("n,nb", [(0, 5), (1, 1), (1000, 7), (50000, 4096)])
+def test_counting_sort_stable(n, nb):
+    keys = np.random.default_rng(n + nb).integers(0, nb, n).astype(np.uint32)
+    order, start = O.counting_sort(keys, nb)
+    assert np.array_equal(order, np.argsort(keys, kind="stable").astype(np.uint32))
+    assert np.array_equal(start, np.concatenate([[0], np.cumsum(np.bincount(keys, minlength=nb))]).astype(np.uint32))
+
+
+def test_particle_sort_invariants():
+    """_sort (mpm.js:614-621): a permutation (checkPermutation), 4^3-block Morton keys non-decreasing
+    (checkSorted), and inside a block the order (rank in cell, cell) with ties in input order."""
+    rng = np.random.default_rng(4)
+    n = 5000
+    dims = [32, 32, 32]
+    rec = np.zeros((n, 32), np.float32)
+    rec[:, 0:3] = rng.uniform(3.2, 28.7, (n, 3))
+    rec[:, 7] = np.arange(n)  # tag (J slot) to identify particles
+    sc = O.Scene(dims, n, 2, 4)
+    sc.set_materials(np.zeros((8, 12), np.float32))
+    sc.write(rec)
+    sc.set_count(n)
+    sc.sort()
+    out = sc.read(n)
+    tags = out[:, 7].astype(np.int64)
+    assert np.array_equal(np.sort(tags), np.arange(n))
+    base = np.maximum(np.floor(out[:, 0:3] - 0.5), 0).astype(np.int64)
+
+    def spread(v):
+        r = np.zeros_like(v)
+        for b in range(10):
+            r |= ((v >> b) & 1) << (3 * b)
+        return r
+    cell = spread(base[:, 0]) | (spread(base[:, 1]) << 1) | (spread(base[:, 2]) << 2)
+    blk = cell >> 6
+    assert np.all(np.diff(blk) >= 0)
+    for b in np.unique(blk)[:50]:
+        idx = np.nonzero(blk == b)[0]
+        c = cell[idx] & 63
+        t = tags[idx]
+        rank = np.array([np.sum((c[:k] == c[k])) for k in range(len(c))])
+        key = rank * 64 + c
+        assert np.all(np.diff(key) > 0)
+        for cc in np.unique(c):  # particles of one cell keep their input order
+            assert np.all(np.diff(t[c == cc]) > 0)
+
+
+# ---- whole substep (physics.test.js, lifecycle.test.js, verify/index.js) --------------------------
+
+def _dam(dims=(64, 64, 64)):
+    dx = 1 / 64
+    pts = O.sample_shape(0, [3 * dx, 3 * dx, 3 * dx, 11 * dx, 35 * dx, 35 * dx], dx / 2, 1.0, 1)
+    dt = 1 / 480
+    rec = O.pack_particles(pts, [0, 0, 0], dx, list(dims), 3.0, 0, [0, 0, 0], dt / dx)
+    sc = O.Scene(list(dims), len(rec), 2, 4)
+    tbl = np.zeros((8, 12), np.float32)
+    tbl[0] = O.material_row({"kind": 0, "density": 1.0, "stiffness": 0.5, "viscosity": 0.1}, 2)
+    sc.set_materials(tbl)
+    sc.write(rec)
+    sc.set_count(len(rec))
+    g = np.array([0.0, np.float32(-9.81 * dt * dt / dx), 0.0], np.float32)
+    return sc, rec, g, tbl
+
+
+def test_dam_break_stable():
+    """physics.test.js:43-60 on the oracle: grid 48 (world size 1), a water box [.05,.4]x[.05,.6]x[.05,.95]
+    and a jelly sphere (.7,.4,.5) r .12, 150 frames of 8 substeps: no NaN, every particle inside
+    [1, dims - 1], mean water J within 0.05 of 1."""
+    from shaderintel_amd import mpm as M
+    dims, dx, dt = [48, 48, 48], 1 / 48, 1 / 480
+    water = O.sample_shape(0, [0.05, 0.05, 0.05, 0.4, 0.6, 0.95], dx / 2, 0.6, 1)
+    jelly = O.sample_shape(1, [0.7, 0.4, 0.5, 0.12], dx / 2, 0.6, len(water) + 1)
+    rec = np.concatenate([O.pack_particles(water, [0, 0, 0], dx, dims, 3.0, 0, [0, 0, 0], dt / dx),
+                          O.pack_particles(jelly, [0, 0, 0], dx, dims, 3.0, 1, [0, 0, 0], dt / dx)])
+    tbl = np.zeros((8, 12), np.float32)
+    tbl[0] = O.material_row({**M.MATERIALS["water"], "kind": 0}, 2)
+    tbl[1] = O.material_row({**M.MATERIALS["jelly"], "kind": 1}, 2)
+    sc = O.Scene(dims, len(rec), 2, 4)
+    sc.set_materials(tbl)
+    sc.write(rec)
+    sc.set_count(len(rec))
+    sc.step(1200, np.array([0.0, np.float32(-9.81 * dt * dt / dx), 0.0], np.float32))
+    out = sc.read(len(rec))
+    assert np.isfinite(out).all()
+    assert out[:, 0:3].min() >= 1.0 and out[:, 0:3].max() <= 47.0
+    is_water = out[:, 3].view(np.uint32) == 0
+    assert abs(out[is_water, 7].mean() - 1) < 0.05
+
+
+def test_conservation_after_p2g():
+    """verify/index.js:175-193: after P2G the grid mass equals the particle mass and the grid momentum
+    equals sum m v (fixed-point rounding aside), with random v, C and tau."""
+    from helpers import randomize_transfer_terms
+    sc, rec, g, tbl = _dam((32, 48, 48))
+    rec = rec[:20000].copy()
+    randomize_transfer_terms(rec[:2000])
+    sc.write(rec)
+    sc.set_count(len(rec))
+    sc.p2g()
+    grid = sc.grid().astype(np.float64) / FP
+    m = float(tbl[0, 1])
+    n = len(rec)
+    assert grid[:, 3].sum() == pytest.approx(n * m, abs=n * 27 / FP)
+    np.testing.assert_allclose(grid[:, :3].sum(0), m * rec[:, 4:7].astype(np.float64).sum(0), atol=n * 27 / FP)
+
+
+def test_retime_preserves_speed():
+    """lifecycle.test.js:294-306: doubling the substeps halves dt; rescaling v and C by dt_new / dt_old
+    keeps the world-space speed within 5%."""
+    sc, rec, g, _ = _dam((32, 48, 48))
+    sc.step(16, g)
+    n = len(rec)
+    v_before = np.linalg.norm(sc.read(n)[:, 4:7].astype(np.float64), axis=1).mean() * 480
+    sc.rescale(0.5)
+    dt2 = 1 / 960
+    g2 = np.array([0.0, np.float32(-9.81 * dt2 * dt2 * 64), 0.0], np.float32)
+    sc.step(2, g2)  # 1/480 s
+    sc2, _, _, _ = _dam((32, 48, 48))
+    sc2.step(17, g)
+    v_a = np.linalg.norm(sc.read(n)[:, 4:7].astype(np.float64), axis=1).mean() * 960
+    v_b = np.linalg.norm(sc2.read(n)[:, 4:7].astype(np.float64), axis=1).mean() * 480
+    assert v_before > 0.1
+    assert v_a == pytest.approx(v_b, rel=0.05)
+
+
+def test_incompressible_poisson_stays_finite():
+    """lifecycle.test.js:274-292: poisson 0.5 (clamped to 0.49) jelly stays finite."""
+    dx = 1 / 32
+    pts = O.sample_shape(0, [0.3, 0.3, 0.3, 0.6, 0.6, 0.6], dx / 2, 1.0, 1)
+    rec = O.pack_particles(pts, [0, 0, 0], dx, [32, 32, 32], 3.0, 0, [0, -1, 0], (1 / 480) / dx)
+    sc = O.Scene([32, 32, 32], len(rec), 2, 4)
+    tbl = np.zeros((8, 12), np.float32)
+    tbl[0] = O.material_row({"kind": 1, "density": 1.0, "stiffness": 1.0, "poisson": 0.5}, 2)
+    sc.set_materials(tbl)
+    sc.write(rec)
+    sc.set_count(len(rec))
+    sc.step(80, np.array([0, -9.81 * (1 / 480) ** 2 * 32, 0], np.float32))
+    assert np.isfinite(sc.read(len(rec))).all()
