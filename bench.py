@@ -74,6 +74,130 @@ def copy_calibration(nbytes: int) -> None:
     log(f"copy calibration: 3 copies of {nbytes} bytes (each reads and writes {nbytes} bytes)")
 
 
+def pctl(a, q):
+    return round(float(np.percentile(a, q)), 3) if len(a) else None
+
+
+def batched(ctx, substeps: int, seed: int = 1, count: int = 64) -> dict:
+    """The batched workload (64 scenes from one master seed, see scenes.batched_blocks): one scene
+    pool stepped with one launch per phase (captured sort periods, autotuned occupancy) against the
+    same scenes as separate simulations, each stepped with its own launches on its own stream."""
+    from shaderintel_amd import scenes
+    from shaderintel_amd.mpm import MPM, ScenePool, load_library, node_blocks, _ptr, _check
+    L = load_library()
+    built = [scenes.host_blocks(edge, blocks) for edge, blocks in scenes.batched_blocks(seed, count)]
+    total = sum(len(b[2]) for b in built)
+    log(f"batched: {count} scenes, {total} particles")
+    pool = ScenePool(ctx, total, count, sum(node_blocks(b[0]) for b in built), max(b[0][0] for b in built))
+    for d, t, r, g in built:
+        pool.add_scene(d, t, r, g)
+    pool.use_graphs(True)
+    tuned = pool.autotune(4)
+    pool.step(8)
+    pool.sync()
+    t0 = time.perf_counter()
+    pool.step(substeps)
+    pool.sync()
+    dt_pool = time.perf_counter() - t0
+    pool.destroy()
+    log(f"batched pool: {substeps} substeps in {dt_pool * 1e3:.1f} ms (tuned {tuned})")
+    sims = []
+    for d, t, r, g in built:
+        sim = MPM(ctx, grid=d[0], size=d[0] / 64.0, maxParticles=len(r), substeps=8)
+        _check(L.mpm_set_materials(sim._h, _ptr(np.ascontiguousarray(t)), 8), "materials")
+        sim.write_particles(r)
+        _check(L.mpm_set_count(sim._h, len(r)), "count")
+        sims.append((sim, np.ascontiguousarray(g)))
+    for sim, g in sims:
+        _check(L.mpm_step(sim._h, 8, _ptr(g), 0.4, 0.9), "step")
+    for sim, _ in sims:
+        sim.sync()
+    # (a) the reference's pattern: one simulation object per scene, each stepped in turn on one
+    # device queue (WebGPU has a single queue), so the scenes run one after another
+    t0 = time.perf_counter()
+    for sim, g in sims:
+        _check(L.mpm_step(sim._h, substeps, _ptr(g), 0.4, 0.9), "step")
+        sim.sync()
+    dt_seq = time.perf_counter() - t0
+    # (b) the same simulations on their own HIP streams, all enqueued before one wait (they overlap)
+    t0 = time.perf_counter()
+    for sim, g in sims:
+        _check(L.mpm_step(sim._h, substeps, _ptr(g), 0.4, 0.9), "step")
+    for sim, _ in sims:
+        sim.sync()
+    dt_sep = time.perf_counter() - t0
+    for sim, _ in sims:
+        sim.destroy()
+    log(f"batched separate: {substeps} substeps in {dt_seq * 1e3:.1f} ms in turn, {dt_sep * 1e3:.1f} ms on 64 streams")
+    rate_pool, rate_seq, rate_sep = (total * substeps / dt for dt in (dt_pool, dt_seq, dt_sep))
+    return {"workload": f"{count} scenes (master seed {seed}): edges 32-128, 1e4-1e6 particles each, "
+                        f"water/jelly/snow/sand, {total} particles, {substeps} substeps",
+            "particles": total, "substeps": substeps,
+            "pool_particle_substeps_per_s": round(rate_pool, 1),
+            "per_scene_in_turn_particle_substeps_per_s": round(rate_seq, 1),
+            "per_scene_streams_particle_substeps_per_s": round(rate_sep, 1),
+            "speedup_vs_per_scene_in_turn": round(rate_pool / rate_seq, 2),
+            "speedup_vs_per_scene_streams": round(rate_pool / rate_sep, 2), "tuned": tuned,
+            "pool_ms": round(dt_pool * 1e3, 2), "in_turn_ms": round(dt_seq * 1e3, 2),
+            "streams_ms": round(dt_sep * 1e3, 2)}
+
+
+def serve(ctx, rate: float, seconds: float, budget_gb: float, substeps: int = 4, lifetime_mean: float = 300,
+          seed: int = 2) -> dict:
+    """Serving: Poisson arrivals at `rate` requests/s for `seconds`; each request is one of the 64
+    batched-workload scenes (drawn at random) simulated for a geometric number of steps (mean
+    `lifetime_mean`) of `substeps` substeps. One pool sized to the HBM budget; the native serving loop
+    admits, steps and retires (continuous batching)."""
+    from shaderintel_amd import scenes
+    from shaderintel_amd.mpm import ScenePool, Server
+    built = [scenes.host_blocks(edge, blocks) for edge, blocks in scenes.batched_blocks(1, 64)]
+    tmpl_bytes = sum(len(b[2]) for b in built) * 128
+    budget = budget_gb * 1e9 - tmpl_bytes
+    max_scenes, max_dim = 2048, 128
+
+    def cfg(cap):  # node blocks: ~1 per 20 particles for this scene mix (grids are sparse)
+        return cap, max_scenes, max(cap // 20, 32768), max_dim
+
+    lo, hi = 1 << 20, 1 << 30
+    while hi - lo > (1 << 20):
+        mid = (lo + hi) // 2
+        lo, hi = (mid, hi) if ScenePool.bytes_for(*cfg(mid)) <= budget else (lo, mid)
+    pool = ScenePool(ctx, *cfg(lo))
+    log(f"serve: pool of {lo} particles / {cfg(lo)[2]} node blocks = {ScenePool.bytes_for(*cfg(lo)) / 1e9:.1f} GB "
+        f"(+{tmpl_bytes / 1e9:.1f} GB templates) for a {budget_gb} GB budget; {rate} req/s for {seconds} s")
+    tids = [pool.add_template(d, t, r, g) for d, t, r, g in built]
+    rng = np.random.default_rng(seed)
+    srv = Server(pool, substeps=substeps)
+    t0 = time.perf_counter()
+    t_next = 0.0
+    submitted = 0
+    while True:
+        t_next += rng.exponential(1.0 / rate)
+        if t_next > seconds:
+            break
+        delay = t0 + t_next - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        srv.submit(tids[int(rng.integers(len(tids)))], int(rng.geometric(1.0 / lifetime_mean)))
+        submitted += 1
+    wall = time.perf_counter() - t0
+    srv.stop(drain=False)
+    st = srv.stats()
+    steps, req, wait = srv.latencies("step_ms"), srv.latencies("request_s"), srv.latencies("wait_s")
+    srv.destroy()
+    pool.destroy()
+    log(f"serve: {st}")
+    return {"workload": f"Poisson {rate} req/s for {seconds} s; scenes drawn from the 64-scene batched mix; lifetime "
+                        f"geometric, mean {lifetime_mean} steps of {substeps} substeps; {budget_gb} GB HBM budget",
+            "rate_req_s": rate, "submitted": st["submitted"], "completed": st["completed"],
+            "rejected": st["rejected"], "waiting_at_end": st["waiting"], "resident_at_end": st["running"],
+            "resident_particles_at_end": st["resident_particles"], "steps": st["steps"],
+            "step_latency_ms_p50": pctl(steps, 50), "step_latency_ms_p99": pctl(steps, 99),
+            "request_latency_s_p50": pctl(req, 50), "request_latency_s_p99": pctl(req, 99),
+            "admission_wait_s_p50": pctl(wait, 50), "admission_wait_s_p99": pctl(wait, 99),
+            "particle_substeps_per_s": round(st["particle_substeps"] / wall, 1), "error": st["error"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +207,12 @@ def main() -> int:
     ap.add_argument("--cpu-substeps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", action="store_true", help="time each tuning knob setting and exit")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched comparison")
+    ap.add_argument("--batched-substeps", type=int, default=200)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
+    ap.add_argument("--serve-rate", type=float, nargs="+", default=[100.0], help="request arrival rates (req/s)")
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the timed region, copy a MIB-MiB device buffer 3 times (a kernel of known traffic "
                          "for scaling HBM byte counters)")
@@ -131,6 +261,9 @@ def main() -> int:
     dt = time.perf_counter() - t0
     phases = sim.phase_times()
     sim.set_profiling(False)
+    from shaderintel_amd.mpm import diagnostics
+    diag = diagnostics(sim._h)
+    log(f"diagnostics after the timed run: {diag}")
     raw = sim.read_particles()
     finite = bool(np.isfinite(raw).all())
     del raw
@@ -170,6 +303,11 @@ def main() -> int:
         "finite": finite,
     }
     sim.destroy()
+    if not args.no_batched:
+        out["batched"] = batched(ctx, args.batched_substeps)
+    if not args.no_serve:
+        runs = [serve(ctx, r, args.serve_seconds, args.serve_budget_gb) for r in args.serve_rate]
+        out["serve"] = runs[0] if len(runs) == 1 else runs
     if args.copy_calibration:
         copy_calibration(args.copy_calibration << 20)
     if not args.no_cpu:

@@ -157,6 +157,60 @@ mpm_status mpm_pool_sync(mpm_pool* pool);
  * pool's particle count and vmax do not change. */
 mpm_status mpm_pool_use_graphs(mpm_pool* pool, int32_t enable);
 mpm_status mpm_pool_set_tuning(mpm_pool* pool, int32_t knob, int32_t value);
+/* Diagnostics (no reference counterpart): device counters of the last binning and P2G. Blocking. */
+typedef struct mpm_diagnostics {
+  uint32_t work_items;          /* occupied 4x4x4-cell particle blocks of the last binning */
+  uint32_t s_blocks;            /* node blocks the tiles of those items can reach */
+  uint32_t d_blocks;            /* further node blocks written by the fallback path in the last P2G */
+  uint32_t tile_nodes_used;     /* tile arena nodes taken in the last P2G (may exceed capacity: overflow) */
+  uint32_t tile_nodes_capacity;
+  uint32_t particles;
+} mpm_diagnostics;
+mpm_status mpm_pool_diagnostics(mpm_pool* pool, mpm_diagnostics* out);
+mpm_status mpm_diagnostics_of(mpm_sim* sim, mpm_diagnostics* out);
+
+/* Times every occupancy variant of G2P and P2G for trial_substeps substeps each (rounded up to whole
+ * sort periods) with HIP events and keeps the fastest; *g2p_waves and *p2g_waves receive it. The
+ * trials are real steps of the pool (every variant gives identical results), so the pool advances
+ * by 12 trials. */
+mpm_status mpm_pool_autotune(mpm_pool* pool, uint32_t trial_substeps, int32_t* g2p_waves, int32_t* p2g_waves);
+
+/* Scene templates: a scene kept on the device for repeated admission (serving). The records are
+ * validated and copied to device memory once; mpm_pool_add_scene_from_template then admits a copy
+ * with a device-to-device copy. Template memory is not part of mpm_pool_bytes. */
+mpm_status mpm_pool_add_template(mpm_pool* pool, const mpm_scene_desc* desc, uint32_t* tmpl);
+mpm_status mpm_pool_remove_template(mpm_pool* pool, uint32_t tmpl);
+mpm_status mpm_pool_add_scene_from_template(mpm_pool* pool, uint32_t tmpl, uint32_t* scene);
+
+/* ---- serving loop (no reference counterpart; ShaderIntel steps one MPM object per scene) ----
+ * A native thread owns the pool from mpm_server_create until mpm_server_stop: the caller must not
+ * call any other function on that pool meanwhile. Requests (a template and a lifetime in steps) are
+ * admitted in arrival order while the pool has room (its capacity is the HBM budget, see
+ * mpm_pool_bytes); a request that does not fit waits and blocks the ones behind it; one that does not
+ * fit an empty pool is rejected. Each step advances every resident scene by `substeps` substeps in
+ * one launch per phase; scenes join and leave between steps (continuous batching). */
+typedef struct mpm_server mpm_server;
+typedef struct mpm_server_config {
+  uint32_t substeps;    /* substeps per step, e.g. 4 */
+  float floor_friction; /* 0.4 in the reference */
+  float vmax;           /* 0.9 in the reference */
+} mpm_server_config;
+typedef struct mpm_server_counters {
+  uint64_t submitted, completed, rejected, waiting, running, steps, particle_substeps;
+  uint32_t resident_particles;
+  int32_t error; /* status of a failed step (the loop stops), else 0 */
+} mpm_server_counters;
+#define MPM_LAT_STEP_MS 0   /* duration of each step, ms */
+#define MPM_LAT_REQUEST_S 1 /* per completed request: submit -> last step done, s */
+#define MPM_LAT_WAIT_S 2    /* per completed request: submit -> admission, s */
+mpm_status mpm_server_create(mpm_pool* pool, const mpm_server_config* cfg, mpm_server** out);
+mpm_status mpm_server_submit(mpm_server* srv, uint32_t tmpl, uint32_t lifetime_steps, uint64_t* ticket);
+mpm_status mpm_server_stats(mpm_server* srv, mpm_server_counters* stats);
+/* Copies up to capacity samples of one latency series; *n receives the series length. */
+mpm_status mpm_server_latencies(mpm_server* srv, int32_t kind, double* out, uint64_t capacity, uint64_t* n);
+/* Stops the loop (drain != 0: first finish every queued and resident request) and joins the thread. */
+mpm_status mpm_server_stop(mpm_server* srv, int32_t drain);
+void mpm_server_destroy(mpm_server* srv);
 
 /* ---- primitives ---- */
 /* Scan.run(src, dst, n) — src/prims/scan.js:292-306: exclusive prefix sum of n u32 (wrapping).

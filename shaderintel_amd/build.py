@@ -32,7 +32,7 @@ HIP_FLAGS = [
     "-fPIC",
     "-Wall",
 ]
-SOURCES = ["prims.hip", "mpm_kernels.hip", "mpm_runtime.cpp", "host_tools.cpp"]
+SOURCES = ["prims.hip", "mpm_kernels.hip", "mpm_runtime.cpp", "host_tools.cpp", "serve.cpp"]
 
 
 def _run(cmd: list[str], cwd: str | None = None) -> None:
@@ -53,7 +53,7 @@ def build_library() -> str:
         _run([HIPCC, *HIP_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     return LIB
 

@@ -307,10 +307,14 @@ __global__ void k_block_info(uint2* __restrict__ info, uint32_t base, uint32_t n
   info[base + i] = make_uint2(scene, bx | (by << 10) | (bz << 20));
 }
 
-__global__ void k_begin_p2g(uint32_t* ctr) {
-  ctr[kCtrEpoch] += 1u;
-  ctr[kCtrD] = 0u;
-  ctr[kCtrWork] = 0u;
+__global__ void k_begin_p2g(uint32_t* ctr, uint32_t* arena_ctr) {
+  const int t = threadIdx.x;  // kArenaRegions threads
+  arena_ctr[t * kArenaStride] = 0u;
+  if (t == 0) {
+    ctr[kCtrEpoch] += 1u;
+    ctr[kCtrD] = 0u;
+    ctr[kCtrWork] = 0u;
+  }
 }
 
 __global__ void k_begin_bin(uint32_t* ctr) {
@@ -492,7 +496,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
     Pool pool, const float4* __restrict__ P, const uint32_t* __restrict__ perm, float4* __restrict__ dst) {
   __shared__ int tile[4 * kTileNodes];
   __shared__ int box[6];
-  __shared__ uint32_t s_item;
+  __shared__ uint32_t s_item, s_off;
   const int tid = threadIdx.x;
   const uint32_t items = pool.ctr[kCtrItems];
   const uint32_t epoch = pool.ctr[kCtrEpoch], bin_epoch = pool.ctr[kCtrBinEpoch];
@@ -508,7 +512,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
     const float vol4 = pool.scenes[pool.wl_block[k] >> mshift].vol4;
     const int wx = 4 * (int)compact3(mb) - kDrift, wy = 4 * (int)compact3(mb >> 1) - kDrift,
               wz = 4 * (int)compact3(mb >> 2) - kDrift;  // lowest in-window base
-    const bool has_slot = k < pool.tile_slots;
     if (tid < 3) { box[tid] = INT_MAX; box[3 + tid] = INT_MIN; }
     __syncthreads();
     int lo0 = INT_MAX, lo1 = INT_MAX, lo2 = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hi2 = INT_MIN;
@@ -533,6 +536,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
     const int ox = box[0], oy = box[1], oz = box[2];
     const int ex = any ? box[3] - ox + 3 : 0, ey = any ? box[4] - oy + 3 : 0, ez = any ? box[5] - oz + 3 : 0;
     const int nodes = ex * ey * ez;  // <= kTileNodes by the window test
+    if (tid == 0) {  // arena space for the tile, from this workgroup's region
+      uint32_t off = 0xffffffffu;
+      const uint32_t region = blockIdx.x % kArenaRegions, rsize = pool.tile_nodes / kArenaRegions;
+      uint32_t* rc = pool.arena_ctr + region * kArenaStride;
+      // the check keeps a full region's counter from running far past its end
+      if (nodes > 0 && __hip_atomic_load(rc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < rsize) {
+        const uint32_t o = atomicAdd(rc, (uint32_t)nodes);
+        if (o <= rsize - (uint32_t)nodes) off = region * rsize + o;
+      }
+      s_off = off;
+      pool.tile_off[k] = off;
+      if (off != 0xffffffffu) pool.tile_hdr[k] = make_int4(ox, oy, oz, ex | (ey << 8) | (ez << 16));
+    }
+    __syncthreads();
+    const uint32_t toff = s_off;
+    const bool has_slot = toff != 0xffffffffu;
     if (has_slot)
       for (int t = tid; t < nodes; t += kThreads) {
         tile[t] = 0;
@@ -566,8 +585,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
     }
     __syncthreads();
     if (has_slot) {
-      if (tid == 0) pool.tile_hdr[k] = make_int4(ox, oy, oz, ex | (ey << 8) | (ez << 16));
-      int4* out = pool.tiles + (size_t)k * kTileNodes;
+      int4* out = pool.tiles + toff;
       for (int t = tid; t < nodes; t += kThreads)
         out[t] = make_int4(tile[t], tile[kTileNodes + t], tile[2 * kTileNodes + t], tile[3 * kTileNodes + t]);
     }
@@ -599,11 +617,12 @@ __device__ __forceinline__ int4 pull_tiles(const Pool& pool, uint32_t scene, int
                                            int n1, int n2, int4 acc) {
   const int pb = 1 << (pool.key_bits / 3 - 2);  // particle blocks per axis a scene's map addresses
   const int cx = g0 - 2 + (lane & 3), cy = g1 - 2 + ((lane >> 2) & 3), cz = g2 - 2 + (lane >> 4);
-  uint32_t k = 0xffffffffu;
+  uint32_t k = 0xffffffffu, off = 0xffffffffu;
   if ((unsigned)cx < (unsigned)pb && (unsigned)cy < (unsigned)pb && (unsigned)cz < (unsigned)pb)
     k = pool.map[(scene << (pool.key_bits - 6)) + morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz)];
+  if (k != 0xffffffffu) off = pool.tile_off[k];
   int4 h = make_int4(0, 0, 0, 0);
-  if (k < pool.tile_slots) h = pool.tile_hdr[k];
+  if (off != 0xffffffffu) h = pool.tile_hdr[k];
   const int ex = h.w & 255, ey = (h.w >> 8) & 255, ez = (h.w >> 16) & 255;
   const bool hit = ex > 0 && h.x < 4 * g0 + 4 && h.x + ex > 4 * g0 && h.y < 4 * g1 + 4 && h.y + ey > 4 * g1 &&
                    h.z < 4 * g2 + 4 && h.z + ez > 4 * g2;
@@ -616,13 +635,13 @@ __device__ __forceinline__ int4 pull_tiles(const Pool& pool, uint32_t scene, int
       if (mask) {
         const int c = __builtin_ctzll(mask);
         mask &= mask - 1;
-        const uint32_t kc = (uint32_t)__builtin_amdgcn_readlane((int)k, c);
+        const uint32_t oc = (uint32_t)__builtin_amdgcn_readlane((int)off, c);
         const int hx = __builtin_amdgcn_readlane(h.x, c), hy = __builtin_amdgcn_readlane(h.y, c),
                   hz = __builtin_amdgcn_readlane(h.z, c), hw = __builtin_amdgcn_readlane(h.w, c);
         const int tx = hw & 255, ty = (hw >> 8) & 255, tz = (hw >> 16) & 255;
         const int lx = n0 - hx, ly = n1 - hy, lz = n2 - hz;
         if ((unsigned)lx < (unsigned)tx && (unsigned)ly < (unsigned)ty && (unsigned)lz < (unsigned)tz)
-          q[u] = pool.tiles[(size_t)kc * kTileNodes + (lz * ty + ly) * tx + lx];
+          q[u] = pool.tiles[(size_t)oc + (uint32_t)((lz * ty + ly) * tx + lx)];
       }
     }
 #pragma unroll
@@ -730,7 +749,7 @@ hipError_t launch_block_info(uint2* info, uint32_t base, uint32_t n, uint32_t sc
   return hipGetLastError();
 }
 hipError_t launch_begin_p2g(const Pool& pool, hipStream_t s) {
-  hipLaunchKernelGGL(k_begin_p2g, dim3(1), dim3(1), 0, s, pool.ctr);
+  hipLaunchKernelGGL(k_begin_p2g, dim3(1), dim3(kArenaRegions), 0, s, pool.ctr, pool.arena_ctr);
   return hipGetLastError();
 }
 hipError_t launch_begin_bin(const Pool& pool, hipStream_t s) {

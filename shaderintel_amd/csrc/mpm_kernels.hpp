@@ -18,11 +18,13 @@
 //    (start offset, key >> 6); map[map_base(scene) + Morton block] = item. Inside a block the order
 //    is (rank within cell, cell), so consecutive particles sit in different cells.
 //  - Tiles: P2G gives every work item one workgroup that sums the 27 fixed-point contributions of
-//    its particles in an LDS tile (LDS integer atomics) and stores the tile box and nodes to tile slot
-//    `item` with plain stores. The grid update of a node block pulls from the tiles of particle
+//    its particles in an LDS tile (LDS integer atomics) sized to the bounding box of their stencils,
+//    takes that many nodes from one region of a bump-allocated arena and stores the box and nodes there with plain
+//    stores (tile_hdr[item], tile_off[item]); an item whose tile no longer fits the arena, or whose
+//    particles all drifted, uses the fallback below. The grid update of a node block pulls from the tiles of particle
 //    blocks G-2..G+1 per axis, so the tiled path needs no global atomic. A particle whose base
 //    drifted more than kDrift cells outside its block since the binning (or of an item beyond the
-//    tile slots) falls back to global atomics on A and stamps the node blocks it wrote.
+//    falls back to global atomics on A and stamps the node blocks it wrote.
 //  - Active node blocks: S = node blocks within reach of any work item (rebuilt at each binning);
 //    D = node blocks written through A in the current P2G. The grid update visits S and D and
 //    returns A to zero where it was written.
@@ -39,6 +41,11 @@ constexpr int kTileSpan = 4 + 2 + 2 * kDrift;                  // nodes per axis
 constexpr int kTileNodes = kTileSpan * kTileSpan * kTileSpan;  // 1728 nodes = 27 KiB of int4
 constexpr int kMatRows = 8;                                    // material rows per scene (row >> 3 = scene)
 constexpr int kColRows = 16;                                   // collider rows per scene
+// The tile arena is split into kArenaRegions regions, each with its own bump counter on its own
+// 64-byte line: workgroup w allocates from region w % kArenaRegions, so allocations do not contend
+// on one address (a single shared counter cost 0.8 ms per P2G at 9k items on MI355X).
+constexpr int kArenaRegions = 256;
+constexpr int kArenaStride = 16;  // u32 words between region counters
 
 enum Counter : int {
   kCtrEpoch = 0,     // P2G epoch (stamps of D)
@@ -73,7 +80,7 @@ struct Pool {
   uint32_t key_bits;       // bits of the per-scene cell key (3 * per-axis bits)
   uint32_t nblocks;        // node blocks in the pool
   uint32_t max_items;      // work items a binning can produce
-  uint32_t tile_slots;
+  uint32_t tile_nodes;     // nodes in the tile arena (kArenaRegions regions of tile_nodes / kArenaRegions)
   int4* A;
   float4* V;
   const uint2* block_info; // node block -> (scene, bx | by << 10 | bz << 20) in the scene's blocks
@@ -84,12 +91,14 @@ struct Pool {
   uint32_t* map;
   uint32_t* wl_start;      // [max_items + 1]
   uint32_t* wl_block;      // [max_items] key >> 6 (scene bits on top)
-  int4* tiles;             // [tile_slots * kTileNodes]
-  int4* tile_hdr;          // [tile_slots] (ox, oy, oz, ex | ey << 8 | ez << 16)
+  int4* tiles;             // [tile_nodes] arena
+  int4* tile_hdr;          // [max_items] (ox, oy, oz, ex | ey << 8 | ez << 16)
+  uint32_t* tile_off;      // [max_items] first arena node of the item's tile, 0xffffffff: none
   const float* mats;       // [scenes * kMatRows * 12]
   const float* cols;       // [scenes * kColRows * 12]
   const SceneGeo* scenes;
   uint32_t* ctr;           // [kCounters]
+  uint32_t* arena_ctr;     // [kArenaRegions * kArenaStride] nodes handed out per region in the current P2G
 };
 
 // Launch helpers; all enqueue on `s` and none synchronizes. P is the current particle buffer.
@@ -102,7 +111,7 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t 
 // block_info of the n = nbx*nby*nbz node blocks of a scene starting at block `base`.
 hipError_t launch_block_info(uint2* info, uint32_t base, uint32_t n, uint32_t scene, uint32_t nbx, uint32_t nby,
                              hipStream_t s);
-// ctr[kCtrEpoch]++, |D| = 0, work = 0 (start of a P2G)
+// ctr[kCtrEpoch]++, |D| = 0, work = 0, arena empty (start of a P2G)
 hipError_t launch_begin_p2g(const Pool& pool, hipStream_t s);
 // ctr[kCtrBinEpoch]++, |S| = 0, items = 0 (start of a binning)
 hipError_t launch_begin_bin(const Pool& pool, hipStream_t s);

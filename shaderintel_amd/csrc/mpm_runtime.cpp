@@ -62,7 +62,6 @@ mpm_status from_hip(hipError_t e, const char* where) {
 enum GridState { kGridClean = 0, kGridA = 1, kGridTiles = 2, kGridVelocity = 3 };
 enum BinMode { kBinOnly = 0, kBinReorder = 1, kBinReorderInP2G = 2 };
 
-constexpr size_t kTileBudgetBytes = size_t(8) << 30;  // tile slots beyond this use the fallback path
 
 int ceil_log2(uint64_t v) {
   int b = 0;
@@ -74,6 +73,20 @@ template <class T>
 hipError_t dmalloc(T** p, size_t count) {
   return hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T));
 }
+
+// A scene kept on the device for repeated admission (serving): its records in the reference layout
+// (scene-local material ids) plus what mpm_scene_desc carries.
+struct SceneTemplate {
+  bool used = false;
+  int32_t dims[3] = {0, 0, 0};
+  int32_t ppc_axis = 2;
+  float bound = 3.0f;
+  float gravity[3] = {0, 0, 0};
+  uint32_t nmat = 0;
+  float mats[8 * 12] = {0};
+  float4* rec = nullptr;  // count records, 8 float4 each
+  uint32_t count = 0;
+};
 
 struct SceneHost {
   bool used = false;
@@ -98,7 +111,7 @@ struct mpm_pool {
   // configuration
   uint32_t cap = 0, max_scenes = 0, max_blocks = 0, sort_every = 4;
   int max_dim = 0;
-  uint32_t key_bits = 0, sort_bits = 0, map_per_scene = 0, max_items = 0, tile_slots = 0;
+  uint32_t key_bits = 0, sort_bits = 0, map_per_scene = 0, max_items = 0, tile_nodes = 0;
   // device memory
   float4* P[2] = {nullptr, nullptr};
   int cur = 0;
@@ -108,9 +121,11 @@ struct mpm_pool {
   uint32_t *stamp = nullptr, *dlist = nullptr, *sstamp = nullptr, *slist = nullptr;
   uint32_t *map = nullptr, *wl_start = nullptr, *wl_block = nullptr;
   int4 *tiles = nullptr, *tile_hdr = nullptr;
+  uint32_t* tile_off = nullptr;
   float *mats = nullptr, *cols = nullptr;
   SceneGeo* scenes_d = nullptr;
   uint32_t* ctr = nullptr;
+  uint32_t* arena_ctr = nullptr;
   uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr}, *order = nullptr, *flags = nullptr;
   uint32_t* sort_scratch = nullptr;
   float4* staging = nullptr;
@@ -120,6 +135,7 @@ struct mpm_pool {
   // host state
   std::vector<SceneHost> scenes;
   std::vector<uint32_t> mem_order;  // scenes in the order their particle runs sit in the pool
+  std::vector<SceneTemplate> templates;
   std::vector<std::pair<uint32_t, uint32_t>> free_blocks;  // (start, length), sorted by start
   uint32_t count = 0;
   uint64_t step_count = 0;
@@ -157,7 +173,7 @@ Pool view(const mpm_pool* p) {
   v.key_bits = p->key_bits;
   v.nblocks = p->max_blocks;
   v.max_items = p->max_items;
-  v.tile_slots = p->tile_slots;
+  v.tile_nodes = p->tile_nodes;
   v.A = p->A;
   v.V = p->V;
   v.block_info = p->block_info;
@@ -170,10 +186,12 @@ Pool view(const mpm_pool* p) {
   v.wl_block = p->wl_block;
   v.tiles = p->tiles;
   v.tile_hdr = p->tile_hdr;
+  v.tile_off = p->tile_off;
   v.mats = p->mats;
   v.cols = p->cols;
   v.scenes = p->scenes_d;
   v.ctr = p->ctr;
+  v.arena_ctr = p->arena_ctr;
   return v;
 }
 
@@ -190,10 +208,12 @@ void free_pool(mpm_pool* p) {
   if (p->ctx) hipSetDevice(p->ctx->device);
   if (p->stream) hipStreamSynchronize(p->stream);
   drop_graphs(p);
+  for (auto& t : p->templates)
+    if (t.rec) hipFree(t.rec);
   for (hipEvent_t e : p->ev_pool) hipEventDestroy(e);
   void* ptrs[] = {p->P[0],   p->P[1],     p->A,        p->V,        p->block_info, p->stamp,   p->dlist,
-                  p->sstamp, p->slist,    p->map,      p->wl_start, p->wl_block,    p->tiles,   p->tile_hdr,
-                  p->mats,   p->cols,     p->scenes_d, p->ctr,      p->keys[0],     p->keys[1], p->vals[0],
+                  p->sstamp, p->slist,    p->map,      p->wl_start, p->wl_block,    p->tiles,   p->tile_hdr, p->tile_off,
+                  p->mats,   p->cols,     p->scenes_d, p->ctr, p->arena_ctr,      p->keys[0],     p->keys[1], p->vals[0],
                   p->vals[1], p->order,   p->flags,    p->sort_scratch, p->staging, p->export_buf};
   for (void* q : ptrs)
     if (q) hipFree(q);
@@ -203,24 +223,25 @@ void free_pool(mpm_pool* p) {
 
 // Device bytes a pool of this shape allocates (used for admission against an HBM budget).
 size_t pool_bytes(uint32_t cap, uint32_t max_scenes, uint32_t max_blocks, int max_dim, uint32_t* map_per_scene,
-                  uint32_t* max_items, uint32_t* tile_slots) {
+                  uint32_t* max_items, uint32_t* tile_nodes) {
   const int bits = std::max(3, ceil_log2((uint64_t)max_dim));
   const uint32_t mps = 1u << (3 * (bits - 2));
   const uint64_t items = std::min<uint64_t>((uint64_t)mps * max_scenes, cap);
-  // a work item holds 512 particles when its 4^3 cells are full (8 per cell); one slot per 16
-  // particles covers sparse surface blocks too, and items beyond the slots take the fallback path
-  const uint64_t slots = std::min<uint64_t>(
-      std::min<uint64_t>(items, std::max<uint64_t>(4096, cap / 16)), kTileBudgetBytes / (sizeof(int4) * kTileNodes));
+  // tile arena: a full 4^3-cell block (512 particles) needs 216-343 nodes, a sparse surface block a
+  // few nodes per particle; 4 nodes per particle (64 B) covers both, and an item whose tile does not
+  // fit takes the fallback path
+  const uint64_t arena = std::min<uint64_t>(0x7fffffffu, std::max<uint64_t>((uint64_t)4096 * kTileNodes, 4ull * cap)) /
+                         kArenaRegions * kArenaRegions;
   if (map_per_scene) *map_per_scene = mps;
   if (max_items) *max_items = (uint32_t)items;
-  if (tile_slots) *tile_slots = (uint32_t)slots;
+  if (tile_nodes) *tile_nodes = (uint32_t)arena;
   size_t b = 0;
   b += 2 * (size_t)cap * 128;                            // P[2]
   b += (size_t)max_blocks * 64 * (16 + 16);              // A, V
   b += (size_t)max_blocks * 4 * 6;                       // block_info, stamp, dlist, sstamp, slist
   b += (size_t)mps * max_scenes * 4;                     // map
   b += (items + 1) * 4 * 2;                              // wl_start, wl_block
-  b += slots * (sizeof(int4) * kTileNodes + 16);         // tiles, headers
+  b += arena * sizeof(int4) + items * 20;               // tile arena, headers, offsets
   b += (size_t)max_scenes * (kMatRows + kColRows) * 48;  // mats, cols
   b += (size_t)cap * 4 * 6 + 4;                          // keys, vals, order, flags
   b += sort_scratch_words(cap) * 4;
@@ -556,7 +577,7 @@ mpm_status create_pool(mpm_ctx* c, uint32_t cap, uint32_t max_scenes, uint32_t m
   p->sort_every = sort_every;
   p->key_bits = (uint32_t)(3 * bits);
   p->sort_bits = p->key_bits + (uint32_t)scene_bits;
-  pool_bytes(cap, max_scenes, max_blocks, max_dim, &p->map_per_scene, &p->max_items, &p->tile_slots);
+  pool_bytes(cap, max_scenes, max_blocks, max_dim, &p->map_per_scene, &p->max_items, &p->tile_nodes);
   p->scenes.resize(max_scenes);
   p->free_blocks.push_back({0u, max_blocks});
   const size_t nodes = (size_t)max_blocks * 64;
@@ -573,12 +594,14 @@ mpm_status create_pool(mpm_ctx* c, uint32_t cap, uint32_t max_scenes, uint32_t m
   if (e == hipSuccess) e = dmalloc(&p->map, (size_t)p->map_per_scene * max_scenes);
   if (e == hipSuccess) e = dmalloc(&p->wl_start, (size_t)p->max_items + 1);
   if (e == hipSuccess) e = dmalloc(&p->wl_block, p->max_items);
-  if (e == hipSuccess) e = dmalloc(&p->tiles, (size_t)p->tile_slots * kTileNodes);
-  if (e == hipSuccess) e = dmalloc(&p->tile_hdr, p->tile_slots);
+  if (e == hipSuccess) e = dmalloc(&p->tiles, (size_t)p->tile_nodes);
+  if (e == hipSuccess) e = dmalloc(&p->tile_hdr, p->max_items);
+  if (e == hipSuccess) e = dmalloc(&p->tile_off, p->max_items);
   if (e == hipSuccess) e = dmalloc(&p->mats, (size_t)max_scenes * kMatRows * 12);
   if (e == hipSuccess) e = dmalloc(&p->cols, (size_t)max_scenes * kColRows * 12);
   if (e == hipSuccess) e = dmalloc(&p->scenes_d, max_scenes);
   if (e == hipSuccess) e = dmalloc(&p->ctr, kCounters);
+  if (e == hipSuccess) e = dmalloc(&p->arena_ctr, (size_t)kArenaRegions * kArenaStride);
   if (e == hipSuccess) e = dmalloc(&p->keys[0], cap);
   if (e == hipSuccess) e = dmalloc(&p->keys[1], cap);
   if (e == hipSuccess) e = dmalloc(&p->vals[0], cap);
@@ -597,6 +620,7 @@ mpm_status create_pool(mpm_ctx* c, uint32_t cap, uint32_t max_scenes, uint32_t m
   if (e == hipSuccess) e = hipMemsetAsync(p->stamp, 0xff, (size_t)max_blocks * 4, p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->sstamp, 0xff, (size_t)max_blocks * 4, p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->ctr, 0, kCounters * 4, p->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(p->arena_ctr, 0, (size_t)kArenaRegions * kArenaStride * 4, p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->mats, 0, (size_t)max_scenes * kMatRows * 48, p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->cols, 0, (size_t)max_scenes * kColRows * 48, p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->scenes_d, 0, (size_t)max_scenes * sizeof(SceneGeo), p->stream);
@@ -656,9 +680,8 @@ mpm_status add_scene(mpm_pool* p, const int32_t dims[3], int32_t ppc_axis, float
   return MPM_OK;
 }
 
-mpm_status validate_records(const mpm_pool* p, uint32_t s, const float* rec, uint32_t n, uint32_t first) {
-  const SceneGeo& geo = p->scenes[s].geo;
-  const int dims[3] = {geo.X, geo.Y, geo.Z};
+// Positions inside [0.5, dims - 1.5) (the range MPM.add and G2P keep them in) and material ids < 8.
+mpm_status validate_positions(const int32_t dims[3], const float* rec, uint32_t n, uint32_t first = 0) {
   for (uint32_t i = 0; i < n; i++) {
     const float* r = rec + 32 * (size_t)i;
     for (int a = 0; a < 3; a++)
@@ -670,6 +693,12 @@ mpm_status validate_records(const mpm_pool* p, uint32_t s, const float* rec, uin
     if (mid >= (uint32_t)kMatRows) return fail(MPM_ERR_INVALID_ARGUMENT, "record %u material %u >= 8", first + i, mid);
   }
   return MPM_OK;
+}
+
+mpm_status validate_records(const mpm_pool* p, uint32_t s, const float* rec, uint32_t n, uint32_t first) {
+  const SceneGeo& geo = p->scenes[s].geo;
+  const int32_t dims[3] = {geo.X, geo.Y, geo.Z};
+  return validate_positions(dims, rec, n, first);
 }
 
 // Writes n records (reference layout, scene-local material ids) to pool slots [at, at + n).
@@ -696,6 +725,54 @@ mpm_status read_records(mpm_pool* p, uint32_t s, float* rec, uint32_t at, uint32
     HIP_TRY(hipMemcpy(rec + 32 * (size_t)done, p->staging, (size_t)k * 128, hipMemcpyDeviceToHost));
     done += k;
   }
+  return MPM_OK;
+}
+
+mpm_status check_materials(const float* mats, uint32_t n, const char* what) {
+  if (n && !mats) return fail(MPM_ERR_INVALID_ARGUMENT, "%s: materials NULL", what);
+  if (n > (uint32_t)kMatRows) return fail(MPM_ERR_INVALID_ARGUMENT, "%s: at most 8 materials (got %u)", what, n);
+  for (uint32_t k = 0; k < n; k++) {
+    uint32_t kind;
+    std::memcpy(&kind, mats + 12 * k, 4);
+    if (kind > 3) return fail(MPM_ERR_INVALID_ARGUMENT, "%s: material %u has kind %u (0-3)", what, k, kind);
+  }
+  return MPM_OK;
+}
+
+// Admits a scene: slot, node blocks, geometry, materials, then its records appended to the pool
+// from host memory (validated) or from a device template. Out-of-memory leaves the pool unchanged.
+mpm_status admit(mpm_pool* p, const int32_t dims[3], int32_t ppc, float bound, const float g[3], const float* mats,
+                 uint32_t nmat, const float* host_rec, const float4* dev_rec, uint32_t count, uint32_t* scene) {
+  if (p->needs_repack) ST_TRY(repack(p));
+  if ((uint64_t)p->count + count > p->cap)
+    return fail(MPM_ERR_OUT_OF_MEMORY, "pool: %u particles do not fit (%u of %u used)", count, p->count, p->cap);
+  uint32_t id = 0;
+  ST_TRY(add_scene(p, dims, ppc, bound, g, mats, nmat, &id));
+  mpm_status st = host_rec ? validate_records(p, id, host_rec, count, 0) : MPM_OK;
+  if (st == MPM_OK && count) {
+    if (host_rec) {
+      st = write_records(p, id, host_rec, p->count, count);
+    } else {
+      st = apply_pending_reorder(p);
+      if (st == MPM_OK) {
+        hipError_t e = launch_records_in(dev_rec, p->P[p->cur], p->cap, p->count, count, id * kMatRows, p->stream);
+        st = from_hip(e, "template admission");
+        p->bin_valid = false;
+      }
+    }
+  }
+  if (st != MPM_OK) {
+    release_blocks(p, p->scenes[id].geo.block_base, p->scenes[id].nblocks);
+    p->scenes[id] = SceneHost();
+    p->mem_order.pop_back();
+    return st;
+  }
+  p->scenes[id].offset = p->count;
+  p->scenes[id].count = count;
+  p->count += count;
+  p->grid_state = kGridClean;
+  drop_graphs(p);
+  *scene = id;
   return MPM_OK;
 }
 
@@ -1008,32 +1085,60 @@ mpm_status mpm_pool_add_scene(mpm_pool* p, const mpm_scene_desc* d, uint32_t* sc
   CHECK_POOL(p);
   if (!d || !scene) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_add_scene: NULL argument");
   if (d->count && !d->records) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_add_scene: records NULL");
-  if (d->num_materials && !d->materials) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_add_scene: materials NULL");
-  for (uint32_t k = 0; k < d->num_materials && k < (uint32_t)kMatRows; k++) {
-    uint32_t kind;
-    std::memcpy(&kind, d->materials + 12 * k, 4);
-    if (kind > 3) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_add_scene: material %u has kind %u (0-3)", k, kind);
+  ST_TRY(check_materials(d->materials, d->num_materials, "mpm_pool_add_scene"));
+  return admit(p, d->dims, d->ppc_axis, d->bound, d->gravity, d->materials, d->num_materials, d->records, nullptr,
+               d->count, scene);
+}
+
+mpm_status mpm_pool_add_template(mpm_pool* p, const mpm_scene_desc* d, uint32_t* tmpl) {
+  CHECK_POOL(p);
+  if (!d || !tmpl) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_add_template: NULL argument");
+  if (d->count && !d->records) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_add_template: records NULL");
+  ST_TRY(check_materials(d->materials, d->num_materials, "mpm_pool_add_template"));
+  for (int a = 0; a < 3; a++)
+    if (d->dims[a] < 8 || d->dims[a] > p->max_dim)
+      return fail(MPM_ERR_INVALID_ARGUMENT, "template: dims[%d] = %d outside [8, %d]", a, d->dims[a], p->max_dim);
+  ST_TRY(validate_positions(d->dims, d->records, d->count));
+  SceneTemplate t;
+  t.used = true;
+  std::memcpy(t.dims, d->dims, sizeof t.dims);
+  t.ppc_axis = d->ppc_axis;
+  t.bound = d->bound;
+  std::memcpy(t.gravity, d->gravity, sizeof t.gravity);
+  t.nmat = d->num_materials;
+  if (t.nmat) std::memcpy(t.mats, d->materials, 48 * (size_t)t.nmat);
+  t.count = d->count;
+  HIP_TRY(dmalloc(&t.rec, (size_t)d->count * 8));
+  hipError_t e = d->count ? hipMemcpy(t.rec, d->records, (size_t)d->count * 128, hipMemcpyHostToDevice) : hipSuccess;
+  if (e != hipSuccess) {
+    hipFree(t.rec);
+    return from_hip(e, "mpm_pool_add_template");
   }
-  if (p->needs_repack) ST_TRY(repack(p));
-  if ((uint64_t)p->count + d->count > p->cap)
-    return fail(MPM_ERR_OUT_OF_MEMORY, "pool: %u particles do not fit (%u of %u used)", d->count, p->count, p->cap);
-  uint32_t id = 0;
-  ST_TRY(add_scene(p, d->dims, d->ppc_axis, d->bound, d->gravity, d->materials, d->num_materials, &id));
-  mpm_status st = validate_records(p, id, d->records, d->count, 0);
-  if (st == MPM_OK && d->count) st = write_records(p, id, d->records, p->count, d->count);
-  if (st != MPM_OK) {
-    release_blocks(p, p->scenes[id].geo.block_base, p->scenes[id].nblocks);
-    p->scenes[id] = SceneHost();
-    p->mem_order.pop_back();
-    return st;
-  }
-  p->scenes[id].offset = p->count;
-  p->scenes[id].count = d->count;
-  p->count += d->count;
-  p->grid_state = kGridClean;
-  drop_graphs(p);
-  *scene = id;
+  uint32_t k = 0;
+  while (k < p->templates.size() && p->templates[k].used) k++;
+  if (k == p->templates.size()) p->templates.push_back(t);
+  else p->templates[k] = t;
+  *tmpl = k;
   return MPM_OK;
+}
+
+mpm_status mpm_pool_remove_template(mpm_pool* p, uint32_t tmpl) {
+  CHECK_POOL(p);
+  if (tmpl >= p->templates.size() || !p->templates[tmpl].used)
+    return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_remove_template: no template %u", tmpl);
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  hipFree(p->templates[tmpl].rec);
+  p->templates[tmpl] = SceneTemplate();
+  return MPM_OK;
+}
+
+mpm_status mpm_pool_add_scene_from_template(mpm_pool* p, uint32_t tmpl, uint32_t* scene) {
+  CHECK_POOL(p);
+  if (!scene) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_add_scene_from_template: NULL argument");
+  if (tmpl >= p->templates.size() || !p->templates[tmpl].used)
+    return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_add_scene_from_template: no template %u", tmpl);
+  const SceneTemplate& t = p->templates[tmpl];
+  return admit(p, t.dims, t.ppc_axis, t.bound, t.gravity, t.mats, t.nmat, nullptr, t.rec, t.count, scene);
 }
 
 mpm_status mpm_pool_remove_scene(mpm_pool* p, uint32_t scene) {
@@ -1102,6 +1207,70 @@ mpm_status mpm_pool_use_graphs(mpm_pool* p, int32_t enable) {
   p->use_graphs = enable != 0;
   if (!p->use_graphs) drop_graphs(p);
   return MPM_OK;
+}
+
+mpm_status mpm_pool_autotune(mpm_pool* p, uint32_t trial_substeps, int32_t* g2p_waves, int32_t* p2g_waves) {
+  CHECK_POOL(p);
+  if (trial_substeps == 0) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_autotune: trial_substeps = 0");
+  const uint32_t trial = (trial_substeps + p->sort_every - 1) / p->sort_every * p->sort_every;
+  const bool graphs = p->use_graphs, prof = p->profiling;
+  p->use_graphs = false;
+  p->profiling = false;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  float best = 1e30f;
+  int bg = p->g2p_waves, bp = p->p2g_waves;
+  mpm_status st = MPM_OK;
+  // align to a sort boundary so every trial covers the same phases
+  while (st == MPM_OK && p->step_count % p->sort_every != 0) st = pool_step(p, 1, 0.9f);
+  for (int g = 3; g <= 6 && st == MPM_OK; g++)
+    for (int q = 4; q <= 6 && st == MPM_OK; q++) {
+      p->g2p_waves = g;
+      p->p2g_waves = q;
+      hipEventRecord(e0, p->stream);
+      st = pool_step(p, trial, 0.9f);
+      hipEventRecord(e1, p->stream);
+      if (st != MPM_OK) break;
+      float ms = 0;
+      hipEventSynchronize(e1);
+      hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) { best = ms; bg = g; bp = q; }
+    }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  p->g2p_waves = bg;
+  p->p2g_waves = bp;
+  p->use_graphs = graphs;
+  p->profiling = prof;
+  drop_graphs(p);
+  if (g2p_waves) *g2p_waves = bg;
+  if (p2g_waves) *p2g_waves = bp;
+  return st;
+}
+
+mpm_status mpm_pool_diagnostics(mpm_pool* p, mpm_diagnostics* out) {
+  CHECK_POOL(p);
+  if (!out) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_diagnostics: NULL argument");
+  uint32_t c[kCounters];
+  std::vector<uint32_t> a((size_t)kArenaRegions * kArenaStride);
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  HIP_TRY(hipMemcpy(c, p->ctr, sizeof c, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(a.data(), p->arena_ctr, a.size() * 4, hipMemcpyDeviceToHost));
+  uint64_t used = 0;
+  for (int r = 0; r < kArenaRegions; r++) used += a[(size_t)r * kArenaStride];
+  out->work_items = c[kCtrItems];
+  out->s_blocks = c[kCtrS];
+  out->d_blocks = c[kCtrD];
+  out->tile_nodes_used = (uint32_t)std::min<uint64_t>(used, 0xffffffffu);
+  out->tile_nodes_capacity = p->tile_nodes;
+  out->particles = p->count;
+  return MPM_OK;
+}
+
+mpm_status mpm_diagnostics_of(mpm_sim* s, mpm_diagnostics* out) {
+  if (!s) return fail(MPM_ERR_INVALID_ARGUMENT, "simulation is NULL");
+  return mpm_pool_diagnostics(s->pool, out);
 }
 
 mpm_status mpm_pool_set_tuning(mpm_pool* p, int32_t knob, int32_t value) {

@@ -68,6 +68,30 @@ class _SceneDesc(ctypes.Structure):
                 ("records", ctypes.c_void_p), ("count", ctypes.c_uint32)]
 
 
+class _Diagnostics(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint32) for k in ("work_items", "s_blocks", "d_blocks", "tile_nodes_used",
+                                                "tile_nodes_capacity", "particles")]
+
+
+def diagnostics(handle, pool: bool = False) -> dict:
+    """Device counters of the last binning and P2G of a simulation (or pool) handle."""
+    d = _Diagnostics()
+    L = load_library()
+    _check((L.mpm_pool_diagnostics if pool else L.mpm_diagnostics_of)(handle, ctypes.byref(d)), "diagnostics")
+    return {k: getattr(d, k) for k, _ in _Diagnostics._fields_}
+
+
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("substeps", ctypes.c_uint32), ("floor_friction", ctypes.c_float), ("vmax", ctypes.c_float)]
+
+
+class _ServerCounters(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
+                ("waiting", ctypes.c_uint64), ("running", ctypes.c_uint64), ("steps", ctypes.c_uint64),
+                ("particle_substeps", ctypes.c_uint64), ("resident_particles", ctypes.c_uint32),
+                ("error", ctypes.c_int32)]
+
+
 _lib = None
 
 
@@ -121,6 +145,18 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "mpm_pool_sync": (i32, [vp]),
         "mpm_pool_use_graphs": (i32, [vp, i32]),
         "mpm_pool_set_tuning": (i32, [vp, i32, i32]),
+        "mpm_pool_autotune": (i32, [vp, u32, P(i32), P(i32)]),
+        "mpm_pool_diagnostics": (i32, [vp, P(_Diagnostics)]),
+        "mpm_diagnostics_of": (i32, [vp, P(_Diagnostics)]),
+        "mpm_pool_add_template": (i32, [vp, P(_SceneDesc), P(u32)]),
+        "mpm_pool_remove_template": (i32, [vp, u32]),
+        "mpm_pool_add_scene_from_template": (i32, [vp, u32, P(u32)]),
+        "mpm_server_create": (i32, [vp, P(_ServerConfig), P(vp)]),
+        "mpm_server_submit": (i32, [vp, u32, u32, P(u64)]),
+        "mpm_server_stats": (i32, [vp, P(_ServerCounters)]),
+        "mpm_server_latencies": (i32, [vp, i32, vp, u64, P(u64)]),
+        "mpm_server_stop": (i32, [vp, i32]),
+        "mpm_server_destroy": (None, [vp]),
         "mpm_status_string": (ctypes.c_char_p, [i32]),
         "mpm_last_error_message": (ctypes.c_char_p, []),
     }
@@ -525,6 +561,39 @@ class ScenePool:
         _check(load_library().mpm_pool_add_scene(self._h, ctypes.byref(d), ctypes.byref(sid)), "ScenePool.add_scene")
         return sid.value
 
+    def _desc(self, dims, materials, records, gravity, ppc_axis, bound):
+        mats = np.ascontiguousarray(materials, np.float32).reshape(-1, 12)
+        rec = np.ascontiguousarray(records, np.float32).reshape(-1, 32)
+        d = _SceneDesc((ctypes.c_int32 * 3)(*[int(x) for x in dims]), int(ppc_axis), float(bound),
+                       (ctypes.c_float * 3)(*[float(x) for x in gravity]), len(mats),
+                       mats.ctypes.data if len(mats) else None, rec.ctypes.data if len(rec) else None, len(rec))
+        return d, (mats, rec)
+
+    def add_template(self, dims, materials: np.ndarray, records: np.ndarray, gravity, ppc_axis: int = 2,
+                     bound: float = 3.0) -> int:
+        """Keeps a scene on the device for repeated admission; returns the template id."""
+        self._alive("ScenePool.add_template")
+        d, keep = self._desc(dims, materials, records, gravity, ppc_axis, bound)
+        tid = ctypes.c_uint32()
+        _check(load_library().mpm_pool_add_template(self._h, ctypes.byref(d), ctypes.byref(tid)),
+               "ScenePool.add_template")
+        return tid.value
+
+    def add_from_template(self, tmpl: int) -> int:
+        self._alive("ScenePool.add_from_template")
+        sid = ctypes.c_uint32()
+        _check(load_library().mpm_pool_add_scene_from_template(self._h, int(tmpl), ctypes.byref(sid)),
+               "ScenePool.add_from_template")
+        return sid.value
+
+    def autotune(self, trial_substeps: int = 8) -> dict:
+        """Times the occupancy variants on the live pool (it advances by the trials) and keeps the best."""
+        self._alive("ScenePool.autotune")
+        g, p = ctypes.c_int32(), ctypes.c_int32()
+        _check(load_library().mpm_pool_autotune(self._h, int(trial_substeps), ctypes.byref(g), ctypes.byref(p)),
+               "ScenePool.autotune")
+        return {"g2p_waves": g.value, "p2g_waves": p.value}
+
     def remove_scene(self, scene: int) -> None:
         self._alive("ScenePool.remove_scene")
         _check(load_library().mpm_pool_remove_scene(self._h, int(scene)), "ScenePool.remove_scene")
@@ -566,4 +635,46 @@ class ScenePool:
     def destroy(self) -> None:
         if self._h:
             load_library().mpm_pool_destroy(self._h)
+            self._h = None
+
+
+class Server:
+    """The native serving loop over a ScenePool (include/mpm.h, mpm_server_*). While it runs, its
+    thread owns the pool."""
+
+    def __init__(self, pool: ScenePool, substeps: int = 4, floor_friction: float = 0.4, vmax: float = 0.9):
+        self.pool = pool
+        cfg = _ServerConfig(int(substeps), float(floor_friction), float(vmax))
+        h = ctypes.c_void_p()
+        _check(load_library().mpm_server_create(pool._h, ctypes.byref(cfg), ctypes.byref(h)), "Server")
+        self._h = h
+
+    def submit(self, tmpl: int, lifetime_steps: int) -> int:
+        t = ctypes.c_uint64()
+        _check(load_library().mpm_server_submit(self._h, int(tmpl), int(lifetime_steps), ctypes.byref(t)),
+               "Server.submit")
+        return t.value
+
+    def stats(self) -> dict:
+        c = _ServerCounters()
+        _check(load_library().mpm_server_stats(self._h, ctypes.byref(c)), "Server.stats")
+        return {k: getattr(c, k) for k, _ in _ServerCounters._fields_}
+
+    def latencies(self, kind: str) -> np.ndarray:
+        code = {"step_ms": 0, "request_s": 1, "wait_s": 2}[kind]
+        n = ctypes.c_uint64()
+        L = load_library()
+        _check(L.mpm_server_latencies(self._h, code, None, 0, ctypes.byref(n)), "Server.latencies")
+        out = np.zeros(n.value, np.float64)
+        if n.value:
+            _check(L.mpm_server_latencies(self._h, code, _ptr(out), n.value, ctypes.byref(n)), "Server.latencies")
+        return out[:n.value]
+
+    def stop(self, drain: bool = False) -> None:
+        if self._h:
+            _check(load_library().mpm_server_stop(self._h, 1 if drain else 0), "Server.stop")
+
+    def destroy(self) -> None:
+        if self._h:
+            load_library().mpm_server_destroy(self._h)
             self._h = None

@@ -120,3 +120,31 @@ def random_blocks(rng: np.random.Generator, grid: int, max_blocks: int = 3, max_
         vel = [float(v) for v in rng.uniform(-8, 8, size=3)]
         blocks.append((mat, lo, hi, vel, float(rng.uniform(0, 0.2))))
     return blocks
+
+
+BATCH_EDGES = (32, 48, 64, 96, 128)
+BATCH_MATERIALS = ("water", "jelly", "snow", SAND30)
+
+
+def batched_blocks(seed: int = 1, count: int = 64):
+    """The batched workload: `count` independent scenes drawn from one master seed. Each has a cubic
+    grid with edge in {32, 48, 64, 96, 128}, one box of water, jelly, snow or sand (30 deg) holding N
+    particles with N log-uniform in [1e4, 1e6] and at most 8 per cell of the interior (edge - 6)^3,
+    placed at a random position with a random velocity (|v_i| <= 4 cells/s). Returns a list of
+    (edge, blocks) for host_blocks."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        edge = int(BATCH_EDGES[int(rng.integers(len(BATCH_EDGES)))])
+        n = int(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
+        n = min(n, 8 * (edge - 6) ** 3)
+        cells = max(1, n // 8)
+        side = min(edge - 6, max(1, int(round(cells ** (1 / 3)))))
+        height = min(edge - 6, max(1, int(round(cells / (side * side)))))
+        size = (side, height, side)
+        lo = [int(rng.integers(3, edge - 3 - s + 1)) for s in size]
+        hi = [l + s for l, s in zip(lo, size)]
+        mat = BATCH_MATERIALS[int(rng.integers(len(BATCH_MATERIALS)))]
+        vel = tuple(float(v) for v in rng.uniform(-4, 4, 3))
+        out.append((edge, [(mat, lo, hi, vel, 0.0)]))
+    return out

@@ -1,0 +1,80 @@
+// CPU check of the serving scheduler (shaderintel_amd/csrc/serve_sched.hpp) against a fake pool with
+// a small particle budget: FIFO admission without overtaking, retirement after the lifetime, rejection
+// of requests that cannot fit an empty pool, and the latency bookkeeping. Exit status 0 = pass.
+#include <cstdio>
+#include <map>
+#include <vector>
+
+#include "serve_sched.hpp"
+
+using namespace mpm_serve;
+
+#define CHECK(c)                                                        \
+  do {                                                                  \
+    if (!(c)) {                                                         \
+      std::fprintf(stderr, "%s:%d: check failed: %s\n", __FILE__, __LINE__, #c); \
+      return 1;                                                         \
+    }                                                                   \
+  } while (0)
+
+struct FakePool {
+  uint32_t budget, used = 0, next_scene = 0;
+  std::map<uint32_t, uint32_t> size_of_tmpl, scene_size;
+  std::vector<uint32_t> admitted_tmpl;
+  int try_admit(const Request& r, bool empty, uint32_t* scene) {
+    const uint32_t n = size_of_tmpl[r.tmpl];
+    if (used + n > budget) return empty ? kNeverFits : kNoRoom;
+    used += n;
+    *scene = next_scene++;
+    scene_size[*scene] = n;
+    admitted_tmpl.push_back(r.tmpl);
+    return kAdmitted;
+  }
+  void remove(uint32_t scene) { used -= scene_size[scene]; scene_size.erase(scene); }
+};
+
+int main() {
+  FakePool pool{100};
+  pool.size_of_tmpl = {{0, 60}, {1, 50}, {2, 30}, {3, 200}, {4, 10}};
+  Scheduler s;
+  auto admit = [&](double t) {
+    return s.admit(t, [&](const Request& r, bool empty, uint32_t* sc) { return pool.try_admit(r, empty, sc); });
+  };
+  auto step = [&](double t) { return s.after_step(t, [&](uint32_t sc) { pool.remove(sc); }); };
+
+  s.submit(0, 2, 0.0);  // 60 particles, 2 steps
+  s.submit(1, 1, 0.1);  // 50: does not fit next to the 60
+  s.submit(2, 1, 0.2);  // 30 would fit, but must not overtake
+  CHECK(admit(1.0) == 1);
+  CHECK(s.running() == 1 && s.waiting() == 2 && pool.used == 60);
+  CHECK(step(2.0) == 0);
+  CHECK(admit(2.0) == 0);  // still blocked behind the 50
+  CHECK(step(3.0) == 1);   // the 60 retires after its 2 steps
+  CHECK(pool.used == 0);
+  CHECK(admit(3.0) == 2);  // 50 + 30 now
+  CHECK(pool.admitted_tmpl.size() == 3 && pool.admitted_tmpl[1] == 1 && pool.admitted_tmpl[2] == 2);
+  CHECK(step(4.0) == 2);
+  CHECK(s.running() == 0 && s.waiting() == 0 && pool.used == 0);
+  // a request larger than an empty pool is rejected; the one behind it proceeds
+  s.submit(3, 5, 5.0);
+  s.submit(4, 1, 5.0);
+  CHECK(admit(6.0) == 1);
+  CHECK(s.rejected().size() == 1 && s.rejected()[0] == 3);
+  CHECK(step(7.0) == 1);
+  // latency bookkeeping: ids 0..2 and 4 completed
+  const auto& c = s.completed();
+  CHECK(c.size() == 4);
+  CHECK(c[0].id == 0 && c[0].wait_s == 1.0 && c[0].total_s == 3.0 && c[0].steps == 2);
+  CHECK(c[1].id == 1 && c[1].wait_s > 2.89 && c[1].wait_s < 2.91 && c[1].total_s > 3.89 && c[1].total_s < 3.91);
+  CHECK(c[3].id == 4 && c[3].wait_s == 1.0 && c[3].total_s == 2.0);
+  CHECK(s.submitted() == 5);
+  // a full pool admits as soon as room appears, in order, many at once
+  for (int k = 0; k < 10; k++) s.submit(4, 1 + k % 3, 10.0);  // ten scenes of 10
+  CHECK(admit(10.0) == 10 && pool.used == 100);
+  s.submit(2, 1, 10.5);
+  CHECK(admit(11.0) == 0);
+  CHECK(step(12.0) == 4);  // lifetimes 1 retire: k = 0, 3, 6, 9
+  CHECK(admit(12.0) == 1 && pool.used == 90);
+  std::printf("ok\n");
+  return 0;
+}

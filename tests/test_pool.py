@@ -148,3 +148,70 @@ def test_pool_admission_limits(ctx):
     sc.step(4, g)
     _same(pool.read_scene(a), sc.read(len(r)), "after refusals")
     pool.destroy()
+
+
+def test_template_admission_matches_host_admission(ctx):
+    """Scenes admitted from device templates equal scenes admitted from host records."""
+    _say("[gpu] template admission vs host admission")
+    built = _build_specs()[:3]
+    total = sum(len(b[2]) for b in built)
+    nb = sum(node_blocks(b[0]) for b in built)
+    a = ScenePool(ctx, total, 4, nb, 64)
+    b = ScenePool(ctx, total, 4, nb, 64)
+    ia = [a.add_scene(d, t, r, g) for d, t, r, g in built]
+    tids = [b.add_template(d, t, r, g) for d, t, r, g in built]
+    ib = [b.add_from_template(k) for k in tids]
+    a.step(6)
+    b.step(6)
+    for x, y in zip(ia, ib):
+        _same(b.read_scene(y), a.read_scene(x), "template scene")
+    a.destroy()
+    b.destroy()
+
+
+def test_autotune_is_result_neutral(ctx):
+    """The autotuner's trials are real steps: a tuned pool equals an untuned pool stepped as far."""
+    _say("[gpu] autotune result-neutral")
+    built = _build_specs()[:2]
+    total = sum(len(b[2]) for b in built)
+    nb = sum(node_blocks(b[0]) for b in built)
+    a = ScenePool(ctx, total, 2, nb, 64)
+    b = ScenePool(ctx, total, 2, nb, 64)
+    ia = [a.add_scene(d, t, r, g) for d, t, r, g in built]
+    ib = [b.add_scene(d, t, r, g) for d, t, r, g in built]
+    best = a.autotune(4)
+    assert 3 <= best["g2p_waves"] <= 6 and 4 <= best["p2g_waves"] <= 6
+    b.step(12 * 4)
+    for x, y in zip(ia, ib):
+        _same(a.read_scene(x), b.read_scene(y), "tuned vs untuned")
+    a.destroy()
+    b.destroy()
+
+
+def test_server_completes_requests_and_rejects_oversize(ctx):
+    """The serving loop admits, steps and retires every request; one that cannot fit is rejected."""
+    _say("[gpu] serving loop: 12 requests, drain")
+    from shaderintel_amd.mpm import Server
+    built = _build_specs()[:3]
+    big = max(len(b[2]) for b in built)
+    pool = ScenePool(ctx, big + 4000, 8, 2 * max(node_blocks(b[0]) for b in built), 64)
+    tids = [pool.add_template(d, t, r, g) for d, t, r, g in built]
+    dims, tbl, rec, g = built[0]
+    huge = pool.add_template(dims, tbl, np.concatenate([rec] * 20), g)  # more particles than the pool holds
+    srv = Server(pool, substeps=4)
+    for k in range(12):
+        srv.submit(tids[k % 3], 1 + k % 4)
+    srv.submit(huge, 2)
+    srv.stop(drain=True)
+    st = srv.stats()
+    assert st["error"] == 0
+    assert st["submitted"] == 13 and st["completed"] == 12 and st["rejected"] == 1
+    assert st["waiting"] == 0 and st["running"] == 0 and st["resident_particles"] == 0
+    assert st["steps"] >= 4 and st["particle_substeps"] > 0
+    assert len(srv.latencies("step_ms")) == st["steps"]
+    req = srv.latencies("request_s")
+    wait = srv.latencies("wait_s")
+    assert len(req) == 12 and np.all(req >= wait) and np.all(wait >= 0)
+    srv.destroy()
+    assert pool.info()["scenes"] == 0
+    pool.destroy()
