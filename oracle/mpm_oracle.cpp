@@ -7,8 +7,11 @@
 //
 // WGSL leaves FMA contraction and the association of mat*vec / mat*mat to the browser's compiler,
 // so "the reference's bits" are adapter-defined. This file fixes ONE faithful reading of each
-// expression: left-to-right evaluation, mat*vec as ((m0*v.x + m1*v.y) + m2*v.z), dot as
-// ((a.x*b.x + a.y*b.y) + a.z*b.z), no contraction (-ffp-contract=off), IEEE division and sqrt,
+// expression: left-to-right evaluation, mat*vec as ((m0*v.x + m1*v.y) + m2*v.z) and dot as
+// ((a.x*b.x + a.y*b.y) + a.z*b.z) with both additions contracted to fused multiply-adds
+// (fma(m2, v.z, fma(m1, v.y, m0*v.x))), cross products as fma(a, b, -(c*d)), G2P's v += wt*gv and
+// B += outer(wt*gv, dpos) contracted element by element, and no other contraction (explicit fmaf
+// calls; the compiler contracts nothing: -ffp-contract=off), IEEE division and sqrt,
 // WGSL clamp(e,lo,hi) = min(max(e,lo),hi) with NaN-discarding min/max, WGSL round = rint (half to
 // even). inverseSqrt(x) is read as 1/sqrt(x); log/exp are the self-contained routines below.
 //
@@ -109,19 +112,23 @@ inline V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 inline V3 scale(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }  // a * s
 inline V3 scale(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }  // s * a
-inline float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+inline float dot(V3 a, V3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
 inline float length(V3 a) { return sqrtf(dot(a, a)); }
 inline V3 divs(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
 inline V3 normalize(V3 a) { return divs(a, length(a)); }
-inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline V3 cross(V3 a, V3 b) {
+  return {fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)), fmaf(a.x, b.y, -(a.y * b.x))};
+}
 inline float fminw(float a, float b) { return fminf(a, b); }
 inline float fmaxw(float a, float b) { return fmaxf(a, b); }
 inline float clampw(float e, float lo, float hi) { return fminf(fmaxf(e, lo), hi); }
 
 inline M3 I3() { return {{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}}; }
 inline M3 mzero() { return {{{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}}; }
-inline V3 mulmv(const M3& m, V3 v) {  // m * v
-  return add(add(scale(m.c[0], v.x), scale(m.c[1], v.y)), scale(m.c[2], v.z));
+inline V3 mulmv(const M3& m, V3 v) {  // m * v = (m0 v.x + m1 v.y) + m2 v.z, both additions contracted
+  return {fmaf(m.c[2].x, v.z, fmaf(m.c[1].x, v.y, m.c[0].x * v.x)),
+          fmaf(m.c[2].y, v.z, fmaf(m.c[1].y, v.y, m.c[0].y * v.x)),
+          fmaf(m.c[2].z, v.z, fmaf(m.c[1].z, v.y, m.c[0].z * v.x))};
 }
 inline M3 mulmm(const M3& a, const M3& b) {  // a * b: column j = a * b[j]
   return {{mulmv(a, b.c[0]), mulmv(a, b.c[1]), mulmv(a, b.c[2])}};
@@ -158,17 +165,17 @@ void jacobi_rotate(M3& A, M3& V, int p, int q) {
   // M = J^T A: rows p and q of every column
   for (int j = 0; j < 3; j++) {
     float ap = at(A, j, p), aq = at(A, j, q);
-    at(A, j, p) = c * ap + ms * aq;
-    at(A, j, q) = s * ap + c * aq;
+    at(A, j, p) = fmaf(ms, aq, c * ap);
+    at(A, j, q) = fmaf(c, aq, s * ap);
   }
   // A = M J and V = V J: columns p and q
   for (int i = 0; i < 3; i++) {
     float mp = at(A, p, i), mq = at(A, q, i);
-    at(A, p, i) = mp * c + mq * ms;
-    at(A, q, i) = mp * s + mq * c;
+    at(A, p, i) = fmaf(mq, ms, mp * c);
+    at(A, q, i) = fmaf(mq, c, mp * s);
     float vp = at(V, p, i), vq = at(V, q, i);
-    at(V, p, i) = vp * c + vq * ms;
-    at(V, q, i) = vp * s + vq * c;
+    at(V, p, i) = fmaf(vq, ms, vp * c);
+    at(V, q, i) = fmaf(vq, c, vp * s);
   }
 }
 
@@ -380,8 +387,11 @@ inline void g2p_particle(float* p, const float* mats, const int dims[3], float b
         V3 dpos = sub(v3((float)a, (float)b, (float)c), fx);
         float wt = (w[a].x * w[b].y) * w[c].z;
         V3 gv = gvat(base[0] + a, base[1] + b, base[2] + c);
-        v = add(v, scale(wt, gv));
-        B = madd(B, outer3(scale(wt, gv), dpos));
+        // v += wt * gv and B += outer(wt * gv, dpos), every multiply-add contracted
+        v = v3(fmaf(wt, gv.x, v.x), fmaf(wt, gv.y, v.y), fmaf(wt, gv.z, v.z));
+        V3 wgv = scale(wt, gv);
+        for (int j = 0; j < 3; j++)
+          for (int r = 0; r < 3; r++) at(B, j, r) = fmaf(comp(wgv, r), comp(dpos, j), at(B, j, r));
       }
   M3 C = mscale(4.0f, B);
   float sp = length(v);

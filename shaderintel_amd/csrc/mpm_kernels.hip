@@ -76,8 +76,13 @@ __device__ __forceinline__ void p2g_particle(const float4 r[8], const float* __r
         const float wy = b == 0 ? w[0].y : (b == 1 ? w[1].y : w[2].y);
         const float wz = c == 0 ? w[0].z : (c == 1 ? w[1].z : w[2].z);
         const float wt = (wx * wy) * wz;
-        const f3 mom = wt * (mv + mul(affine, dpos));
-        add(bx + a, by + b, bz + c, to_fixed(mom.x), to_fixed(mom.y), to_fixed(mom.z), to_fixed(wt * mass));
+        // toFixed(wt * q) = rint(clamp(wt * q * 2^22)): the 2^22 scaling is exact, so folding it into the
+        // weight (wts = wt * 2^22) gives the same float for every q whose product is normal, and one that
+        // rounds to the same integer (0) when wt * q is subnormal; overflow clamps alike.
+        const float wts = wt * kFP;
+        const f3 mom = wts * (mv + mul(affine, dpos));
+        add(bx + a, by + b, bz + c, fixed_of_scaled(mom.x), fixed_of_scaled(mom.y), fixed_of_scaled(mom.z),
+            fixed_of_scaled(wts * mass));
       }
 }
 
@@ -155,9 +160,11 @@ __device__ __forceinline__ float4 grid_node(int4 q, int ix, int iy, int iz, cons
 
 // G2P for particle i (mpm.js:479-559): gather velocity and APIC C, advect, update F, plasticity,
 // stress. Writes the new record to the 8 streams and returns it in r.
-__device__ __forceinline__ void g2p_particle(const float* __restrict__ mats, const float4* __restrict__ V,
-                                             const SceneGeo& geo, float4* __restrict__ P, uint32_t cap, uint32_t i,
-                                             float vmax, float4 r[8]) {
+// gather(bx, by, bz, a, b, c) returns the grid velocity (and mass) of node (bx + a, by + b, bz + c).
+template <class Gather>
+__device__ __forceinline__ void g2p_particle(const float* __restrict__ mats, const SceneGeo& geo,
+                                             float4* __restrict__ P, uint32_t cap, uint32_t i, float vmax,
+                                             float4 r[8], Gather&& gather) {
   const float4 x4 = P[i];
   const float* m0 = mats + 12 * __float_as_uint(x4.w);
   const float bound = geo.bound;
@@ -180,10 +187,10 @@ __device__ __forceinline__ void g2p_particle(const float* __restrict__ mats, con
         const float wy = b == 0 ? w[0].y : (b == 1 ? w[1].y : w[2].y);
         const float wz = c == 0 ? w[0].z : (c == 1 ? w[1].z : w[2].z);
         const float wt = (wx * wy) * wz;
-        const float4 g4 = V[node_addr(bx + a, by + b, bz + c, geo)];
+        const float4 g4 = gather(bx, by, bz, a, b, c);
         const f3 gv = mk(g4.x, g4.y, g4.z);
-        v = v + wt * gv;
-        B = B + outer(wt * gv, dpos);
+        v = mk(fmaf(wt, gv.x, v.x), fmaf(wt, gv.y, v.y), fmaf(wt, gv.z, v.z));  // v += wt * gv, contracted
+        B = add_outer(B, wt * gv, dpos);
       }
   const m3 C = 4.0f * B;
   const float sp = len3(v);
@@ -260,7 +267,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
   float4 r[8];
-  g2p_particle(pool.mats, pool.V, pool.scenes[__float_as_uint(P[i].w) >> 3], P, pool.cap, i, vmax, r);
+  const SceneGeo& geo = pool.scenes[__float_as_uint(P[i].w) >> 3];
+  const float4* __restrict__ V = pool.V;
+  g2p_particle(pool.mats, geo, P, pool.cap, i, vmax, r,
+               [&](int bx, int by, int bz, int a, int b, int c) { return V[node_addr(bx + a, by + b, bz + c, geo)]; });
 }
 
 __global__ void k_records_in(const float4* __restrict__ aos, float4* __restrict__ P, uint32_t cap, uint32_t offset,

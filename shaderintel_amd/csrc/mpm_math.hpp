@@ -1,10 +1,13 @@
 // Device math for the MLS-MPM kernels (gfx950).
 //
 // Every expression is evaluated in a fixed, documented order so that the GPU result is a pure
-// function of the input bits: no contraction into FMA (the library is compiled with
-// -ffp-contract=off), IEEE division and square root (HIP's default correctly rounded fp32
-// div/sqrt), mat*vec as ((m0*v.x + m1*v.y) + m2*v.z), dot as ((a.x*b.x + a.y*b.y) + a.z*b.z),
-// WGSL clamp(e, lo, hi) = min(max(e, lo), hi), WGSL round = round-half-to-even (rint), and
+// function of the input bits. The compiler contracts nothing (-ffp-contract=off); the fused
+// multiply-adds WGSL lets an implementation form are written out as fmaf where this build forms
+// them: mat*vec ((m0*v.x + m1*v.y) + m2*v.z) and dot ((a.x*b.x + a.y*b.y) + a.z*b.z) with both
+// additions fused, cross products as fma(a, b, -(c*d)), and G2P's v += wt*gv and
+// B += outer(wt*gv, dpos) element by element. IEEE division and square root (HIP's default correctly
+// rounded fp32 div/sqrt), WGSL clamp(e, lo, hi) = min(max(e, lo), hi), WGSL round =
+// round-half-to-even (rint), and
 // inverseSqrt read as 1/sqrt. exp and log are the self-contained Cody-Waite routines below, so
 // no vendor math library decides any bit of the result.
 //
@@ -27,16 +30,22 @@ MPM_HD f3 operator-(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
 MPM_HD f3 operator*(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
 MPM_HD f3 operator*(float s, f3 a) { return mk(s * a.x, s * a.y, s * a.z); }
 MPM_HD f3 operator/(f3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
-MPM_HD float dot3(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+MPM_HD float dot3(f3 a, f3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
 MPM_HD float len3(f3 a) { return sqrtf(dot3(a, a)); }
 MPM_HD f3 norm3(f3 a) { return a / len3(a); }
-MPM_HD f3 cross3(f3 a, f3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+MPM_HD f3 cross3(f3 a, f3 b) {
+  return mk(fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)), fmaf(a.x, b.y, -(a.y * b.x)));
+}
 MPM_HD float clampf(float e, float lo, float hi) { return fminf(fmaxf(e, lo), hi); }
 
 MPM_HD m3 mat(f3 a, f3 b, f3 c) { m3 m; m.c[0] = a; m.c[1] = b; m.c[2] = c; return m; }
 MPM_HD m3 ident() { return mat(mk(1, 0, 0), mk(0, 1, 0), mk(0, 0, 1)); }
 MPM_HD m3 zero3() { return mat(mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)); }
-MPM_HD f3 mul(const m3& m, f3 v) { return (m.c[0] * v.x + m.c[1] * v.y) + m.c[2] * v.z; }
+MPM_HD f3 mul(const m3& m, f3 v) {
+  return mk(fmaf(m.c[2].x, v.z, fmaf(m.c[1].x, v.y, m.c[0].x * v.x)),
+            fmaf(m.c[2].y, v.z, fmaf(m.c[1].y, v.y, m.c[0].y * v.x)),
+            fmaf(m.c[2].z, v.z, fmaf(m.c[1].z, v.y, m.c[0].z * v.x)));
+}
 MPM_HD m3 mul(const m3& a, const m3& b) { return mat(mul(a, b.c[0]), mul(a, b.c[1]), mul(a, b.c[2])); }
 MPM_HD m3 operator+(const m3& a, const m3& b) { return mat(a.c[0] + b.c[0], a.c[1] + b.c[1], a.c[2] + b.c[2]); }
 MPM_HD m3 operator-(const m3& a, const m3& b) { return mat(a.c[0] - b.c[0], a.c[1] - b.c[1], a.c[2] - b.c[2]); }
@@ -47,6 +56,12 @@ MPM_HD m3 tr(const m3& m) {
 MPM_HD float trace(const m3& m) { return (m.c[0].x + m.c[1].y) + m.c[2].z; }
 MPM_HD m3 diag(f3 d) { return mat(mk(d.x, 0, 0), mk(0, d.y, 0), mk(0, 0, d.z)); }
 MPM_HD m3 outer(f3 a, f3 b) { return mat(a * b.x, a * b.y, a * b.z); }
+// B + outer(a, b), each element contracted: B[j][r] = fma(a[r], b[j], B[j][r]).
+MPM_HD m3 add_outer(const m3& B, f3 a, f3 b) {
+  return mat(mk(fmaf(a.x, b.x, B.c[0].x), fmaf(a.y, b.x, B.c[0].y), fmaf(a.z, b.x, B.c[0].z)),
+             mk(fmaf(a.x, b.y, B.c[1].x), fmaf(a.y, b.y, B.c[1].y), fmaf(a.z, b.y, B.c[1].z)),
+             mk(fmaf(a.x, b.z, B.c[2].x), fmaf(a.y, b.z, B.c[2].y), fmaf(a.z, b.z, B.c[2].z)));
+}
 MPM_HD float det3(const m3& m) { return dot3(m.c[0], cross3(m.c[1], m.c[2])); }
 
 // ---- elementary functions (Cody & Waite 1980 reduction + Taylor / atanh series in float) ----
@@ -107,6 +122,8 @@ constexpr float kFP = 4194304.0f;           // 2^22
 constexpr float kInvFP = 1.0f / 4194304.0f;
 
 MPM_HD int32_t to_fixed(float x) { return (int32_t)rintf(clampf(x * kFP, -2.0e9f, 2.0e9f)); }
+// to_fixed of a value already scaled by 2^22
+MPM_HD int32_t fixed_of_scaled(float xs) { return (int32_t)rintf(clampf(xs, -2.0e9f, 2.0e9f)); }
 
 MPM_HD void bspline(f3 fx, f3 w[3]) {
   f3 t0 = mk(1.5f, 1.5f, 1.5f) - fx;
@@ -132,8 +149,8 @@ MPM_HD float& el(m3& m, int c, int r) { return r == 0 ? m.c[c].x : (r == 1 ? m.c
 MPM_HD float el(const m3& m, int c, int r) { return r == 0 ? m.c[c].x : (r == 1 ? m.c[c].y : m.c[c].z); }
 
 // One cyclic-Jacobi rotation in the (P, Q) plane: A <- J^T A J, V <- V J. Only rows / columns P and
-// Q change, each as the two-term sum the full 3x3 products would form, in the same order (their other
-// terms are multiples of zero).
+// Q change, each as the two-term sum the full 3x3 products would form (mul above: the first nonzero
+// product rounded, the second fused onto it; their other terms are multiples of zero).
 template <int P, int Q>
 MPM_HD void jacobi(m3& A, m3& V) {
   float apq = el(A, Q, P);
@@ -147,17 +164,17 @@ MPM_HD void jacobi(m3& A, m3& V) {
 #pragma unroll
   for (int j = 0; j < 3; j++) {
     const float ap = el(A, j, P), aq = el(A, j, Q);
-    el(A, j, P) = c * ap + ms * aq;
-    el(A, j, Q) = s * ap + c * aq;
+    el(A, j, P) = fmaf(ms, aq, c * ap);
+    el(A, j, Q) = fmaf(c, aq, s * ap);
   }
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     const float mp = el(A, P, i), mq = el(A, Q, i);
-    el(A, P, i) = mp * c + mq * ms;
-    el(A, Q, i) = mp * s + mq * c;
+    el(A, P, i) = fmaf(mq, ms, mp * c);
+    el(A, Q, i) = fmaf(mq, c, mp * s);
     const float vp = el(V, P, i), vq = el(V, Q, i);
-    el(V, P, i) = vp * c + vq * ms;
-    el(V, Q, i) = vp * s + vq * c;
+    el(V, P, i) = fmaf(vq, ms, vp * c);
+    el(V, Q, i) = fmaf(vq, c, vp * s);
   }
 }
 
