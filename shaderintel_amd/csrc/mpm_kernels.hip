@@ -403,13 +403,19 @@ __global__ void k_bin_emit(Pool pool, const uint32_t* __restrict__ keys, uint32_
 }
 
 // Within-block transposition of the sorted order: the particles of item k, sorted by cell, are
-// reordered by (rank within cell, cell). pos(r, c) = sum_c' min(n_c', r) + #{c' < c : n_c' > r} over
-// the 64 cells c' of the block with counts n_c'. out[start + pos] = in[start + j].
+// reordered by (rank within cell, cell). pos(r, c) = S(r) + #{c' < c : n_c' > r} with
+// S(r) = sum_c' min(n_c', r) over the 64 cells c' of the block with counts n_c'; S(r) and the mask of
+// cells with n_c' > r are tabulated per item for r < kRankTable (a cell holds ~8 particles), so a
+// particle's position is one table read and a popcount. out[start + pos] = in[start + j].
+constexpr int kRankTable = 128;
 __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uint32_t* __restrict__ keys,
                                                             const uint32_t* __restrict__ in,
                                                             uint32_t* __restrict__ out) {
   __shared__ uint32_t ncell[64];
   __shared__ uint32_t cstart[64];
+  __shared__ uint32_t srank[kRankTable];
+  __shared__ uint64_t mrank[kRankTable];
+  __shared__ uint32_t s_rmax;
   const int tid = threadIdx.x;
   const uint32_t items = pool.ctr[kCtrItems];
   for (uint32_t k = blockIdx.x; k < items; k += gridDim.x) {
@@ -419,41 +425,58 @@ __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uin
     __syncthreads();
     for (uint32_t j = tid; j < cnt; j += kThreads) atomicAdd(&ncell[keys[start + j] & 63u], 1u);
     __syncthreads();
-    if (tid < 64) {  // exclusive scan of the cell counts (one wave)
+    if (tid < 64) {  // exclusive scan and maximum of the cell counts (one wave)
       const uint32_t c = ncell[tid];
-      uint32_t inc = c;
+      uint32_t inc = c, mx = c;
 #pragma unroll
       for (int d = 1; d < 64; d <<= 1) {
         const uint32_t o = __shfl_up(inc, d, 64);
         if (tid >= d) inc += o;
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
       }
       cstart[tid] = inc - c;
+      if (tid == 0) s_rmax = mx;
+    }
+    __syncthreads();
+    const uint32_t rtab = min(s_rmax, (uint32_t)kRankTable);
+    for (uint32_t r = tid; r < rtab; r += kThreads) {
+      uint32_t sr = 0;
+      uint64_t m = 0;
+      for (uint32_t q = 0; q < 64; q++) {
+        const uint32_t nq = ncell[q];
+        sr += min(nq, r);
+        m |= (uint64_t)(nq > r ? 1u : 0u) << q;
+      }
+      srank[r] = sr;
+      mrank[r] = m;
     }
     __syncthreads();
     for (uint32_t j = tid; j < cnt; j += kThreads) {
       const uint32_t c = keys[start + j] & 63u;
       const uint32_t r = j - cstart[c];
-      uint32_t pos = 0;
-      for (uint32_t q = 0; q < 64; q++) {
-        const uint32_t nq = ncell[q];
-        pos += min(nq, r) + ((q < c && nq > r) ? 1u : 0u);
+      uint32_t pos;
+      if (r < (uint32_t)kRankTable) {
+        pos = srank[r] + (uint32_t)__builtin_popcountll(mrank[r] & ((1ull << c) - 1ull));
+      } else {
+        pos = 0;
+        for (uint32_t q = 0; q < 64; q++) {
+          const uint32_t nq = ncell[q];
+          pos += min(nq, r) + ((q < c && nq > r) ? 1u : 0u);
+        }
       }
       out[start + pos] = in[start + j];
     }
   }
 }
 
-// First touch of node block b under `epoch` appends it to `list`.
-__device__ __forceinline__ void add_to_set(uint32_t b, uint32_t epoch, uint32_t* stamp, uint32_t* list, uint32_t* count) {
-  if (__hip_atomic_load(&stamp[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch)
-    if (atomicExch(&stamp[b], epoch) != epoch) list[atomicAdd(count, 1u)] = b;
-}
-
-// S: the node blocks [B-1, B+2]^3 of every work item (everything its tile can reach).
+// S: the node blocks [B-1, B+2]^3 of every work item (everything its tile can reach). One wave per
+// item (lane = node block of the 4^3 neighbourhood); the wave's newly added blocks are appended with
+// one atomic on the list counter.
 __global__ __launch_bounds__(kThreads) void k_bin_dilate(Pool pool) {
   const uint32_t cnt = pool.ctr[kCtrItems];
   const uint32_t epoch = pool.ctr[kCtrBinEpoch];
   const uint32_t mshift = pool.key_bits - 6;
+  const int lane = threadIdx.x & 63;
   for (uint32_t t = blockIdx.x * kThreads + threadIdx.x; t < cnt * 64; t += gridDim.x * kThreads) {
     const uint32_t k = t >> 6, d = t & 63;  // one item per wave
     const uint32_t wb = __builtin_amdgcn_readfirstlane(pool.wl_block[k]);
@@ -461,9 +484,21 @@ __global__ __launch_bounds__(kThreads) void k_bin_dilate(Pool pool) {
     const uint32_t mb = wb & ((1u << mshift) - 1u);
     const int gx = (int)compact3(mb) - 1 + (int)(d & 3), gy = (int)compact3(mb >> 1) - 1 + (int)((d >> 2) & 3),
               gz = (int)compact3(mb >> 2) - 1 + (int)(d >> 4);
-    if (gx < 0 || gy < 0 || gz < 0 || gx >= geo.nbx || gy >= geo.nby || gz >= geo.nbz) continue;
-    add_to_set(geo.block_base + (uint32_t)gx + (uint32_t)geo.nbx * ((uint32_t)gy + (uint32_t)geo.nby * (uint32_t)gz),
-               epoch, pool.sstamp, pool.slist, pool.ctr + kCtrS);
+    bool added = false;
+    uint32_t b = 0;
+    if (gx >= 0 && gy >= 0 && gz >= 0 && gx < geo.nbx && gy < geo.nby && gz < geo.nbz) {
+      b = geo.block_base + (uint32_t)gx + (uint32_t)geo.nbx * ((uint32_t)gy + (uint32_t)geo.nby * (uint32_t)gz);
+      if (__hip_atomic_load(&pool.sstamp[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch)
+        added = atomicExch(&pool.sstamp[b], epoch) != epoch;
+    }
+    const uint64_t mask = __builtin_amdgcn_ballot_w64(added);
+    if (mask) {
+      const int leader = __builtin_ctzll(mask);
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(pool.ctr + kCtrS, (uint32_t)__builtin_popcountll(mask));
+      base = (uint32_t)__shfl((int)base, leader, 64);
+      if (added) pool.slist[base + (uint32_t)__builtin_popcountll(mask & ((1ull << lane) - 1ull))] = b;
+    }
   }
 }
 
