@@ -96,6 +96,50 @@ mpm_status mpm_rescale(mpm_sim* sim, float k);
 mpm_status mpm_read_particles(mpm_sim* sim, float* rec, uint32_t offset, uint32_t n);
 /* grid.read() — src/verify/index.js:178. Blocking; 4 words per node, reference layout. */
 mpm_status mpm_read_grid(mpm_sim* sim, int32_t* words);
+/* readPositions (mpm.js:624-630) without the full records: for particles [offset, offset + n),
+ * pts[4i..] = (x * dx + origin computed in f64 and rounded to f32, material id as u32 bits) — the
+ * position and material slots the render layers read (stride-8 records at position 0, material in
+ * x.w, mpm.js:642-679); vel (optional, may be NULL) = (v * vscale in f64 -> f32, J), e.g.
+ * vscale = dx / substepDt for world units per second. 16 (or 32) bytes per particle cross PCIe
+ * instead of 128. Blocking. */
+mpm_status mpm_read_points(mpm_sim* sim, float* pts, float* vel, uint32_t offset, uint32_t n, const double origin[3],
+                           double dx, double vscale);
+/* Device pointers of the 8 particle streams (stream k holds vec4 k of every record as float4,
+ * capacity entries), for consumers on the same GPU (e.g. an interop renderer). Valid until the next
+ * call that steps, sorts or writes: a sort's reorder swaps the particle buffers, as P and P2 swap in
+ * the reference (types/shaderintel.d.ts:873-874). */
+mpm_status mpm_particle_streams(mpm_sim* sim, void* streams[8], uint32_t* capacity);
+
+/* The grid between the grid update and G2P — the point where the reference calls its coupling
+ * (this.coupling?.substep(s), mpm.js:596; coupling.js:203-221 reads and writes the f32 (v, m) grid).
+ * Device layout of the velocities: float4 (v.xyz, mass) per node in 4x4x4-node blocks;
+ * node (x, y, z) is element 64 * (block_base + x/4 + NBX * (y/4 + NBY * (z/4))) + (x%4 + 4 * (y%4 + 4 * (z%4))).
+ * Only the blocks of the two device lists hold this substep's values (others are stale). */
+typedef struct mpm_grid_view {
+  void* velocity;               /* device float4 array */
+  uint32_t block_base;          /* first block of the simulation */
+  int32_t blocks[3];            /* NBX, NBY, NBZ = ceil(dims / 4) */
+  int32_t dims[3];
+  const uint32_t* s_blocks;     /* device list of blocks the particles reach (NULL after a global-atomic P2G) */
+  const uint32_t* d_blocks;     /* device list of further blocks written by the fallback path */
+  const uint32_t* counts;       /* device [2]: |d_blocks|, |s_blocks| */
+  void* stream;                 /* the simulation's hipStream_t: enqueue work on it, do not synchronize */
+  uint32_t substep;             /* index of the substep within the current mpm_step call */
+  uint64_t step_count;
+} mpm_grid_view;
+typedef void (*mpm_grid_hook)(void* user, const mpm_grid_view* view);
+/* Native after-grid-update hook (the coupling call site, mpm.js:596): called on the caller's thread
+ * at every substep, after the grid update has been enqueued; work the hook enqueues on view->stream
+ * runs before G2P. NULL removes it. While a hook is set, mpm_step runs without captured graphs. */
+mpm_status mpm_set_grid_hook(mpm_sim* sim, mpm_grid_hook hook, void* user);
+/* One substep split at the same point, for callers that edit the grid from the host: begin = sort
+ * when due, P2G and grid update (mpm.js:591-595); then mpm_read_grid / mpm_write_grid; end = G2P
+ * (mpm.js:598-601). */
+mpm_status mpm_substep_begin(mpm_sim* sim, const float g_lattice[3], float floor_friction);
+mpm_status mpm_substep_end(mpm_sim* sim, float vmax);
+/* Writes every node's (v.xyz, mass) from f32 bits in the reference grid layout (4 words per node,
+ * node = x + X*(y + Y*z)); valid after a grid update (between mpm_substep_begin and _end). */
+mpm_status mpm_write_grid(mpm_sim* sim, const int32_t* words);
 /* ctx.idle() — src/core/context.js:198-204. */
 mpm_status mpm_sync(mpm_sim* sim);
 /* ctx.region timings (src/core/profiler.js): milliseconds of the last mpm_step per phase
@@ -221,6 +265,18 @@ mpm_status mpm_scan_u32(mpm_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_
  * order[start[k] + r] = index of the r-th element (in index order) with key k. */
 mpm_status mpm_counting_sort(mpm_ctx* ctx, const uint32_t* keys, uint64_t n, uint32_t num_buckets, uint32_t* order,
                              uint32_t* start);
+
+/* SpatialGrid (src/prims/grid.js:24-117) in one call: counting sort of n points (float4, xyz used)
+ * into the uniform grid over [lo, hi) with cells of cell_size (dims = max(1, ceil((hi - lo) /
+ * cell_size)), cell = clamp(floor((p - lo) * f32(1 / cell_size)), 0, dims - 1), x-fastest), then
+ * for each of m query points the sorted-order indices j of the 27 cells around its cell, in the
+ * order SpatialGrid.neighbors visits them (z, then y, then the row's contiguous x range).
+ * Outputs (host): order[n] (sorted position -> point; stable), start[numCells + 1], offsets[m + 1]
+ * and neighbors[offsets[m]] (CSR). *total receives offsets[m]; if neighbors is NULL or capacity is
+ * smaller, only the counts are produced (call again with room). */
+mpm_status mpm_spatial_grid(mpm_ctx* ctx, const float* points, uint32_t n, const double lo[3], const double hi[3],
+                            double cell_size, const float* queries, uint32_t m, uint32_t* order, uint32_t* start,
+                            uint32_t* offsets, uint32_t* neighbors, uint64_t capacity, uint64_t* total);
 
 /* ---- host helpers (no device work) ---- */
 /* _uploadMaterials row for one material — mpm.js:170-196 (float64 math, float32 out). */

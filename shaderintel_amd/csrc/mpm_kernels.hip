@@ -755,6 +755,31 @@ __global__ void k_grid_export(Pool pool, SceneGeo geo, int mode, int4* __restric
   }
 }
 
+__global__ void k_points_out(const float4* __restrict__ P, uint32_t cap, uint32_t offset, uint32_t n, uint32_t row_base,
+                             double o0, double o1, double o2, double dx, double vscale, float4* __restrict__ pts,
+                             float4* __restrict__ vel) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 x = P[offset + i];
+  pts[i] = make_float4((float)((double)x.x * dx + o0), (float)((double)x.y * dx + o1), (float)((double)x.z * dx + o2),
+                       __uint_as_float(__float_as_uint(x.w) - row_base));
+  if (vel) {
+    const float4 v = P[(size_t)cap + offset + i];
+    vel[i] = make_float4((float)((double)v.x * vscale), (float)((double)v.y * vscale), (float)((double)v.z * vscale), v.w);
+  }
+}
+
+__global__ void k_grid_import(Pool pool, SceneGeo geo, const int4* __restrict__ in) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t nodes = (size_t)geo.X * geo.Y * geo.Z;
+  if (i >= nodes) return;
+  const int gx = (int)(i % (size_t)geo.X), gy = (int)((i / (size_t)geo.X) % (size_t)geo.Y),
+            gz = (int)(i / ((size_t)geo.X * geo.Y));
+  const int4 w = in[i];
+  pool.V[node_addr(gx, gy, gz, geo)] =
+      make_float4(__int_as_float(w.x), __int_as_float(w.y), __int_as_float(w.z), __int_as_float(w.w));
+}
+
 inline unsigned blocks_for(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 inline unsigned persistent_grid(uint32_t max_work, unsigned per_block, unsigned cap) {
   unsigned g = (unsigned)((max_work + per_block - 1) / per_block);
@@ -882,6 +907,18 @@ hipError_t launch_g2p(const Pool& pool, float4* P, uint32_t n, float vmax, int w
     case 6: hipLaunchKernelGGL(k_g2p<6>, grid, dim3(kThreads), 0, s, pool, P, n, vmax); break;
     default: hipLaunchKernelGGL(k_g2p<3>, grid, dim3(kThreads), 0, s, pool, P, n, vmax); break;
   }
+  return hipGetLastError();
+}
+hipError_t launch_points_out(const float4* P, uint32_t cap, uint32_t offset, uint32_t n, uint32_t row_base,
+                             const double origin[3], double dx, double vscale, float4* pts, float4* vel, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_points_out, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, cap, offset, n, row_base, origin[0],
+                     origin[1], origin[2], dx, vscale, pts, vel);
+  return hipGetLastError();
+}
+hipError_t launch_grid_import(const Pool& pool, const SceneGeo& geo, const int4* in, hipStream_t s) {
+  const size_t nodes = (size_t)geo.X * geo.Y * geo.Z;
+  hipLaunchKernelGGL(k_grid_import, dim3(blocks_for(nodes, 256)), dim3(256), 0, s, pool, geo, in);
   return hipGetLastError();
 }
 hipError_t launch_grid_export(const Pool& pool, const SceneGeo& geo, int mode, int4* out, hipStream_t s) {

@@ -151,5 +151,11 @@ hipError_t launch_g2p(const Pool& pool, float4* P, uint32_t n, float vmax, int w
 // Reference-layout export of one scene: mode 0 = A (i32 words), mode 1 = V (f32 bits, 0 outside the
 // node blocks the last grid update wrote).
 hipError_t launch_grid_export(const Pool& pool, const SceneGeo& geo, int mode, int4* out, hipStream_t s);
+// readPositions (mpm.js:624-630) on the device: pts[i] = (x * dx + origin in f64 rounded to f32,
+// material id bits); vel (optional) = (v * vscale in f64 rounded to f32, J). Particles [offset, offset + n).
+hipError_t launch_points_out(const float4* P, uint32_t cap, uint32_t offset, uint32_t n, uint32_t row_base,
+                             const double origin[3], double dx, double vscale, float4* pts, float4* vel, hipStream_t s);
+// Writes one scene's velocities from reference-layout words (f32 bits of v.xyz, mass) into V.
+hipError_t launch_grid_import(const Pool& pool, const SceneGeo& geo, const int4* in, hipStream_t s);
 
 }  // namespace mpm

@@ -147,6 +147,9 @@ struct mpm_pool {
   bool a_dirty_s = false, a_dirty_d = false;  // A holds data over S / over D
   int g2p_waves = 3, p2g_waves = 4;
   bool use_graphs = false;
+  mpm_grid_hook hook = nullptr;  // called after every grid update (not with graphs)
+  void* hook_user = nullptr;
+  bool mid_substep = false;      // between mpm_substep_begin and mpm_substep_end
   hipGraphExec_t period_graph[2] = {nullptr, nullptr};  // one sort period, by parity of cur
   uint64_t graph_key = 0;                                // (count, scenes, vmax) the graphs were captured for
   // profiling: five events per profiled substep (sort, p2g, grid, g2p start; end)
@@ -469,8 +472,9 @@ mpm_status mark(mpm_pool* p) {
   return MPM_OK;
 }
 
-// One substep on the stream (MPM.step's loop body, mpm.js:591-601).
-mpm_status substep(mpm_pool* p, float vmax) {
+// First part of a substep (MPM.step's loop body up to the grid update and the coupling call,
+// mpm.js:591-596): sort when due, P2G, grid update, then the after-grid-update hook if one is set.
+mpm_status substep_front(mpm_pool* p, uint32_t index) {
   ST_TRY(mark(p));
   if (p->step_count % p->sort_every == 0) {
     ST_TRY(do_bin(p, kBinReorderInP2G));
@@ -480,12 +484,40 @@ mpm_status substep(mpm_pool* p, float vmax) {
   ST_TRY(mark(p));
   ST_TRY(do_p2g(p, false));
   ST_TRY(mark(p));
+  const bool use_s = p->grid_state == kGridTiles;
   ST_TRY(do_grid(p));
+  if (p->hook) {
+    const SceneGeo& geo = p->scenes[0].geo;
+    mpm_grid_view v;
+    std::memset(&v, 0, sizeof v);
+    v.velocity = p->V;
+    v.block_base = geo.block_base;
+    v.blocks[0] = geo.nbx; v.blocks[1] = geo.nby; v.blocks[2] = geo.nbz;
+    v.dims[0] = geo.X; v.dims[1] = geo.Y; v.dims[2] = geo.Z;
+    v.s_blocks = use_s ? p->slist : nullptr;
+    v.d_blocks = p->dlist;
+    v.counts = p->ctr + kCtrD;
+    v.stream = p->stream;
+    v.substep = index;
+    v.step_count = p->step_count;
+    p->hook(p->hook_user, &v);
+  }
+  return MPM_OK;
+}
+
+// Rest of a substep: G2P (mpm.js:598-601).
+mpm_status substep_back(mpm_pool* p, float vmax) {
   ST_TRY(mark(p));
   ST_TRY(do_g2p(p, vmax));
   ST_TRY(mark(p));
   p->step_count++;
   return MPM_OK;
+}
+
+// One substep on the stream (MPM.step's loop body, mpm.js:591-601).
+mpm_status substep(mpm_pool* p, float vmax, uint32_t index = 0) {
+  ST_TRY(substep_front(p, index));
+  return substep_back(p, vmax);
 }
 
 // One sort period (sort_every substeps starting at a sort) captured once per parity of the current
@@ -531,7 +563,7 @@ mpm_status pool_step(mpm_pool* p, uint32_t substeps, float vmax) {
   uint32_t k = 0;
   while (k < substeps) {
     const bool at_sort = p->step_count % p->sort_every == 0;
-    if (p->use_graphs && !p->profiling && at_sort && substeps - k >= p->sort_every) {
+    if (p->use_graphs && !p->profiling && !p->hook && at_sort && substeps - k >= p->sort_every) {
       ST_TRY(clean_a(p));
       ST_TRY(apply_pending_reorder(p));
       hipGraphExec_t ge = nullptr;
@@ -548,7 +580,7 @@ mpm_status pool_step(mpm_pool* p, uint32_t substeps, float vmax) {
       p->step_count += p->sort_every;
       k += p->sort_every;
     } else {
-      ST_TRY(substep(p, vmax));
+      ST_TRY(substep(p, vmax, k));
       k++;
     }
   }
@@ -776,6 +808,17 @@ mpm_status admit(mpm_pool* p, const int32_t dims[3], int32_t ppc, float bound, c
   return MPM_OK;
 }
 
+// Lattice gravity and floor friction of scene 0 (a simulation's only scene).
+mpm_status set_step_params(mpm_pool* p, const float g[3], float ff) {
+  SceneGeo& geo = p->scenes[0].geo;
+  if (geo.g[0] != g[0] || geo.g[1] != g[1] || geo.g[2] != g[2] || geo.floor_friction != ff) {
+    geo.g[0] = g[0]; geo.g[1] = g[1]; geo.g[2] = g[2];
+    geo.floor_friction = ff;
+    ST_TRY(upload_scene(p, 0));
+  }
+  return MPM_OK;
+}
+
 #define CHECK_SIM(s)                                                        \
   do {                                                                      \
     if (!(s)) return fail(MPM_ERR_INVALID_ARGUMENT, "simulation is NULL");  \
@@ -918,6 +961,7 @@ mpm_status mpm_set_colliders(mpm_sim* s, const float* tbl, uint32_t n) {
 
 mpm_status mpm_write_particles(mpm_sim* s, const float* rec, uint32_t offset, uint32_t n) {
   CHECK_SIM(s);
+  if (s->pool->mid_substep) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_write_particles: inside a substep begun with mpm_substep_begin");
   if (n == 0) return MPM_OK;
   if (!rec) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_write_particles: records NULL");
   mpm_pool* p = s->pool;
@@ -939,6 +983,7 @@ mpm_status mpm_read_particles(mpm_sim* s, float* rec, uint32_t offset, uint32_t 
 
 mpm_status mpm_set_count(mpm_sim* s, uint32_t n) {
   CHECK_SIM(s);
+  if (s->pool->mid_substep) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_count: inside a substep begun with mpm_substep_begin");
   mpm_pool* p = s->pool;
   if (n > p->cap) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_count: %u exceeds capacity %u", n, p->cap);
   if (n != p->count) {
@@ -973,17 +1018,14 @@ mpm_status mpm_step(mpm_sim* s, uint32_t substeps, const float g[3], float ff, f
   CHECK_SIM(s);
   if (!g) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_step: gravity is NULL");
   mpm_pool* p = s->pool;
-  SceneGeo& geo = p->scenes[0].geo;
-  if (geo.g[0] != g[0] || geo.g[1] != g[1] || geo.g[2] != g[2] || geo.floor_friction != ff) {
-    geo.g[0] = g[0]; geo.g[1] = g[1]; geo.g[2] = g[2];
-    geo.floor_friction = ff;
-    ST_TRY(upload_scene(p, 0));
-  }
+  if (p->mid_substep) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_step: inside a substep begun with mpm_substep_begin");
+  ST_TRY(set_step_params(p, g, ff));
   return pool_step(p, substeps, vmax);
 }
 
 mpm_status mpm_p2g(mpm_sim* s, int32_t force_global) {
   CHECK_SIM(s);
+  if (s->pool->mid_substep) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_p2g: inside a substep begun with mpm_substep_begin");
   mpm_pool* p = s->pool;
   ST_TRY(do_p2g(p, force_global != 0));
   if (p->grid_state == kGridTiles) {  // fold the tiles into A so the grid reads as the i32 P2G result
@@ -996,14 +1038,100 @@ mpm_status mpm_p2g(mpm_sim* s, int32_t force_global) {
 
 mpm_status mpm_sort(mpm_sim* s) {
   CHECK_SIM(s);
+  if (s->pool->mid_substep) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_sort: inside a substep begun with mpm_substep_begin");
   return do_bin(s->pool, kBinReorder);
 }
 
 mpm_status mpm_rescale(mpm_sim* s, float k) {
   CHECK_SIM(s);
+  if (s->pool->mid_substep) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_rescale: inside a substep begun with mpm_substep_begin");
   mpm_pool* p = s->pool;
   ST_TRY(apply_pending_reorder(p));
   HIP_TRY(launch_rescale(p->P[p->cur], p->cap, 0, p->count, k, p->stream));
+  return MPM_OK;
+}
+
+mpm_status mpm_set_grid_hook(mpm_sim* s, mpm_grid_hook hook, void* user) {
+  CHECK_SIM(s);
+  s->pool->hook = hook;
+  s->pool->hook_user = user;
+  drop_graphs(s->pool);
+  return MPM_OK;
+}
+
+mpm_status mpm_substep_begin(mpm_sim* s, const float g[3], float ff) {
+  CHECK_SIM(s);
+  if (!g) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_substep_begin: gravity is NULL");
+  mpm_pool* p = s->pool;
+  if (p->mid_substep) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_substep_begin: the previous substep was not ended");
+  ST_TRY(set_step_params(p, g, ff));
+  if (p->needs_repack) ST_TRY(repack(p));
+  if (p->count == 0) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_substep_begin: no particles");
+  ST_TRY(substep_front(p, 0));
+  p->mid_substep = true;
+  return MPM_OK;
+}
+
+mpm_status mpm_substep_end(mpm_sim* s, float vmax) {
+  CHECK_SIM(s);
+  mpm_pool* p = s->pool;
+  if (!p->mid_substep) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_substep_end: no substep was begun");
+  p->mid_substep = false;
+  return substep_back(p, vmax);
+}
+
+mpm_status mpm_write_grid(mpm_sim* s, const int32_t* words) {
+  CHECK_SIM(s);
+  if (!words) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_write_grid: words NULL");
+  mpm_pool* p = s->pool;
+  if (p->grid_state != kGridVelocity)
+    return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_write_grid: the grid holds no velocities (call after a grid update)");
+  const SceneGeo& geo = p->scenes[0].geo;
+  const size_t nodes = (size_t)geo.X * geo.Y * geo.Z;
+  if (p->export_nodes < nodes) {
+    if (p->export_buf) hipFree(p->export_buf);
+    p->export_buf = nullptr;
+    HIP_TRY(dmalloc(&p->export_buf, nodes));
+    p->export_nodes = nodes;
+  }
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  HIP_TRY(hipMemcpy(p->export_buf, words, nodes * 16, hipMemcpyHostToDevice));
+  HIP_TRY(launch_grid_import(view(p), geo, p->export_buf, p->stream));
+  return MPM_OK;
+}
+
+mpm_status mpm_read_points(mpm_sim* s, float* pts, float* vel, uint32_t offset, uint32_t n, const double origin[3],
+                           double dx, double vscale) {
+  CHECK_SIM(s);
+  if (n == 0) return MPM_OK;
+  if (!pts || !origin) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_read_points: NULL argument");
+  mpm_pool* p = s->pool;
+  if ((uint64_t)offset + n > p->count)
+    return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_read_points: offset %u + n %u exceeds count %u", offset, n, p->count);
+  ST_TRY(apply_pending_reorder(p));
+  // staging holds staging_records * 8 float4: room for 4 * staging_records points and velocities
+  const uint32_t chunk = p->staging_records * 4;
+  for (uint32_t done = 0; done < n;) {
+    const uint32_t k = std::min(n - done, chunk);
+    float4* dp = p->staging;
+    float4* dv = vel ? p->staging + chunk : nullptr;
+    HIP_TRY(launch_points_out(p->P[p->cur], p->cap, offset + done, k, 0, origin, dx, vscale, dp, dv, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    HIP_TRY(hipMemcpy(pts + 4 * (size_t)done, dp, (size_t)k * 16, hipMemcpyDeviceToHost));
+    if (vel) HIP_TRY(hipMemcpy(vel + 4 * (size_t)done, dv, (size_t)k * 16, hipMemcpyDeviceToHost));
+    done += k;
+  }
+  return MPM_OK;
+}
+
+mpm_status mpm_particle_streams(mpm_sim* s, void* streams[8], uint32_t* capacity) {
+  CHECK_SIM(s);
+  if (!streams) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_particle_streams: NULL argument");
+  mpm_pool* p = s->pool;
+  ST_TRY(apply_pending_reorder(p));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  for (int k = 0; k < 8; k++) streams[k] = p->P[p->cur] + (size_t)k * p->cap;
+  if (capacity) *capacity = p->cap;
   return MPM_OK;
 }
 
@@ -1346,6 +1474,78 @@ mpm_status mpm_counting_sort(mpm_ctx* c, const uint32_t* keys, uint64_t n, uint3
   for (uint32_t* p : {k0, k1, v0, v1, cnt, scr})
     if (p) hipFree(p);
   return from_hip(e, "mpm_counting_sort");
+}
+
+mpm_status mpm_spatial_grid(mpm_ctx* c, const float* points, uint32_t n, const double lo[3], const double hi[3],
+                            double cell_size, const float* queries, uint32_t m, uint32_t* order, uint32_t* start,
+                            uint32_t* offsets, uint32_t* neighbors, uint64_t capacity, uint64_t* total) {
+  if (!c) return fail(MPM_ERR_INVALID_ARGUMENT, "context is NULL");
+  if (!lo || !hi || !(cell_size > 0) || (n && !points) || (m && (!queries || !offsets)))
+    return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_spatial_grid: bad argument");
+  GridSpec g;
+  uint64_t cells = 1;
+  for (int a = 0; a < 3; a++) {  // grid.js:31-33
+    g.origin[a] = (float)lo[a];
+    g.dims[a] = (uint32_t)std::max(1.0, std::ceil((hi[a] - lo[a]) / cell_size));
+    cells *= g.dims[a];
+  }
+  if (cells > (1ull << 26)) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_spatial_grid: %llu cells is too many; increase cell_size",
+                                        (unsigned long long)cells);
+  g.inv = (float)(1.0 / cell_size);
+  ST_TRY(set_dev(c));
+  const size_t nb1 = (size_t)cells + 1;
+  float4 *dpts = nullptr, *dq = nullptr;
+  uint32_t *k0 = nullptr, *k1 = nullptr, *v0 = nullptr, *v1 = nullptr, *cnt = nullptr, *scr = nullptr, *qoff = nullptr,
+           *nb = nullptr;
+  uint32_t *rk = nullptr, *rv = nullptr;
+  uint64_t tot = 0;
+  int bits = 0;
+  while (bits < 32 && (1ull << bits) < cells) bits++;
+  hipError_t e = dmalloc(&dpts, n);
+  if (e == hipSuccess) e = dmalloc(&dq, m);
+  if (e == hipSuccess) e = dmalloc(&k0, n);
+  if (e == hipSuccess) e = dmalloc(&k1, n);
+  if (e == hipSuccess) e = dmalloc(&v0, n);
+  if (e == hipSuccess) e = dmalloc(&v1, n);
+  if (e == hipSuccess) e = dmalloc(&cnt, nb1);
+  if (e == hipSuccess) e = dmalloc(&qoff, (size_t)m + 1);
+  if (e == hipSuccess)
+    e = dmalloc(&scr, std::max(sort_scratch_words(n), std::max(scan_scratch_words(nb1), scan_scratch_words((size_t)m + 1))));
+  if (e == hipSuccess && n) e = hipMemcpy(dpts, points, (size_t)n * 16, hipMemcpyHostToDevice);
+  if (e == hipSuccess && m) e = hipMemcpy(dq, queries, (size_t)m * 16, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemsetAsync(cnt, 0, nb1 * 4, c->stream);
+  if (e == hipSuccess) e = grid_cells(dpts, n, g, k0, c->stream);
+  if (e == hipSuccess) e = bucket_count(k0, n, cnt, c->stream);
+  if (e == hipSuccess) e = scan_exclusive(cnt, nb1, scr, c->stream);  // start (grid.js:74-75)
+  rk = k0;
+  rv = v0;
+  if (e == hipSuccess) e = iota_u32(v0, n, c->stream);
+  if (e == hipSuccess) e = radix_sort_pairs(k0, v0, k1, v1, n, bits, scr, c->stream, &rk, &rv);  // order, stable
+  if (e == hipSuccess) e = grid_neighbor_count(dq, m, g, cnt, qoff, c->stream);
+  if (e == hipSuccess && m) e = scan_exclusive(qoff, (size_t)m + 1, scr, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess && m) {
+    uint32_t t32 = 0;
+    e = hipMemcpy(&t32, qoff + m, 4, hipMemcpyDeviceToHost);
+    tot = t32;
+  }
+  if (e == hipSuccess && m && neighbors && capacity >= tot && tot > 0) {
+    e = dmalloc(&nb, tot);
+    if (e == hipSuccess) e = grid_neighbor_fill(dq, m, g, cnt, qoff, nb, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(neighbors, nb, tot * 4, hipMemcpyDeviceToHost);
+  }
+  if (e == hipSuccess && order && n) e = hipMemcpy(order, rv, (size_t)n * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && start) e = hipMemcpy(start, cnt, nb1 * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && m) e = hipMemcpy(offsets, qoff, ((size_t)m + 1) * 4, hipMemcpyDeviceToHost);
+  for (void* q : {(void*)dpts, (void*)dq, (void*)k0, (void*)k1, (void*)v0, (void*)v1, (void*)cnt, (void*)scr,
+                  (void*)qoff, (void*)nb})
+    if (q) hipFree(q);
+  if (total) *total = tot;
+  if (e == hipSuccess && m && neighbors && capacity < tot)
+    return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_spatial_grid: %llu neighbour entries exceed capacity %llu",
+                (unsigned long long)tot, (unsigned long long)capacity);
+  return from_hip(e, "mpm_spatial_grid");
 }
 
 }  // extern "C"

@@ -252,6 +252,76 @@ hipError_t bucket_count(const uint32_t* keys, size_t n, uint32_t* cnt, hipStream
   return hipGetLastError();
 }
 
+__device__ __forceinline__ void grid_coord(float4 p, const GridSpec& g, int c[3]) {
+  const float q[3] = {p.x, p.y, p.z};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const int v = (int)floorf((q[a] - g.origin[a]) * g.inv);
+    c[a] = min(max(v, 0), (int)g.dims[a] - 1);
+  }
+}
+
+__global__ void k_grid_cells(const float4* __restrict__ pts, size_t n, GridSpec g, uint32_t* __restrict__ cells) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c[3];
+  grid_coord(pts[i], g, c);
+  cells[i] = (uint32_t)c[0] + g.dims[0] * ((uint32_t)c[1] + g.dims[1] * (uint32_t)c[2]);
+}
+
+// Visits the 9 row ranges [start[row + x0], start[row + x1 + 1]) around the query's cell.
+template <class F>
+__device__ __forceinline__ void grid_rows(float4 q, const GridSpec& g, const uint32_t* __restrict__ start, F&& f) {
+  int c[3];
+  grid_coord(q, g, c);
+  const uint32_t x0 = (uint32_t)max(c[0] - 1, 0), x1 = (uint32_t)min(c[0] + 1, (int)g.dims[0] - 1);
+  for (int z = max(c[2] - 1, 0); z <= min(c[2] + 1, (int)g.dims[2] - 1); z++)
+    for (int y = max(c[1] - 1, 0); y <= min(c[1] + 1, (int)g.dims[1] - 1); y++) {
+      const uint32_t row = g.dims[0] * ((uint32_t)y + g.dims[1] * (uint32_t)z);
+      f(start[row + x0], start[row + x1 + 1]);
+    }
+}
+
+__global__ void k_grid_neighbor_count(const float4* __restrict__ qs, size_t m, GridSpec g,
+                                      const uint32_t* __restrict__ start, uint32_t* __restrict__ count) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  uint32_t n = 0;
+  grid_rows(qs[i], g, start, [&](uint32_t b, uint32_t e) { n += e - b; });
+  count[i] = n;
+}
+
+__global__ void k_grid_neighbor_fill(const float4* __restrict__ qs, size_t m, GridSpec g,
+                                     const uint32_t* __restrict__ start, const uint32_t* __restrict__ offsets,
+                                     uint32_t* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  uint32_t o = offsets[i];
+  grid_rows(qs[i], g, start, [&](uint32_t b, uint32_t e) {
+    for (uint32_t j = b; j < e; j++) out[o++] = j;
+  });
+}
+
+hipError_t grid_cells(const float4* points, size_t n, GridSpec g, uint32_t* cells, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_grid_cells, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, points, n, g, cells);
+  return hipGetLastError();
+}
+hipError_t grid_neighbor_count(const float4* queries, size_t m, GridSpec g, const uint32_t* start, uint32_t* count,
+                               hipStream_t stream) {
+  if (m == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_grid_neighbor_count, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, queries, m, g,
+                     start, count);
+  return hipGetLastError();
+}
+hipError_t grid_neighbor_fill(const float4* queries, size_t m, GridSpec g, const uint32_t* start,
+                              const uint32_t* offsets, uint32_t* out, hipStream_t stream) {
+  if (m == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_grid_neighbor_fill, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, queries, m, g,
+                     start, offsets, out);
+  return hipGetLastError();
+}
+
 hipError_t iota_u32(uint32_t* vals, size_t n, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_iota, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, vals, n);

@@ -26,4 +26,21 @@ hipError_t bucket_count(const uint32_t* keys, size_t n, uint32_t* cnt, hipStream
 // vals[i] = i for i < n.
 hipError_t iota_u32(uint32_t* vals, size_t n, hipStream_t stream);
 
+// Uniform grid of SpatialGrid (src/prims/grid.js:14-21): cell of each point,
+// gridIndex(clamp(floor((p - origin) * inv), 0, dims - 1)) in f32, points as float4 (xyz used).
+struct GridSpec {
+  float origin[3];
+  float inv;
+  uint32_t dims[3];
+};
+hipError_t grid_cells(const float4* points, size_t n, GridSpec g, uint32_t* cells, hipStream_t stream);
+// Number of sorted-order entries in the 27 cells around each query (the 9 contiguous row ranges of
+// SpatialGrid.neighbors, grid.js:101-117): count[q].
+hipError_t grid_neighbor_count(const float4* queries, size_t m, GridSpec g, const uint32_t* start, uint32_t* count,
+                               hipStream_t stream);
+// The sorted-order indices j of those entries, written at offsets[q] in the visiting order (z, then
+// y, then j over the row's x range).
+hipError_t grid_neighbor_fill(const float4* queries, size_t m, GridSpec g, const uint32_t* start,
+                              const uint32_t* offsets, uint32_t* out, hipStream_t stream);
+
 }  // namespace mpm

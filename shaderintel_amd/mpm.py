@@ -81,6 +81,17 @@ def diagnostics(handle, pool: bool = False) -> dict:
     return {k: getattr(d, k) for k, _ in _Diagnostics._fields_}
 
 
+class GridView(ctypes.Structure):
+    """mpm_grid_view (include/mpm.h): what an after-grid-update hook receives."""
+    _fields_ = [("velocity", ctypes.c_void_p), ("block_base", ctypes.c_uint32), ("blocks", ctypes.c_int32 * 3),
+                ("dims", ctypes.c_int32 * 3), ("s_blocks", ctypes.c_void_p), ("d_blocks", ctypes.c_void_p),
+                ("counts", ctypes.c_void_p), ("stream", ctypes.c_void_p), ("substep", ctypes.c_uint32),
+                ("step_count", ctypes.c_uint64)]
+
+
+GRID_HOOK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(GridView))
+
+
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("substeps", ctypes.c_uint32), ("floor_friction", ctypes.c_float), ("vmax", ctypes.c_float)]
 
@@ -147,6 +158,13 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "mpm_pool_set_tuning": (i32, [vp, i32, i32]),
         "mpm_pool_autotune": (i32, [vp, u32, P(i32), P(i32)]),
         "mpm_pool_diagnostics": (i32, [vp, P(_Diagnostics)]),
+        "mpm_set_grid_hook": (i32, [vp, GRID_HOOK, vp]),
+        "mpm_read_points": (i32, [vp, vp, vp, u32, u32, vp, f64, f64]),
+        "mpm_particle_streams": (i32, [vp, vp, P(u32)]),
+        "mpm_spatial_grid": (i32, [vp, vp, u32, vp, vp, f64, vp, u32, vp, vp, vp, vp, u64, P(u64)]),
+        "mpm_substep_begin": (i32, [vp, vp, f32]),
+        "mpm_substep_end": (i32, [vp, f32]),
+        "mpm_write_grid": (i32, [vp, vp]),
         "mpm_diagnostics_of": (i32, [vp, P(_Diagnostics)]),
         "mpm_pool_add_template": (i32, [vp, P(_SceneDesc), P(u32)]),
         "mpm_pool_remove_template": (i32, [vp, u32]),
@@ -179,6 +197,29 @@ def _ptr(a: np.ndarray) -> ctypes.c_void_p:
     """Pointer to an array's data that keeps the array alive while the pointer object lives (so a
     temporary passed straight into a ctypes call is not freed before the call runs)."""
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def spatial_grid(ctx, points: np.ndarray, lo, hi, cell_size: float, queries: np.ndarray):
+    """SpatialGrid build + neighbour ranges (src/prims/grid.js:24-117) on the device: returns
+    (order, start, offsets, neighbors) with neighbors[offsets[q]:offsets[q+1]] the sorted-order
+    indices in the 27 cells around query q."""
+    L = load_library()
+    pts = np.zeros((len(points), 4), np.float32)
+    pts[:, :3] = np.asarray(points, np.float32).reshape(-1, 3)
+    qs = np.zeros((len(queries), 4), np.float32)
+    qs[:, :3] = np.asarray(queries, np.float32).reshape(-1, 3)
+    lo, hi = np.asarray(lo, np.float64), np.asarray(hi, np.float64)
+    dims = [max(1, int(np.ceil((hi[a] - lo[a]) / cell_size))) for a in range(3)]
+    order = np.zeros(len(pts), np.uint32)
+    start = np.zeros(dims[0] * dims[1] * dims[2] + 1, np.uint32)
+    offsets = np.zeros(len(qs) + 1, np.uint32)
+    total = ctypes.c_uint64()
+    args = [ctx._h, _ptr(pts), len(pts), _ptr(lo), _ptr(hi), float(cell_size), _ptr(qs), len(qs), _ptr(order),
+            _ptr(start), _ptr(offsets)]
+    _check(L.mpm_spatial_grid(*args, None, 0, ctypes.byref(total)), "spatial_grid")
+    nb = np.zeros(max(total.value, 1), np.uint32)
+    _check(L.mpm_spatial_grid(*args, _ptr(nb), total.value, ctypes.byref(total)), "spatial_grid")
+    return order, start, offsets, nb[:total.value]
 
 
 class Context:
@@ -478,6 +519,47 @@ class MPM:
         out = np.empty((self.numNodes, 4), np.int32)
         _check(load_library().mpm_read_grid(self._h, _ptr(out)), "MPM.grid.read")
         return out
+
+    def substep_begin(self) -> None:
+        """First part of one substep (sort when due, P2G, grid update): the grid then holds f32
+        (v, m) for read_grid / write_grid, as at the reference's coupling call (mpm.js:596)."""
+        self._alive("MPM.substep_begin")
+        _check(load_library().mpm_substep_begin(self._h, _ptr(self.lattice_gravity()), 0.4), "MPM.substep_begin")
+
+    def substep_end(self) -> None:
+        """G2P: finishes the substep begun with substep_begin."""
+        self._alive("MPM.substep_end")
+        _check(load_library().mpm_substep_end(self._h, 0.9), "MPM.substep_end")
+
+    def write_grid(self, words: np.ndarray) -> None:
+        """Writes every node's f32 (v.xyz, mass) bits, reference layout (numNodes, 4) int32."""
+        self._alive("MPM.grid.write")
+        w = np.ascontiguousarray(words, np.int32).reshape(self.numNodes, 4)
+        _check(load_library().mpm_write_grid(self._h, _ptr(w)), "MPM.grid.write")
+
+    def set_grid_hook(self, fn) -> None:
+        """Native after-grid-update hook: fn(user, GridView*) is a GRID_HOOK (ctypes) or None."""
+        self._hook = fn  # keep the ctypes thunk alive
+        _check(load_library().mpm_set_grid_hook(self._h, fn if fn is not None else GRID_HOOK(), None), "MPM.set_grid_hook")
+
+    def read_points(self, velocity: bool = False):
+        """World positions + material ids (count, 4) float32 computed on the device (readPositions,
+        mpm.js:624-630), and optionally velocities in world units/s with J."""
+        self._alive("MPM.read_points")
+        n = self.count
+        pts = np.zeros((n, 4), np.float32)
+        vel = np.zeros((n, 4), np.float32) if velocity else None
+        _check(load_library().mpm_read_points(self._h, _ptr(pts), _ptr(vel) if velocity else None, 0, n,
+                                              _ptr(np.asarray(self.origin, np.float64)), self.dx,
+                                              self.dx / self.substepDt), "MPM.read_points")
+        return (pts, vel) if velocity else pts
+
+    def particle_streams(self):
+        """Device pointers of the 8 particle streams and their capacity (valid until the next step)."""
+        ptrs = (ctypes.c_void_p * 8)()
+        cap = ctypes.c_uint32()
+        _check(load_library().mpm_particle_streams(self._h, ptrs, ctypes.byref(cap)), "MPM.particle_streams")
+        return [p or 0 for p in ptrs], cap.value
 
     def readPositions(self) -> np.ndarray:  # mpm.js:624-630
         raw = self.read_particles()
