@@ -8,6 +8,16 @@ seeded inputs. Particles and grid stay resident in HBM; the timed region is exac
 substeps between two stream synchronizations. Per-phase durations come from HIP events recorded on
 the simulation's own stream inside the timed region.
 
+Two more sections run by default (skip with --no-batched / --no-serve):
+- "batched": 64 independent scenes (edges 32-128, 1e4-1e6 particles, one material box each, master
+  seed 1) stepped as one scene pool (one launch per phase for all scenes, captured sort periods,
+  autotuned occupancy) against the same scenes as separate simulations stepped in turn (the
+  reference's one-object-per-scene pattern on a single queue) and on 64 concurrent streams.
+- "serve": the native serving loop with Poisson arrivals (--serve-rate req/s for --serve-seconds),
+  each request one of those scenes for a geometric number of 4-substep steps (mean 300), admitted
+  FIFO into a pool sized to a 200 GB HBM budget; reports p50/p99 step latency, request latency and
+  admission wait.
+
 Prints one JSON line (the last line of stdout).
 """
 from __future__ import annotations

@@ -292,6 +292,67 @@ napi_value ReadGrid(napi_env env, napi_callback_info info) {
   return resolved(env, arr);
 }
 
+// readPoints(sim, n, origin Float64Array(3), dx, vscale, withVelocity) -> Promise<[Float32Array, Float32Array|null]>
+napi_value ReadPoints(napi_env env, napi_callback_info info) {
+  ARGS(6);
+  SimBox* b = sim_of(env, argv[0]);
+  if (!b) return nullptr;
+  const uint32_t n = (uint32_t)num(env, argv[1], 0);
+  size_t olen = 0;
+  double* origin = static_cast<double*>(typed(env, argv[2], napi_float64_array, &olen));
+  if (!origin || olen < 3) { throw_arg(env, "MPM.readPoints: origin must be a Float64Array(3)"); return nullptr; }
+  bool with_vel = false;
+  napi_get_value_bool(env, argv[5], &with_vel);
+  void *pd = nullptr, *vd = nullptr;
+  napi_value pts = make_typed(env, napi_float32_array, (size_t)n * 4, 4, &pd);
+  napi_value vel = nullptr;
+  if (with_vel) vel = make_typed(env, napi_float32_array, (size_t)n * 4, 4, &vd);
+  else napi_get_null(env, &vel);
+  mpm_status st = mpm_read_points(b->sim, static_cast<float*>(pd), static_cast<float*>(vd), 0, n, origin,
+                                  num(env, argv[3], 1), num(env, argv[4], 1));
+  if (st) { throw_status(env, st, "MPM.readPoints"); return nullptr; }
+  napi_value out;
+  napi_create_array_with_length(env, 2, &out);
+  napi_set_element(env, out, 0, pts);
+  napi_set_element(env, out, 1, vel);
+  return resolved(env, out);
+}
+
+// substepBegin(sim, gravity Float32Array(3), floorFriction)
+napi_value SubstepBegin(napi_env env, napi_callback_info info) {
+  ARGS(3);
+  SimBox* b = sim_of(env, argv[0]);
+  if (!b) return nullptr;
+  size_t len = 0;
+  float* g = static_cast<float*>(typed(env, argv[1], napi_float32_array, &len));
+  if (!g || len < 3) { throw_arg(env, "MPM.substepBegin: gravity must be a Float32Array(3)"); return nullptr; }
+  mpm_status st = mpm_substep_begin(b->sim, g, (float)num(env, argv[2], 0.4));
+  if (st) { throw_status(env, st, "MPM.substepBegin"); return nullptr; }
+  return undef(env);
+}
+
+napi_value SubstepEnd(napi_env env, napi_callback_info info) {
+  ARGS(2);
+  SimBox* b = sim_of(env, argv[0]);
+  if (!b) return nullptr;
+  mpm_status st = mpm_substep_end(b->sim, (float)num(env, argv[1], 0.9));
+  if (st) { throw_status(env, st, "MPM.substepEnd"); return nullptr; }
+  return undef(env);
+}
+
+// writeGrid(sim, Int32Array(numNodes * 4)): f32 bits of (v.xyz, mass) per node, reference layout
+napi_value WriteGrid(napi_env env, napi_callback_info info) {
+  ARGS(2);
+  SimBox* b = sim_of(env, argv[0]);
+  if (!b) return nullptr;
+  size_t len = 0;
+  int32_t* w = static_cast<int32_t*>(typed(env, argv[1], napi_int32_array, &len));
+  if (!w) { throw_arg(env, "MPM.grid.write: words must be an Int32Array"); return nullptr; }
+  mpm_status st = mpm_write_grid(b->sim, w);
+  if (st) { throw_status(env, st, "MPM.grid.write"); return nullptr; }
+  return undef(env);
+}
+
 napi_value SetCount(napi_env env, napi_callback_info info) {
   ARGS(2);
   SimBox* b = sim_of(env, argv[0]);
@@ -462,7 +523,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"setCount", SetCount}, {"setStepCount", SetStepCount}, {"getStepCount", GetStepCount}, {"step", Step},
       {"p2g", P2G}, {"sort", Sort}, {"rescale", Rescale}, {"sync", Sync}, {"scanU32", ScanU32},
       {"countingSort", CountingSort}, {"materialRow", MaterialRow}, {"sampleShape", SampleShape},
-      {"packParticles", PackParticles},
+      {"packParticles", PackParticles}, {"readPoints", ReadPoints}, {"substepBegin", SubstepBegin},
+      {"substepEnd", SubstepEnd}, {"writeGrid", WriteGrid},
   };
   for (const Fn& f : fns) {
     napi_value v;

@@ -102,6 +102,8 @@ class MPM {
     };
     this.grid = {
       read() { return call(() => native().readGrid(sim._alive('MPM.grid.read'), sim.numNodes)); },
+      /** Between substepBegin and substepEnd: every node's f32 (v, m) bits, reference layout. */
+      write(words) { call(() => native().writeGrid(sim._alive('MPM.grid.write'), words)); },
       get destroyed() { return sim.destroyed; },
     };
   }
@@ -228,12 +230,28 @@ class MPM {
   _sort() { call(() => native().sort(this._alive('MPM._sort'))); }
 
   async readPositions() {                                                         // mpm.js:624-630
-    const raw = await this.P.read(this.count * 8);
+    // x * dx + origin is evaluated on the device in f64 and rounded to f32, as here in JS
+    const pts = (await this.readPoints())[0];
     const out = new Float32Array(this.count * 3);
     for (let k = 0; k < this.count; k++)
-      for (let a = 0; a < 3; a++) out[3 * k + a] = raw[32 * k + a] * this.dx + this.origin[a];
+      for (let a = 0; a < 3; a++) out[3 * k + a] = pts[4 * k + a];
     return out;
   }
+
+  /** [points, velocities]: (world xyz, material id bits) per particle and, if asked, (world v, J). */
+  readPoints(withVelocity) {
+    return call(() => native().readPoints(this._alive('MPM.readPoints'), this.count, Float64Array.from(this.origin),
+      this.dx, this.dx / this.substepDt, !!withVelocity));
+  }
+
+  /** One substep split at the coupling call site (mpm.js:596): sort when due, P2G, grid update. */
+  substepBegin() {
+    call(() => native().setColliders(this._alive('MPM.substepBegin'), this.colliderTable(), Math.min(16, this.colliders.length)));
+    call(() => native().substepBegin(this._h, this.latticeGravity(), 0.4));
+  }
+
+  /** G2P: ends the substep begun with substepBegin. */
+  substepEnd() { call(() => native().substepEnd(this._alive('MPM.substepEnd'), 0.9)); }
 
   palette() { return this.materials.map((m) => m.color || [0.7, 0.7, 0.7]); }
   fluidMask() { return this.materials.reduce((m, d, k) => (d.kind === FLUID ? m | (1 << k) : m), 0); }

@@ -22,8 +22,18 @@ async function main() {
   fs.writeFileSync(path.join(outdir, 'after.f32'), Buffer.from(after.buffer));
   const grid = await sim.grid.read();
   fs.writeFileSync(path.join(outdir, 'grid.i32'), Buffer.from(grid.buffer));
+  const pos = await sim.readPositions();
+  fs.writeFileSync(path.join(outdir, 'positions.f32'), Buffer.from(pos.buffer));
+  // one more substep split at the coupling call site, with the grid read back and written unchanged
+  sim.substepBegin();
+  const mid = await sim.grid.read();
+  sim.grid.write(mid);
+  sim.substepEnd();
+  const after2 = await sim.P.read(sim.count * 8);
+  fs.writeFileSync(path.join(outdir, 'after2.f32'), Buffer.from(after2.buffer));
   const law = await SI.checkMPM(sim);
-  const summary = { count: sim.count, dims: sim.dims, stepCount: sim.stepCount, gravity: Array.from(sim.latticeGravity()), law };
+  const summary = { count: sim.count, dims: sim.dims, stepCount: sim.stepCount, gravity: Array.from(sim.latticeGravity()), law,
+    dx: sim.dx };
   sim.destroy(); sim.destroy();
   let threw = null;
   try { sim.step(); } catch (e) { threw = e.code; }

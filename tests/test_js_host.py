@@ -74,6 +74,11 @@ def test_js_scene_matches_oracle(tmp_path, frames):
     sc.step(8 * frames, np.asarray(summary["gravity"], np.float32))
     want = sc.read(n)
     assert np.array_equal(after.view(np.uint32), want.view(np.uint32))
-    assert summary["stepCount"] == 8 * frames
+    pos = np.fromfile(tmp_path / "positions.f32", np.float32).reshape(n, 3)
+    assert np.array_equal(pos.view(np.uint32), (after[:, :3].astype(np.float64) * summary["dx"]).astype(np.float32).view(np.uint32))
+    after2 = np.fromfile(tmp_path / "after2.f32", np.float32).reshape(n, 32)
+    sc.step(1, np.asarray(summary["gravity"], np.float32))
+    assert np.array_equal(after2.view(np.uint32), sc.read(n).view(np.uint32))
+    assert summary["stepCount"] == 8 * frames + 1
     assert summary["law"]["ok"], summary["law"]
     assert summary["afterDestroy"] == "destroyed"
