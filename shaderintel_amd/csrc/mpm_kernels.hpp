@@ -19,12 +19,12 @@
 //    is (rank within cell, cell), so consecutive particles sit in different cells.
 //  - Tiles: P2G gives every work item one workgroup that sums the 27 fixed-point contributions of
 //    its particles in an LDS tile (LDS integer atomics) sized to the bounding box of their stencils,
-//    takes that many nodes from one region of a bump-allocated arena and stores the box and nodes there with plain
-//    stores (tile_hdr[item], tile_off[item]); an item whose tile no longer fits the arena, or whose
-//    particles all drifted, uses the fallback below. The grid update of a node block pulls from the tiles of particle
-//    blocks G-2..G+1 per axis, so the tiled path needs no global atomic. A particle whose base
-//    drifted more than kDrift cells outside its block since the binning (or of an item beyond the
-//    falls back to global atomics on A and stamps the node blocks it wrote.
+//    takes that many nodes from one region of a bump-allocated arena and stores the box and the
+//    nodes there with plain stores (tile_hdr[item], tile_off[item]). The grid update of a node
+//    block pulls from the tiles of particle blocks G-2..G+1 per axis, so the tiled path needs no
+//    global atomic. A particle whose base drifted more than kDrift cells outside its block since
+//    the binning, or every particle of an item whose tile does not fit its arena region, falls back
+//    to global atomics on A and stamps the node blocks it wrote.
 //  - Active node blocks: S = node blocks within reach of any work item (rebuilt at each binning);
 //    D = node blocks written through A in the current P2G. The grid update visits S and D and
 //    returns A to zero where it was written.
