@@ -88,6 +88,17 @@ def pctl(a, q):
     return round(float(np.percentile(a, q)), 3) if len(a) else None
 
 
+_BUILT = {}
+
+
+def batched_scenes(seed: int = 1, count: int = 64):
+    """The batched workload's scenes built on the host (cached: the serving runs reuse them)."""
+    from shaderintel_amd import scenes
+    if (seed, count) not in _BUILT:
+        _BUILT[(seed, count)] = [scenes.host_blocks(edge, blocks) for edge, blocks in scenes.batched_blocks(seed, count)]
+    return _BUILT[(seed, count)]
+
+
 def batched(ctx, substeps: int, seed: int = 1, count: int = 64) -> dict:
     """The batched workload (64 scenes from one master seed, see scenes.batched_blocks): one scene
     pool stepped with one launch per phase (captured sort periods, autotuned occupancy) against the
@@ -95,7 +106,7 @@ def batched(ctx, substeps: int, seed: int = 1, count: int = 64) -> dict:
     from shaderintel_amd import scenes
     from shaderintel_amd.mpm import MPM, ScenePool, load_library, node_blocks, _ptr, _check
     L = load_library()
-    built = [scenes.host_blocks(edge, blocks) for edge, blocks in scenes.batched_blocks(seed, count)]
+    built = batched_scenes(seed, count)
     total = sum(len(b[2]) for b in built)
     log(f"batched: {count} scenes, {total} particles")
     pool = ScenePool(ctx, total, count, sum(node_blocks(b[0]) for b in built), max(b[0][0] for b in built))
@@ -158,9 +169,8 @@ def serve(ctx, rate: float, seconds: float, budget_gb: float, substeps: int = 4,
     batched-workload scenes (drawn at random) simulated for a geometric number of steps (mean
     `lifetime_mean`) of `substeps` substeps. One pool sized to the HBM budget; the native serving loop
     admits, steps and retires (continuous batching)."""
-    from shaderintel_amd import scenes
     from shaderintel_amd.mpm import ScenePool, Server
-    built = [scenes.host_blocks(edge, blocks) for edge, blocks in scenes.batched_blocks(1, 64)]
+    built = batched_scenes(1, 64)
     tmpl_bytes = sum(len(b[2]) for b in built) * 128
     budget = budget_gb * 1e9 - tmpl_bytes
     max_scenes, max_dim = 2048, 128
@@ -220,8 +230,9 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched comparison")
     ap.add_argument("--batched-substeps", type=int, default=200)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
-    ap.add_argument("--serve-rate", type=float, nargs="+", default=[100.0], help="request arrival rates (req/s)")
-    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-rate", type=float, nargs="+", default=[100.0, 300.0, 1000.0, 3000.0, 5000.0],
+                    help="request arrival rates (req/s)")
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the timed region, copy a MIB-MiB device buffer 3 times (a kernel of known traffic "
@@ -316,8 +327,7 @@ def main() -> int:
     if not args.no_batched:
         out["batched"] = batched(ctx, args.batched_substeps)
     if not args.no_serve:
-        runs = [serve(ctx, r, args.serve_seconds, args.serve_budget_gb) for r in args.serve_rate]
-        out["serve"] = runs[0] if len(runs) == 1 else runs
+        out["serve"] = [serve(ctx, r, args.serve_seconds, args.serve_budget_gb) for r in args.serve_rate]
     if args.copy_calibration:
         copy_calibration(args.copy_calibration << 20)
     if not args.no_cpu:
