@@ -8,6 +8,12 @@ seeded inputs. Particles and grid stay resident in HBM; the timed region is exac
 substeps between two stream synchronizations. Per-phase durations come from HIP events recorded on
 the simulation's own stream inside the timed region.
 
+The dominant kernel's HBM figure is also set against a device-to-device copy measured in the same
+run. "small_configs" reports the 64^3 dam break (65,536 particles, 100 substeps) and the 128^3
+water + jelly scene (1,048,576 particles, 1000 substeps), best of 3 runs each (skip with
+--no-small); both are cache- or launch-bound. The CPU baseline is the oracle on 10 substeps of
+the bench scene, plus the dam break in full and 10 substeps of the 128^3 scene.
+
 Two more sections run by default (skip with --no-batched / --no-serve):
 - "batched": 64 independent scenes (edges 32-128, 1e4-1e6 particles, one material box each, master
   seed 1) stepped as one scene pool (one launch per phase for all scenes, captured sort periods,
@@ -44,9 +50,9 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene: str, substeps: int) -> dict:
-    """The oracle (CPU restatement of the reference algorithm) on a bounded sample of the same
-    workload: the full scene, `substeps` substeps, all host threads."""
+def cpu_rate(scene: str, substeps: int) -> tuple:
+    """The oracle (CPU restatement of the reference algorithm) on the full scene for `substeps`
+    substeps, all host threads: (particle-substeps/s, particles, threads)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_mpm as O
     from shaderintel_amd import scenes
@@ -62,9 +68,83 @@ def cpu_baseline(scene: str, substeps: int) -> dict:
     sc.step(substeps, g)
     dt = time.perf_counter() - t0
     n = scenes.expected_count(scene)
-    return {"value": n * substeps / dt, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
-            "sample": f"{scene}: all {n} particles, {substeps} substeps (1 sort), OpenMP over {threads} threads; "
-                      "CPU restatement of the reference algorithm, not the reference's WebGPU path"}
+    return n * substeps / dt, n, threads
+
+
+def cpu_baseline(scene: str, substeps: int, small: bool = True) -> dict:
+    """The oracle on a bounded sample of the bench workload (the full scene for `substeps` substeps),
+    and, with `small`, on the 64^3 dam break for its full 100 substeps and the 128^3 water + jelly
+    scene for 10 substeps."""
+    rate, n, threads = cpu_rate(scene, substeps)
+    sorts = (substeps + 3) // 4
+    out = {"value": rate, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+           "sample": f"{scene}: all {n} particles, {substeps} substeps ({sorts} sorts), OpenMP over {threads} "
+                     "threads; CPU restatement of the reference algorithm, not the reference's WebGPU path"}
+    if small:
+        out["small_configs"] = {}
+        for name, k in (("dam64", 100), ("mixed128", 10)):
+            r, m, _ = cpu_rate(name, k)
+            out["small_configs"][name] = {"particles": m, "substeps": k, "particle_substeps_per_s": round(r, 1)}
+    return out
+
+
+def gpu_small_configs(ctx) -> dict:
+    """The 64^3 dam break (65,536 particles, 100 substeps) and the 128^3 water + jelly scene
+    (1,048,576 particles, 1000 substeps) on the GPU: best of 3 batches, each the scene's full run,
+    between stream synchronizations. Both are cache- or launch-bound, reported in absolute terms."""
+    from shaderintel_amd import scenes
+    from shaderintel_amd.mpm import MPM, load_library, _ptr, _check
+    L = load_library()
+    out = {}
+    for name, k in (("dam64", 100), ("mixed128", 1000)):
+        sim = MPM(ctx, **scenes.options(name))
+        n = scenes.populate(sim, name)
+        g = sim.lattice_gravity()
+        _check(L.mpm_step(sim._h, 8, _ptr(g), 0.4, 0.9), "step")
+        sim.sync()
+        best = None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            _check(L.mpm_step(sim._h, k, _ptr(g), 0.4, 0.9), "step")
+            sim.sync()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        sim.destroy()
+        out[name] = {"particles": n, "substeps": k, "ms_per_substep": round(best * 1e3 / k, 4),
+                     "particle_substeps_per_s": round(n * k / best, 1)}
+        log(f"{name}: {n} particles, {k} substeps, best {best * 1e3:.1f} ms")
+    return out
+
+
+def copy_bandwidth(nbytes: int = 1 << 31, reps: int = 5) -> float:
+    """Device-to-device copy rate (GB/s, bytes read + written) of the HIP runtime's copy kernel on
+    an `nbytes` buffer, timed with HIP events on the null stream: the measured ceiling the path's
+    HBM figure is set against, beside the 8 TB/s spec peak."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    try:
+        assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+        assert hip.hipEventCreate(ctypes.byref(e0)) == 0 and hip.hipEventCreate(ctypes.byref(e1)) == 0
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # warm-up, hipMemcpyDeviceToDevice
+        best = None
+        for _ in range(reps):
+            assert hip.hipEventRecord(e0, None) == 0
+            assert hip.hipMemcpyAsync(b, a, ctypes.c_size_t(nbytes), 3, None) == 0
+            assert hip.hipEventRecord(e1, None) == 0
+            assert hip.hipEventSynchronize(e1) == 0
+            ms = ctypes.c_float()
+            assert hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1) == 0
+            best = ms.value if best is None else min(best, ms.value)
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return 2.0 * nbytes / (best * 1e-3) / 1e9
 
 
 def copy_calibration(nbytes: int) -> None:
@@ -224,12 +304,14 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--scene", default="mixed256")
-    ap.add_argument("--cpu-substeps", type=int, default=4)
+    ap.add_argument("--cpu-substeps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", action="store_true", help="time each tuning knob setting and exit")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched comparison")
     ap.add_argument("--batched-substeps", type=int, default=200)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
+    ap.add_argument("--no-small", action="store_true", help="skip the 64^3 and 128^3 scenes")
+    ap.add_argument("--no-copy", action="store_true", help="skip the device-to-device copy measurement")
     ap.add_argument("--serve-rate", type=float, nargs="+", default=[100.0, 300.0, 1000.0, 3000.0, 5000.0],
                     help="request arrival rates (req/s)")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
@@ -324,6 +406,13 @@ def main() -> int:
         "finite": finite,
     }
     sim.destroy()
+    if not args.no_copy:
+        copy_gbs = copy_bandwidth()
+        roofline["copy_gbs"] = round(copy_gbs, 1)
+        roofline["frac_of_copy"] = round(achieved / copy_gbs, 4)
+        log(f"device-to-device copy: {copy_gbs:.0f} GB/s")
+    if not args.no_small:
+        out["small_configs"] = gpu_small_configs(ctx)
     if not args.no_batched:
         out["batched"] = batched(ctx, args.batched_substeps)
     if not args.no_serve:
@@ -332,7 +421,7 @@ def main() -> int:
         copy_calibration(args.copy_calibration << 20)
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline(args.scene, args.cpu_substeps)
+        out["cpu_baseline"] = cpu_baseline(args.scene, args.cpu_substeps, small=not args.no_small)
         log(f"cpu baseline {out['cpu_baseline']['value']:.3e} particle-substeps/s")
     print(json.dumps(out), flush=True)
     return 0
