@@ -158,17 +158,19 @@ __device__ __forceinline__ float4 grid_node(int4 q, int ix, int iy, int iz, cons
   return make_float4(v.x, v.y, v.z, m);
 }
 
-// G2P for particle i (mpm.js:479-559): gather velocity and APIC C, advect, update F, plasticity,
-// stress. Writes the new record to the 8 streams and returns it in r.
+// G2P for one particle (mpm.js:479-559): gather velocity and APIC C, advect, update F, plasticity,
+// stress. Reads record si of src, writes the new record to slot di of dst (the 8 streams) and
+// returns it in r. In place: src == dst and si == di; every stream a slot's new value depends on is
+// read before that slot is written.
 // gather(bx, by, bz, a, b, c) returns the grid velocity (and mass) of node (bx + a, by + b, bz + c).
 template <class Gather>
-__device__ __forceinline__ void g2p_particle(const float* __restrict__ mats, const SceneGeo& geo,
-                                             float4* __restrict__ P, uint32_t cap, uint32_t i, float vmax,
+__device__ __forceinline__ void g2p_particle(const float* __restrict__ mats, const SceneGeo& geo, const float4* src,
+                                             uint32_t si, float4* dst, uint32_t di, uint32_t cap, float vmax,
                                              float4 r[8], Gather&& gather) {
-  const float4 x4 = P[i];
+  const float4 x4 = src[si];
   const float* m0 = mats + 12 * __float_as_uint(x4.w);
   const float bound = geo.bound;
-  const float vw0 = P[(size_t)cap + i].w;
+  const float vw0 = src[(size_t)cap + si].w;
   const f3 xp = mk(x4.x, x4.y, x4.z);
   const int bx = (int)floorf(xp.x - 0.5f), by = (int)floorf(xp.y - 0.5f), bz = (int)floorf(xp.z - 0.5f);
   const f3 fx = xp - mk((float)bx, (float)by, (float)bz);
@@ -207,7 +209,7 @@ __device__ __forceinline__ void g2p_particle(const float* __restrict__ mats, con
     tau = (m0[2] * fminf(J - 1.0f, 0.02f)) * ident() + (m0[4] * J) * (C + tr(C));
     F = ident();
   } else {
-    const float4 f0 = P[(size_t)5 * cap + i], f1 = P[(size_t)6 * cap + i], f2 = P[(size_t)7 * cap + i];
+    const float4 f0 = src[(size_t)5 * cap + si], f1 = src[(size_t)6 * cap + si], f2 = src[(size_t)7 * cap + si];
     F = mul(ident() + C, mat(mk(f0.x, f0.y, f0.z), mk(f1.x, f1.y, f1.z), mk(f2.x, f2.y, f2.z)));
     const float mu0 = m0[2], la0 = m0[3];
     if (kind == 1u) {
@@ -258,7 +260,7 @@ __device__ __forceinline__ void g2p_particle(const float* __restrict__ mats, con
   r[6] = make_float4(F.c[1].x, F.c[1].y, F.c[1].z, tau.c[2].x);
   r[7] = make_float4(F.c[2].x, F.c[2].y, F.c[2].z, tau.c[2].y);
 #pragma unroll
-  for (int k = 0; k < 8; k++) P[(size_t)k * cap + i] = r[k];
+  for (int k = 0; k < 8; k++) dst[(size_t)k * cap + di] = r[k];
 }
 
 template <int WAVES>
@@ -269,7 +271,23 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   float4 r[8];
   const SceneGeo& geo = pool.scenes[__float_as_uint(P[i].w) >> 3];
   const float4* __restrict__ V = pool.V;
-  g2p_particle(pool.mats, geo, P, pool.cap, i, vmax, r,
+  g2p_particle(pool.mats, geo, P, i, P, i, pool.cap, vmax, r,
+               [&](int bx, int by, int bz, int a, int b, int c) { return V[node_addr(bx + a, by + b, bz + c, geo)]; });
+}
+
+// G2P of the substep that sorted: slot i of the new order takes particle perm[i] of the old one, so
+// the sort's reorder rides on the record writes G2P makes anyway (src and dst are the two buffers).
+template <int WAVES>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_g2p_perm(
+    Pool pool, const float4* __restrict__ src, float4* __restrict__ dst, const uint32_t* __restrict__ perm, uint32_t n,
+    float vmax) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t si = perm[i];
+  float4 r[8];
+  const SceneGeo& geo = pool.scenes[__float_as_uint(src[si].w) >> 3];
+  const float4* __restrict__ V = pool.V;
+  g2p_particle(pool.mats, geo, src, si, dst, i, pool.cap, vmax, r,
                [&](int bx, int by, int bz, int a, int b, int c) { return V[node_addr(bx + a, by + b, bz + c, geo)]; });
 }
 
@@ -538,7 +556,7 @@ __device__ __forceinline__ void fallback_particle(const Pool& pool, const float4
 // LDS atomics hit different nodes).
 template <int WAVES>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_p2g_binned(
-    Pool pool, const float4* __restrict__ P, const uint32_t* __restrict__ perm, float4* __restrict__ dst) {
+    Pool pool, const float4* __restrict__ P, const uint32_t* __restrict__ perm) {
   __shared__ int tile[4 * kTileNodes];
   __shared__ int box[6];
   __shared__ uint32_t s_item, s_off;
@@ -609,9 +627,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
       const uint32_t j = start + li;
       float4 r[8];
       load_record(P, pool.cap, perm ? perm[j] : j, r);
-      if (dst)  // the sort's reorder folded into this pass: record j of the sorted order goes to slot j
-#pragma unroll
-        for (int q = 0; q < 8; q++) dst[(size_t)q * pool.cap + j] = r[q];
       const int b0 = (int)floorf(r[0].x - 0.5f), b1 = (int)floorf(r[0].y - 0.5f), b2 = (int)floorf(r[0].z - 0.5f);
       const bool inwin = (unsigned)(b0 - wx) < (unsigned)(4 + 2 * kDrift) &&
                          (unsigned)(b1 - wy) < (unsigned)(4 + 2 * kDrift) && (unsigned)(b2 - wz) < (unsigned)(4 + 2 * kDrift);
@@ -859,13 +874,12 @@ hipError_t launch_bin_dilate(const Pool& pool, hipStream_t s) {
   hipLaunchKernelGGL(k_bin_dilate, dim3(persistent_grid(pool.max_items, 4, 4096)), dim3(kThreads), 0, s, pool);
   return hipGetLastError();
 }
-hipError_t launch_p2g_binned(const Pool& pool, const float4* P, const uint32_t* perm, float4* dst, int waves,
-                             hipStream_t s) {
+hipError_t launch_p2g_binned(const Pool& pool, const float4* P, const uint32_t* perm, int waves, hipStream_t s) {
   const dim3 grid(persistent_grid(pool.max_items, 1, 2048));
   switch (waves) {
-    case 5: hipLaunchKernelGGL(k_p2g_binned<5>, grid, dim3(kThreads), 0, s, pool, P, perm, dst); break;
-    case 6: hipLaunchKernelGGL(k_p2g_binned<6>, grid, dim3(kThreads), 0, s, pool, P, perm, dst); break;
-    default: hipLaunchKernelGGL(k_p2g_binned<4>, grid, dim3(kThreads), 0, s, pool, P, perm, dst); break;
+    case 5: hipLaunchKernelGGL(k_p2g_binned<5>, grid, dim3(kThreads), 0, s, pool, P, perm); break;
+    case 6: hipLaunchKernelGGL(k_p2g_binned<6>, grid, dim3(kThreads), 0, s, pool, P, perm); break;
+    default: hipLaunchKernelGGL(k_p2g_binned<4>, grid, dim3(kThreads), 0, s, pool, P, perm); break;
   }
   return hipGetLastError();
 }
@@ -896,6 +910,18 @@ hipError_t launch_resolve_tiles(const Pool& pool, hipStream_t s) {
 hipError_t launch_clear_a(const Pool& pool, bool use_s, hipStream_t s) {
   hipLaunchKernelGGL(k_clear_a, dim3(persistent_grid(2 * pool.nblocks, 4, 8192)), dim3(kThreads), 0, s, pool,
                      use_s ? 1 : 0);
+  return hipGetLastError();
+}
+hipError_t launch_g2p_perm(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, uint32_t n, float vmax,
+                           int waves, hipStream_t s) {
+  if (!n) return hipSuccess;
+  const dim3 grid(blocks_for(n, kThreads));
+  switch (waves) {
+    case 4: hipLaunchKernelGGL(k_g2p_perm<4>, grid, dim3(kThreads), 0, s, pool, src, dst, perm, n, vmax); break;
+    case 5: hipLaunchKernelGGL(k_g2p_perm<5>, grid, dim3(kThreads), 0, s, pool, src, dst, perm, n, vmax); break;
+    case 6: hipLaunchKernelGGL(k_g2p_perm<6>, grid, dim3(kThreads), 0, s, pool, src, dst, perm, n, vmax); break;
+    default: hipLaunchKernelGGL(k_g2p_perm<3>, grid, dim3(kThreads), 0, s, pool, src, dst, perm, n, vmax); break;
+  }
   return hipGetLastError();
 }
 hipError_t launch_g2p(const Pool& pool, float4* P, uint32_t n, float vmax, int waves, hipStream_t s) {

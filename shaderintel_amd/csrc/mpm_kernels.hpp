@@ -133,10 +133,8 @@ hipError_t launch_bin_transpose(const Pool& pool, const uint32_t* keys_sorted, c
 hipError_t launch_bin_dilate(const Pool& pool, hipStream_t s);
 
 // P2G over the work items into tiles (+ fallback atomics on A). perm: sorted position -> particle
-// (nullptr: identity). dst != nullptr also stores every record at its sorted position in dst (the
-// sort's reorder folded into this pass).
-hipError_t launch_p2g_binned(const Pool& pool, const float4* P, const uint32_t* perm, float4* dst, int waves,
-                             hipStream_t s);
+// (nullptr: identity).
+hipError_t launch_p2g_binned(const Pool& pool, const float4* P, const uint32_t* perm, int waves, hipStream_t s);
 // P2G with one global atomic per contribution (the reference's forceGlobal path), any order.
 hipError_t launch_p2g_global(const Pool& pool, const float4* P, uint32_t n, hipStream_t s);
 // Writes n (<= 64) 32-bit words to dst from a kernel argument (ordered with the stream's work).
@@ -148,6 +146,9 @@ hipError_t launch_resolve_tiles(const Pool& pool, hipStream_t s);
 // Zeroes A over S (use_s) and D.
 hipError_t launch_clear_a(const Pool& pool, bool use_s, hipStream_t s);
 hipError_t launch_g2p(const Pool& pool, float4* P, uint32_t n, float vmax, int waves, hipStream_t s);
+// G2P that also applies a pending reorder: slot i of dst takes particle perm[i] of src.
+hipError_t launch_g2p_perm(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, uint32_t n, float vmax,
+                           int waves, hipStream_t s);
 // Reference-layout export of one scene: mode 0 = A (i32 words), mode 1 = V (f32 bits, 0 outside the
 // node blocks the last grid update wrote).
 hipError_t launch_grid_export(const Pool& pool, const SceneGeo& geo, int mode, int4* out, hipStream_t s);

@@ -60,7 +60,7 @@ mpm_status from_hip(hipError_t e, const char* where) {
 // What the grid buffers hold after the last call: nothing, the i32 P2G result in A, the P2G result
 // split over tiles and A (inside a step only), or velocities in V.
 enum GridState { kGridClean = 0, kGridA = 1, kGridTiles = 2, kGridVelocity = 3 };
-enum BinMode { kBinOnly = 0, kBinReorder = 1, kBinReorderInP2G = 2 };
+enum BinMode { kBinOnly = 0, kBinReorder = 1, kBinReorderInG2P = 2 };
 
 
 int ceil_log2(uint64_t v) {
@@ -141,7 +141,7 @@ struct mpm_pool {
   uint64_t step_count = 0;
   bool bin_valid = false;
   const uint32_t* perm = nullptr;  // sorted position -> particle (binning without reorder)
-  bool reorder_pending = false;    // perm is the sort's new order, applied by the next P2G
+  bool reorder_in_g2p = false;     // perm is the sort's new order, applied by this substep's G2P
   bool needs_repack = false;       // a removed scene's particles are still in the pool
   int grid_state = kGridClean;
   bool a_dirty_s = false, a_dirty_d = false;  // A holds data over S / over D
@@ -342,17 +342,17 @@ mpm_status clean_a(mpm_pool* p) {
 }
 
 mpm_status apply_pending_reorder(mpm_pool* p) {
-  if (!p->reorder_pending) return MPM_OK;
+  if (!p->reorder_in_g2p) return MPM_OK;
   HIP_TRY(launch_permute(p->P[p->cur], p->P[p->cur ^ 1], p->cap, p->count, p->perm, p->stream));
   p->cur ^= 1;
   p->perm = nullptr;
-  p->reorder_pending = false;
+  p->reorder_in_g2p = false;
   recompute_offsets(p);
   return MPM_OK;
 }
 
-// Sort keys, then reorder the particles now (MPM._sort, mpm.js:614-621), at the next P2G, or not at
-// all (the sorted order is kept as a permutation), and rebuild work items, map and S.
+// Sort keys, then reorder the particles now (MPM._sort, mpm.js:614-621), at this substep's G2P, or
+// not at all (the sorted order is kept as a permutation), and rebuild work items, map and S.
 mpm_status repack(mpm_pool* p);
 
 mpm_status do_bin(mpm_pool* p, int mode) {
@@ -361,7 +361,7 @@ mpm_status do_bin(mpm_pool* p, int mode) {
   const Pool pv = view(p);
   HIP_TRY(launch_begin_bin(pv, p->stream));
   p->perm = nullptr;
-  p->reorder_pending = false;
+  p->reorder_in_g2p = false;
   if (p->count == 0) {
     p->bin_valid = true;
     return MPM_OK;
@@ -381,7 +381,7 @@ mpm_status do_bin(mpm_pool* p, int mode) {
     recompute_offsets(p);
   } else {
     p->perm = p->order;
-    p->reorder_pending = mode == kBinReorderInP2G;
+    p->reorder_in_g2p = mode == kBinReorderInG2P;
   }
   HIP_TRY(launch_bin_dilate(pv, p->stream));
   p->bin_valid = true;
@@ -434,14 +434,7 @@ mpm_status do_p2g(mpm_pool* p, bool force_global) {
     HIP_TRY(launch_p2g_global(pv, p->P[p->cur], p->count, p->stream));
     p->grid_state = kGridA;
   } else {
-    float4* dst = p->reorder_pending ? p->P[p->cur ^ 1] : nullptr;
-    HIP_TRY(launch_p2g_binned(pv, p->P[p->cur], p->perm, dst, p->p2g_waves, p->stream));
-    if (p->reorder_pending) {
-      p->cur ^= 1;
-      p->perm = nullptr;
-      p->reorder_pending = false;
-      recompute_offsets(p);
-    }
+    HIP_TRY(launch_p2g_binned(pv, p->P[p->cur], p->perm, p->p2g_waves, p->stream));
     p->grid_state = kGridTiles;
   }
   p->a_dirty_d = true;
@@ -457,6 +450,14 @@ mpm_status do_grid(mpm_pool* p) {
 }
 
 mpm_status do_g2p(mpm_pool* p, float vmax) {  // mpm.js:598-600
+  if (p->reorder_in_g2p) {  // the sort's reorder, applied by the record writes
+    HIP_TRY(launch_g2p_perm(view(p), p->P[p->cur], p->P[p->cur ^ 1], p->perm, p->count, vmax, p->g2p_waves, p->stream));
+    p->cur ^= 1;
+    p->perm = nullptr;
+    p->reorder_in_g2p = false;
+    recompute_offsets(p);
+    return MPM_OK;
+  }
   HIP_TRY(launch_g2p(view(p), p->P[p->cur], p->count, vmax, p->g2p_waves, p->stream));
   return MPM_OK;
 }
@@ -477,7 +478,7 @@ mpm_status mark(mpm_pool* p) {
 mpm_status substep_front(mpm_pool* p, uint32_t index) {
   ST_TRY(mark(p));
   if (p->step_count % p->sort_every == 0) {
-    ST_TRY(do_bin(p, kBinReorderInP2G));
+    ST_TRY(do_bin(p, kBinReorderInG2P));
   } else if (!p->bin_valid) {
     ST_TRY(do_bin(p, kBinOnly));
   }
@@ -572,7 +573,7 @@ mpm_status pool_step(mpm_pool* p, uint32_t substeps, float vmax) {
       // host-side effects of one period: the sort's reorder swapped the buffers once
       p->cur ^= 1;
       p->perm = nullptr;
-      p->reorder_pending = false;
+      p->reorder_in_g2p = false;
       p->bin_valid = true;
       p->grid_state = kGridVelocity;
       p->a_dirty_d = p->a_dirty_s = false;
