@@ -104,10 +104,16 @@ mpm_status mpm_read_grid(mpm_sim* sim, int32_t* words);
  * instead of 128. Blocking. */
 mpm_status mpm_read_points(mpm_sim* sim, float* pts, float* vel, uint32_t offset, uint32_t n, const double origin[3],
                            double dx, double vscale);
-/* Device pointers of the 8 particle streams (stream k holds vec4 k of every record as float4,
- * capacity entries), for consumers on the same GPU (e.g. an interop renderer). Valid until the next
+/* Device pointers of the 8 particle streams S0..S7 (capacity float4 entries each), for consumers on
+ * the same GPU (e.g. an interop renderer). Float f of a reference record (mpm.js:54-63) is component
+ * MPM_REC_SLOT[f] & 3 of stream MPM_REC_SLOT[f] >> 2: S0 = (x.xyz, material), S1 = (v.xyz, tau01),
+ * S2..S4 = (C column k, tau_kk), S5 = (tau02, tau12, J, F22), S6 = (F00, F10, F20, F01),
+ * S7 = (F11, F21, F02, F12) with Frc = row r of column c (the reference's f0..f2 are F's columns).
+ * Positions and the material slot are S0 as the render layers read them (mpm.js:642-679). Valid until the next
  * call that steps, sorts or writes: a sort's reorder swaps the particle buffers, as P and P2 swap in
  * the reference (types/shaderintel.d.ts:873-874). */
+#define MPM_REC_SLOT                                                                                   \
+  {0, 1, 2, 3, 4, 5, 6, 22, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 24, 25, 26, 7, 27, 28, 29, 20, 30, 31, 23, 21}
 mpm_status mpm_particle_streams(mpm_sim* sim, void* streams[8], uint32_t* capacity);
 
 /* The grid between the grid update and G2P — the point where the reference calls its coupling

@@ -38,9 +38,12 @@ __device__ __forceinline__ SceneGeo uniform_geo(const Pool& pool, uint32_t scene
 
 __device__ __forceinline__ const SceneGeo& scene_of_row(const Pool& pool, uint32_t row) { return pool.scenes[row >> 3]; }
 
-__device__ __forceinline__ void load_record(const float4* __restrict__ P, uint32_t cap, uint32_t i, float4 r[8]) {
+// The streams P2G reads: S0-S5 (x, row, v, C, tau; S5 also carries J and F22, which P2G ignores).
+constexpr int kP2GStreams = 6;
+__device__ __forceinline__ void load_p2g_record(const float4* __restrict__ P, uint32_t cap, uint32_t i,
+                                                float4 r[kP2GStreams]) {
 #pragma unroll
-  for (int k = 0; k < 8; k++) r[k] = P[(size_t)k * cap + i];
+  for (int k = 0; k < kP2GStreams; k++) r[k] = P[(size_t)k * cap + i];
 }
 
 __device__ __forceinline__ uint32_t compact3(uint32_t x) {  // inverse of spread3
@@ -54,14 +57,15 @@ __device__ __forceinline__ uint32_t compact3(uint32_t x) {  // inverse of spread
 
 // The 27 fixed-point contributions of one particle (mpm.js:318-349). add(gx, gy, gz, q0..q3).
 template <class Add>
-__device__ __forceinline__ void p2g_particle(const float4 r[8], const float* __restrict__ mats, float vol4, Add&& add) {
+__device__ __forceinline__ void p2g_particle(const float4 r[kP2GStreams], const float* __restrict__ mats, float vol4,
+                                             Add&& add) {
   const f3 xp = mk(r[0].x, r[0].y, r[0].z);
   const int bx = (int)floorf(xp.x - 0.5f), by = (int)floorf(xp.y - 0.5f), bz = (int)floorf(xp.z - 0.5f);
   const f3 fx = xp - mk((float)bx, (float)by, (float)bz);
   f3 w[3];
   bspline(fx, w);
   const float mass = mats[12 * __float_as_uint(r[0].w) + 1];
-  const m3 tau = mat(mk(r[2].w, r[5].w, r[6].w), mk(r[5].w, r[3].w, r[7].w), mk(r[6].w, r[7].w, r[4].w));
+  const m3 tau = mat(mk(r[2].w, r[1].w, r[5].x), mk(r[1].w, r[3].w, r[5].y), mk(r[5].x, r[5].y, r[4].w));
   const m3 C = mat(mk(r[2].x, r[2].y, r[2].z), mk(r[3].x, r[3].y, r[3].z), mk(r[4].x, r[4].y, r[4].z));
   const m3 affine = (-vol4) * tau + mass * C;
   const f3 mv = mass * mk(r[1].x, r[1].y, r[1].z);
@@ -168,9 +172,10 @@ __device__ __forceinline__ void g2p_particle(const float* __restrict__ mats, con
                                              uint32_t si, float4* dst, uint32_t di, uint32_t cap, float vmax,
                                              float4 r[8], Gather&& gather) {
   const float4 x4 = src[si];
+  const float4 s5 = src[(size_t)5 * cap + si];  // t02, t12, J, F22
   const float* m0 = mats + 12 * __float_as_uint(x4.w);
   const float bound = geo.bound;
-  const float vw0 = src[(size_t)cap + si].w;
+  const float vw0 = s5.z;
   const f3 xp = mk(x4.x, x4.y, x4.z);
   const int bx = (int)floorf(xp.x - 0.5f), by = (int)floorf(xp.y - 0.5f), bz = (int)floorf(xp.z - 0.5f);
   const f3 fx = xp - mk((float)bx, (float)by, (float)bz);
@@ -209,8 +214,8 @@ __device__ __forceinline__ void g2p_particle(const float* __restrict__ mats, con
     tau = (m0[2] * fminf(J - 1.0f, 0.02f)) * ident() + (m0[4] * J) * (C + tr(C));
     F = ident();
   } else {
-    const float4 f0 = src[(size_t)5 * cap + si], f1 = src[(size_t)6 * cap + si], f2 = src[(size_t)7 * cap + si];
-    F = mul(ident() + C, mat(mk(f0.x, f0.y, f0.z), mk(f1.x, f1.y, f1.z), mk(f2.x, f2.y, f2.z)));
+    const float4 f6 = src[(size_t)6 * cap + si], f7 = src[(size_t)7 * cap + si];
+    F = mul(ident() + C, mat(mk(f6.x, f6.y, f6.z), mk(f6.w, f7.x, f7.y), mk(f7.z, f7.w, s5.w)));
     const float mu0 = m0[2], la0 = m0[3];
     if (kind == 1u) {
       const float J = fmaxf(det3(F), 0.05f);
@@ -252,13 +257,13 @@ __device__ __forceinline__ void g2p_particle(const float* __restrict__ mats, con
     }
   }
   r[0] = make_float4(x.x, x.y, x.z, x4.w);
-  r[1] = make_float4(v.x, v.y, v.z, vw);
+  r[1] = make_float4(v.x, v.y, v.z, tau.c[1].x);
   r[2] = make_float4(C.c[0].x, C.c[0].y, C.c[0].z, tau.c[0].x);
   r[3] = make_float4(C.c[1].x, C.c[1].y, C.c[1].z, tau.c[1].y);
   r[4] = make_float4(C.c[2].x, C.c[2].y, C.c[2].z, tau.c[2].z);
-  r[5] = make_float4(F.c[0].x, F.c[0].y, F.c[0].z, tau.c[1].x);
-  r[6] = make_float4(F.c[1].x, F.c[1].y, F.c[1].z, tau.c[2].x);
-  r[7] = make_float4(F.c[2].x, F.c[2].y, F.c[2].z, tau.c[2].y);
+  r[5] = make_float4(tau.c[2].x, tau.c[2].y, vw, F.c[2].z);
+  r[6] = make_float4(F.c[0].x, F.c[0].y, F.c[0].z, F.c[1].x);
+  r[7] = make_float4(F.c[1].y, F.c[1].z, F.c[2].x, F.c[2].y);
 #pragma unroll
   for (int k = 0; k < 8; k++) dst[(size_t)k * cap + di] = r[k];
 }
@@ -291,24 +296,35 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
                [&](int bx, int by, int bz, int a, int b, int c) { return V[node_addr(bx + a, by + b, bz + c, geo)]; });
 }
 
+__device__ __forceinline__ float& comp4(float4& q, int c) { return c == 0 ? q.x : (c == 1 ? q.y : (c == 2 ? q.z : q.w)); }
+
+// Reference records (32 floats each) <-> the 8 streams, one particle per thread (kRecSlot).
 __global__ void k_records_in(const float4* __restrict__ aos, float4* __restrict__ P, uint32_t cap, uint32_t offset,
                              uint32_t n, uint32_t row_base) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // one float4 per thread
-  if (t >= (size_t)n * 8) return;
-  const uint32_t i = (uint32_t)(t >> 3), k = (uint32_t)(t & 7);
-  float4 q = aos[t];
-  if (k == 0) q.w = __uint_as_float(__float_as_uint(q.w) + row_base);  // scene-local -> pool material row
-  P[(size_t)k * cap + offset + i] = q;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 in[8], out[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) in[k] = aos[(size_t)i * 8 + k];
+#pragma unroll
+  for (int f = 0; f < 32; f++) comp4(out[kRecSlot[f] >> 2], kRecSlot[f] & 3) = comp4(in[f >> 2], f & 3);
+  out[0].w = __uint_as_float(__float_as_uint(out[0].w) + row_base);  // scene-local -> pool material row
+#pragma unroll
+  for (int k = 0; k < 8; k++) P[(size_t)k * cap + offset + i] = out[k];
 }
 
 __global__ void k_records_out(const float4* __restrict__ P, uint32_t cap, uint32_t offset, uint32_t n,
                               uint32_t row_base, float4* __restrict__ aos) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)n * 8) return;
-  const uint32_t i = (uint32_t)(t >> 3), k = (uint32_t)(t & 7);
-  float4 q = P[(size_t)k * cap + offset + i];
-  if (k == 0) q.w = __uint_as_float(__float_as_uint(q.w) - row_base);
-  aos[t] = q;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 in[8], out[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) in[k] = P[(size_t)k * cap + offset + i];
+#pragma unroll
+  for (int f = 0; f < 32; f++) comp4(out[f >> 2], f & 3) = comp4(in[kRecSlot[f] >> 2], kRecSlot[f] & 3);
+  out[0].w = __uint_as_float(__float_as_uint(out[0].w) - row_base);
+#pragma unroll
+  for (int k = 0; k < 8; k++) aos[(size_t)i * 8 + k] = out[k];
 }
 
 __global__ void k_rescale(float4* __restrict__ P, uint32_t cap, uint32_t offset, uint32_t n, float k) {  // mpm.js:405-413
@@ -338,10 +354,10 @@ __global__ void k_block_info(uint2* __restrict__ info, uint32_t base, uint32_t n
 __global__ void k_begin_p2g(uint32_t* ctr, uint32_t* arena_ctr) {
   const int t = threadIdx.x;  // kArenaRegions threads
   arena_ctr[t * kArenaStride] = 0u;
+  if (t < kWorkSplits) arena_ctr[kArenaRegions * kArenaStride + kWorkStride * t] = 0u;
   if (t == 0) {
     ctr[kCtrEpoch] += 1u;
     ctr[kCtrD] = 0u;
-    ctr[kCtrWork] = 0u;
   }
 }
 
@@ -529,7 +545,7 @@ __device__ __forceinline__ void stamp_fallback(const Pool& pool, uint32_t b, uin
       if (!use_s || pool.sstamp[b] != bin_epoch) pool.dlist[atomicAdd(pool.ctr + kCtrD, 1u)] = b;
 }
 
-__device__ __forceinline__ void fallback_particle(const Pool& pool, const float4 r[8], uint32_t epoch, bool use_s,
+__device__ __forceinline__ void fallback_particle(const Pool& pool, const float4 r[kP2GStreams], uint32_t epoch, bool use_s,
                                                   uint32_t bin_epoch) {
   const SceneGeo& geo = scene_of_row(pool, __float_as_uint(r[0].w));
   const float vol4 = geo.vol4;
@@ -564,9 +580,25 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   const uint32_t items = pool.ctr[kCtrItems];
   const uint32_t epoch = pool.ctr[kCtrEpoch], bin_epoch = pool.ctr[kCtrBinEpoch];
   const uint32_t mshift = pool.key_bits - 6;
+  uint32_t* const wctr = pool.arena_ctr + kArenaRegions * kArenaStride;
+  uint32_t range = blockIdx.x % kWorkSplits, spent = 0;  // ranges found used up (tid 0 only)
   for (;;) {
     __syncthreads();  // the previous item is finished with the tile and with s_item
-    if (tid == 0) s_item = atomicAdd(pool.ctr + kCtrWork, 1u);
+    if (tid == 0) {
+      uint32_t next = 0xffffffffu;
+      while (spent < (uint32_t)kWorkSplits) {
+        const uint32_t lo = (uint32_t)((uint64_t)items * range / kWorkSplits);
+        const uint32_t hi = (uint32_t)((uint64_t)items * (range + 1) / kWorkSplits);
+        const uint32_t c = atomicAdd(wctr + kWorkStride * range, 1u);
+        if (c < hi - lo) {
+          next = lo + c;
+          break;
+        }
+        spent++;
+        range = (range + 1) % kWorkSplits;
+      }
+      s_item = next;
+    }
     __syncthreads();
     const uint32_t k = s_item;
     if (k >= items) break;
@@ -625,8 +657,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
     __syncthreads();
     for (uint32_t li = tid; li < cnt; li += kThreads) {
       const uint32_t j = start + li;
-      float4 r[8];
-      load_record(P, pool.cap, perm ? perm[j] : j, r);
+      float4 r[kP2GStreams];
+      load_p2g_record(P, pool.cap, perm ? perm[j] : j, r);
       const int b0 = (int)floorf(r[0].x - 0.5f), b1 = (int)floorf(r[0].y - 0.5f), b2 = (int)floorf(r[0].z - 0.5f);
       const bool inwin = (unsigned)(b0 - wx) < (unsigned)(4 + 2 * kDrift) &&
                          (unsigned)(b1 - wy) < (unsigned)(4 + 2 * kDrift) && (unsigned)(b2 - wz) < (unsigned)(4 + 2 * kDrift);
@@ -657,8 +689,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
 __global__ __launch_bounds__(kThreads) void k_p2g_global(Pool pool, const float4* __restrict__ P, uint32_t n) {
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
-  float4 r[8];
-  load_record(P, pool.cap, i, r);
+  float4 r[kP2GStreams];
+  load_p2g_record(P, pool.cap, i, r);
   fallback_particle(pool, r, pool.ctr[kCtrEpoch], false, 0);
 }
 
@@ -780,7 +812,8 @@ __global__ void k_points_out(const float4* __restrict__ P, uint32_t cap, uint32_
                        __uint_as_float(__float_as_uint(x.w) - row_base));
   if (vel) {
     const float4 v = P[(size_t)cap + offset + i];
-    vel[i] = make_float4((float)((double)v.x * vscale), (float)((double)v.y * vscale), (float)((double)v.z * vscale), v.w);
+    const float J = P[(size_t)5 * cap + offset + i].z;
+    vel[i] = make_float4((float)((double)v.x * vscale), (float)((double)v.y * vscale), (float)((double)v.z * vscale), J);
   }
 }
 
@@ -807,13 +840,13 @@ inline unsigned persistent_grid(uint32_t max_work, unsigned per_block, unsigned 
 hipError_t launch_records_in(const float4* aos, float4* P, uint32_t cap, uint32_t offset, uint32_t n, uint32_t row_base,
                              hipStream_t s) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_records_in, dim3(blocks_for((size_t)n * 8, 256)), dim3(256), 0, s, aos, P, cap, offset, n, row_base);
+  hipLaunchKernelGGL(k_records_in, dim3(blocks_for(n, 256)), dim3(256), 0, s, aos, P, cap, offset, n, row_base);
   return hipGetLastError();
 }
 hipError_t launch_records_out(const float4* P, uint32_t cap, uint32_t offset, uint32_t n, uint32_t row_base, float4* aos,
                               hipStream_t s) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_records_out, dim3(blocks_for((size_t)n * 8, 256)), dim3(256), 0, s, P, cap, offset, n, row_base,
+  hipLaunchKernelGGL(k_records_out, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, cap, offset, n, row_base,
                      aos);
   return hipGetLastError();
 }

@@ -4,11 +4,16 @@
 // simulation is a pool with one scene. Every phase is one launch over all scenes.
 //
 // Device data layout
-//  - Particles: one pool of structure-of-float4 streams. Stream k (0..7) holds vec4 k of every
-//    record (x|row, v|J, c0|t00, c1|t11, c2|t22, f0|t01, f1|t02, f2|t12), so a wavefront reading one
-//    stream touches 64 consecutive 16-byte elements. Positions are in the scene's own lattice
-//    coordinates; x.w holds the particle's global material row (scene * 8 + material), so
-//    row >> 3 is the particle's scene.
+//  - Particles: one pool of 8 structure-of-float4 streams (S0..S7, 128 bytes per particle), so a
+//    wavefront reading one stream touches 64 consecutive 16-byte elements. The 32 floats of the
+//    reference record are regrouped so that P2G reads 96 bytes (S0-S5) and G2P reads 64 (S0, S5,
+//    S6, S7; fluids only S0 and S5):
+//      S0 = x.xyz | row            S1 = v.xyz | t01
+//      S2 = C col 0 | t00          S3 = C col 1 | t11          S4 = C col 2 | t22
+//      S5 = t02, t12, J, F22       S6 = F00, F10, F20, F01     S7 = F11, F21, F02, F12
+//    (t = the stress tau, F col-major: Frc = row r of column c; kRecSlot maps the reference record.)
+//    Positions are in the scene's own lattice coordinates; x.w holds the particle's global material
+//    row (scene * 8 + material), so row >> 3 is the particle's scene.
 //  - Grid: a pool of 4x4x4-node blocks. Scene s owns blocks [block_base, block_base + NBX*NBY*NBZ),
 //    block-major inside (block = bx + NBX*(by + NBY*bz), node = lx + 4*(ly + 4*lz)): 1 KiB of int4
 //    (A, the i32 fixed-point grid written only by the global-atomic paths) or of float4 (V,
@@ -39,6 +44,18 @@ namespace mpm {
 constexpr int kDrift = 3;                                      // cells a base may drift outside its block
 constexpr int kTileSpan = 4 + 2 + 2 * kDrift;                  // nodes per axis a work item can touch (12)
 constexpr int kTileNodes = kTileSpan * kTileSpan * kTileSpan;  // 1728 nodes = 27 KiB of int4
+// Float f of the reference's 32-float record (x, v, c0, c1, c2, f0, f1, f2 as vec4f, mpm.js:54-63)
+// is component kRecSlot[f] & 3 of stream kRecSlot[f] >> 2.
+constexpr uint8_t kRecSlot[32] = {
+    0,  1,  2,  3,   // x.xyz, row -> S0
+    4,  5,  6,  22,  // v.xyz -> S1.xyz; J -> S5.z
+    8,  9,  10, 11,  // c0.xyz, t00 -> S2
+    12, 13, 14, 15,  // c1.xyz, t11 -> S3
+    16, 17, 18, 19,  // c2.xyz, t22 -> S4
+    24, 25, 26, 7,   // f0.xyz -> S6.xyz; t01 -> S1.w
+    27, 28, 29, 20,  // f1.xyz -> S6.w, S7.x, S7.y; t02 -> S5.x
+    30, 31, 23, 21,  // f2.xyz -> S7.z, S7.w, S5.w; t12 -> S5.y
+};
 constexpr int kMatRows = 8;                                    // material rows per scene (row >> 3 = scene)
 constexpr int kColRows = 16;                                   // collider rows per scene
 // The tile arena is split into kArenaRegions regions, each with its own bump counter on its own
@@ -46,6 +63,14 @@ constexpr int kColRows = 16;                                   // collider rows 
 // on one address (a single shared counter cost 0.8 ms per P2G at 9k items on MI355X).
 constexpr int kArenaRegions = 256;
 constexpr int kArenaStride = 16;  // u32 words between region counters
+// Work items of P2G are split into kWorkSplits contiguous ranges, each with its own counter on its
+// own 128-byte line after the arena's region counters. Workgroup w starts on range w % kWorkSplits
+// (the XCD it was dispatched to) and moves to the next range when its own is used up, so no single
+// address takes every item's atomic (one shared counter cost 7% of P2G on MI355X).
+constexpr int kWorkSplits = 8;
+constexpr int kWorkStride = 32;
+constexpr int kArenaCtrWords = kArenaRegions * kArenaStride + kWorkSplits * kWorkStride;
+static_assert(kWorkSplits <= kArenaRegions, "k_begin_p2g clears the work counters with its region threads");
 
 enum Counter : int {
   kCtrEpoch = 0,     // P2G epoch (stamps of D)
@@ -53,7 +78,7 @@ enum Counter : int {
   kCtrItems = 2,     // work items of the current binning
   kCtrD = 3,         // |D|
   kCtrS = 4,         // |S|
-  kCtrWork = 5,      // next work item of an item-persistent kernel
+  kCtrWork = 5,      // unused (P2G's work counters are in arena_ctr)
   kCtrVEpoch = 6,    // P2G epoch whose grid update wrote V
   kCtrVBinEpoch = 7, // binning epoch of that grid update (0xffffffff: S not used)
   kCounters = 8
@@ -98,7 +123,8 @@ struct Pool {
   const float* cols;       // [scenes * kColRows * 12]
   const SceneGeo* scenes;
   uint32_t* ctr;           // [kCounters]
-  uint32_t* arena_ctr;     // [kArenaRegions * kArenaStride] nodes handed out per region in the current P2G
+  uint32_t* arena_ctr;     // [kArenaCtrWords]: nodes handed out per region in the current P2G, then the
+                           // kWorkSplits work counters
 };
 
 // Launch helpers; all enqueue on `s` and none synchronizes. P is the current particle buffer.

@@ -634,7 +634,7 @@ mpm_status create_pool(mpm_ctx* c, uint32_t cap, uint32_t max_scenes, uint32_t m
   if (e == hipSuccess) e = dmalloc(&p->cols, (size_t)max_scenes * kColRows * 12);
   if (e == hipSuccess) e = dmalloc(&p->scenes_d, max_scenes);
   if (e == hipSuccess) e = dmalloc(&p->ctr, kCounters);
-  if (e == hipSuccess) e = dmalloc(&p->arena_ctr, (size_t)kArenaRegions * kArenaStride);
+  if (e == hipSuccess) e = dmalloc(&p->arena_ctr, (size_t)kArenaCtrWords);
   if (e == hipSuccess) e = dmalloc(&p->keys[0], cap);
   if (e == hipSuccess) e = dmalloc(&p->keys[1], cap);
   if (e == hipSuccess) e = dmalloc(&p->vals[0], cap);
@@ -653,7 +653,7 @@ mpm_status create_pool(mpm_ctx* c, uint32_t cap, uint32_t max_scenes, uint32_t m
   if (e == hipSuccess) e = hipMemsetAsync(p->stamp, 0xff, (size_t)max_blocks * 4, p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->sstamp, 0xff, (size_t)max_blocks * 4, p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->ctr, 0, kCounters * 4, p->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(p->arena_ctr, 0, (size_t)kArenaRegions * kArenaStride * 4, p->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(p->arena_ctr, 0, (size_t)kArenaCtrWords * 4, p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->mats, 0, (size_t)max_scenes * kMatRows * 48, p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->cols, 0, (size_t)max_scenes * kColRows * 48, p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->scenes_d, 0, (size_t)max_scenes * sizeof(SceneGeo), p->stream);

@@ -24,6 +24,10 @@ LIB_PATH = os.path.join(_PKG, "lib", "libmpm_hip.so")
 FLUID, JELLY, SNOW, SAND = 0, 1, 2, 3
 MPM_KINDS = {"fluid": FLUID, "jelly": JELLY, "elastic": JELLY, "snow": SNOW, "sand": SAND}
 MAX_DENSITY = 50  # mpm.js:34
+# Device particle streams (MPM_REC_SLOT in include/mpm.h): float f of a reference record (mpm.js:54-63)
+# is component REC_SLOT[f] % 4 of stream REC_SLOT[f] // 4.
+REC_SLOT = (0, 1, 2, 3, 4, 5, 6, 22, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 24, 25, 26, 7, 27, 28, 29, 20, 30,
+            31, 23, 21)
 
 # mpm.js:38-46
 MATERIALS = {
@@ -555,7 +559,9 @@ class MPM:
         return (pts, vel) if velocity else pts
 
     def particle_streams(self):
-        """Device pointers of the 8 particle streams and their capacity (valid until the next step)."""
+        """Device pointers of the 8 particle streams and their capacity (valid until the next step).
+
+        Float f of a reference record is component ``REC_SLOT[f] % 4`` of stream ``REC_SLOT[f] // 4``."""
         ptrs = (ctypes.c_void_p * 8)()
         cap = ctypes.c_uint32()
         _check(load_library().mpm_particle_streams(self._h, ptrs, ctypes.byref(cap)), "MPM.particle_streams")

@@ -47,3 +47,21 @@ def test_host_helpers_without_gpu():
     L.mpm_material_row(1, 1.0, 0.35, 0.3, 0.0, 10.0, 0.025, 0.0075, 30.0, 2, out.ctypes.data_as(ctypes.c_void_p))
     assert out[1] == np.float32(1 / 8)
     assert SIError("x", "no-device").code == "no-device"
+
+
+def test_record_slot_tables_agree():
+    """The stream slot of each reference record float is stated three times: the public header
+    (MPM_REC_SLOT), the kernels (kRecSlot) and the Python mirror (REC_SLOT). They must agree and be a
+    permutation of the 32 record floats."""
+    from shaderintel_amd.mpm import REC_SLOT
+    hdr = open(os.path.join(ROOT, "include", "mpm.h")).read()
+    m = re.search(r"#define MPM_REC_SLOT\s*\\\s*\{([^}]*)\}", hdr)
+    assert m
+    header = [int(v) for v in m.group(1).split(",")]
+    ker = open(os.path.join(ROOT, "shaderintel_amd", "csrc", "mpm_kernels.hpp")).read()
+    m = re.search(r"kRecSlot\[32\] = \{(.*?)\};", ker, flags=re.S)
+    assert m
+    kernels = [int(v) for v in re.findall(r"\b\d+\b", re.sub(r"//[^\n]*", "", m.group(1)))]
+    assert header == kernels == list(REC_SLOT)
+    assert sorted(header) == list(range(32))
+    assert header[:4] == [0, 1, 2, 3]  # position and material stay in stream 0 (the render layers' slots)

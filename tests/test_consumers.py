@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from helpers import bits
-from shaderintel_amd.mpm import MPM, spatial_grid
+from shaderintel_amd.mpm import MPM, REC_SLOT, spatial_grid
 
 pytestmark = pytest.mark.gpu
 
@@ -40,11 +40,15 @@ def test_particle_streams_hold_the_records(ctx):
     assert cap >= sim.count and all(ptrs)
     hip = ctypes.CDLL("libamdhip64.so")
     n = sim.count
+    streams = np.zeros((8, n, 4), np.float32)
     for k in range(8):
-        host = np.zeros((n, 4), np.float32)
-        assert hip.hipMemcpy(host.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(ptrs[k]), ctypes.c_size_t(n * 16),
-                             2) == 0  # hipMemcpyDeviceToHost
-        assert np.array_equal(bits(host), bits(raw[:, 4 * k:4 * k + 4])), f"stream {k}"
+        assert hip.hipMemcpy(streams[k].ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(ptrs[k]),
+                             ctypes.c_size_t(n * 16), 2) == 0  # hipMemcpyDeviceToHost
+    assert sorted(REC_SLOT) == list(range(32))
+    for f, slot in enumerate(REC_SLOT):
+        assert np.array_equal(bits(streams[slot // 4, :, slot % 4]), bits(raw[:, f])), f"record float {f}"
+    # positions and the material slot where the render layers read them (mpm.js:642-679)
+    assert np.array_equal(bits(streams[0]), bits(raw[:, 0:4]))
     sim.destroy()
 
 
