@@ -436,69 +436,86 @@ __global__ void k_bin_emit(Pool pool, const uint32_t* __restrict__ keys, uint32_
   }
 }
 
-// Within-block transposition of the sorted order: the particles of item k, sorted by cell, are
-// reordered by (rank within cell, cell). pos(r, c) = S(r) + #{c' < c : n_c' > r} with
-// S(r) = sum_c' min(n_c', r) over the 64 cells c' of the block with counts n_c'; S(r) and the mask of
-// cells with n_c' > r are tabulated per item for r < kRankTable (a cell holds ~8 particles), so a
-// particle's position is one table read and a popcount. out[start + pos] = in[start + j].
+// Within-block transposition: the particles of item k arrive grouped by block and otherwise in
+// their previous order; they leave ordered by (rank within cell, cell), where a particle's rank
+// within its cell counts the earlier particles of the same cell (a stable order). Output position
+// pos(r, c) = S(r) + #{c' < c : n_c' > r} with S(r) = sum_c' min(n_c', r) over the 64 cells c' of
+// the block with counts n_c'. One wavefront per item, lane c owning cell c: it counts the cells,
+// tabulates S(r) and the mask of cells with n_c' > r for r < kRankTable (a cell holds ~8
+// particles; S(r + 1) = S(r) + popcount(mask(r))), then ranks the item in index order (ballots find
+// the lanes of equal cell) so a position is one table read and a popcount.
+// out[start + pos] = in[start + j].
 constexpr int kRankTable = 128;
+constexpr int kBinWaves = kThreads / 64;
+__device__ __forceinline__ uint64_t cell_peers(uint32_t c, uint64_t valid) {
+  uint64_t peers = valid;
+#pragma unroll
+  for (int b = 0; b < 6; b++) {
+    const bool bit = (c >> b) & 1u;
+    const uint64_t m = __ballot(bit);
+    peers &= bit ? m : ~m;
+  }
+  return peers;
+}
 __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uint32_t* __restrict__ keys,
                                                             const uint32_t* __restrict__ in,
                                                             uint32_t* __restrict__ out) {
-  __shared__ uint32_t ncell[64];
-  __shared__ uint32_t cstart[64];
-  __shared__ uint32_t srank[kRankTable];
-  __shared__ uint64_t mrank[kRankTable];
-  __shared__ uint32_t s_rmax;
-  const int tid = threadIdx.x;
+  __shared__ uint32_t wrun[kBinWaves][64];  // per wave: cell counts, then the next rank per cell
+  __shared__ uint32_t wcnt[kBinWaves][64];
+  __shared__ uint32_t srank[kBinWaves][kRankTable];
+  __shared__ uint64_t mrank[kBinWaves][kRankTable];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   const uint32_t items = pool.ctr[kCtrItems];
-  for (uint32_t k = blockIdx.x; k < items; k += gridDim.x) {
-    const uint32_t start = pool.wl_start[k], cnt = pool.wl_start[k + 1] - start;
-    __syncthreads();
-    if (tid < 64) ncell[tid] = 0;
-    __syncthreads();
-    for (uint32_t j = tid; j < cnt; j += kThreads) atomicAdd(&ncell[keys[start + j] & 63u], 1u);
-    __syncthreads();
-    if (tid < 64) {  // exclusive scan and maximum of the cell counts (one wave)
-      const uint32_t c = ncell[tid];
-      uint32_t inc = c, mx = c;
+  for (uint32_t k = blockIdx.x * kBinWaves + w; k < items; k += gridDim.x * kBinWaves) {
+    const uint32_t start = __builtin_amdgcn_readfirstlane(pool.wl_start[k]);
+    const uint32_t cnt = __builtin_amdgcn_readfirstlane(pool.wl_start[k + 1]) - start;
+    wrun[w][lane] = 0;
+    for (uint32_t j0 = 0; j0 < cnt; j0 += 64) {
+      const uint32_t j = j0 + lane;
+      const uint64_t valid = __ballot(j < cnt);
+      const uint32_t c = j < cnt ? keys[start + j] & 63u : 0u;
+      const uint64_t peers = cell_peers(c, valid);
+      if (j < cnt && (peers & below) == 0) wrun[w][c] += (uint32_t)__popcll(peers);
+    }
+    const uint32_t nc = wrun[w][lane];  // particles in cell `lane`
+    wcnt[w][lane] = nc;
+    wrun[w][lane] = 0;
+    uint32_t rmax = nc;
 #pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t o = __shfl_up(inc, d, 64);
-        if (tid >= d) inc += o;
-        mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
+    for (int d = 1; d < 64; d <<= 1) rmax = max(rmax, (uint32_t)__shfl_xor((int)rmax, d, 64));
+    const uint32_t rtab = min(rmax, (uint32_t)kRankTable);
+    uint32_t sr = 0;
+    for (uint32_t r = 0; r < rtab; r++) {
+      const uint64_t m = __ballot(nc > r);
+      if (lane == 0) {
+        srank[w][r] = sr;
+        mrank[w][r] = m;
       }
-      cstart[tid] = inc - c;
-      if (tid == 0) s_rmax = mx;
+      sr += (uint32_t)__popcll(m);
     }
-    __syncthreads();
-    const uint32_t rtab = min(s_rmax, (uint32_t)kRankTable);
-    for (uint32_t r = tid; r < rtab; r += kThreads) {
-      uint32_t sr = 0;
-      uint64_t m = 0;
-      for (uint32_t q = 0; q < 64; q++) {
-        const uint32_t nq = ncell[q];
-        sr += min(nq, r);
-        m |= (uint64_t)(nq > r ? 1u : 0u) << q;
-      }
-      srank[r] = sr;
-      mrank[r] = m;
-    }
-    __syncthreads();
-    for (uint32_t j = tid; j < cnt; j += kThreads) {
-      const uint32_t c = keys[start + j] & 63u;
-      const uint32_t r = j - cstart[c];
-      uint32_t pos;
-      if (r < (uint32_t)kRankTable) {
-        pos = srank[r] + (uint32_t)__builtin_popcountll(mrank[r] & ((1ull << c) - 1ull));
-      } else {
-        pos = 0;
-        for (uint32_t q = 0; q < 64; q++) {
-          const uint32_t nq = ncell[q];
-          pos += min(nq, r) + ((q < c && nq > r) ? 1u : 0u);
+    for (uint32_t j0 = 0; j0 < cnt; j0 += 64) {
+      const uint32_t j = j0 + lane;
+      const uint64_t valid = __ballot(j < cnt);
+      const uint32_t c = j < cnt ? keys[start + j] & 63u : 0u;
+      const uint64_t peers = cell_peers(c, valid);
+      if (j < cnt) {
+        const uint32_t r = wrun[w][c] + (uint32_t)__popcll(peers & below);
+        uint32_t pos;
+        if (r < (uint32_t)kRankTable) {
+          pos = srank[w][r] + (uint32_t)__builtin_popcountll(mrank[w][r] & ((1ull << c) - 1ull));
+        } else {
+          pos = 0;
+          for (uint32_t q = 0; q < 64; q++) {
+            const uint32_t nq = wcnt[w][q];
+            pos += min(nq, r) + ((q < c && nq > r) ? 1u : 0u);
+          }
         }
+        out[start + pos] = in[start + j];
       }
-      out[start + pos] = in[start + j];
+      // every lane has read wrun[w][c]; the first lane of each cell advances it
+      if (j < cnt && (peers & below) == 0) wrun[w][c] += (uint32_t)__popcll(peers);
     }
   }
 }

@@ -368,7 +368,8 @@ mpm_status do_bin(mpm_pool* p, int mode) {
   }
   HIP_TRY(launch_keys(pv, p->P[p->cur], p->count, p->keys[0], p->vals[0], p->stream));
   uint32_t *rk = nullptr, *rv = nullptr;
-  HIP_TRY(radix_sort_pairs(p->keys[0], p->vals[0], p->keys[1], p->vals[1], p->count, (int)p->sort_bits,
+  // by block (key bits 6 and up); the cell order inside a block is made by the transposition
+  HIP_TRY(radix_sort_pairs(p->keys[0], p->vals[0], p->keys[1], p->vals[1], p->count, 6, (int)p->sort_bits,
                            p->sort_scratch, p->stream, &rk, &rv));
   HIP_TRY(launch_fill_u32(p->map, 0xffffffffu, p->map_per_scene * p->max_scenes, p->stream));
   HIP_TRY(launch_bin_flags(rk, p->count, p->flags, p->stream));
@@ -397,7 +398,7 @@ mpm_status repack(mpm_pool* p) {
   if (p->count) {
     HIP_TRY(launch_live_keys(pv, p->P[p->cur], p->count, p->keys[0], p->vals[0], p->stream));
     uint32_t *rk = nullptr, *rv = nullptr;
-    HIP_TRY(radix_sort_pairs(p->keys[0], p->vals[0], p->keys[1], p->vals[1], p->count, 1, p->sort_scratch, p->stream,
+    HIP_TRY(radix_sort_pairs(p->keys[0], p->vals[0], p->keys[1], p->vals[1], p->count, 0, 1, p->sort_scratch, p->stream,
                              &rk, &rv));
     uint32_t live = 0;
     for (auto& sc : p->scenes)
@@ -1468,7 +1469,7 @@ mpm_status mpm_counting_sort(mpm_ctx* c, const uint32_t* keys, uint64_t n, uint3
   if (e == hipSuccess) e = scan_exclusive(cnt, nb1, scr, c->stream);
   uint32_t *rk = k0, *rv = v0;
   if (e == hipSuccess) e = iota_u32(v0, n, c->stream);
-  if (e == hipSuccess) e = radix_sort_pairs(k0, v0, k1, v1, n, bits, scr, c->stream, &rk, &rv);
+  if (e == hipSuccess) e = radix_sort_pairs(k0, v0, k1, v1, n, 0, bits, scr, c->stream, &rk, &rv);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e == hipSuccess && n) e = hipMemcpy(order, rv, n * 4, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(start, cnt, nb1 * 4, hipMemcpyDeviceToHost);
@@ -1521,7 +1522,7 @@ mpm_status mpm_spatial_grid(mpm_ctx* c, const float* points, uint32_t n, const d
   rk = k0;
   rv = v0;
   if (e == hipSuccess) e = iota_u32(v0, n, c->stream);
-  if (e == hipSuccess) e = radix_sort_pairs(k0, v0, k1, v1, n, bits, scr, c->stream, &rk, &rv);  // order, stable
+  if (e == hipSuccess) e = radix_sort_pairs(k0, v0, k1, v1, n, 0, bits, scr, c->stream, &rk, &rv);  // order, stable
   if (e == hipSuccess) e = grid_neighbor_count(dq, m, g, cnt, qoff, c->stream);
   if (e == hipSuccess && m) e = scan_exclusive(qoff, (size_t)m + 1, scr, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

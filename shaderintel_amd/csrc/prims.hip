@@ -4,11 +4,12 @@
 // partials — the scheme of src/prims/scan.js:252-320, written for 64-wide wavefronts (one wave
 // scan with cross-lane shuffles, then one LDS step across the 4 waves of a workgroup).
 //
-// Radix sort: LSD, 8-bit digits. Per pass: a per-tile digit histogram stored digit-major
-// (hist[d * tiles + t]), one exclusive scan over it (so tile t's slot for digit d follows every
-// earlier tile's), then a scatter in which each element's rank among equal digits is found with
-// eight wave ballots. Elements are visited in index order, so every pass is stable and the final
-// order is a deterministic function of the keys (no atomics decide it).
+// Radix sort: LSD over a bit range, in passes of up to 9-bit digits. Per pass: a per-tile digit
+// histogram stored digit-major (hist[d * tiles + t]), one exclusive scan over it (so tile t's slot
+// for digit d follows every earlier tile's), then a scatter that ranks the tile in LDS (an
+// element's rank among equal digits comes from wave ballots, in index order), stages it in sorted
+// order and writes runs of equal digits. Every pass is stable and the final order is a
+// deterministic function of the keys (no atomic decides a position).
 #include "prims.hpp"
 
 namespace mpm {
@@ -123,66 +124,175 @@ hipError_t scan_level(uint32_t* data, size_t n, uint32_t* scratch, hipStream_t s
 }
 
 // ---- radix sort ----
-__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys, size_t n, int shift,
-                                                             uint32_t tiles, uint32_t* __restrict__ hist) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
-  __syncthreads();
-  const size_t base = (size_t)blockIdx.x * kSortTile;
-#pragma unroll 4
-  for (int j = 0; j < kSortItems; j++) {
-    size_t i = base + (size_t)j * kSortThreads + threadIdx.x;
-    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+// Digits of up to kRadixMaxBits bits. A tile is kSortTile consecutive elements; wave w of a tile's
+// workgroup owns the contiguous chunk [w * kSortChunk, (w + 1) * kSortChunk) of it, read 64 elements
+// at a time in index order.
+constexpr int kRadixMaxBits = 9;
+constexpr int kRadixMaxBuckets = 1 << kRadixMaxBits;
+constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kSortChunk = kSortTile / kSortWaves;  // 1024
+
+// Lanes of the wave whose digit equals this lane's (among the lanes in `valid`).
+__device__ __forceinline__ uint64_t digit_peers(uint32_t d, int bits, uint64_t valid) {
+  uint64_t peers = valid;
+  for (int b = 0; b < bits; b++) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t m = __ballot(bit);
+    peers &= bit ? m : ~m;
   }
-  __syncthreads();
-  hist[(size_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+  return peers;
 }
 
+// Per-tile digit counts, digit-major (hist[d * tiles + t]). Each thread counts 16 consecutive keys
+// (four 16-byte loads), adding a run of equal digits to the LDS histogram at once (the keys arrive
+// nearly sorted).
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys, size_t n, int shift,
+                                                             int bits, uint32_t tiles, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[kRadixMaxBuckets];
+  const int nb = 1 << bits;
+  const uint32_t mask = (uint32_t)nb - 1u;
+  for (int d = threadIdx.x; d < nb; d += kSortThreads) h[d] = 0;
+  __syncthreads();
+  const size_t first = (size_t)blockIdx.x * kSortTile + (size_t)threadIdx.x * kSortItems;
+  uint32_t k[kSortItems];
+  int valid = 0;
+  if (first + kSortItems <= n) {  // keys is 16-byte aligned and first is a multiple of 16
+    const uint4* q = reinterpret_cast<const uint4*>(keys + first);
+#pragma unroll
+    for (int j = 0; j < kSortItems / 4; j++) {
+      const uint4 v = q[j];
+      k[4 * j] = v.x; k[4 * j + 1] = v.y; k[4 * j + 2] = v.z; k[4 * j + 3] = v.w;
+    }
+    valid = kSortItems;
+  } else {
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) k[j] = first + j < n ? keys[first + j] : 0u;
+    valid = first < n ? (int)(n - first) : 0;
+  }
+  uint32_t run_d = (k[0] >> shift) & mask, run_n = 0;
+#pragma unroll
+  for (int j = 0; j < kSortItems; j++) {
+    if (j < valid) {
+      const uint32_t d = (k[j] >> shift) & mask;
+      if (d != run_d) {
+        atomicAdd(&h[run_d], run_n);
+        run_d = d;
+        run_n = 0;
+      }
+      run_n++;
+    }
+  }
+  if (run_n) atomicAdd(&h[run_d], run_n);
+  __syncthreads();
+  for (int d = threadIdx.x; d < nb; d += kSortThreads) hist[(size_t)d * tiles + blockIdx.x] = h[d];
+}
+
+// Stable scatter of one tile. hist holds the exclusive scan of the digit-major counts, so
+// hist[d * tiles + t] is where tile t's first element of digit d goes. Wave w ranks its chunk in
+// index order (an element's rank among equal digits of the chunk: a running LDS count per digit plus
+// its rank among the equal-digit lanes of its 64-element step, found with ballots); after the
+// per-digit offsets of the tile and of the earlier waves are known, the tile is staged in LDS in
+// sorted order and written out in runs of equal digits (coalesced).
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                                 const uint32_t* __restrict__ vin,
                                                                 uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                                size_t n, int shift, uint32_t tiles,
+                                                                size_t n, int shift, int bits, uint32_t tiles,
                                                                 const uint32_t* __restrict__ hist) {
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t wcnt[kSortThreads / 64][256];
+  __shared__ uint32_t skey[kSortTile];
+  __shared__ uint32_t sval[kSortTile];
+  __shared__ uint32_t run[kSortWaves][kRadixMaxBuckets];  // chunk counts, then each wave's first local slot
+  __shared__ uint32_t gofs[kRadixMaxBuckets];             // output index of local slot 0 of digit d
+  __shared__ uint32_t wsum[kSortWaves];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  run[tid] = hist[(size_t)tid * tiles + blockIdx.x];
-#pragma unroll
-  for (int k = 0; k < kSortThreads / 64; k++) wcnt[k][tid] = 0;
-  __syncthreads();
-  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int nb = 1 << bits;
+  const uint32_t mask = (uint32_t)nb - 1u;
   const size_t base = (size_t)blockIdx.x * kSortTile;
+  const uint32_t count = (uint32_t)((n - base) < (size_t)kSortTile ? (n - base) : (size_t)kSortTile);
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int d = tid; d < nb; d += kSortThreads)
+#pragma unroll
+    for (int k = 0; k < kSortWaves; k++) run[k][d] = 0;
+  __syncthreads();
+  // 1. load the chunk (64 consecutive elements per step) and rank it within the chunk
+  uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
+#pragma unroll
   for (int j = 0; j < kSortItems; j++) {
-    const size_t i = base + (size_t)j * kSortThreads + tid;
-    const bool valid = i < n;
-    const uint32_t key = valid ? kin[i] : 0u;
-    const uint32_t val = valid ? vin[i] : 0u;
-    const uint32_t d = (key >> shift) & 255u;
-    uint64_t peers = __ballot(valid);
+    const uint32_t e = (uint32_t)(w * kSortChunk + j * 64 + lane);
+    const bool valid = e < count;
+    key[j] = valid ? kin[base + e] : 0u;
+    val[j] = valid ? vin[base + e] : 0u;
+  }
 #pragma unroll
-    for (int b = 0; b < 8; b++) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t m = __ballot(bit);
-      peers &= bit ? m : ~m;
-    }
-    const uint32_t rank = (uint32_t)__popcll(peers & below);
-    if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = run[d] + rank;
-      for (int k = 0; k < w; k++) pos += wcnt[k][d];
-      kout[pos] = key;
-      vout[pos] = val;
-    }
-    __syncthreads();
-    uint32_t add = 0;
+  for (int j = 0; j < kSortItems; j++) {
+    const uint32_t e = (uint32_t)(w * kSortChunk + j * 64 + lane);
+    const uint64_t valid = __ballot(e < count);
+    rank[j] = 0;
+    if (!valid) continue;
+    const uint32_t d = (key[j] >> shift) & mask;
+    const uint64_t peers = digit_peers(d, bits, valid);
+    if (e < count) rank[j] = run[w][d] + (uint32_t)__popcll(peers & below);
+    // every lane has read run[w][d]; the first lane of each digit advances it
+    if (e < count && (peers & below) == 0) run[w][d] += (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  // 2. exclusive scan over digits of the tile counts; wave k's elements of digit d start after the
+  // earlier waves' ones
+  {
+    constexpr int kPer = kRadixMaxBuckets / kSortThreads;  // digits per thread (2)
+    uint32_t c[kPer], s = 0;
 #pragma unroll
-    for (int k = 0; k < kSortThreads / 64; k++) {
-      add += wcnt[k][tid];
-      wcnt[k][tid] = 0;
+    for (int q = 0; q < kPer; q++) {
+      const int d = tid * kPer + q;
+      c[q] = 0;
+      if (d < nb)
+#pragma unroll
+        for (int k = 0; k < kSortWaves; k++) c[q] += run[k][d];
+      s += c[q];
     }
-    run[tid] += add;
+    uint32_t inc = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t t = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += t;
+    }
+    if (lane == 63) wsum[w] = inc;
     __syncthreads();
+    uint32_t off = inc - s;
+    for (int k = 0; k < w; k++) off += wsum[k];
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      const int d = tid * kPer + q;
+      if (d < nb) {
+        gofs[d] = hist[(size_t)d * tiles + blockIdx.x] - off;
+        uint32_t o = off;
+#pragma unroll
+        for (int k = 0; k < kSortWaves; k++) {
+          const uint32_t ck = run[k][d];
+          run[k][d] = o;
+          o += ck;
+        }
+      }
+      off += c[q];
+    }
+  }
+  __syncthreads();
+  // 3. stage the tile in sorted order
+#pragma unroll
+  for (int j = 0; j < kSortItems; j++) {
+    const uint32_t e = (uint32_t)(w * kSortChunk + j * 64 + lane);
+    if (e < count) {
+      const uint32_t slot = run[w][(key[j] >> shift) & mask] + rank[j];
+      skey[slot] = key[j];
+      sval[slot] = val[j];
+    }
+  }
+  __syncthreads();
+  // 4. write out: local slot s of digit d goes to gofs[d] + s
+  for (uint32_t s = (uint32_t)tid; s < count; s += kSortThreads) {
+    const uint32_t k = skey[s];
+    const uint32_t pos = gofs[(k >> shift) & mask] + s;
+    kout[pos] = k;
+    vout[pos] = sval[s];
   }
 }
 
@@ -210,7 +320,7 @@ size_t scan_scratch_words(size_t n) {
 
 size_t sort_scratch_words(size_t n) {
   size_t tiles = (n + kSortTile - 1) / kSortTile;
-  size_t h = 256 * tiles;
+  size_t h = (size_t)kRadixMaxBuckets * tiles;
   return ((h + 3) & ~(size_t)3) + scan_scratch_words(h);
 }
 
@@ -219,22 +329,28 @@ hipError_t scan_exclusive(uint32_t* data, size_t n, uint32_t* scratch, hipStream
   return scan_level(data, n, scratch, stream);
 }
 
-hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, size_t n, int bits,
-                            uint32_t* scratch, hipStream_t stream, uint32_t** result_keys, uint32_t** result_vals) {
+hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, size_t n, int lo_bit,
+                            int hi_bit, uint32_t* scratch, hipStream_t stream, uint32_t** result_keys,
+                            uint32_t** result_vals) {
   *result_keys = keys;
   *result_vals = vals;
-  if (n == 0) return hipSuccess;
+  if (n == 0 || hi_bit <= lo_bit) return hipSuccess;
+  if (lo_bit < 0 || hi_bit > 32) return hipErrorInvalidValue;
+  const int span = hi_bit - lo_bit;
+  const int passes = (span + kRadixMaxBits - 1) / kRadixMaxBits;
+  const int bits = (span + passes - 1) / passes;
   const uint32_t tiles = (uint32_t)((n + kSortTile - 1) / kSortTile);
   uint32_t* hist = scratch;
-  uint32_t* scan_tmp = scratch + (((size_t)256 * tiles + 3) & ~(size_t)3);
+  uint32_t* scan_tmp = scratch + (((size_t)kRadixMaxBuckets * tiles + 3) & ~(size_t)3);
   uint32_t *ka = keys, *va = vals, *kb = keys_alt, *vb = vals_alt;
-  for (int shift = 0; shift < bits; shift += 8) {
-    hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kSortThreads), 0, stream, (const uint32_t*)ka, n, shift, tiles,
+  for (int shift = lo_bit; shift < hi_bit; shift += bits) {
+    const int b = hi_bit - shift < bits ? hi_bit - shift : bits;
+    hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kSortThreads), 0, stream, (const uint32_t*)ka, n, shift, b, tiles,
                        hist);
-    hipError_t e = scan_exclusive(hist, (size_t)256 * tiles, scan_tmp, stream);
+    hipError_t e = scan_exclusive(hist, ((size_t)1 << b) * tiles, scan_tmp, stream);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kSortThreads), 0, stream, (const uint32_t*)ka,
-                       (const uint32_t*)va, kb, vb, n, shift, tiles, (const uint32_t*)hist);
+                       (const uint32_t*)va, kb, vb, n, shift, b, tiles, (const uint32_t*)hist);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     uint32_t* t;
