@@ -148,8 +148,11 @@ mpm_status mpm_substep_end(mpm_sim* sim, float vmax);
 mpm_status mpm_write_grid(mpm_sim* sim, const int32_t* words);
 /* ctx.idle() — src/core/context.js:198-204. */
 mpm_status mpm_sync(mpm_sim* sim);
-/* ctx.region timings (src/core/profiler.js): milliseconds of the last mpm_step per phase
- * {sort, p2g, grid, g2p}, measured with HIP events when profiling is enabled. */
+/* ctx.region timings (src/core/profiler.js): milliseconds per phase {sort, p2g, grid, g2p} summed
+ * over the substeps stepped since profiling was enabled or the times were last read, measured with
+ * HIP events. enable = 1: events between the phases of every substep (all four phases); enable = 2:
+ * only G2P, timed by its own dispatch (start and end recorded by the launch, no extra packets on
+ * the stream, so the substep runs as it does unprofiled); 0: off. */
 mpm_status mpm_set_profiling(mpm_sim* sim, int32_t enable);
 mpm_status mpm_phase_times(mpm_sim* sim, float ms[4]);
 

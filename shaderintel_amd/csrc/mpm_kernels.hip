@@ -6,6 +6,8 @@
 // wrapping integer additions, so its sums do not depend on thread order (mpm.js:64-66, 340-345).
 #include <climits>
 
+#include <hip/hip_ext.h>
+
 #include "mpm_kernels.hpp"
 #include "mpm_math.hpp"
 
@@ -367,10 +369,6 @@ __global__ void k_begin_bin(uint32_t* ctr) {
   ctr[kCtrItems] = 0u;
 }
 
-__global__ void k_mark_v(uint32_t* ctr, int use_s) {
-  ctr[kCtrVEpoch] = ctr[kCtrEpoch];
-  ctr[kCtrVBinEpoch] = use_s ? ctr[kCtrBinEpoch] : 0xffffffffu;
-}
 
 // Sort key: scene in the top bits, then the Morton code of the stencil base cell (a refinement of
 // the reference's 4^3-block key, mpm.js:567-568); vals = current index, so the radix sort's
@@ -411,28 +409,91 @@ __global__ void k_permute(const float4* __restrict__ src, float4* __restrict__ d
 }
 
 // ---- binning ----
-__global__ void k_bin_flags(const uint32_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ flags) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > n) return;
-  flags[i] = (i < n && (i == 0 || (keys[i] >> 6) != (keys[i - 1] >> 6))) ? 1u : 0u;
+// Work items from the block-sorted keys: a head is a particle whose block (key >> 6) differs from
+// its predecessor's. Tiles of kItemTile keys, 16 consecutive keys per thread: k_bin_count stores
+// the heads of each tile, and after an exclusive scan of those counts k_bin_emit numbers the heads
+// in order (item k starts at the k-th head) and writes wl_start, wl_block and the map entry.
+constexpr int kItemPer = 16;
+constexpr uint32_t kItemTile = kThreads * kItemPer;
+
+__device__ __forceinline__ uint32_t head_mask(const uint32_t* __restrict__ keys, uint32_t n, uint32_t first,
+                                              uint32_t k[kItemPer]) {
+  uint32_t prev = first > 0 && first <= n ? keys[first - 1] >> 6 : 0xffffffffu;
+  uint32_t m = 0;
+  if (first + kItemPer <= n) {  // keys is 16-byte aligned and first a multiple of 16
+    const uint4* q = reinterpret_cast<const uint4*>(keys + first);
+#pragma unroll
+    for (int j = 0; j < kItemPer / 4; j++) {
+      const uint4 v = q[j];
+      k[4 * j] = v.x; k[4 * j + 1] = v.y; k[4 * j + 2] = v.z; k[4 * j + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kItemPer; j++) k[j] = first + j < n ? keys[first + j] : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < kItemPer; j++) {
+    const uint32_t b = k[j] >> 6;
+    if (first + j < n && (first + j == 0 || b != prev)) m |= 1u << j;
+    prev = b;
+  }
+  return m;
 }
 
-__global__ void k_bin_emit(Pool pool, const uint32_t* __restrict__ keys, uint32_t n,
-                           const uint32_t* __restrict__ scanned) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t blk = keys[i] >> 6;
-  if (i == 0 || (keys[i - 1] >> 6) != blk) {
-    const uint32_t k = scanned[i];
-    const uint32_t scene = keys[i] >> pool.key_bits;
-    const uint32_t mblock = blk & ((1u << (pool.key_bits - 6)) - 1u);
-    pool.wl_start[k] = i;
-    pool.wl_block[k] = blk;
-    pool.map[pool.scenes[scene].map_base + mblock] = k;
+__device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t wsum[kThreads / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += o;
   }
-  if (i == n - 1) {
-    pool.wl_start[scanned[n]] = n;
-    pool.ctr[kCtrItems] = scanned[n];
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (int q = 0; q < kThreads / 64; q++) {
+    const uint32_t x = wsum[q];
+    if (q < w) off += x;
+    tot += x;
+  }
+  *total = tot;
+  return off + inc - v;
+}
+
+__global__ __launch_bounds__(kThreads) void k_bin_count(const uint32_t* __restrict__ keys, uint32_t n,
+                                                        uint32_t* __restrict__ tile_heads) {
+  uint32_t k[kItemPer];
+  const uint32_t m = head_mask(keys, n, blockIdx.x * kItemTile + threadIdx.x * kItemPer, k);
+  uint32_t tot;
+  block_exclusive_sum((uint32_t)__builtin_popcount(m), &tot);
+  if (threadIdx.x == 0) tile_heads[blockIdx.x] = tot;
+  if (blockIdx.x == 0 && threadIdx.x == 0) tile_heads[gridDim.x] = 0;  // becomes the total after the scan
+}
+
+// tile_first[t]: the exclusive scan of the tile head counts (tiles + 1 entries; the last is the total).
+__global__ __launch_bounds__(kThreads) void k_bin_emit(Pool pool, const uint32_t* __restrict__ keys, uint32_t n,
+                                                       const uint32_t* __restrict__ tile_first, uint32_t tiles) {
+  uint32_t k[kItemPer];
+  const uint32_t first = blockIdx.x * kItemTile + threadIdx.x * kItemPer;
+  uint32_t m = head_mask(keys, n, first, k);
+  uint32_t tot;
+  uint32_t item = tile_first[blockIdx.x] + block_exclusive_sum((uint32_t)__builtin_popcount(m), &tot);
+  const uint32_t mmask = (1u << (pool.key_bits - 6)) - 1u;
+  while (m) {
+    const int j = __builtin_ctz(m);
+    m &= m - 1;
+    const uint32_t blk = k[j] >> 6;
+    pool.wl_start[item] = first + j;
+    pool.wl_block[item] = blk;
+    pool.map[pool.scenes[k[j] >> pool.key_bits].map_base + (blk & mmask)] = item;
+    item++;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const uint32_t items = tile_first[tiles];
+    pool.wl_start[items] = n;
+    pool.ctr[kCtrItems] = items;
   }
 }
 
@@ -770,6 +831,10 @@ template <bool VELOCITY>
 __global__ __launch_bounds__(kThreads) void k_grid_pull(Pool pool, int use_s) {
   const uint32_t ns = use_s ? pool.ctr[kCtrS] : 0u, nd = pool.ctr[kCtrD];
   const uint32_t epoch = pool.ctr[kCtrEpoch];
+  if (VELOCITY && blockIdx.x == 0 && threadIdx.x == 0) {  // label V: the P2G and binning epochs it comes from
+    pool.ctr[kCtrVEpoch] = epoch;
+    pool.ctr[kCtrVBinEpoch] = use_s ? pool.ctr[kCtrBinEpoch] : 0xffffffffu;
+  }
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (uint32_t j = blockIdx.x * 4 + w; j < ns + nd; j += gridDim.x * 4) {
     const uint32_t b = __builtin_amdgcn_readfirstlane(j < ns ? pool.slist[j] : pool.dlist[j - ns]);
@@ -906,13 +971,15 @@ hipError_t launch_permute(const float4* src, float4* dst, uint32_t cap, uint32_t
   hipLaunchKernelGGL(k_permute, dim3(blocks_for(n, 256)), dim3(256), 0, s, src, dst, cap, n, perm);
   return hipGetLastError();
 }
-hipError_t launch_bin_flags(const uint32_t* keys, uint32_t n, uint32_t* flags, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_flags, dim3(blocks_for((size_t)n + 1, 256)), dim3(256), 0, s, keys, n, flags);
+uint32_t bin_tiles(uint32_t n) { return (uint32_t)(((size_t)n + kItemTile - 1) / kItemTile); }
+hipError_t launch_bin_count(const uint32_t* keys, uint32_t n, uint32_t* tile_heads, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_bin_count, dim3(bin_tiles(n)), dim3(kThreads), 0, s, keys, n, tile_heads);
   return hipGetLastError();
 }
-hipError_t launch_bin_emit(const Pool& pool, const uint32_t* keys, uint32_t n, const uint32_t* scanned, hipStream_t s) {
+hipError_t launch_bin_emit(const Pool& pool, const uint32_t* keys, uint32_t n, const uint32_t* tile_first, hipStream_t s) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_bin_emit, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, keys, n, scanned);
+  hipLaunchKernelGGL(k_bin_emit, dim3(bin_tiles(n)), dim3(kThreads), 0, s, pool, keys, n, tile_first, bin_tiles(n));
   return hipGetLastError();
 }
 hipError_t launch_bin_transpose(const Pool& pool, const uint32_t* keys, const uint32_t* in, uint32_t* out, hipStream_t s) {
@@ -948,9 +1015,6 @@ hipError_t launch_write_words(uint32_t* dst, const uint32_t* src, uint32_t n, hi
 hipError_t launch_grid_update(const Pool& pool, bool use_s, hipStream_t s) {
   hipLaunchKernelGGL(k_grid_pull<true>, dim3(persistent_grid(2 * pool.nblocks, 4, 8192)), dim3(kThreads), 0, s, pool,
                      use_s ? 1 : 0);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_mark_v, dim3(1), dim3(1), 0, s, pool.ctr, use_s ? 1 : 0);
   return hipGetLastError();
 }
 hipError_t launch_resolve_tiles(const Pool& pool, hipStream_t s) {
@@ -962,26 +1026,37 @@ hipError_t launch_clear_a(const Pool& pool, bool use_s, hipStream_t s) {
                      use_s ? 1 : 0);
   return hipGetLastError();
 }
+// G2P launches; with ev_start / ev_stop (either may be null) the dispatch itself records them
+// (hipExtLaunchKernelGGL), so timing a launch adds no packet to the stream.
+#define MPM_G2P_LAUNCH(KERNEL, GRID, S, EV0, EV1, ...)                                              \
+  do {                                                                                              \
+    if ((EV0) || (EV1))                                                                             \
+      hipExtLaunchKernelGGL(KERNEL, GRID, dim3(kThreads), 0, S, EV0, EV1, 0, __VA_ARGS__);         \
+    else                                                                                            \
+      hipLaunchKernelGGL(KERNEL, GRID, dim3(kThreads), 0, S, __VA_ARGS__);                          \
+  } while (0)
+
 hipError_t launch_g2p_perm(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, uint32_t n, float vmax,
-                           int waves, hipStream_t s) {
+                           int waves, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop) {
   if (!n) return hipSuccess;
   const dim3 grid(blocks_for(n, kThreads));
   switch (waves) {
-    case 4: hipLaunchKernelGGL(k_g2p_perm<4>, grid, dim3(kThreads), 0, s, pool, src, dst, perm, n, vmax); break;
-    case 5: hipLaunchKernelGGL(k_g2p_perm<5>, grid, dim3(kThreads), 0, s, pool, src, dst, perm, n, vmax); break;
-    case 6: hipLaunchKernelGGL(k_g2p_perm<6>, grid, dim3(kThreads), 0, s, pool, src, dst, perm, n, vmax); break;
-    default: hipLaunchKernelGGL(k_g2p_perm<3>, grid, dim3(kThreads), 0, s, pool, src, dst, perm, n, vmax); break;
+    case 4: MPM_G2P_LAUNCH(k_g2p_perm<4>, grid, s, ev_start, ev_stop, pool, src, dst, perm, n, vmax); break;
+    case 5: MPM_G2P_LAUNCH(k_g2p_perm<5>, grid, s, ev_start, ev_stop, pool, src, dst, perm, n, vmax); break;
+    case 6: MPM_G2P_LAUNCH(k_g2p_perm<6>, grid, s, ev_start, ev_stop, pool, src, dst, perm, n, vmax); break;
+    default: MPM_G2P_LAUNCH(k_g2p_perm<3>, grid, s, ev_start, ev_stop, pool, src, dst, perm, n, vmax); break;
   }
   return hipGetLastError();
 }
-hipError_t launch_g2p(const Pool& pool, float4* P, uint32_t n, float vmax, int waves, hipStream_t s) {
+hipError_t launch_g2p(const Pool& pool, float4* P, uint32_t n, float vmax, int waves, hipStream_t s, hipEvent_t ev_start,
+                      hipEvent_t ev_stop) {
   if (!n) return hipSuccess;
   const dim3 grid(blocks_for(n, kThreads));
   switch (waves) {
-    case 4: hipLaunchKernelGGL(k_g2p<4>, grid, dim3(kThreads), 0, s, pool, P, n, vmax); break;
-    case 5: hipLaunchKernelGGL(k_g2p<5>, grid, dim3(kThreads), 0, s, pool, P, n, vmax); break;
-    case 6: hipLaunchKernelGGL(k_g2p<6>, grid, dim3(kThreads), 0, s, pool, P, n, vmax); break;
-    default: hipLaunchKernelGGL(k_g2p<3>, grid, dim3(kThreads), 0, s, pool, P, n, vmax); break;
+    case 4: MPM_G2P_LAUNCH(k_g2p<4>, grid, s, ev_start, ev_stop, pool, P, n, vmax); break;
+    case 5: MPM_G2P_LAUNCH(k_g2p<5>, grid, s, ev_start, ev_stop, pool, P, n, vmax); break;
+    case 6: MPM_G2P_LAUNCH(k_g2p<6>, grid, s, ev_start, ev_stop, pool, P, n, vmax); break;
+    default: MPM_G2P_LAUNCH(k_g2p<3>, grid, s, ev_start, ev_stop, pool, P, n, vmax); break;
   }
   return hipGetLastError();
 }

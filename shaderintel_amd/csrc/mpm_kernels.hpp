@@ -149,10 +149,12 @@ hipError_t launch_keys(const Pool& pool, const float4* P, uint32_t n, uint32_t* 
 // partition); vals = index.
 hipError_t launch_live_keys(const Pool& pool, const float4* P, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s);
 hipError_t launch_permute(const float4* src, float4* dst, uint32_t cap, uint32_t n, const uint32_t* perm, hipStream_t s);
-// Binning from the sorted keys: head flags (n + 1 words), work items + map, within-block
-// transposition of the sorted indices into `out`, and S.
-hipError_t launch_bin_flags(const uint32_t* keys_sorted, uint32_t n, uint32_t* flags, hipStream_t s);
-hipError_t launch_bin_emit(const Pool& pool, const uint32_t* keys_sorted, uint32_t n, const uint32_t* scanned,
+// Binning from the block-sorted keys: per-tile head counts (bin_tiles(n) + 1 words, the last one 0),
+// then, after an exclusive scan of those words (the last becomes the item count), the work items +
+// map; the within-block transposition of the sorted indices into `out`; and S.
+uint32_t bin_tiles(uint32_t n);
+hipError_t launch_bin_count(const uint32_t* keys_sorted, uint32_t n, uint32_t* tile_heads, hipStream_t s);
+hipError_t launch_bin_emit(const Pool& pool, const uint32_t* keys_sorted, uint32_t n, const uint32_t* tile_first,
                            hipStream_t s);
 hipError_t launch_bin_transpose(const Pool& pool, const uint32_t* keys_sorted, const uint32_t* in, uint32_t* out,
                                 hipStream_t s);
@@ -171,10 +173,12 @@ hipError_t launch_grid_update(const Pool& pool, bool use_s, hipStream_t s);
 hipError_t launch_resolve_tiles(const Pool& pool, hipStream_t s);
 // Zeroes A over S (use_s) and D.
 hipError_t launch_clear_a(const Pool& pool, bool use_s, hipStream_t s);
-hipError_t launch_g2p(const Pool& pool, float4* P, uint32_t n, float vmax, int waves, hipStream_t s);
+// G2P; ev_start / ev_stop (optional) are recorded by the dispatch itself at the kernel's start / end.
+hipError_t launch_g2p(const Pool& pool, float4* P, uint32_t n, float vmax, int waves, hipStream_t s,
+                      hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
 // G2P that also applies a pending reorder: slot i of dst takes particle perm[i] of src.
 hipError_t launch_g2p_perm(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, uint32_t n, float vmax,
-                           int waves, hipStream_t s);
+                           int waves, hipStream_t s, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
 // Reference-layout export of one scene: mode 0 = A (i32 words), mode 1 = V (f32 bits, 0 outside the
 // node blocks the last grid update wrote).
 hipError_t launch_grid_export(const Pool& pool, const SceneGeo& geo, int mode, int4* out, hipStream_t s);

@@ -152,8 +152,9 @@ struct mpm_pool {
   bool mid_substep = false;      // between mpm_substep_begin and mpm_substep_end
   hipGraphExec_t period_graph[2] = {nullptr, nullptr};  // one sort period, by parity of cur
   uint64_t graph_key = 0;                                // (count, scenes, vmax) the graphs were captured for
-  // profiling: five events per profiled substep (sort, p2g, grid, g2p start; end)
-  bool profiling = false;
+  // profiling 1: five events per profiled substep (sort, p2g, grid, g2p start; end);
+  // profiling 2: two events per G2P launch, recorded by the dispatch itself (start, end)
+  int profiling = 0;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   float phase_ms[4] = {0, 0, 0, 0};
@@ -372,9 +373,13 @@ mpm_status do_bin(mpm_pool* p, int mode) {
   HIP_TRY(radix_sort_pairs(p->keys[0], p->vals[0], p->keys[1], p->vals[1], p->count, 6, (int)p->sort_bits,
                            p->sort_scratch, p->stream, &rk, &rv));
   HIP_TRY(launch_fill_u32(p->map, 0xffffffffu, p->map_per_scene * p->max_scenes, p->stream));
-  HIP_TRY(launch_bin_flags(rk, p->count, p->flags, p->stream));
-  HIP_TRY(scan_exclusive(p->flags, (size_t)p->count + 1, p->sort_scratch, p->stream));
-  HIP_TRY(launch_bin_emit(pv, rk, p->count, p->flags, p->stream));
+  {  // work items: heads per tile, scanned, then numbered (the sort's scratch is free again)
+    const uint32_t tiles = bin_tiles(p->count);
+    uint32_t* tile_first = p->sort_scratch;
+    HIP_TRY(launch_bin_count(rk, p->count, tile_first, p->stream));
+    HIP_TRY(scan_exclusive(tile_first, (size_t)tiles + 1, p->sort_scratch + ((tiles + 4) & ~3u), p->stream));
+    HIP_TRY(launch_bin_emit(pv, rk, p->count, tile_first, p->stream));
+  }
   HIP_TRY(launch_bin_transpose(pv, rk, rv, p->order, p->stream));
   if (mode == kBinReorder) {
     HIP_TRY(launch_permute(p->P[p->cur], p->P[p->cur ^ 1], p->cap, p->count, p->order, p->stream));
@@ -450,29 +455,44 @@ mpm_status do_grid(mpm_pool* p) {
   return MPM_OK;
 }
 
+// The next unused profiling event of the pool.
+mpm_status next_event(mpm_pool* p, hipEvent_t* out) {
+  if (p->ev_used == p->ev_pool.size()) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    p->ev_pool.push_back(e);
+  }
+  *out = p->ev_pool[p->ev_used++];
+  return MPM_OK;
+}
+
+mpm_status mark(mpm_pool* p) {
+  if (p->profiling != 1) return MPM_OK;
+  hipEvent_t e;
+  ST_TRY(next_event(p, &e));
+  HIP_TRY(hipEventRecord(e, p->stream));
+  return MPM_OK;
+}
+
 mpm_status do_g2p(mpm_pool* p, float vmax) {  // mpm.js:598-600
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (p->profiling == 2) {
+    ST_TRY(next_event(p, &e0));
+    ST_TRY(next_event(p, &e1));
+  }
   if (p->reorder_in_g2p) {  // the sort's reorder, applied by the record writes
-    HIP_TRY(launch_g2p_perm(view(p), p->P[p->cur], p->P[p->cur ^ 1], p->perm, p->count, vmax, p->g2p_waves, p->stream));
+    HIP_TRY(launch_g2p_perm(view(p), p->P[p->cur], p->P[p->cur ^ 1], p->perm, p->count, vmax, p->g2p_waves, p->stream,
+                            e0, e1));
     p->cur ^= 1;
     p->perm = nullptr;
     p->reorder_in_g2p = false;
     recompute_offsets(p);
     return MPM_OK;
   }
-  HIP_TRY(launch_g2p(view(p), p->P[p->cur], p->count, vmax, p->g2p_waves, p->stream));
+  HIP_TRY(launch_g2p(view(p), p->P[p->cur], p->count, vmax, p->g2p_waves, p->stream, e0, e1));
   return MPM_OK;
 }
 
-mpm_status mark(mpm_pool* p) {
-  if (!p->profiling) return MPM_OK;
-  if (p->ev_used == p->ev_pool.size()) {
-    hipEvent_t e;
-    HIP_TRY(hipEventCreate(&e));
-    p->ev_pool.push_back(e);
-  }
-  HIP_TRY(hipEventRecord(p->ev_pool[p->ev_used++], p->stream));
-  return MPM_OK;
-}
 
 // First part of a substep (MPM.step's loop body up to the grid update and the coupling call,
 // mpm.js:591-596): sort when due, P2G, grid update, then the after-grid-update hook if one is set.
@@ -1173,7 +1193,8 @@ mpm_status mpm_set_tuning(mpm_sim* s, int32_t knob, int32_t value) {
 mpm_status mpm_set_profiling(mpm_sim* s, int32_t enable) {
   if (!s) return fail(MPM_ERR_INVALID_ARGUMENT, "simulation is NULL");
   mpm_pool* p = s->pool;
-  p->profiling = enable != 0;
+  if (enable < 0 || enable > 2) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_profiling: mode %d not in 0..2", enable);
+  p->profiling = enable;
   p->ev_used = 0;
   for (float& m : p->phase_ms) m = 0.0f;
   return MPM_OK;
@@ -1183,14 +1204,24 @@ mpm_status mpm_phase_times(mpm_sim* s, float ms[4]) {
   if (!s || !ms) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_phase_times: NULL argument");
   mpm_pool* p = s->pool;
   ST_TRY(set_dev(p->ctx));
-  const size_t groups = p->ev_used / 5;
-  if (groups) HIP_TRY(hipEventSynchronize(p->ev_pool[groups * 5 - 1]));
-  for (size_t g = 0; g < groups; g++)
-    for (int ph = 0; ph < 4; ph++) {
+  if (p->profiling == 2) {  // (start, end) of each G2P launch
+    const size_t pairs = p->ev_used / 2;
+    if (pairs) HIP_TRY(hipEventSynchronize(p->ev_pool[pairs * 2 - 1]));
+    for (size_t g = 0; g < pairs; g++) {
       float t = 0;
-      HIP_TRY(hipEventElapsedTime(&t, p->ev_pool[g * 5 + ph], p->ev_pool[g * 5 + ph + 1]));
-      p->phase_ms[ph] += t;
+      HIP_TRY(hipEventElapsedTime(&t, p->ev_pool[2 * g], p->ev_pool[2 * g + 1]));
+      p->phase_ms[3] += t;
     }
+  } else {
+    const size_t groups = p->ev_used / 5;
+    if (groups) HIP_TRY(hipEventSynchronize(p->ev_pool[groups * 5 - 1]));
+    for (size_t g = 0; g < groups; g++)
+      for (int ph = 0; ph < 4; ph++) {
+        float t = 0;
+        HIP_TRY(hipEventElapsedTime(&t, p->ev_pool[g * 5 + ph], p->ev_pool[g * 5 + ph + 1]));
+        p->phase_ms[ph] += t;
+      }
+  }
   p->ev_used = 0;
   for (int ph = 0; ph < 4; ph++) ms[ph] = p->phase_ms[ph];
   return MPM_OK;
@@ -1343,9 +1374,10 @@ mpm_status mpm_pool_autotune(mpm_pool* p, uint32_t trial_substeps, int32_t* g2p_
   CHECK_POOL(p);
   if (trial_substeps == 0) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_autotune: trial_substeps = 0");
   const uint32_t trial = (trial_substeps + p->sort_every - 1) / p->sort_every * p->sort_every;
-  const bool graphs = p->use_graphs, prof = p->profiling;
+  const bool graphs = p->use_graphs;
+  const int prof = p->profiling;
   p->use_graphs = false;
-  p->profiling = false;
+  p->profiling = 0;
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));

@@ -581,8 +581,10 @@ class MPM:
         for k, v in knobs.items():
             _check(load_library().mpm_set_tuning(self._h, self.TUNE[k], int(v)), "MPM.set_tuning")
 
-    def set_profiling(self, on: bool) -> None:
-        _check(load_library().mpm_set_profiling(self._h, 1 if on else 0))
+    def set_profiling(self, on) -> None:
+        """True / 1: events between the phases of every substep; 2: G2P launches only, timed by
+        their own dispatch (no extra packets on the stream); False / 0: off."""
+        _check(load_library().mpm_set_profiling(self._h, int(on)))
 
     def phase_times(self) -> dict:
         ms = np.zeros(4, np.float32)
