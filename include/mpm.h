@@ -160,6 +160,8 @@ mpm_status mpm_phase_times(mpm_sim* sim, float ms[4]);
  * computes the same integer grid sums and the same per-particle float sequence). */
 #define MPM_TUNE_G2P_WAVES 2 /* occupancy target of the G2P kernel, 3..6 waves per SIMD (default 3) */
 #define MPM_TUNE_P2G_WAVES 4 /* occupancy target of the P2G kernel, 4..6 (default 4) */
+#define MPM_TUNE_G2P_ITEMS 5 /* 1: G2P walks P2G's work items with each item's velocity box in LDS
+                              * (default); 0: one thread per particle, gathering from the grid */
 mpm_status mpm_set_tuning(mpm_sim* sim, int32_t knob, int32_t value);
 
 /* ---- scene pools: many independent simulations advanced by the same launches ----

@@ -165,26 +165,34 @@ __device__ __forceinline__ float4 grid_node(int4 q, int ix, int iy, int iz, cons
 }
 
 // G2P for one particle (mpm.js:479-559): gather velocity and APIC C, advect, update F, plasticity,
-// stress. Reads record si of src, writes the new record to slot di of dst (the 8 streams) and
-// returns it in r. In place: src == dst and si == di; every stream a slot's new value depends on is
-// read before that slot is written.
-// gather(bx, by, bz, a, b, c) returns the grid velocity (and mass) of node (bx + a, by + b, bz + c).
+// stress. Split in three: g2p_load (the record's x, J and F22 and the stencil base), g2p_gather
+// (the 27-node sums; gather(bx, by, bz, a, b, c) returns the grid velocity of node
+// (bx + a, by + b, bz + c)) and g2p_finish. g2p_finish reads record si of src and writes the new
+// record to slot di of dst (the 8 streams). In place: src == dst and si == di; every stream a
+// slot's new value depends on is read before that slot is written.
+struct G2PIn {
+  float4 x4, s5;  // S0 (x, row) and S5 (t02, t12, J, F22) of the record
+  f3 xp, fx;
+  int bx, by, bz;
+};
+
+__device__ __forceinline__ void g2p_load(const float4* src, uint32_t si, uint32_t cap, G2PIn& in) {
+  in.x4 = src[si];
+  in.s5 = src[(size_t)5 * cap + si];
+  in.xp = mk(in.x4.x, in.x4.y, in.x4.z);
+  in.bx = (int)floorf(in.xp.x - 0.5f);
+  in.by = (int)floorf(in.xp.y - 0.5f);
+  in.bz = (int)floorf(in.xp.z - 0.5f);
+  in.fx = in.xp - mk((float)in.bx, (float)in.by, (float)in.bz);
+}
+
 template <class Gather>
-__device__ __forceinline__ void g2p_particle(const float* __restrict__ mats, const SceneGeo& geo, const float4* src,
-                                             uint32_t si, float4* dst, uint32_t di, uint32_t cap, float vmax,
-                                             float4 r[8], Gather&& gather) {
-  const float4 x4 = src[si];
-  const float4 s5 = src[(size_t)5 * cap + si];  // t02, t12, J, F22
-  const float* m0 = mats + 12 * __float_as_uint(x4.w);
-  const float bound = geo.bound;
-  const float vw0 = s5.z;
-  const f3 xp = mk(x4.x, x4.y, x4.z);
-  const int bx = (int)floorf(xp.x - 0.5f), by = (int)floorf(xp.y - 0.5f), bz = (int)floorf(xp.z - 0.5f);
-  const f3 fx = xp - mk((float)bx, (float)by, (float)bz);
+__device__ __forceinline__ void g2p_gather(const G2PIn& in, f3& v, m3& B, Gather&& gather) {
+  const f3 fx = in.fx;
   f3 w[3];
   bspline(fx, w);
-  f3 v = mk(0, 0, 0);
-  m3 B = zero3();
+  v = mk(0, 0, 0);
+  B = zero3();
 #pragma unroll
   for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -196,11 +204,21 @@ __device__ __forceinline__ void g2p_particle(const float* __restrict__ mats, con
         const float wy = b == 0 ? w[0].y : (b == 1 ? w[1].y : w[2].y);
         const float wz = c == 0 ? w[0].z : (c == 1 ? w[1].z : w[2].z);
         const float wt = (wx * wy) * wz;
-        const float4 g4 = gather(bx, by, bz, a, b, c);
+        const float4 g4 = gather(in.bx, in.by, in.bz, a, b, c);
         const f3 gv = mk(g4.x, g4.y, g4.z);
         v = mk(fmaf(wt, gv.x, v.x), fmaf(wt, gv.y, v.y), fmaf(wt, gv.z, v.z));  // v += wt * gv, contracted
         B = add_outer(B, wt * gv, dpos);
       }
+}
+
+__device__ __forceinline__ void g2p_finish(const float* __restrict__ mats, const SceneGeo& geo, const float4* src,
+                                           uint32_t si, float4* dst, uint32_t di, uint32_t cap, float vmax,
+                                           const G2PIn& in, f3 v, const m3& B) {
+  const float4 x4 = in.x4, s5 = in.s5;
+  const float* m0 = mats + 12 * __float_as_uint(x4.w);
+  const float bound = geo.bound;
+  const float vw0 = s5.z;
+  const f3 xp = in.xp;
   const m3 C = 4.0f * B;
   const float sp = len3(v);
   if (sp > vmax) v = v * (vmax / sp);
@@ -258,6 +276,7 @@ __device__ __forceinline__ void g2p_particle(const float* __restrict__ mats, con
       tau = mul(mul(U, diag(dd)), tr(U));
     }
   }
+  float4 r[8];
   r[0] = make_float4(x.x, x.y, x.z, x4.w);
   r[1] = make_float4(v.x, v.y, v.z, tau.c[1].x);
   r[2] = make_float4(C.c[0].x, C.c[0].y, C.c[0].z, tau.c[0].x);
@@ -270,15 +289,26 @@ __device__ __forceinline__ void g2p_particle(const float* __restrict__ mats, con
   for (int k = 0; k < 8; k++) dst[(size_t)k * cap + di] = r[k];
 }
 
+template <class Gather>
+__device__ __forceinline__ void g2p_particle(const float* __restrict__ mats, const SceneGeo& geo, const float4* src,
+                                             uint32_t si, float4* dst, uint32_t di, uint32_t cap, float vmax,
+                                             Gather&& gather) {
+  G2PIn in;
+  g2p_load(src, si, cap, in);
+  f3 v;
+  m3 B;
+  g2p_gather(in, v, B, gather);
+  g2p_finish(mats, geo, src, si, dst, di, cap, vmax, in, v, B);
+}
+
 template <int WAVES>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_g2p(
     Pool pool, float4* __restrict__ P, uint32_t n, float vmax) {
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
-  float4 r[8];
   const SceneGeo& geo = pool.scenes[__float_as_uint(P[i].w) >> 3];
   const float4* __restrict__ V = pool.V;
-  g2p_particle(pool.mats, geo, P, i, P, i, pool.cap, vmax, r,
+  g2p_particle(pool.mats, geo, P, i, P, i, pool.cap, vmax,
                [&](int bx, int by, int bz, int a, int b, int c) { return V[node_addr(bx + a, by + b, bz + c, geo)]; });
 }
 
@@ -291,11 +321,84 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
   const uint32_t si = perm[i];
-  float4 r[8];
   const SceneGeo& geo = pool.scenes[__float_as_uint(src[si].w) >> 3];
   const float4* __restrict__ V = pool.V;
-  g2p_particle(pool.mats, geo, src, si, dst, i, pool.cap, vmax, r,
+  g2p_particle(pool.mats, geo, src, si, dst, i, pool.cap, vmax,
                [&](int bx, int by, int bz, int a, int b, int c) { return V[node_addr(bx + a, by + b, bz + c, geo)]; });
+}
+
+// Work item k of the current binning for an item-persistent kernel: the kWorkSplits counters at
+// wctr split the items into contiguous ranges; a workgroup starts on range blockIdx % kWorkSplits
+// and moves on when its range is used up. Called by thread 0; 0xffffffff when every range is done.
+__device__ __forceinline__ uint32_t next_item(uint32_t* wctr, uint32_t items, uint32_t& range, uint32_t& spent) {
+  while (spent < (uint32_t)kWorkSplits) {
+    const uint32_t lo = (uint32_t)((uint64_t)items * range / kWorkSplits);
+    const uint32_t hi = (uint32_t)((uint64_t)items * (range + 1) / kWorkSplits);
+    const uint32_t c = atomicAdd(wctr + kWorkStride * range, 1u);
+    if (c < hi - lo) return lo + c;
+    spent++;
+    range = (range + 1) % kWorkSplits;
+  }
+  return 0xffffffffu;
+}
+
+// G2P over the work items of this substep's P2G. The item's tile box (tile_hdr: the nodes its
+// in-window particles' stencils cover, unchanged since P2G because no particle has moved) is copied
+// from V into LDS, and the 27-node gathers of the item's particles read it there; a wavefront with
+// a particle outside the box (it took P2G's fallback path) gathers from V for all its lanes (the
+// same values). perm: sorted position -> record (nullptr: the identity). sorted_out: record perm[j]
+// of src goes to slot j of dst (the sort's reorder applied by the writes); otherwise the update is
+// in place (src == dst, each record back to its own slot).
+template <int WAVES>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_g2p_items(
+    Pool pool, const float4* src, float4* dst, const uint32_t* __restrict__ perm, int sorted_out, float vmax) {
+  __shared__ float4 box[kTileNodes];
+  __shared__ uint32_t s_item;
+  const int tid = threadIdx.x;
+  const uint32_t items = pool.ctr[kCtrItems];
+  const uint32_t mshift = pool.key_bits - 6;
+  uint32_t* const wctr = pool.arena_ctr + kArenaRegions * kArenaStride + kWorkSplits * kWorkStride;
+  uint32_t range = blockIdx.x % kWorkSplits, spent = 0;
+  const float4* __restrict__ V = pool.V;
+  for (;;) {
+    __syncthreads();  // the previous item is finished with the box and with s_item
+    if (tid == 0) s_item = next_item(wctr, items, range, spent);
+    __syncthreads();
+    const uint32_t k = s_item;
+    if (k >= items) break;
+    const uint32_t start = pool.wl_start[k], cnt = pool.wl_start[k + 1] - start;
+    const SceneGeo geo = uniform_geo(pool, pool.wl_block[k] >> mshift);
+    int ox = 0, oy = 0, oz = 0, ex = 0, ey = 0, ez = 0;
+    if (pool.tile_off[k] != 0xffffffffu) {
+      const int4 h = pool.tile_hdr[k];
+      ox = h.x; oy = h.y; oz = h.z;
+      ex = h.w & 255; ey = (h.w >> 8) & 255; ez = (h.w >> 16) & 255;
+    }
+    const int nodes = ex * ey * ez;  // <= kTileNodes
+    for (int t = tid; t < nodes; t += kThreads) {
+      const int lx = t % ex, ly = (t / ex) % ey, lz = t / (ex * ey);
+      box[t] = V[node_addr(ox + lx, oy + ly, oz + lz, geo)];
+    }
+    __syncthreads();
+    for (uint32_t li = tid; li < cnt; li += kThreads) {
+      const uint32_t j = start + li;
+      const uint32_t si = perm ? perm[j] : j;
+      G2PIn in;
+      g2p_load(src, si, pool.cap, in);
+      const bool inbox = (unsigned)(in.bx - ox) <= (unsigned)(ex - 3) && (unsigned)(in.by - oy) <= (unsigned)(ey - 3) &&
+                         (unsigned)(in.bz - oz) <= (unsigned)(ez - 3) && nodes > 0;
+      f3 v;
+      m3 B;
+      if (__builtin_amdgcn_ballot_w64(!inbox) == 0) {
+        const int tb = ((in.bz - oz) * ey + (in.by - oy)) * ex + (in.bx - ox);
+        g2p_gather(in, v, B, [&](int, int, int, int a, int b, int c) { return box[tb + (c * ey + b) * ex + a]; });
+      } else {
+        g2p_gather(in, v, B,
+                   [&](int bx, int by, int bz, int a, int b, int c) { return V[node_addr(bx + a, by + b, bz + c, geo)]; });
+      }
+      g2p_finish(pool.mats, geo, src, si, dst, sorted_out ? j : si, pool.cap, vmax, in, v, B);
+    }
+  }
 }
 
 __device__ __forceinline__ float& comp4(float4& q, int c) { return c == 0 ? q.x : (c == 1 ? q.y : (c == 2 ? q.z : q.w)); }
@@ -356,7 +459,7 @@ __global__ void k_block_info(uint2* __restrict__ info, uint32_t base, uint32_t n
 __global__ void k_begin_p2g(uint32_t* ctr, uint32_t* arena_ctr) {
   const int t = threadIdx.x;  // kArenaRegions threads
   arena_ctr[t * kArenaStride] = 0u;
-  if (t < kWorkSplits) arena_ctr[kArenaRegions * kArenaStride + kWorkStride * t] = 0u;
+  if (t < 2 * kWorkSplits) arena_ctr[kArenaRegions * kArenaStride + kWorkStride * t] = 0u;  // P2G's, G2P's
   if (t == 0) {
     ctr[kCtrEpoch] += 1u;
     ctr[kCtrD] = 0u;
@@ -662,21 +765,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   uint32_t range = blockIdx.x % kWorkSplits, spent = 0;  // ranges found used up (tid 0 only)
   for (;;) {
     __syncthreads();  // the previous item is finished with the tile and with s_item
-    if (tid == 0) {
-      uint32_t next = 0xffffffffu;
-      while (spent < (uint32_t)kWorkSplits) {
-        const uint32_t lo = (uint32_t)((uint64_t)items * range / kWorkSplits);
-        const uint32_t hi = (uint32_t)((uint64_t)items * (range + 1) / kWorkSplits);
-        const uint32_t c = atomicAdd(wctr + kWorkStride * range, 1u);
-        if (c < hi - lo) {
-          next = lo + c;
-          break;
-        }
-        spent++;
-        range = (range + 1) % kWorkSplits;
-      }
-      s_item = next;
-    }
+    if (tid == 0) s_item = next_item(wctr, items, range, spent);
     __syncthreads();
     const uint32_t k = s_item;
     if (k >= items) break;
@@ -1045,6 +1134,19 @@ hipError_t launch_g2p_perm(const Pool& pool, const float4* src, float4* dst, con
     case 5: MPM_G2P_LAUNCH(k_g2p_perm<5>, grid, s, ev_start, ev_stop, pool, src, dst, perm, n, vmax); break;
     case 6: MPM_G2P_LAUNCH(k_g2p_perm<6>, grid, s, ev_start, ev_stop, pool, src, dst, perm, n, vmax); break;
     default: MPM_G2P_LAUNCH(k_g2p_perm<3>, grid, s, ev_start, ev_stop, pool, src, dst, perm, n, vmax); break;
+  }
+  return hipGetLastError();
+}
+hipError_t launch_g2p_items(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, bool sorted_out,
+                            float vmax, int waves, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop) {
+  if (sorted_out ? (!perm || src == dst) : src != dst) return hipErrorInvalidValue;
+  const dim3 grid(persistent_grid(pool.max_items, 1, 2048));
+  const int so = sorted_out ? 1 : 0;
+  switch (waves) {
+    case 4: MPM_G2P_LAUNCH(k_g2p_items<4>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax); break;
+    case 5: MPM_G2P_LAUNCH(k_g2p_items<5>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax); break;
+    case 6: MPM_G2P_LAUNCH(k_g2p_items<6>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax); break;
+    default: MPM_G2P_LAUNCH(k_g2p_items<3>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax); break;
   }
   return hipGetLastError();
 }

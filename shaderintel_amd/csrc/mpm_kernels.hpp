@@ -69,8 +69,9 @@ constexpr int kArenaStride = 16;  // u32 words between region counters
 // address takes every item's atomic (one shared counter cost 7% of P2G on MI355X).
 constexpr int kWorkSplits = 8;
 constexpr int kWorkStride = 32;
-constexpr int kArenaCtrWords = kArenaRegions * kArenaStride + kWorkSplits * kWorkStride;
-static_assert(kWorkSplits <= kArenaRegions, "k_begin_p2g clears the work counters with its region threads");
+// Two sets of work counters: P2G's, then G2P's (which walks the same items).
+constexpr int kArenaCtrWords = kArenaRegions * kArenaStride + 2 * kWorkSplits * kWorkStride;
+static_assert(2 * kWorkSplits <= kArenaRegions, "k_begin_p2g clears the work counters with its region threads");
 
 enum Counter : int {
   kCtrEpoch = 0,     // P2G epoch (stamps of D)
@@ -124,7 +125,7 @@ struct Pool {
   const SceneGeo* scenes;
   uint32_t* ctr;           // [kCounters]
   uint32_t* arena_ctr;     // [kArenaCtrWords]: nodes handed out per region in the current P2G, then the
-                           // kWorkSplits work counters
+                           // work counters of P2G and of G2P
 };
 
 // Launch helpers; all enqueue on `s` and none synchronizes. P is the current particle buffer.
@@ -176,6 +177,12 @@ hipError_t launch_clear_a(const Pool& pool, bool use_s, hipStream_t s);
 // G2P; ev_start / ev_stop (optional) are recorded by the dispatch itself at the kernel's start / end.
 hipError_t launch_g2p(const Pool& pool, float4* P, uint32_t n, float vmax, int waves, hipStream_t s,
                       hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
+// G2P over the work items of this substep's binned P2G, the velocities of each item's tile box
+// staged in LDS. perm: sorted position -> record (nullptr: identity). sorted_out: slot i of dst
+// takes record perm[i] of src (as launch_g2p_perm); otherwise in place (src == dst).
+hipError_t launch_g2p_items(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, bool sorted_out,
+                            float vmax, int waves, hipStream_t s, hipEvent_t ev_start = nullptr,
+                            hipEvent_t ev_stop = nullptr);
 // G2P that also applies a pending reorder: slot i of dst takes particle perm[i] of src.
 hipError_t launch_g2p_perm(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, uint32_t n, float vmax,
                            int waves, hipStream_t s, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);

@@ -146,6 +146,8 @@ struct mpm_pool {
   int grid_state = kGridClean;
   bool a_dirty_s = false, a_dirty_d = false;  // A holds data over S / over D
   int g2p_waves = 3, p2g_waves = 4;
+  bool g2p_items = true;     // G2P by work item (LDS velocity boxes) after a binned P2G
+  bool tiles_fresh = false;  // the tiles and work items are those of this substep's binned P2G
   bool use_graphs = false;
   mpm_grid_hook hook = nullptr;  // called after every grid update (not with graphs)
   void* hook_user = nullptr;
@@ -439,9 +441,11 @@ mpm_status do_p2g(mpm_pool* p, bool force_global) {
   if (force_global) {
     HIP_TRY(launch_p2g_global(pv, p->P[p->cur], p->count, p->stream));
     p->grid_state = kGridA;
+    p->tiles_fresh = false;
   } else {
     HIP_TRY(launch_p2g_binned(pv, p->P[p->cur], p->perm, p->p2g_waves, p->stream));
     p->grid_state = kGridTiles;
+    p->tiles_fresh = true;
   }
   p->a_dirty_d = true;
   return MPM_OK;
@@ -480,16 +484,25 @@ mpm_status do_g2p(mpm_pool* p, float vmax) {  // mpm.js:598-600
     ST_TRY(next_event(p, &e0));
     ST_TRY(next_event(p, &e1));
   }
+  const bool items = p->g2p_items && p->tiles_fresh;
+  p->tiles_fresh = false;  // one G2P per P2G: the G2P work counters are those begin_p2g cleared
   if (p->reorder_in_g2p) {  // the sort's reorder, applied by the record writes
-    HIP_TRY(launch_g2p_perm(view(p), p->P[p->cur], p->P[p->cur ^ 1], p->perm, p->count, vmax, p->g2p_waves, p->stream,
-                            e0, e1));
+    if (items)
+      HIP_TRY(launch_g2p_items(view(p), p->P[p->cur], p->P[p->cur ^ 1], p->perm, true, vmax, p->g2p_waves, p->stream,
+                               e0, e1));
+    else
+      HIP_TRY(launch_g2p_perm(view(p), p->P[p->cur], p->P[p->cur ^ 1], p->perm, p->count, vmax, p->g2p_waves, p->stream,
+                              e0, e1));
     p->cur ^= 1;
     p->perm = nullptr;
     p->reorder_in_g2p = false;
     recompute_offsets(p);
     return MPM_OK;
   }
-  HIP_TRY(launch_g2p(view(p), p->P[p->cur], p->count, vmax, p->g2p_waves, p->stream, e0, e1));
+  if (items)  // in place; perm (kBinOnly) maps the items' sorted positions to the records
+    HIP_TRY(launch_g2p_items(view(p), p->P[p->cur], p->P[p->cur], p->perm, false, vmax, p->g2p_waves, p->stream, e0, e1));
+  else
+    HIP_TRY(launch_g2p(view(p), p->P[p->cur], p->count, vmax, p->g2p_waves, p->stream, e0, e1));
   return MPM_OK;
 }
 
@@ -1441,6 +1454,11 @@ mpm_status mpm_pool_set_tuning(mpm_pool* p, int32_t knob, int32_t value) {
     case MPM_TUNE_G2P_WAVES:
       if (value < 3 || value > 6) break;
       p->g2p_waves = value;
+      drop_graphs(p);
+      return MPM_OK;
+    case MPM_TUNE_G2P_ITEMS:
+      if (value < 0 || value > 1) break;
+      p->g2p_items = value != 0;
       drop_graphs(p);
       return MPM_OK;
     case MPM_TUNE_P2G_WAVES:
