@@ -213,7 +213,7 @@ __device__ __forceinline__ void g2p_gather(const G2PIn& in, f3& v, m3& B, Gather
 
 __device__ __forceinline__ void g2p_finish(const float* __restrict__ mats, const SceneGeo& geo, const float4* src,
                                            uint32_t si, float4* dst, uint32_t di, uint32_t cap, float vmax,
-                                           const G2PIn& in, f3 v, const m3& B) {
+                                           const G2PIn& in, f3 v, const m3& B, float4 (&r)[8]) {
   const float4 x4 = in.x4, s5 = in.s5;
   const float* m0 = mats + 12 * __float_as_uint(x4.w);
   const float bound = geo.bound;
@@ -276,7 +276,6 @@ __device__ __forceinline__ void g2p_finish(const float* __restrict__ mats, const
       tau = mul(mul(U, diag(dd)), tr(U));
     }
   }
-  float4 r[8];
   r[0] = make_float4(x.x, x.y, x.z, x4.w);
   r[1] = make_float4(v.x, v.y, v.z, tau.c[1].x);
   r[2] = make_float4(C.c[0].x, C.c[0].y, C.c[0].z, tau.c[0].x);
@@ -298,7 +297,8 @@ __device__ __forceinline__ void g2p_particle(const float* __restrict__ mats, con
   f3 v;
   m3 B;
   g2p_gather(in, v, B, gather);
-  g2p_finish(mats, geo, src, si, dst, di, cap, vmax, in, v, B);
+  float4 r[8];
+  g2p_finish(mats, geo, src, si, dst, di, cap, vmax, in, v, B, r);
 }
 
 template <int WAVES>
@@ -396,7 +396,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
         g2p_gather(in, v, B,
                    [&](int bx, int by, int bz, int a, int b, int c) { return V[node_addr(bx + a, by + b, bz + c, geo)]; });
       }
-      g2p_finish(pool.mats, geo, src, si, dst, sorted_out ? j : si, pool.cap, vmax, in, v, B);
+      float4 r[8];
+      g2p_finish(pool.mats, geo, src, si, dst, sorted_out ? j : si, pool.cap, vmax, in, v, B, r);
     }
   }
 }
@@ -851,6 +852,145 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   }
 }
 
+// G2P of one substep fused with the P2G of the next (when no sort comes between them, the work
+// items stay valid): one workgroup per work item. The item's grid-velocity box (its tile box of this
+// substep's P2G, when it has at most kFuseBoxNodes nodes) is staged in LDS for the gathers; each
+// particle's new record is written out (to slot j of dst when sorted_out, the sort's reorder, else
+// in place) and, still in registers, scattered into an LDS tile covering the item's whole window
+// (bases within kDrift cells of the block, 12^3 nodes), or through P2G's fallback when its new base
+// left the window. The tile is stored over the bounding box of the new in-window stencils in arena
+// space, which becomes the item's tile for the next grid update; if the arena has no room, the box
+// is added to A with atomics instead (integer sums, the same result). perm: sorted position ->
+// record of src (nullptr: identity).
+constexpr int kFuseBoxNodes = 1024;
+template <int WAVES>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_g2p_p2g(
+    Pool pool, const float4* src, float4* dst, const uint32_t* __restrict__ perm, int sorted_out, float vmax) {
+  __shared__ float4 vbox[kFuseBoxNodes];
+  __shared__ int tile[4 * kTileNodes];
+  __shared__ int bbox[6];
+  __shared__ uint32_t s_item, s_off;
+  const int tid = threadIdx.x;
+  const uint32_t items = pool.ctr[kCtrItems];
+  const uint32_t epoch = pool.ctr[kCtrEpoch], bin_epoch = pool.ctr[kCtrBinEpoch];
+  const uint32_t mshift = pool.key_bits - 6;
+  // P2G's work counters (the launch_begin_p2g before this kernel cleared them; G2P's stay fresh for
+  // the next substep's G2P, which has no launch_begin_p2g of its own before it)
+  uint32_t* const wctr = pool.arena_ctr + kArenaRegions * kArenaStride;
+  uint32_t range = blockIdx.x % kWorkSplits, spent = 0;
+  const float4* __restrict__ V = pool.V;
+  constexpr int S = kTileSpan;  // window tile: S^3 nodes from the lowest in-window base
+  for (;;) {
+    __syncthreads();  // the previous item is finished with the LDS and with s_item
+    if (tid == 0) s_item = next_item(wctr, items, range, spent);
+    __syncthreads();
+    const uint32_t k = s_item;
+    if (k >= items) break;
+    const uint32_t start = pool.wl_start[k], cnt = pool.wl_start[k + 1] - start;
+    const uint32_t wb = pool.wl_block[k];
+    const SceneGeo geo = uniform_geo(pool, wb >> mshift);
+    const uint32_t mb = wb & ((1u << mshift) - 1u);
+    const int wx = 4 * (int)compact3(mb) - kDrift, wy = 4 * (int)compact3(mb >> 1) - kDrift,
+              wz = 4 * (int)compact3(mb >> 2) - kDrift;  // lowest in-window base
+    int ox = 0, oy = 0, oz = 0, ex = 0, ey = 0, ez = 0;
+    if (pool.tile_off[k] != 0xffffffffu) {
+      const int4 h = pool.tile_hdr[k];
+      ox = h.x; oy = h.y; oz = h.z;
+      ex = h.w & 255; ey = (h.w >> 8) & 255; ez = (h.w >> 16) & 255;
+    }
+    int nodes = ex * ey * ez;
+    if (nodes > kFuseBoxNodes) nodes = 0;  // gathers from V
+    for (int t = tid; t < nodes; t += kThreads) {
+      const int lx = t % ex, ly = (t / ex) % ey, lz = t / (ex * ey);
+      vbox[t] = V[node_addr(ox + lx, oy + ly, oz + lz, geo)];
+    }
+    for (int t = tid; t < kTileNodes; t += kThreads)
+      reinterpret_cast<int4*>(tile)[t] = make_int4(0, 0, 0, 0);  // the 4 planes, kTileNodes int4 in all
+    if (tid < 3) { bbox[tid] = INT_MAX; bbox[3 + tid] = INT_MIN; }
+    __syncthreads();
+    int lo0 = INT_MAX, lo1 = INT_MAX, lo2 = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hi2 = INT_MIN;
+    for (uint32_t li = tid; li < cnt; li += kThreads) {
+      const uint32_t j = start + li;
+      const uint32_t si = perm ? perm[j] : j;
+      G2PIn in;
+      g2p_load(src, si, pool.cap, in);
+      const bool inbox = nodes > 0 && (unsigned)(in.bx - ox) <= (unsigned)(ex - 3) &&
+                         (unsigned)(in.by - oy) <= (unsigned)(ey - 3) && (unsigned)(in.bz - oz) <= (unsigned)(ez - 3);
+      f3 v;
+      m3 B;
+      if (__builtin_amdgcn_ballot_w64(!inbox) == 0) {
+        const int tb = ((in.bz - oz) * ey + (in.by - oy)) * ex + (in.bx - ox);
+        g2p_gather(in, v, B, [&](int, int, int, int a, int b, int c) { return vbox[tb + (c * ey + b) * ex + a]; });
+      } else {
+        g2p_gather(in, v, B,
+                   [&](int bx, int by, int bz, int a, int b, int c) { return V[node_addr(bx + a, by + b, bz + c, geo)]; });
+      }
+      float4 r[8];
+      g2p_finish(pool.mats, geo, src, si, dst, sorted_out ? j : si, pool.cap, vmax, in, v, B, r);
+      // the next substep's P2G of the new record
+      const int b0 = (int)floorf(r[0].x - 0.5f), b1 = (int)floorf(r[0].y - 0.5f), b2 = (int)floorf(r[0].z - 0.5f);
+      const bool inwin = (unsigned)(b0 - wx) < (unsigned)(4 + 2 * kDrift) &&
+                         (unsigned)(b1 - wy) < (unsigned)(4 + 2 * kDrift) && (unsigned)(b2 - wz) < (unsigned)(4 + 2 * kDrift);
+      if (inwin) {
+        lo0 = min(lo0, b0); lo1 = min(lo1, b1); lo2 = min(lo2, b2);
+        hi0 = max(hi0, b0); hi1 = max(hi1, b1); hi2 = max(hi2, b2);
+        const int tb = ((b2 - wz) * S + (b1 - wy)) * S + (b0 - wx);
+        p2g_particle(r, pool.mats, geo.vol4, [&](int gx, int gy, int gz, int q0, int q1, int q2, int q3) {
+          const int t = tb + ((gz - b2) * S + (gy - b1)) * S + (gx - b0);
+          atomicAdd(&tile[t], q0);
+          atomicAdd(&tile[kTileNodes + t], q1);
+          atomicAdd(&tile[2 * kTileNodes + t], q2);
+          atomicAdd(&tile[3 * kTileNodes + t], q3);
+        });
+      } else {
+        fallback_particle(pool, r, epoch, true, bin_epoch);
+      }
+    }
+    lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
+    hi0 = wave_max(hi0); hi1 = wave_max(hi1); hi2 = wave_max(hi2);
+    if ((tid & 63) == 0) {
+      atomicMin(&bbox[0], lo0); atomicMin(&bbox[1], lo1); atomicMin(&bbox[2], lo2);
+      atomicMax(&bbox[3], hi0); atomicMax(&bbox[4], hi1); atomicMax(&bbox[5], hi2);
+    }
+    __syncthreads();
+    const bool any = bbox[0] != INT_MAX;
+    const int px = bbox[0], py = bbox[1], pz = bbox[2];
+    const int fx = any ? bbox[3] - px + 3 : 0, fy = any ? bbox[4] - py + 3 : 0, fz = any ? bbox[5] - pz + 3 : 0;
+    const int pnodes = fx * fy * fz;  // <= kTileNodes: inside the window tile
+    if (tid == 0) {  // arena space for the tile, from this workgroup's region
+      uint32_t off = 0xffffffffu;
+      const uint32_t region = blockIdx.x % kArenaRegions, rsize = pool.tile_nodes / kArenaRegions;
+      uint32_t* rc = pool.arena_ctr + region * kArenaStride;
+      if (pnodes > 0 && __hip_atomic_load(rc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < rsize) {
+        const uint32_t o = atomicAdd(rc, (uint32_t)pnodes);
+        if (o <= rsize - (uint32_t)pnodes) off = region * rsize + o;
+      }
+      s_off = off;
+      pool.tile_off[k] = off;
+      if (off != 0xffffffffu) pool.tile_hdr[k] = make_int4(px, py, pz, fx | (fy << 8) | (fz << 16));
+    }
+    __syncthreads();
+    const uint32_t toff = s_off;
+    for (int t = tid; t < pnodes; t += kThreads) {
+      const int lx = t % fx, ly = (t / fx) % fy, lz = t / (fx * fy);
+      const int w = ((pz + lz - wz) * S + (py + ly - wy)) * S + (px + lx - wx);
+      const int4 q = make_int4(tile[w], tile[kTileNodes + w], tile[2 * kTileNodes + w], tile[3 * kTileNodes + w]);
+      if (toff != 0xffffffffu) {
+        pool.tiles[toff + (uint32_t)t] = q;
+      } else if (q.x | q.y | q.z | q.w) {  // no arena room: add the node to A
+        const int gx = px + lx, gy = py + ly, gz = pz + lz;
+        stamp_fallback(pool, geo.block_base + (uint32_t)(gx >> 2) + (uint32_t)geo.nbx * ((uint32_t)(gy >> 2) +
+                                 (uint32_t)geo.nby * (uint32_t)(gz >> 2)), epoch, true, bin_epoch);
+        int* g = reinterpret_cast<int*>(pool.A + node_addr(gx, gy, gz, geo));
+        atomicAdd(g, q.x);
+        atomicAdd(g + 1, q.y);
+        atomicAdd(g + 2, q.z);
+        atomicAdd(g + 3, q.w);
+      }
+    }
+  }
+}
+
 // Global-atomic P2G (the reference's forceGlobal path, mpm.js:342-345), any particle order; every
 // written node block goes to D.
 __global__ __launch_bounds__(kThreads) void k_p2g_global(Pool pool, const float4* __restrict__ P, uint32_t n) {
@@ -1147,6 +1287,17 @@ hipError_t launch_g2p_items(const Pool& pool, const float4* src, float4* dst, co
     case 5: MPM_G2P_LAUNCH(k_g2p_items<5>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax); break;
     case 6: MPM_G2P_LAUNCH(k_g2p_items<6>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax); break;
     default: MPM_G2P_LAUNCH(k_g2p_items<3>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax); break;
+  }
+  return hipGetLastError();
+}
+hipError_t launch_g2p_p2g(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, bool sorted_out,
+                          float vmax, int waves, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop) {
+  if (sorted_out ? (!perm || src == dst) : src != dst) return hipErrorInvalidValue;
+  const dim3 grid(persistent_grid(pool.max_items, 1, 2048));
+  const int so = sorted_out ? 1 : 0;
+  switch (waves) {
+    case 4: MPM_G2P_LAUNCH(k_g2p_p2g<4>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax); break;
+    default: MPM_G2P_LAUNCH(k_g2p_p2g<3>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax); break;
   }
   return hipGetLastError();
 }

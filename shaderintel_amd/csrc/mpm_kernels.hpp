@@ -183,6 +183,12 @@ hipError_t launch_g2p(const Pool& pool, float4* P, uint32_t n, float vmax, int w
 hipError_t launch_g2p_items(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, bool sorted_out,
                             float vmax, int waves, hipStream_t s, hipEvent_t ev_start = nullptr,
                             hipEvent_t ev_stop = nullptr);
+// G2P (as launch_g2p_items) fused with the next substep's P2G over the same work items: the new
+// records are scattered into tiles (+ fallback atomics on A) without being read back. Needs
+// launch_begin_p2g after the grid update; leaves the grid state of a binned P2G.
+hipError_t launch_g2p_p2g(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, bool sorted_out,
+                          float vmax, int waves, hipStream_t s, hipEvent_t ev_start = nullptr,
+                          hipEvent_t ev_stop = nullptr);
 // G2P that also applies a pending reorder: slot i of dst takes particle perm[i] of src.
 hipError_t launch_g2p_perm(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, uint32_t n, float vmax,
                            int waves, hipStream_t s, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);

@@ -148,6 +148,8 @@ struct mpm_pool {
   int g2p_waves = 3, p2g_waves = 4;
   bool g2p_items = true;     // G2P by work item (LDS velocity boxes) after a binned P2G
   bool tiles_fresh = false;  // the tiles and work items are those of this substep's binned P2G
+  bool fuse = true;          // G2P fused with the next substep's P2G when no sort comes between them
+  bool p2g_ahead = false;    // the next substep's P2G is done (by a fused G2P of this mpm_step call)
   bool use_graphs = false;
   mpm_grid_hook hook = nullptr;  // called after every grid update (not with graphs)
   void* hook_user = nullptr;
@@ -433,6 +435,11 @@ mpm_status repack(mpm_pool* p) {
 // P2G (mpm.js:607-612). Binned: into tiles (+ A for stragglers), grid state kGridTiles. Global: every
 // contribution a global atomic on A, grid state kGridA.
 mpm_status do_p2g(mpm_pool* p, bool force_global) {
+  if (p->p2g_ahead && !force_global) {  // done by the fused G2P of the previous substep
+    p->p2g_ahead = false;
+    return MPM_OK;
+  }
+  p->p2g_ahead = false;
   ST_TRY(clean_a(p));
   if (!force_global && !p->bin_valid) ST_TRY(do_bin(p, kBinOnly));
   if (force_global) ST_TRY(apply_pending_reorder(p));
@@ -478,7 +485,9 @@ mpm_status mark(mpm_pool* p) {
   return MPM_OK;
 }
 
-mpm_status do_g2p(mpm_pool* p, float vmax) {  // mpm.js:598-600
+// G2P (mpm.js:598-600). fuse_next: the next substep follows in the same call and does not sort, so
+// its P2G may be done by this G2P (same work items).
+mpm_status do_g2p(mpm_pool* p, float vmax, bool fuse_next) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (p->profiling == 2) {
     ST_TRY(next_event(p, &e0));
@@ -486,6 +495,24 @@ mpm_status do_g2p(mpm_pool* p, float vmax) {  // mpm.js:598-600
   }
   const bool items = p->g2p_items && p->tiles_fresh;
   p->tiles_fresh = false;  // one G2P per P2G: the G2P work counters are those begin_p2g cleared
+  if (items && fuse_next && p->fuse) {
+    ST_TRY(clean_a(p));
+    HIP_TRY(launch_begin_p2g(view(p), p->stream));  // the next P2G's epoch, arena and work counters
+    const bool sorted_out = p->reorder_in_g2p;
+    HIP_TRY(launch_g2p_p2g(view(p), p->P[p->cur], p->P[p->cur ^ (sorted_out ? 1 : 0)], p->perm, sorted_out, vmax,
+                           p->g2p_waves, p->stream, e0, e1));
+    if (sorted_out) {
+      p->cur ^= 1;
+      p->perm = nullptr;
+      p->reorder_in_g2p = false;
+      recompute_offsets(p);
+    }
+    p->grid_state = kGridTiles;
+    p->a_dirty_d = true;
+    p->tiles_fresh = true;
+    p->p2g_ahead = true;
+    return MPM_OK;
+  }
   if (p->reorder_in_g2p) {  // the sort's reorder, applied by the record writes
     if (items)
       HIP_TRY(launch_g2p_items(view(p), p->P[p->cur], p->P[p->cur ^ 1], p->perm, true, vmax, p->g2p_waves, p->stream,
@@ -541,18 +568,19 @@ mpm_status substep_front(mpm_pool* p, uint32_t index) {
 }
 
 // Rest of a substep: G2P (mpm.js:598-601).
-mpm_status substep_back(mpm_pool* p, float vmax) {
+mpm_status substep_back(mpm_pool* p, float vmax, bool fuse_next = false) {
   ST_TRY(mark(p));
-  ST_TRY(do_g2p(p, vmax));
+  ST_TRY(do_g2p(p, vmax, fuse_next));
   ST_TRY(mark(p));
   p->step_count++;
   return MPM_OK;
 }
 
 // One substep on the stream (MPM.step's loop body, mpm.js:591-601).
-mpm_status substep(mpm_pool* p, float vmax, uint32_t index = 0) {
+mpm_status substep(mpm_pool* p, float vmax, uint32_t index = 0, bool more = false) {
   ST_TRY(substep_front(p, index));
-  return substep_back(p, vmax);
+  // fuse this G2P with the next P2G when another substep follows in this call and does not sort
+  return substep_back(p, vmax, more && (p->step_count + 1) % p->sort_every != 0);
 }
 
 // One sort period (sort_every substeps starting at a sort) captured once per parity of the current
@@ -575,7 +603,7 @@ mpm_status period_graph(mpm_pool* p, float vmax, hipGraphExec_t* out) {
   const uint64_t step0 = p->step_count;
   HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
   mpm_status st = MPM_OK;
-  for (uint32_t k = 0; k < p->sort_every && st == MPM_OK; k++) st = substep(p, vmax);
+  for (uint32_t k = 0; k < p->sort_every && st == MPM_OK; k++) st = substep(p, vmax, k, k + 1 < p->sort_every);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(p->stream, &g);
   p->step_count = step0;
@@ -615,7 +643,7 @@ mpm_status pool_step(mpm_pool* p, uint32_t substeps, float vmax) {
       p->step_count += p->sort_every;
       k += p->sort_every;
     } else {
-      ST_TRY(substep(p, vmax, k));
+      ST_TRY(substep(p, vmax, k, k + 1 < substeps));
       k++;
     }
   }
@@ -1454,6 +1482,11 @@ mpm_status mpm_pool_set_tuning(mpm_pool* p, int32_t knob, int32_t value) {
     case MPM_TUNE_G2P_WAVES:
       if (value < 3 || value > 6) break;
       p->g2p_waves = value;
+      drop_graphs(p);
+      return MPM_OK;
+    case MPM_TUNE_FUSE:
+      if (value < 0 || value > 1) break;
+      p->fuse = value != 0;
       drop_graphs(p);
       return MPM_OK;
     case MPM_TUNE_G2P_ITEMS:

@@ -574,10 +574,10 @@ class MPM:
     def sync(self) -> None:
         _check(load_library().mpm_sync(self._h))
 
-    TUNE = {"g2p_waves": 2, "p2g_waves": 4, "g2p_items": 5}
+    TUNE = {"g2p_waves": 2, "p2g_waves": 4, "g2p_items": 5, "fuse": 6}
 
     def set_tuning(self, **knobs) -> None:
-        """Kernel variant knobs (result-neutral): g2p_waves, p2g_waves, g2p_items."""
+        """Kernel variant knobs (result-neutral): g2p_waves, p2g_waves, g2p_items, fuse."""
         for k, v in knobs.items():
             _check(load_library().mpm_set_tuning(self._h, self.TUNE[k], int(v)), "MPM.set_tuning")
 
