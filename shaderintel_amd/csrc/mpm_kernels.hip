@@ -57,8 +57,37 @@ __device__ __forceinline__ uint32_t compact3(uint32_t x) {  // inverse of spread
   return x;
 }
 
+// Sort key of a particle at x (x.xyz in its scene's lattice): the scene in the top bits, then the
+// Morton code of the stencil base cell (a refinement of the reference's 4^3-block key,
+// mpm.js:567-568); particles of an inactive scene get 0xffffffff (sorted behind every live one).
+__device__ __forceinline__ uint32_t sort_key(const Pool& pool, uint32_t scene, uint32_t active, float4 x) {
+  if (!active) return 0xffffffffu;
+  return (scene << pool.key_bits) |
+         morton3((uint32_t)fmaxf(floorf(x.x - 0.5f), 0.0f), (uint32_t)fmaxf(floorf(x.y - 0.5f), 0.0f),
+                 (uint32_t)fmaxf(floorf(x.z - 0.5f), 0.0f));
+}
+
+// toFixed's clamp to +-2e9 (mpm.js:64-66) can only act on a particle whose per-node quantity
+// q = mv + affine*dpos exceeds ~1130 in some component (wts <= 0.421875 * 2^22 after rounding) or
+// whose mass does. p2g_unclamped(r) bounds |q| over all 27 nodes from the record (|dpos| <= 1.5 per
+// axis) and is true when the bound stays under 1024, with every comparison false on NaN or inf; for
+// such a particle rint(clamp(x)) == rint(x) for every contribution, so the clamp can be skipped.
+__device__ __forceinline__ bool p2g_unclamped(const float4 r[kP2GStreams], const float* __restrict__ mats, float vol4) {
+  const float mass = fabsf(mats[12 * __float_as_uint(r[0].w) + 1]);
+  const float t00 = fabsf(r[2].w), t11 = fabsf(r[3].w), t22 = fabsf(r[4].w);
+  const float t01 = fabsf(r[1].w), t02 = fabsf(r[5].x), t12 = fabsf(r[5].y);
+  const float vs = fabsf(vol4);
+  // row i of |affine| <= vol4 |tau row i| + mass |C row i| (C row i = component i of the columns)
+  const float a0 = ((t00 + t01) + t02) * vs + ((fabsf(r[2].x) + fabsf(r[3].x)) + fabsf(r[4].x)) * mass;
+  const float a1 = ((t01 + t11) + t12) * vs + ((fabsf(r[2].y) + fabsf(r[3].y)) + fabsf(r[4].y)) * mass;
+  const float a2 = ((t02 + t12) + t22) * vs + ((fabsf(r[2].z) + fabsf(r[3].z)) + fabsf(r[4].z)) * mass;
+  return mass <= 1024.0f && fabsf(r[1].x) * mass + 1.5f * a0 <= 1024.0f && fabsf(r[1].y) * mass + 1.5f * a1 <= 1024.0f &&
+         fabsf(r[1].z) * mass + 1.5f * a2 <= 1024.0f;
+}
+
 // The 27 fixed-point contributions of one particle (mpm.js:318-349). add(gx, gy, gz, q0..q3).
-template <class Add>
+// CLAMP = false skips toFixed's clamp: only for particles p2g_unclamped accepts.
+template <bool CLAMP = true, class Add>
 __device__ __forceinline__ void p2g_particle(const float4 r[kP2GStreams], const float* __restrict__ mats, float vol4,
                                              Add&& add) {
   const f3 xp = mk(r[0].x, r[0].y, r[0].z);
@@ -87,8 +116,12 @@ __device__ __forceinline__ void p2g_particle(const float4 r[kP2GStreams], const 
         // rounds to the same integer (0) when wt * q is subnormal; overflow clamps alike.
         const float wts = wt * kFP;
         const f3 mom = wts * (mv + mul(affine, dpos));
-        add(bx + a, by + b, bz + c, fixed_of_scaled(mom.x), fixed_of_scaled(mom.y), fixed_of_scaled(mom.z),
-            fixed_of_scaled(wts * mass));
+        if (CLAMP)
+          add(bx + a, by + b, bz + c, fixed_of_scaled(mom.x), fixed_of_scaled(mom.y), fixed_of_scaled(mom.z),
+              fixed_of_scaled(wts * mass));
+        else
+          add(bx + a, by + b, bz + c, (int32_t)rintf(mom.x), (int32_t)rintf(mom.y), (int32_t)rintf(mom.z),
+              (int32_t)rintf(wts * mass));
       }
 }
 
@@ -351,7 +384,8 @@ __device__ __forceinline__ uint32_t next_item(uint32_t* wctr, uint32_t items, ui
 // in place (src == dst, each record back to its own slot).
 template <int WAVES>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_g2p_items(
-    Pool pool, const float4* src, float4* dst, const uint32_t* __restrict__ perm, int sorted_out, float vmax) {
+    Pool pool, const float4* src, float4* dst, const uint32_t* __restrict__ perm, int sorted_out, float vmax,
+    uint32_t* __restrict__ keys_out) {
   __shared__ float4 box[kTileNodes];
   __shared__ uint32_t s_item;
   const int tid = threadIdx.x;
@@ -367,7 +401,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
     const uint32_t k = s_item;
     if (k >= items) break;
     const uint32_t start = pool.wl_start[k], cnt = pool.wl_start[k + 1] - start;
-    const SceneGeo geo = uniform_geo(pool, pool.wl_block[k] >> mshift);
+    const uint32_t scene = pool.wl_block[k] >> mshift;
+    const SceneGeo geo = uniform_geo(pool, scene);
     int ox = 0, oy = 0, oz = 0, ex = 0, ey = 0, ez = 0;
     if (pool.tile_off[k] != 0xffffffffu) {
       const int4 h = pool.tile_hdr[k];
@@ -397,7 +432,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
                    [&](int bx, int by, int bz, int a, int b, int c) { return V[node_addr(bx + a, by + b, bz + c, geo)]; });
       }
       float4 r[8];
-      g2p_finish(pool.mats, geo, src, si, dst, sorted_out ? j : si, pool.cap, vmax, in, v, B, r);
+      const uint32_t di = sorted_out ? j : si;
+      g2p_finish(pool.mats, geo, src, si, dst, di, pool.cap, vmax, in, v, B, r);
+      if (keys_out) keys_out[di] = sort_key(pool, scene, geo.active, r[0]);  // for the next substep's sort
     }
   }
 }
@@ -474,24 +511,14 @@ __global__ void k_begin_bin(uint32_t* ctr) {
 }
 
 
-// Sort key: scene in the top bits, then the Morton code of the stencil base cell (a refinement of
-// the reference's 4^3-block key, mpm.js:567-568); vals = current index, so the radix sort's
-// stable order breaks ties by it.
-__global__ void k_keys(Pool pool, const float4* __restrict__ P, uint32_t n, uint32_t* __restrict__ keys,
-                       uint32_t* __restrict__ vals) {
+// Sort keys of the particles in their current order (sort_key); the radix sort's values are the
+// indices, so its stable order breaks ties by them.
+__global__ void k_keys(Pool pool, const float4* __restrict__ P, uint32_t n, uint32_t* __restrict__ keys) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 x = P[i];
   const uint32_t scene = __float_as_uint(x.w) >> 3;
-  if (!pool.scenes[scene].active) {
-    keys[i] = 0xffffffffu;
-    vals[i] = i;
-    return;
-  }
-  keys[i] = (scene << pool.key_bits) |
-            morton3((uint32_t)fmaxf(floorf(x.x - 0.5f), 0.0f), (uint32_t)fmaxf(floorf(x.y - 0.5f), 0.0f),
-                    (uint32_t)fmaxf(floorf(x.z - 0.5f), 0.0f));
-  vals[i] = i;
+  keys[i] = sort_key(pool, scene, pool.scenes[scene].active, x);
 }
 
 __global__ void k_live_keys(Pool pool, const float4* __restrict__ P, uint32_t n, uint32_t* __restrict__ keys,
@@ -832,13 +859,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
                          (unsigned)(b1 - wy) < (unsigned)(4 + 2 * kDrift) && (unsigned)(b2 - wz) < (unsigned)(4 + 2 * kDrift);
       if (inwin && has_slot) {
         const int tb = ((b2 - oz) * ey + (b1 - oy)) * ex + (b0 - ox);
-        p2g_particle(r, pool.mats, vol4, [&](int gx, int gy, int gz, int q0, int q1, int q2, int q3) {
+        auto to_tile = [&](int gx, int gy, int gz, int q0, int q1, int q2, int q3) {
           const int t = tb + ((gz - b2) * ey + (gy - b1)) * ex + (gx - b0);
           atomicAdd(&tile[t], q0);
           atomicAdd(&tile[kTileNodes + t], q1);
           atomicAdd(&tile[2 * kTileNodes + t], q2);
           atomicAdd(&tile[3 * kTileNodes + t], q3);
-        });
+        };
+        p2g_particle(r, pool.mats, vol4, to_tile);
       } else {
         fallback_particle(pool, r, epoch, true, bin_epoch);
       }
@@ -935,13 +963,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
         lo0 = min(lo0, b0); lo1 = min(lo1, b1); lo2 = min(lo2, b2);
         hi0 = max(hi0, b0); hi1 = max(hi1, b1); hi2 = max(hi2, b2);
         const int tb = ((b2 - wz) * S + (b1 - wy)) * S + (b0 - wx);
-        p2g_particle(r, pool.mats, geo.vol4, [&](int gx, int gy, int gz, int q0, int q1, int q2, int q3) {
+        auto to_tile = [&](int gx, int gy, int gz, int q0, int q1, int q2, int q3) {
           const int t = tb + ((gz - b2) * S + (gy - b1)) * S + (gx - b0);
           atomicAdd(&tile[t], q0);
           atomicAdd(&tile[kTileNodes + t], q1);
           atomicAdd(&tile[2 * kTileNodes + t], q2);
           atomicAdd(&tile[3 * kTileNodes + t], q3);
-        });
+        };
+        // one wave-uniform choice: the clamp-free form when no active lane's particle can reach the clamp
+        if (__builtin_amdgcn_ballot_w64(!p2g_unclamped(r, pool.mats, geo.vol4)) == 0)
+          p2g_particle<false>(r, pool.mats, geo.vol4, to_tile);
+        else
+          p2g_particle<true>(r, pool.mats, geo.vol4, to_tile);
       } else {
         fallback_particle(pool, r, epoch, true, bin_epoch);
       }
@@ -1185,9 +1218,9 @@ hipError_t launch_begin_bin(const Pool& pool, hipStream_t s) {
   hipLaunchKernelGGL(k_begin_bin, dim3(1), dim3(1), 0, s, pool.ctr);
   return hipGetLastError();
 }
-hipError_t launch_keys(const Pool& pool, const float4* P, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s) {
+hipError_t launch_keys(const Pool& pool, const float4* P, uint32_t n, uint32_t* keys, hipStream_t s) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, P, n, keys, vals);
+  hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, P, n, keys);
   return hipGetLastError();
 }
 hipError_t launch_live_keys(const Pool& pool, const float4* P, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s) {
@@ -1278,15 +1311,16 @@ hipError_t launch_g2p_perm(const Pool& pool, const float4* src, float4* dst, con
   return hipGetLastError();
 }
 hipError_t launch_g2p_items(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, bool sorted_out,
-                            float vmax, int waves, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop) {
+                            float vmax, int waves, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop,
+                            uint32_t* keys_out) {
   if (sorted_out ? (!perm || src == dst) : src != dst) return hipErrorInvalidValue;
   const dim3 grid(persistent_grid(pool.max_items, 1, 2048));
   const int so = sorted_out ? 1 : 0;
   switch (waves) {
-    case 4: MPM_G2P_LAUNCH(k_g2p_items<4>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax); break;
-    case 5: MPM_G2P_LAUNCH(k_g2p_items<5>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax); break;
-    case 6: MPM_G2P_LAUNCH(k_g2p_items<6>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax); break;
-    default: MPM_G2P_LAUNCH(k_g2p_items<3>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax); break;
+    case 4: MPM_G2P_LAUNCH(k_g2p_items<4>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax, keys_out); break;
+    case 5: MPM_G2P_LAUNCH(k_g2p_items<5>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax, keys_out); break;
+    case 6: MPM_G2P_LAUNCH(k_g2p_items<6>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax, keys_out); break;
+    default: MPM_G2P_LAUNCH(k_g2p_items<3>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax, keys_out); break;
   }
   return hipGetLastError();
 }

@@ -143,9 +143,9 @@ hipError_t launch_begin_p2g(const Pool& pool, hipStream_t s);
 // ctr[kCtrBinEpoch]++, |S| = 0, items = 0 (start of a binning)
 hipError_t launch_begin_bin(const Pool& pool, hipStream_t s);
 
-// Sort keys (scene << key_bits | Morton(base cell)) and identity values; particles of inactive
-// scenes get key 0xffffffff (sorted behind every live particle).
-hipError_t launch_keys(const Pool& pool, const float4* P, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s);
+// Sort keys (scene << key_bits | Morton(base cell)) of the particles in their current order;
+// particles of inactive scenes get key 0xffffffff (sorted behind every live particle).
+hipError_t launch_keys(const Pool& pool, const float4* P, uint32_t n, uint32_t* keys, hipStream_t s);
 // Keys 0 for particles of active scenes, 1 for removed ones (a stable sort on them is a stable
 // partition); vals = index.
 hipError_t launch_live_keys(const Pool& pool, const float4* P, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s);
@@ -179,10 +179,11 @@ hipError_t launch_g2p(const Pool& pool, float4* P, uint32_t n, float vmax, int w
                       hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
 // G2P over the work items of this substep's binned P2G, the velocities of each item's tile box
 // staged in LDS. perm: sorted position -> record (nullptr: identity). sorted_out: slot i of dst
-// takes record perm[i] of src (as launch_g2p_perm); otherwise in place (src == dst).
+// takes record perm[i] of src (as launch_g2p_perm); otherwise in place (src == dst). keys_out
+// (optional): the sort key of each new record at its slot (as launch_keys computes them after it).
 hipError_t launch_g2p_items(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, bool sorted_out,
                             float vmax, int waves, hipStream_t s, hipEvent_t ev_start = nullptr,
-                            hipEvent_t ev_stop = nullptr);
+                            hipEvent_t ev_stop = nullptr, uint32_t* keys_out = nullptr);
 // G2P (as launch_g2p_items) fused with the next substep's P2G over the same work items: the new
 // records are scattered into tiles (+ fallback atomics on A) without being read back. Needs
 // launch_begin_p2g after the grid update; leaves the grid state of a binned P2G.

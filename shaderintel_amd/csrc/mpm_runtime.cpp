@@ -150,6 +150,8 @@ struct mpm_pool {
   bool tiles_fresh = false;  // the tiles and work items are those of this substep's binned P2G
   bool fuse = true;          // G2P fused with the next substep's P2G when no sort comes between them
   bool p2g_ahead = false;    // the next substep's P2G is done (by a fused G2P of this mpm_step call)
+  bool keys_ahead = false;   // keys[0] holds the sort keys of the current particles (written by the
+                             // G2P before a sort in the same mpm_step call)
   bool use_graphs = false;
   mpm_grid_hook hook = nullptr;  // called after every grid update (not with graphs)
   void* hook_user = nullptr;
@@ -361,7 +363,10 @@ mpm_status apply_pending_reorder(mpm_pool* p) {
 mpm_status repack(mpm_pool* p);
 
 mpm_status do_bin(mpm_pool* p, int mode) {
-  if (p->needs_repack) ST_TRY(repack(p));  // binning assumes every particle belongs to a live scene
+  if (p->needs_repack) {  // binning assumes every particle belongs to a live scene
+    p->keys_ahead = false;
+    ST_TRY(repack(p));
+  }
   ST_TRY(clean_a(p));
   const Pool pv = view(p);
   HIP_TRY(launch_begin_bin(pv, p->stream));
@@ -371,11 +376,13 @@ mpm_status do_bin(mpm_pool* p, int mode) {
     p->bin_valid = true;
     return MPM_OK;
   }
-  HIP_TRY(launch_keys(pv, p->P[p->cur], p->count, p->keys[0], p->vals[0], p->stream));
+  if (!p->keys_ahead) HIP_TRY(launch_keys(pv, p->P[p->cur], p->count, p->keys[0], p->stream));
+  p->keys_ahead = false;
   uint32_t *rk = nullptr, *rv = nullptr;
-  // by block (key bits 6 and up); the cell order inside a block is made by the transposition
+  // by block (key bits 6 and up), values the current indices; the cell order inside a block is made
+  // by the transposition
   HIP_TRY(radix_sort_pairs(p->keys[0], p->vals[0], p->keys[1], p->vals[1], p->count, 6, (int)p->sort_bits,
-                           p->sort_scratch, p->stream, &rk, &rv));
+                           p->sort_scratch, p->stream, &rk, &rv, true));
   HIP_TRY(launch_fill_u32(p->map, 0xffffffffu, p->map_per_scene * p->max_scenes, p->stream));
   {  // work items: heads per tile, scanned, then numbered (the sort's scratch is free again)
     const uint32_t tiles = bin_tiles(p->count);
@@ -487,7 +494,7 @@ mpm_status mark(mpm_pool* p) {
 
 // G2P (mpm.js:598-600). fuse_next: the next substep follows in the same call and does not sort, so
 // its P2G may be done by this G2P (same work items).
-mpm_status do_g2p(mpm_pool* p, float vmax, bool fuse_next) {
+mpm_status do_g2p(mpm_pool* p, float vmax, bool fuse_next, bool keys_next) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (p->profiling == 2) {
     ST_TRY(next_event(p, &e0));
@@ -526,8 +533,11 @@ mpm_status do_g2p(mpm_pool* p, float vmax, bool fuse_next) {
     recompute_offsets(p);
     return MPM_OK;
   }
-  if (items)  // in place; perm (kBinOnly) maps the items' sorted positions to the records
-    HIP_TRY(launch_g2p_items(view(p), p->P[p->cur], p->P[p->cur], p->perm, false, vmax, p->g2p_waves, p->stream, e0, e1));
+  if (items) {  // in place; perm (kBinOnly) maps the items' sorted positions to the records
+    HIP_TRY(launch_g2p_items(view(p), p->P[p->cur], p->P[p->cur], p->perm, false, vmax, p->g2p_waves, p->stream, e0, e1,
+                             keys_next ? p->keys[0] : nullptr));
+    p->keys_ahead = keys_next;
+  }
   else
     HIP_TRY(launch_g2p(view(p), p->P[p->cur], p->count, vmax, p->g2p_waves, p->stream, e0, e1));
   return MPM_OK;
@@ -568,9 +578,9 @@ mpm_status substep_front(mpm_pool* p, uint32_t index) {
 }
 
 // Rest of a substep: G2P (mpm.js:598-601).
-mpm_status substep_back(mpm_pool* p, float vmax, bool fuse_next = false) {
+mpm_status substep_back(mpm_pool* p, float vmax, bool fuse_next = false, bool keys_next = false) {
   ST_TRY(mark(p));
-  ST_TRY(do_g2p(p, vmax, fuse_next));
+  ST_TRY(do_g2p(p, vmax, fuse_next, keys_next));
   ST_TRY(mark(p));
   p->step_count++;
   return MPM_OK;
@@ -579,8 +589,10 @@ mpm_status substep_back(mpm_pool* p, float vmax, bool fuse_next = false) {
 // One substep on the stream (MPM.step's loop body, mpm.js:591-601).
 mpm_status substep(mpm_pool* p, float vmax, uint32_t index = 0, bool more = false) {
   ST_TRY(substep_front(p, index));
-  // fuse this G2P with the next P2G when another substep follows in this call and does not sort
-  return substep_back(p, vmax, more && (p->step_count + 1) % p->sort_every != 0);
+  // another substep follows in this call: fuse this G2P with its P2G when it does not sort, else
+  // let this G2P write the sort keys
+  const bool next_sorts = (p->step_count + 1) % p->sort_every == 0;
+  return substep_back(p, vmax, more && !next_sorts, more && next_sorts);
 }
 
 // One sort period (sort_every substeps starting at a sort) captured once per parity of the current
@@ -621,6 +633,7 @@ mpm_status period_graph(mpm_pool* p, float vmax, hipGraphExec_t* out) {
 }
 
 mpm_status pool_step(mpm_pool* p, uint32_t substeps, float vmax) {
+  p->p2g_ahead = p->keys_ahead = false;  // only ever set for the next substep of the same call
   if (p->needs_repack) ST_TRY(repack(p));
   if (p->count == 0) return MPM_OK;  // mpm.js:584
   uint32_t k = 0;
@@ -629,6 +642,7 @@ mpm_status pool_step(mpm_pool* p, uint32_t substeps, float vmax) {
     if (p->use_graphs && !p->profiling && !p->hook && at_sort && substeps - k >= p->sort_every) {
       ST_TRY(clean_a(p));
       ST_TRY(apply_pending_reorder(p));
+      p->keys_ahead = false;  // a captured period computes its own sort keys
       hipGraphExec_t ge = nullptr;
       ST_TRY(period_graph(p, vmax, &ge));
       HIP_TRY(hipGraphLaunch(ge, p->stream));

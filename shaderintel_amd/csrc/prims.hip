@@ -197,7 +197,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
                                                                 const uint32_t* __restrict__ vin,
                                                                 uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                                 size_t n, int shift, int bits, uint32_t tiles,
-                                                                const uint32_t* __restrict__ hist) {
+                                                                const uint32_t* __restrict__ hist, int vals_iota) {
   __shared__ uint32_t skey[kSortTile];
   __shared__ uint32_t sval[kSortTile];
   __shared__ uint32_t run[kSortWaves][kRadixMaxBuckets];  // chunk counts, then each wave's first local slot
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
     const uint32_t e = (uint32_t)(w * kSortChunk + j * 64 + lane);
     const bool valid = e < count;
     key[j] = valid ? kin[base + e] : 0u;
-    val[j] = valid ? vin[base + e] : 0u;
+    val[j] = valid ? (vals_iota ? (uint32_t)(base + e) : vin[base + e]) : 0u;
   }
 #pragma unroll
   for (int j = 0; j < kSortItems; j++) {
@@ -331,10 +331,11 @@ hipError_t scan_exclusive(uint32_t* data, size_t n, uint32_t* scratch, hipStream
 
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, size_t n, int lo_bit,
                             int hi_bit, uint32_t* scratch, hipStream_t stream, uint32_t** result_keys,
-                            uint32_t** result_vals) {
+                            uint32_t** result_vals, bool vals_iota) {
   *result_keys = keys;
   *result_vals = vals;
-  if (n == 0 || hi_bit <= lo_bit) return hipSuccess;
+  if (n == 0) return hipSuccess;
+  if (hi_bit <= lo_bit) return vals_iota ? iota_u32(vals, n, stream) : hipSuccess;
   if (lo_bit < 0 || hi_bit > 32) return hipErrorInvalidValue;
   const int span = hi_bit - lo_bit;
   const int passes = (span + kRadixMaxBits - 1) / kRadixMaxBits;
@@ -350,7 +351,8 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, 
     hipError_t e = scan_exclusive(hist, ((size_t)1 << b) * tiles, scan_tmp, stream);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kSortThreads), 0, stream, (const uint32_t*)ka,
-                       (const uint32_t*)va, kb, vb, n, shift, b, tiles, (const uint32_t*)hist);
+                       (const uint32_t*)va, kb, vb, n, shift, b, tiles, (const uint32_t*)hist,
+                       vals_iota && shift == lo_bit ? 1 : 0);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     uint32_t* t;
