@@ -615,10 +615,13 @@ __global__ __launch_bounds__(kThreads) void k_bin_emit(Pool pool, const uint32_t
   while (m) {
     const int j = __builtin_ctz(m);
     m &= m - 1;
-    const uint32_t blk = k[j] >> 6;
+    const uint32_t blk = k[j] >> 6, scene = k[j] >> pool.key_bits, mb = blk & mmask;
     pool.wl_start[item] = first + j;
     pool.wl_block[item] = blk;
-    pool.map[pool.scenes[k[j] >> pool.key_bits].map_base + (blk & mmask)] = item;
+    pool.map[pool.scenes[scene].map_base + mb] = item;
+    const uint32_t pb = 1u << (pool.key_bits / 3 - 2), W = occ_row_words(pb);
+    const uint32_t bx = compact3(mb), by = compact3(mb >> 1), bz = compact3(mb >> 2);
+    atomicOr(pool.occ + ((size_t)scene * pb * pb + (size_t)bz * pb + by) * W + (bx >> 6), 1ull << (bx & 63u));
     item++;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -712,35 +715,64 @@ __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uin
   }
 }
 
-// S: the node blocks [B-1, B+2]^3 of every work item (everything its tile can reach). One wave per
-// item (lane = node block of the 4^3 neighbourhood); the wave's newly added blocks are appended with
-// one atomic on the list counter.
-__global__ __launch_bounds__(kThreads) void k_bin_dilate(Pool pool) {
-  const uint32_t cnt = pool.ctr[kCtrItems];
+// S: the node blocks [B-1, B+2]^3 of every work item B (everything its tile can reach), i.e. node
+// block b when a particle block in [b-2, b+1]^3 is occupied. One thread per 64-block word of a row
+// (by, bz) of a scene's node-block grid: the row's word ORs the x-dilated occupancy words
+// (O | O<<1 | O<<2 | O>>1) of the 16 rows (by-2..by+1, bz-2..bz+1); its set bits inside the grid are
+// appended to slist (one atomic per wave) and stamped.
+__device__ __forceinline__ unsigned long long occ_word(const Pool& pool, uint32_t scene, uint32_t pb, uint32_t W, int by,
+                                                       int bz, int w) {
+  if (by < 0 || bz < 0 || by >= (int)pb || bz >= (int)pb || w < 0 || w >= (int)W) return 0ull;
+  return pool.occ[((size_t)scene * pb * pb + (size_t)bz * pb + (size_t)by) * W + (size_t)w];
+}
+__global__ __launch_bounds__(kThreads) void k_bin_dilate(Pool pool, uint32_t max_scenes) {
+  const uint32_t pb = 1u << (pool.key_bits / 3 - 2), W = occ_row_words(pb);
   const uint32_t epoch = pool.ctr[kCtrBinEpoch];
-  const uint32_t mshift = pool.key_bits - 6;
   const int lane = threadIdx.x & 63;
-  for (uint32_t t = blockIdx.x * kThreads + threadIdx.x; t < cnt * 64; t += gridDim.x * kThreads) {
-    const uint32_t k = t >> 6, d = t & 63;  // one item per wave
-    const uint32_t wb = __builtin_amdgcn_readfirstlane(pool.wl_block[k]);
-    const SceneGeo geo = uniform_geo(pool, wb >> mshift);
-    const uint32_t mb = wb & ((1u << mshift) - 1u);
-    const int gx = (int)compact3(mb) - 1 + (int)(d & 3), gy = (int)compact3(mb >> 1) - 1 + (int)((d >> 2) & 3),
-              gz = (int)compact3(mb >> 2) - 1 + (int)(d >> 4);
-    bool added = false;
-    uint32_t b = 0;
-    if (gx >= 0 && gy >= 0 && gz >= 0 && gx < geo.nbx && gy < geo.nby && gz < geo.nbz) {
-      b = geo.block_base + (uint32_t)gx + (uint32_t)geo.nbx * ((uint32_t)gy + (uint32_t)geo.nby * (uint32_t)gz);
-      if (__hip_atomic_load(&pool.sstamp[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch)
-        added = atomicExch(&pool.sstamp[b], epoch) != epoch;
+  const uint64_t total = (uint64_t)max_scenes * pb * pb * W;
+  for (uint64_t t0 = (uint64_t)blockIdx.x * kThreads; t0 < total; t0 += (uint64_t)gridDim.x * kThreads) {
+    const uint64_t t = t0 + threadIdx.x;  // the loop bound is uniform over the block
+    unsigned long long d = 0;
+    uint32_t scene = 0, base = 0;
+    int w = 0, by = 0, bz = 0, nbx = 0, nby = 0;
+    if (t < total) {
+      w = (int)(t % W);
+      by = (int)((t / W) % pb);
+      bz = (int)((t / ((uint64_t)W * pb)) % pb);
+      scene = (uint32_t)(t / ((uint64_t)W * pb * pb));
+      const SceneGeo& geo = pool.scenes[scene];
+      nbx = geo.nbx; nby = geo.nby;
+      base = geo.block_base;
+      if (geo.active && by < geo.nby && bz < geo.nbz && 64 * w < geo.nbx) {
+        for (int dz = -2; dz <= 1; dz++)
+          for (int dy = -2; dy <= 1; dy++) {
+            const unsigned long long o = occ_word(pool, scene, pb, W, by + dy, bz + dz, w);
+            const unsigned long long lo = occ_word(pool, scene, pb, W, by + dy, bz + dz, w - 1);
+            const unsigned long long hi = occ_word(pool, scene, pb, W, by + dy, bz + dz, w + 1);
+            d |= o | (o << 1) | (lo >> 63) | (o << 2) | (lo >> 62) | (o >> 1) | (hi << 63);
+          }
+        const int valid = geo.nbx - 64 * w;  // bits of this word inside the grid
+        if (valid < 64) d &= (1ull << valid) - 1ull;
+      }
     }
-    const uint64_t mask = __builtin_amdgcn_ballot_w64(added);
-    if (mask) {
-      const int leader = __builtin_ctzll(mask);
-      uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(pool.ctr + kCtrS, (uint32_t)__builtin_popcountll(mask));
-      base = (uint32_t)__shfl((int)base, leader, 64);
-      if (added) pool.slist[base + (uint32_t)__builtin_popcountll(mask & ((1ull << lane) - 1ull))] = b;
+    // wave prefix of the counts, one atomic per wave
+    const uint32_t c = (uint32_t)__popcll(d);
+    uint32_t inc = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t x = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += x;
+    }
+    uint32_t wbase = 0;
+    if (lane == 63 && inc) wbase = atomicAdd(pool.ctr + kCtrS, inc);
+    wbase = (uint32_t)__shfl((int)wbase, 63, 64);
+    uint32_t at = wbase + inc - c;
+    while (d) {
+      const int i = __builtin_ctzll(d);
+      d &= d - 1;
+      const uint32_t b = base + (uint32_t)(64 * w + i) + (uint32_t)nbx * ((uint32_t)by + (uint32_t)nby * (uint32_t)bz);
+      pool.slist[at++] = b;
+      pool.sstamp[b] = epoch;
     }
   }
 }
@@ -1249,8 +1281,11 @@ hipError_t launch_bin_transpose(const Pool& pool, const uint32_t* keys, const ui
                      in, out);
   return hipGetLastError();
 }
-hipError_t launch_bin_dilate(const Pool& pool, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_dilate, dim3(persistent_grid(pool.max_items, 4, 4096)), dim3(kThreads), 0, s, pool);
+hipError_t launch_bin_dilate(const Pool& pool, uint32_t max_scenes, hipStream_t s) {
+  const uint32_t pb = 1u << (pool.key_bits / 3 - 2);
+  const uint64_t total = (uint64_t)max_scenes * pb * pb * occ_row_words(pb);
+  hipLaunchKernelGGL(k_bin_dilate, dim3((unsigned)std::min<uint64_t>((total + kThreads - 1) / kThreads, 8192)),
+                     dim3(kThreads), 0, s, pool, max_scenes);
   return hipGetLastError();
 }
 hipError_t launch_p2g_binned(const Pool& pool, const float4* P, const uint32_t* perm, int waves, hipStream_t s) {

@@ -100,6 +100,9 @@ struct SceneGeo {
   uint32_t active;        // 0: the scene's particles are dropped at the next sort
 };
 
+// 64-bit words per row of a scene's particle-block occupancy bitmap (pb blocks per axis).
+__host__ __device__ inline uint32_t occ_row_words(uint32_t pb) { return (pb + 63u) / 64u; }
+
 // Pointers and sizes fixed for the pool's lifetime, passed by value to every kernel.
 struct Pool {
   uint32_t cap;            // particle capacity of each P buffer
@@ -124,6 +127,8 @@ struct Pool {
   const float* cols;       // [scenes * kColRows * 12]
   const SceneGeo* scenes;
   uint32_t* ctr;           // [kCounters]
+  unsigned long long* occ;  // [max_scenes * pb * pb * occ_row_words(pb)] occupied particle blocks of the
+                            // binning, one bit per block: row (by, bz) of a scene, bit bx
   uint32_t* arena_ctr;     // [kArenaCtrWords]: nodes handed out per region in the current P2G, then the
                            // work counters of P2G and of G2P
 };
@@ -159,7 +164,9 @@ hipError_t launch_bin_emit(const Pool& pool, const uint32_t* keys_sorted, uint32
                            hipStream_t s);
 hipError_t launch_bin_transpose(const Pool& pool, const uint32_t* keys_sorted, const uint32_t* in, uint32_t* out,
                                 hipStream_t s);
-hipError_t launch_bin_dilate(const Pool& pool, hipStream_t s);
+// S from the occupancy bitmap: each node block of a scene's grid with an occupied particle block
+// within [-2, +1] per axis, appended to slist and stamped with the binning epoch.
+hipError_t launch_bin_dilate(const Pool& pool, uint32_t max_scenes, hipStream_t s);
 
 // P2G over the work items into tiles (+ fallback atomics on A). perm: sorted position -> particle
 // (nullptr: identity).

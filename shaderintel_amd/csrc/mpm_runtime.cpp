@@ -126,6 +126,7 @@ struct mpm_pool {
   SceneGeo* scenes_d = nullptr;
   uint32_t* ctr = nullptr;
   uint32_t* arena_ctr = nullptr;
+  unsigned long long* occ = nullptr;
   uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr}, *order = nullptr, *flags = nullptr;
   uint32_t* sort_scratch = nullptr;
   float4* staging = nullptr;
@@ -202,6 +203,7 @@ Pool view(const mpm_pool* p) {
   v.scenes = p->scenes_d;
   v.ctr = p->ctr;
   v.arena_ctr = p->arena_ctr;
+  v.occ = p->occ;
   return v;
 }
 
@@ -223,7 +225,7 @@ void free_pool(mpm_pool* p) {
   for (hipEvent_t e : p->ev_pool) hipEventDestroy(e);
   void* ptrs[] = {p->P[0],   p->P[1],     p->A,        p->V,        p->block_info, p->stamp,   p->dlist,
                   p->sstamp, p->slist,    p->map,      p->wl_start, p->wl_block,    p->tiles,   p->tile_hdr, p->tile_off,
-                  p->mats,   p->cols,     p->scenes_d, p->ctr, p->arena_ctr,      p->keys[0],     p->keys[1], p->vals[0],
+                  p->mats,   p->cols,     p->scenes_d, p->ctr, p->arena_ctr,      p->occ,      p->keys[0],     p->keys[1], p->vals[0],
                   p->vals[1], p->order,   p->flags,    p->sort_scratch, p->staging, p->export_buf};
   for (void* q : ptrs)
     if (q) hipFree(q);
@@ -232,6 +234,13 @@ void free_pool(mpm_pool* p) {
 }
 
 // Device bytes a pool of this shape allocates (used for admission against an HBM budget).
+// 64-bit words of one scene's particle-block occupancy bitmap (mps = pb^3 map entries).
+uint32_t occ_words(uint32_t mps) {
+  uint32_t pb = 1;
+  while (pb * pb * pb < mps) pb *= 2;
+  return pb * pb * occ_row_words(pb);
+}
+
 size_t pool_bytes(uint32_t cap, uint32_t max_scenes, uint32_t max_blocks, int max_dim, uint32_t* map_per_scene,
                   uint32_t* max_items, uint32_t* tile_nodes) {
   const int bits = std::max(3, ceil_log2((uint64_t)max_dim));
@@ -250,6 +259,7 @@ size_t pool_bytes(uint32_t cap, uint32_t max_scenes, uint32_t max_blocks, int ma
   b += (size_t)max_blocks * 64 * (16 + 16);              // A, V
   b += (size_t)max_blocks * 4 * 6;                       // block_info, stamp, dlist, sstamp, slist
   b += (size_t)mps * max_scenes * 4;                     // map
+  b += (size_t)occ_words(mps) * max_scenes * 8;          // occupancy bitmap
   b += (items + 1) * 4 * 2;                              // wl_start, wl_block
   b += arena * sizeof(int4) + items * 20;               // tile arena, headers, offsets
   b += (size_t)max_scenes * (kMatRows + kColRows) * 48;  // mats, cols
@@ -384,6 +394,8 @@ mpm_status do_bin(mpm_pool* p, int mode) {
   HIP_TRY(radix_sort_pairs(p->keys[0], p->vals[0], p->keys[1], p->vals[1], p->count, 6, (int)p->sort_bits,
                            p->sort_scratch, p->stream, &rk, &rv, true));
   HIP_TRY(launch_fill_u32(p->map, 0xffffffffu, p->map_per_scene * p->max_scenes, p->stream));
+  HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(p->occ), 0u, 2 * occ_words(p->map_per_scene) * p->max_scenes,
+                          p->stream));
   {  // work items: heads per tile, scanned, then numbered (the sort's scratch is free again)
     const uint32_t tiles = bin_tiles(p->count);
     uint32_t* tile_first = p->sort_scratch;
@@ -400,7 +412,7 @@ mpm_status do_bin(mpm_pool* p, int mode) {
     p->perm = p->order;
     p->reorder_in_g2p = mode == kBinReorderInG2P;
   }
-  HIP_TRY(launch_bin_dilate(pv, p->stream));
+  HIP_TRY(launch_bin_dilate(pv, p->max_scenes, p->stream));
   p->bin_valid = true;
   return MPM_OK;
 }
@@ -701,6 +713,7 @@ mpm_status create_pool(mpm_ctx* c, uint32_t cap, uint32_t max_scenes, uint32_t m
   if (e == hipSuccess) e = dmalloc(&p->sstamp, max_blocks);
   if (e == hipSuccess) e = dmalloc(&p->slist, max_blocks);
   if (e == hipSuccess) e = dmalloc(&p->map, (size_t)p->map_per_scene * max_scenes);
+  if (e == hipSuccess) e = dmalloc(&p->occ, (size_t)occ_words(p->map_per_scene) * max_scenes);
   if (e == hipSuccess) e = dmalloc(&p->wl_start, (size_t)p->max_items + 1);
   if (e == hipSuccess) e = dmalloc(&p->wl_block, p->max_items);
   if (e == hipSuccess) e = dmalloc(&p->tiles, (size_t)p->tile_nodes);
