@@ -181,7 +181,7 @@ MPM_HD void jacobi(m3& A, m3& V) {
 MPM_HD void svd3(const m3& F, m3& U, f3& s, m3& V) {
   m3 A = mul(tr(F), F);
   V = ident();
-#pragma unroll 1
+#pragma unroll  // the 12 rotations unrolled: no loop-carried register moves
   for (int sweep = 0; sweep < 4; sweep++) {
     jacobi<0, 1>(A, V);
     jacobi<0, 2>(A, V);
