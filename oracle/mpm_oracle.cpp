@@ -219,6 +219,7 @@ void svd3(const M3& F, M3& U, V3& s, M3& V) {  // linalg.js:46-69
 constexpr float FP = 4194304.0f;
 constexpr float INV_FP = 1.0f / 4194304.0f;
 inline int32_t to_fixed(float x) { return (int32_t)rintf(clampw(x * FP, -2.0e9f, 2.0e9f)); }
+inline int32_t to_fixed_scaled(float xs) { return (int32_t)rintf(clampw(xs, -2.0e9f, 2.0e9f)); }  // x * 2^22 given
 inline void bspline(V3 fx, V3 w[3]) {
   V3 t0 = sub(v3(1.5f, 1.5f, 1.5f), fx);
   V3 t1 = sub(fx, v3(1.0f, 1.0f, 1.0f));
@@ -271,15 +272,28 @@ inline void p2g_particle(const float* p, const float* mats, float vol4, AddFn&& 
   M3 C = {{{p[8], p[9], p[10]}, {p[12], p[13], p[14]}, {p[16], p[17], p[18]}}};
   M3 affine = madd(mscale(-vol4, tau), mscale(mass, C));
   V3 mv = scale(mass, v3(p[4], p[5], p[6]));
+  // mom = wt * (mv + affine * dpos), dpos = (a, b, c) - fx, with the sum evaluated as
+  // ((u + a A0) + b A1) + c A2 where u = mv - affine * fx and A0..A2 are the columns of affine (1 A = A
+  // and 2 A are exact; a term with a zero factor is left out). toFixed(wt * q) is computed as
+  // rint(clamp(wts * q)) with wts = (wx wy)(wz 2^22): the power-of-two scaling is exact, so this is
+  // toFixed's value for every normal product (a subnormal wt q rounds to 0 either way).
+  V3 u = sub(mv, mulmv(affine, fx));
+  V3 A0 = affine.c[0], A1 = affine.c[1], A2 = affine.c[2];
+  V3 Ua[3] = {u, add(u, A0), add(u, scale(2.0f, A0))};
+  float wzs[3] = {w[0].z * FP, w[1].z * FP, w[2].z * FP};
   for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++)
+    for (int b = 0; b < 3; b++) {
+      float wxy = w[a].x * w[b].y;
+      V3 Uab = b == 0 ? Ua[a] : add(Ua[a], b == 1 ? A1 : scale(2.0f, A1));
       for (int c = 0; c < 3; c++) {
-        V3 dpos = sub(v3((float)a, (float)b, (float)c), fx);
-        float wt = (w[a].x * w[b].y) * w[c].z;
-        V3 mom = scale(wt, add(mv, mulmv(affine, dpos)));
-        int32_t q[4] = {to_fixed(mom.x), to_fixed(mom.y), to_fixed(mom.z), to_fixed(wt * mass)};
+        V3 U = c == 0 ? Uab : add(Uab, c == 1 ? A2 : scale(2.0f, A2));
+        float wts = wxy * wzs[c];
+        V3 mom = scale(wts, U);
+        int32_t q[4] = {to_fixed_scaled(mom.x), to_fixed_scaled(mom.y), to_fixed_scaled(mom.z),
+                        to_fixed_scaled(wts * mass)};
         addq(base[0] + a, base[1] + b, base[2] + c, q);
       }
+    }
 }
 
 void p2g(Scene& S) {  // mpm.js:607-612 (clear + dispatch), scatter mpm.js:318-349
@@ -379,20 +393,30 @@ inline void g2p_particle(float* p, const float* mats, const int dims[3], float b
   V3 fx = sub(xp, v3((float)base[0], (float)base[1], (float)base[2]));
   V3 w[3];
   bspline(fx, w);
+  // v = sum wt gv and B = sum outer(wt gv, dpos) (mpm.js:487-496), wt = wx_a wy_b wz_c,
+  // dpos = (a, b, c) - fx, summed per (a, b) row of three nodes: S = sum_c wz_c gv,
+  // T = sum_c (wz_c dz_c) gv, then v += wxy S, B col x += (wxy dx_a) S, B col y += (wxy dy_b) S,
+  // B col z += wxy T; every multiply-add contracted (the same sums, the rounding fixed this way)
+  V3 d[3] = {sub(v3(0, 0, 0), fx), sub(v3(1, 1, 1), fx), sub(v3(2, 2, 2), fx)};
+  float wzd[3] = {w[0].z * d[0].z, w[1].z * d[1].z, w[2].z * d[2].z};
   V3 v = v3(0, 0, 0);
   M3 B = mzero();
   for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++)
-      for (int c = 0; c < 3; c++) {
-        V3 dpos = sub(v3((float)a, (float)b, (float)c), fx);
-        float wt = (w[a].x * w[b].y) * w[c].z;
-        V3 gv = gvat(base[0] + a, base[1] + b, base[2] + c);
-        // v += wt * gv and B += outer(wt * gv, dpos), every multiply-add contracted
-        v = v3(fmaf(wt, gv.x, v.x), fmaf(wt, gv.y, v.y), fmaf(wt, gv.z, v.z));
-        V3 wgv = scale(wt, gv);
-        for (int j = 0; j < 3; j++)
-          for (int r = 0; r < 3; r++) at(B, j, r) = fmaf(comp(wgv, r), comp(dpos, j), at(B, j, r));
+    for (int b = 0; b < 3; b++) {
+      V3 g[3];
+      for (int c = 0; c < 3; c++) g[c] = gvat(base[0] + a, base[1] + b, base[2] + c);
+      V3 S = scale(w[0].z, g[0]), T = scale(wzd[0], g[0]);
+      for (int c = 1; c < 3; c++) {
+        S = v3(fmaf(w[c].z, g[c].x, S.x), fmaf(w[c].z, g[c].y, S.y), fmaf(w[c].z, g[c].z, S.z));
+        T = v3(fmaf(wzd[c], g[c].x, T.x), fmaf(wzd[c], g[c].y, T.y), fmaf(wzd[c], g[c].z, T.z));
       }
+      float wxy = w[a].x * w[b].y;
+      float bxs = wxy * d[a].x, bys = wxy * d[b].y;
+      v = v3(fmaf(wxy, S.x, v.x), fmaf(wxy, S.y, v.y), fmaf(wxy, S.z, v.z));
+      B.c[0] = v3(fmaf(bxs, S.x, B.c[0].x), fmaf(bxs, S.y, B.c[0].y), fmaf(bxs, S.z, B.c[0].z));
+      B.c[1] = v3(fmaf(bys, S.x, B.c[1].x), fmaf(bys, S.y, B.c[1].y), fmaf(bys, S.z, B.c[1].z));
+      B.c[2] = v3(fmaf(wxy, T.x, B.c[2].x), fmaf(wxy, T.y, B.c[2].y), fmaf(wxy, T.z, B.c[2].z));
+    }
   M3 C = mscale(4.0f, B);
   float sp = length(v);
   if (sp > vmax) v = scale(v, vmax / sp);

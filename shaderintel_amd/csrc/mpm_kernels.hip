@@ -100,22 +100,28 @@ __device__ __forceinline__ void p2g_particle(const float4 r[kP2GStreams], const 
   const m3 C = mat(mk(r[2].x, r[2].y, r[2].z), mk(r[3].x, r[3].y, r[3].z), mk(r[4].x, r[4].y, r[4].z));
   const m3 affine = (-vol4) * tau + mass * C;
   const f3 mv = mass * mk(r[1].x, r[1].y, r[1].z);
+  // q(a, b, c) = mv + affine (dpos) with dpos = (a, b, c) - fx, evaluated as
+  // ((u + a A0) + b A1) + c A2 with u = mv - affine fx and A0..A2 the columns of affine (a A = A, 2A
+  // exactly; a term with a zero factor is left out): the same sum with its rounding fixed this way.
+  const f3 u = mv - mul(affine, fx);
+  const f3 A0 = affine.c[0], A1 = affine.c[1], A2 = affine.c[2];
+  const f3 Ua[3] = {u, u + A0, u + 2.0f * A0};
+  const float wzs[3] = {w[0].z * kFP, w[1].z * kFP, w[2].z * kFP};  // 2^22 folded in (exact)
 #pragma unroll
   for (int a = 0; a < 3; a++)
 #pragma unroll
-    for (int b = 0; b < 3; b++)
+    for (int b = 0; b < 3; b++) {
+      const float wx = a == 0 ? w[0].x : (a == 1 ? w[1].x : w[2].x);
+      const float wy = b == 0 ? w[0].y : (b == 1 ? w[1].y : w[2].y);
+      const float wxy = wx * wy;
+      const f3 Uab = b == 0 ? Ua[a] : Ua[a] + (b == 1 ? A1 : 2.0f * A1);
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const f3 dpos = mk((float)a, (float)b, (float)c) - fx;
-        const float wx = a == 0 ? w[0].x : (a == 1 ? w[1].x : w[2].x);
-        const float wy = b == 0 ? w[0].y : (b == 1 ? w[1].y : w[2].y);
-        const float wz = c == 0 ? w[0].z : (c == 1 ? w[1].z : w[2].z);
-        const float wt = (wx * wy) * wz;
-        // toFixed(wt * q) = rint(clamp(wt * q * 2^22)): the 2^22 scaling is exact, so folding it into the
-        // weight (wts = wt * 2^22) gives the same float for every q whose product is normal, and one that
-        // rounds to the same integer (0) when wt * q is subnormal; overflow clamps alike.
-        const float wts = wt * kFP;
-        const f3 mom = wts * (mv + mul(affine, dpos));
+        const f3 U = c == 0 ? Uab : Uab + (c == 1 ? A2 : 2.0f * A2);
+        // wts = wt * 2^22 with wt = (wx wy) wz: toFixed(wt q) = rint(clamp(wts q)) since the power-of-
+        // two scaling is exact (a subnormal wt q rounds to 0 either way; overflow clamps alike)
+        const float wts = wxy * wzs[c];
+        const f3 mom = wts * U;
         if (CLAMP)
           add(bx + a, by + b, bz + c, fixed_of_scaled(mom.x), fixed_of_scaled(mom.y), fixed_of_scaled(mom.z),
               fixed_of_scaled(wts * mass));
@@ -123,6 +129,7 @@ __device__ __forceinline__ void p2g_particle(const float4 r[kP2GStreams], const 
           add(bx + a, by + b, bz + c, (int32_t)rintf(mom.x), (int32_t)rintf(mom.y), (int32_t)rintf(mom.z),
               (int32_t)rintf(wts * mass));
       }
+    }
 }
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -219,29 +226,44 @@ __device__ __forceinline__ void g2p_load(const float4* src, uint32_t si, uint32_
   in.fx = in.xp - mk((float)in.bx, (float)in.by, (float)in.bz);
 }
 
+// v = sum wt gv and B = sum outer(wt gv, dpos) over the 27 nodes (mpm.js:487-496) with
+// wt = wx_a wy_b wz_c and dpos = (a, b, c) - fx, summed per (a, b) row of three nodes:
+// S = sum_c wz_c gv and T = sum_c (wz_c dz_c) gv, then v += wxy S, B col x += (wxy dx_a) S,
+// B col y += (wxy dy_b) S, B col z += wxy T (wxy = wx_a wy_b); every multiply-add contracted. The
+// same sums as node by node, with the rounding fixed this way.
 template <class Gather>
 __device__ __forceinline__ void g2p_gather(const G2PIn& in, f3& v, m3& B, Gather&& gather) {
   const f3 fx = in.fx;
   f3 w[3];
   bspline(fx, w);
+  const f3 d[3] = {mk(0.0f, 0.0f, 0.0f) - fx, mk(1.0f, 1.0f, 1.0f) - fx, mk(2.0f, 2.0f, 2.0f) - fx};
+  const float wzd[3] = {w[0].z * d[0].z, w[1].z * d[1].z, w[2].z * d[2].z};
   v = mk(0, 0, 0);
   B = zero3();
 #pragma unroll
   for (int a = 0; a < 3; a++)
 #pragma unroll
-    for (int b = 0; b < 3; b++)
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        const f3 dpos = mk((float)a, (float)b, (float)c) - fx;
-        const float wx = a == 0 ? w[0].x : (a == 1 ? w[1].x : w[2].x);
-        const float wy = b == 0 ? w[0].y : (b == 1 ? w[1].y : w[2].y);
-        const float wz = c == 0 ? w[0].z : (c == 1 ? w[1].z : w[2].z);
-        const float wt = (wx * wy) * wz;
-        const float4 g4 = gather(in.bx, in.by, in.bz, a, b, c);
-        const f3 gv = mk(g4.x, g4.y, g4.z);
-        v = mk(fmaf(wt, gv.x, v.x), fmaf(wt, gv.y, v.y), fmaf(wt, gv.z, v.z));  // v += wt * gv, contracted
-        B = add_outer(B, wt * gv, dpos);
-      }
+    for (int b = 0; b < 3; b++) {
+      const float wx = a == 0 ? w[0].x : (a == 1 ? w[1].x : w[2].x);
+      const float wy = b == 0 ? w[0].y : (b == 1 ? w[1].y : w[2].y);
+      const float dxa = a == 0 ? d[0].x : (a == 1 ? d[1].x : d[2].x);
+      const float dyb = b == 0 ? d[0].y : (b == 1 ? d[1].y : d[2].y);
+      const float4 g0 = gather(in.bx, in.by, in.bz, a, b, 0);
+      const float4 g1 = gather(in.bx, in.by, in.bz, a, b, 1);
+      const float4 g2 = gather(in.bx, in.by, in.bz, a, b, 2);
+      f3 S = mk(w[0].z * g0.x, w[0].z * g0.y, w[0].z * g0.z);
+      S = mk(fmaf(w[1].z, g1.x, S.x), fmaf(w[1].z, g1.y, S.y), fmaf(w[1].z, g1.z, S.z));
+      S = mk(fmaf(w[2].z, g2.x, S.x), fmaf(w[2].z, g2.y, S.y), fmaf(w[2].z, g2.z, S.z));
+      f3 T = mk(wzd[0] * g0.x, wzd[0] * g0.y, wzd[0] * g0.z);
+      T = mk(fmaf(wzd[1], g1.x, T.x), fmaf(wzd[1], g1.y, T.y), fmaf(wzd[1], g1.z, T.z));
+      T = mk(fmaf(wzd[2], g2.x, T.x), fmaf(wzd[2], g2.y, T.y), fmaf(wzd[2], g2.z, T.z));
+      const float wxy = wx * wy;
+      const float bxs = wxy * dxa, bys = wxy * dyb;
+      v = mk(fmaf(wxy, S.x, v.x), fmaf(wxy, S.y, v.y), fmaf(wxy, S.z, v.z));
+      B.c[0] = mk(fmaf(bxs, S.x, B.c[0].x), fmaf(bxs, S.y, B.c[0].y), fmaf(bxs, S.z, B.c[0].z));
+      B.c[1] = mk(fmaf(bys, S.x, B.c[1].x), fmaf(bys, S.y, B.c[1].y), fmaf(bys, S.z, B.c[1].z));
+      B.c[2] = mk(fmaf(wxy, T.x, B.c[2].x), fmaf(wxy, T.y, B.c[2].y), fmaf(wxy, T.z, B.c[2].z));
+    }
 }
 
 __device__ __forceinline__ void g2p_finish(const float* __restrict__ mats, const SceneGeo& geo, const float4* src,
