@@ -43,9 +43,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MPM particle-substeps/s at 8M particles (1 GPU); batched-serve p50/p99 step latency"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per particle for each phase: P2G reads the 128-byte record; G2P reads and
-# writes it (256 bytes). Grid and sort traffic are not credited.
-ALGO_BYTES = {"mpm p2g": 128, "mpm g2p": 256}
+# Algorithmic bytes per particle for each kernel: P2G reads the 128-byte record; G2P reads and
+# writes it (256 bytes); the fused kernel does one G2P and the next substep's P2G (384 bytes, the
+# whole particle-substep). Grid and sort traffic are not credited.
+ALGO_BYTES = {"mpm p2g": 128, "mpm g2p": 256, "mpm g2p_p2g": 384}
 
 
 def log(msg: str) -> None:
@@ -366,7 +367,8 @@ def main() -> int:
     run(args.steps)
     sim.sync()
     dt = time.perf_counter() - t0
-    g2p_ms = sim.phase_times()["mpm g2p"]
+    ktimes = sim.kernel_times()  # {"g2p": (ms, launches), "g2p_p2g": (ms, launches)}
+    g2p_ms = ktimes["g2p"][0] + ktimes["g2p_p2g"][0]
     sim.set_profiling(False)
     # Phase breakdown: the next args.steps substeps with events between the phases (those events
     # add packets to the stream, so this pass is reported beside the throughput, not as it).
@@ -391,12 +393,17 @@ def main() -> int:
     sorts = sum(1 for s in range(args.warmup + args.steps, args.warmup + 2 * args.steps) if s % 4 == 0)
     per_launch["mpm sort"] = phases["mpm sort"] / max(sorts, 1)
     phase_breakdown = {k.replace("mpm ", ""): round(v, 4) for k, v in per_launch.items()}
-    per_launch["mpm g2p"] = g2p_ms / args.steps  # from the timed region itself
-    dom = max(ALGO_BYTES, key=lambda k: per_launch[k])
-    achieved = ALGO_BYTES[dom] * n / (per_launch[dom] * 1e-3) / 1e9
+    # average launch durations of the timed region itself (dispatch-recorded events)
+    launch_ms = {"mpm " + k: v[0] / v[1] for k, v in ktimes.items() if v[1]}
+    launch_ms.setdefault("mpm p2g", per_launch["mpm p2g"])
+    # the dominant kernel: the fused G2P + P2G when the substeps were fused, else G2P
+    dom = "mpm g2p_p2g" if "mpm g2p_p2g" in launch_ms else "mpm g2p"
+    per_launch[dom] = launch_ms[dom]
+    achieved = ALGO_BYTES[dom] * n / (launch_ms[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom.replace(" ", "_"),
                 "algo_bytes_per_launch": ALGO_BYTES[dom] * n,
+                "launches": {k: {"ms_total": round(v[0], 3), "launches": v[1]} for k, v in ktimes.items()},
                 "avg_launch_ms": round(per_launch[dom], 4)}
     # HBM bytes per launch of the dominant kernel, from the committed rocprofv3 counter passes
     # (FETCH_SIZE / WRITE_SIZE scaled by a copy kernel of known size) on the same workload

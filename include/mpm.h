@@ -155,6 +155,10 @@ mpm_status mpm_sync(mpm_sim* sim);
  * the stream, so the substep runs as it does unprofiled); 0: off. */
 mpm_status mpm_set_profiling(mpm_sim* sim, int32_t enable);
 mpm_status mpm_phase_times(mpm_sim* sim, float ms[4]);
+/* Profiling mode 2 (no counterpart in the reference): milliseconds and launch counts of the G2P
+ * launches since profiling was enabled, split into {G2P alone, G2P fused with the next substep's
+ * P2G}. */
+mpm_status mpm_kernel_times(mpm_sim* sim, float ms[2], uint32_t launches[2]);
 
 /* Tuning knobs (no counterpart in the reference; results do not depend on them — every variant
  * computes the same integer grid sums and the same per-particle float sequence). */

@@ -165,6 +165,9 @@ struct mpm_pool {
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   float phase_ms[4] = {0, 0, 0, 0};
+  std::vector<uint8_t> ev_fused;    // profiling 2: per (start, end) pair, 1 for a fused G2P + P2G launch
+  float kernel_ms[2] = {0, 0};      // profiling 2: summed G2P launches {G2P alone, fused G2P + P2G}
+  uint32_t kernel_launches[2] = {0, 0};
 };
 
 struct mpm_sim {
@@ -514,6 +517,7 @@ mpm_status do_g2p(mpm_pool* p, float vmax, bool fuse_next, bool keys_next) {
   }
   const bool items = p->g2p_items && p->tiles_fresh;
   p->tiles_fresh = false;  // one G2P per P2G: the G2P work counters are those begin_p2g cleared
+  if (e0) p->ev_fused.push_back(items && fuse_next && p->fuse ? 1 : 0);
   if (items && fuse_next && p->fuse) {
     ST_TRY(clean_a(p));
     HIP_TRY(launch_begin_p2g(view(p), p->stream));  // the next P2G's epoch, arena and work counters
@@ -1264,7 +1268,39 @@ mpm_status mpm_set_profiling(mpm_sim* s, int32_t enable) {
   if (enable < 0 || enable > 2) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_profiling: mode %d not in 0..2", enable);
   p->profiling = enable;
   p->ev_used = 0;
+  p->ev_fused.clear();
   for (float& m : p->phase_ms) m = 0.0f;
+  p->kernel_ms[0] = p->kernel_ms[1] = 0.0f;
+  p->kernel_launches[0] = p->kernel_launches[1] = 0;
+  return MPM_OK;
+}
+
+// Profiling 2: fold the recorded (start, end) pairs of the G2P launches into kernel_ms / phase_ms.
+mpm_status collect_kernel_events(mpm_pool* p) {
+  const size_t pairs = p->ev_used / 2;
+  if (pairs) HIP_TRY(hipEventSynchronize(p->ev_pool[pairs * 2 - 1]));
+  for (size_t g = 0; g < pairs; g++) {
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, p->ev_pool[2 * g], p->ev_pool[2 * g + 1]));
+    const int f = g < p->ev_fused.size() ? p->ev_fused[g] : 0;
+    p->kernel_ms[f] += t;
+    p->kernel_launches[f]++;
+    p->phase_ms[3] += t;
+  }
+  p->ev_used = 0;
+  p->ev_fused.clear();
+  return MPM_OK;
+}
+
+mpm_status mpm_kernel_times(mpm_sim* s, float ms[2], uint32_t launches[2]) {
+  if (!s || !ms || !launches) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_kernel_times: NULL argument");
+  mpm_pool* p = s->pool;
+  ST_TRY(set_dev(p->ctx));
+  if (p->profiling == 2) ST_TRY(collect_kernel_events(p));
+  for (int k = 0; k < 2; k++) {
+    ms[k] = p->kernel_ms[k];
+    launches[k] = p->kernel_launches[k];
+  }
   return MPM_OK;
 }
 
@@ -1273,13 +1309,7 @@ mpm_status mpm_phase_times(mpm_sim* s, float ms[4]) {
   mpm_pool* p = s->pool;
   ST_TRY(set_dev(p->ctx));
   if (p->profiling == 2) {  // (start, end) of each G2P launch
-    const size_t pairs = p->ev_used / 2;
-    if (pairs) HIP_TRY(hipEventSynchronize(p->ev_pool[pairs * 2 - 1]));
-    for (size_t g = 0; g < pairs; g++) {
-      float t = 0;
-      HIP_TRY(hipEventElapsedTime(&t, p->ev_pool[2 * g], p->ev_pool[2 * g + 1]));
-      p->phase_ms[3] += t;
-    }
+    ST_TRY(collect_kernel_events(p));
   } else {
     const size_t groups = p->ev_used / 5;
     if (groups) HIP_TRY(hipEventSynchronize(p->ev_pool[groups * 5 - 1]));

@@ -141,6 +141,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "mpm_read_grid": (i32, [vp, vp]),
         "mpm_sync": (i32, [vp]),
         "mpm_set_profiling": (i32, [vp, i32]),
+        "mpm_kernel_times": (i32, [vp, vp, vp]),
         "mpm_set_tuning": (i32, [vp, i32, i32]),
         "mpm_phase_times": (i32, [vp, vp]),
         "mpm_scan_u32": (i32, [vp, vp, vp, u64]),
@@ -585,6 +586,13 @@ class MPM:
         """True / 1: events between the phases of every substep; 2: G2P launches only, timed by
         their own dispatch (no extra packets on the stream); False / 0: off."""
         _check(load_library().mpm_set_profiling(self._h, int(on)))
+
+    def kernel_times(self) -> dict:
+        """Profiling mode 2: {"g2p": (ms, launches), "g2p_p2g": (ms, launches)} (G2P alone, fused)."""
+        ms = np.zeros(2, np.float32)
+        n = np.zeros(2, np.uint32)
+        _check(load_library().mpm_kernel_times(self._h, _ptr(ms), _ptr(n)), "MPM.kernel_times")
+        return {"g2p": (float(ms[0]), int(n[0])), "g2p_p2g": (float(ms[1]), int(n[1]))}
 
     def phase_times(self) -> dict:
         ms = np.zeros(4, np.float32)
