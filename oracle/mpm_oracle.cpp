@@ -13,7 +13,8 @@
 // B += outer(wt*gv, dpos) contracted element by element, and no other contraction (explicit fmaf
 // calls; the compiler contracts nothing: -ffp-contract=off), IEEE division and sqrt,
 // WGSL clamp(e,lo,hi) = min(max(e,lo),hi) with NaN-discarding min/max, WGSL round = rint (half to
-// even). inverseSqrt(x) is read as 1/sqrt(x); log/exp are the self-contained routines below.
+// even). inverseSqrt is rsqrt_1to2 (it is used on [1, 2] only); log/exp are the self-contained
+// routines below.
 //
 // Every function here has a pinning test under tests/ (closed forms, invariants, brute force or
 // numpy references); tests/test_oracle_*.py name the function each test pins.
@@ -148,6 +149,23 @@ inline M3 outer3(V3 a, V3 b) { return {{scale(a, b.x), scale(a, b.y), scale(a, b
 // det(M) = dot(M[0], cross(M[1], M[2]))
 inline float determinant(const M3& m) { return dot(m.c[0], cross(m.c[1], m.c[2])); }
 
+// WGSL inverseSqrt(x) for x in [1, 2] (the only place the path uses it: the Jacobi rotation's
+// t^2 + 1, |t| <= 1). WGSL gives inverseSqrt an accuracy of 2 ulp and no exact result, so this is
+// one faithful reading: a quadratic approximation of x^-1/2 on [1, 2] (coefficients fitted to its
+// relative error, 3.2e-3 at most) refined by two Newton steps y += y (1/2 - (x/2) y^2), each a
+// fused multiply-add. Within 1 ulp of 1/sqrt(x) for every float in [1, 2]
+// (tests/test_oracle_math.py checks all 2^23 + 1 of them).
+float rsqrt_1to2(float x) {
+  float y = fmaf(fmaf(0.14768398f, x, -0.7304885f), x, 1.5796084f);
+  const float hx = 0.5f * x;
+  for (int k = 0; k < 2; k++) {
+    const float t = y * y;
+    const float e = fmaf(-hx, t, 0.5f);
+    y = fmaf(y, e, y);
+  }
+  return y;
+}
+
 // ---- linalg (src/shaders/linalg.js:14-69) ----
 // One cyclic-Jacobi rotation (linalg.js:14-25): A <- J^T A J, V <- V J with J the Givens rotation
 // in the (p, q) plane. Only rows / columns p and q change; each changed entry is the two-term sum
@@ -159,7 +177,7 @@ void jacobi_rotate(M3& A, M3& V, int p, int q) {
   float theta = (at(A, q, q) - at(A, p, p)) / (2.0f * apq);
   float sgn = theta >= 0.0f ? 1.0f : -1.0f;
   float t = sgn / (fabsf(theta) + sqrtf(theta * theta + 1.0f));
-  float c = 1.0f / sqrtf(t * t + 1.0f);
+  float c = rsqrt_1to2(t * t + 1.0f);  // inverseSqrt (linalg.js:20); |t| <= 1, so t^2 + 1 is in [1, 2]
   float s = t * c;
   float ms = -s;
   // M = J^T A: rows p and q of every column
@@ -560,6 +578,9 @@ float or_expf(float x) { return si_expf(x); }
 float or_logf(float x) { return si_logf(x); }
 uint32_t or_morton3(uint32_t x, uint32_t y, uint32_t z) { return morton3(x, y, z); }
 int32_t or_to_fixed(float x) { return to_fixed(x); }
+void or_rsqrt_1to2(const float* x, float* y, uint64_t n) {
+  for (uint64_t i = 0; i < n; i++) y[i] = rsqrt_1to2(x[i]);
+}
 void or_bspline(const float fx[3], float w[9]) {
   V3 ww[3];
   bspline(v3(fx[0], fx[1], fx[2]), ww);

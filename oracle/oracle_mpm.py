@@ -33,7 +33,7 @@ def lib() -> ctypes.CDLL:
                                         ctypes.c_float, ctypes.c_double, ctypes.c_int64)
     sig = {
         "or_expf": (f32, [f32]), "or_logf": (f32, [f32]), "or_morton3": (u32, [u32, u32, u32]),
-        "or_to_fixed": (i32, [f32]), "or_bspline": (None, [vp, vp]), "or_svd3": (None, [vp, vp, vp, vp]),
+        "or_to_fixed": (i32, [f32]), "or_rsqrt_1to2": (None, [vp, vp, u64]), "or_bspline": (None, [vp, vp]), "or_svd3": (None, [vp, vp, vp, vp]),
         "or_scan_u32": (None, [vp, vp, u64]), "or_counting_sort": (ctypes.c_int, [vp, u64, u32, vp, vp]),
         "or_material_row": (None, [ctypes.c_int, f64, f64, f64, f64, f64, f64, f64, f64, ctypes.c_int, vp]),
         "or_sample_shape": (i64, [ctypes.c_int, vp, f64, f64, u32, vp, i64]),
@@ -67,6 +67,14 @@ def expf(x: float) -> float:
 
 def logf(x: float) -> float:
     return lib().or_logf(x)
+
+
+def rsqrt_1to2(x: np.ndarray) -> np.ndarray:
+    """The oracle's inverseSqrt on [1, 2], elementwise (float32)."""
+    x = np.ascontiguousarray(np.asarray(x, np.float32))
+    y = np.empty_like(x)
+    lib().or_rsqrt_1to2(_p(x), _p(y), x.size)
+    return y
 
 
 def svd3(F: np.ndarray):

@@ -7,9 +7,9 @@
 // additions fused, cross products as fma(a, b, -(c*d)), and G2P's v += wt*gv and
 // B += outer(wt*gv, dpos) element by element. IEEE division and square root (HIP's default correctly
 // rounded fp32 div/sqrt), WGSL clamp(e, lo, hi) = min(max(e, lo), hi), WGSL round =
-// round-half-to-even (rint), and
-// inverseSqrt read as 1/sqrt. exp and log are the self-contained Cody-Waite routines below, so
-// no vendor math library decides any bit of the result.
+// round-half-to-even (rint). inverseSqrt (used on [1, 2] only) is rsqrt_1to2 below, and exp and
+// log are the self-contained Cody-Waite routines below, so no vendor math library decides any bit
+// of the result.
 //
 // The WGSL being restated is src/physics/mpm.js:64-78 (module `mpm`) and
 // src/shaders/linalg.js:8-69 (module `linalg`).
@@ -144,6 +144,22 @@ MPM_HD uint32_t spread3(uint32_t x) {  // 10-bit interleave helper of morton3
 }
 MPM_HD uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) { return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2); }
 
+// WGSL inverseSqrt(x) for x in [1, 2] (the Jacobi rotation's t^2 + 1 with |t| <= 1): a quadratic
+// approximation of x^-1/2 on [1, 2] refined by two Newton steps y += y (1/2 - (x/2) y^2), every
+// step a fused multiply-add. Within 1 ulp of 1/sqrt(x) over [1, 2] (WGSL allows 2 ulp); NaN in,
+// NaN out.
+MPM_HD float rsqrt_1to2(float x) {
+  float y = fmaf(fmaf(0.14768398f, x, -0.7304885f), x, 1.5796084f);
+  const float hx = 0.5f * x;
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const float t = y * y;
+    const float e = fmaf(-hx, t, 0.5f);
+    y = fmaf(y, e, y);
+  }
+  return y;
+}
+
 // ---- 3x3 symmetric eigen-decomposition and SVD (cyclic Jacobi, 4 sweeps) ----
 MPM_HD float& el(m3& m, int c, int r) { return r == 0 ? m.c[c].x : (r == 1 ? m.c[c].y : m.c[c].z); }
 MPM_HD float el(const m3& m, int c, int r) { return r == 0 ? m.c[c].x : (r == 1 ? m.c[c].y : m.c[c].z); }
@@ -158,7 +174,7 @@ MPM_HD void jacobi(m3& A, m3& V) {
   float theta = (el(A, Q, Q) - el(A, P, P)) / (2.0f * apq);
   float sgn = theta >= 0.0f ? 1.0f : -1.0f;
   float t = sgn / (fabsf(theta) + sqrtf(theta * theta + 1.0f));
-  float c = 1.0f / sqrtf(t * t + 1.0f);
+  float c = rsqrt_1to2(t * t + 1.0f);  // inverseSqrt (linalg.js:20); |t| <= 1
   float s = t * c;
   float ms = -s;
 #pragma unroll
