@@ -827,17 +827,68 @@ __device__ __forceinline__ void fallback_particle(const Pool& pool, const float4
   });
 }
 
+// The end of an item's P2G (after a barrier that follows the bbox atomics): the LDS window tile
+// (S^3 nodes from the lowest in-window base (wx, wy, wz), 4 planes) is stored over the bounding box
+// bbox of the in-window stencil bases (+ 2 nodes) in arena space, recorded in tile_off / tile_hdr; when
+// the arena has no room, the box's non-zero nodes are added to A with atomics instead (and their
+// node blocks stamped), which gives the same sums. Every thread of the workgroup calls it.
+__device__ __forceinline__ void store_item_tile(const Pool& pool, uint32_t k, const SceneGeo& geo, const int* tile,
+                                                const int* bbox, int wx, int wy, int wz, uint32_t epoch,
+                                                uint32_t bin_epoch, uint32_t& s_off) {
+  constexpr int S = kTileSpan;
+  const int tid = threadIdx.x;
+  const bool any = bbox[0] != INT_MAX;
+  const int px = bbox[0], py = bbox[1], pz = bbox[2];
+  const int fx = any ? bbox[3] - px + 3 : 0, fy = any ? bbox[4] - py + 3 : 0, fz = any ? bbox[5] - pz + 3 : 0;
+  const int pnodes = fx * fy * fz;  // <= kTileNodes: inside the window tile
+  if (tid == 0) {  // arena space for the tile, from this workgroup's region
+    uint32_t off = 0xffffffffu;
+    const uint32_t region = blockIdx.x % kArenaRegions, rsize = pool.tile_nodes / kArenaRegions;
+    uint32_t* rc = pool.arena_ctr + region * kArenaStride;
+    // the check keeps a full region's counter from running far past its end
+    if (pnodes > 0 && __hip_atomic_load(rc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < rsize) {
+      const uint32_t o = atomicAdd(rc, (uint32_t)pnodes);
+      if (o <= rsize - (uint32_t)pnodes) off = region * rsize + o;
+    }
+    s_off = off;
+    pool.tile_off[k] = off;
+    if (off != 0xffffffffu) pool.tile_hdr[k] = make_int4(px, py, pz, fx | (fy << 8) | (fz << 16));
+  }
+  __syncthreads();
+  const uint32_t toff = s_off;
+  for (int t = tid; t < pnodes; t += kThreads) {
+    const int lx = t % fx, ly = (t / fx) % fy, lz = t / (fx * fy);
+    const int w = ((pz + lz - wz) * S + (py + ly - wy)) * S + (px + lx - wx);
+    const int4 q = make_int4(tile[w], tile[kTileNodes + w], tile[2 * kTileNodes + w], tile[3 * kTileNodes + w]);
+    if (toff != 0xffffffffu) {
+      pool.tiles[toff + (uint32_t)t] = q;
+    } else if (q.x | q.y | q.z | q.w) {  // no arena room: add the node to A
+      const int gx = px + lx, gy = py + ly, gz = pz + lz;
+      stamp_fallback(pool, geo.block_base + (uint32_t)(gx >> 2) + (uint32_t)geo.nbx * ((uint32_t)(gy >> 2) +
+                               (uint32_t)geo.nby * (uint32_t)(gz >> 2)), epoch, true, bin_epoch);
+      int* g = reinterpret_cast<int*>(pool.A + node_addr(gx, gy, gz, geo));
+      atomicAdd(g, q.x);
+      atomicAdd(g + 1, q.y);
+      atomicAdd(g + 2, q.z);
+      atomicAdd(g + 3, q.w);
+    }
+  }
+}
+
 // P2G over the binning's work items (mpm.js:306-401 restated for aligned particle blocks). One
-// 256-thread workgroup per item at a time, items handed out by a device counter. The LDS tile
-// covers the bounding box of the stencils of the item's particles whose base lies within kDrift
-// cells of the block. Inside a block the sort orders particles by (rank in cell, cell), so the 64
-// lanes of one instruction read consecutive records (coalesced) that sit in different cells (their
-// LDS atomics hit different nodes).
+// 256-thread workgroup per item at a time, items from per-XCD counters. Each particle whose base
+// lies within kDrift cells of the block scatters its 27x4 fixed-point contributions with LDS
+// integer atomics into a tile covering the item's whole window (12^3 nodes); the tile is then
+// stored with plain stores over the bounding box of those stencils, in space taken from the
+// arena. The others, and the whole tile when the arena has no room, go to A with global atomics.
+// Inside a block the sort orders particles by (rank in cell, cell), so the 64 lanes of one
+// instruction read consecutive records (coalesced) that sit in different cells (their LDS atomics
+// hit different nodes).
 template <int WAVES>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_p2g_binned(
     Pool pool, const float4* __restrict__ P, const uint32_t* __restrict__ perm) {
   __shared__ int tile[4 * kTileNodes];
-  __shared__ int box[6];
+  __shared__ int bbox[6];
   __shared__ uint32_t s_item, s_off;
   const int tid = threadIdx.x;
   const uint32_t items = pool.ctr[kCtrItems];
@@ -845,6 +896,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   const uint32_t mshift = pool.key_bits - 6;
   uint32_t* const wctr = pool.arena_ctr + kArenaRegions * kArenaStride;
   uint32_t range = blockIdx.x % kWorkSplits, spent = 0;  // ranges found used up (tid 0 only)
+  constexpr int S = kTileSpan;  // window tile: S^3 nodes from the lowest in-window base
   for (;;) {
     __syncthreads();  // the previous item is finished with the tile and with s_item
     if (tid == 0) s_item = next_item(wctr, items, range, spent);
@@ -852,58 +904,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
     const uint32_t k = s_item;
     if (k >= items) break;
     const uint32_t start = pool.wl_start[k], cnt = pool.wl_start[k + 1] - start;
-    const uint32_t mb = pool.wl_block[k] & ((1u << mshift) - 1u);
-    const float vol4 = pool.scenes[pool.wl_block[k] >> mshift].vol4;
+    const uint32_t wb = pool.wl_block[k];
+    const SceneGeo geo = uniform_geo(pool, wb >> mshift);
+    const uint32_t mb = wb & ((1u << mshift) - 1u);
     const int wx = 4 * (int)compact3(mb) - kDrift, wy = 4 * (int)compact3(mb >> 1) - kDrift,
               wz = 4 * (int)compact3(mb >> 2) - kDrift;  // lowest in-window base
-    if (tid < 3) { box[tid] = INT_MAX; box[3 + tid] = INT_MIN; }
+    for (int t = tid; t < kTileNodes; t += kThreads)
+      reinterpret_cast<int4*>(tile)[t] = make_int4(0, 0, 0, 0);  // the 4 planes, kTileNodes int4 in all
+    if (tid < 3) { bbox[tid] = INT_MAX; bbox[3 + tid] = INT_MIN; }
     __syncthreads();
     int lo0 = INT_MAX, lo1 = INT_MAX, lo2 = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hi2 = INT_MIN;
-    for (uint32_t v = tid; v < cnt; v += kThreads) {
-      const uint32_t j = start + v;
-      const float4 x = P[perm ? perm[j] : j];
-      const int b0 = (int)floorf(x.x - 0.5f), b1 = (int)floorf(x.y - 0.5f), b2 = (int)floorf(x.z - 0.5f);
-      if ((unsigned)(b0 - wx) < (unsigned)(4 + 2 * kDrift) && (unsigned)(b1 - wy) < (unsigned)(4 + 2 * kDrift) &&
-          (unsigned)(b2 - wz) < (unsigned)(4 + 2 * kDrift)) {
-        lo0 = min(lo0, b0); lo1 = min(lo1, b1); lo2 = min(lo2, b2);
-        hi0 = max(hi0, b0); hi1 = max(hi1, b1); hi2 = max(hi2, b2);
-      }
-    }
-    lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
-    hi0 = wave_max(hi0); hi1 = wave_max(hi1); hi2 = wave_max(hi2);
-    if ((tid & 63) == 0) {
-      atomicMin(&box[0], lo0); atomicMin(&box[1], lo1); atomicMin(&box[2], lo2);
-      atomicMax(&box[3], hi0); atomicMax(&box[4], hi1); atomicMax(&box[5], hi2);
-    }
-    __syncthreads();
-    const bool any = box[0] != INT_MAX;
-    const int ox = box[0], oy = box[1], oz = box[2];
-    const int ex = any ? box[3] - ox + 3 : 0, ey = any ? box[4] - oy + 3 : 0, ez = any ? box[5] - oz + 3 : 0;
-    const int nodes = ex * ey * ez;  // <= kTileNodes by the window test
-    if (tid == 0) {  // arena space for the tile, from this workgroup's region
-      uint32_t off = 0xffffffffu;
-      const uint32_t region = blockIdx.x % kArenaRegions, rsize = pool.tile_nodes / kArenaRegions;
-      uint32_t* rc = pool.arena_ctr + region * kArenaStride;
-      // the check keeps a full region's counter from running far past its end
-      if (nodes > 0 && __hip_atomic_load(rc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < rsize) {
-        const uint32_t o = atomicAdd(rc, (uint32_t)nodes);
-        if (o <= rsize - (uint32_t)nodes) off = region * rsize + o;
-      }
-      s_off = off;
-      pool.tile_off[k] = off;
-      if (off != 0xffffffffu) pool.tile_hdr[k] = make_int4(ox, oy, oz, ex | (ey << 8) | (ez << 16));
-    }
-    __syncthreads();
-    const uint32_t toff = s_off;
-    const bool has_slot = toff != 0xffffffffu;
-    if (has_slot)
-      for (int t = tid; t < nodes; t += kThreads) {
-        tile[t] = 0;
-        tile[kTileNodes + t] = 0;
-        tile[2 * kTileNodes + t] = 0;
-        tile[3 * kTileNodes + t] = 0;
-      }
-    __syncthreads();
     for (uint32_t li = tid; li < cnt; li += kThreads) {
       const uint32_t j = start + li;
       float4 r[kP2GStreams];
@@ -911,26 +921,29 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
       const int b0 = (int)floorf(r[0].x - 0.5f), b1 = (int)floorf(r[0].y - 0.5f), b2 = (int)floorf(r[0].z - 0.5f);
       const bool inwin = (unsigned)(b0 - wx) < (unsigned)(4 + 2 * kDrift) &&
                          (unsigned)(b1 - wy) < (unsigned)(4 + 2 * kDrift) && (unsigned)(b2 - wz) < (unsigned)(4 + 2 * kDrift);
-      if (inwin && has_slot) {
-        const int tb = ((b2 - oz) * ey + (b1 - oy)) * ex + (b0 - ox);
-        auto to_tile = [&](int gx, int gy, int gz, int q0, int q1, int q2, int q3) {
-          const int t = tb + ((gz - b2) * ey + (gy - b1)) * ex + (gx - b0);
+      if (inwin) {
+        lo0 = min(lo0, b0); lo1 = min(lo1, b1); lo2 = min(lo2, b2);
+        hi0 = max(hi0, b0); hi1 = max(hi1, b1); hi2 = max(hi2, b2);
+        const int tb = ((b2 - wz) * S + (b1 - wy)) * S + (b0 - wx);
+        p2g_particle(r, pool.mats, geo.vol4, [&](int gx, int gy, int gz, int q0, int q1, int q2, int q3) {
+          const int t = tb + ((gz - b2) * S + (gy - b1)) * S + (gx - b0);
           atomicAdd(&tile[t], q0);
           atomicAdd(&tile[kTileNodes + t], q1);
           atomicAdd(&tile[2 * kTileNodes + t], q2);
           atomicAdd(&tile[3 * kTileNodes + t], q3);
-        };
-        p2g_particle(r, pool.mats, vol4, to_tile);
+        });
       } else {
         fallback_particle(pool, r, epoch, true, bin_epoch);
       }
     }
-    __syncthreads();
-    if (has_slot) {
-      int4* out = pool.tiles + toff;
-      for (int t = tid; t < nodes; t += kThreads)
-        out[t] = make_int4(tile[t], tile[kTileNodes + t], tile[2 * kTileNodes + t], tile[3 * kTileNodes + t]);
+    lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
+    hi0 = wave_max(hi0); hi1 = wave_max(hi1); hi2 = wave_max(hi2);
+    if ((tid & 63) == 0) {
+      atomicMin(&bbox[0], lo0); atomicMin(&bbox[1], lo1); atomicMin(&bbox[2], lo2);
+      atomicMax(&bbox[3], hi0); atomicMax(&bbox[4], hi1); atomicMax(&bbox[5], hi2);
     }
+    __syncthreads();
+    store_item_tile(pool, k, geo, tile, bbox, wx, wy, wz, epoch, bin_epoch, s_off);
   }
 }
 
@@ -1040,41 +1053,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
       atomicMax(&bbox[3], hi0); atomicMax(&bbox[4], hi1); atomicMax(&bbox[5], hi2);
     }
     __syncthreads();
-    const bool any = bbox[0] != INT_MAX;
-    const int px = bbox[0], py = bbox[1], pz = bbox[2];
-    const int fx = any ? bbox[3] - px + 3 : 0, fy = any ? bbox[4] - py + 3 : 0, fz = any ? bbox[5] - pz + 3 : 0;
-    const int pnodes = fx * fy * fz;  // <= kTileNodes: inside the window tile
-    if (tid == 0) {  // arena space for the tile, from this workgroup's region
-      uint32_t off = 0xffffffffu;
-      const uint32_t region = blockIdx.x % kArenaRegions, rsize = pool.tile_nodes / kArenaRegions;
-      uint32_t* rc = pool.arena_ctr + region * kArenaStride;
-      if (pnodes > 0 && __hip_atomic_load(rc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < rsize) {
-        const uint32_t o = atomicAdd(rc, (uint32_t)pnodes);
-        if (o <= rsize - (uint32_t)pnodes) off = region * rsize + o;
-      }
-      s_off = off;
-      pool.tile_off[k] = off;
-      if (off != 0xffffffffu) pool.tile_hdr[k] = make_int4(px, py, pz, fx | (fy << 8) | (fz << 16));
-    }
-    __syncthreads();
-    const uint32_t toff = s_off;
-    for (int t = tid; t < pnodes; t += kThreads) {
-      const int lx = t % fx, ly = (t / fx) % fy, lz = t / (fx * fy);
-      const int w = ((pz + lz - wz) * S + (py + ly - wy)) * S + (px + lx - wx);
-      const int4 q = make_int4(tile[w], tile[kTileNodes + w], tile[2 * kTileNodes + w], tile[3 * kTileNodes + w]);
-      if (toff != 0xffffffffu) {
-        pool.tiles[toff + (uint32_t)t] = q;
-      } else if (q.x | q.y | q.z | q.w) {  // no arena room: add the node to A
-        const int gx = px + lx, gy = py + ly, gz = pz + lz;
-        stamp_fallback(pool, geo.block_base + (uint32_t)(gx >> 2) + (uint32_t)geo.nbx * ((uint32_t)(gy >> 2) +
-                                 (uint32_t)geo.nby * (uint32_t)(gz >> 2)), epoch, true, bin_epoch);
-        int* g = reinterpret_cast<int*>(pool.A + node_addr(gx, gy, gz, geo));
-        atomicAdd(g, q.x);
-        atomicAdd(g + 1, q.y);
-        atomicAdd(g + 2, q.z);
-        atomicAdd(g + 3, q.w);
-      }
-    }
+    store_item_tile(pool, k, geo, tile, bbox, wx, wy, wz, epoch, bin_epoch, s_off);
   }
 }
 
