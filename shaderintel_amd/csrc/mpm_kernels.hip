@@ -16,6 +16,7 @@ namespace mpm {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kFuseThreads = 192;  // the fused G2P + P2G: 3 waves per workgroup, 4 workgroups per CU
 
 __device__ __forceinline__ uint32_t node_addr(int gx, int gy, int gz, const SceneGeo& g) {
   const uint32_t b = g.block_base + (uint32_t)(gx >> 2) +
@@ -216,14 +217,21 @@ struct G2PIn {
   int bx, by, bz;
 };
 
-__device__ __forceinline__ void g2p_load(const float4* src, uint32_t si, uint32_t cap, G2PIn& in) {
+// g2p_load = g2p_fetch (the two loads) + g2p_base (what follows from them).
+__device__ __forceinline__ void g2p_fetch(const float4* src, uint32_t si, uint32_t cap, G2PIn& in) {
   in.x4 = src[si];
   in.s5 = src[(size_t)5 * cap + si];
+}
+__device__ __forceinline__ void g2p_base(G2PIn& in) {
   in.xp = mk(in.x4.x, in.x4.y, in.x4.z);
   in.bx = (int)floorf(in.xp.x - 0.5f);
   in.by = (int)floorf(in.xp.y - 0.5f);
   in.bz = (int)floorf(in.xp.z - 0.5f);
   in.fx = in.xp - mk((float)in.bx, (float)in.by, (float)in.bz);
+}
+__device__ __forceinline__ void g2p_load(const float4* src, uint32_t si, uint32_t cap, G2PIn& in) {
+  g2p_fetch(src, si, cap, in);
+  g2p_base(in);
 }
 
 // v = sum wt gv and B = sum outer(wt gv, dpos) over the 27 nodes (mpm.js:487-496) with
@@ -416,18 +424,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   uint32_t* const wctr = pool.arena_ctr + kArenaRegions * kArenaStride + kWorkSplits * kWorkStride;
   uint32_t range = blockIdx.x % kWorkSplits, spent = 0;
   const float4* __restrict__ V = pool.V;
+  if (tid == 0) s_item = next_item(wctr, items, range, spent);
+  __syncthreads();
   for (;;) {
-    __syncthreads();  // the previous item is finished with the box and with s_item
-    if (tid == 0) s_item = next_item(wctr, items, range, spent);
-    __syncthreads();
     const uint32_t k = s_item;
     if (k >= items) break;
     const uint32_t start = pool.wl_start[k], cnt = pool.wl_start[k + 1] - start;
     const uint32_t scene = pool.wl_block[k] >> mshift;
+    const bool has_box = pool.tile_off[k] != 0xffffffffu;
+    const int4 h = pool.tile_hdr[k];  // meaningful when has_box; loaded beside tile_off
     const SceneGeo geo = uniform_geo(pool, scene);
     int ox = 0, oy = 0, oz = 0, ex = 0, ey = 0, ez = 0;
-    if (pool.tile_off[k] != 0xffffffffu) {
-      const int4 h = pool.tile_hdr[k];
+    if (has_box) {
       ox = h.x; oy = h.y; oz = h.z;
       ex = h.w & 255; ey = (h.w >> 8) & 255; ez = (h.w >> 16) & 255;
     }
@@ -458,6 +466,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
       g2p_finish(pool.mats, geo, src, si, dst, di, pool.cap, vmax, in, v, B, r);
       if (keys_out) keys_out[di] = sort_key(pool, scene, geo.active, r[0]);  // for the next substep's sort
     }
+    // the next item, fetched while the other waves finish (every thread read k before the barrier above)
+    if (tid == 0) s_item = next_item(wctr, items, range, spent);
+    __syncthreads();  // the item is finished with the box; s_item holds the next item
   }
 }
 
@@ -827,14 +838,29 @@ __device__ __forceinline__ void fallback_particle(const Pool& pool, const float4
   });
 }
 
+// Work-item loop state of a workgroup (kept by thread 0): the work counter range it takes items
+// from, the ranges found used up, and whether its arena region was found full in this P2G.
+struct ItemCursor {
+  uint32_t* wctr;
+  uint32_t items, range, spent;
+  bool full;
+};
+
 // The end of an item's P2G (after a barrier that follows the bbox atomics): the LDS window tile
 // (S^3 nodes from the lowest in-window base (wx, wy, wz), 4 planes) is stored over the bounding box
 // bbox of the in-window stencil bases (+ 2 nodes) in arena space, recorded in tile_off / tile_hdr; when
 // the arena has no room, the box's non-zero nodes are added to A with atomics instead (and their
-// node blocks stamped), which gives the same sums. Every thread of the workgroup calls it.
-__device__ __forceinline__ void store_item_tile(const Pool& pool, uint32_t k, const SceneGeo& geo, const int* tile,
+// node blocks stamped), which gives the same sums. The nodes read are cleared, which leaves the
+// whole tile zero for the next item (the stencils of the in-window particles lie in the box). Thread 0 takes the arena space and the
+// workgroup's next item (into s_item) with two atomics issued together: one round trip. Once an add
+// finds the region full, the workgroup stops adding to it (an add past the end takes nothing, so a
+// region's counter ends at most one tile per workgroup past its size). Every thread of the
+// workgroup calls it; s_item is read after the caller's next barrier.
+template <int THREADS>
+__device__ __forceinline__ void store_item_tile(const Pool& pool, uint32_t k, const SceneGeo& geo, int* tile,
                                                 const int* bbox, int wx, int wy, int wz, uint32_t epoch,
-                                                uint32_t bin_epoch, uint32_t& s_off) {
+                                                uint32_t bin_epoch, uint32_t& s_off, ItemCursor& cur,
+                                                uint32_t& s_item) {
   constexpr int S = kTileSpan;
   const int tid = threadIdx.x;
   const bool any = bbox[0] != INT_MAX;
@@ -842,13 +868,17 @@ __device__ __forceinline__ void store_item_tile(const Pool& pool, uint32_t k, co
   const int fx = any ? bbox[3] - px + 3 : 0, fy = any ? bbox[4] - py + 3 : 0, fz = any ? bbox[5] - pz + 3 : 0;
   const int pnodes = fx * fy * fz;  // <= kTileNodes: inside the window tile
   if (tid == 0) {  // arena space for the tile, from this workgroup's region
-    uint32_t off = 0xffffffffu;
     const uint32_t region = blockIdx.x % kArenaRegions, rsize = pool.tile_nodes / kArenaRegions;
-    uint32_t* rc = pool.arena_ctr + region * kArenaStride;
-    // the check keeps a full region's counter from running far past its end
-    if (pnodes > 0 && __hip_atomic_load(rc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < rsize) {
-      const uint32_t o = atomicAdd(rc, (uint32_t)pnodes);
-      if (o <= rsize - (uint32_t)pnodes) off = region * rsize + o;
+    const bool add = pnodes > 0 && !cur.full;
+    uint32_t o = 0;
+    if (add) o = atomicAdd(pool.arena_ctr + region * kArenaStride, (uint32_t)pnodes);
+    s_item = next_item(cur.wctr, cur.items, cur.range, cur.spent);
+    uint32_t off = 0xffffffffu;
+    if (add) {
+      if (o <= rsize - (uint32_t)pnodes)  // pnodes <= kTileNodes < rsize
+        off = region * rsize + o;
+      else
+        cur.full = true;
     }
     s_off = off;
     pool.tile_off[k] = off;
@@ -856,10 +886,14 @@ __device__ __forceinline__ void store_item_tile(const Pool& pool, uint32_t k, co
   }
   __syncthreads();
   const uint32_t toff = s_off;
-  for (int t = tid; t < pnodes; t += kThreads) {
+  for (int t = tid; t < pnodes; t += THREADS) {
     const int lx = t % fx, ly = (t / fx) % fy, lz = t / (fx * fy);
     const int w = ((pz + lz - wz) * S + (py + ly - wy)) * S + (px + lx - wx);
     const int4 q = make_int4(tile[w], tile[kTileNodes + w], tile[2 * kTileNodes + w], tile[3 * kTileNodes + w]);
+    tile[w] = 0;
+    tile[kTileNodes + w] = 0;
+    tile[2 * kTileNodes + w] = 0;
+    tile[3 * kTileNodes + w] = 0;
     if (toff != 0xffffffffu) {
       pool.tiles[toff + (uint32_t)t] = q;
     } else if (q.x | q.y | q.z | q.w) {  // no arena room: add the node to A
@@ -894,30 +928,29 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   const uint32_t items = pool.ctr[kCtrItems];
   const uint32_t epoch = pool.ctr[kCtrEpoch], bin_epoch = pool.ctr[kCtrBinEpoch];
   const uint32_t mshift = pool.key_bits - 6;
-  uint32_t* const wctr = pool.arena_ctr + kArenaRegions * kArenaStride;
-  uint32_t range = blockIdx.x % kWorkSplits, spent = 0;  // ranges found used up (tid 0 only)
+  ItemCursor cur{pool.arena_ctr + kArenaRegions * kArenaStride, items, blockIdx.x % kWorkSplits, 0, false};
   constexpr int S = kTileSpan;  // window tile: S^3 nodes from the lowest in-window base
+  for (int t = tid; t < kTileNodes; t += kThreads)
+    reinterpret_cast<int4*>(tile)[t] = make_int4(0, 0, 0, 0);  // the 4 planes; each item leaves them zero
+  if (tid == 0) s_item = next_item(cur.wctr, cur.items, cur.range, cur.spent);
+  __syncthreads();
   for (;;) {
-    __syncthreads();  // the previous item is finished with the tile and with s_item
-    if (tid == 0) s_item = next_item(wctr, items, range, spent);
-    __syncthreads();
     const uint32_t k = s_item;
     if (k >= items) break;
     const uint32_t start = pool.wl_start[k], cnt = pool.wl_start[k + 1] - start;
     const uint32_t wb = pool.wl_block[k];
+    // this thread's first record
+    uint32_t li = tid;
+    float4 r[kP2GStreams] = {};
+    if (li < cnt) load_p2g_record(P, pool.cap, perm ? perm[start + li] : start + li, r);
     const SceneGeo geo = uniform_geo(pool, wb >> mshift);
     const uint32_t mb = wb & ((1u << mshift) - 1u);
     const int wx = 4 * (int)compact3(mb) - kDrift, wy = 4 * (int)compact3(mb >> 1) - kDrift,
               wz = 4 * (int)compact3(mb >> 2) - kDrift;  // lowest in-window base
-    for (int t = tid; t < kTileNodes; t += kThreads)
-      reinterpret_cast<int4*>(tile)[t] = make_int4(0, 0, 0, 0);  // the 4 planes, kTileNodes int4 in all
     if (tid < 3) { bbox[tid] = INT_MAX; bbox[3 + tid] = INT_MIN; }
     __syncthreads();
     int lo0 = INT_MAX, lo1 = INT_MAX, lo2 = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hi2 = INT_MIN;
-    for (uint32_t li = tid; li < cnt; li += kThreads) {
-      const uint32_t j = start + li;
-      float4 r[kP2GStreams];
-      load_p2g_record(P, pool.cap, perm ? perm[j] : j, r);
+    for (; li < cnt; li += kThreads) {
       const int b0 = (int)floorf(r[0].x - 0.5f), b1 = (int)floorf(r[0].y - 0.5f), b2 = (int)floorf(r[0].z - 0.5f);
       const bool inwin = (unsigned)(b0 - wx) < (unsigned)(4 + 2 * kDrift) &&
                          (unsigned)(b1 - wy) < (unsigned)(4 + 2 * kDrift) && (unsigned)(b2 - wz) < (unsigned)(4 + 2 * kDrift);
@@ -935,6 +968,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
       } else {
         fallback_particle(pool, r, epoch, true, bin_epoch);
       }
+      if (li + kThreads < cnt) {  // the next record of this thread
+        const uint32_t jn = start + li + kThreads;
+        load_p2g_record(P, pool.cap, perm ? perm[jn] : jn, r);
+      }
     }
     lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
     hi0 = wave_max(hi0); hi1 = wave_max(hi1); hi2 = wave_max(hi2);
@@ -943,7 +980,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
       atomicMax(&bbox[3], hi0); atomicMax(&bbox[4], hi1); atomicMax(&bbox[5], hi2);
     }
     __syncthreads();
-    store_item_tile(pool, k, geo, tile, bbox, wx, wy, wz, epoch, bin_epoch, s_off);
+    store_item_tile<kThreads>(pool, k, geo, tile, bbox, wx, wy, wz, epoch, bin_epoch, s_off, cur, s_item);
+    __syncthreads();  // the tile is read out and zero again
   }
 }
 
@@ -957,9 +995,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
 // space, which becomes the item's tile for the next grid update; if the arena has no room, the box
 // is added to A with atomics instead (integer sums, the same result). perm: sorted position ->
 // record of src (nullptr: identity).
-constexpr int kFuseBoxNodes = 1024;
+constexpr int kFuseBoxNodes = 729;
 template <int WAVES>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_g2p_p2g(
+__global__ __launch_bounds__(kFuseThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_g2p_p2g(
     Pool pool, const float4* src, float4* dst, const uint32_t* __restrict__ perm, int sorted_out, float vmax) {
   __shared__ float4 vbox[kFuseBoxNodes];
   __shared__ int tile[4 * kTileNodes];
@@ -971,44 +1009,48 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   const uint32_t mshift = pool.key_bits - 6;
   // P2G's work counters (the launch_begin_p2g before this kernel cleared them; G2P's stay fresh for
   // the next substep's G2P, which has no launch_begin_p2g of its own before it)
-  uint32_t* const wctr = pool.arena_ctr + kArenaRegions * kArenaStride;
-  uint32_t range = blockIdx.x % kWorkSplits, spent = 0;
+  ItemCursor cur{pool.arena_ctr + kArenaRegions * kArenaStride, items, blockIdx.x % kWorkSplits, 0, false};
   const float4* __restrict__ V = pool.V;
   constexpr int S = kTileSpan;  // window tile: S^3 nodes from the lowest in-window base
+  for (int t = tid; t < kTileNodes; t += kFuseThreads)
+    reinterpret_cast<int4*>(tile)[t] = make_int4(0, 0, 0, 0);  // the 4 planes; each item leaves them zero
+  if (tid == 0) s_item = next_item(cur.wctr, cur.items, cur.range, cur.spent);
+  __syncthreads();
   for (;;) {
-    __syncthreads();  // the previous item is finished with the LDS and with s_item
-    if (tid == 0) s_item = next_item(wctr, items, range, spent);
-    __syncthreads();
     const uint32_t k = s_item;
     if (k >= items) break;
     const uint32_t start = pool.wl_start[k], cnt = pool.wl_start[k + 1] - start;
     const uint32_t wb = pool.wl_block[k];
+    const bool has_box = pool.tile_off[k] != 0xffffffffu;
+    const int4 h = pool.tile_hdr[k];  // meaningful when has_box; loaded beside tile_off
+    // this thread's first record, loaded while the velocity box is staged
+    uint32_t li = tid, si = 0;
+    G2PIn in = {};
+    if (li < cnt) {
+      si = perm ? perm[start + li] : start + li;
+      g2p_fetch(src, si, pool.cap, in);
+    }
     const SceneGeo geo = uniform_geo(pool, wb >> mshift);
     const uint32_t mb = wb & ((1u << mshift) - 1u);
     const int wx = 4 * (int)compact3(mb) - kDrift, wy = 4 * (int)compact3(mb >> 1) - kDrift,
               wz = 4 * (int)compact3(mb >> 2) - kDrift;  // lowest in-window base
     int ox = 0, oy = 0, oz = 0, ex = 0, ey = 0, ez = 0;
-    if (pool.tile_off[k] != 0xffffffffu) {
-      const int4 h = pool.tile_hdr[k];
+    if (has_box) {
       ox = h.x; oy = h.y; oz = h.z;
       ex = h.w & 255; ey = (h.w >> 8) & 255; ez = (h.w >> 16) & 255;
     }
     int nodes = ex * ey * ez;
     if (nodes > kFuseBoxNodes) nodes = 0;  // gathers from V
-    for (int t = tid; t < nodes; t += kThreads) {
+    for (int t = tid; t < nodes; t += kFuseThreads) {
       const int lx = t % ex, ly = (t / ex) % ey, lz = t / (ex * ey);
       vbox[t] = V[node_addr(ox + lx, oy + ly, oz + lz, geo)];
     }
-    for (int t = tid; t < kTileNodes; t += kThreads)
-      reinterpret_cast<int4*>(tile)[t] = make_int4(0, 0, 0, 0);  // the 4 planes, kTileNodes int4 in all
     if (tid < 3) { bbox[tid] = INT_MAX; bbox[3 + tid] = INT_MIN; }
     __syncthreads();
     int lo0 = INT_MAX, lo1 = INT_MAX, lo2 = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hi2 = INT_MIN;
-    for (uint32_t li = tid; li < cnt; li += kThreads) {
+    for (; li < cnt; li += kFuseThreads) {
       const uint32_t j = start + li;
-      const uint32_t si = perm ? perm[j] : j;
-      G2PIn in;
-      g2p_load(src, si, pool.cap, in);
+      g2p_base(in);
       const bool inbox = nodes > 0 && (unsigned)(in.bx - ox) <= (unsigned)(ex - 3) &&
                          (unsigned)(in.by - oy) <= (unsigned)(ey - 3) && (unsigned)(in.bz - oz) <= (unsigned)(ez - 3);
       f3 v;
@@ -1045,6 +1087,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
       } else {
         fallback_particle(pool, r, epoch, true, bin_epoch);
       }
+      if (li + kFuseThreads < cnt) {  // the next record of this thread
+        si = perm ? perm[j + kFuseThreads] : j + kFuseThreads;
+        g2p_fetch(src, si, pool.cap, in);
+      }
     }
     lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
     hi0 = wave_max(hi0); hi1 = wave_max(hi1); hi2 = wave_max(hi2);
@@ -1053,7 +1099,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
       atomicMax(&bbox[3], hi0); atomicMax(&bbox[4], hi1); atomicMax(&bbox[5], hi2);
     }
     __syncthreads();
-    store_item_tile(pool, k, geo, tile, bbox, wx, wy, wz, epoch, bin_epoch, s_off);
+    store_item_tile<kFuseThreads>(pool, k, geo, tile, bbox, wx, wy, wz, epoch, bin_epoch, s_off, cur, s_item);
+    __syncthreads();  // the tile is read out and zero again
   }
 }
 
@@ -1326,13 +1373,14 @@ hipError_t launch_clear_a(const Pool& pool, bool use_s, hipStream_t s) {
 }
 // G2P launches; with ev_start / ev_stop (either may be null) the dispatch itself records them
 // (hipExtLaunchKernelGGL), so timing a launch adds no packet to the stream.
-#define MPM_G2P_LAUNCH(KERNEL, GRID, S, EV0, EV1, ...)                                              \
+#define MPM_G2P_LAUNCH_T(KERNEL, GRID, THREADS, S, EV0, EV1, ...)                                   \
   do {                                                                                              \
     if ((EV0) || (EV1))                                                                             \
-      hipExtLaunchKernelGGL(KERNEL, GRID, dim3(kThreads), 0, S, EV0, EV1, 0, __VA_ARGS__);         \
+      hipExtLaunchKernelGGL(KERNEL, GRID, dim3(THREADS), 0, S, EV0, EV1, 0, __VA_ARGS__);          \
     else                                                                                            \
-      hipLaunchKernelGGL(KERNEL, GRID, dim3(kThreads), 0, S, __VA_ARGS__);                          \
+      hipLaunchKernelGGL(KERNEL, GRID, dim3(THREADS), 0, S, __VA_ARGS__);                           \
   } while (0)
+#define MPM_G2P_LAUNCH(KERNEL, GRID, S, EV0, EV1, ...) MPM_G2P_LAUNCH_T(KERNEL, GRID, kThreads, S, EV0, EV1, __VA_ARGS__)
 
 hipError_t launch_g2p_perm(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, uint32_t n, float vmax,
                            int waves, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop) {
@@ -1366,8 +1414,8 @@ hipError_t launch_g2p_p2g(const Pool& pool, const float4* src, float4* dst, cons
   const dim3 grid(persistent_grid(pool.max_items, 1, 2048));
   const int so = sorted_out ? 1 : 0;
   switch (waves) {
-    case 4: MPM_G2P_LAUNCH(k_g2p_p2g<4>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax); break;
-    default: MPM_G2P_LAUNCH(k_g2p_p2g<3>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax); break;
+    case 4: MPM_G2P_LAUNCH_T(k_g2p_p2g<4>, grid, kFuseThreads, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax); break;
+    default: MPM_G2P_LAUNCH_T(k_g2p_p2g<3>, grid, kFuseThreads, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax); break;
   }
   return hipGetLastError();
 }
