@@ -543,6 +543,21 @@ __global__ void k_begin_bin(uint32_t* ctr) {
   ctr[kCtrItems] = 0u;
 }
 
+// k_begin_bin, the map cleared to 0xffffffff (map_n words) and the occupancy bitmap to 0 (occ_n
+// 32-bit words), in one launch.
+__global__ void k_bin_reset(Pool pool, uint32_t map_n, uint32_t occ_n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < map_n)
+    pool.map[i] = 0xffffffffu;
+  else if (i - map_n < occ_n)
+    reinterpret_cast<uint32_t*>(pool.occ)[i - map_n] = 0u;
+  if (i == 0) {
+    pool.ctr[kCtrBinEpoch] += 1u;
+    pool.ctr[kCtrS] = 0u;
+    pool.ctr[kCtrItems] = 0u;
+  }
+}
+
 
 // Sort keys of the particles in their current order (sort_key); the radix sort's values are the
 // indices, so its stable order breaks ties by them.
@@ -636,14 +651,31 @@ __global__ __launch_bounds__(kThreads) void k_bin_count(const uint32_t* __restri
   if (blockIdx.x == 0 && threadIdx.x == 0) tile_heads[gridDim.x] = 0;  // becomes the total after the scan
 }
 
-// tile_first[t]: the exclusive scan of the tile head counts (tiles + 1 entries; the last is the total).
+// scanned: tile_first[t] is the exclusive scan of the tile head counts (tiles + 1 entries; the last
+// is the total). Otherwise it holds the counts themselves, and each workgroup adds up those of the
+// tiles before it (workgroup 0: of all tiles, the total); for at most kItemScanFree tiles.
+constexpr uint32_t kItemScanFree = 4096;
 __global__ __launch_bounds__(kThreads) void k_bin_emit(Pool pool, const uint32_t* __restrict__ keys, uint32_t n,
-                                                       const uint32_t* __restrict__ tile_first, uint32_t tiles) {
+                                                       const uint32_t* __restrict__ tile_first, uint32_t tiles,
+                                                       int scanned) {
   uint32_t k[kItemPer];
   const uint32_t first = blockIdx.x * kItemTile + threadIdx.x * kItemPer;
   uint32_t m = head_mask(keys, n, first, k);
+  uint32_t base = 0, total = 0;
+  if (scanned) {
+    base = tile_first[blockIdx.x];
+  } else {
+    const uint32_t lim = blockIdx.x == 0 ? tiles : blockIdx.x;
+    uint32_t ps = 0;
+    for (uint32_t t = threadIdx.x; t < lim; t += kThreads) ps += tile_first[t];
+    uint32_t sum;
+    block_exclusive_sum(ps, &sum);
+    __syncthreads();  // the block sum's LDS is used again below
+    base = blockIdx.x == 0 ? 0u : sum;
+    total = sum;
+  }
   uint32_t tot;
-  uint32_t item = tile_first[blockIdx.x] + block_exclusive_sum((uint32_t)__builtin_popcount(m), &tot);
+  uint32_t item = base + block_exclusive_sum((uint32_t)__builtin_popcount(m), &tot);
   const uint32_t mmask = (1u << (pool.key_bits - 6)) - 1u;
   while (m) {
     const int j = __builtin_ctz(m);
@@ -658,7 +690,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_emit(Pool pool, const uint32_t
     item++;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    const uint32_t items = tile_first[tiles];
+    const uint32_t items = scanned ? tile_first[tiles] : total;
     pool.wl_start[items] = n;
     pool.ctr[kCtrItems] = items;
   }
@@ -1298,6 +1330,12 @@ hipError_t launch_begin_bin(const Pool& pool, hipStream_t s) {
   hipLaunchKernelGGL(k_begin_bin, dim3(1), dim3(1), 0, s, pool.ctr);
   return hipGetLastError();
 }
+hipError_t launch_bin_reset(const Pool& pool, uint32_t map_words, uint32_t occ_words32, hipStream_t s) {
+  const uint64_t n = (uint64_t)map_words + occ_words32;
+  if (n == 0 || n > 0xffffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_bin_reset, dim3(blocks_for((uint32_t)n, 256)), dim3(256), 0, s, pool, map_words, occ_words32);
+  return hipGetLastError();
+}
 hipError_t launch_keys(const Pool& pool, const float4* P, uint32_t n, uint32_t* keys, hipStream_t s) {
   if (!n) return hipSuccess;
   hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, 256)), dim3(256), 0, s, pool, P, n, keys);
@@ -1319,9 +1357,13 @@ hipError_t launch_bin_count(const uint32_t* keys, uint32_t n, uint32_t* tile_hea
   hipLaunchKernelGGL(k_bin_count, dim3(bin_tiles(n)), dim3(kThreads), 0, s, keys, n, tile_heads);
   return hipGetLastError();
 }
-hipError_t launch_bin_emit(const Pool& pool, const uint32_t* keys, uint32_t n, const uint32_t* tile_first, hipStream_t s) {
+bool bin_emit_scans(uint32_t n) { return bin_tiles(n) <= kItemScanFree; }
+hipError_t launch_bin_emit(const Pool& pool, const uint32_t* keys, uint32_t n, const uint32_t* tile_first, bool scanned,
+                           hipStream_t s) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_bin_emit, dim3(bin_tiles(n)), dim3(kThreads), 0, s, pool, keys, n, tile_first, bin_tiles(n));
+  if (!scanned && !bin_emit_scans(n)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_bin_emit, dim3(bin_tiles(n)), dim3(kThreads), 0, s, pool, keys, n, tile_first, bin_tiles(n),
+                     scanned ? 1 : 0);
   return hipGetLastError();
 }
 hipError_t launch_bin_transpose(const Pool& pool, const uint32_t* keys, const uint32_t* in, uint32_t* out, hipStream_t s) {

@@ -147,6 +147,8 @@ hipError_t launch_block_info(uint2* info, uint32_t base, uint32_t n, uint32_t sc
 hipError_t launch_begin_p2g(const Pool& pool, hipStream_t s);
 // ctr[kCtrBinEpoch]++, |S| = 0, items = 0 (start of a binning)
 hipError_t launch_begin_bin(const Pool& pool, hipStream_t s);
+// launch_begin_bin + the map set to 0xffffffff + the occupancy bitmap cleared (one launch)
+hipError_t launch_bin_reset(const Pool& pool, uint32_t map_words, uint32_t occ_words32, hipStream_t s);
 
 // Sort keys (scene << key_bits | Morton(base cell)) of the particles in their current order;
 // particles of inactive scenes get key 0xffffffff (sorted behind every live particle).
@@ -160,8 +162,10 @@ hipError_t launch_permute(const float4* src, float4* dst, uint32_t cap, uint32_t
 // map; the within-block transposition of the sorted indices into `out`; and S.
 uint32_t bin_tiles(uint32_t n);
 hipError_t launch_bin_count(const uint32_t* keys_sorted, uint32_t n, uint32_t* tile_heads, hipStream_t s);
+// scanned: tile_first holds the scanned head counts; otherwise the counts (bin_emit_scans(n) must hold)
+bool bin_emit_scans(uint32_t n);
 hipError_t launch_bin_emit(const Pool& pool, const uint32_t* keys_sorted, uint32_t n, const uint32_t* tile_first,
-                           hipStream_t s);
+                           bool scanned, hipStream_t s);
 hipError_t launch_bin_transpose(const Pool& pool, const uint32_t* keys_sorted, const uint32_t* in, uint32_t* out,
                                 hipStream_t s);
 // S from the occupancy bitmap: each node block of a scene's grid with an occupied particle block

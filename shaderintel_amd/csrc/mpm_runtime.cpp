@@ -382,10 +382,10 @@ mpm_status do_bin(mpm_pool* p, int mode) {
   }
   ST_TRY(clean_a(p));
   const Pool pv = view(p);
-  HIP_TRY(launch_begin_bin(pv, p->stream));
   p->perm = nullptr;
   p->reorder_in_g2p = false;
   if (p->count == 0) {
+    HIP_TRY(launch_begin_bin(pv, p->stream));
     p->bin_valid = true;
     return MPM_OK;
   }
@@ -396,15 +396,16 @@ mpm_status do_bin(mpm_pool* p, int mode) {
   // by the transposition
   HIP_TRY(radix_sort_pairs(p->keys[0], p->vals[0], p->keys[1], p->vals[1], p->count, 6, (int)p->sort_bits,
                            p->sort_scratch, p->stream, &rk, &rv, true));
-  HIP_TRY(launch_fill_u32(p->map, 0xffffffffu, p->map_per_scene * p->max_scenes, p->stream));
-  HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(p->occ), 0u, 2 * occ_words(p->map_per_scene) * p->max_scenes,
-                          p->stream));
+  // a new binning epoch, S and the items reset, the map and the occupancy bitmap cleared
+  HIP_TRY(launch_bin_reset(pv, p->map_per_scene * p->max_scenes, 2 * occ_words(p->map_per_scene) * p->max_scenes,
+                           p->stream));
   {  // work items: heads per tile, scanned, then numbered (the sort's scratch is free again)
     const uint32_t tiles = bin_tiles(p->count);
     uint32_t* tile_first = p->sort_scratch;
     HIP_TRY(launch_bin_count(rk, p->count, tile_first, p->stream));
-    HIP_TRY(scan_exclusive(tile_first, (size_t)tiles + 1, p->sort_scratch + ((tiles + 4) & ~3u), p->stream));
-    HIP_TRY(launch_bin_emit(pv, rk, p->count, tile_first, p->stream));
+    const bool scan = !bin_emit_scans(p->count);  // few tiles: the emit adds up the counts itself
+    if (scan) HIP_TRY(scan_exclusive(tile_first, (size_t)tiles + 1, p->sort_scratch + ((tiles + 4) & ~3u), p->stream));
+    HIP_TRY(launch_bin_emit(pv, rk, p->count, tile_first, scan, p->stream));
   }
   HIP_TRY(launch_bin_transpose(pv, rk, rv, p->order, p->stream));
   if (mode == kBinReorder) {

@@ -68,7 +68,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __
   if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
-// Each thread scans 16 contiguous items; tile offsets come from the scanned partials.
+// Each thread scans 16 contiguous items. Tile offsets: none (use_offsets 0, a single tile), the
+// scanned partials (1), or the sum of the unscanned partials of the earlier tiles, added up here (2;
+// at most kScanTile tiles, so each thread adds at most kScanItems of them).
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict__ data, size_t n,
                                                             const uint32_t* __restrict__ partials, int use_offsets) {
   const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
@@ -89,7 +91,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict
   for (int j = 0; j < kScanItems; j++) s += v[j];
   uint32_t tot;
   uint32_t acc = block_excl_scan(s, &tot);
-  if (use_offsets) acc += partials[blockIdx.x];
+  if (use_offsets == 1) acc += partials[blockIdx.x];
+  if (use_offsets == 2) {
+    uint32_t ps = 0;
+    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kScanThreads) ps += partials[t];
+    uint32_t before;
+    block_excl_scan(ps, &before);  // the workgroup total of ps
+    acc += before;
+  }
 #pragma unroll
   for (int j = 0; j < kScanItems; j++) {
     uint32_t a = v[j];
@@ -116,6 +125,11 @@ hipError_t scan_level(uint32_t* data, size_t n, uint32_t* scratch, hipStream_t s
   uint32_t* partials = scratch;
   hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)tiles), dim3(kScanThreads), 0, stream, (const uint32_t*)data, n,
                      partials);
+  if (tiles <= (size_t)kScanTile) {  // each tile adds up the partials before it: no third launch
+    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)tiles), dim3(kScanThreads), 0, stream, data, n,
+                       (const uint32_t*)partials, 2);
+    return hipGetLastError();
+  }
   hipError_t e = scan_level(partials, tiles, scratch + ((tiles + 3) & ~(size_t)3), stream);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_scan_down, dim3((unsigned)tiles), dim3(kScanThreads), 0, stream, data, n,
