@@ -732,12 +732,9 @@ __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uin
     const uint32_t start = __builtin_amdgcn_readfirstlane(pool.wl_start[k]);
     const uint32_t cnt = __builtin_amdgcn_readfirstlane(pool.wl_start[k + 1]) - start;
     wrun[w][lane] = 0;
-    for (uint32_t j0 = 0; j0 < cnt; j0 += 64) {
+    for (uint32_t j0 = 0; j0 < cnt; j0 += 64) {  // the cell counts (LDS atomics of this wave, in order)
       const uint32_t j = j0 + lane;
-      const uint64_t valid = __ballot(j < cnt);
-      const uint32_t c = j < cnt ? keys[start + j] & 63u : 0u;
-      const uint64_t peers = cell_peers(c, valid);
-      if (j < cnt && (peers & below) == 0) wrun[w][c] += (uint32_t)__popcll(peers);
+      if (j < cnt) atomicAdd(&wrun[w][keys[start + j] & 63u], 1u);
     }
     const uint32_t nc = wrun[w][lane];  // particles in cell `lane`
     wcnt[w][lane] = nc;
