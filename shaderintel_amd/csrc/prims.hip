@@ -159,15 +159,21 @@ __device__ __forceinline__ uint64_t digit_peers(uint32_t d, int bits, uint64_t v
 
 // Per-tile digit counts, digit-major (hist[d * tiles + t]). Each thread counts 16 consecutive keys
 // (four 16-byte loads), adding a run of equal digits to the LDS histogram at once (the keys arrive
-// nearly sorted).
+// nearly sorted). Workgroups are dealt to the 8 XCDs in turn, so workgroup b takes tile
+// (b % 8) * per + b / 8 (per = ceil(tiles / 8)): each XCD counts a contiguous run of tiles, and a
+// cache line of a digit's row (32 tiles) is written from one XCD's L2 instead of from all eight.
+constexpr uint32_t kXcds = 8;
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys, size_t n, int shift,
                                                              int bits, uint32_t tiles, uint32_t* __restrict__ hist) {
   __shared__ uint32_t h[kRadixMaxBuckets];
+  const uint32_t per = (tiles + kXcds - 1) / kXcds;
+  const uint32_t tile = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+  if (tile >= tiles) return;  // the whole workgroup: no barrier is reached
   const int nb = 1 << bits;
   const uint32_t mask = (uint32_t)nb - 1u;
   for (int d = threadIdx.x; d < nb; d += kSortThreads) h[d] = 0;
   __syncthreads();
-  const size_t first = (size_t)blockIdx.x * kSortTile + (size_t)threadIdx.x * kSortItems;
+  const size_t first = (size_t)tile * kSortTile + (size_t)threadIdx.x * kSortItems;
   uint32_t k[kSortItems];
   int valid = 0;
   if (first + kSortItems <= n) {  // keys is 16-byte aligned and first is a multiple of 16
@@ -198,7 +204,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
   }
   if (run_n) atomicAdd(&h[run_d], run_n);
   __syncthreads();
-  for (int d = threadIdx.x; d < nb; d += kSortThreads) hist[(size_t)d * tiles + blockIdx.x] = h[d];
+  for (int d = threadIdx.x; d < nb; d += kSortThreads) hist[(size_t)d * tiles + tile] = h[d];
 }
 
 // Stable scatter of one tile. hist holds the exclusive scan of the digit-major counts, so
@@ -218,9 +224,13 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
   __shared__ uint32_t gofs[kRadixMaxBuckets];             // output index of local slot 0 of digit d
   __shared__ uint32_t wsum[kSortWaves];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  // the histogram's tile order: each XCD takes a contiguous run of tiles (their offsets share lines)
+  const uint32_t per = (tiles + kXcds - 1) / kXcds;
+  const uint32_t tile = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+  if (tile >= tiles) return;  // the whole workgroup: no barrier is reached
   const int nb = 1 << bits;
   const uint32_t mask = (uint32_t)nb - 1u;
-  const size_t base = (size_t)blockIdx.x * kSortTile;
+  const size_t base = (size_t)tile * kSortTile;
   const uint32_t count = (uint32_t)((n - base) < (size_t)kSortTile ? (n - base) : (size_t)kSortTile);
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int d = tid; d < nb; d += kSortThreads)
@@ -277,7 +287,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
     for (int q = 0; q < kPer; q++) {
       const int d = tid * kPer + q;
       if (d < nb) {
-        gofs[d] = hist[(size_t)d * tiles + blockIdx.x] - off;
+        gofs[d] = hist[(size_t)d * tiles + tile] - off;
         uint32_t o = off;
 #pragma unroll
         for (int k = 0; k < kSortWaves; k++) {
@@ -360,11 +370,12 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, 
   uint32_t *ka = keys, *va = vals, *kb = keys_alt, *vb = vals_alt;
   for (int shift = lo_bit; shift < hi_bit; shift += bits) {
     const int b = hi_bit - shift < bits ? hi_bit - shift : bits;
-    hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kSortThreads), 0, stream, (const uint32_t*)ka, n, shift, b, tiles,
-                       hist);
+    hipLaunchKernelGGL(k_radix_hist, dim3((tiles + kXcds - 1) / kXcds * kXcds), dim3(kSortThreads), 0, stream,
+                       (const uint32_t*)ka, n, shift, b, tiles, hist);
     hipError_t e = scan_exclusive(hist, ((size_t)1 << b) * tiles, scan_tmp, stream);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kSortThreads), 0, stream, (const uint32_t*)ka,
+    hipLaunchKernelGGL(k_radix_scatter, dim3((tiles + kXcds - 1) / kXcds * kXcds), dim3(kSortThreads), 0, stream,
+                       (const uint32_t*)ka,
                        (const uint32_t*)va, kb, vb, n, shift, b, tiles, (const uint32_t*)hist,
                        vals_iota && shift == lo_bit ? 1 : 0);
     e = hipGetLastError();
