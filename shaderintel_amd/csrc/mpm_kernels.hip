@@ -728,7 +728,12 @@ __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uin
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   const uint32_t items = pool.ctr[kCtrItems];
-  for (uint32_t k = blockIdx.x * kBinWaves + w; k < items; k += gridDim.x * kBinWaves) {
+  // workgroups are dealt to the 8 XCDs in turn (the grid is a multiple of 8): XCD x takes the
+  // contiguous items [x * per, (x + 1) * per), so the particle ranges it writes share no cache line
+  // with another XCD's but at the ends
+  const uint32_t per = (items + 7u) / 8u, xcd = blockIdx.x % 8u, wgs = gridDim.x / 8u;
+  const uint32_t kend = min(items, (xcd + 1u) * per);
+  for (uint32_t k = xcd * per + (blockIdx.x / 8u) * kBinWaves + w; k < kend; k += wgs * kBinWaves) {
     const uint32_t start = __builtin_amdgcn_readfirstlane(pool.wl_start[k]);
     const uint32_t cnt = __builtin_amdgcn_readfirstlane(pool.wl_start[k + 1]) - start;
     wrun[w][lane] = 0;
@@ -1364,7 +1369,7 @@ hipError_t launch_bin_emit(const Pool& pool, const uint32_t* keys, uint32_t n, c
   return hipGetLastError();
 }
 hipError_t launch_bin_transpose(const Pool& pool, const uint32_t* keys, const uint32_t* in, uint32_t* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_transpose, dim3(persistent_grid(pool.max_items, 1, 4096)), dim3(kThreads), 0, s, pool, keys,
+  hipLaunchKernelGGL(k_bin_transpose, dim3((persistent_grid(pool.max_items, 1, 4096) + 7u) / 8u * 8u), dim3(kThreads), 0, s, pool, keys,
                      in, out);
   return hipGetLastError();
 }
