@@ -684,6 +684,13 @@ __global__ __launch_bounds__(kThreads) void k_bin_emit(Pool pool, const uint32_t
     pool.wl_start[item] = first + j;
     pool.wl_block[item] = blk;
     pool.map[pool.scenes[scene].map_base + mb] = item;
+    {  // the item's velocity box until its first P2G: the stencils of bases in the block, in the grid
+      const SceneGeo& g = pool.scenes[scene];
+      const int ox = 4 * (int)compact3(mb), oy = 4 * (int)compact3(mb >> 1), oz = 4 * (int)compact3(mb >> 2);
+      const int ex = max(min(6, g.X - ox), 0), ey = max(min(6, g.Y - oy), 0), ez = max(min(6, g.Z - oz), 0);
+      pool.tile_off[item] = 0u;
+      pool.tile_hdr[item] = make_int4(ox, oy, oz, ex | (ey << 8) | (ez << 16));
+    }
     const uint32_t pb = 1u << (pool.key_bits / 3 - 2), W = occ_row_words(pb);
     const uint32_t bx = compact3(mb), by = compact3(mb >> 1), bz = compact3(mb >> 2);
     atomicOr(pool.occ + ((size_t)scene * pb * pb + (size_t)bz * pb + by) * W + (bx >> 6), 1ull << (bx & 63u));
@@ -1028,11 +1035,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
 // left the window. The tile is stored over the bounding box of the new in-window stencils in arena
 // space, which becomes the item's tile for the next grid update; if the arena has no room, the box
 // is added to A with atomics instead (integer sums, the same result). perm: sorted position ->
-// record of src (nullptr: identity).
+// record of src (nullptr: identity). keys_out (optional): the sort key of each new record, at its
+// slot (the keys of a sort that follows this substep).
 constexpr int kFuseBoxNodes = 729;
 template <int WAVES>
 __global__ __launch_bounds__(kFuseThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_g2p_p2g(
-    Pool pool, const float4* src, float4* dst, const uint32_t* __restrict__ perm, int sorted_out, float vmax) {
+    Pool pool, const float4* src, float4* dst, const uint32_t* __restrict__ perm, int sorted_out, float vmax,
+    uint32_t* __restrict__ keys_out) {
   __shared__ float4 vbox[kFuseBoxNodes];
   __shared__ int tile[4 * kTileNodes];
   __shared__ int bbox[6];
@@ -1098,6 +1107,7 @@ __global__ __launch_bounds__(kFuseThreads) __attribute__((amdgpu_waves_per_eu(WA
       }
       float4 r[8];
       g2p_finish(pool.mats, geo, src, si, dst, sorted_out ? j : si, pool.cap, vmax, in, v, B, r);
+      if (keys_out) keys_out[sorted_out ? j : si] = sort_key(pool, wb >> mshift, geo.active, r[0]);
       // the next substep's P2G of the new record
       const int b0 = (int)floorf(r[0].x - 0.5f), b1 = (int)floorf(r[0].y - 0.5f), b2 = (int)floorf(r[0].z - 0.5f);
       const bool inwin = (unsigned)(b0 - wx) < (unsigned)(4 + 2 * kDrift) &&
@@ -1453,13 +1463,14 @@ hipError_t launch_g2p_items(const Pool& pool, const float4* src, float4* dst, co
   return hipGetLastError();
 }
 hipError_t launch_g2p_p2g(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, bool sorted_out,
-                          float vmax, int waves, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop) {
+                          float vmax, int waves, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop,
+                          uint32_t* keys_out) {
   if (sorted_out ? (!perm || src == dst) : src != dst) return hipErrorInvalidValue;
   const dim3 grid(persistent_grid(pool.max_items, 1, 2048));
   const int so = sorted_out ? 1 : 0;
   switch (waves) {
-    case 4: MPM_G2P_LAUNCH_T(k_g2p_p2g<4>, grid, kFuseThreads, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax); break;
-    default: MPM_G2P_LAUNCH_T(k_g2p_p2g<3>, grid, kFuseThreads, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax); break;
+    case 4: MPM_G2P_LAUNCH_T(k_g2p_p2g<4>, grid, kFuseThreads, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax, keys_out); break;
+    default: MPM_G2P_LAUNCH_T(k_g2p_p2g<3>, grid, kFuseThreads, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax, keys_out); break;
   }
   return hipGetLastError();
 }

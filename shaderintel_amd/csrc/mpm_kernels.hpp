@@ -197,10 +197,11 @@ hipError_t launch_g2p_items(const Pool& pool, const float4* src, float4* dst, co
                             hipEvent_t ev_stop = nullptr, uint32_t* keys_out = nullptr);
 // G2P (as launch_g2p_items) fused with the next substep's P2G over the same work items: the new
 // records are scattered into tiles (+ fallback atomics on A) without being read back. Needs
-// launch_begin_p2g after the grid update; leaves the grid state of a binned P2G.
+// launch_begin_p2g after the grid update; leaves the grid state of a binned P2G. keys_out: as
+// launch_g2p_items (the sort keys of the new records).
 hipError_t launch_g2p_p2g(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, bool sorted_out,
                           float vmax, int waves, hipStream_t s, hipEvent_t ev_start = nullptr,
-                          hipEvent_t ev_stop = nullptr);
+                          hipEvent_t ev_stop = nullptr, uint32_t* keys_out = nullptr);
 // G2P that also applies a pending reorder: slot i of dst takes particle perm[i] of src.
 hipError_t launch_g2p_perm(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, uint32_t n, float vmax,
                            int waves, hipStream_t s, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);

@@ -151,6 +151,7 @@ struct mpm_pool {
   bool tiles_fresh = false;  // the tiles and work items are those of this substep's binned P2G
   bool fuse = true;          // G2P fused with the next substep's P2G when no sort comes between them
   bool p2g_ahead = false;    // the next substep's P2G is done (by a fused G2P of this mpm_step call)
+  bool p2g_presort = false;  // ... before that substep's sort, over the items of the sort before it
   bool keys_ahead = false;   // keys[0] holds the sort keys of the current particles (written by the
                              // G2P before a sort in the same mpm_step call)
   bool use_graphs = false;
@@ -166,6 +167,7 @@ struct mpm_pool {
   size_t ev_used = 0;
   float phase_ms[4] = {0, 0, 0, 0};
   std::vector<uint8_t> ev_fused;    // profiling 2: per (start, end) pair, 1 for a fused G2P + P2G launch
+  std::vector<uint8_t> ev_presort;  // profiling 1: per group of 5 marks, 1 when the grid update ran first
   float kernel_ms[2] = {0, 0};      // profiling 2: summed G2P launches {G2P alone, fused G2P + P2G}
   uint32_t kernel_launches[2] = {0, 0};
 };
@@ -509,8 +511,12 @@ mpm_status mark(mpm_pool* p) {
 }
 
 // G2P (mpm.js:598-600). fuse_next: the next substep follows in the same call and does not sort, so
-// its P2G may be done by this G2P (same work items).
-mpm_status do_g2p(mpm_pool* p, float vmax, bool fuse_next, bool keys_next) {
+// its P2G may be done by this G2P (same work items). keys_next: the next substep sorts; this G2P
+// writes the sort keys. presort_next (with keys_next): the next substep follows in the same call, and
+// its P2G may be done by this G2P too, over the current work items, ahead of its sort (P2G's sums
+// do not depend on the particle order or on the items they are gathered by); that substep then
+// runs its grid update before the sort rebuilds the map and S.
+mpm_status do_g2p(mpm_pool* p, float vmax, bool fuse_next, bool keys_next, bool presort_next) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (p->profiling == 2) {
     ST_TRY(next_event(p, &e0));
@@ -518,13 +524,16 @@ mpm_status do_g2p(mpm_pool* p, float vmax, bool fuse_next, bool keys_next) {
   }
   const bool items = p->g2p_items && p->tiles_fresh;
   p->tiles_fresh = false;  // one G2P per P2G: the G2P work counters are those begin_p2g cleared
-  if (e0) p->ev_fused.push_back(items && fuse_next && p->fuse ? 1 : 0);
-  if (items && fuse_next && p->fuse) {
+  const bool presort = items && keys_next && presort_next && p->fuse && !p->hook;
+  if (e0) p->ev_fused.push_back(items && (fuse_next || presort) && p->fuse ? 1 : 0);
+  if (items && (fuse_next || presort) && p->fuse) {
     ST_TRY(clean_a(p));
     HIP_TRY(launch_begin_p2g(view(p), p->stream));  // the next P2G's epoch, arena and work counters
     const bool sorted_out = p->reorder_in_g2p;
     HIP_TRY(launch_g2p_p2g(view(p), p->P[p->cur], p->P[p->cur ^ (sorted_out ? 1 : 0)], p->perm, sorted_out, vmax,
-                           p->g2p_waves, p->stream, e0, e1));
+                           p->g2p_waves, p->stream, e0, e1, presort ? p->keys[0] : nullptr));
+    p->keys_ahead = presort;
+    p->p2g_presort = presort;
     if (sorted_out) {
       p->cur ^= 1;
       p->perm = nullptr;
@@ -564,6 +573,20 @@ mpm_status do_g2p(mpm_pool* p, float vmax, bool fuse_next, bool keys_next) {
 // First part of a substep (MPM.step's loop body up to the grid update and the coupling call,
 // mpm.js:591-596): sort when due, P2G, grid update, then the after-grid-update hook if one is set.
 mpm_status substep_front(mpm_pool* p, uint32_t index) {
+  if (p->p2g_presort) {  // P2G done ahead of this substep's sort (no hook is set): grid update, then sort
+    p->p2g_presort = false;
+    ST_TRY(do_p2g(p, false));  // consumes p2g_ahead; nothing is launched
+    if (p->profiling == 1) p->ev_presort.push_back(1);  // marks: grid, (P2G), sort
+    ST_TRY(mark(p));
+    ST_TRY(do_grid(p));
+    ST_TRY(mark(p));
+    ST_TRY(mark(p));
+    ST_TRY(do_bin(p, kBinReorderInG2P));  // the keys came with the P2G
+    // the new items' velocity boxes are their blocks' stencil boxes (written with the items)
+    p->tiles_fresh = true;
+    return MPM_OK;
+  }
+  if (p->profiling == 1) p->ev_presort.push_back(0);
   ST_TRY(mark(p));
   if (p->step_count % p->sort_every == 0) {
     ST_TRY(do_bin(p, kBinReorderInG2P));
@@ -595,21 +618,24 @@ mpm_status substep_front(mpm_pool* p, uint32_t index) {
 }
 
 // Rest of a substep: G2P (mpm.js:598-601).
-mpm_status substep_back(mpm_pool* p, float vmax, bool fuse_next = false, bool keys_next = false) {
+mpm_status substep_back(mpm_pool* p, float vmax, bool fuse_next = false, bool keys_next = false,
+                        bool presort_next = false) {
   ST_TRY(mark(p));
-  ST_TRY(do_g2p(p, vmax, fuse_next, keys_next));
+  ST_TRY(do_g2p(p, vmax, fuse_next, keys_next, presort_next));
   ST_TRY(mark(p));
   p->step_count++;
   return MPM_OK;
 }
 
 // One substep on the stream (MPM.step's loop body, mpm.js:591-601).
-mpm_status substep(mpm_pool* p, float vmax, uint32_t index = 0, bool more = false) {
+// more: another substep follows in this call. presort_ok: that substep runs eagerly (not from a
+// captured period), so its P2G may be done ahead of its sort.
+mpm_status substep(mpm_pool* p, float vmax, uint32_t index = 0, bool more = false, bool presort_ok = false) {
   ST_TRY(substep_front(p, index));
   // another substep follows in this call: fuse this G2P with its P2G when it does not sort, else
-  // let this G2P write the sort keys
+  // let this G2P write the sort keys (and, eagerly, do the P2G ahead of the sort as well)
   const bool next_sorts = (p->step_count + 1) % p->sort_every == 0;
-  return substep_back(p, vmax, more && !next_sorts, more && next_sorts);
+  return substep_back(p, vmax, more && !next_sorts, more && next_sorts, more && next_sorts && presort_ok);
 }
 
 // One sort period (sort_every substeps starting at a sort) captured once per parity of the current
@@ -650,13 +676,17 @@ mpm_status period_graph(mpm_pool* p, float vmax, hipGraphExec_t* out) {
 }
 
 mpm_status pool_step(mpm_pool* p, uint32_t substeps, float vmax) {
-  p->p2g_ahead = p->keys_ahead = false;  // only ever set for the next substep of the same call
+  p->p2g_ahead = p->keys_ahead = p->p2g_presort = false;  // only ever set for the next substep of the same call
   if (p->needs_repack) ST_TRY(repack(p));
   if (p->count == 0) return MPM_OK;  // mpm.js:584
   uint32_t k = 0;
   while (k < substeps) {
     const bool at_sort = p->step_count % p->sort_every == 0;
-    if (p->use_graphs && !p->profiling && !p->hook && at_sort && substeps - k >= p->sort_every) {
+    // a sort period from a captured graph (which starts from a plain sort)
+    auto graph_at = [&](uint64_t step, uint32_t left) {
+      return p->use_graphs && !p->profiling && !p->hook && step % p->sort_every == 0 && left >= p->sort_every;
+    };
+    if (graph_at(p->step_count, substeps - k) && !p->p2g_presort) {
       ST_TRY(clean_a(p));
       ST_TRY(apply_pending_reorder(p));
       p->keys_ahead = false;  // a captured period computes its own sort keys
@@ -674,7 +704,8 @@ mpm_status pool_step(mpm_pool* p, uint32_t substeps, float vmax) {
       p->step_count += p->sort_every;
       k += p->sort_every;
     } else {
-      ST_TRY(substep(p, vmax, k, k + 1 < substeps));
+      const bool more = k + 1 < substeps;
+      ST_TRY(substep(p, vmax, k, more, more && !graph_at(p->step_count + 1, substeps - k - 1)));
       k++;
     }
   }
@@ -1162,6 +1193,7 @@ mpm_status mpm_substep_begin(mpm_sim* s, const float g[3], float ff) {
   ST_TRY(set_step_params(p, g, ff));
   if (p->needs_repack) ST_TRY(repack(p));
   if (p->count == 0) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_substep_begin: no particles");
+  p->p2g_ahead = p->keys_ahead = p->p2g_presort = false;  // work ahead is only kept within one mpm_step
   ST_TRY(substep_front(p, 0));
   p->mid_substep = true;
   return MPM_OK;
@@ -1270,6 +1302,7 @@ mpm_status mpm_set_profiling(mpm_sim* s, int32_t enable) {
   p->profiling = enable;
   p->ev_used = 0;
   p->ev_fused.clear();
+  p->ev_presort.clear();
   for (float& m : p->phase_ms) m = 0.0f;
   p->kernel_ms[0] = p->kernel_ms[1] = 0.0f;
   p->kernel_launches[0] = p->kernel_launches[1] = 0;
@@ -1314,14 +1347,15 @@ mpm_status mpm_phase_times(mpm_sim* s, float ms[4]) {
   } else {
     const size_t groups = p->ev_used / 5;
     if (groups) HIP_TRY(hipEventSynchronize(p->ev_pool[groups * 5 - 1]));
-    for (size_t g = 0; g < groups; g++)
-      for (int ph = 0; ph < 4; ph++) {
-        float t = 0;
-        HIP_TRY(hipEventElapsedTime(&t, p->ev_pool[g * 5 + ph], p->ev_pool[g * 5 + ph + 1]));
-        p->phase_ms[ph] += t;
-      }
+    for (size_t g = 0; g < groups; g++) {
+      float t[4] = {0, 0, 0, 0};
+      for (int ph = 0; ph < 4; ph++) HIP_TRY(hipEventElapsedTime(&t[ph], p->ev_pool[g * 5 + ph], p->ev_pool[g * 5 + ph + 1]));
+      if (g < p->ev_presort.size() && p->ev_presort[g]) std::swap(t[0], t[2]);  // that group ran grid, P2G, sort
+      for (int ph = 0; ph < 4; ph++) p->phase_ms[ph] += t[ph];
+    }
   }
   p->ev_used = 0;
+  p->ev_presort.clear();
   for (int ph = 0; ph < 4; ph++) ms[ph] = p->phase_ms[ph];
   return MPM_OK;
 }
