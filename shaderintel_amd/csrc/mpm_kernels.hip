@@ -1176,9 +1176,12 @@ __device__ __forceinline__ int4 pull_tiles(const Pool& pool, uint32_t scene, int
   uint32_t k = 0xffffffffu, off = 0xffffffffu;
   if ((unsigned)cx < (unsigned)pb && (unsigned)cy < (unsigned)pb && (unsigned)cz < (unsigned)pb)
     k = pool.map[(scene << (pool.key_bits - 6)) + morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz)];
-  if (k != 0xffffffffu) off = pool.tile_off[k];
   int4 h = make_int4(0, 0, 0, 0);
-  if (off != 0xffffffffu) h = pool.tile_hdr[k];
+  if (k != 0xffffffffu) {  // both in one round trip; the header counts only when off is valid
+    off = pool.tile_off[k];
+    h = pool.tile_hdr[k];
+  }
+  if (off == 0xffffffffu) h = make_int4(0, 0, 0, 0);
   const int ex = h.w & 255, ey = (h.w >> 8) & 255, ez = (h.w >> 16) & 255;
   const bool hit = ex > 0 && h.x < 4 * g0 + 4 && h.x + ex > 4 * g0 && h.y < 4 * g1 + 4 && h.y + ey > 4 * g1 &&
                    h.z < 4 * g2 + 4 && h.z + ez > 4 * g2;
