@@ -166,8 +166,9 @@ mpm_status mpm_kernel_times(mpm_sim* sim, float ms[2], uint32_t launches[2]);
 #define MPM_TUNE_P2G_WAVES 4 /* occupancy target of the P2G kernel, 4..6 (default 4) */
 #define MPM_TUNE_G2P_ITEMS 5 /* 1: G2P walks P2G's work items with each item's velocity box in LDS
                               * (default); 0: one thread per particle, gathering from the grid */
-#define MPM_TUNE_FUSE 6      /* 1: G2P fused with the next substep's P2G when no sort comes between them
-                              * (default; needs MPM_TUNE_G2P_ITEMS); 0: separate launches */
+#define MPM_TUNE_FUSE 6      /* 1: G2P fused with the next substep's P2G within one mpm_step call, before
+                              * a sort too (over the current work items; default; needs
+                              * MPM_TUNE_G2P_ITEMS); 0: separate launches */
 mpm_status mpm_set_tuning(mpm_sim* sim, int32_t knob, int32_t value);
 
 /* ---- scene pools: many independent simulations advanced by the same launches ----
