@@ -889,14 +889,15 @@ struct ItemCursor {
 
 // The end of an item's P2G (after a barrier that follows the bbox atomics): the LDS window tile
 // (S^3 nodes from the lowest in-window base (wx, wy, wz), 4 planes) is stored over the bounding box
-// bbox of the in-window stencil bases (+ 2 nodes) in arena space, recorded in tile_off / tile_hdr; when
-// the arena has no room, the box's non-zero nodes are added to A with atomics instead (and their
-// node blocks stamped), which gives the same sums. The nodes read are cleared, which leaves the
-// whole tile zero for the next item (the stencils of the in-window particles lie in the box). Thread 0 takes the arena space and the
-// workgroup's next item (into s_item) with two atomics issued together: one round trip. Once an add
-// finds the region full, the workgroup stops adding to it (an add past the end takes nothing, so a
-// region's counter ends at most one tile per workgroup past its size). Every thread of the
-// workgroup calls it; s_item is read after the caller's next barrier.
+// bbox of the in-window stencil bases (+ 2 nodes) in arena space, recorded in tile_off / tile_hdr;
+// when the arena has no room, the box's non-zero nodes are added to A with atomics instead (and
+// their node blocks stamped), which gives the same sums. The nodes read are cleared, which leaves
+// the whole tile zero for the next item (the stencils of the in-window particles lie in the box).
+// Thread 0 takes the arena space and the workgroup's next item (into s_item) with two atomics
+// issued together: one round trip. Once an add finds the region full, the workgroup stops adding
+// to it (an add past the end takes nothing, so a region's counter ends at most one tile per
+// workgroup past its size). Every thread of the workgroup calls it; s_item is read after the
+// caller's next barrier.
 template <int THREADS>
 __device__ __forceinline__ void store_item_tile(const Pool& pool, uint32_t k, const SceneGeo& geo, int* tile,
                                                 const int* bbox, int wx, int wy, int wz, uint32_t epoch,
@@ -1382,8 +1383,8 @@ hipError_t launch_bin_emit(const Pool& pool, const uint32_t* keys, uint32_t n, c
   return hipGetLastError();
 }
 hipError_t launch_bin_transpose(const Pool& pool, const uint32_t* keys, const uint32_t* in, uint32_t* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_transpose, dim3((persistent_grid(pool.max_items, 1, 4096) + 7u) / 8u * 8u), dim3(kThreads), 0, s, pool, keys,
-                     in, out);
+  const unsigned grid = (persistent_grid(pool.max_items, 1, 4096) + 7u) / 8u * 8u;  // a multiple of 8
+  hipLaunchKernelGGL(k_bin_transpose, dim3(grid), dim3(kThreads), 0, s, pool, keys, in, out);
   return hipGetLastError();
 }
 hipError_t launch_bin_dilate(const Pool& pool, uint32_t max_scenes, hipStream_t s) {
@@ -1472,8 +1473,12 @@ hipError_t launch_g2p_p2g(const Pool& pool, const float4* src, float4* dst, cons
   const dim3 grid(persistent_grid(pool.max_items, 1, 2048));
   const int so = sorted_out ? 1 : 0;
   switch (waves) {
-    case 4: MPM_G2P_LAUNCH_T(k_g2p_p2g<4>, grid, kFuseThreads, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax, keys_out); break;
-    default: MPM_G2P_LAUNCH_T(k_g2p_p2g<3>, grid, kFuseThreads, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax, keys_out); break;
+    case 4:
+      MPM_G2P_LAUNCH_T(k_g2p_p2g<4>, grid, kFuseThreads, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax, keys_out);
+      break;
+    default:
+      MPM_G2P_LAUNCH_T(k_g2p_p2g<3>, grid, kFuseThreads, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax, keys_out);
+      break;
   }
   return hipGetLastError();
 }
