@@ -108,10 +108,6 @@ struct mpm_ctx {
 struct mpm_pool {
   mpm_ctx* ctx = nullptr;
   hipStream_t stream = nullptr;
-  // a second stream (default priority) for the grid update that runs beside a sort's radix passes,
-  // with the events that fork it from and join it to `stream`
-  hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // configuration
   uint32_t cap = 0, max_scenes = 0, max_blocks = 0, sort_every = 4;
   int max_dim = 0;
@@ -232,10 +228,6 @@ void free_pool(mpm_pool* p) {
   for (auto& t : p->templates)
     if (t.rec) hipFree(t.rec);
   for (hipEvent_t e : p->ev_pool) hipEventDestroy(e);
-  if (p->side) hipStreamSynchronize(p->side);
-  if (p->ev_fork) hipEventDestroy(p->ev_fork);
-  if (p->ev_join) hipEventDestroy(p->ev_join);
-  if (p->side) hipStreamDestroy(p->side);
   void* ptrs[] = {p->P[0],   p->P[1],     p->A,        p->V,        p->block_info, p->stamp,   p->dlist,
                   p->sstamp, p->slist,    p->map,      p->wl_start, p->wl_block,    p->tiles,   p->tile_hdr, p->tile_off,
                   p->mats,   p->cols,     p->scenes_d, p->ctr, p->arena_ctr,      p->occ,      p->keys[0],     p->keys[1], p->vals[0],
@@ -385,13 +377,7 @@ mpm_status apply_pending_reorder(mpm_pool* p) {
 // not at all (the sorted order is kept as a permutation), and rebuild work items, map and S.
 mpm_status repack(mpm_pool* p);
 
-// join: an event recorded on another stream, waited for after the radix passes and before the
-// map, S and items are rebuilt (work on that stream may still read the previous ones).
-mpm_status do_bin(mpm_pool* p, int mode, hipEvent_t join = nullptr) {
-  if (join && (p->needs_repack || p->count == 0)) {  // no radix passes to overlap: join first
-    HIP_TRY(hipStreamWaitEvent(p->stream, join, 0));
-    join = nullptr;
-  }
+mpm_status do_bin(mpm_pool* p, int mode) {
   if (p->needs_repack) {  // binning assumes every particle belongs to a live scene
     p->keys_ahead = false;
     ST_TRY(repack(p));
@@ -412,7 +398,6 @@ mpm_status do_bin(mpm_pool* p, int mode, hipEvent_t join = nullptr) {
   // by the transposition
   HIP_TRY(radix_sort_pairs(p->keys[0], p->vals[0], p->keys[1], p->vals[1], p->count, 6, (int)p->sort_bits,
                            p->sort_scratch, p->stream, &rk, &rv, true));
-  if (join) HIP_TRY(hipStreamWaitEvent(p->stream, join, 0));
   // a new binning epoch, S and the items reset, the map and the occupancy bitmap cleared
   HIP_TRY(launch_bin_reset(pv, p->map_per_scene * p->max_scenes, 2 * occ_words(p->map_per_scene) * p->max_scenes,
                            p->stream));
@@ -591,25 +576,12 @@ mpm_status substep_front(mpm_pool* p, uint32_t index) {
   if (p->p2g_presort) {  // P2G done ahead of this substep's sort (no hook is set): grid update, then sort
     p->p2g_presort = false;
     ST_TRY(do_p2g(p, false));  // consumes p2g_ahead; nothing is launched
-    if (p->profiling == 1) {  // in order, with marks: grid, (P2G), sort
-      p->ev_presort.push_back(1);
-      ST_TRY(mark(p));
-      ST_TRY(do_grid(p));
-      ST_TRY(mark(p));
-      ST_TRY(mark(p));
-      ST_TRY(do_bin(p, kBinReorderInG2P));  // the keys came with the P2G
-    } else {
-      // the grid update (through the current map, S and tiles) on the side stream, beside the
-      // sort's radix passes (keys and sort buffers only); the binning waits for it before it
-      // rebuilds the map, S and the items
-      HIP_TRY(hipEventRecord(p->ev_fork, p->stream));
-      HIP_TRY(hipStreamWaitEvent(p->side, p->ev_fork, 0));
-      HIP_TRY(launch_grid_update(view(p), p->grid_state == kGridTiles, p->side));
-      p->a_dirty_d = false;
-      p->grid_state = kGridVelocity;
-      HIP_TRY(hipEventRecord(p->ev_join, p->side));
-      ST_TRY(do_bin(p, kBinReorderInG2P, p->ev_join));
-    }
+    if (p->profiling == 1) p->ev_presort.push_back(1);  // marks: grid, (P2G), sort
+    ST_TRY(mark(p));
+    ST_TRY(do_grid(p));
+    ST_TRY(mark(p));
+    ST_TRY(mark(p));
+    ST_TRY(do_bin(p, kBinReorderInG2P));  // the keys came with the P2G
     // the new items' velocity boxes are their blocks' stencil boxes (written with the items)
     p->tiles_fresh = true;
     return MPM_OK;
@@ -767,9 +739,6 @@ mpm_status create_pool(mpm_ctx* c, uint32_t cap, uint32_t max_scenes, uint32_t m
   p->free_blocks.push_back({0u, max_blocks});
   const size_t nodes = (size_t)max_blocks * 64;
   hipError_t e = hipStreamCreate(&p->stream);
-  if (e == hipSuccess) e = hipStreamCreate(&p->side);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming);
   if (e == hipSuccess) e = dmalloc(&p->P[0], (size_t)cap * 8);
   if (e == hipSuccess) e = dmalloc(&p->P[1], (size_t)cap * 8);
   if (e == hipSuccess) e = dmalloc(&p->A, nodes);
