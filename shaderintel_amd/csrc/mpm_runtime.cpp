@@ -397,7 +397,7 @@ mpm_status do_bin(mpm_pool* p, int mode) {
   // by block (key bits 6 and up), values the current indices; the cell order inside a block is made
   // by the transposition
   HIP_TRY(radix_sort_pairs(p->keys[0], p->vals[0], p->keys[1], p->vals[1], p->count, 6, (int)p->sort_bits,
-                           p->sort_scratch, p->stream, &rk, &rv, true));
+                           p->sort_scratch, p->stream, &rk, &rv, true, true));
   // a new binning epoch, S and the items reset, the map and the occupancy bitmap cleared
   HIP_TRY(launch_bin_reset(pv, p->map_per_scene * p->max_scenes, 2 * occ_words(p->map_per_scene) * p->max_scenes,
                            p->stream));

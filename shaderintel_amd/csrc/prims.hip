@@ -212,14 +212,17 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
 // index order (an element's rank among equal digits of the chunk: a running LDS count per digit plus
 // its rank among the equal-digit lanes of its 64-element step, found with ballots); after the
 // per-digit offsets of the tile and of the earlier waves are known, the tile is staged in LDS in
-// sorted order and written out in runs of equal digits (coalesced).
+// sorted order and written out in runs of equal digits (coalesced). DIRECT: each element is written
+// straight to its place, without the staging (for input already nearly in digit order, where the
+// lanes of a step fall in few runs; less LDS, more workgroups per CU).
+template <bool DIRECT>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                                 const uint32_t* __restrict__ vin,
                                                                 uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                                 size_t n, int shift, int bits, uint32_t tiles,
                                                                 const uint32_t* __restrict__ hist, int vals_iota) {
-  __shared__ uint32_t skey[kSortTile];
-  __shared__ uint32_t sval[kSortTile];
+  __shared__ uint32_t skey[DIRECT ? 1 : kSortTile];
+  __shared__ uint32_t sval[DIRECT ? 1 : kSortTile];
   __shared__ uint32_t run[kSortWaves][kRadixMaxBuckets];  // chunk counts, then each wave's first local slot
   __shared__ uint32_t gofs[kRadixMaxBuckets];             // output index of local slot 0 of digit d
   __shared__ uint32_t wsum[kSortWaves];
@@ -300,6 +303,19 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
     }
   }
   __syncthreads();
+  if (DIRECT) {  // local slot s of digit d goes to gofs[d] + s
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
+      const uint32_t e = (uint32_t)(w * kSortChunk + j * 64 + lane);
+      if (e < count) {
+        const uint32_t d = (key[j] >> shift) & mask;
+        const uint32_t pos = gofs[d] + run[w][d] + rank[j];
+        kout[pos] = key[j];
+        vout[pos] = val[j];
+      }
+    }
+    return;
+  }
   // 3. stage the tile in sorted order
 #pragma unroll
   for (int j = 0; j < kSortItems; j++) {
@@ -355,7 +371,7 @@ hipError_t scan_exclusive(uint32_t* data, size_t n, uint32_t* scratch, hipStream
 
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, size_t n, int lo_bit,
                             int hi_bit, uint32_t* scratch, hipStream_t stream, uint32_t** result_keys,
-                            uint32_t** result_vals, bool vals_iota) {
+                            uint32_t** result_vals, bool vals_iota, bool nearly_sorted) {
   *result_keys = keys;
   *result_vals = vals;
   if (n == 0) return hipSuccess;
@@ -374,10 +390,14 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, 
                        (const uint32_t*)ka, n, shift, b, tiles, hist);
     hipError_t e = scan_exclusive(hist, ((size_t)1 << b) * tiles, scan_tmp, stream);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_radix_scatter, dim3((tiles + kXcds - 1) / kXcds * kXcds), dim3(kSortThreads), 0, stream,
-                       (const uint32_t*)ka,
-                       (const uint32_t*)va, kb, vb, n, shift, b, tiles, (const uint32_t*)hist,
-                       vals_iota && shift == lo_bit ? 1 : 0);
+    const dim3 grid((tiles + kXcds - 1) / kXcds * kXcds);
+    const int iota = vals_iota && shift == lo_bit ? 1 : 0;
+    if (nearly_sorted && shift == lo_bit)  // the first pass's input is the previous order
+      hipLaunchKernelGGL(k_radix_scatter<true>, grid, dim3(kSortThreads), 0, stream, (const uint32_t*)ka,
+                         (const uint32_t*)va, kb, vb, n, shift, b, tiles, (const uint32_t*)hist, iota);
+    else
+      hipLaunchKernelGGL(k_radix_scatter<false>, grid, dim3(kSortThreads), 0, stream, (const uint32_t*)ka,
+                         (const uint32_t*)va, kb, vb, n, shift, b, tiles, (const uint32_t*)hist, iota);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     uint32_t* t;
