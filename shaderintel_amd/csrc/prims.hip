@@ -392,7 +392,9 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, 
     if (e != hipSuccess) return e;
     const dim3 grid((tiles + kXcds - 1) / kXcds * kXcds);
     const int iota = vals_iota && shift == lo_bit ? 1 : 0;
-    if (nearly_sorted && shift == lo_bit)  // the first pass's input is the previous order
+    // nearly sorted input stays nearly sorted by the next digit after a stable pass (the elements of
+    // one full key stay together), so both passes can write directly
+    if (nearly_sorted)
       hipLaunchKernelGGL(k_radix_scatter<true>, grid, dim3(kSortThreads), 0, stream, (const uint32_t*)ka,
                          (const uint32_t*)va, kb, vb, n, shift, b, tiles, (const uint32_t*)hist, iota);
     else

@@ -19,7 +19,7 @@ hipError_t scan_exclusive(uint32_t* data, size_t n, uint32_t* scratch, hipStream
 // keys_alt / vals_alt are ping-pong buffers of n entries. On return *result_keys / *result_vals
 // point to whichever pair holds the sorted data. scratch: sort_scratch_words(n). vals_iota: the
 // input values are taken to be 0..n-1 (vals is not read, only used as a buffer). nearly_sorted:
-// the keys come nearly in order (a previous sort's order), so the first pass writes its elements
+// the keys come nearly in order (a previous sort's order), so each pass writes its elements
 // straight to their places (the results are the same either way).
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, size_t n, int lo_bit,
                             int hi_bit, uint32_t* scratch, hipStream_t stream, uint32_t** result_keys,
