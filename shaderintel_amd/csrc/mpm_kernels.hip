@@ -729,6 +729,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uin
                                                             uint32_t* __restrict__ out) {
   __shared__ uint32_t wrun[kBinWaves][64];  // per wave: cell counts, then the next rank per cell
   __shared__ uint32_t wcnt[kBinWaves][64];
+  __shared__ uint32_t wslot[kBinWaves][64];  // per wave: a lane of the step that has the cell
   __shared__ uint32_t srank[kBinWaves][kRankTable];
   __shared__ uint64_t mrank[kBinWaves][kRankTable];
   const int lane = threadIdx.x & 63;
@@ -766,11 +767,24 @@ __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uin
     }
     for (uint32_t j0 = 0; j0 < cnt; j0 += 64) {
       const uint32_t j = j0 + lane;
-      const uint64_t valid = __ballot(j < cnt);
-      const uint32_t c = j < cnt ? keys[start + j] & 63u : 0u;
-      const uint64_t peers = cell_peers(c, valid);
-      if (j < cnt) {
-        const uint32_t r = wrun[w][c] + (uint32_t)__popcll(peers & below);
+      const bool act = j < cnt;
+      const uint32_t c = act ? keys[start + j] & 63u : 0u;
+      // Are the step's cells distinct? Each lane writes its lane index to its cell's slot and reads
+      // it back (this wave's LDS accesses run in order); a lane that reads another index shares its
+      // cell. If none does, each rank is its cell's count so far (an LDS atomic per lane, all on
+      // different cells); otherwise the ballots below rank the lanes of each cell in lane order.
+      if (act) wslot[w][c] = (uint32_t)lane;
+      const bool clash = act && wslot[w][c] != (uint32_t)lane;
+      const bool distinct = __ballot(clash) == 0;
+      uint64_t peers = 0;
+      uint32_t r = 0;
+      if (distinct) {
+        if (act) r = atomicAdd(&wrun[w][c], 1u);
+      } else {
+        peers = cell_peers(c, __ballot(act));
+        if (act) r = wrun[w][c] + (uint32_t)__popcll(peers & below);
+      }
+      if (act) {
         uint32_t pos;
         if (r < (uint32_t)kRankTable) {
           pos = srank[w][r] + (uint32_t)__builtin_popcountll(mrank[w][r] & ((1ull << c) - 1ull));
@@ -783,8 +797,8 @@ __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uin
         }
         out[start + pos] = in[start + j];
       }
-      // every lane has read wrun[w][c]; the first lane of each cell advances it
-      if (j < cnt && (peers & below) == 0) wrun[w][c] += (uint32_t)__popcll(peers);
+      // (ballot path) every lane has read wrun[w][c]; the first lane of each cell advances it
+      if (!distinct && act && (peers & below) == 0) wrun[w][c] += (uint32_t)__popcll(peers);
     }
   }
 }
