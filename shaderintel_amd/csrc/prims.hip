@@ -256,7 +256,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
     rank[j] = 0;
     if (!valid) continue;
     const uint32_t d = (key[j] >> shift) & mask;
-    const uint64_t peers = digit_peers(d, bits, valid);
+    // the valid lanes are a prefix, so lane 0 is one; when every valid lane has its digit (runs of
+    // equal keys), they are all peers, else the ballots sort them out
+    const uint32_t d0 = (uint32_t)__builtin_amdgcn_readlane((int)d, 0);
+    const uint64_t peers = __ballot(e < count && d != d0) == 0 ? valid : digit_peers(d, bits, valid);
     if (e < count) rank[j] = run[w][d] + (uint32_t)__popcll(peers & below);
     // every lane has read run[w][d]; the first lane of each digit advances it
     if (e < count && (peers & below) == 0) run[w][d] += (uint32_t)__popcll(peers);
