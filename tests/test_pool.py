@@ -105,7 +105,7 @@ def test_pool_remove_and_readmit(ctx):
 def test_pool_graphs_match_eager(ctx):
     """Sort periods replayed from captured graphs give the eager launches' results (several replays,
     both buffer parities, with eager reads in between)."""
-    _say("[gpu] pool graphs vs eager, 24 substeps")
+    _say("[gpu] pool graphs vs eager, 56 substeps in calls of 8, 13 and 6")
     built = _build_specs()
     total = sum(len(b[2]) for b in built)
     nb = sum(node_blocks(b[0]) for b in built)
@@ -115,13 +115,16 @@ def test_pool_graphs_match_eager(ctx):
         pool.use_graphs(graphs)
         ids = [pool.add_scene(d, t, r, g) for d, t, r, g in built]
         pools.append((pool, ids))
-    for rep in range(3):
+    # calls of 8 (two periods), then 13 (three periods and an eager substep) and 6 (an eager period
+    # start after the sort phase moved): the eager pool does each sorting substep's P2G ahead of its
+    # sort, the graph pool replays plain periods; both give the same bits
+    for rep, n in enumerate((8, 8, 8, 13, 6, 13)):
         outs = []
         for pool, ids in pools:
-            pool.step(8)
+            pool.step(n)
             outs.append([pool.read_scene(i) for i in ids])
         for a, b in zip(*outs):
-            _same(b, a, f"graph replay {rep}")
+            _same(b, a, f"call {rep} ({n} substeps)")
     for pool, _ in pools:
         pool.destroy()
 
