@@ -158,7 +158,9 @@ struct mpm_pool {
   mpm_grid_hook hook = nullptr;  // called after every grid update (not with graphs)
   void* hook_user = nullptr;
   bool mid_substep = false;      // between mpm_substep_begin and mpm_substep_end
-  hipGraphExec_t period_graph[2] = {nullptr, nullptr};  // one sort period, by parity of cur
+  // one sort period, by parity of cur, whether its sort's P2G was done ahead (entry) and whether its
+  // last G2P does the next one's (exit)
+  hipGraphExec_t period_graph[2][2][2] = {};
   uint64_t graph_key = 0;                                // (count, scenes, vmax) the graphs were captured for
   // profiling 1: five events per profiled substep (sort, p2g, grid, g2p start; end);
   // profiling 2: two events per G2P launch, recorded by the dispatch itself (start, end)
@@ -213,11 +215,13 @@ Pool view(const mpm_pool* p) {
 }
 
 void drop_graphs(mpm_pool* p) {
-  for (auto& g : p->period_graph)
-    if (g) {
-      hipGraphExecDestroy(g);
-      g = nullptr;
-    }
+  for (auto& by_entry : p->period_graph)
+    for (auto& by_exit : by_entry)
+      for (auto& g : by_exit)
+        if (g) {
+          hipGraphExecDestroy(g);
+          g = nullptr;
+        }
 }
 
 void free_pool(mpm_pool* p) {
@@ -639,8 +643,10 @@ mpm_status substep(mpm_pool* p, float vmax, uint32_t index = 0, bool more = fals
 }
 
 // One sort period (sort_every substeps starting at a sort) captured once per parity of the current
-// particle buffer and replayed; valid while the pool's particle count, scenes and vmax do not change.
-mpm_status period_graph(mpm_pool* p, float vmax, hipGraphExec_t* out) {
+// particle buffer, entry (the sort's P2G already done by the G2P before it) and exit (the period's
+// last G2P does the next sort's P2G), and replayed; valid while the pool's particle count, scenes
+// and vmax do not change.
+mpm_status period_graph(mpm_pool* p, float vmax, bool exit_ahead, hipGraphExec_t* out) {
   uint32_t vbits;
   std::memcpy(&vbits, &vmax, 4);
   const uint64_t key = ((uint64_t)p->count << 32) ^ (uint64_t)vbits;
@@ -648,30 +654,49 @@ mpm_status period_graph(mpm_pool* p, float vmax, hipGraphExec_t* out) {
     drop_graphs(p);
     p->graph_key = key;
   }
-  const int parity = p->cur;
-  if (p->period_graph[parity]) {
-    *out = p->period_graph[parity];
+  const int parity = p->cur, entry = p->p2g_presort ? 1 : 0, exit = exit_ahead ? 1 : 0;
+  hipGraphExec_t& slot = p->period_graph[parity][entry][exit];
+  if (slot) {
+    *out = slot;
     return MPM_OK;
   }
   // capture one period from the current host-side state, then restore that state: the launch below
   // replays exactly what was captured
   const uint64_t step0 = p->step_count;
+  const bool ahead0 = p->p2g_ahead, keys0 = p->keys_ahead, presort0 = p->p2g_presort, fresh0 = p->tiles_fresh;
+  const bool dirty_d0 = p->a_dirty_d, dirty_s0 = p->a_dirty_s, bin0 = p->bin_valid, reorder0 = p->reorder_in_g2p;
+  const int grid0 = p->grid_state;
+  const uint32_t* const perm0 = p->perm;
   HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
   mpm_status st = MPM_OK;
-  for (uint32_t k = 0; k < p->sort_every && st == MPM_OK; k++) st = substep(p, vmax, k, k + 1 < p->sort_every);
+  for (uint32_t k = 0; k < p->sort_every && st == MPM_OK; k++) {
+    const bool last = k + 1 == p->sort_every;
+    st = substep(p, vmax, k, !last || exit_ahead, last && exit_ahead);
+  }
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(p->stream, &g);
   p->step_count = step0;
   p->cur = parity;
+  p->p2g_ahead = ahead0;
+  p->keys_ahead = keys0;
+  p->p2g_presort = presort0;
+  p->tiles_fresh = fresh0;
+  p->a_dirty_d = dirty_d0;
+  p->a_dirty_s = dirty_s0;
+  p->bin_valid = bin0;
+  p->reorder_in_g2p = reorder0;
+  p->grid_state = grid0;
+  p->perm = perm0;
+  recompute_offsets(p);
   if (st != MPM_OK) {
     if (g) hipGraphDestroy(g);
     return st;
   }
   if (e != hipSuccess) return from_hip(e, "hipStreamEndCapture");
-  e = hipGraphInstantiate(&p->period_graph[parity], g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&slot, g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
   if (e != hipSuccess) return from_hip(e, "hipGraphInstantiate");
-  *out = p->period_graph[parity];
+  *out = slot;
   return MPM_OK;
 }
 
@@ -682,30 +707,34 @@ mpm_status pool_step(mpm_pool* p, uint32_t substeps, float vmax) {
   uint32_t k = 0;
   while (k < substeps) {
     const bool at_sort = p->step_count % p->sort_every == 0;
-    // a sort period from a captured graph (which starts from a plain sort)
-    auto graph_at = [&](uint64_t step, uint32_t left) {
-      return p->use_graphs && !p->profiling && !p->hook && step % p->sort_every == 0 && left >= p->sort_every;
-    };
-    if (graph_at(p->step_count, substeps - k) && !p->p2g_presort) {
-      ST_TRY(clean_a(p));
-      ST_TRY(apply_pending_reorder(p));
-      p->keys_ahead = false;  // a captured period computes its own sort keys
+    // a sort period from a captured graph
+    if (p->use_graphs && !p->profiling && !p->hook && at_sort && substeps - k >= p->sort_every) {
+      // the period's last G2P does the next sort's P2G when another substep follows in this call
+      const bool exit_ahead = substeps - k > p->sort_every;
+      if (!p->p2g_presort) {  // (otherwise A holds part of the P2G done ahead, and its keys are set)
+        ST_TRY(clean_a(p));
+        ST_TRY(apply_pending_reorder(p));
+        p->keys_ahead = false;  // the captured period computes its own sort keys
+      }
       hipGraphExec_t ge = nullptr;
-      ST_TRY(period_graph(p, vmax, &ge));
+      ST_TRY(period_graph(p, vmax, exit_ahead, &ge));
       HIP_TRY(hipGraphLaunch(ge, p->stream));
       // host-side effects of one period: the sort's reorder swapped the buffers once
       p->cur ^= 1;
       p->perm = nullptr;
       p->reorder_in_g2p = false;
       p->bin_valid = true;
-      p->grid_state = kGridVelocity;
-      p->a_dirty_d = p->a_dirty_s = false;
+      p->a_dirty_s = false;
+      p->grid_state = exit_ahead ? kGridTiles : kGridVelocity;
+      p->a_dirty_d = exit_ahead;
+      p->tiles_fresh = exit_ahead;
+      p->p2g_ahead = p->keys_ahead = p->p2g_presort = exit_ahead;
       recompute_offsets(p);
       p->step_count += p->sort_every;
       k += p->sort_every;
     } else {
       const bool more = k + 1 < substeps;
-      ST_TRY(substep(p, vmax, k, more, more && !graph_at(p->step_count + 1, substeps - k - 1)));
+      ST_TRY(substep(p, vmax, k, more, more));
       k++;
     }
   }
