@@ -1109,6 +1109,12 @@ __global__ __launch_bounds__(kFuseThreads) __attribute__((amdgpu_waves_per_eu(WA
     for (; li < cnt; li += kFuseThreads) {
       const uint32_t j = start + li;
       g2p_base(in);
+      uint32_t si_next = 0;
+      G2PIn in_next;
+      if (li + kFuseThreads < cnt) {  // the next record of this thread, loaded while this one runs
+        si_next = perm ? perm[j + kFuseThreads] : j + kFuseThreads;
+        g2p_fetch(src, si_next, pool.cap, in_next);
+      }
       const bool inbox = nodes > 0 && (unsigned)(in.bx - ox) <= (unsigned)(ex - 3) &&
                          (unsigned)(in.by - oy) <= (unsigned)(ey - 3) && (unsigned)(in.bz - oz) <= (unsigned)(ez - 3);
       f3 v;
@@ -1146,10 +1152,8 @@ __global__ __launch_bounds__(kFuseThreads) __attribute__((amdgpu_waves_per_eu(WA
       } else {
         fallback_particle(pool, r, epoch, true, bin_epoch);
       }
-      if (li + kFuseThreads < cnt) {  // the next record of this thread
-        si = perm ? perm[j + kFuseThreads] : j + kFuseThreads;
-        g2p_fetch(src, si, pool.cap, in);
-      }
+      in = in_next;
+      si = si_next;
     }
     lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
     hi0 = wave_max(hi0); hi1 = wave_max(hi1); hi2 = wave_max(hi2);
