@@ -1021,7 +1021,10 @@ mpm_status mpm_ctx_create(int device, mpm_ctx** out) {
   mpm_ctx* c = new (std::nothrow) mpm_ctx;
   if (!c) return fail(MPM_ERR_OUT_OF_MEMORY, "mpm_ctx_create: host allocation");
   c->device = device;
-  std::snprintf(c->name, sizeof c->name, "%s (%s)", p.name, p.gcnArchName);
+  if (p.name[0])  // the marketing name can be empty (then the device index stands in)
+    std::snprintf(c->name, sizeof c->name, "%s (%s)", p.name, p.gcnArchName);
+  else
+    std::snprintf(c->name, sizeof c->name, "HIP device %d (%s)", device, p.gcnArchName);
   c->cus = p.multiProcessorCount;
   e = hipStreamCreate(&c->stream);
   if (e != hipSuccess) {
