@@ -12,6 +12,9 @@
 // (fma(m2, v.z, fma(m1, v.y, m0*v.x))), cross products as fma(a, b, -(c*d)), G2P's v += wt*gv and
 // B += outer(wt*gv, dpos) contracted element by element, and no other contraction (explicit fmaf
 // calls; the compiler contracts nothing: -ffp-contract=off), IEEE division and sqrt,
+// P2G's per-node momentum exactly as mpm.js:330-335 writes it (tests/test_oracle_literal.py checks
+// every i32 word against an independent node-by-node numpy reading), G2P's 27-node sums taken per
+// row of three nodes (within the a priori rounding bound of the node-by-node readings; same test),
 // WGSL clamp(e,lo,hi) = min(max(e,lo),hi) with NaN-discarding min/max, WGSL round = rint (half to
 // even). inverseSqrt is rsqrt_1to2 (it is used on [1, 2] only); log/exp are the self-contained
 // routines below.
@@ -237,7 +240,6 @@ void svd3(const M3& F, M3& U, V3& s, M3& V) {  // linalg.js:46-69
 constexpr float FP = 4194304.0f;
 constexpr float INV_FP = 1.0f / 4194304.0f;
 inline int32_t to_fixed(float x) { return (int32_t)rintf(clampw(x * FP, -2.0e9f, 2.0e9f)); }
-inline int32_t to_fixed_scaled(float xs) { return (int32_t)rintf(clampw(xs, -2.0e9f, 2.0e9f)); }  // x * 2^22 given
 inline void bspline(V3 fx, V3 w[3]) {
   V3 t0 = sub(v3(1.5f, 1.5f, 1.5f), fx);
   V3 t1 = sub(fx, v3(1.0f, 1.0f, 1.0f));
@@ -290,28 +292,17 @@ inline void p2g_particle(const float* p, const float* mats, float vol4, AddFn&& 
   M3 C = {{{p[8], p[9], p[10]}, {p[12], p[13], p[14]}, {p[16], p[17], p[18]}}};
   M3 affine = madd(mscale(-vol4, tau), mscale(mass, C));
   V3 mv = scale(mass, v3(p[4], p[5], p[6]));
-  // mom = wt * (mv + affine * dpos), dpos = (a, b, c) - fx, with the sum evaluated as
-  // ((u + a A0) + b A1) + c A2 where u = mv - affine * fx and A0..A2 are the columns of affine (1 A = A
-  // and 2 A are exact; a term with a zero factor is left out). toFixed(wt * q) is computed as
-  // rint(clamp(wts * q)) with wts = (wx wy)(wz 2^22): the power-of-two scaling is exact, so this is
-  // toFixed's value for every normal product (a subnormal wt q rounds to 0 either way).
-  V3 u = sub(mv, mulmv(affine, fx));
-  V3 A0 = affine.c[0], A1 = affine.c[1], A2 = affine.c[2];
-  V3 Ua[3] = {u, add(u, A0), add(u, scale(2.0f, A0))};
-  float wzs[3] = {w[0].z * FP, w[1].z * FP, w[2].z * FP};
+  // per node, as mpm.js:330-335 writes it: dpos = vec3f(off) - fx, wt = w[a].x * w[b].y * w[c].z,
+  // mom = wt * (mv + affine * dpos), q = (toFixed(mom), toFixed(wt * mass))
   for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++) {
-      float wxy = w[a].x * w[b].y;
-      V3 Uab = b == 0 ? Ua[a] : add(Ua[a], b == 1 ? A1 : scale(2.0f, A1));
+    for (int b = 0; b < 3; b++)
       for (int c = 0; c < 3; c++) {
-        V3 U = c == 0 ? Uab : add(Uab, c == 1 ? A2 : scale(2.0f, A2));
-        float wts = wxy * wzs[c];
-        V3 mom = scale(wts, U);
-        int32_t q[4] = {to_fixed_scaled(mom.x), to_fixed_scaled(mom.y), to_fixed_scaled(mom.z),
-                        to_fixed_scaled(wts * mass)};
+        V3 dpos = sub(v3((float)a, (float)b, (float)c), fx);
+        float wt = (w[a].x * w[b].y) * w[c].z;
+        V3 mom = scale(wt, add(mv, mulmv(affine, dpos)));
+        int32_t q[4] = {to_fixed(mom.x), to_fixed(mom.y), to_fixed(mom.z), to_fixed(wt * mass)};
         addq(base[0] + a, base[1] + b, base[2] + c, q);
       }
-    }
 }
 
 void p2g(Scene& S) {  // mpm.js:607-612 (clear + dispatch), scatter mpm.js:318-349

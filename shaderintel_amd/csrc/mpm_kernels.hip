@@ -101,24 +101,27 @@ __device__ __forceinline__ void p2g_particle(const float4 r[kP2GStreams], const 
   const m3 C = mat(mk(r[2].x, r[2].y, r[2].z), mk(r[3].x, r[3].y, r[3].z), mk(r[4].x, r[4].y, r[4].z));
   const m3 affine = (-vol4) * tau + mass * C;
   const f3 mv = mass * mk(r[1].x, r[1].y, r[1].z);
-  // q(a, b, c) = mv + affine (dpos) with dpos = (a, b, c) - fx, evaluated as
-  // ((u + a A0) + b A1) + c A2 with u = mv - affine fx and A0..A2 the columns of affine (a A = A, 2A
-  // exactly; a term with a zero factor is left out): the same sum with its rounding fixed this way.
-  const f3 u = mv - mul(affine, fx);
+  // q(a, b, c) = mv + affine * dpos with dpos = vec3f(a, b, c) - fx (mpm.js:330-334), the mat*vec
+  // evaluated as fma(A2, dz_c, fma(A1, dy_b, A0 * dx_a)) (mul's order) with its partial products
+  // shared between nodes: A0 * dx_a once per a, the first fma once per (a, b). Node by node the
+  // same operations on the same operands as the per-node expression.
   const f3 A0 = affine.c[0], A1 = affine.c[1], A2 = affine.c[2];
-  const f3 Ua[3] = {u, u + A0, u + 2.0f * A0};
+  const float dx[3] = {0.0f - fx.x, 1.0f - fx.x, 2.0f - fx.x};
+  const float dy[3] = {0.0f - fx.y, 1.0f - fx.y, 2.0f - fx.y};
+  const float dz[3] = {0.0f - fx.z, 1.0f - fx.z, 2.0f - fx.z};
   const float wzs[3] = {w[0].z * kFP, w[1].z * kFP, w[2].z * kFP};  // 2^22 folded in (exact)
 #pragma unroll
-  for (int a = 0; a < 3; a++)
+  for (int a = 0; a < 3; a++) {
+    const f3 ta = mk(A0.x * dx[a], A0.y * dx[a], A0.z * dx[a]);
 #pragma unroll
     for (int b = 0; b < 3; b++) {
       const float wx = a == 0 ? w[0].x : (a == 1 ? w[1].x : w[2].x);
       const float wy = b == 0 ? w[0].y : (b == 1 ? w[1].y : w[2].y);
       const float wxy = wx * wy;
-      const f3 Uab = b == 0 ? Ua[a] : Ua[a] + (b == 1 ? A1 : 2.0f * A1);
+      const f3 tab = mk(fmaf(A1.x, dy[b], ta.x), fmaf(A1.y, dy[b], ta.y), fmaf(A1.z, dy[b], ta.z));
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const f3 U = c == 0 ? Uab : Uab + (c == 1 ? A2 : 2.0f * A2);
+        const f3 U = mv + mk(fmaf(A2.x, dz[c], tab.x), fmaf(A2.y, dz[c], tab.y), fmaf(A2.z, dz[c], tab.z));
         // wts = wt * 2^22 with wt = (wx wy) wz: toFixed(wt q) = rint(clamp(wts q)) since the power-of-
         // two scaling is exact (a subnormal wt q rounds to 0 either way; overflow clamps alike)
         const float wts = wxy * wzs[c];
@@ -131,6 +134,7 @@ __device__ __forceinline__ void p2g_particle(const float4 r[kP2GStreams], const 
               (int32_t)rintf(wts * mass));
       }
     }
+  }
 }
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -770,11 +774,17 @@ __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uin
       const bool act = j < cnt;
       const uint32_t c = act ? keys[start + j] & 63u : 0u;
       // Are the step's cells distinct? Each lane writes its lane index to its cell's slot and reads
-      // it back (this wave's LDS accesses run in order); a lane that reads another index shares its
-      // cell. If none does, each rank is its cell's count so far (an LDS atomic per lane, all on
-      // different cells); otherwise the ballots below rank the lanes of each cell in lane order.
-      if (act) wslot[w][c] = (uint32_t)lane;
-      const bool clash = act && wslot[w][c] != (uint32_t)lane;
+      // it back: the wave's one store instruction completes for every lane before its load
+      // instruction reads (LDS instructions of a wave execute in order), so a lane that reads
+      // another index shares its cell with that lane. The slot is accessed as volatile: another
+      // lane's store is invisible to the compiler, which would otherwise forward each lane's own
+      // store to its load and drop the check. If no lane clashes, each rank is its cell's count so
+      // far (an LDS atomic per lane, all on different addresses, so their order does not matter);
+      // otherwise the ballots below rank the lanes of each cell in lane order.
+      volatile __attribute__((address_space(3))) uint32_t* const slot =
+          (volatile __attribute__((address_space(3))) uint32_t*)wslot[w];
+      if (act) slot[c] = (uint32_t)lane;
+      const bool clash = act && slot[c] != (uint32_t)lane;
       const bool distinct = __ballot(clash) == 0;
       uint64_t peers = 0;
       uint32_t r = 0;

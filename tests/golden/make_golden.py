@@ -1,6 +1,9 @@
-"""Writes tests/golden/mpm_small.npz: small input / output vectors of the MLS-MPM substep made by the
-oracle (the CPU restatement under oracle/). The CPU suite checks that the oracle still reproduces
-them; the GPU suite checks the HIP path against them without the oracle.
+"""Writes tests/golden/mpm_small.npz: small input / output vectors of the MLS-MPM substep. The i32
+P2G words come from tests/literal.py, the node-by-node numpy restatement of mpm.js:318-349 that
+shares no code with the oracle, and are checked equal to the oracle's before anything is written;
+the records after 1 and 6 substeps and the grid after the 6th update come from the oracle (the CPU
+restatement under oracle/). The CPU suite checks that the oracle still reproduces them; the GPU
+suite checks the HIP path against them without the oracle.
 
 Scene: 16^3 grid, ppc 2, bound 3, sort every 4; water, jelly, snow and sand boxes (2,304
 particles) with randomized v, C, tau and a sheared F; lattice gravity (0, -2.725e-3, 0).
@@ -18,6 +21,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "..", "..", "oracle"))
 import oracle_mpm as O  # noqa: E402
+
+sys.path.insert(0, os.path.join(HERE, ".."))
+import literal  # noqa: E402
 
 MATS = [{"kind": 0, "density": 1.0, "stiffness": 1.0, "viscosity": 0.0},
         {"kind": 1, "density": 1.0, "stiffness": 0.35, "poisson": 0.3},
@@ -72,6 +78,10 @@ def run(rec, tbl, g):
 def main():
     rec, tbl, g = build()
     grid_p2g, rec1, rec6, grid6 = run(rec, tbl, g)
+    per_node = literal.p2g(rec, tbl, 0.5, DIMS)
+    if not np.array_equal(per_node, grid_p2g):
+        raise SystemExit(f"oracle P2G differs from the per-node restatement in {int((per_node != grid_p2g).sum())} words")
+    grid_p2g = per_node
     np.savez_compressed(os.path.join(HERE, "mpm_small.npz"), rec0=rec, materials=tbl, g=g, grid_p2g=grid_p2g,
                         rec1=rec1, rec6=rec6, grid6=grid6)
     print(f"wrote mpm_small.npz: {len(rec)} particles")
