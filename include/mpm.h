@@ -278,6 +278,16 @@ void mpm_server_destroy(mpm_server* srv);
 /* Scan.run(src, dst, n) — src/prims/scan.js:292-306: exclusive prefix sum of n u32 (wrapping).
  * Host arrays in and out (may alias). */
 mpm_status mpm_scan_u32(mpm_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n);
+/* No counterpart in the reference: mpm_scan_u32 for callers that cannot allocate device memory
+ * (work under HIP graph capture, tight serving loops, harnesses that forbid allocation). in and out
+ * are DEVICE arrays of n u32 (out may equal in); the scan is enqueued on `stream` (a hipStream_t;
+ * NULL is the null stream) and the call returns without waiting. It calls no allocator: the tile
+ * partial sums live in a fixed array in each device's memory (257 KiB, held until the process
+ * ends), so n is at most MPM_SCAN_NOALLOC_MAX (larger n: MPM_ERR_INVALID_ARGUMENT). Every call on
+ * one device shares that array: calls that overlap on one device (other streams or threads) race,
+ * and must be ordered by the caller. */
+#define MPM_SCAN_NOALLOC_MAX (1ull << 28)
+mpm_status mpm_scan_u32_noalloc(mpm_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, void* stream);
 /* SpatialGrid.build — src/prims/grid.js:70-79: counting sort of n keys in [0, num_buckets):
  * start[k] = first slot of bucket k (num_buckets + 1 entries, start[num_buckets] = n) and
  * order[start[k] + r] = index of the r-th element (in index order) with key k. */

@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <utility>
@@ -103,7 +104,37 @@ struct mpm_ctx {
   char name[256] = {0};
   int cus = 0;
   hipStream_t stream = nullptr;
+  uint32_t* scratch = nullptr;  // this device's address of `scratch` (below)
 };
+
+namespace {
+
+// Scratch of mpm_scan_u32_noalloc (the scan's tile partial sums): a fixed array in the memory of
+// every device the library runs on, held until the process ends, so that entry point calls no
+// allocator. In host code `scratch` names a host-side handle, not a device address: its address
+// on a device comes from hipGetSymbolAddress, taken once per device at mpm_ctx_create with that
+// device current, and only that address is passed to kernels.
+constexpr uint64_t kNoAllocScanMax = 1ull << 28;  // elements (MPM_SCAN_NOALLOC_MAX)
+constexpr size_t kScratchWords = 65792;          // >= scan_scratch_words(kNoAllocScanMax) = 65556
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+std::mutex g_scratch_mu;
+std::vector<uint32_t*> g_scratch_addr;  // per device index, nullptr until resolved
+
+// This device's address of `scratch` (the device is current).
+hipError_t scratch_address(int device, uint32_t** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((size_t)device >= g_scratch_addr.size()) g_scratch_addr.resize((size_t)device + 1, nullptr);
+  if (!g_scratch_addr[(size_t)device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_addr[(size_t)device] = static_cast<uint32_t*>(p);
+  }
+  *out = g_scratch_addr[(size_t)device];
+  return hipSuccess;
+}
+
+}  // namespace
 
 struct mpm_pool {
   mpm_ctx* ctx = nullptr;
@@ -1018,9 +1049,12 @@ mpm_status mpm_ctx_create(int device, mpm_ctx** out) {
   if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
     return fail(MPM_ERR_UNSUPPORTED, "mpm_ctx_create: device %d is %s; this library is built for gfx950 only", device,
                 p.gcnArchName);
+  uint32_t* scr = nullptr;
+  HIP_TRY(scratch_address(device, &scr));
   mpm_ctx* c = new (std::nothrow) mpm_ctx;
   if (!c) return fail(MPM_ERR_OUT_OF_MEMORY, "mpm_ctx_create: host allocation");
   c->device = device;
+  c->scratch = scr;
   if (p.name[0])  // the marketing name can be empty (then the device index stands in)
     std::snprintf(c->name, sizeof c->name, "%s (%s)", p.name, p.gcnArchName);
   else
@@ -1646,6 +1680,19 @@ mpm_status mpm_scan_u32(mpm_ctx* c, const uint32_t* in, uint32_t* out, uint64_t 
   if (d) hipFree(d);
   if (tmp) hipFree(tmp);
   return from_hip(e, "mpm_scan_u32");
+}
+
+mpm_status mpm_scan_u32_noalloc(mpm_ctx* c, const uint32_t* in, uint32_t* out, uint64_t n, void* stream) {
+  if (!c) return fail(MPM_ERR_INVALID_ARGUMENT, "context is NULL");
+  if (n == 0) return MPM_OK;
+  if (!in || !out) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_scan_u32_noalloc: NULL array");
+  if (n > kNoAllocScanMax || scan_scratch_words(n) > kScratchWords)
+    return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_scan_u32_noalloc: n = %llu exceeds the maximum %llu (2^28)",
+                (unsigned long long)n, (unsigned long long)kNoAllocScanMax);
+  if (!c->scratch) return fail(MPM_ERR_ERROR, "mpm_scan_u32_noalloc: no scratch address for device %d", c->device);
+  ST_TRY(set_dev(c));
+  HIP_TRY(scan_exclusive_to(in, out, n, c->scratch, static_cast<hipStream_t>(stream)));
+  return MPM_OK;
 }
 
 mpm_status mpm_counting_sort(mpm_ctx* c, const uint32_t* keys, uint64_t n, uint32_t num_buckets, uint32_t* order,

@@ -68,16 +68,17 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __
   if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
-// Each thread scans 16 contiguous items. Tile offsets: none (use_offsets 0, a single tile), the
-// scanned partials (1), or the sum of the unscanned partials of the earlier tiles, added up here (2;
-// at most kScanTile tiles, so each thread adds at most kScanItems of them).
-__global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict__ data, size_t n,
+// Each thread scans 16 contiguous items of src into dst (the same array for an in-place scan; a
+// thread reads its items before it writes them). Tile offsets: none (use_offsets 0, a single tile),
+// the scanned partials (1), or the sum of the unscanned partials of the earlier tiles, added up here
+// (2; at most kScanTile tiles, so each thread adds at most kScanItems of them).
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t* src, uint32_t* dst, size_t n,
                                                             const uint32_t* __restrict__ partials, int use_offsets) {
   const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
   uint32_t s = 0;
-  if (base + kScanItems <= n && (((uintptr_t)(data + base)) & 15) == 0) {
-    const uint4* p = reinterpret_cast<const uint4*>(data + base);
+  if (base + kScanItems <= n && (((uintptr_t)(src + base)) & 15) == 0) {
+    const uint4* p = reinterpret_cast<const uint4*>(src + base);
 #pragma unroll
     for (int j = 0; j < kScanItems / 4; j++) {
       uint4 q = p[j];
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict
     }
   } else {
 #pragma unroll
-    for (int j = 0; j < kScanItems; j++) v[j] = (base + j < n) ? data[base + j] : 0u;
+    for (int j = 0; j < kScanItems; j++) v[j] = (base + j < n) ? src[base + j] : 0u;
   }
 #pragma unroll
   for (int j = 0; j < kScanItems; j++) s += v[j];
@@ -105,34 +106,34 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict
     v[j] = acc;
     acc += a;
   }
-  if (base + kScanItems <= n && (((uintptr_t)(data + base)) & 15) == 0) {
-    uint4* p = reinterpret_cast<uint4*>(data + base);
+  if (base + kScanItems <= n && (((uintptr_t)(dst + base)) & 15) == 0) {
+    uint4* p = reinterpret_cast<uint4*>(dst + base);
 #pragma unroll
     for (int j = 0; j < kScanItems / 4; j++) p[j] = make_uint4(v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]);
   } else {
 #pragma unroll
     for (int j = 0; j < kScanItems; j++)
-      if (base + j < n) data[base + j] = v[j];
+      if (base + j < n) dst[base + j] = v[j];
   }
 }
 
-hipError_t scan_level(uint32_t* data, size_t n, uint32_t* scratch, hipStream_t stream) {
+// Exclusive scan of n words of src into dst (may be the same array); the tile partials go to scratch.
+hipError_t scan_level(const uint32_t* src, uint32_t* dst, size_t n, uint32_t* scratch, hipStream_t stream) {
   const size_t tiles = (n + kScanTile - 1) / kScanTile;
   if (tiles <= 1) {
-    hipLaunchKernelGGL(k_scan_down, dim3(1), dim3(kScanThreads), 0, stream, data, n, (const uint32_t*)nullptr, 0);
+    hipLaunchKernelGGL(k_scan_down, dim3(1), dim3(kScanThreads), 0, stream, src, dst, n, (const uint32_t*)nullptr, 0);
     return hipGetLastError();
   }
   uint32_t* partials = scratch;
-  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)tiles), dim3(kScanThreads), 0, stream, (const uint32_t*)data, n,
-                     partials);
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)tiles), dim3(kScanThreads), 0, stream, src, n, partials);
   if (tiles <= (size_t)kScanTile) {  // each tile adds up the partials before it: no third launch
-    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)tiles), dim3(kScanThreads), 0, stream, data, n,
+    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)tiles), dim3(kScanThreads), 0, stream, src, dst, n,
                        (const uint32_t*)partials, 2);
     return hipGetLastError();
   }
-  hipError_t e = scan_level(partials, tiles, scratch + ((tiles + 3) & ~(size_t)3), stream);
+  hipError_t e = scan_level(partials, partials, tiles, scratch + ((tiles + 3) & ~(size_t)3), stream);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)tiles), dim3(kScanThreads), 0, stream, data, n,
+  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)tiles), dim3(kScanThreads), 0, stream, src, dst, n,
                      (const uint32_t*)partials, 1);
   return hipGetLastError();
 }
@@ -369,7 +370,12 @@ size_t sort_scratch_words(size_t n) {
 
 hipError_t scan_exclusive(uint32_t* data, size_t n, uint32_t* scratch, hipStream_t stream) {
   if (n == 0) return hipSuccess;
-  return scan_level(data, n, scratch, stream);
+  return scan_level(data, data, n, scratch, stream);
+}
+
+hipError_t scan_exclusive_to(const uint32_t* src, uint32_t* dst, size_t n, uint32_t* scratch, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  return scan_level(src, dst, n, scratch, stream);
 }
 
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, size_t n, int lo_bit,

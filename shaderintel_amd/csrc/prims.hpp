@@ -13,6 +13,8 @@ size_t sort_scratch_words(size_t n);
 
 // In-place exclusive prefix sum of n u32 (wrapping) on `stream`. scratch: scan_scratch_words(n).
 hipError_t scan_exclusive(uint32_t* data, size_t n, uint32_t* scratch, hipStream_t stream);
+// The same from src into dst (dst may equal src; src is not written otherwise).
+hipError_t scan_exclusive_to(const uint32_t* src, uint32_t* dst, size_t n, uint32_t* scratch, hipStream_t stream);
 
 // Stable LSD radix sort of (keys, vals) by key bits [lo_bit, hi_bit) (bits outside the range do not
 // take part: equal in-range bits keep the input order), in passes of up to 9 bits.
