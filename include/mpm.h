@@ -231,6 +231,30 @@ typedef struct mpm_diagnostics {
 mpm_status mpm_pool_diagnostics(mpm_pool* pool, mpm_diagnostics* out);
 mpm_status mpm_diagnostics_of(mpm_sim* sim, mpm_diagnostics* out);
 
+/* Momentum bookkeeping of the grid update (no counterpart in the reference: the mass and momentum
+ * law checkMPM states for P2G, src/verify/index.js:175-193, carried through the grid update,
+ * mpm.js:428-470). While enabled, every grid update adds, over the nodes with mass (p = the node's
+ * P2G momentum words, m and v = its mass and velocity after the update, vf = p/m + g = its velocity
+ * before the walls, floor and colliders act, all f32 as the update computes them), the sums below
+ * as wrapping 64-bit integers, so they do not depend on the order the GPU takes them in. Per
+ * substep: sum(m v) - sum(p) = sum(m g) + boundary impulse, up to f32 rounding. */
+typedef struct mpm_grid_law {
+  int64_t mass_words;  /* sum of P2G mass words (i32 fixed point, 2^22 per unit) */
+  int64_t p2g[3];      /* sum of P2G momentum words (2^22) */
+  int64_t momentum[3]; /* sum of m v after the update, each f32 product as rint(x * 2^32) */
+  int64_t gravity[3];  /* sum of m g (2^32) */
+  int64_t boundary[3]; /* sum of m (v - vf): the impulse of walls, floor friction and colliders (2^32) */
+  uint64_t nodes;      /* node updates with mass */
+  uint64_t updates;    /* grid updates summed */
+} mpm_grid_law;
+/* enable = 1 starts the sums from zero (and restarts them when already on); 0 stops them. */
+mpm_status mpm_set_grid_law(mpm_sim* sim, int32_t enable);
+/* Blocking read of the sums since they were started or last reset (reset = 1 zeroes them). */
+mpm_status mpm_read_grid_law(mpm_sim* sim, mpm_grid_law* out, int32_t reset);
+/* The same over all scenes of a pool. */
+mpm_status mpm_pool_set_grid_law(mpm_pool* pool, int32_t enable);
+mpm_status mpm_pool_read_grid_law(mpm_pool* pool, mpm_grid_law* out, int32_t reset);
+
 /* Times every occupancy variant of G2P and P2G for trial_substeps substeps each (rounded up to whole
  * sort periods) with HIP events and keeps the fastest; *g2p_waves and *p2g_waves receive it. The
  * trials are real steps of the pool (every variant gives identical results), so the pool advances

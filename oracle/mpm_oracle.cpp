@@ -271,6 +271,8 @@ struct Scene {
   float mats[96] = {0};
   float cols[192] = {0};
   uint32_t ncol = 0;
+  bool law_on = false;         // accumulate the grid update's momentum bookkeeping (law below)
+  uint64_t law[16] = {0};
   std::vector<float> P;        // 32 * cap
   std::vector<int32_t> grid;   // 4 * nodes (i32 after P2G; f32 bits after the grid update)
   size_t nodes() const { return (size_t)dims[0] * dims[1] * dims[2]; }
@@ -333,12 +335,14 @@ float collider_sdf(uint32_t kind, V3 d, const float* s) {  // mpm.js:419-426
 }
 
 // Grid update for one node (mpm.js:428-470). Returns the f32 output (v, m) of the node.
+// vfree (optional) receives p/m + g, the velocity before the boundary rules.
 void grid_node(const int32_t* gin, float* out, int ix, int iy, int iz, const int dims[3], const float g[3],
-               float bound, const float* cols, uint32_t ncol, float floor_friction) {
+               float bound, const float* cols, uint32_t ncol, float floor_friction, V3* vfree = nullptr) {
   out[0] = out[1] = out[2] = out[3] = 0.0f;
   if (!(gin[3] > 0)) return;
   float m = (float)gin[3] * INV_FP;
   V3 v = v3(((float)gin[0] * INV_FP) / m + g[0], ((float)gin[1] * INV_FP) / m + g[1], ((float)gin[2] * INV_FP) / m + g[2]);
+  if (vfree) *vfree = v;
   V3 c = v3((float)ix, (float)iy, (float)iz);
   V3 hi = v3((float)dims[0] - bound, (float)dims[1] - bound, (float)dims[2] - bound);
   if (c.y < bound && v.y < 0.0f) {
@@ -382,15 +386,51 @@ void grid_node(const int32_t* gin, float* out, int ix, int iy, int iz, const int
   out[0] = v.x; out[1] = v.y; out[2] = v.z; out[3] = m;
 }
 
+// A float quantity as 2^32 fixed point in 64 bits: rint(x * 2^32) (the scaling is exact), clamped to
+// +-2^62 first (NaN clamps to -2^62).
+inline uint64_t law_fixed(float x) {
+  const float y = fminf(fmaxf(x * 4294967296.0f, -4.611686018427387904e18f), 4.611686018427387904e18f);
+  return (uint64_t)(int64_t)rintf(y);
+}
+
+// The grid update's momentum bookkeeping for one node with mass (no counterpart in the reference:
+// the momentum law checkMPM states for P2G, verify/index.js:175-193, carried through the grid
+// update, mpm.js:428-470). With p the P2G momentum words, m and v the node's mass and velocity after
+// the update and vf = p/m + g the velocity before the boundary rules (all f32 as the update computes
+// them): [0] mass word, [1..3] momentum words (2^22), [4..6] m v, [7..9] m g, [10..12] m (v - vf),
+// the impulse of the walls, the floor friction and the colliders (2^32, f32 products), [13] nodes.
+// Wrapping 64-bit sums: independent of the order they are taken in.
+inline void law_node(uint64_t* L, const int32_t* q, const float* out, V3 vf, const float g[3]) {
+  const float m = out[3];
+  const float vo[3] = {out[0], out[1], out[2]}, vfa[3] = {vf.x, vf.y, vf.z};
+  L[0] += (uint64_t)(int64_t)q[3];
+  for (int c = 0; c < 3; c++) {
+    L[1 + c] += (uint64_t)(int64_t)q[c];
+    L[4 + c] += law_fixed(m * vo[c]);
+    L[7 + c] += law_fixed(m * g[c]);
+    L[10 + c] += law_fixed(m * (vo[c] - vfa[c]));
+  }
+  L[13] += 1;
+}
+
 void grid_update(Scene& S, const float g[3], float floor_friction) {  // mpm.js:414-471, 593-595
-  const int X = S.dims[0], Y = S.dims[1], Z = S.dims[2];
-#pragma omp parallel for schedule(static)
+  const int X = S.dims[0], Y = S.dims[1];
+  uint64_t L[14] = {0};
+  const bool law = S.law_on;
+#pragma omp parallel for schedule(static) reduction(+ : L[:14])
   for (int64_t i = 0; i < (int64_t)S.nodes(); i++) {
     int ix = (int)(i % X), iy = (int)((i / X) % Y), iz = (int)(i / ((int64_t)X * Y));
-    (void)Z;
     float out[4];
-    grid_node(&S.grid[4 * i], out, ix, iy, iz, S.dims, g, S.bound, S.cols, S.ncol, floor_friction);
+    V3 vf = v3(0, 0, 0);
+    int32_t q[4];
+    std::memcpy(q, &S.grid[4 * i], 16);
+    grid_node(q, out, ix, iy, iz, S.dims, g, S.bound, S.cols, S.ncol, floor_friction, &vf);
+    if (law && q[3] > 0) law_node(L, q, out, vf, g);
     std::memcpy(&S.grid[4 * i], out, 16);
+  }
+  if (law) {
+    for (int k = 0; k < 14; k++) S.law[k] += L[k];
+    S.law[14] += 1;  // grid updates summed
   }
 }
 
@@ -775,6 +815,18 @@ void or_keys(void* s, uint32_t* keys) {
   for (uint32_t i = 0; i < S->count; i++) keys[i] = particle_key(&S->P[32 * (size_t)i]);
 }
 // MPM.step's substep loop (mpm.js:590-602).
+// The grid update's momentum bookkeeping (law_node): on = 1 starts it from zero, 0 stops it.
+void or_set_grid_law(void* s, int on) {
+  Scene* S = static_cast<Scene*>(s);
+  S->law_on = on != 0;
+  std::memset(S->law, 0, sizeof S->law);
+}
+// out[16]: the sums since the last reset ([14] = grid updates); reset = 1 zeroes them.
+void or_read_grid_law(void* s, uint64_t* out, int reset) {
+  Scene* S = static_cast<Scene*>(s);
+  std::memcpy(out, S->law, sizeof S->law);
+  if (reset) std::memset(S->law, 0, sizeof S->law);
+}
 void or_step(void* s, uint32_t substeps, const float* g, float floor_friction, float vmax) {
   Scene* S = static_cast<Scene*>(s);
   if (S->count == 0) return;

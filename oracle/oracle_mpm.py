@@ -47,6 +47,7 @@ def lib() -> ctypes.CDLL:
         "or_grid_update": (None, [vp, vp, f32]), "or_g2p": (None, [vp, f32]), "or_sort": (None, [vp]),
         "or_rescale": (None, [vp, f32]), "or_keys": (None, [vp, vp]), "or_step": (None, [vp, u32, vp, f32, f32]),
         "or_num_threads": (ctypes.c_int, []), "or_set_num_threads": (None, [ctypes.c_int]),
+        "or_set_grid_law": (None, [vp, ctypes.c_int]), "or_read_grid_law": (None, [vp, vp, ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -192,6 +193,16 @@ class Scene:
 
     def rescale(self, k: float) -> None:
         lib().or_rescale(self._h, k)
+
+    def set_grid_law(self, on: bool) -> None:
+        """Starts (from zero) or stops the grid update's momentum bookkeeping."""
+        lib().or_set_grid_law(self._h, 1 if on else 0)
+
+    def read_grid_law(self, reset: bool = False) -> np.ndarray:
+        """The bookkeeping sums since the last reset as 16 uint64 (see law_node in mpm_oracle.cpp)."""
+        out = np.zeros(16, np.uint64)
+        lib().or_read_grid_law(self._h, _p(out), 1 if reset else 0)
+        return out
 
     def step(self, substeps: int, g, floor_friction: float = 0.4, vmax: float = 0.9) -> None:
         lib().or_step(self._h, substeps, _p(np.asarray(g, np.float32)), floor_friction, vmax)

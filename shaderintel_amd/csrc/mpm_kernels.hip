@@ -159,13 +159,16 @@ __device__ __forceinline__ float collider_sdf(uint32_t kind, f3 d, const float* 
 }
 
 // One node of the grid update (mpm.js:428-470): fixed point -> velocity, gravity, slip walls,
-// Coulomb floor, SDF colliders of the node's scene.
+// Coulomb floor, SDF colliders of the node's scene. vfree receives p/m + g (the velocity before the
+// walls, floor and colliders act; zero for a node without mass).
 __device__ __forceinline__ float4 grid_node(int4 q, int ix, int iy, int iz, const SceneGeo& geo,
-                                            const float* __restrict__ cols) {
+                                            const float* __restrict__ cols, f3& vfree) {
+  vfree = mk(0.0f, 0.0f, 0.0f);
   if (!(q.w > 0)) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const float m = (float)q.w * kInvFP;
   f3 v = mk(((float)q.x * kInvFP) / m + geo.g[0], ((float)q.y * kInvFP) / m + geo.g[1],
             ((float)q.z * kInvFP) / m + geo.g[2]);
+  vfree = v;
   const f3 c = mk((float)ix, (float)iy, (float)iz);
   const float bound = geo.bound;
   const f3 hi = mk((float)geo.X - bound, (float)geo.Y - bound, (float)geo.Z - bound);
@@ -1243,6 +1246,46 @@ __device__ __forceinline__ int4 pull_tiles(const Pool& pool, uint32_t scene, int
   return acc;
 }
 
+// A float quantity as 2^32 fixed point in 64 bits: rint(x * 2^32) (the scaling is exact), clamped to
+// +-2^62 first (NaN clamps to -2^62).
+__device__ __forceinline__ unsigned long long law_fixed(float x) {
+  const float y = fminf(fmaxf(x * 4294967296.0f, -4.611686018427387904e18f), 4.611686018427387904e18f);
+  return (unsigned long long)(long long)rintf(y);
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+// The momentum bookkeeping of one node block (kLawWords, mpm_kernels.hpp): each lane's node terms,
+// summed over the wave, added by lane 0 into the workgroup's slot. Called by every lane of the wave.
+__device__ __forceinline__ void law_block(unsigned long long* __restrict__ law, int4 q, float4 vo, f3 vf,
+                                          const float g[3], int lane) {
+  const bool on = q.w > 0;
+  unsigned long long t[14];
+  t[0] = on ? (unsigned long long)(long long)q.w : 0ull;
+  t[1] = on ? (unsigned long long)(long long)q.x : 0ull;
+  t[2] = on ? (unsigned long long)(long long)q.y : 0ull;
+  t[3] = on ? (unsigned long long)(long long)q.z : 0ull;
+  const float m = vo.w;
+  const float v[3] = {vo.x, vo.y, vo.z}, f[3] = {vf.x, vf.y, vf.z};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    t[4 + c] = on ? law_fixed(m * v[c]) : 0ull;
+    t[7 + c] = on ? law_fixed(m * g[c]) : 0ull;
+    t[10 + c] = on ? law_fixed(m * (v[c] - f[c])) : 0ull;
+  }
+  t[13] = on ? 1ull : 0ull;
+  unsigned long long* slot = law + (size_t)(blockIdx.x % kLawSlots) * kLawWords;
+#pragma unroll
+  for (int k = 0; k < 14; k++) {
+    const unsigned long long s = wave_sum_u64(t[k]);
+    if (lane == 0 && s) atomicAdd(slot + k, s);
+  }
+}
+
 // Grid update (VELOCITY = true, mpm.js:428-470) or tile resolve into A (VELOCITY = false) over S
 // (when use_s) then D, one wavefront per node block.
 template <bool VELOCITY>
@@ -1252,6 +1295,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_pull(Pool pool, int use_s) {
   if (VELOCITY && blockIdx.x == 0 && threadIdx.x == 0) {  // label V: the P2G and binning epochs it comes from
     pool.ctr[kCtrVEpoch] = epoch;
     pool.ctr[kCtrVBinEpoch] = use_s ? pool.ctr[kCtrBinEpoch] : 0xffffffffu;
+    if (pool.law) atomicAdd(pool.law + 14, 1ull);  // one more grid update in the bookkeeping
   }
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (uint32_t j = blockIdx.x * 4 + w; j < ns + nd; j += gridDim.x * 4) {
@@ -1267,8 +1311,11 @@ __global__ __launch_bounds__(kThreads) void k_grid_pull(Pool pool, int use_s) {
     if (written) acc = pool.A[addr];
     if (j < ns) acc = pull_tiles(pool, scene, g0, g1, g2, lane, n0, n1, n2, acc);
     if (VELOCITY) {
-      pool.V[addr] = grid_node(acc, n0, n1, n2, geo, pool.cols + 12 * kColRows * scene);
+      f3 vf;
+      const float4 vo = grid_node(acc, n0, n1, n2, geo, pool.cols + 12 * kColRows * scene, vf);
+      pool.V[addr] = vo;
       if (written) pool.A[addr] = make_int4(0, 0, 0, 0);
+      if (pool.law) law_block(pool.law, acc, vo, vf, geo.g, lane);
     } else {
       pool.A[addr] = acc;
     }

@@ -131,7 +131,17 @@ struct Pool {
                             // binning, one bit per block: row (by, bz) of a scene, bit bx
   uint32_t* arena_ctr;     // [kArenaCtrWords]: nodes handed out per region in the current P2G, then the
                            // work counters of P2G and of G2P
+  unsigned long long* law; // [kLawSlots * kLawWords] grid-update momentum bookkeeping, nullptr: off
 };
+
+// Momentum bookkeeping of the grid update (enabled per pool; no counterpart in the reference). For
+// every node with mass, with p its P2G momentum words, m and v its mass and velocity after the
+// update and vf = p/m + g its velocity before the walls, floor and colliders act, in wrapping 64-bit
+// words: [0] mass words, [1..3] momentum words (2^22 fixed point), [4..6] m v, [7..9] m g,
+// [10..12] m (v - vf), each f32 product as rint(x 2^32); [13] nodes; [14] grid updates (slot 0).
+// Workgroups add into slot blockIdx % kLawSlots; the host sums the slots.
+constexpr int kLawSlots = 64;
+constexpr int kLawWords = 16;
 
 // Launch helpers; all enqueue on `s` and none synchronizes. P is the current particle buffer.
 hipError_t launch_records_in(const float4* aos, float4* P, uint32_t cap, uint32_t offset, uint32_t n, uint32_t row_base,

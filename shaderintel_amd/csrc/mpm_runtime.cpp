@@ -160,6 +160,8 @@ struct mpm_pool {
   unsigned long long* occ = nullptr;
   uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr}, *order = nullptr, *flags = nullptr;
   uint32_t* sort_scratch = nullptr;
+  unsigned long long* law = nullptr;  // grid-update momentum bookkeeping (kLawSlots x kLawWords), lazily
+  bool law_on = false;
   float4* staging = nullptr;
   uint32_t staging_records = 0;
   int4* export_buf = nullptr;
@@ -242,6 +244,7 @@ Pool view(const mpm_pool* p) {
   v.ctr = p->ctr;
   v.arena_ctr = p->arena_ctr;
   v.occ = p->occ;
+  v.law = p->law_on ? p->law : nullptr;
   return v;
 }
 
@@ -266,7 +269,7 @@ void free_pool(mpm_pool* p) {
   void* ptrs[] = {p->P[0],   p->P[1],     p->A,        p->V,        p->block_info, p->stamp,   p->dlist,
                   p->sstamp, p->slist,    p->map,      p->wl_start, p->wl_block,    p->tiles,   p->tile_hdr, p->tile_off,
                   p->mats,   p->cols,     p->scenes_d, p->ctr, p->arena_ctr,      p->occ,      p->keys[0],     p->keys[1], p->vals[0],
-                  p->vals[1], p->order,   p->flags,    p->sort_scratch, p->staging, p->export_buf};
+                  p->vals[1], p->order,   p->flags,    p->sort_scratch, p->staging, p->export_buf, p->law};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (p->stream) hipStreamDestroy(p->stream);
@@ -1608,6 +1611,52 @@ mpm_status mpm_pool_autotune(mpm_pool* p, uint32_t trial_substeps, int32_t* g2p_
   if (g2p_waves) *g2p_waves = bg;
   if (p2g_waves) *p2g_waves = bp;
   return st;
+}
+
+mpm_status mpm_pool_set_grid_law(mpm_pool* p, int32_t enable) {
+  CHECK_POOL(p);
+  const uint32_t words = 2u * kLawSlots * kLawWords;  // u32 words
+  if (enable) {
+    if (!p->law) HIP_TRY(dmalloc(&p->law, (size_t)kLawSlots * kLawWords));
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(p->law), 0u, words, p->stream));
+  }
+  if (p->law_on != (enable != 0)) drop_graphs(p);  // the grid update's arguments change
+  p->law_on = enable != 0;
+  return MPM_OK;
+}
+
+mpm_status mpm_pool_read_grid_law(mpm_pool* p, mpm_grid_law* out, int32_t reset) {
+  CHECK_POOL(p);
+  if (!out) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_read_grid_law: NULL argument");
+  std::memset(out, 0, sizeof *out);
+  if (!p->law) return MPM_OK;
+  std::vector<unsigned long long> h((size_t)kLawSlots * kLawWords);
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  HIP_TRY(hipMemcpy(h.data(), p->law, h.size() * 8, hipMemcpyDeviceToHost));
+  unsigned long long t[kLawWords] = {0};
+  for (int s = 0; s < kLawSlots; s++)
+    for (int k = 0; k < kLawWords; k++) t[k] += h[(size_t)s * kLawWords + k];  // wrapping sums
+  out->mass_words = (int64_t)t[0];
+  for (int c = 0; c < 3; c++) {
+    out->p2g[c] = (int64_t)t[1 + c];
+    out->momentum[c] = (int64_t)t[4 + c];
+    out->gravity[c] = (int64_t)t[7 + c];
+    out->boundary[c] = (int64_t)t[10 + c];
+  }
+  out->nodes = t[13];
+  out->updates = t[14];
+  if (reset) HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(p->law), 0u, 2u * kLawSlots * kLawWords, p->stream));
+  return MPM_OK;
+}
+
+mpm_status mpm_set_grid_law(mpm_sim* s, int32_t enable) {
+  if (!s) return fail(MPM_ERR_INVALID_ARGUMENT, "simulation is NULL");
+  return mpm_pool_set_grid_law(s->pool, enable);
+}
+
+mpm_status mpm_read_grid_law(mpm_sim* s, mpm_grid_law* out, int32_t reset) {
+  if (!s) return fail(MPM_ERR_INVALID_ARGUMENT, "simulation is NULL");
+  return mpm_pool_read_grid_law(s->pool, out, reset);
 }
 
 mpm_status mpm_pool_diagnostics(mpm_pool* p, mpm_diagnostics* out) {

@@ -85,6 +85,28 @@ def diagnostics(handle, pool: bool = False) -> dict:
     return {k: getattr(d, k) for k, _ in _Diagnostics._fields_}
 
 
+class _GridLaw(ctypes.Structure):
+    _fields_ = [("mass_words", ctypes.c_int64), ("p2g", ctypes.c_int64 * 3), ("momentum", ctypes.c_int64 * 3),
+                ("gravity", ctypes.c_int64 * 3), ("boundary", ctypes.c_int64 * 3), ("nodes", ctypes.c_uint64),
+                ("updates", ctypes.c_uint64)]
+
+
+def grid_law(handle, pool: bool = False, reset: bool = False) -> dict:
+    """The grid update's momentum bookkeeping of a simulation (or pool) handle (mpm_grid_law): the raw
+    64-bit sums ("raw", 14 ints: mass words, p2g[3], momentum[3], gravity[3], boundary[3], nodes) and
+    the same in lattice units (mass, p2g, momentum, gravity, boundary as float64)."""
+    d = _GridLaw()
+    L = load_library()
+    _check((L.mpm_pool_read_grid_law if pool else L.mpm_read_grid_law)(handle, ctypes.byref(d), 1 if reset else 0),
+           "grid_law")
+    raw = [d.mass_words, *d.p2g, *d.momentum, *d.gravity, *d.boundary, d.nodes]
+    return {"raw": [int(x) for x in raw], "updates": int(d.updates), "nodes": int(d.nodes),
+            "mass": d.mass_words / 2.0 ** 22, "p2g": np.array(d.p2g[:], np.float64) / 2.0 ** 22,
+            "momentum": np.array(d.momentum[:], np.float64) / 2.0 ** 32,
+            "gravity": np.array(d.gravity[:], np.float64) / 2.0 ** 32,
+            "boundary": np.array(d.boundary[:], np.float64) / 2.0 ** 32}
+
+
 class GridView(ctypes.Structure):
     """mpm_grid_view (include/mpm.h): what an after-grid-update hook receives."""
     _fields_ = [("velocity", ctypes.c_void_p), ("block_base", ctypes.c_uint32), ("blocks", ctypes.c_int32 * 3),
@@ -163,6 +185,11 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "mpm_pool_set_tuning": (i32, [vp, i32, i32]),
         "mpm_pool_autotune": (i32, [vp, u32, P(i32), P(i32)]),
         "mpm_pool_diagnostics": (i32, [vp, P(_Diagnostics)]),
+        "mpm_set_grid_law": (i32, [vp, i32]),
+        "mpm_read_grid_law": (i32, [vp, P(_GridLaw), i32]),
+        "mpm_pool_set_grid_law": (i32, [vp, i32]),
+        "mpm_pool_read_grid_law": (i32, [vp, P(_GridLaw), i32]),
+        "mpm_scan_u32_noalloc": (i32, [vp, vp, vp, u64, vp]),
         "mpm_set_grid_hook": (i32, [vp, GRID_HOOK, vp]),
         "mpm_read_points": (i32, [vp, vp, vp, u32, u32, vp, f64, f64]),
         "mpm_particle_streams": (i32, [vp, vp, P(u32)]),
@@ -574,6 +601,15 @@ class MPM:
 
     def sync(self) -> None:
         _check(load_library().mpm_sync(self._h))
+
+    def set_grid_law(self, on: bool) -> None:
+        """Starts (from zero) or stops the grid update's momentum bookkeeping (mpm_set_grid_law)."""
+        _check(load_library().mpm_set_grid_law(self._h, 1 if on else 0), "MPM.set_grid_law")
+
+    def read_grid_law(self, reset: bool = False) -> dict:
+        """The bookkeeping sums since they were started or last reset (see grid_law)."""
+        self._alive("MPM.read_grid_law")
+        return grid_law(self._h, reset=reset)
 
     TUNE = {"g2p_waves": 2, "p2g_waves": 4, "g2p_items": 5, "fuse": 6}
 

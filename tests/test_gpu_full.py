@@ -1,6 +1,7 @@
 """The 8,388,608-particle four-material scene on its 256^3 grid (water, jelly, snow, Drucker-Prager
 sand): the measured workload at its full size. Twelve substeps (three sorts, nine fused G2P + P2G
-launches) bit for bit against the oracle, then the conservation law of the P2G grid at that size."""
+launches) bit for bit against the oracle, the grid update's momentum sums bit for bit and the
+momentum law they state, then the conservation law of the P2G grid at that size."""
 import numpy as np
 import pytest
 
@@ -27,6 +28,8 @@ def full(ctx):
     sc.set_count(n)
     del rec
     g = sim.lattice_gravity()
+    sim.set_grid_law(True)
+    sc.set_grid_law(True)
     assert load_library().mpm_step(sim._h, SUBSTEPS, _ptr(g), 0.4, 0.9) == 0
     sc.step(SUBSTEPS, g)
     yield sim, sc, n
@@ -42,6 +45,21 @@ def test_full_scene_matches_oracle(full):
     del got, want
     bad = np.argwhere(bits(sim.read_grid()) != bits(sc.grid()))
     assert len(bad) == 0, f"grid: {len(bad)} words differ, first {tuple(bad[0])}"
+
+
+def test_full_scene_momentum_law(full):
+    """The grid update's sums over the twelve substeps are the oracle's, word for word, and the grid
+    momentum after the updates minus the P2G momentum is the gravity plus boundary impulse within
+    1e-5 of the impulses."""
+    sim, sc, n = full
+    law = sim.read_grid_law()
+    want = sc.read_grid_law()
+    assert law["raw"] == [int(np.int64(x)) for x in want[:14].astype(np.int64)]
+    assert law["updates"] == SUBSTEPS == int(want[14])
+    G, B = law["gravity"], law["boundary"]
+    resid = law["momentum"] - law["p2g"] - G - B
+    assert np.abs(resid).max() <= 1e-5 * (np.abs(G).sum() + np.abs(B).sum()), (resid, G, B)
+    assert B[1] > 0  # the floor holds the blocks up
 
 
 def test_full_scene_conservation(full):
