@@ -162,13 +162,16 @@ mpm_status mpm_kernel_times(mpm_sim* sim, float ms[2], uint32_t launches[2]);
 
 /* Tuning knobs (no counterpart in the reference; results do not depend on them — every variant
  * computes the same integer grid sums and the same per-particle float sequence). */
-#define MPM_TUNE_G2P_WAVES 2 /* occupancy target of the G2P kernel, 3..6 waves per SIMD (default 3) */
+#define MPM_TUNE_G2P_WAVES 2 /* occupancy target of the G2P kernel without a fused P2G (a call's last
+                              * substep), 3..6 waves per SIMD (default 3) */
 #define MPM_TUNE_P2G_WAVES 4 /* occupancy target of the P2G kernel, 4..6 (default 4) */
 #define MPM_TUNE_G2P_ITEMS 5 /* 1: G2P walks P2G's work items with each item's velocity box in LDS
                               * (default); 0: one thread per particle, gathering from the grid */
 #define MPM_TUNE_FUSE 6      /* 1: G2P fused with the next substep's P2G within one mpm_step call, before
                               * a sort too (over the current work items; default; needs
                               * MPM_TUNE_G2P_ITEMS); 0: separate launches */
+#define MPM_TUNE_FUSE_SHAPE 7 /* workgroup shape of the fused G2P + P2G: 0 = 192 threads at 3 waves per
+                               * SIMD, 1 = 256 threads at 4 (default) */
 mpm_status mpm_set_tuning(mpm_sim* sim, int32_t knob, int32_t value);
 
 /* ---- scene pools: many independent simulations advanced by the same launches ----
@@ -255,11 +258,15 @@ mpm_status mpm_read_grid_law(mpm_sim* sim, mpm_grid_law* out, int32_t reset);
 mpm_status mpm_pool_set_grid_law(mpm_pool* pool, int32_t enable);
 mpm_status mpm_pool_read_grid_law(mpm_pool* pool, mpm_grid_law* out, int32_t reset);
 
-/* Times every occupancy variant of G2P and P2G for trial_substeps substeps each (rounded up to whole
- * sort periods) with HIP events and keeps the fastest; *g2p_waves and *p2g_waves receive it. The
- * trials are real steps of the pool (every variant gives identical results), so the pool advances
- * by 12 trials. */
+/* Times the kernel shapes for trial_substeps substeps each (rounded up to whole sort periods) with HIP
+ * events and keeps the fastest: first the fused G2P + P2G's workgroup shape (MPM_TUNE_FUSE_SHAPE, 2
+ * trials), then the occupancy of the G2P without a fused P2G and of the binned P2G (12 trials).
+ * *g2p_waves and *p2g_waves receive the chosen occupancies (mpm_pool_tuning reads every knob). The
+ * trials are real steps of the pool with the vmax of its last step (every variant gives identical
+ * results), so the pool advances by 14 trials. */
 mpm_status mpm_pool_autotune(mpm_pool* pool, uint32_t trial_substeps, int32_t* g2p_waves, int32_t* p2g_waves);
+/* The current value of a tuning knob. */
+mpm_status mpm_pool_tuning(mpm_pool* pool, int32_t knob, int32_t* value);
 
 /* Scene templates: a scene kept on the device for repeated admission (serving). The records are
  * validated and copied to device memory once; mpm_pool_add_scene_from_template then admits a copy

@@ -16,7 +16,12 @@ namespace mpm {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kFuseThreads = 192;  // the fused G2P + P2G: 3 waves per workgroup, 4 workgroups per CU
+// Shapes of the fused G2P + P2G (40 KB of LDS per workgroup, so 4 workgroups per CU): 256 threads
+// at the 4-waves-per-SIMD register budget (128 VGPRs; every SIMD holds 4 waves), or 192 threads at
+// 3 waves per SIMD (room for 168 VGPRs).
+constexpr int kFuseShapes = 2;
+constexpr int kFuseThreadsOf[kFuseShapes] = {192, 256};
+constexpr int kFuseWavesOf[kFuseShapes] = {3, 4};
 
 __device__ __forceinline__ uint32_t node_addr(int gx, int gy, int gz, const SceneGeo& g) {
   const uint32_t b = g.block_base + (uint32_t)(gx >> 2) +
@@ -146,6 +151,22 @@ __device__ __forceinline__ int wave_max(int v) {
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
   return v;
+}
+
+// Exact n / d for 1 <= d, n < 2^20, d <= 4096, by a multiply-high with m = ceil(2^32 / d): the excess
+// e = m d - 2^32 lies in [0, d - 1], so n m / 2^32 = n / d + n e / (d 2^32) with n e < 2^32, which
+// stays below floor(n / d) + 1 (the box loops index at most 2^11 nodes with extents <= 12^2).
+struct BoxDiv {
+  uint32_t m, d;
+};
+__device__ __forceinline__ BoxDiv box_div(uint32_t d) { return BoxDiv{d > 1u ? 0xffffffffu / d + 1u : 0u, d}; }
+__device__ __forceinline__ uint32_t box_quot(uint32_t n, BoxDiv q) { return q.d > 1u ? __umulhi(n, q.m) : n; }
+// Node t of an ex x ey x ez box (x fastest) as (lx, ly, lz); dx = box_div(ex), dxy = box_div(ex * ey).
+__device__ __forceinline__ void box_coords(int t, int ex, int ey, BoxDiv dx, BoxDiv dxy, int& lx, int& ly, int& lz) {
+  const uint32_t r = box_quot((uint32_t)t, dx);   // t / ex
+  lz = (int)box_quot((uint32_t)t, dxy);           // t / (ex ey)
+  lx = t - (int)r * ex;
+  ly = (int)r - lz * ey;
 }
 
 __device__ __forceinline__ float collider_sdf(uint32_t kind, f3 d, const float* s) {  // mpm.js:419-426
@@ -447,8 +468,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
       ex = h.w & 255; ey = (h.w >> 8) & 255; ez = (h.w >> 16) & 255;
     }
     const int nodes = ex * ey * ez;  // <= kTileNodes
+    const BoxDiv dx = box_div((uint32_t)ex), dxy = box_div((uint32_t)(ex * ey));
     for (int t = tid; t < nodes; t += kThreads) {
-      const int lx = t % ex, ly = (t / ex) % ey, lz = t / (ex * ey);
+      int lx, ly, lz;
+      box_coords(t, ex, ey, dx, dxy, lx, ly, lz);
       box[t] = V[node_addr(ox + lx, oy + ly, oz + lz, geo)];
     }
     __syncthreads();
@@ -955,8 +978,10 @@ __device__ __forceinline__ void store_item_tile(const Pool& pool, uint32_t k, co
   }
   __syncthreads();
   const uint32_t toff = s_off;
+  const BoxDiv dx = box_div((uint32_t)fx), dxy = box_div((uint32_t)(fx * fy));
   for (int t = tid; t < pnodes; t += THREADS) {
-    const int lx = t % fx, ly = (t / fx) % fy, lz = t / (fx * fy);
+    int lx, ly, lz;
+    box_coords(t, fx, fy, dx, dxy, lx, ly, lz);
     const int w = ((pz + lz - wz) * S + (py + ly - wy)) * S + (px + lx - wx);
     const int4 q = make_int4(tile[w], tile[kTileNodes + w], tile[2 * kTileNodes + w], tile[3 * kTileNodes + w]);
     tile[w] = 0;
@@ -1065,9 +1090,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
 // is added to A with atomics instead (integer sums, the same result). perm: sorted position ->
 // record of src (nullptr: identity). keys_out (optional): the sort key of each new record, at its
 // slot (the keys of a sort that follows this substep).
-constexpr int kFuseBoxNodes = 729;
-template <int WAVES>
-__global__ __launch_bounds__(kFuseThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_g2p_p2g(
+//
+// The box is stored with fixed strides when it fits (x stride kBoxRow = 12, plane stride kBoxPlane =
+// 96, so at most 12 x 8 x 8 nodes): the 27 gather offsets are then immediates, and since 12 = 4 mod 8,
+// the 4 x 4 cells of one layer of a block (the lanes that share an LDS cycle) read distinct banks.
+// A box that does not fit is stored packed (runtime strides) when it has at most kFuseBoxNodes nodes.
+constexpr int kFuseBoxNodes = 768;
+constexpr int kBoxRow = 12, kBoxPlane = 96;
+static_assert(kBoxPlane * 8 == kFuseBoxNodes, "the fixed-stride box holds 8 planes");
+template <int SHAPE, int kFuseThreads = kFuseThreadsOf[SHAPE]>
+__global__ __launch_bounds__(kFuseThreadsOf[SHAPE]) __attribute__((amdgpu_waves_per_eu(kFuseWavesOf[SHAPE], 8))) void k_g2p_p2g(
     Pool pool, const float4* src, float4* dst, const uint32_t* __restrict__ perm, int sorted_out, float vmax,
     uint32_t* __restrict__ keys_out) {
   __shared__ float4 vbox[kFuseBoxNodes];
@@ -1094,7 +1126,8 @@ __global__ __launch_bounds__(kFuseThreads) __attribute__((amdgpu_waves_per_eu(WA
     const uint32_t wb = pool.wl_block[k];
     const bool has_box = pool.tile_off[k] != 0xffffffffu;
     const int4 h = pool.tile_hdr[k];  // meaningful when has_box; loaded beside tile_off
-    // this thread's first record, loaded while the velocity box is staged
+    // this thread's first record, loaded while the velocity box is staged (later ones at the top of
+    // each pass: a record held one pass ahead takes the registers of a fourth wave per SIMD)
     uint32_t li = tid, si = 0;
     G2PIn in = {};
     if (li < cnt) {
@@ -1112,29 +1145,36 @@ __global__ __launch_bounds__(kFuseThreads) __attribute__((amdgpu_waves_per_eu(WA
     }
     int nodes = ex * ey * ez;
     if (nodes > kFuseBoxNodes) nodes = 0;  // gathers from V
+    const bool fixed = ex <= kBoxRow && ey * kBoxRow <= kBoxPlane && ez * kBoxPlane <= kFuseBoxNodes;
+    const int sx = fixed ? kBoxRow : ex, sp = fixed ? kBoxPlane : ex * ey;  // strides of the stored box
+    const BoxDiv dx = box_div((uint32_t)ex), dxy = box_div((uint32_t)(ex * ey));
     for (int t = tid; t < nodes; t += kFuseThreads) {
-      const int lx = t % ex, ly = (t / ex) % ey, lz = t / (ex * ey);
-      vbox[t] = V[node_addr(ox + lx, oy + ly, oz + lz, geo)];
+      int lx, ly, lz;
+      box_coords(t, ex, ey, dx, dxy, lx, ly, lz);
+      vbox[lx + sx * ly + sp * lz] = V[node_addr(ox + lx, oy + ly, oz + lz, geo)];
     }
     if (tid < 3) { bbox[tid] = INT_MAX; bbox[3 + tid] = INT_MIN; }
     __syncthreads();
     int lo0 = INT_MAX, lo1 = INT_MAX, lo2 = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hi2 = INT_MIN;
     for (; li < cnt; li += kFuseThreads) {
       const uint32_t j = start + li;
-      g2p_base(in);
-      uint32_t si_next = 0;
-      G2PIn in_next;
-      if (li + kFuseThreads < cnt) {  // the next record of this thread, loaded while this one runs
-        si_next = perm ? perm[j + kFuseThreads] : j + kFuseThreads;
-        g2p_fetch(src, si_next, pool.cap, in_next);
+      if (li != (uint32_t)tid) {
+        si = perm ? perm[j] : j;
+        g2p_fetch(src, si, pool.cap, in);
       }
+      g2p_base(in);
       const bool inbox = nodes > 0 && (unsigned)(in.bx - ox) <= (unsigned)(ex - 3) &&
                          (unsigned)(in.by - oy) <= (unsigned)(ey - 3) && (unsigned)(in.bz - oz) <= (unsigned)(ez - 3);
       f3 v;
       m3 B;
       if (__builtin_amdgcn_ballot_w64(!inbox) == 0) {
-        const int tb = ((in.bz - oz) * ey + (in.by - oy)) * ex + (in.bx - ox);
-        g2p_gather(in, v, B, [&](int, int, int, int a, int b, int c) { return vbox[tb + (c * ey + b) * ex + a]; });
+        const int tb = (in.bz - oz) * sp + (in.by - oy) * sx + (in.bx - ox);
+        if (fixed)
+          g2p_gather(in, v, B, [&](int, int, int, int a, int b, int c) {
+            return vbox[tb + c * kBoxPlane + b * kBoxRow + a];
+          });
+        else
+          g2p_gather(in, v, B, [&](int, int, int, int a, int b, int c) { return vbox[tb + c * sp + b * sx + a]; });
       } else {
         g2p_gather(in, v, B,
                    [&](int bx, int by, int bz, int a, int b, int c) { return V[node_addr(bx + a, by + b, bz + c, geo)]; });
@@ -1165,8 +1205,6 @@ __global__ __launch_bounds__(kFuseThreads) __attribute__((amdgpu_waves_per_eu(WA
       } else {
         fallback_particle(pool, r, epoch, true, bin_epoch);
       }
-      in = in_next;
-      si = si_next;
     }
     lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
     hi0 = wave_max(hi0); hi1 = wave_max(hi1); hi2 = wave_max(hi2);
@@ -1542,17 +1580,19 @@ hipError_t launch_g2p_items(const Pool& pool, const float4* src, float4* dst, co
   return hipGetLastError();
 }
 hipError_t launch_g2p_p2g(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, bool sorted_out,
-                          float vmax, int waves, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop,
+                          float vmax, int shape, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop,
                           uint32_t* keys_out) {
   if (sorted_out ? (!perm || src == dst) : src != dst) return hipErrorInvalidValue;
   const dim3 grid(persistent_grid(pool.max_items, 1, 2048));
   const int so = sorted_out ? 1 : 0;
-  switch (waves) {
-    case 4:
-      MPM_G2P_LAUNCH_T(k_g2p_p2g<4>, grid, kFuseThreads, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax, keys_out);
+  switch (shape) {
+    case 0:
+      MPM_G2P_LAUNCH_T(k_g2p_p2g<0>, grid, kFuseThreadsOf[0], s, ev_start, ev_stop, pool, src, dst, perm, so, vmax,
+                       keys_out);
       break;
     default:
-      MPM_G2P_LAUNCH_T(k_g2p_p2g<3>, grid, kFuseThreads, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax, keys_out);
+      MPM_G2P_LAUNCH_T(k_g2p_p2g<1>, grid, kFuseThreadsOf[1], s, ev_start, ev_stop, pool, src, dst, perm, so, vmax,
+                       keys_out);
       break;
   }
   return hipGetLastError();

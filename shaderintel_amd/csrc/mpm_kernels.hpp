@@ -209,8 +209,9 @@ hipError_t launch_g2p_items(const Pool& pool, const float4* src, float4* dst, co
 // records are scattered into tiles (+ fallback atomics on A) without being read back. Needs
 // launch_begin_p2g after the grid update; leaves the grid state of a binned P2G. keys_out: as
 // launch_g2p_items (the sort keys of the new records).
+// shape: 0 = 192-thread workgroups at 3 waves per SIMD, 1 = 256 threads at 4 (result-neutral).
 hipError_t launch_g2p_p2g(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, bool sorted_out,
-                          float vmax, int waves, hipStream_t s, hipEvent_t ev_start = nullptr,
+                          float vmax, int shape, hipStream_t s, hipEvent_t ev_start = nullptr,
                           hipEvent_t ev_stop = nullptr, uint32_t* keys_out = nullptr);
 // G2P that also applies a pending reorder: slot i of dst takes particle perm[i] of src.
 hipError_t launch_g2p_perm(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, uint32_t n, float vmax,

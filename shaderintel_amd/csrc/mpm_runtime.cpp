@@ -180,6 +180,8 @@ struct mpm_pool {
   int grid_state = kGridClean;
   bool a_dirty_s = false, a_dirty_d = false;  // A holds data over S / over D
   int g2p_waves = 3, p2g_waves = 4;
+  int fuse_shape = 1;        // fused G2P + P2G workgroup shape (launch_g2p_p2g)
+  float last_vmax = 0.9f;    // vmax of the last step (the autotuner's trials step with it)
   bool g2p_items = true;     // G2P by work item (LDS velocity boxes) after a binned P2G
   bool tiles_fresh = false;  // the tiles and work items are those of this substep's binned P2G
   bool fuse = true;          // G2P fused with the next substep's P2G when no sort comes between them
@@ -569,7 +571,7 @@ mpm_status do_g2p(mpm_pool* p, float vmax, bool fuse_next, bool keys_next, bool 
     HIP_TRY(launch_begin_p2g(view(p), p->stream));  // the next P2G's epoch, arena and work counters
     const bool sorted_out = p->reorder_in_g2p;
     HIP_TRY(launch_g2p_p2g(view(p), p->P[p->cur], p->P[p->cur ^ (sorted_out ? 1 : 0)], p->perm, sorted_out, vmax,
-                           p->g2p_waves, p->stream, e0, e1, presort ? p->keys[0] : nullptr));
+                           p->fuse_shape, p->stream, e0, e1, presort ? p->keys[0] : nullptr));
     p->keys_ahead = presort;
     p->p2g_presort = presort;
     if (sorted_out) {
@@ -736,6 +738,7 @@ mpm_status period_graph(mpm_pool* p, float vmax, bool exit_ahead, hipGraphExec_t
 
 mpm_status pool_step(mpm_pool* p, uint32_t substeps, float vmax) {
   p->p2g_ahead = p->keys_ahead = p->p2g_presort = false;  // only ever set for the next substep of the same call
+  p->last_vmax = vmax;
   if (p->needs_repack) ST_TRY(repack(p));
   if (p->count == 0) return MPM_OK;  // mpm.js:584
   uint32_t k = 0;
@@ -1578,28 +1581,45 @@ mpm_status mpm_pool_autotune(mpm_pool* p, uint32_t trial_substeps, int32_t* g2p_
   const uint32_t trial = (trial_substeps + p->sort_every - 1) / p->sort_every * p->sort_every;
   const bool graphs = p->use_graphs;
   const int prof = p->profiling;
+  const float vmax = p->last_vmax;  // the trials step as the caller does
   p->use_graphs = false;
   p->profiling = 0;
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
-  float best = 1e30f;
-  int bg = p->g2p_waves, bp = p->p2g_waves;
   mpm_status st = MPM_OK;
+  // one trial of `trial` substeps with the current knobs; returns its milliseconds
+  auto time_trial = [&](float* ms) -> mpm_status {
+    hipEventRecord(e0, p->stream);
+    ST_TRY(pool_step(p, trial, vmax));
+    hipEventRecord(e1, p->stream);
+    hipEventSynchronize(e1);
+    hipEventElapsedTime(ms, e0, e1);
+    return MPM_OK;
+  };
   // align to a sort boundary so every trial covers the same phases
-  while (st == MPM_OK && p->step_count % p->sort_every != 0) st = pool_step(p, 1, 0.9f);
+  while (st == MPM_OK && p->step_count % p->sort_every != 0) st = pool_step(p, 1, vmax);
+  // the fused G2P + P2G (3 substeps of 4) first: its workgroup shape; then the occupancy of the G2P
+  // without a fused P2G (a call's last substep) and of the binned P2G (a call's first), each tried
+  // with the others at their best so far
+  float best = 1e30f;
+  int bf = p->fuse_shape;
+  for (int f = 0; f < 2 && st == MPM_OK; f++) {
+    p->fuse_shape = f;
+    float ms = 0;
+    st = time_trial(&ms);
+    if (st == MPM_OK && ms < best) { best = ms; bf = f; }
+  }
+  p->fuse_shape = bf;
+  best = 1e30f;
+  int bg = p->g2p_waves, bp = p->p2g_waves;
   for (int g = 3; g <= 6 && st == MPM_OK; g++)
     for (int q = 4; q <= 6 && st == MPM_OK; q++) {
       p->g2p_waves = g;
       p->p2g_waves = q;
-      hipEventRecord(e0, p->stream);
-      st = pool_step(p, trial, 0.9f);
-      hipEventRecord(e1, p->stream);
-      if (st != MPM_OK) break;
       float ms = 0;
-      hipEventSynchronize(e1);
-      hipEventElapsedTime(&ms, e0, e1);
-      if (ms < best) { best = ms; bg = g; bp = q; }
+      st = time_trial(&ms);
+      if (st == MPM_OK && ms < best) { best = ms; bg = g; bp = q; }
     }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
@@ -1611,6 +1631,18 @@ mpm_status mpm_pool_autotune(mpm_pool* p, uint32_t trial_substeps, int32_t* g2p_
   if (g2p_waves) *g2p_waves = bg;
   if (p2g_waves) *p2g_waves = bp;
   return st;
+}
+
+mpm_status mpm_pool_tuning(mpm_pool* p, int32_t knob, int32_t* value) {
+  if (!p || !value) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_tuning: NULL argument");
+  switch (knob) {
+    case MPM_TUNE_G2P_WAVES: *value = p->g2p_waves; return MPM_OK;
+    case MPM_TUNE_P2G_WAVES: *value = p->p2g_waves; return MPM_OK;
+    case MPM_TUNE_G2P_ITEMS: *value = p->g2p_items ? 1 : 0; return MPM_OK;
+    case MPM_TUNE_FUSE: *value = p->fuse ? 1 : 0; return MPM_OK;
+    case MPM_TUNE_FUSE_SHAPE: *value = p->fuse_shape; return MPM_OK;
+    default: return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_tuning: unknown knob %d", knob);
+  }
 }
 
 mpm_status mpm_pool_set_grid_law(mpm_pool* p, int32_t enable) {
@@ -1704,6 +1736,11 @@ mpm_status mpm_pool_set_tuning(mpm_pool* p, int32_t knob, int32_t value) {
     case MPM_TUNE_P2G_WAVES:
       if (value < 4 || value > 6) break;
       p->p2g_waves = value;
+      drop_graphs(p);
+      return MPM_OK;
+    case MPM_TUNE_FUSE_SHAPE:
+      if (value < 0 || value > 1) break;
+      p->fuse_shape = value;
       drop_graphs(p);
       return MPM_OK;
     default:

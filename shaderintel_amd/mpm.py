@@ -184,6 +184,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "mpm_pool_use_graphs": (i32, [vp, i32]),
         "mpm_pool_set_tuning": (i32, [vp, i32, i32]),
         "mpm_pool_autotune": (i32, [vp, u32, P(i32), P(i32)]),
+        "mpm_pool_tuning": (i32, [vp, i32, P(i32)]),
         "mpm_pool_diagnostics": (i32, [vp, P(_Diagnostics)]),
         "mpm_set_grid_law": (i32, [vp, i32]),
         "mpm_read_grid_law": (i32, [vp, P(_GridLaw), i32]),
@@ -611,10 +612,10 @@ class MPM:
         self._alive("MPM.read_grid_law")
         return grid_law(self._h, reset=reset)
 
-    TUNE = {"g2p_waves": 2, "p2g_waves": 4, "g2p_items": 5, "fuse": 6}
+    TUNE = {"g2p_waves": 2, "p2g_waves": 4, "g2p_items": 5, "fuse": 6, "fuse_shape": 7}
 
     def set_tuning(self, **knobs) -> None:
-        """Kernel variant knobs (result-neutral): g2p_waves, p2g_waves, g2p_items, fuse."""
+        """Kernel variant knobs (result-neutral): g2p_waves, p2g_waves, g2p_items, fuse, fuse_shape."""
         for k, v in knobs.items():
             _check(load_library().mpm_set_tuning(self._h, self.TUNE[k], int(v)), "MPM.set_tuning")
 
@@ -721,12 +722,21 @@ class ScenePool:
         return sid.value
 
     def autotune(self, trial_substeps: int = 8) -> dict:
-        """Times the occupancy variants on the live pool (it advances by the trials) and keeps the best."""
+        """Times the kernel shapes on the live pool (it advances by the trials) and keeps the best."""
         self._alive("ScenePool.autotune")
         g, p = ctypes.c_int32(), ctypes.c_int32()
         _check(load_library().mpm_pool_autotune(self._h, int(trial_substeps), ctypes.byref(g), ctypes.byref(p)),
                "ScenePool.autotune")
-        return {"g2p_waves": g.value, "p2g_waves": p.value}
+        return self.tuning()
+
+    def tuning(self) -> dict:
+        """The current value of every tuning knob."""
+        out = {}
+        for k, code in MPM.TUNE.items():
+            v = ctypes.c_int32()
+            _check(load_library().mpm_pool_tuning(self._h, code, ctypes.byref(v)), "ScenePool.tuning")
+            out[k] = v.value
+        return out
 
     def remove_scene(self, scene: int) -> None:
         self._alive("ScenePool.remove_scene")
