@@ -418,13 +418,27 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
                [&](int bx, int by, int bz, int a, int b, int c) { return V[node_addr(bx + a, by + b, bz + c, geo)]; });
 }
 
-// Work item k of the current binning for an item-persistent kernel: the kWorkSplits counters at
-// wctr split the items into contiguous ranges; a workgroup starts on range blockIdx % kWorkSplits
-// and moves on when its range is used up. Called by thread 0; 0xffffffff when every range is done.
+// Work items of an item-persistent kernel. Workgroups are dealt to the 8 XCDs in turn (b % 8), and
+// each XCD takes contiguous items so that neighbouring blocks share an L2.
+//  - first_item: the first s = min(items, 8 floor(grid / 8)) items are assigned statically, XCD x
+//    taking [s x / 8, s (x + 1) / 8) in workgroup order: no atomic for a workgroup's first item, and a
+//    workgroup without one (fewer items than workgroups) learns it from one load.
+//  - next_item: the items after s are split into kWorkSplits contiguous ranges, each with its own
+//    counter at wctr; a workgroup starts on range blockIdx % kWorkSplits (its XCD) and moves on when
+//    its range is used up. Called by thread 0; 0xffffffff when every range is done (at once when no
+//    items are left after the static ones).
+__device__ __forceinline__ uint32_t static_items(uint32_t items) { return min(items, gridDim.x / 8u * 8u); }
+__device__ __forceinline__ uint32_t first_item(uint32_t items) {
+  const uint32_t s = static_items(items), x = blockIdx.x % 8u, i = blockIdx.x / 8u;
+  const uint32_t lo = (uint32_t)((uint64_t)s * x / 8u), hi = (uint32_t)((uint64_t)s * (x + 1u) / 8u);
+  return lo + i < hi ? lo + i : 0xffffffffu;
+}
 __device__ __forceinline__ uint32_t next_item(uint32_t* wctr, uint32_t items, uint32_t& range, uint32_t& spent) {
+  const uint32_t s = static_items(items), rest = items - s;
+  if (rest == 0) return 0xffffffffu;
   while (spent < (uint32_t)kWorkSplits) {
-    const uint32_t lo = (uint32_t)((uint64_t)items * range / kWorkSplits);
-    const uint32_t hi = (uint32_t)((uint64_t)items * (range + 1) / kWorkSplits);
+    const uint32_t lo = s + (uint32_t)((uint64_t)rest * range / kWorkSplits);
+    const uint32_t hi = s + (uint32_t)((uint64_t)rest * (range + 1) / kWorkSplits);
     const uint32_t c = atomicAdd(wctr + kWorkStride * range, 1u);
     if (c < hi - lo) return lo + c;
     spent++;
@@ -452,7 +466,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   uint32_t* const wctr = pool.arena_ctr + kArenaRegions * kArenaStride + kWorkSplits * kWorkStride;
   uint32_t range = blockIdx.x % kWorkSplits, spent = 0;
   const float4* __restrict__ V = pool.V;
-  if (tid == 0) s_item = next_item(wctr, items, range, spent);
+  if (tid == 0) s_item = first_item(items);
   __syncthreads();
   for (;;) {
     const uint32_t k = s_item;
@@ -1026,7 +1040,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   constexpr int S = kTileSpan;  // window tile: S^3 nodes from the lowest in-window base
   for (int t = tid; t < kTileNodes; t += kThreads)
     reinterpret_cast<int4*>(tile)[t] = make_int4(0, 0, 0, 0);  // the 4 planes; each item leaves them zero
-  if (tid == 0) s_item = next_item(cur.wctr, cur.items, cur.range, cur.spent);
+  if (tid == 0) s_item = first_item(cur.items);
   __syncthreads();
   for (;;) {
     const uint32_t k = s_item;
@@ -1117,7 +1131,7 @@ __global__ __launch_bounds__(kFuseThreadsOf[SHAPE]) __attribute__((amdgpu_waves_
   constexpr int S = kTileSpan;  // window tile: S^3 nodes from the lowest in-window base
   for (int t = tid; t < kTileNodes; t += kFuseThreads)
     reinterpret_cast<int4*>(tile)[t] = make_int4(0, 0, 0, 0);  // the 4 planes; each item leaves them zero
-  if (tid == 0) s_item = next_item(cur.wctr, cur.items, cur.range, cur.spent);
+  if (tid == 0) s_item = first_item(cur.items);
   __syncthreads();
   for (;;) {
     const uint32_t k = s_item;
@@ -1508,7 +1522,7 @@ hipError_t launch_bin_dilate(const Pool& pool, uint32_t max_scenes, hipStream_t 
   return hipGetLastError();
 }
 hipError_t launch_p2g_binned(const Pool& pool, const float4* P, const uint32_t* perm, int waves, hipStream_t s) {
-  const dim3 grid(persistent_grid(pool.max_items, 1, 2048));
+  const dim3 grid((persistent_grid(pool.max_items, 1, 2048) + 7u) / 8u * 8u);  // a multiple of 8 (first_item)
   switch (waves) {
     case 5: hipLaunchKernelGGL(k_p2g_binned<5>, grid, dim3(kThreads), 0, s, pool, P, perm); break;
     case 6: hipLaunchKernelGGL(k_p2g_binned<6>, grid, dim3(kThreads), 0, s, pool, P, perm); break;
@@ -1569,7 +1583,7 @@ hipError_t launch_g2p_items(const Pool& pool, const float4* src, float4* dst, co
                             float vmax, int waves, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop,
                             uint32_t* keys_out) {
   if (sorted_out ? (!perm || src == dst) : src != dst) return hipErrorInvalidValue;
-  const dim3 grid(persistent_grid(pool.max_items, 1, 2048));
+  const dim3 grid((persistent_grid(pool.max_items, 1, 2048) + 7u) / 8u * 8u);  // a multiple of 8 (first_item)
   const int so = sorted_out ? 1 : 0;
   switch (waves) {
     case 4: MPM_G2P_LAUNCH(k_g2p_items<4>, grid, s, ev_start, ev_stop, pool, src, dst, perm, so, vmax, keys_out); break;
@@ -1583,7 +1597,7 @@ hipError_t launch_g2p_p2g(const Pool& pool, const float4* src, float4* dst, cons
                           float vmax, int shape, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop,
                           uint32_t* keys_out) {
   if (sorted_out ? (!perm || src == dst) : src != dst) return hipErrorInvalidValue;
-  const dim3 grid(persistent_grid(pool.max_items, 1, 2048));
+  const dim3 grid((persistent_grid(pool.max_items, 1, 2048) + 7u) / 8u * 8u);  // a multiple of 8 (first_item)
   const int so = sorted_out ? 1 : 0;
   switch (shape) {
     case 0:
