@@ -758,6 +758,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_emit(Pool pool, const uint32_t
 // out[start + pos] = in[start + j].
 constexpr int kRankTable = 128;
 constexpr int kBinWaves = kThreads / 64;
+constexpr int kCellCache = 1024;  // cells of an item's first 1024 particles kept in LDS by the count pass
 __device__ __forceinline__ uint64_t cell_peers(uint32_t c, uint64_t valid) {
   uint64_t peers = valid;
 #pragma unroll
@@ -773,9 +774,9 @@ __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uin
                                                             uint32_t* __restrict__ out) {
   __shared__ uint32_t wrun[kBinWaves][64];  // per wave: cell counts, then the next rank per cell
   __shared__ uint32_t wcnt[kBinWaves][64];
-  __shared__ uint32_t wslot[kBinWaves][64];  // per wave: a lane of the step that has the cell
   __shared__ uint32_t srank[kBinWaves][kRankTable];
   __shared__ uint64_t mrank[kBinWaves][kRankTable];
+  __shared__ uint8_t wcell[kBinWaves][kCellCache];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -789,9 +790,13 @@ __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uin
     const uint32_t start = __builtin_amdgcn_readfirstlane(pool.wl_start[k]);
     const uint32_t cnt = __builtin_amdgcn_readfirstlane(pool.wl_start[k + 1]) - start;
     wrun[w][lane] = 0;
-    for (uint32_t j0 = 0; j0 < cnt; j0 += 64) {  // the cell counts (LDS atomics of this wave, in order)
+    for (uint32_t j0 = 0; j0 < cnt; j0 += 64) {  // the cell counts (LDS atomics; their order does not matter)
       const uint32_t j = j0 + lane;
-      if (j < cnt) atomicAdd(&wrun[w][keys[start + j] & 63u], 1u);
+      if (j < cnt) {
+        const uint32_t c = keys[start + j] & 63u;
+        atomicAdd(&wrun[w][c], 1u);
+        if (j < (uint32_t)kCellCache) wcell[w][j] = (uint8_t)c;
+      }
     }
     const uint32_t nc = wrun[w][lane];  // particles in cell `lane`
     wcnt[w][lane] = nc;
@@ -812,28 +817,12 @@ __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uin
     for (uint32_t j0 = 0; j0 < cnt; j0 += 64) {
       const uint32_t j = j0 + lane;
       const bool act = j < cnt;
-      const uint32_t c = act ? keys[start + j] & 63u : 0u;
-      // Are the step's cells distinct? Each lane writes its lane index to its cell's slot and reads
-      // it back: the wave's one store instruction completes for every lane before its load
-      // instruction reads (LDS instructions of a wave execute in order), so a lane that reads
-      // another index shares its cell with that lane. The slot is accessed as volatile: another
-      // lane's store is invisible to the compiler, which would otherwise forward each lane's own
-      // store to its load and drop the check. If no lane clashes, each rank is its cell's count so
-      // far (an LDS atomic per lane, all on different addresses, so their order does not matter);
-      // otherwise the ballots below rank the lanes of each cell in lane order.
-      volatile __attribute__((address_space(3))) uint32_t* const slot =
-          (volatile __attribute__((address_space(3))) uint32_t*)wslot[w];
-      if (act) slot[c] = (uint32_t)lane;
-      const bool clash = act && slot[c] != (uint32_t)lane;
-      const bool distinct = __ballot(clash) == 0;
-      uint64_t peers = 0;
-      uint32_t r = 0;
-      if (distinct) {
-        if (act) r = atomicAdd(&wrun[w][c], 1u);
-      } else {
-        peers = cell_peers(c, __ballot(act));
-        if (act) r = wrun[w][c] + (uint32_t)__popcll(peers & below);
-      }
+      const uint32_t v = act ? in[start + j] : 0u;
+      const uint32_t c = !act ? 0u : (j < (uint32_t)kCellCache ? (uint32_t)wcell[w][j] : keys[start + j] & 63u);
+      // rank within the cell: the count of its earlier steps (wrun) plus the lanes of this step
+      // below this one with the same cell (ballots over the cell's 6 bits; no atomic decides an order)
+      const uint64_t peers = cell_peers(c, __ballot(act));
+      const uint32_t r = act ? wrun[w][c] + (uint32_t)__popcll(peers & below) : 0u;
       if (act) {
         uint32_t pos;
         if (r < (uint32_t)kRankTable) {
@@ -845,10 +834,10 @@ __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uin
             pos += min(nq, r) + ((q < c && nq > r) ? 1u : 0u);
           }
         }
-        out[start + pos] = in[start + j];
+        out[start + pos] = v;
       }
-      // (ballot path) every lane has read wrun[w][c]; the first lane of each cell advances it
-      if (!distinct && act && (peers & below) == 0) wrun[w][c] += (uint32_t)__popcll(peers);
+      // every lane has read wrun[w][c]; the first lane of each cell advances it
+      if (act && (peers & below) == 0) wrun[w][c] += (uint32_t)__popcll(peers);
     }
   }
 }
