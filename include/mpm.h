@@ -281,7 +281,9 @@ mpm_status mpm_pool_add_scene_from_template(mpm_pool* pool, uint32_t tmpl, uint3
  * admitted in arrival order while the pool has room (its capacity is the HBM budget, see
  * mpm_pool_bytes); a request that does not fit waits and blocks the ones behind it; one that does not
  * fit an empty pool is rejected. Each step advances every resident scene by `substeps` substeps in
- * one launch per phase; scenes join and leave between steps (continuous batching). */
+ * one launch per phase; scenes join and leave between steps (continuous batching). The loop keeps one
+ * step in flight: what follows step k (retirements, admissions, step k + 1) is enqueued before it
+ * waits for step k. */
 typedef struct mpm_server mpm_server;
 typedef struct mpm_server_config {
   uint32_t substeps;    /* substeps per step, e.g. 4 */
@@ -293,11 +295,23 @@ typedef struct mpm_server_counters {
   uint32_t resident_particles;
   int32_t error; /* status of a failed step (the loop stops), else 0 */
 } mpm_server_counters;
-#define MPM_LAT_STEP_MS 0   /* duration of each step, ms */
-#define MPM_LAT_REQUEST_S 1 /* per completed request: submit -> last step done, s */
-#define MPM_LAT_WAIT_S 2    /* per completed request: submit -> admission, s */
+#define MPM_LAT_STEP_MS 0       /* device time of each step (HIP events around it), ms */
+#define MPM_LAT_REQUEST_S 1     /* per completed request: submit -> its last step done, s */
+#define MPM_LAT_WAIT_S 2        /* per completed request: submit -> admission, s */
+#define MPM_LAT_REQUEST_AGE_S 3 /* per request not rejected, censored: completed ones by their latency,
+                                 * waiting and running ones by their age so far (a lower bound), s */
 mpm_status mpm_server_create(mpm_pool* pool, const mpm_server_config* cfg, mpm_server** out);
 mpm_status mpm_server_submit(mpm_server* srv, uint32_t tmpl, uint32_t lifetime_steps, uint64_t* ticket);
+/* mpm_server_submit, and when the request's last step is done the scene's records (reference layout,
+ * scene-local material ids, pool order; min(count, capacity) of them) are copied into `result`
+ * before the scene leaves the pool. `result` must stay valid until mpm_server_wait reports the
+ * ticket done. */
+mpm_status mpm_server_submit_result(mpm_server* srv, uint32_t tmpl, uint32_t lifetime_steps, float* result,
+                                    uint32_t capacity, uint64_t* ticket);
+/* Waits up to timeout_ms (negative: without limit) until a ticket is done (its last step finished on
+ * the device) or rejected, or the loop has stopped. *state: 0 waiting, 1 running, 2 done, 3 rejected;
+ * *count (may be NULL): records copied for mpm_server_submit_result. */
+mpm_status mpm_server_wait(mpm_server* srv, uint64_t ticket, int32_t timeout_ms, int32_t* state, uint32_t* count);
 mpm_status mpm_server_stats(mpm_server* srv, mpm_server_counters* stats);
 /* Copies up to capacity samples of one latency series; *n receives the series length. */
 mpm_status mpm_server_latencies(mpm_server* srv, int32_t kind, double* out, uint64_t capacity, uint64_t* n);

@@ -20,6 +20,7 @@
 
 #include "../../include/mpm.h"
 #include "mpm_kernels.hpp"
+#include "pool_internal.hpp"
 #include "prims.hpp"
 
 using namespace mpm;
@@ -212,6 +213,8 @@ struct mpm_pool {
 struct mpm_sim {
   mpm_pool* pool = nullptr;  // a pool with exactly one scene (id 0)
 };
+
+hipStream_t mpm::pool_stream(mpm_pool* pool) { return pool->stream; }
 
 namespace {
 

@@ -5,11 +5,18 @@
 // caller sizes to an HBM budget with mpm_pool_bytes. Scenes come from device-resident templates, so
 // admission is a device-to-device copy. No reference counterpart: ShaderIntel steps one MPM object per
 // scene from the JS thread (src/physics/mpm.js:582-604).
+//
+// The loop keeps one step in flight: lifetimes are counted on the host, so the retirements and
+// admissions that follow step k, and step k + 1 itself, are enqueued on the pool's stream before the
+// loop waits for step k; every pool call is stream-ordered, so the device never idles between steps
+// for host bookkeeping. Step durations come from HIP events around each step; a request's latency
+// is stamped when the step that ended its lifetime is seen done.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <mutex>
 #include <new>
 #include <string>
@@ -17,6 +24,7 @@
 #include <vector>
 
 #include "../../include/mpm.h"
+#include "pool_internal.hpp"
 #include "serve_sched.hpp"
 
 namespace {
@@ -25,6 +33,13 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// One enqueued step: events around it on the pool's stream, its index and the particles it advanced.
+struct Flight {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  uint64_t step = 0;
+  uint32_t particles = 0;
+};
+
 }  // namespace
 
 struct mpm_server {
@@ -32,30 +47,74 @@ struct mpm_server {
   mpm_server_config cfg{};
   std::thread th;
   std::mutex mu;
-  std::condition_variable cv;
-  bool stop = false, drain = false;
+  std::condition_variable cv;       // wakes the loop (submissions, stop)
+  std::condition_variable done_cv;  // wakes mpm_server_wait (tickets retired or rejected)
+  bool stop = false, drain = false, finished = false;
   mpm_serve::Scheduler sched;
   std::vector<double> step_ms;
-  uint64_t steps = 0, particle_substeps = 0;
+  uint64_t steps = 0, enqueued = 0, particle_substeps = 0;
   uint32_t resident_particles = 0;
   mpm_status error = MPM_OK;
   std::string error_msg;
+  std::vector<Flight> spare;  // events of finished flights, for reuse
 };
 
 namespace {
 
+mpm_status new_flight(mpm_server* s, Flight* f) {
+  if (!s->spare.empty()) {
+    *f = s->spare.back();
+    s->spare.pop_back();
+    return MPM_OK;
+  }
+  if (hipEventCreate(&f->e0) != hipSuccess) return MPM_ERR_ERROR;
+  if (hipEventCreate(&f->e1) != hipSuccess) {
+    hipEventDestroy(f->e0);
+    return MPM_ERR_ERROR;
+  }
+  return MPM_OK;
+}
+
+// Waits for the oldest enqueued step (called with the lock released) and accounts for it (lock held).
+mpm_status finish_oldest(mpm_server* s, std::deque<Flight>& fl, std::unique_lock<std::mutex>& lk) {
+  Flight f = fl.front();
+  fl.pop_front();
+  lk.unlock();
+  const hipError_t e = hipEventSynchronize(f.e1);
+  const double t = now_s();
+  float ms = 0.0f;
+  if (e == hipSuccess) hipEventElapsedTime(&ms, f.e0, f.e1);
+  lk.lock();
+  s->spare.push_back(f);
+  if (e != hipSuccess) return MPM_ERR_ERROR;
+  s->step_ms.push_back((double)ms);
+  s->steps++;
+  s->particle_substeps += (uint64_t)f.particles * s->cfg.substeps;
+  s->sched.finish_step(f.step, t);
+  s->done_cv.notify_all();
+  return MPM_OK;
+}
+
 void serve_loop(mpm_server* s) {
+  const hipStream_t stream = mpm::pool_stream(s->pool);
+  std::deque<Flight> fl;
   std::unique_lock<std::mutex> lk(s->mu);
+  mpm_status st = MPM_OK;
   for (;;) {
     if (s->stop && (!s->drain || (s->sched.waiting() == 0 && s->sched.running() == 0))) break;
-    // admission (pool calls happen on this thread only)
+    // admission (pool calls happen on this thread only; all of them are ordered on the pool's stream)
     s->sched.admit(now_s(), [&](const mpm_serve::Request& r, bool pool_empty, uint32_t* scene) {
-      const mpm_status st = mpm_pool_add_scene_from_template(s->pool, r.tmpl, scene);
-      if (st == MPM_OK) return (int)mpm_serve::kAdmitted;
-      if (st == MPM_ERR_OUT_OF_MEMORY && !pool_empty) return (int)mpm_serve::kNoRoom;
+      const mpm_status a = mpm_pool_add_scene_from_template(s->pool, r.tmpl, scene);
+      if (a == MPM_OK) return (int)mpm_serve::kAdmitted;
+      if (a == MPM_ERR_OUT_OF_MEMORY && !pool_empty) return (int)mpm_serve::kNoRoom;
       return (int)mpm_serve::kNeverFits;
     });
+    s->done_cv.notify_all();  // rejections
     if (s->sched.running() == 0) {
+      if (!fl.empty()) {
+        if ((st = finish_oldest(s, fl, lk)) != MPM_OK) break;
+        continue;
+      }
       if (s->stop) {
         if (s->sched.waiting() == 0) break;
         continue;
@@ -63,25 +122,44 @@ void serve_loop(mpm_server* s) {
       s->cv.wait(lk);
       continue;
     }
-    uint32_t parts = 0;
-    mpm_pool_info(s->pool, &parts, nullptr, nullptr);
+    Flight f;
+    if ((st = new_flight(s, &f)) != MPM_OK) break;
+    f.step = s->enqueued++;
+    mpm_pool_info(s->pool, &f.particles, nullptr, nullptr);
     lk.unlock();
-    const double t0 = now_s();
-    mpm_status st = mpm_pool_step(s->pool, s->cfg.substeps, s->cfg.floor_friction, s->cfg.vmax);
-    if (st == MPM_OK) st = mpm_pool_sync(s->pool);
-    const double t1 = now_s();
+    hipEventRecord(f.e0, stream);
+    st = mpm_pool_step(s->pool, s->cfg.substeps, s->cfg.floor_friction, s->cfg.vmax);
+    hipEventRecord(f.e1, stream);
     lk.lock();
     if (st != MPM_OK) {
-      s->error = st;
-      s->error_msg = mpm_last_error_message();
+      s->spare.push_back(f);
       break;
     }
-    s->step_ms.push_back((t1 - t0) * 1e3);
-    s->steps++;
-    s->particle_substeps += (uint64_t)parts * s->cfg.substeps;
-    s->sched.after_step(t1, [&](uint32_t scene) { mpm_pool_remove_scene(s->pool, scene); });
+    fl.push_back(f);
+    // the sessions this step ends leave now (their results, if asked for, are read first: that read
+    // waits for the step), so the next step's admissions see their room
+    s->sched.after_step(f.step, now_s(), [&](const mpm_serve::Request& r) -> uint32_t {
+      uint32_t n = 0;
+      if (r.result && r.capacity) {
+        const mpm_status rs = mpm_pool_read_scene(s->pool, r.scene, r.result, r.capacity, &n);
+        if (rs != MPM_OK && st == MPM_OK) st = rs;
+        n = std::min(n, r.capacity);
+      }
+      mpm_pool_remove_scene(s->pool, r.scene);
+      return n;
+    });
+    if (st != MPM_OK) break;
     mpm_pool_info(s->pool, &s->resident_particles, nullptr, nullptr);
+    while (fl.size() > 1 && st == MPM_OK) st = finish_oldest(s, fl, lk);  // one step stays in flight
+    if (st != MPM_OK) break;
   }
+  while (!fl.empty() && st == MPM_OK) st = finish_oldest(s, fl, lk);
+  if (st != MPM_OK && s->error == MPM_OK) {
+    s->error = st;
+    s->error_msg = mpm_last_error_message();
+  }
+  s->finished = true;
+  s->done_cv.notify_all();
 }
 
 }  // namespace
@@ -105,13 +183,39 @@ mpm_status mpm_server_create(mpm_pool* pool, const mpm_server_config* cfg, mpm_s
   return MPM_OK;
 }
 
-mpm_status mpm_server_submit(mpm_server* s, uint32_t tmpl, uint32_t lifetime_steps, uint64_t* ticket) {
-  if (!s) return MPM_ERR_INVALID_ARGUMENT;
+mpm_status mpm_server_submit_result(mpm_server* s, uint32_t tmpl, uint32_t lifetime_steps, float* result,
+                                    uint32_t capacity, uint64_t* ticket) {
+  if (!s || (capacity && !result)) return MPM_ERR_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->stop) return MPM_ERR_DESTROYED;
-  const uint64_t id = s->sched.submit(tmpl, lifetime_steps, now_s());
+  const uint64_t id = s->sched.submit(tmpl, lifetime_steps, now_s(), result, capacity);
   if (ticket) *ticket = id;
   s->cv.notify_one();
+  return MPM_OK;
+}
+
+mpm_status mpm_server_submit(mpm_server* s, uint32_t tmpl, uint32_t lifetime_steps, uint64_t* ticket) {
+  return mpm_server_submit_result(s, tmpl, lifetime_steps, nullptr, 0, ticket);
+}
+
+mpm_status mpm_server_wait(mpm_server* s, uint64_t ticket, int32_t timeout_ms, int32_t* state, uint32_t* count) {
+  if (!s || !state) return MPM_ERR_INVALID_ARGUMENT;
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (!s->sched.known(ticket)) return MPM_ERR_INVALID_ARGUMENT;
+  auto settled = [&] {
+    const mpm_serve::State q = s->sched.state(ticket);
+    if (q == mpm_serve::kRejected) return true;
+    if (q != mpm_serve::kDone) return s->finished;
+    for (const auto& c : s->sched.completed())  // done once its latency is stamped (its step finished)
+      if (c.id == ticket) return c.stamped || s->finished;
+    return s->finished;
+  };
+  if (timeout_ms < 0)
+    s->done_cv.wait(lk, settled);
+  else
+    s->done_cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), settled);
+  *state = (int32_t)s->sched.state(ticket);
+  if (count) *count = s->sched.count(ticket);
   return MPM_OK;
 }
 
@@ -138,7 +242,10 @@ mpm_status mpm_server_latencies(mpm_server* s, int32_t kind, double* out, uint64
   if (kind == MPM_LAT_STEP_MS) {
     v = s->step_ms;
   } else if (kind == MPM_LAT_REQUEST_S || kind == MPM_LAT_WAIT_S) {
-    for (const auto& c : s->sched.completed()) v.push_back(kind == MPM_LAT_REQUEST_S ? c.total_s : c.wait_s);
+    for (const auto& c : s->sched.completed())
+      if (c.stamped) v.push_back(kind == MPM_LAT_REQUEST_S ? c.total_s : c.wait_s);
+  } else if (kind == MPM_LAT_REQUEST_AGE_S) {
+    v = s->sched.ages(now_s());
   } else {
     return MPM_ERR_INVALID_ARGUMENT;
   }
@@ -162,6 +269,10 @@ mpm_status mpm_server_stop(mpm_server* s, int32_t drain) {
 void mpm_server_destroy(mpm_server* s) {
   if (!s) return;
   mpm_server_stop(s, 0);
+  for (Flight& f : s->spare) {
+    hipEventDestroy(f.e0);
+    hipEventDestroy(f.e1);
+  }
   delete s;
 }
 

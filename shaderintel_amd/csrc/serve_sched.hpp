@@ -1,11 +1,12 @@
 // Host-side scheduling of the serving loop (no device code): a FIFO admission queue, the set of
-// resident sessions with their remaining lifetimes, and latency accounting.
+// resident sessions with their remaining lifetimes, ticket states and latency accounting.
 //
 // A request asks for one scene (a template id) to be simulated for `lifetime` steps. Requests are
 // admitted in arrival order while the pool has room; a request that does not fit blocks the ones
-// behind it (no overtaking, so large scenes are not starved). Between steps, sessions whose lifetime
-// is over leave and free their room (continuous batching). A request that cannot fit even into an
-// empty pool is rejected.
+// behind it (no overtaking, so large scenes are not starved). When the step that ends a session's
+// lifetime has been enqueued, the session leaves and frees its room for the next step's admissions
+// (continuous batching); its latency is stamped once that step's device work is known to be done
+// (finish_step). A request that cannot fit even into an empty pool is rejected.
 #pragma once
 #include <cstdint>
 #include <deque>
@@ -16,31 +17,41 @@ namespace mpm_serve {
 struct Request {
   uint64_t id = 0;
   uint32_t tmpl = 0;
-  uint32_t lifetime = 1;   // steps
-  uint32_t done = 0;       // steps completed
-  double t_submit = 0;     // seconds, caller's clock
+  uint32_t lifetime = 1;     // steps
+  uint32_t done = 0;         // steps enqueued
+  double t_submit = 0;       // seconds, caller's clock
   double t_admit = -1;
   uint32_t scene = 0xffffffffu;
+  float* result = nullptr;   // optional: the scene's records are copied here before it leaves
+  uint32_t capacity = 0;     // records result holds
 };
 
 struct Completed {
   uint64_t id;
   double wait_s;   // submit -> admission
-  double total_s;  // submit -> last step done
+  double total_s;  // submit -> last step done (provisional until finish_step stamps it)
   uint32_t steps;
+  uint64_t step;   // index of the step that ended the lifetime
+  bool stamped;
 };
 
 enum Admit { kAdmitted = 0, kNoRoom = 1, kNeverFits = 2 };
+enum State : uint8_t { kWaiting = 0, kRunning = 1, kDone = 2, kRejected = 3 };
 
 class Scheduler {
  public:
-  uint64_t submit(uint32_t tmpl, uint32_t lifetime, double now) {
+  uint64_t submit(uint32_t tmpl, uint32_t lifetime, double now, float* result = nullptr, uint32_t capacity = 0) {
     Request r;
     r.id = next_id_++;
     r.tmpl = tmpl;
     r.lifetime = lifetime ? lifetime : 1;
     r.t_submit = now;
+    r.result = result;
+    r.capacity = capacity;
     waiting_.push_back(r);
+    state_.push_back(kWaiting);
+    t_submit_.push_back(now);
+    count_.push_back(0);
     return r.id;
   }
 
@@ -55,11 +66,13 @@ class Scheduler {
       if (a == kNoRoom) break;
       if (a == kNeverFits) {
         rejected_.push_back(r.id);
+        state_[r.id] = kRejected;
         waiting_.pop_front();
         continue;
       }
       r.scene = scene;
       r.t_admit = now;
+      state_[r.id] = kRunning;
       running_.push_back(r);
       waiting_.pop_front();
       n++;
@@ -67,18 +80,20 @@ class Scheduler {
     return n;
   }
 
-  // One step of every resident session is done: sessions at the end of their lifetime are removed
-  // (remove(scene)) and recorded. Returns how many retired.
-  template <class Remove>
-  int after_step(double now, Remove&& remove) {
+  // Step `step` of every resident session has been enqueued: sessions at the end of their lifetime
+  // are retired (retire(const Request&) reads their results if asked and removes their scene) and
+  // recorded, their latency provisional (`now`) until finish_step. Returns how many retired.
+  template <class Retire>
+  int after_step(uint64_t step, double now, Retire&& retire) {
     int n = 0;
     size_t w = 0;
     for (size_t i = 0; i < running_.size(); i++) {
       Request& r = running_[i];
       r.done++;
       if (r.done >= r.lifetime) {
-        remove(r.scene);
-        completed_.push_back({r.id, r.t_admit - r.t_submit, now - r.t_submit, r.done});
+        count_[r.id] = retire(r);
+        completed_.push_back({r.id, r.t_admit - r.t_submit, now - r.t_submit, r.done, step, false});
+        state_[r.id] = kDone;
         n++;
       } else {
         running_[w++] = r;
@@ -88,6 +103,28 @@ class Scheduler {
     return n;
   }
 
+  // Step `step` has finished on the device at time t: stamp the latency of the sessions it retired.
+  void finish_step(uint64_t step, double t) {
+    while (stamp_ < completed_.size() && completed_[stamp_].step <= step) {
+      Completed& c = completed_[stamp_++];
+      c.total_s = t - t_submit_[c.id];
+      c.stamped = true;
+    }
+  }
+
+  // Censored request ages at time `now`: completed requests by their latency, waiting and running
+  // ones by their age so far (a lower bound of their latency); rejected ones are left out.
+  std::vector<double> ages(double now) const {
+    std::vector<double> v;
+    for (const auto& c : completed_) v.push_back(c.stamped ? c.total_s : now - t_submit_[c.id]);
+    for (const auto& r : running_) v.push_back(now - r.t_submit);
+    for (const auto& r : waiting_) v.push_back(now - r.t_submit);
+    return v;
+  }
+
+  State state(uint64_t id) const { return id < state_.size() ? (State)state_[id] : kRejected; }
+  uint32_t count(uint64_t id) const { return id < count_.size() ? count_[id] : 0u; }
+  bool known(uint64_t id) const { return id < state_.size(); }
   size_t waiting() const { return waiting_.size(); }
   size_t running() const { return running_.size(); }
   const std::vector<Request>& resident() const { return running_; }
@@ -97,10 +134,14 @@ class Scheduler {
 
  private:
   uint64_t next_id_ = 0;
+  size_t stamp_ = 0;
   std::deque<Request> waiting_;
   std::vector<Request> running_;
   std::vector<Completed> completed_;
   std::vector<uint64_t> rejected_;
+  std::vector<uint8_t> state_;
+  std::vector<double> t_submit_;
+  std::vector<uint32_t> count_;
 };
 
 }  // namespace mpm_serve

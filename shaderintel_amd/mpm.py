@@ -204,6 +204,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "mpm_pool_add_scene_from_template": (i32, [vp, u32, P(u32)]),
         "mpm_server_create": (i32, [vp, P(_ServerConfig), P(vp)]),
         "mpm_server_submit": (i32, [vp, u32, u32, P(u64)]),
+        "mpm_server_submit_result": (i32, [vp, u32, u32, vp, u32, P(u64)]),
+        "mpm_server_wait": (i32, [vp, u64, i32, P(i32), P(u32)]),
         "mpm_server_stats": (i32, [vp, P(_ServerCounters)]),
         "mpm_server_latencies": (i32, [vp, i32, vp, u64, P(u64)]),
         "mpm_server_stop": (i32, [vp, i32]),
@@ -789,15 +791,37 @@ class Server:
     def __init__(self, pool: ScenePool, substeps: int = 4, floor_friction: float = 0.4, vmax: float = 0.9):
         self.pool = pool
         cfg = _ServerConfig(int(substeps), float(floor_friction), float(vmax))
+        self._results: dict = {}  # result arrays the native loop writes into, kept alive until settled
         h = ctypes.c_void_p()
         _check(load_library().mpm_server_create(pool._h, ctypes.byref(cfg), ctypes.byref(h)), "Server")
         self._h = h
 
-    def submit(self, tmpl: int, lifetime_steps: int) -> int:
+    def submit(self, tmpl: int, lifetime_steps: int, result: np.ndarray | None = None) -> int:
+        """Queues a request; with `result` ((n, 32) float32, kept alive here until wait() reports the
+        ticket done) the scene's final records are copied into it before the scene leaves."""
         t = ctypes.c_uint64()
-        _check(load_library().mpm_server_submit(self._h, int(tmpl), int(lifetime_steps), ctypes.byref(t)),
-               "Server.submit")
+        L = load_library()
+        if result is None:
+            _check(L.mpm_server_submit(self._h, int(tmpl), int(lifetime_steps), ctypes.byref(t)), "Server.submit")
+        else:
+            if result.dtype != np.float32 or result.ndim != 2 or result.shape[1] != 32 or not result.flags.c_contiguous:
+                raise SIError("Server.submit: result must be a C-contiguous (n, 32) float32 array", "invalid-argument")
+            _check(L.mpm_server_submit_result(self._h, int(tmpl), int(lifetime_steps), _ptr(result), result.shape[0],
+                                              ctypes.byref(t)), "Server.submit")
+            self._results[t.value] = result
         return t.value
+
+    STATES = ("waiting", "running", "done", "rejected")
+
+    def wait(self, ticket: int, timeout_ms: int = -1) -> tuple[str, int]:
+        """(state, records copied) of a ticket, after waiting up to timeout_ms (-1: until it settles)."""
+        st, n = ctypes.c_int32(), ctypes.c_uint32()
+        _check(load_library().mpm_server_wait(self._h, int(ticket), int(timeout_ms), ctypes.byref(st), ctypes.byref(n)),
+               "Server.wait")
+        state = self.STATES[st.value]
+        if state in ("done", "rejected"):
+            self._results.pop(int(ticket), None)
+        return state, n.value
 
     def stats(self) -> dict:
         c = _ServerCounters()
@@ -805,7 +829,7 @@ class Server:
         return {k: getattr(c, k) for k, _ in _ServerCounters._fields_}
 
     def latencies(self, kind: str) -> np.ndarray:
-        code = {"step_ms": 0, "request_s": 1, "wait_s": 2}[kind]
+        code = {"step_ms": 0, "request_s": 1, "wait_s": 2, "age_s": 3}[kind]
         n = ctypes.c_uint64()
         L = load_library()
         _check(L.mpm_server_latencies(self._h, code, None, 0, ctypes.byref(n)), "Server.latencies")

@@ -1,6 +1,7 @@
 // CPU check of the serving scheduler (shaderintel_amd/csrc/serve_sched.hpp) against a fake pool with
 // a small particle budget: FIFO admission without overtaking, retirement after the lifetime, rejection
-// of requests that cannot fit an empty pool, and the latency bookkeeping. Exit status 0 = pass.
+// of requests that cannot fit an empty pool, ticket states, result counts, latencies stamped when a
+// step's device work ends, and censored ages. Exit status 0 = pass.
 #include <cstdio>
 #include <map>
 #include <vector>
@@ -40,13 +41,28 @@ int main() {
   auto admit = [&](double t) {
     return s.admit(t, [&](const Request& r, bool empty, uint32_t* sc) { return pool.try_admit(r, empty, sc); });
   };
-  auto step = [&](double t) { return s.after_step(t, [&](uint32_t sc) { pool.remove(sc); }); };
+  uint64_t step_index = 0;
+  auto step = [&](double t) {  // enqueue a step and see it finish at t
+    const uint64_t k = step_index++;
+    const int n = s.after_step(k, t - 0.5, [&](const Request& r) {
+      pool.remove(r.scene);
+      return r.result ? 7u : 0u;  // records a real loop would copy
+    });
+    s.finish_step(k, t);
+    return n;
+  };
 
-  s.submit(0, 2, 0.0);  // 60 particles, 2 steps
-  s.submit(1, 1, 0.1);  // 50: does not fit next to the 60
-  s.submit(2, 1, 0.2);  // 30 would fit, but must not overtake
+  float out[8];
+  s.submit(0, 2, 0.0);             // 60 particles, 2 steps
+  s.submit(1, 1, 0.1, out, 8);     // 50: does not fit next to the 60 (results wanted)
+  s.submit(2, 1, 0.2);             // 30 would fit, but must not overtake
   CHECK(admit(1.0) == 1);
   CHECK(s.running() == 1 && s.waiting() == 2 && pool.used == 60);
+  CHECK(s.state(0) == kRunning && s.state(1) == kWaiting && s.state(2) == kWaiting && !s.known(9));
+  {  // censored ages at t = 1.5: the running one and both waiting ones by their age so far
+    const std::vector<double> a = s.ages(1.5);
+    CHECK(a.size() == 3 && a[0] == 1.5 && a[1] > 1.39 && a[1] < 1.41 && a[2] > 1.29 && a[2] < 1.31);
+  }
   CHECK(step(2.0) == 0);
   CHECK(admit(2.0) == 0);  // still blocked behind the 50
   CHECK(step(3.0) == 1);   // the 60 retires after its 2 steps
@@ -55,16 +71,17 @@ int main() {
   CHECK(pool.admitted_tmpl.size() == 3 && pool.admitted_tmpl[1] == 1 && pool.admitted_tmpl[2] == 2);
   CHECK(step(4.0) == 2);
   CHECK(s.running() == 0 && s.waiting() == 0 && pool.used == 0);
+  CHECK(s.state(1) == kDone && s.count(1) == 7 && s.count(2) == 0);
   // a request larger than an empty pool is rejected; the one behind it proceeds
   s.submit(3, 5, 5.0);
   s.submit(4, 1, 5.0);
   CHECK(admit(6.0) == 1);
-  CHECK(s.rejected().size() == 1 && s.rejected()[0] == 3);
+  CHECK(s.rejected().size() == 1 && s.rejected()[0] == 3 && s.state(3) == kRejected);
   CHECK(step(7.0) == 1);
   // latency bookkeeping: ids 0..2 and 4 completed
   const auto& c = s.completed();
   CHECK(c.size() == 4);
-  CHECK(c[0].id == 0 && c[0].wait_s == 1.0 && c[0].total_s == 3.0 && c[0].steps == 2);
+  CHECK(c[0].id == 0 && c[0].wait_s == 1.0 && c[0].total_s == 3.0 && c[0].steps == 2 && c[0].stamped);
   CHECK(c[1].id == 1 && c[1].wait_s > 2.89 && c[1].wait_s < 2.91 && c[1].total_s > 3.89 && c[1].total_s < 3.91);
   CHECK(c[3].id == 4 && c[3].wait_s == 1.0 && c[3].total_s == 2.0);
   CHECK(s.submitted() == 5);
@@ -75,6 +92,13 @@ int main() {
   CHECK(admit(11.0) == 0);
   CHECK(step(12.0) == 4);  // lifetimes 1 retire: k = 0, 3, 6, 9
   CHECK(admit(12.0) == 1 && pool.used == 90);
+  {  // a retired session's latency is provisional until its step is finished
+    const uint64_t k = step_index++;
+    CHECK(s.after_step(k, 13.0, [&](const Request& r) { pool.remove(r.scene); return 0u; }) >= 1);
+    CHECK(!s.completed().back().stamped);
+    s.finish_step(k, 14.0);
+    CHECK(s.completed().back().stamped && s.completed().back().total_s == 14.0 - 10.5);  // the one submitted at 10.5
+  }
   std::printf("ok\n");
   return 0;
 }

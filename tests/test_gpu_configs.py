@@ -1,0 +1,75 @@
+"""The benchmark's other workloads on the GPU, against the oracle:
+- the 128^3 water + jelly scene at full size (1,048,576 particles): 12 substeps (3 sorts) bit for bit,
+  the grid update's momentum sums bit for bit, and checkMPM's conservation law (src/verify/index.js:
+  175-193) on the P2G grid;
+- the 64-scene batched mix from master seed 1 (edges 32-128, 1e4-1e6 particles, water / jelly / snow /
+  sand): one scene pool with captured sort periods, 8 substeps in two calls, each scene bit for bit
+  against the oracle running it alone."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle_mpm as O
+from helpers import bits
+from shaderintel_amd import scenes
+from shaderintel_amd.mpm import MPM, ScenePool, load_library, node_blocks, _ptr
+
+pytestmark = pytest.mark.gpu
+
+
+def _first_diff(a: np.ndarray, b: np.ndarray) -> str:
+    bad = np.argwhere(bits(a).reshape(len(a), -1) != bits(b).reshape(len(b), -1))
+    return "identical" if len(bad) == 0 else f"{len(bad)} words differ, first {tuple(bad[0])}"
+
+
+def test_mixed128_full_size(ctx):
+    print("[gpu] configs: mixed128 full size, 12 substeps", flush=True)
+    sim = MPM(ctx, **scenes.options("mixed128"))
+    n = scenes.populate(sim, "mixed128")
+    assert n == scenes.expected_count("mixed128") == 1048576
+    rec = sim.read_particles()
+    sc = O.Scene(sim.dims, n, sim.ppcAxis, 4)
+    sc.set_materials(sim.material_table())
+    sc.write(rec)
+    sc.set_count(n)
+    g = sim.lattice_gravity()
+    sim.set_grid_law(True)
+    sc.set_grid_law(True)
+    assert load_library().mpm_step(sim._h, 12, _ptr(g), 0.4, 0.9) == 0
+    sc.step(12, g)
+    got, want = sim.read_particles(), sc.read(n)
+    assert np.array_equal(bits(got), bits(want)), _first_diff(got, want)
+    assert np.array_equal(sim.read_grid(), sc.grid())
+    law = sim.read_grid_law()
+    assert law["raw"] == [int(x) for x in sc.read_grid_law()[:14].astype(np.int64)]
+    # checkMPM on the P2G grid of the stepped particles
+    mass = sim.material_table()[:, 1].astype(np.float64)[got[:, 3].view(np.uint32)]
+    pm, pv = mass.sum(), (mass[:, None] * got[:, 4:7].astype(np.float64)).sum(0)
+    sim.p2g()
+    words = sim.read_grid().astype(np.float64) / 2.0 ** 22
+    assert abs(words[:, 3].sum() - pm) / pm < 1e-4
+    assert np.linalg.norm(words[:, :3].sum(0) - pv) / pm < 1e-4 + n * 27 * 4 / 2.0 ** 22 / pm
+    sim.destroy()
+
+
+def test_batched_mix_matches_each_scene_alone(ctx):
+    built = [scenes.host_blocks(edge, blocks) for edge, blocks in scenes.batched_blocks(1, 64)]
+    total = sum(len(b[2]) for b in built)
+    print(f"[gpu] configs: 64-scene batched mix, {total} particles, 8 substeps", flush=True)
+    pool = ScenePool(ctx, total, 64, sum(node_blocks(b[0]) for b in built), max(b[0][0] for b in built))
+    ids = [pool.add_scene(d, t, r, g) for d, t, r, g in built]
+    pool.use_graphs(True)
+    pool.step(4)
+    pool.step(4)
+    for sid, (dims, tbl, rec, g) in zip(ids, built):
+        sc = O.Scene(dims, len(rec), 2, 4)
+        sc.set_materials(tbl)
+        sc.write(rec)
+        sc.set_count(len(rec))
+        sc.step(4, g)
+        sc.step(4, g)
+        got, want = pool.read_scene(sid), sc.read(len(rec))
+        assert np.array_equal(bits(got), bits(want)), f"scene {sid} ({dims[0]}^3, {len(rec)}): {_first_diff(got, want)}"
+        del sc
+    pool.destroy()
