@@ -88,6 +88,11 @@ mpm_status mpm_step(mpm_sim* sim, uint32_t substeps, const float g_lattice[3], f
 /* MPM.p2g(P, forceGlobal) — mpm.js:607-612: clear the grid and scatter; the grid is left in i32
  * fixed point for mpm_read_grid. force_global skips the LDS tiles (the reference's baseline). */
 mpm_status mpm_p2g(mpm_sim* sim, int32_t force_global);
+/* MPM.p2g(P, forceGlobal, grid) with a grid array other than the simulation's (mpm.js:607-612;
+ * physics.test.js:39 compares two such grids): the P2G result in reference layout (4 i32 words per
+ * node) is written to `words` (numNodes * 4). Unlike the reference, the simulation's own grid then
+ * holds the same P2G result (as after mpm_p2g). Blocking. */
+mpm_status mpm_p2g_to(mpm_sim* sim, int32_t force_global, int32_t* words);
 /* MPM._sort — mpm.js:614-621: stable counting sort of the particles by stencil cell. */
 mpm_status mpm_sort(mpm_sim* sim);
 /* rescaleK — mpm.js:405-413: v and C times k (used when substeps / timeScale change). */

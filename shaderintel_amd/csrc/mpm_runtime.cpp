@@ -1237,6 +1237,13 @@ mpm_status mpm_p2g(mpm_sim* s, int32_t force_global) {
   return MPM_OK;
 }
 
+mpm_status mpm_p2g_to(mpm_sim* s, int32_t force_global, int32_t* words) {
+  CHECK_SIM(s);
+  if (!words) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_p2g_to: words NULL");
+  ST_TRY(mpm_p2g(s, force_global));
+  return mpm_read_grid(s, words);
+}
+
 mpm_status mpm_sort(mpm_sim* s) {
   CHECK_SIM(s);
   if (s->pool->mid_substep) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_sort: inside a substep begun with mpm_substep_begin");

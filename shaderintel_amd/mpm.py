@@ -157,6 +157,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "mpm_get_step_count": (i32, [vp, P(u64)]),
         "mpm_step": (i32, [vp, u32, vp, f32, f32]),
         "mpm_p2g": (i32, [vp, i32]),
+        "mpm_p2g_to": (i32, [vp, i32, vp]),
         "mpm_sort": (i32, [vp]),
         "mpm_rescale": (i32, [vp, f32]),
         "mpm_read_particles": (i32, [vp, vp, u32, u32]),
@@ -257,6 +258,28 @@ def spatial_grid(ctx, points: np.ndarray, lo, hi, cell_size: float, queries: np.
     return order, start, offsets, nb[:total.value]
 
 
+class GArray:
+    """A typed array of n elements (gpu.array, src/core/context.js): here 'vec4i', the target of
+    MPM.p2g(P, forceGlobal, grid); read() returns its int32 words (4 per element)."""
+
+    KINDS = {"vec4i": (np.int32, 4), "i32": (np.int32, 1), "u32": (np.uint32, 1), "f32": (np.float32, 1),
+             "vec4f": (np.float32, 4)}
+
+    def __init__(self, kind: str, n: int):
+        if kind not in self.KINDS:
+            raise SIError(f"gpu.array: unknown type {kind!r}", "invalid-argument")
+        dt, w = self.KINDS[kind]
+        self.kind, self.length = kind, int(n)
+        self._data = np.zeros(self.length * w, dt)
+
+    def read(self) -> np.ndarray:
+        return self._data.copy()
+
+    def write(self, data, offset: int = 0) -> None:
+        a = np.asarray(data, self._data.dtype).reshape(-1)
+        self._data[offset:offset + a.size] = a
+
+
 class Context:
     """A HIP device (stands in for the reference's WebGPU Context from SI.init, context.js:32-100)."""
 
@@ -271,8 +294,13 @@ class Context:
         self.name = buf.value.decode()
         self.cus = cus.value
 
-    def mpm(self, **o) -> "MPM":  # gpu.mpm(o), src/index.js:87
-        return MPM(self, **o)
+    def mpm(self, o: dict | None = None, **kw) -> "MPM":  # gpu.mpm(o), src/index.js:87
+        return MPM(self, **{**(o or {}), **kw})
+
+    def array(self, kind: str, n: int) -> "GArray":
+        """gpu.array(type, n) for the grids MPM.p2g can target: 'vec4i' of n nodes (host-backed here;
+        the library writes it, read() returns it)."""
+        return GArray(kind, n)
 
     def scan(self, a: np.ndarray) -> np.ndarray:
         """Scan.run (src/prims/scan.js:292-306): exclusive prefix sum of u32 (wrapping)."""
@@ -362,6 +390,8 @@ class MPM:
         h = ctypes.c_void_p()
         _check(load_library().mpm_sim_create(ctx._h, ctypes.byref(cfg), ctypes.byref(h)), "MPM")
         self._h = h
+        self.P = _ParticleView(self)   # GArray-like views (offsets and counts in vec4 units)
+        self.grid = _GridView(self)
 
     # ---- fields ----
     @property
@@ -452,9 +482,15 @@ class MPM:
     # ---- emission (mpm.js:207-232) ----
     def add(self, shape: dict, material=None, velocity=None, jitter=None, seed=None, noise: float = 0.0,
             noiseSeed=None) -> int:
-        """Fill a shape with ppcAxis^3 particles per cell. noise: std-dev of Gaussian velocity noise
-        (world units/s) — an extension the reference does not have."""
+        """Fill a shape with ppcAxis^3 particles per cell (mpm.js:207-232). The options may also come as
+        the reference's dict, add(shape, {"material": ..., "velocity": ..., "jitter": ..., "seed": ...}).
+        noise: std-dev of Gaussian velocity noise (world units/s) — an extension the reference does
+        not have."""
         self._alive("MPM.add")
+        if isinstance(material, dict) and "kind" not in material:  # the reference's opts object
+            o = material
+            material, velocity = o.get("material"), o.get("velocity", velocity)
+            jitter, seed = o.get("jitter", jitter), o.get("seed", seed)
         mat = self._material(material or shape.get("material") or "water")
         vel = velocity or shape.get("velocity") or [0, 0, 0]
         count = self.count
@@ -525,9 +561,20 @@ class MPM:
         g = self.lattice_gravity()
         _check(L.mpm_step(self._h, self.substeps, _ptr(g), 0.4, 0.9), "MPM.step")
 
-    def p2g(self, force_global: bool = False) -> None:
+    def p2g(self, P=None, forceGlobal: bool = False, grid=None, force_global: bool | None = None) -> None:
+        """MPM.p2g(P, forceGlobal, grid) (mpm.js:607-612): clear the grid and scatter the particles; the
+        grid is left in i32 fixed point. P must be the simulation's own particle buffer; grid is this
+        simulation's (default) or a gpu.array('vec4i', numNodes), which receives the result."""
         self._alive("MPM.p2g")
-        _check(load_library().mpm_p2g(self._h, 1 if force_global else 0), "MPM.p2g")
+        if P is not None and P is not self.P:
+            raise SIError("MPM.p2g: only the simulation's own particle buffer is supported", "unsupported")
+        fg = 1 if (forceGlobal if force_global is None else force_global) else 0
+        if grid is None or grid is self.grid:
+            _check(load_library().mpm_p2g(self._h, fg), "MPM.p2g")
+            return
+        if not isinstance(grid, GArray) or grid.kind != "vec4i" or grid.length != self.numNodes:
+            raise SIError(f"MPM.p2g: grid must be a vec4i array of {self.numNodes} nodes", "invalid-argument")
+        _check(load_library().mpm_p2g_to(self._h, fg, _ptr(grid._data)), "MPM.p2g")
 
     def _sort(self) -> None:
         self._alive("MPM._sort")
@@ -654,6 +701,32 @@ class MPM:
         self.destroyed = True
         load_library().mpm_sim_destroy(self._h)
         self._h = None
+
+
+class _ParticleView:
+    """MPM.P: read(n4) returns n4 vec4 (n4 / 8 records) of the particle buffer as a flat float32 array;
+    write(data, offset4) writes records at vec4 offset offset4 (mpm.js:229, 625)."""
+
+    def __init__(self, sim: "MPM"):
+        self._sim = sim
+
+    def read(self, n4: int | None = None) -> np.ndarray:
+        sim = self._sim
+        n = sim.count if n4 is None else int(n4) // 8
+        return sim.read_particles(n).reshape(-1)
+
+    def write(self, data, offset4: int = 0) -> None:
+        self._sim.write_particles(np.asarray(data, np.float32).reshape(-1, 32), int(offset4) // 8)
+
+
+class _GridView:
+    """MPM.grid: read() returns the grid's int32 words (4 per node, reference layout)."""
+
+    def __init__(self, sim: "MPM"):
+        self._sim = sim
+
+    def read(self) -> np.ndarray:
+        return self._sim.read_grid().reshape(-1)
 
 
 def node_blocks(dims) -> int:
