@@ -82,6 +82,8 @@ mpm_status mpm_get_count(mpm_sim* sim, uint32_t* n);
 /* MPM.stepCount — mpm.js:105, 601 (drives the sort phase). */
 mpm_status mpm_set_step_count(mpm_sim* sim, uint64_t n);
 mpm_status mpm_get_step_count(mpm_sim* sim, uint64_t* n);
+/* MPM.sortEvery — mpm.js:101, 591: sort every n substeps from now on (n >= 1). */
+mpm_status mpm_set_sort_every(mpm_sim* sim, uint32_t n);
 /* MPM.step's substep loop — mpm.js:590-602. g_lattice = gravity*dt^2/dx as float (mpm.js:588);
  * the reference passes floor_friction 0.4 and vmax 0.9 (mpm.js:594, 599). */
 mpm_status mpm_step(mpm_sim* sim, uint32_t substeps, const float g_lattice[3], float floor_friction, float vmax);
@@ -99,6 +101,20 @@ mpm_status mpm_sort(mpm_sim* sim);
 mpm_status mpm_rescale(mpm_sim* sim, float k);
 /* P.read(n*8) — mpm.js:625. Blocking; reference layout out. */
 mpm_status mpm_read_particles(mpm_sim* sim, float* rec, uint32_t offset, uint32_t n);
+/* The reference's reads are Promises that resolve later (P.read / grid.read, src/core/context.js:253-270).
+ * mpm_read_*_begin enqueues the snapshot on the simulation's stream now, ordered after the work
+ * enqueued before it and before the work enqueued after it, into library-owned device staging
+ * (stream-ordered allocation), and returns at once. mpm_read_finish, which may run on any thread
+ * (it touches the read, not the simulation), waits for that snapshot, copies it to `dst` (NULL:
+ * discard it) and frees the read. Every begun read must be finished. Sizes: particles n * 128 bytes
+ * (reference records), grid numNodes * 16 (reference words), points n * 16 (n * 32 with velocities:
+ * the points, then the velocities). */
+typedef struct mpm_read mpm_read;
+mpm_status mpm_read_particles_begin(mpm_sim* sim, uint32_t offset, uint32_t n, mpm_read** out);
+mpm_status mpm_read_grid_begin(mpm_sim* sim, mpm_read** out);
+mpm_status mpm_read_points_begin(mpm_sim* sim, uint32_t offset, uint32_t n, const double origin[3], double dx,
+                                 double vscale, int32_t with_velocity, mpm_read** out);
+mpm_status mpm_read_finish(mpm_read* read, void* dst);
 /* grid.read() — src/verify/index.js:178. Blocking; 4 words per node, reference layout. */
 mpm_status mpm_read_grid(mpm_sim* sim, int32_t* words);
 /* readPositions (mpm.js:624-630) without the full records: for particles [offset, offset + n),

@@ -792,6 +792,8 @@ void or_read_particles(void* s, float* rec, uint32_t offset, uint32_t n) {
 void or_set_count(void* s, uint32_t n) { static_cast<Scene*>(s)->count = n; }
 void or_set_step_count(void* s, uint64_t n) { static_cast<Scene*>(s)->step_count = n; }
 uint64_t or_get_step_count(void* s) { return static_cast<Scene*>(s)->step_count; }
+// MPM.sortEvery = n (mpm.js:101, 591): the sort test of the next substeps uses n
+void or_set_sort_every(void* s, int n) { static_cast<Scene*>(s)->sort_every = n > 0 ? n : 1; }
 void or_read_grid(void* s, int32_t* words) {
   Scene* S = static_cast<Scene*>(s);
   std::memcpy(words, S->grid.data(), S->grid.size() * 4);

@@ -43,6 +43,7 @@ def lib() -> ctypes.CDLL:
         "or_set_colliders": (None, [vp, vp, u32]), "or_write_particles": (None, [vp, vp, u32, u32]),
         "or_read_particles": (None, [vp, vp, u32, u32]), "or_set_count": (None, [vp, u32]),
         "or_set_step_count": (None, [vp, u64]), "or_get_step_count": (u64, [vp]),
+        "or_set_sort_every": (None, [vp, ctypes.c_int]),
         "or_read_grid": (None, [vp, vp]), "or_write_grid": (None, [vp, vp]), "or_p2g": (None, [vp]),
         "or_grid_update": (None, [vp, vp, f32]), "or_g2p": (None, [vp, f32]), "or_sort": (None, [vp]),
         "or_rescale": (None, [vp, f32]), "or_keys": (None, [vp, vp]), "or_step": (None, [vp, u32, vp, f32, f32]),
@@ -164,6 +165,9 @@ class Scene:
     @step_count.setter
     def step_count(self, n: int) -> None:
         lib().or_set_step_count(self._h, n)
+
+    def set_sort_every(self, n: int) -> None:
+        lib().or_set_sort_every(self._h, n)
 
     def grid(self) -> np.ndarray:
         out = np.empty((self.nodes, 4), np.int32)

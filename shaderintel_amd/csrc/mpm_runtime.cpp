@@ -1209,6 +1209,17 @@ mpm_status mpm_set_step_count(mpm_sim* s, uint64_t n) {
   return MPM_OK;
 }
 
+mpm_status mpm_set_sort_every(mpm_sim* s, uint32_t n) {
+  CHECK_SIM(s);
+  if (n == 0) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_sort_every: 0");
+  if (s->pool->mid_substep) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_sort_every: inside a substep begun with mpm_substep_begin");
+  if (s->pool->sort_every != n) {
+    s->pool->sort_every = n;
+    drop_graphs(s->pool);  // a captured period is sort_every substeps long
+  }
+  return MPM_OK;
+}
+
 mpm_status mpm_get_step_count(mpm_sim* s, uint64_t* n) {
   if (!s || !n) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_get_step_count: NULL argument");
   *n = s->pool->step_count;
@@ -1342,6 +1353,106 @@ mpm_status mpm_particle_streams(mpm_sim* s, void* streams[8], uint32_t* capacity
   for (int k = 0; k < 8; k++) streams[k] = p->P[p->cur] + (size_t)k * p->cap;
   if (capacity) *capacity = p->cap;
   return MPM_OK;
+}
+
+// ---- reads that resolve later (P.read / grid.read are Promises in the reference) ----
+struct mpm_read {
+  int device = 0;
+  void* staging = nullptr;  // device memory, stream-ordered allocation on the simulation's stream
+  size_t bytes = 0;
+  hipEvent_t done = nullptr;
+};
+
+}  // extern "C"
+
+namespace {
+
+// Allocates `bytes` of staging on the simulation's stream, lets `fill` enqueue the snapshot into it,
+// and records the event the finish waits for.
+template <class Fill>
+mpm_status begin_read(mpm_pool* p, size_t bytes, mpm_read** out, Fill&& fill) {
+  mpm_read* r = new (std::nothrow) mpm_read;
+  if (!r) return fail(MPM_ERR_OUT_OF_MEMORY, "read: host allocation");
+  r->device = p->ctx->device;
+  r->bytes = bytes;
+  hipError_t e = hipEventCreateWithFlags(&r->done, hipEventDisableTiming);
+  if (e == hipSuccess && bytes) e = hipMallocAsync(&r->staging, bytes, p->stream);
+  if (e == hipSuccess && bytes) e = fill(r->staging);
+  if (e == hipSuccess) e = hipEventRecord(r->done, p->stream);
+  if (e != hipSuccess) {
+    if (r->staging) hipFreeAsync(r->staging, p->stream);
+    if (r->done) hipEventDestroy(r->done);
+    delete r;
+    return from_hip(e, "read");
+  }
+  *out = r;
+  return MPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+mpm_status mpm_read_particles_begin(mpm_sim* s, uint32_t offset, uint32_t n, mpm_read** out) {
+  CHECK_SIM(s);
+  if (!out) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_read_particles_begin: out is NULL");
+  mpm_pool* p = s->pool;
+  if ((uint64_t)offset + n > p->cap)
+    return fail(MPM_ERR_INVALID_ARGUMENT, "P.read: offset %u + n %u exceeds capacity %u", offset, n, p->cap);
+  ST_TRY(apply_pending_reorder(p));
+  return begin_read(p, (size_t)n * 128, out, [&](void* stg) {
+    return launch_records_out(p->P[p->cur], p->cap, offset, n, 0, static_cast<float4*>(stg), p->stream);
+  });
+}
+
+mpm_status mpm_read_grid_begin(mpm_sim* s, mpm_read** out) {
+  CHECK_SIM(s);
+  if (!out) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_read_grid_begin: out is NULL");
+  mpm_pool* p = s->pool;
+  const SceneGeo& geo = p->scenes[0].geo;
+  const size_t nodes = (size_t)geo.X * geo.Y * geo.Z;
+  const int state = p->grid_state;
+  return begin_read(p, nodes * 16, out, [&](void* stg) {
+    if (state != kGridA && state != kGridVelocity)
+      return launch_fill_u32(static_cast<uint32_t*>(stg), 0u, (uint32_t)(nodes * 4), p->stream);
+    return launch_grid_export(view(p), geo, state == kGridA ? 0 : 1, static_cast<int4*>(stg), p->stream);
+  });
+}
+
+mpm_status mpm_read_points_begin(mpm_sim* s, uint32_t offset, uint32_t n, const double origin[3], double dx,
+                                 double vscale, int32_t with_velocity, mpm_read** out) {
+  CHECK_SIM(s);
+  if (!out || !origin) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_read_points_begin: NULL argument");
+  mpm_pool* p = s->pool;
+  if ((uint64_t)offset + n > p->count)
+    return fail(MPM_ERR_INVALID_ARGUMENT, "readPoints: offset %u + n %u exceeds count %u", offset, n, p->count);
+  ST_TRY(apply_pending_reorder(p));
+  return begin_read(p, (size_t)n * 16 * (with_velocity ? 2 : 1), out, [&](void* stg) {
+    float4* pts = static_cast<float4*>(stg);
+    return launch_points_out(p->P[p->cur], p->cap, offset, n, 0, origin, dx, vscale, pts,
+                             with_velocity ? pts + n : nullptr, p->stream);
+  });
+}
+
+mpm_status mpm_read_finish(mpm_read* r, void* dst) {
+  if (!r) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_read_finish: NULL read");
+  hipError_t e = hipSetDevice(r->device);
+  hipStream_t cs = nullptr;
+  if (e == hipSuccess) e = hipEventSynchronize(r->done);
+  if (e == hipSuccess && r->bytes) e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e == hipSuccess && r->bytes && dst) e = hipMemcpyAsync(dst, r->staging, r->bytes, hipMemcpyDeviceToHost, cs);
+  if (r->staging) {  // freed in any case; without the copy stream, by a synchronising free
+    const hipError_t f = cs ? hipFreeAsync(r->staging, cs) : hipFree(r->staging);
+    if (e == hipSuccess) e = f;
+  }
+  if (cs) {
+    const hipError_t f = hipStreamSynchronize(cs);
+    if (e == hipSuccess) e = f;
+    hipStreamDestroy(cs);
+  }
+  hipEventDestroy(r->done);
+  delete r;
+  return from_hip(e, "read");
 }
 
 mpm_status mpm_read_grid(mpm_sim* s, int32_t* words) {

@@ -144,11 +144,59 @@ CtxBox* ctx_of(napi_env env, napi_value v) {
   return c;
 }
 
-napi_value resolved(napi_env env, napi_value value) {
-  napi_deferred d;
-  napi_value promise;
-  napi_create_promise(env, &d, &promise);
-  napi_resolve_deferred(env, d, value);
+// A read that resolves later (the reference's P.read / grid.read, src/core/context.js:253-270): the
+// snapshot is enqueued on the simulation's stream by the caller (mpm_read_*_begin, on the JS thread,
+// so it is ordered with the calls around it); the wait and the copy into the result's buffer run on
+// a worker thread (napi_async_work), and the Promise settles on the JS thread.
+struct ReadJob {
+  napi_async_work work = nullptr;
+  napi_deferred deferred = nullptr;
+  napi_ref result = nullptr;  // the value the Promise resolves with (holds the buffer being filled)
+  mpm_read* read = nullptr;
+  void* dst = nullptr;
+  mpm_status st = MPM_OK;
+  std::string err, what;
+};
+
+void ReadExecute(napi_env, void* data) {
+  ReadJob* j = static_cast<ReadJob*>(data);
+  j->st = mpm_read_finish(j->read, j->dst);
+  j->read = nullptr;
+  if (j->st) j->err = mpm_last_error_message();
+}
+
+void ReadComplete(napi_env env, napi_status status, void* data) {
+  ReadJob* j = static_cast<ReadJob*>(data);
+  napi_value v;
+  napi_get_reference_value(env, j->result, &v);
+  if (status == napi_ok && j->st == MPM_OK) {
+    napi_resolve_deferred(env, j->deferred, v);
+  } else {
+    const mpm_status st = j->st ? j->st : MPM_ERR_ERROR;
+    std::string msg = "ShaderIntel: " + j->what + ": " + (j->err.empty() ? "read cancelled" : j->err);
+    napi_value m, code, err;
+    napi_create_string_utf8(env, msg.c_str(), NAPI_AUTO_LENGTH, &m);
+    napi_create_string_utf8(env, mpm_status_string(st), NAPI_AUTO_LENGTH, &code);
+    napi_create_error(env, code, m, &err);
+    napi_set_named_property(env, err, "code", code);
+    napi_reject_deferred(env, j->deferred, err);
+  }
+  napi_delete_reference(env, j->result);
+  napi_delete_async_work(env, j->work);
+  delete j;
+}
+
+napi_value queue_read(napi_env env, mpm_read* read, void* dst, napi_value result, const char* what) {
+  ReadJob* j = new ReadJob;
+  j->read = read;
+  j->dst = dst;
+  j->what = what;
+  napi_value promise, name;
+  napi_create_promise(env, &j->deferred, &promise);
+  napi_create_reference(env, result, 1, &j->result);
+  napi_create_string_utf8(env, what, NAPI_AUTO_LENGTH, &name);
+  napi_create_async_work(env, nullptr, name, ReadExecute, ReadComplete, j, &j->work);
+  napi_queue_async_work(env, j->work);
   return promise;
 }
 
@@ -273,11 +321,12 @@ napi_value ReadParticles(napi_env env, napi_callback_info info) {
   SimBox* b = sim_of(env, argv[0]);
   if (!b) return nullptr;
   uint32_t offset = (uint32_t)num(env, argv[1], 0), n = (uint32_t)num(env, argv[2], 0);
+  mpm_read* r = nullptr;
+  mpm_status st = mpm_read_particles_begin(b->sim, offset, n, &r);
+  if (st) { throw_status(env, st, "MPM.P.read"); return nullptr; }
   void* data = nullptr;
   napi_value arr = make_typed(env, napi_float32_array, (size_t)n * 32, 4, &data);
-  mpm_status st = mpm_read_particles(b->sim, static_cast<float*>(data), offset, n);
-  if (st) { throw_status(env, st, "MPM.P.read"); return nullptr; }
-  return resolved(env, arr);
+  return queue_read(env, r, data, arr, "MPM.P.read");
 }
 
 napi_value ReadGrid(napi_env env, napi_callback_info info) {
@@ -285,11 +334,36 @@ napi_value ReadGrid(napi_env env, napi_callback_info info) {
   SimBox* b = sim_of(env, argv[0]);
   if (!b) return nullptr;
   double nodes = num(env, argv[1], 0);
+  mpm_read* r = nullptr;
+  mpm_status st = mpm_read_grid_begin(b->sim, &r);
+  if (st) { throw_status(env, st, "MPM.grid.read"); return nullptr; }
   void* data = nullptr;
   napi_value arr = make_typed(env, napi_int32_array, (size_t)nodes * 4, 4, &data);
-  mpm_status st = mpm_read_grid(b->sim, static_cast<int32_t*>(data));
-  if (st) { throw_status(env, st, "MPM.grid.read"); return nullptr; }
-  return resolved(env, arr);
+  return queue_read(env, r, data, arr, "MPM.grid.read");
+}
+
+// p2gTo(sim, forceGlobal, Int32Array(numNodes * 4)): P2G into another grid array (blocking)
+napi_value P2GTo(napi_env env, napi_callback_info info) {
+  ARGS(3);
+  SimBox* b = sim_of(env, argv[0]);
+  if (!b) return nullptr;
+  bool fg = false;
+  napi_get_value_bool(env, argv[1], &fg);
+  size_t len = 0;
+  int32_t* w = static_cast<int32_t*>(typed(env, argv[2], napi_int32_array, &len));
+  if (!w) { throw_arg(env, "MPM.p2g: grid must be a vec4i array"); return nullptr; }
+  mpm_status st = mpm_p2g_to(b->sim, fg ? 1 : 0, w);
+  if (st) { throw_status(env, st, "MPM.p2g"); return nullptr; }
+  return undef(env);
+}
+
+napi_value SetSortEvery(napi_env env, napi_callback_info info) {
+  ARGS(2);
+  SimBox* b = sim_of(env, argv[0]);
+  if (!b) return nullptr;
+  mpm_status st = mpm_set_sort_every(b->sim, (uint32_t)num(env, argv[1], 4));
+  if (st) { throw_status(env, st, "MPM.sortEvery"); return nullptr; }
+  return undef(env);
 }
 
 // readPoints(sim, n, origin Float64Array(3), dx, vscale, withVelocity) -> Promise<[Float32Array, Float32Array|null]>
@@ -303,19 +377,22 @@ napi_value ReadPoints(napi_env env, napi_callback_info info) {
   if (!origin || olen < 3) { throw_arg(env, "MPM.readPoints: origin must be a Float64Array(3)"); return nullptr; }
   bool with_vel = false;
   napi_get_value_bool(env, argv[5], &with_vel);
-  void *pd = nullptr, *vd = nullptr;
-  napi_value pts = make_typed(env, napi_float32_array, (size_t)n * 4, 4, &pd);
-  napi_value vel = nullptr;
-  if (with_vel) vel = make_typed(env, napi_float32_array, (size_t)n * 4, 4, &vd);
-  else napi_get_null(env, &vel);
-  mpm_status st = mpm_read_points(b->sim, static_cast<float*>(pd), static_cast<float*>(vd), 0, n, origin,
-                                  num(env, argv[3], 1), num(env, argv[4], 1));
+  mpm_read* r = nullptr;
+  mpm_status st = mpm_read_points_begin(b->sim, 0, n, origin, num(env, argv[3], 1), num(env, argv[4], 1),
+                                        with_vel ? 1 : 0, &r);
   if (st) { throw_status(env, st, "MPM.readPoints"); return nullptr; }
+  // one buffer: the points, then (with velocities) the velocities
+  void* data = nullptr;
+  napi_value ab, pts, vel;
+  napi_create_arraybuffer(env, (size_t)n * 16 * (with_vel ? 2 : 1), &data, &ab);
+  napi_create_typedarray(env, napi_float32_array, (size_t)n * 4, ab, 0, &pts);
+  if (with_vel) napi_create_typedarray(env, napi_float32_array, (size_t)n * 4, ab, (size_t)n * 16, &vel);
+  else napi_get_null(env, &vel);
   napi_value out;
   napi_create_array_with_length(env, 2, &out);
   napi_set_element(env, out, 0, pts);
   napi_set_element(env, out, 1, vel);
-  return resolved(env, out);
+  return queue_read(env, r, data, out, "MPM.readPoints");
 }
 
 // substepBegin(sim, gravity Float32Array(3), floorFriction)
@@ -524,7 +601,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"p2g", P2G}, {"sort", Sort}, {"rescale", Rescale}, {"sync", Sync}, {"scanU32", ScanU32},
       {"countingSort", CountingSort}, {"materialRow", MaterialRow}, {"sampleShape", SampleShape},
       {"packParticles", PackParticles}, {"readPoints", ReadPoints}, {"substepBegin", SubstepBegin},
-      {"substepEnd", SubstepEnd}, {"writeGrid", WriteGrid},
+      {"substepEnd", SubstepEnd}, {"writeGrid", WriteGrid}, {"p2gTo", P2GTo}, {"setSortEvery", SetSortEvery},
   };
   for (const Fn& f : fns) {
     napi_value v;

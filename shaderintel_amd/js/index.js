@@ -29,6 +29,10 @@ function call(fn) {             // rethrow addon errors as SIError with the same
   }
 }
 
+function callAsync(fn) {        // a native read that resolves later: both kinds of failure as SIError
+  return call(fn).catch((e) => { throw e instanceof SIError ? e : new SIError(e.message, e.code || 'error'); });
+}
+
 const FLUID = 0, JELLY = 1, SNOW = 2, SAND = 3;
 const MPM_KINDS = { fluid: FLUID, jelly: JELLY, elastic: JELLY, snow: SNOW, sand: SAND };
 const MAX_DENSITY = 50;
@@ -55,6 +59,12 @@ class Context {
     this.destroyed = false;
   }
   mpm(o) { return new MPM(this, o); }                         // gpu.mpm(o), src/index.js:87
+  /** gpu.array(type, n) for the grids MPM.p2g can target ('vec4i', n nodes); read() resolves later. */
+  array(type, n) {
+    if (type !== 'vec4i') throw new SIError(`gpu.array: only 'vec4i' grids are supported here (got ${type})`, 'unsupported');
+    const data = new Int32Array(n * 4);
+    return { type, length: n, data, read() { return Promise.resolve(data.slice()); } };
+  }
   /** Scan.run (src/prims/scan.js:292-306): exclusive prefix sum of a Uint32Array. */
   scan(a) { return call(() => native().scanU32(this._h, a)); }
   /** SpatialGrid.build (src/prims/grid.js:70-79): { order, start } of a counting sort by key. */
@@ -82,7 +92,7 @@ class MPM {
     this.gravity = o.gravity || [0, -9.81, 0];
     this._substeps = Math.max(1, Math.round(o.substeps || 8));
     this._timeScale = dflt(o.timeScale, 1);
-    this.sortEvery = o.sortEvery || 4;
+    this._sortEvery = o.sortEvery || 4;
     this.ppcAxis = o.ppcAxis || 2;
     this.bound = 3;
     this.count = 0;
@@ -92,16 +102,16 @@ class MPM {
     this.numNodes = this.dims[0] * this.dims[1] * this.dims[2];
     this.destroyed = false;
     this._h = call(() => native().simCreate(ctx._h, { dims: this.dims, maxParticles: this.maxParticles,
-      ppcAxis: this.ppcAxis, bound: this.bound, sortEvery: this.sortEvery, flags: 0 }));
+      ppcAxis: this.ppcAxis, bound: this.bound, sortEvery: this._sortEvery, flags: 0 }));
     const sim = this;
     // GArray-like views of the particle buffer and the grid (offsets and counts in vec4 units).
     this.P = {
-      read(n4) { return call(() => native().readParticles(sim._alive('MPM.P.read'), 0, Math.floor((n4 === undefined ? sim.count * 8 : n4) / 8))); },
+      read(n4) { return callAsync(() => native().readParticles(sim._alive('MPM.P.read'), 0, Math.floor((n4 === undefined ? sim.count * 8 : n4) / 8))); },
       write(data, offset4) { call(() => native().writeParticles(sim._alive('MPM.P.write'), data, Math.floor((offset4 || 0) / 8))); },
       get destroyed() { return sim.destroyed; },
     };
     this.grid = {
-      read() { return call(() => native().readGrid(sim._alive('MPM.grid.read'), sim.numNodes)); },
+      read() { return callAsync(() => native().readGrid(sim._alive('MPM.grid.read'), sim.numNodes)); },
       /** Between substepBegin and substepEnd: every node's f32 (v, m) bits, reference layout. */
       write(words) { call(() => native().writeGrid(sim._alive('MPM.grid.write'), words)); },
       get destroyed() { return sim.destroyed; },
@@ -221,9 +231,23 @@ class MPM {
     call(() => native().step(this._h, this.substeps, this.latticeGravity(), 0.4, 0.9));
   }
 
-  /** Clear the grid and scatter the particles (P2G); the grid is left in i32 fixed point. */
-  p2g(P, forceGlobal) {
+  /** MPM.sortEvery (mpm.js:101, 591): sort every n substeps, from the next substep on. */
+  get sortEvery() { return this._sortEvery; }
+  set sortEvery(n) {
+    call(() => native().setSortEvery(this._alive('MPM.sortEvery'), n));
+    this._sortEvery = n;
+  }
+
+  /** Clear the grid and scatter the particles (P2G, mpm.js:607-612); the grid is left in i32 fixed
+   * point. grid: this simulation's (default) or a gpu.array('vec4i', numNodes), which receives it. */
+  p2g(P, forceGlobal, grid) {
     if (P && P !== this.P) throw new SIError('MPM.p2g: only the simulation\'s own particle buffer is supported', 'unsupported');
+    if (grid && grid !== this.grid) {
+      if (grid.type !== 'vec4i' || grid.length !== this.numNodes)
+        throw new SIError(`MPM.p2g: grid must be a vec4i array of ${this.numNodes} nodes`, 'invalid-argument');
+      call(() => native().p2gTo(this._alive('MPM.p2g'), !!forceGlobal, grid.data));
+      return;
+    }
     call(() => native().p2g(this._alive('MPM.p2g'), !!forceGlobal));
   }
 
@@ -240,7 +264,7 @@ class MPM {
 
   /** [points, velocities]: (world xyz, material id bits) per particle and, if asked, (world v, J). */
   readPoints(withVelocity) {
-    return call(() => native().readPoints(this._alive('MPM.readPoints'), this.count, Float64Array.from(this.origin),
+    return callAsync(() => native().readPoints(this._alive('MPM.readPoints'), this.count, Float64Array.from(this.origin),
       this.dx, this.dx / this.substepDt, !!withVelocity));
   }
 

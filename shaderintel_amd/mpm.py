@@ -194,6 +194,11 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "mpm_scan_u32_noalloc": (i32, [vp, vp, vp, u64, vp]),
         "mpm_set_grid_hook": (i32, [vp, GRID_HOOK, vp]),
         "mpm_read_points": (i32, [vp, vp, vp, u32, u32, vp, f64, f64]),
+        "mpm_read_particles_begin": (i32, [vp, u32, u32, P(vp)]),
+        "mpm_read_grid_begin": (i32, [vp, P(vp)]),
+        "mpm_read_points_begin": (i32, [vp, u32, u32, vp, f64, f64, i32, P(vp)]),
+        "mpm_read_finish": (i32, [vp, vp]),
+        "mpm_set_sort_every": (i32, [vp, u32]),
         "mpm_particle_streams": (i32, [vp, vp, P(u32)]),
         "mpm_spatial_grid": (i32, [vp, vp, u32, vp, vp, f64, vp, u32, vp, vp, vp, vp, u64, P(u64)]),
         "mpm_substep_begin": (i32, [vp, vp, f32]),
@@ -361,6 +366,30 @@ def sample_shape(shape: dict, spacing: float, jitter: float, seed: int) -> np.nd
     return out
 
 
+class PendingRead:
+    """A read enqueued on a simulation's stream (mpm_read_*_begin); result() waits for it, copies it
+    into a host array and returns that array. It holds device staging memory until result() runs."""
+
+    def __init__(self, handle: ctypes.c_void_p, out: np.ndarray, what: str):
+        self._h, self._out, self._what = handle, out, what
+
+    def done(self) -> bool:
+        return self._h is None
+
+    def result(self) -> np.ndarray:
+        if self._h is not None:
+            h, self._h = self._h, None
+            _check(load_library().mpm_read_finish(h, _ptr(self._out)), self._what)
+        return self._out
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None:
+            try:
+                load_library().mpm_read_finish(self._h, None)
+            except Exception:
+                pass
+
+
 class MPM:
     """MLS-MPM simulation on one MI355X (src/physics/mpm.js:82-680)."""
 
@@ -377,7 +406,7 @@ class MPM:
         self.gravity = list(gravity or [0, -9.81, 0])
         self._substeps = max(1, round(substeps or 8))
         self._timeScale = 1 if timeScale is None else timeScale
-        self.sortEvery = sortEvery or 4
+        self._sortEvery = sortEvery or 4
         self.ppcAxis = ppcAxis or 2
         self.bound = 3
         self.materials: list[dict] = []
@@ -386,7 +415,7 @@ class MPM:
         self.destroyed = False
         self.numNodes = self.dims[0] * self.dims[1] * self.dims[2]
         cfg = _Config((ctypes.c_int32 * 3)(*self.dims), self.maxParticles, self.ppcAxis, float(self.bound),
-                      self.sortEvery, 0)
+                      self._sortEvery, 0)
         h = ctypes.c_void_p()
         _check(load_library().mpm_sim_create(ctx._h, ctypes.byref(cfg), ctypes.byref(h)), "MPM")
         self._h = h
@@ -394,6 +423,17 @@ class MPM:
         self.grid = _GridView(self)
 
     # ---- fields ----
+    @property
+    def sortEvery(self) -> int:
+        """Sort every n substeps (mpm.js:101, 591); setting it applies from the next substep on."""
+        return self._sortEvery
+
+    @sortEvery.setter
+    def sortEvery(self, n: int) -> None:
+        self._alive("MPM.sortEvery")
+        _check(load_library().mpm_set_sort_every(self._h, int(n)), "MPM.sortEvery")
+        self._sortEvery = int(n)
+
     @property
     def count(self) -> int:
         n = ctypes.c_uint32()
@@ -594,6 +634,22 @@ class MPM:
         out = np.empty((n, 32), np.float32)
         _check(load_library().mpm_read_particles(self._h, _ptr(out), offset, n), "MPM.P.read")
         return out
+
+    def read_particles_async(self, n: int | None = None, offset: int = 0) -> "PendingRead":
+        """P.read that resolves later: the snapshot is taken in stream order now, result() waits for
+        it (the caller's thread is free meanwhile)."""
+        self._alive("MPM.P.read")
+        n = self.count if n is None else n
+        h = ctypes.c_void_p()
+        _check(load_library().mpm_read_particles_begin(self._h, offset, n, ctypes.byref(h)), "MPM.P.read")
+        return PendingRead(h, np.empty((n, 32), np.float32), "MPM.P.read")
+
+    def read_grid_async(self) -> "PendingRead":
+        """grid.read that resolves later (see read_particles_async)."""
+        self._alive("MPM.grid.read")
+        h = ctypes.c_void_p()
+        _check(load_library().mpm_read_grid_begin(self._h, ctypes.byref(h)), "MPM.grid.read")
+        return PendingRead(h, np.empty((self.numNodes, 4), np.int32), "MPM.grid.read")
 
     def read_grid(self) -> np.ndarray:
         """grid.read (blocking): (numNodes, 4) int32 words in the reference layout."""

@@ -82,3 +82,33 @@ def test_js_scene_matches_oracle(tmp_path, frames):
     assert summary["stepCount"] == 8 * frames + 1
     assert summary["law"]["ok"], summary["law"]
     assert summary["afterDestroy"] == "destroyed"
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_reads_resolve_later(tmp_path):
+    """P.read / grid.read / readPoints are Promises settled off the JS thread (napi_async_work):
+    a read issued between two steps holds the state after the first (with sortEvery set to 2 first,
+    checked against the oracle); P2G into a second grid array with the global path equals the tiled
+    P2G into the simulation's grid; and with about a thousand substeps queued ahead of a read, a
+    setImmediate callback queued after it runs first."""
+    print("[gpu] JS host: reads that resolve later", flush=True)
+    subprocess.run([NODE, os.path.join(ROOT, "tests", "js", "async_read.js"), str(tmp_path)], check=True, timeout=120)
+    s = json.loads((tmp_path / "summary.json").read_text())
+    n = s["count"]
+    rec = {k: np.fromfile(tmp_path / f"{k}.f32", np.float32).reshape(n, 32) for k in ("before", "mid", "after")}
+    sc = O.Scene(s["dims"], n, 2, 4)
+    sc.set_materials(np.fromfile(tmp_path / "materials.f32", np.float32))
+    sc.write(rec["before"])
+    sc.set_count(n)
+    sc.set_sort_every(2)
+    g = np.asarray(s["gravity"], np.float32)
+    sc.step(8, g)
+    assert np.array_equal(rec["mid"].view(np.uint32), sc.read(n).view(np.uint32))
+    sc.step(8, g)
+    assert np.array_equal(rec["after"].view(np.uint32), sc.read(n).view(np.uint32))
+    assert s["sortEvery"] == 2
+    assert s["p2gDiffer"] == 0 and s["p2gNonzero"] > 1000
+    assert s["badGrid"] == "invalid-argument"
+    assert s["order"] == ["timer", "read"], s
+    assert s["bigFinite"] == s["bigCount"] > 500000
