@@ -18,13 +18,23 @@ the bench scene, plus the dam break in full and 10 substeps of the 128^3 scene.
 
 Two more sections run by default (skip with --no-batched / --no-serve):
 - "batched": 64 independent scenes (edges 32-128, 1e4-1e6 particles, one material box each, master
-  seed 1) stepped as one scene pool (one launch per phase for all scenes, captured sort periods,
-  autotuned occupancy) against the same scenes as separate simulations stepped in turn (the
-  reference's one-object-per-scene pattern on a single queue) and on 64 concurrent streams.
-- "serve": the native serving loop with Poisson arrivals (--serve-rate req/s for --serve-seconds),
-  each request one of those scenes for a geometric number of 4-substep steps (mean 300), admitted
-  FIFO into a pool sized to a 200 GB HBM budget; reports p50/p99 step latency, request latency and
-  admission wait.
+  seed 1) stepped as one scene pool (one launch per phase for all scenes, with and without captured
+  sort periods, autotuned shapes) against the same scenes stepped per scene: as separate
+  simulations in turn (the reference's one-object-per-scene pattern on a single queue), on 64
+  concurrent streams, and as one-scene pools replaying captured sort periods on 64 streams.
+- "serve": the native serving loop under scene-step requests at --serve-rate requests/s for
+  --serve-seconds (plus one overload rate, --serve-overload): sessions drawn from those scenes
+  arrive as a Poisson stream at rate / 300 and live a geometric number of 4-substep steps (mean
+  300), each step one request issued when the session's previous step is done; admitted FIFO into
+  a pool sized to a 200 GB HBM budget. Reports p50/p99 device step time, p50/p99 request latency
+  over the served requests and censored (pending requests counted by their age at the end of the
+  window), and the served request rate.
+
+The roofline record follows the contract: "achieved" and "frac" count the algorithmic bytes (384 B
+per particle-substep) over the dominant kernel's average launch time. Beside them, "traffic" is the
+HBM bytes one launch moves (committed rocprofv3 counter passes, scaled by a copy of known size),
+"traffic_frac" that traffic's rate against the 8 TB/s peak, and "limiter" what the committed SQ
+counters show bounds the kernel.
 
 Prints one JSON line (the last line of stdout).
 """
@@ -184,14 +194,25 @@ def batched_scenes(seed: int = 1, count: int = 64):
 
 def batched(ctx, substeps: int, seed: int = 1, count: int = 64) -> dict:
     """The batched workload (64 scenes from one master seed, see scenes.batched_blocks): one scene
-    pool stepped with one launch per phase (captured sort periods, autotuned occupancy) against the
-    same scenes as separate simulations, each stepped with its own launches on its own stream."""
-    from shaderintel_amd import scenes
+    pool stepped with one launch per phase (captured sort periods, autotuned shapes), and without
+    captured graphs, against the same scenes stepped per scene: (a) as separate simulations in turn
+    on one queue (the reference's pattern), (b) as separate simulations on their own streams, all
+    enqueued before one wait, and (c) as one-scene pools with captured sort periods on their own
+    streams. Each figure is the best of two timed runs."""
     from shaderintel_amd.mpm import MPM, ScenePool, load_library, node_blocks, _ptr, _check
     L = load_library()
     built = batched_scenes(seed, count)
     total = sum(len(b[2]) for b in built)
     log(f"batched: {count} scenes, {total} particles")
+
+    def best(fn, reps=2):
+        out = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            out.append(time.perf_counter() - t0)
+        return min(out)
+
     pool = ScenePool(ctx, total, count, sum(node_blocks(b[0]) for b in built), max(b[0][0] for b in built))
     for d, t, r, g in built:
         pool.add_scene(d, t, r, g)
@@ -199,12 +220,18 @@ def batched(ctx, substeps: int, seed: int = 1, count: int = 64) -> dict:
     tuned = pool.autotune(4)
     pool.step(8)
     pool.sync()
-    t0 = time.perf_counter()
-    pool.step(substeps)
+
+    def run_pool():
+        pool.step(substeps)
+        pool.sync()
+    dt_pool = best(run_pool)
+    pool.use_graphs(False)
+    pool.step(8)
     pool.sync()
-    dt_pool = time.perf_counter() - t0
+    dt_pool_eager = best(run_pool)
     pool.destroy()
-    log(f"batched pool: {substeps} substeps in {dt_pool * 1e3:.1f} ms (tuned {tuned})")
+    log(f"batched pool: {substeps} substeps in {dt_pool * 1e3:.1f} ms with graphs, {dt_pool_eager * 1e3:.1f} ms eager "
+        f"(tuned {tuned})")
     sims = []
     for d, t, r, g in built:
         sim = MPM(ctx, grid=d[0], size=d[0] / 64.0, maxParticles=len(r), substeps=8)
@@ -216,42 +243,69 @@ def batched(ctx, substeps: int, seed: int = 1, count: int = 64) -> dict:
         _check(L.mpm_step(sim._h, 8, _ptr(g), 0.4, 0.9), "step")
     for sim, _ in sims:
         sim.sync()
-    # (a) the reference's pattern: one simulation object per scene, each stepped in turn on one
-    # device queue (WebGPU has a single queue), so the scenes run one after another
-    t0 = time.perf_counter()
-    for sim, g in sims:
-        _check(L.mpm_step(sim._h, substeps, _ptr(g), 0.4, 0.9), "step")
-        sim.sync()
-    dt_seq = time.perf_counter() - t0
-    # (b) the same simulations on their own HIP streams, all enqueued before one wait (they overlap)
-    t0 = time.perf_counter()
-    for sim, g in sims:
-        _check(L.mpm_step(sim._h, substeps, _ptr(g), 0.4, 0.9), "step")
-    for sim, _ in sims:
-        sim.sync()
-    dt_sep = time.perf_counter() - t0
+
+    def in_turn():  # (a) one simulation object per scene, each stepped in turn on one queue
+        for sim, g in sims:
+            _check(L.mpm_step(sim._h, substeps, _ptr(g), 0.4, 0.9), "step")
+            sim.sync()
+
+    def on_streams():  # (b) the same simulations on their own HIP streams (they overlap)
+        for sim, g in sims:
+            _check(L.mpm_step(sim._h, substeps, _ptr(g), 0.4, 0.9), "step")
+        for sim, _ in sims:
+            sim.sync()
+    dt_seq = best(in_turn)
+    dt_sep = best(on_streams)
     for sim, _ in sims:
         sim.destroy()
-    log(f"batched separate: {substeps} substeps in {dt_seq * 1e3:.1f} ms in turn, {dt_sep * 1e3:.1f} ms on 64 streams")
-    rate_pool, rate_seq, rate_sep = (total * substeps / dt for dt in (dt_pool, dt_seq, dt_sep))
+    pools = []
+    for d, t, r, g in built:
+        p1 = ScenePool(ctx, len(r), 1, node_blocks(d), d[0])
+        p1.add_scene(d, t, r, g)
+        p1.use_graphs(True)
+        p1.step(8)
+        pools.append(p1)
+    for p1 in pools:
+        p1.sync()
+
+    def graphs_on_streams():  # (c) one-scene pools replaying captured sort periods, own streams
+        for p1 in pools:
+            p1.step(substeps)
+        for p1 in pools:
+            p1.sync()
+    dt_gr = best(graphs_on_streams)
+    for p1 in pools:
+        p1.destroy()
+    log(f"batched separate: {substeps} substeps in {dt_seq * 1e3:.1f} ms in turn, {dt_sep * 1e3:.1f} ms on 64 streams, "
+        f"{dt_gr * 1e3:.1f} ms as graphs on 64 streams")
+    rate = {k: total * substeps / dt for k, dt in
+            (("pool", dt_pool), ("pool_eager", dt_pool_eager), ("seq", dt_seq), ("sep", dt_sep), ("gr", dt_gr))}
     return {"workload": f"{count} scenes (master seed {seed}): edges 32-128, 1e4-1e6 particles each, "
                         f"water/jelly/snow/sand, {total} particles, {substeps} substeps",
             "particles": total, "substeps": substeps,
-            "pool_particle_substeps_per_s": round(rate_pool, 1),
-            "per_scene_in_turn_particle_substeps_per_s": round(rate_seq, 1),
-            "per_scene_streams_particle_substeps_per_s": round(rate_sep, 1),
-            "speedup_vs_per_scene_in_turn": round(rate_pool / rate_seq, 2),
-            "speedup_vs_per_scene_streams": round(rate_pool / rate_sep, 2), "tuned": tuned,
-            "pool_ms": round(dt_pool * 1e3, 2), "in_turn_ms": round(dt_seq * 1e3, 2),
-            "streams_ms": round(dt_sep * 1e3, 2)}
+            "pool_particle_substeps_per_s": round(rate["pool"], 1),
+            "pool_eager_particle_substeps_per_s": round(rate["pool_eager"], 1),
+            "per_scene_in_turn_particle_substeps_per_s": round(rate["seq"], 1),
+            "per_scene_streams_particle_substeps_per_s": round(rate["sep"], 1),
+            "per_scene_graphs_streams_particle_substeps_per_s": round(rate["gr"], 1),
+            "speedup_vs_per_scene_in_turn": round(rate["pool"] / rate["seq"], 2),
+            "speedup_vs_per_scene_streams": round(rate["pool"] / rate["sep"], 2),
+            "speedup_vs_per_scene_graphs_streams": round(rate["pool"] / rate["gr"], 2),
+            "tuned": tuned, "pool_ms": round(dt_pool * 1e3, 2), "pool_eager_ms": round(dt_pool_eager * 1e3, 2),
+            "in_turn_ms": round(dt_seq * 1e3, 2), "streams_ms": round(dt_sep * 1e3, 2),
+            "graphs_streams_ms": round(dt_gr * 1e3, 2)}
 
 
 def serve(ctx, rate: float, seconds: float, budget_gb: float, substeps: int = 4, lifetime_mean: float = 300,
-          seed: int = 2) -> dict:
-    """Serving: Poisson arrivals at `rate` requests/s for `seconds`; each request is one of the 64
-    batched-workload scenes (drawn at random) simulated for a geometric number of steps (mean
-    `lifetime_mean`) of `substeps` substeps. One pool sized to the HBM budget; the native serving loop
-    admits, steps and retires (continuous batching)."""
+          seed: int = 2, graph_after: int = 8) -> dict:
+    """Serving: a stream of scene-step requests at `rate` requests/s for `seconds`. A request advances
+    one scene by one step of `substeps` substeps. Scenes (sessions) are drawn from the 64
+    batched-workload scenes and live a geometric number of steps (mean `lifetime_mean`); a session
+    issues its next step request when its previous step is done, so sessions arrive as a Poisson
+    stream at rate / lifetime_mean and the step requests come at `rate` on average. One pool sized to
+    the HBM budget; the native serving loop admits, steps every resident session together and retires
+    (continuous batching). Latencies: per scene-step request (issue -> its step done), over the served
+    ones and censored (pending requests by their age at the end of the window)."""
     from shaderintel_amd.mpm import ScenePool, Server
     built = batched_scenes(1, 64)
     tmpl_bytes = sum(len(b[2]) for b in built) * 128
@@ -266,39 +320,55 @@ def serve(ctx, rate: float, seconds: float, budget_gb: float, substeps: int = 4,
         mid = (lo + hi) // 2
         lo, hi = (mid, hi) if ScenePool.bytes_for(*cfg(mid)) <= budget else (lo, mid)
     pool = ScenePool(ctx, *cfg(lo))
+    session_rate = rate / lifetime_mean
     log(f"serve: pool of {lo} particles / {cfg(lo)[2]} node blocks = {ScenePool.bytes_for(*cfg(lo)) / 1e9:.1f} GB "
-        f"(+{tmpl_bytes / 1e9:.1f} GB templates) for a {budget_gb} GB budget; {rate} req/s for {seconds} s")
+        f"(+{tmpl_bytes / 1e9:.1f} GB templates) for a {budget_gb} GB budget; {rate} step req/s "
+        f"({session_rate:.2f} sessions/s) for {seconds} s")
     tids = [pool.add_template(d, t, r, g) for d, t, r, g in built]
     rng = np.random.default_rng(seed)
-    srv = Server(pool, substeps=substeps)
+    srv = Server(pool, substeps=substeps, graph_after=graph_after)
     t0 = time.perf_counter()
     t_next = 0.0
-    submitted = 0
     while True:
-        t_next += rng.exponential(1.0 / rate)
+        t_next += rng.exponential(1.0 / session_rate)
         if t_next > seconds:
             break
         delay = t0 + t_next - time.perf_counter()
         if delay > 0:
             time.sleep(delay)
         srv.submit(tids[int(rng.integers(len(tids)))], int(rng.geometric(1.0 / lifetime_mean)))
-        submitted += 1
+    delay = t0 + seconds - time.perf_counter()
+    if delay > 0:
+        time.sleep(delay)
     wall = time.perf_counter() - t0
+    ages = srv.latencies("scene_step_age_s")  # taken at the end of the window, before the stop
+    served = srv.latencies("scene_step_s")
     srv.stop(drain=False)
     st = srv.stats()
-    steps, req, wait = srv.latencies("step_ms"), srv.latencies("request_s"), srv.latencies("wait_s")
+    steps, req = srv.latencies("step_ms"), srv.latencies("request_s")
     srv.destroy()
     pool.destroy()
     log(f"serve: {st}")
-    return {"workload": f"Poisson {rate} req/s for {seconds} s; scenes drawn from the 64-scene batched mix; lifetime "
-                        f"geometric, mean {lifetime_mean} steps of {substeps} substeps; {budget_gb} GB HBM budget",
-            "rate_req_s": rate, "submitted": st["submitted"], "completed": st["completed"],
-            "rejected": st["rejected"], "waiting_at_end": st["waiting"], "resident_at_end": st["running"],
-            "resident_particles_at_end": st["resident_particles"], "steps": st["steps"],
+    mean_particles = float(np.mean([len(b[2]) for b in built]))
+    return {"workload": f"scene-step requests at {rate}/s for {seconds} s: sessions drawn from the 64-scene batched "
+                        f"mix arrive as a Poisson stream at {session_rate:.3g}/s and live a geometric number of "
+                        f"steps (mean {lifetime_mean}), one request per step of {substeps} substeps, issued when "
+                        f"the session's previous step is done; {budget_gb} GB HBM budget"
+                        + (f"; captured sort periods once the resident scenes are unchanged for {graph_after} steps"
+                           if graph_after else "; eager launches"),
+            "rate_req_s": rate, "sessions_submitted": st["submitted"], "sessions_completed": st["completed"],
+            "sessions_rejected": st["rejected"], "sessions_waiting_at_end": st["waiting"],
+            "sessions_resident_at_end": st["running"], "resident_particles_at_end": st["resident_particles"],
+            "steps": st["steps"], "scene_steps_served": st["scene_steps"],
+            "scene_steps_per_s": round(st["scene_steps"] / wall, 1),
             "step_latency_ms_p50": pctl(steps, 50), "step_latency_ms_p99": pctl(steps, 99),
-            "request_latency_s_p50": pctl(req, 50), "request_latency_s_p99": pctl(req, 99),
-            "admission_wait_s_p50": pctl(wait, 50), "admission_wait_s_p99": pctl(wait, 99),
-            "particle_substeps_per_s": round(st["particle_substeps"] / wall, 1), "error": st["error"]}
+            "request_latency_ms_p50": pctl(served * 1e3, 50), "request_latency_ms_p99": pctl(served * 1e3, 99),
+            "request_latency_censored_ms_p50": pctl(ages * 1e3, 50),
+            "request_latency_censored_ms_p99": pctl(ages * 1e3, 99),
+            "pending_at_end": int(len(ages) - len(served)),
+            "session_latency_s_p50": pctl(req, 50), "session_latency_s_p99": pctl(req, 99), "graph_after": graph_after,
+            "particle_substeps_per_s": round(st["particle_substeps"] / wall, 1),
+            "mean_particles_per_request": round(mean_particles, 1), "error": st["error"]}
 
 
 def main() -> int:
@@ -316,7 +386,12 @@ def main() -> int:
     ap.add_argument("--no-small", action="store_true", help="skip the 64^3 and 128^3 scenes")
     ap.add_argument("--no-copy", action="store_true", help="skip the device-to-device copy measurement")
     ap.add_argument("--serve-rate", type=float, nargs="+", default=[100.0, 300.0, 1000.0, 3000.0, 5000.0],
-                    help="request arrival rates (req/s)")
+                    help="scene-step request rates (req/s)")
+    ap.add_argument("--serve-overload", type=float, default=20000.0,
+                    help="one more rate above saturation (0: none): its served rate is the saturation rate")
+    ap.add_argument("--serve-graph-after", type=int, default=8,
+                    help="the serving loop replays captured sort periods once the resident scenes are unchanged "
+                         "for this many steps (0: eager)")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -410,9 +485,15 @@ def main() -> int:
     tpath = os.path.join(ROOT, "profiles", f"traffic_{args.scene}.json")
     if os.path.exists(tpath):
         t = json.load(open(tpath))
-        if t.get("kernel") == "k_" + dom.split()[-1]:
+        if t.get("kernel", "").split("<")[0] == "k_" + dom.split()[-1]:
             roofline["traffic"] = round(t["traffic_bytes"])
             roofline["traffic_source"] = t["source"]
+            # the HBM rate the measured traffic implies at this run's launch time
+            traffic_gbs = t["traffic_bytes"] / (launch_ms[dom] * 1e-3) / 1e9
+            roofline["traffic_gbs"] = round(traffic_gbs, 1)
+            roofline["traffic_frac"] = round(traffic_gbs / HBM_PEAK_GBS, 4)
+            if "limiter" in t:
+                roofline["limiter"] = t["limiter"]
     substep_bytes = 384 * n
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "particle-substeps/s", "n_gpus": 1,
@@ -440,7 +521,13 @@ def main() -> int:
     if not args.no_batched:
         out["batched"] = batched(ctx, args.batched_substeps)
     if not args.no_serve:
-        out["serve"] = [serve(ctx, r, args.serve_seconds, args.serve_budget_gb) for r in args.serve_rate]
+        ga = args.serve_graph_after
+        out["serve"] = [serve(ctx, r, args.serve_seconds, args.serve_budget_gb, graph_after=ga) for r in args.serve_rate]
+        if args.serve_overload > 0:
+            over = serve(ctx, args.serve_overload, args.serve_seconds, args.serve_budget_gb, graph_after=ga)
+            over["overload"] = True
+            out["serve"].append(over)
+            out["serve_saturation_req_s"] = over["scene_steps_per_s"]
     if args.copy_calibration:
         copy_calibration(args.copy_calibration << 20)
     if not args.no_cpu:

@@ -310,17 +310,25 @@ typedef struct mpm_server_config {
   uint32_t substeps;    /* substeps per step, e.g. 4 */
   float floor_friction; /* 0.4 in the reference */
   float vmax;           /* 0.9 in the reference */
+  uint32_t graph_after; /* 0: eager launches; n > 0: replay captured sort periods (mpm_pool_use_graphs)
+                         * once the resident scenes have not changed for n steps */
 } mpm_server_config;
 typedef struct mpm_server_counters {
   uint64_t submitted, completed, rejected, waiting, running, steps, particle_substeps;
   uint32_t resident_particles;
-  int32_t error; /* status of a failed step (the loop stops), else 0 */
+  int32_t error;        /* status of a failed step (the loop stops), else 0 */
+  uint64_t scene_steps; /* scene-step requests served: the sessions of every finished step */
 } mpm_server_counters;
 #define MPM_LAT_STEP_MS 0       /* device time of each step (HIP events around it), ms */
 #define MPM_LAT_REQUEST_S 1     /* per completed request: submit -> its last step done, s */
 #define MPM_LAT_WAIT_S 2        /* per completed request: submit -> admission, s */
 #define MPM_LAT_REQUEST_AGE_S 3 /* per request not rejected, censored: completed ones by their latency,
                                  * waiting and running ones by their age so far (a lower bound), s */
+/* Scene-step requests: each step of a session is one request, issued when the session's previous
+ * step is done (its first on arrival); latency = issue -> end of the step that serves it. */
+#define MPM_LAT_SCENE_STEP_S 4     /* per served scene-step request, s */
+#define MPM_LAT_SCENE_STEP_AGE_S 5 /* censored: served ones by their latency, each resident and waiting
+                                    * session's pending request by its age so far, s */
 mpm_status mpm_server_create(mpm_pool* pool, const mpm_server_config* cfg, mpm_server** out);
 mpm_status mpm_server_submit(mpm_server* srv, uint32_t tmpl, uint32_t lifetime_steps, uint64_t* ticket);
 /* mpm_server_submit, and when the request's last step is done the scene's records (reference layout,

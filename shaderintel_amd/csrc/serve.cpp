@@ -10,7 +10,9 @@
 // admissions that follow step k, and step k + 1 itself, are enqueued on the pool's stream before the
 // loop waits for step k; every pool call is stream-ordered, so the device never idles between steps
 // for host bookkeeping. Step durations come from HIP events around each step; a request's latency
-// is stamped when the step that ended its lifetime is seen done.
+// is stamped when the step that ended its lifetime is seen done. With graph_after = n, the pool
+// replays captured sort periods once the resident scenes have stayed the same for n steps (a change
+// of scenes drops the captured graphs, so under churn the loop launches eagerly).
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -33,11 +35,13 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// One enqueued step: events around it on the pool's stream, its index and the particles it advanced.
+// One enqueued step: events around it on the pool's stream, its index, the particles it advanced,
+// the sessions that took part and the arrival times of those for which it was the first step.
 struct Flight {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   uint64_t step = 0;
-  uint32_t particles = 0;
+  uint32_t particles = 0, sessions = 0;
+  std::vector<double> firsts;
 };
 
 }  // namespace
@@ -63,8 +67,9 @@ namespace {
 
 mpm_status new_flight(mpm_server* s, Flight* f) {
   if (!s->spare.empty()) {
-    *f = s->spare.back();
+    *f = std::move(s->spare.back());
     s->spare.pop_back();
+    f->firsts.clear();
     return MPM_OK;
   }
   if (hipEventCreate(&f->e0) != hipSuccess) return MPM_ERR_ERROR;
@@ -77,7 +82,7 @@ mpm_status new_flight(mpm_server* s, Flight* f) {
 
 // Waits for the oldest enqueued step (called with the lock released) and accounts for it (lock held).
 mpm_status finish_oldest(mpm_server* s, std::deque<Flight>& fl, std::unique_lock<std::mutex>& lk) {
-  Flight f = fl.front();
+  Flight f = std::move(fl.front());
   fl.pop_front();
   lk.unlock();
   const hipError_t e = hipEventSynchronize(f.e1);
@@ -85,12 +90,15 @@ mpm_status finish_oldest(mpm_server* s, std::deque<Flight>& fl, std::unique_lock
   float ms = 0.0f;
   if (e == hipSuccess) hipEventElapsedTime(&ms, f.e0, f.e1);
   lk.lock();
-  s->spare.push_back(f);
-  if (e != hipSuccess) return MPM_ERR_ERROR;
+  if (e != hipSuccess) {
+    s->spare.push_back(std::move(f));
+    return MPM_ERR_ERROR;
+  }
   s->step_ms.push_back((double)ms);
   s->steps++;
   s->particle_substeps += (uint64_t)f.particles * s->cfg.substeps;
-  s->sched.finish_step(f.step, t);
+  s->sched.finish_step(f.step, t, f.sessions, &f.firsts);
+  s->spare.push_back(std::move(f));
   s->done_cv.notify_all();
   return MPM_OK;
 }
@@ -100,10 +108,13 @@ void serve_loop(mpm_server* s) {
   std::deque<Flight> fl;
   std::unique_lock<std::mutex> lk(s->mu);
   mpm_status st = MPM_OK;
+  uint32_t stable = 0;  // steps since the resident scenes last changed
+  bool graphs = false;
+  mpm_pool_use_graphs(s->pool, 0);
   for (;;) {
     if (s->stop && (!s->drain || (s->sched.waiting() == 0 && s->sched.running() == 0))) break;
     // admission (pool calls happen on this thread only; all of them are ordered on the pool's stream)
-    s->sched.admit(now_s(), [&](const mpm_serve::Request& r, bool pool_empty, uint32_t* scene) {
+    const int admitted = s->sched.admit(now_s(), [&](const mpm_serve::Request& r, bool pool_empty, uint32_t* scene) {
       const mpm_status a = mpm_pool_add_scene_from_template(s->pool, r.tmpl, scene);
       if (a == MPM_OK) return (int)mpm_serve::kAdmitted;
       if (a == MPM_ERR_OUT_OF_MEMORY && !pool_empty) return (int)mpm_serve::kNoRoom;
@@ -122,9 +133,17 @@ void serve_loop(mpm_server* s) {
       s->cv.wait(lk);
       continue;
     }
+    if (admitted) stable = 0;
+    // captured sort periods only while the resident scenes stay the same (a change drops them)
+    const bool want = s->cfg.graph_after && stable >= s->cfg.graph_after;
+    if (want != graphs) {
+      if ((st = mpm_pool_use_graphs(s->pool, want ? 1 : 0)) != MPM_OK) break;
+      graphs = want;
+    }
     Flight f;
     if ((st = new_flight(s, &f)) != MPM_OK) break;
     f.step = s->enqueued++;
+    f.sessions = s->sched.begin_step(f.step, &f.firsts);
     mpm_pool_info(s->pool, &f.particles, nullptr, nullptr);
     lk.unlock();
     hipEventRecord(f.e0, stream);
@@ -132,13 +151,13 @@ void serve_loop(mpm_server* s) {
     hipEventRecord(f.e1, stream);
     lk.lock();
     if (st != MPM_OK) {
-      s->spare.push_back(f);
+      s->spare.push_back(std::move(f));
       break;
     }
-    fl.push_back(f);
+    fl.push_back(std::move(f));
     // the sessions this step ends leave now (their results, if asked for, are read first: that read
     // waits for the step), so the next step's admissions see their room
-    s->sched.after_step(f.step, now_s(), [&](const mpm_serve::Request& r) -> uint32_t {
+    const int retired = s->sched.after_step(f.step, now_s(), [&](const mpm_serve::Request& r) -> uint32_t {
       uint32_t n = 0;
       if (r.result && r.capacity) {
         const mpm_status rs = mpm_pool_read_scene(s->pool, r.scene, r.result, r.capacity, &n);
@@ -149,6 +168,7 @@ void serve_loop(mpm_server* s) {
       return n;
     });
     if (st != MPM_OK) break;
+    stable = retired ? 0 : stable + 1;
     mpm_pool_info(s->pool, &s->resident_particles, nullptr, nullptr);
     while (fl.size() > 1 && st == MPM_OK) st = finish_oldest(s, fl, lk);  // one step stays in flight
     if (st != MPM_OK) break;
@@ -232,6 +252,7 @@ mpm_status mpm_server_stats(mpm_server* s, mpm_server_counters* st) {
   st->particle_substeps = s->particle_substeps;
   st->resident_particles = s->resident_particles;
   st->error = s->error;
+  st->scene_steps = s->sched.scene_steps();
   return MPM_OK;
 }
 
@@ -246,6 +267,10 @@ mpm_status mpm_server_latencies(mpm_server* s, int32_t kind, double* out, uint64
       if (c.stamped) v.push_back(kind == MPM_LAT_REQUEST_S ? c.total_s : c.wait_s);
   } else if (kind == MPM_LAT_REQUEST_AGE_S) {
     v = s->sched.ages(now_s());
+  } else if (kind == MPM_LAT_SCENE_STEP_S) {
+    v = s->sched.scene_step_latencies();
+  } else if (kind == MPM_LAT_SCENE_STEP_AGE_S) {
+    v = s->sched.scene_step_ages(now_s());
   } else {
     return MPM_ERR_INVALID_ARGUMENT;
   }

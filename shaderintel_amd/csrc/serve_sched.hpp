@@ -7,6 +7,11 @@
 // lifetime has been enqueued, the session leaves and frees its room for the next step's admissions
 // (continuous batching); its latency is stamped once that step's device work is known to be done
 // (finish_step). A request that cannot fit even into an empty pool is rejected.
+//
+// Each step of a session is one scene-step request, issued when the session's previous step is done
+// (the first one when the session arrives): its latency runs from that moment to the end of the
+// step that serves it. Every resident session takes part in every step, so a session's requests
+// after its first are issued at the end of the previous step.
 #pragma once
 #include <cstdint>
 #include <deque>
@@ -24,6 +29,7 @@ struct Request {
   uint32_t scene = 0xffffffffu;
   float* result = nullptr;   // optional: the scene's records are copied here before it leaves
   uint32_t capacity = 0;     // records result holds
+  uint64_t first_step = ~0ull;  // index of the session's first step
 };
 
 struct Completed {
@@ -80,6 +86,17 @@ class Scheduler {
     return n;
   }
 
+  // Step `step` is about to be enqueued with every resident session: returns how many take part and
+  // appends the arrival times of those for which it is the first step.
+  uint32_t begin_step(uint64_t step, std::vector<double>* first_arrivals) {
+    for (Request& r : running_)
+      if (r.done == 0) {
+        r.first_step = step;
+        if (first_arrivals) first_arrivals->push_back(r.t_submit);
+      }
+    return (uint32_t)running_.size();
+  }
+
   // Step `step` of every resident session has been enqueued: sessions at the end of their lifetime
   // are retired (retire(const Request&) reads their results if asked and removes their scene) and
   // recorded, their latency provisional (`now`) until finish_step. Returns how many retired.
@@ -103,13 +120,42 @@ class Scheduler {
     return n;
   }
 
-  // Step `step` has finished on the device at time t: stamp the latency of the sessions it retired.
-  void finish_step(uint64_t step, double t) {
+  // Step `step` has finished on the device at time t: stamp the latency of the sessions it retired,
+  // and record the latencies of its scene-step requests: `sessions` took part, of which those whose
+  // arrival times are in first_arrivals took their first step (issued on arrival); the others were
+  // issued at the end of the previous step (t_last).
+  void finish_step(uint64_t step, double t, uint32_t sessions = 0, const std::vector<double>* first_arrivals = nullptr) {
     while (stamp_ < completed_.size() && completed_[stamp_].step <= step) {
       Completed& c = completed_[stamp_++];
       c.total_s = t - t_submit_[c.id];
       c.stamped = true;
     }
+    const uint32_t firsts = first_arrivals ? (uint32_t)first_arrivals->size() : 0u;
+    if (first_arrivals)
+      for (double a : *first_arrivals) step_lat_.push_back(t - a);
+    for (uint32_t k = firsts; k < sessions && has_last_; k++) step_lat_.push_back(t - t_last_);
+    scene_steps_ += sessions;
+    t_last_ = t;
+    has_last_ = true;
+    last_done_ = step;
+    any_done_ = true;
+  }
+
+  // Latencies of the scene-step requests served so far (s).
+  const std::vector<double>& scene_step_latencies() const { return step_lat_; }
+  uint64_t scene_steps() const { return scene_steps_; }
+
+  // Censored scene-step ages at time `now`: the served ones by their latency; each resident session's
+  // pending request by its age (issued at the end of the last finished step once the session has
+  // had one, else on arrival); each waiting session's first request by its age.
+  std::vector<double> scene_step_ages(double now) const {
+    std::vector<double> v = step_lat_;
+    for (const auto& r : running_) {
+      const bool had_one = any_done_ && r.first_step != ~0ull && r.first_step <= last_done_;
+      v.push_back(now - (had_one ? t_last_ : r.t_submit));
+    }
+    for (const auto& r : waiting_) v.push_back(now - r.t_submit);
+    return v;
   }
 
   // Censored request ages at time `now`: completed requests by their latency, waiting and running
@@ -135,6 +181,10 @@ class Scheduler {
  private:
   uint64_t next_id_ = 0;
   size_t stamp_ = 0;
+  std::vector<double> step_lat_;
+  uint64_t scene_steps_ = 0, last_done_ = 0;
+  double t_last_ = 0;
+  bool has_last_ = false, any_done_ = false;
   std::deque<Request> waiting_;
   std::vector<Request> running_;
   std::vector<Completed> completed_;

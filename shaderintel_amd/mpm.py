@@ -119,14 +119,15 @@ GRID_HOOK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(GridView))
 
 
 class _ServerConfig(ctypes.Structure):
-    _fields_ = [("substeps", ctypes.c_uint32), ("floor_friction", ctypes.c_float), ("vmax", ctypes.c_float)]
+    _fields_ = [("substeps", ctypes.c_uint32), ("floor_friction", ctypes.c_float), ("vmax", ctypes.c_float),
+                ("graph_after", ctypes.c_uint32)]
 
 
 class _ServerCounters(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
                 ("waiting", ctypes.c_uint64), ("running", ctypes.c_uint64), ("steps", ctypes.c_uint64),
                 ("particle_substeps", ctypes.c_uint64), ("resident_particles", ctypes.c_uint32),
-                ("error", ctypes.c_int32)]
+                ("error", ctypes.c_int32), ("scene_steps", ctypes.c_uint64)]
 
 
 _lib = None
@@ -917,9 +918,10 @@ class Server:
     """The native serving loop over a ScenePool (include/mpm.h, mpm_server_*). While it runs, its
     thread owns the pool."""
 
-    def __init__(self, pool: ScenePool, substeps: int = 4, floor_friction: float = 0.4, vmax: float = 0.9):
+    def __init__(self, pool: ScenePool, substeps: int = 4, floor_friction: float = 0.4, vmax: float = 0.9,
+                 graph_after: int = 0):
         self.pool = pool
-        cfg = _ServerConfig(int(substeps), float(floor_friction), float(vmax))
+        cfg = _ServerConfig(int(substeps), float(floor_friction), float(vmax), int(graph_after))
         self._results: dict = {}  # result arrays the native loop writes into, kept alive until settled
         h = ctypes.c_void_p()
         _check(load_library().mpm_server_create(pool._h, ctypes.byref(cfg), ctypes.byref(h)), "Server")
@@ -958,7 +960,7 @@ class Server:
         return {k: getattr(c, k) for k, _ in _ServerCounters._fields_}
 
     def latencies(self, kind: str) -> np.ndarray:
-        code = {"step_ms": 0, "request_s": 1, "wait_s": 2, "age_s": 3}[kind]
+        code = {"step_ms": 0, "request_s": 1, "wait_s": 2, "age_s": 3, "scene_step_s": 4, "scene_step_age_s": 5}[kind]
         n = ctypes.c_uint64()
         L = load_library()
         _check(L.mpm_server_latencies(self._h, code, None, 0, ctypes.byref(n)), "Server.latencies")

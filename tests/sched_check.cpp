@@ -1,7 +1,8 @@
 // CPU check of the serving scheduler (shaderintel_amd/csrc/serve_sched.hpp) against a fake pool with
 // a small particle budget: FIFO admission without overtaking, retirement after the lifetime, rejection
 // of requests that cannot fit an empty pool, ticket states, result counts, latencies stamped when a
-// step's device work ends, and censored ages. Exit status 0 = pass.
+// step's device work ends, censored ages, and the scene-step requests' latencies and ages. Exit
+// status 0 = pass.
 #include <cstdio>
 #include <map>
 #include <vector>
@@ -98,6 +99,38 @@ int main() {
     CHECK(!s.completed().back().stamped);
     s.finish_step(k, 14.0);
     CHECK(s.completed().back().stamped && s.completed().back().total_s == 14.0 - 10.5);  // the one submitted at 10.5
+  }
+  {  // scene-step requests: a session's first on arrival, the next ones at the end of the previous step
+    Scheduler q;
+    FakePool qp{100};
+    qp.size_of_tmpl = {{0, 10}};
+    auto qadmit = [&](double t) {
+      return q.admit(t, [&](const Request& r, bool empty, uint32_t* sc) { return qp.try_admit(r, empty, sc); });
+    };
+    auto qstep = [&](uint64_t k, double t_end) {
+      std::vector<double> firsts;
+      const uint32_t n = q.begin_step(k, &firsts);
+      q.after_step(k, t_end - 0.1, [&](const Request& r) { qp.remove(r.scene); return 0u; });
+      q.finish_step(k, t_end, n, &firsts);
+      return n;
+    };
+    q.submit(0, 3, 0.0);  // A: three steps
+    q.submit(0, 1, 0.2);  // B: one step
+    CHECK(qadmit(0.5) == 2);
+    CHECK(qstep(0, 1.0) == 2);  // A's and B's first requests: issued at 0.0 and 0.2
+    q.submit(0, 1, 1.1);        // C
+    CHECK(qadmit(1.2) == 1);
+    CHECK(qstep(1, 1.5) == 2);  // A's second (issued at 1.0) and C's first (issued at 1.1)
+    const std::vector<double>& l = q.scene_step_latencies();
+    CHECK(l.size() == 4 && l[0] == 1.0 && l[1] > 0.79 && l[1] < 0.81 && l[2] > 0.39 && l[2] < 0.41 && l[3] == 0.5);
+    CHECK(q.scene_steps() == 4);
+    q.submit(0, 2, 1.6);  // D: waiting (not admitted yet)
+    const std::vector<double> a = q.scene_step_ages(1.7);
+    // the four served, then A (pending since 1.5), then D (since 1.6); C retired
+    CHECK(a.size() == 6 && a[4] > 0.19 && a[4] < 0.21 && a[5] > 0.09 && a[5] < 0.11);
+    CHECK(qadmit(1.7) == 1);
+    const std::vector<double> b = q.scene_step_ages(1.8);  // D resident, its first request pending since 1.6
+    CHECK(b.size() == 6 && b[5] > 0.19 && b[5] < 0.21);
   }
   std::printf("ok\n");
   return 0;
