@@ -1,0 +1,72 @@
+// Work-item scheduling and tile-arena arithmetic of the item-persistent kernels, in host + device
+// form: the kernels (mpm_kernels.hip) call these with gridDim / blockIdx and device atomics, and a
+// CPU check (tests/item_loop_check.cpp) runs the same functions over a full scene's item list with
+// bounds asserts.
+#pragma once
+#include <stdint.h>
+
+#include "mpm_kernels.hpp"
+#include "mpm_math.hpp"
+
+namespace mpm {
+
+// Work items of an item-persistent kernel. Workgroups are dealt to the 8 XCDs in turn (b % 8), and
+// each XCD takes contiguous items so that neighbouring blocks share an L2.
+//  - first_item_of: the first s = min(items, 8 floor(grid / 8)) items are assigned statically, XCD x
+//    taking [s x / 8, s (x + 1) / 8) in workgroup order: no atomic for a workgroup's first item, and a
+//    workgroup without one (fewer items than workgroups) learns it from one load.
+//  - next_item_of: the items after s are split into kWorkSplits contiguous ranges, each with its own
+//    counter (take(range) = that counter's atomic fetch-and-increment); a workgroup starts on range
+//    blockIdx % kWorkSplits (its XCD) and moves on when its range is used up. Called by thread 0;
+//    0xffffffff when every range is done (at once when no items are left after the static ones).
+MPM_HD uint32_t static_items_of(uint32_t items, uint32_t grid) {
+  const uint32_t s = grid / 8u * 8u;
+  return items < s ? items : s;
+}
+MPM_HD uint32_t first_item_of(uint32_t items, uint32_t grid, uint32_t block) {
+  const uint32_t s = static_items_of(items, grid), x = block % 8u, i = block / 8u;
+  const uint32_t lo = (uint32_t)((uint64_t)s * x / 8u), hi = (uint32_t)((uint64_t)s * (x + 1u) / 8u);
+  return lo + i < hi ? lo + i : 0xffffffffu;
+}
+template <class Take>
+MPM_HD uint32_t next_item_of(Take&& take, uint32_t items, uint32_t grid, uint32_t& range, uint32_t& spent) {
+  const uint32_t s = static_items_of(items, grid), rest = items - s;
+  if (rest == 0) return 0xffffffffu;
+  while (spent < (uint32_t)kWorkSplits) {
+    const uint32_t lo = s + (uint32_t)((uint64_t)rest * range / kWorkSplits);
+    const uint32_t hi = s + (uint32_t)((uint64_t)rest * (range + 1) / kWorkSplits);
+    const uint32_t c = take(range);
+    if (c < hi - lo) return lo + c;
+    spent++;
+    range = (range + 1) % kWorkSplits;
+  }
+  return 0xffffffffu;
+}
+
+// One axis of a Morton code (the inverse of spread3): block coordinates from a block key.
+MPM_HD uint32_t compact3(uint32_t x) {
+  x &= 0x09249249u;
+  x = (x ^ (x >> 2)) & 0x030c30c3u;
+  x = (x ^ (x >> 4)) & 0x0300f00fu;
+  x = (x ^ (x >> 8)) & 0xff0000ffu;
+  x = (x ^ (x >> 16)) & 0x000003ffu;
+  return x;
+}
+
+// A stencil base (b0, b1, b2) is in the window of the work item whose lowest in-window base is
+// (wx, wy, wz) when it lies within kDrift cells of the item's 4^3-cell block on every axis.
+MPM_HD bool in_window(int b0, int b1, int b2, int wx, int wy, int wz) {
+  return (unsigned)(b0 - wx) < (unsigned)(4 + 2 * kDrift) && (unsigned)(b1 - wy) < (unsigned)(4 + 2 * kDrift) &&
+         (unsigned)(b2 - wz) < (unsigned)(4 + 2 * kDrift);
+}
+
+// Arena offset of a tile of `pnodes` nodes for which region `region` (rsize nodes) returned o from
+// its bump counter, or 0xffffffff when it does not fit (the tile then goes to A with atomics). A
+// tile box never exceeds the window tile (pnodes <= kTileNodes, by the window test); the bound is
+// checked here too, so no box, whatever its size, can take space past its region's end.
+MPM_HD uint32_t arena_offset(uint32_t region, uint32_t rsize, uint32_t o, uint32_t pnodes) {
+  const bool fits = pnodes <= (uint32_t)kTileNodes && pnodes <= rsize && o <= rsize - pnodes;
+  return fits ? region * rsize + o : 0xffffffffu;
+}
+
+}  // namespace mpm

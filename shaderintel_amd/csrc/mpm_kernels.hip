@@ -8,6 +8,7 @@
 
 #include <hip/hip_ext.h>
 
+#include "item_sched.hpp"
 #include "mpm_kernels.hpp"
 #include "mpm_math.hpp"
 
@@ -52,15 +53,6 @@ __device__ __forceinline__ void load_p2g_record(const float4* __restrict__ P, ui
                                                 float4 r[kP2GStreams]) {
 #pragma unroll
   for (int k = 0; k < kP2GStreams; k++) r[k] = P[(size_t)k * cap + i];
-}
-
-__device__ __forceinline__ uint32_t compact3(uint32_t x) {  // inverse of spread3
-  x &= 0x09249249u;
-  x = (x ^ (x >> 2)) & 0x030c30c3u;
-  x = (x ^ (x >> 4)) & 0x0300f00fu;
-  x = (x ^ (x >> 8)) & 0xff0000ffu;
-  x = (x ^ (x >> 16)) & 0x000003ffu;
-  return x;
 }
 
 // Sort key of a particle at x (x.xyz in its scene's lattice): the scene in the top bits, then the
@@ -418,33 +410,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
                [&](int bx, int by, int bz, int a, int b, int c) { return V[node_addr(bx + a, by + b, bz + c, geo)]; });
 }
 
-// Work items of an item-persistent kernel. Workgroups are dealt to the 8 XCDs in turn (b % 8), and
-// each XCD takes contiguous items so that neighbouring blocks share an L2.
-//  - first_item: the first s = min(items, 8 floor(grid / 8)) items are assigned statically, XCD x
-//    taking [s x / 8, s (x + 1) / 8) in workgroup order: no atomic for a workgroup's first item, and a
-//    workgroup without one (fewer items than workgroups) learns it from one load.
-//  - next_item: the items after s are split into kWorkSplits contiguous ranges, each with its own
-//    counter at wctr; a workgroup starts on range blockIdx % kWorkSplits (its XCD) and moves on when
-//    its range is used up. Called by thread 0; 0xffffffff when every range is done (at once when no
-//    items are left after the static ones).
-__device__ __forceinline__ uint32_t static_items(uint32_t items) { return min(items, gridDim.x / 8u * 8u); }
-__device__ __forceinline__ uint32_t first_item(uint32_t items) {
-  const uint32_t s = static_items(items), x = blockIdx.x % 8u, i = blockIdx.x / 8u;
-  const uint32_t lo = (uint32_t)((uint64_t)s * x / 8u), hi = (uint32_t)((uint64_t)s * (x + 1u) / 8u);
-  return lo + i < hi ? lo + i : 0xffffffffu;
-}
+// Work items (item_sched.hpp) with this launch's grid and the device work counters at wctr.
+__device__ __forceinline__ uint32_t first_item(uint32_t items) { return first_item_of(items, gridDim.x, blockIdx.x); }
 __device__ __forceinline__ uint32_t next_item(uint32_t* wctr, uint32_t items, uint32_t& range, uint32_t& spent) {
-  const uint32_t s = static_items(items), rest = items - s;
-  if (rest == 0) return 0xffffffffu;
-  while (spent < (uint32_t)kWorkSplits) {
-    const uint32_t lo = s + (uint32_t)((uint64_t)rest * range / kWorkSplits);
-    const uint32_t hi = s + (uint32_t)((uint64_t)rest * (range + 1) / kWorkSplits);
-    const uint32_t c = atomicAdd(wctr + kWorkStride * range, 1u);
-    if (c < hi - lo) return lo + c;
-    spent++;
-    range = (range + 1) % kWorkSplits;
-  }
-  return 0xffffffffu;
+  return next_item_of([&](uint32_t r) { return atomicAdd(wctr + kWorkStride * r, 1u); }, items, gridDim.x, range,
+                      spent);
 }
 
 // G2P over the work items of this substep's P2G. The item's tile box (tile_hdr: the nodes its
@@ -970,10 +940,8 @@ __device__ __forceinline__ void store_item_tile(const Pool& pool, uint32_t k, co
     s_item = next_item(cur.wctr, cur.items, cur.range, cur.spent);
     uint32_t off = 0xffffffffu;
     if (add) {
-      if (o <= rsize - (uint32_t)pnodes)  // pnodes <= kTileNodes < rsize
-        off = region * rsize + o;
-      else
-        cur.full = true;
+      off = arena_offset(region, rsize, o, (uint32_t)pnodes);
+      if (off == 0xffffffffu) cur.full = true;
     }
     s_off = off;
     pool.tile_off[k] = off;
@@ -1049,8 +1017,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
     int lo0 = INT_MAX, lo1 = INT_MAX, lo2 = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hi2 = INT_MIN;
     for (; li < cnt; li += kThreads) {
       const int b0 = (int)floorf(r[0].x - 0.5f), b1 = (int)floorf(r[0].y - 0.5f), b2 = (int)floorf(r[0].z - 0.5f);
-      const bool inwin = (unsigned)(b0 - wx) < (unsigned)(4 + 2 * kDrift) &&
-                         (unsigned)(b1 - wy) < (unsigned)(4 + 2 * kDrift) && (unsigned)(b2 - wz) < (unsigned)(4 + 2 * kDrift);
+      const bool inwin = in_window(b0, b1, b2, wx, wy, wz);
       if (inwin) {
         lo0 = min(lo0, b0); lo1 = min(lo1, b1); lo2 = min(lo2, b2);
         hi0 = max(hi0, b0); hi1 = max(hi1, b1); hi2 = max(hi2, b2);
@@ -1187,8 +1154,7 @@ __global__ __launch_bounds__(kFuseThreadsOf[SHAPE]) __attribute__((amdgpu_waves_
       if (keys_out) keys_out[sorted_out ? j : si] = sort_key(pool, wb >> mshift, geo.active, r[0]);
       // the next substep's P2G of the new record
       const int b0 = (int)floorf(r[0].x - 0.5f), b1 = (int)floorf(r[0].y - 0.5f), b2 = (int)floorf(r[0].z - 0.5f);
-      const bool inwin = (unsigned)(b0 - wx) < (unsigned)(4 + 2 * kDrift) &&
-                         (unsigned)(b1 - wy) < (unsigned)(4 + 2 * kDrift) && (unsigned)(b2 - wz) < (unsigned)(4 + 2 * kDrift);
+      const bool inwin = in_window(b0, b1, b2, wx, wy, wz);
       if (inwin) {
         lo0 = min(lo0, b0); lo1 = min(lo1, b1); lo2 = min(lo2, b2);
         hi0 = max(hi0, b0); hi1 = max(hi1, b1); hi2 = max(hi2, b2);
