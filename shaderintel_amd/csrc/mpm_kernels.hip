@@ -557,14 +557,10 @@ __global__ void k_begin_bin(uint32_t* ctr) {
   ctr[kCtrItems] = 0u;
 }
 
-// k_begin_bin, the map cleared to 0xffffffff (map_n words) and the occupancy bitmap to 0 (occ_n
-// 32-bit words), in one launch.
-__global__ void k_bin_reset(Pool pool, uint32_t map_n, uint32_t occ_n) {
+// k_begin_bin and the occupancy bitmap cleared (occ_n 32-bit words), in one launch.
+__global__ void k_bin_reset(Pool pool, uint32_t occ_n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < map_n)
-    pool.map[i] = 0xffffffffu;
-  else if (i - map_n < occ_n)
-    reinterpret_cast<uint32_t*>(pool.occ)[i - map_n] = 0u;
+  if (i < occ_n) reinterpret_cast<uint32_t*>(pool.occ)[i] = 0u;
   if (i == 0) {
     pool.ctr[kCtrBinEpoch] += 1u;
     pool.ctr[kCtrS] = 0u;
@@ -605,7 +601,7 @@ __global__ void k_permute(const float4* __restrict__ src, float4* __restrict__ d
 // Work items from the block-sorted keys: a head is a particle whose block (key >> 6) differs from
 // its predecessor's. Tiles of kItemTile keys, 16 consecutive keys per thread: k_bin_count stores
 // the heads of each tile, and after an exclusive scan of those counts k_bin_emit numbers the heads
-// in order (item k starts at the k-th head) and writes wl_start, wl_block and the map entry.
+// in order (item k starts at the k-th head) and writes wl_start and wl_block.
 constexpr int kItemPer = 16;
 constexpr uint32_t kItemTile = kThreads * kItemPer;
 
@@ -697,7 +693,6 @@ __global__ __launch_bounds__(kThreads) void k_bin_emit(Pool pool, const uint32_t
     const uint32_t blk = k[j] >> 6, scene = k[j] >> pool.key_bits, mb = blk & mmask;
     pool.wl_start[item] = first + j;
     pool.wl_block[item] = blk;
-    pool.map[pool.scenes[scene].map_base + mb] = item;
     {  // the item's velocity box until its first P2G: the stencils of bases in the block, in the grid
       const SceneGeo& g = pool.scenes[scene];
       const int ox = 4 * (int)compact3(mb), oy = 4 * (int)compact3(mb >> 1), oz = 4 * (int)compact3(mb >> 2);
@@ -868,6 +863,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_dilate(Pool pool, uint32_t max
       const int i = __builtin_ctzll(d);
       d &= d - 1;
       const uint32_t b = base + (uint32_t)(64 * w + i) + (uint32_t)nbx * ((uint32_t)by + (uint32_t)nby * (uint32_t)bz);
+      pool.sinfo[at] = make_uint2(scene, (uint32_t)(64 * w + i) | ((uint32_t)by << 10) | ((uint32_t)bz << 20));
       pool.slist[at++] = b;
       pool.sstamp[b] = epoch;
     }
@@ -922,7 +918,7 @@ struct ItemCursor {
 // workgroup past its size). Every thread of the workgroup calls it; s_item is read after the
 // caller's next barrier.
 template <int THREADS>
-__device__ __forceinline__ void store_item_tile(const Pool& pool, uint32_t k, const SceneGeo& geo, int* tile,
+__device__ __forceinline__ void store_item_tile(const Pool& pool, uint32_t k, uint32_t wb, const SceneGeo& geo, int* tile,
                                                 const int* bbox, int wx, int wy, int wz, uint32_t epoch,
                                                 uint32_t bin_epoch, uint32_t& s_off, ItemCursor& cur,
                                                 uint32_t& s_item) {
@@ -945,7 +941,10 @@ __device__ __forceinline__ void store_item_tile(const Pool& pool, uint32_t k, co
     }
     s_off = off;
     pool.tile_off[k] = off;
-    if (off != 0xffffffffu) pool.tile_hdr[k] = make_int4(px, py, pz, fx | (fy << 8) | (fz << 16));
+    const int ext = fx | (fy << 8) | (fz << 16);
+    if (off != 0xffffffffu) pool.tile_hdr[k] = make_int4(px, py, pz, ext);
+    // the grid update finds the tile by the block's slot (wb: scene and Morton block)
+    pool.mtile[wb] = make_uint4(off, (uint32_t)px | ((uint32_t)py << 10) | ((uint32_t)pz << 20), (uint32_t)ext, epoch);
   }
   __syncthreads();
   const uint32_t toff = s_off;
@@ -1044,7 +1043,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
       atomicMax(&bbox[3], hi0); atomicMax(&bbox[4], hi1); atomicMax(&bbox[5], hi2);
     }
     __syncthreads();
-    store_item_tile<kThreads>(pool, k, geo, tile, bbox, wx, wy, wz, epoch, bin_epoch, s_off, cur, s_item);
+    store_item_tile<kThreads>(pool, k, wb, geo, tile, bbox, wx, wy, wz, epoch, bin_epoch, s_off, cur, s_item);
     __syncthreads();  // the tile is read out and zero again
   }
 }
@@ -1182,7 +1181,7 @@ __global__ __launch_bounds__(kFuseThreadsOf[SHAPE]) __attribute__((amdgpu_waves_
       atomicMax(&bbox[3], hi0); atomicMax(&bbox[4], hi1); atomicMax(&bbox[5], hi2);
     }
     __syncthreads();
-    store_item_tile<kFuseThreads>(pool, k, geo, tile, bbox, wx, wy, wz, epoch, bin_epoch, s_off, cur, s_item);
+    store_item_tile<kFuseThreads>(pool, k, wb, geo, tile, bbox, wx, wy, wz, epoch, bin_epoch, s_off, cur, s_item);
     __syncthreads();  // the tile is read out and zero again
   }
 }
@@ -1203,32 +1202,31 @@ __global__ void k_write_words(Words64 src, uint32_t* dst, uint32_t n) {
   if (i < n) dst[i] = src.w[i];
 }
 
+constexpr int kPullBatch = 8;  // tile loads a grid-update wavefront keeps in flight
 // Sum of the P2G result at scene-local node (n0, n1, n2) of node block (g0, g1, g2): the tiles of
 // particle blocks G-2..G+1 per axis that cover the node. Lane L looks up candidate particle block L
-// (its map entry and tile header: one load each for the whole wave); the candidates whose tile box
-// overlaps the node block are then visited with wave-uniform tile indices, up to four tile loads in
-// flight at a time. Integer sums, so the visiting order does not matter.
-__device__ __forceinline__ int4 pull_tiles(const Pool& pool, uint32_t scene, int g0, int g1, int g2, int lane, int n0,
-                                           int n1, int n2, int4 acc) {
-  const int pb = 1 << (pool.key_bits / 3 - 2);  // particle blocks per axis a scene's map addresses
+// (its mtile slot: one load for the whole wave); the candidates whose tile box
+// overlaps the node block are then visited with wave-uniform tile indices, up to kPullBatch tile
+// loads in flight at a time. Integer sums, so the visiting order does not matter.
+__device__ __forceinline__ int4 pull_tiles(const Pool& pool, uint32_t scene, uint32_t epoch, int g0, int g1, int g2,
+                                           int lane, int n0, int n1, int n2, int4 acc) {
+  const int pb = 1 << (pool.key_bits / 3 - 2);  // particle blocks per axis a scene's slots address
   const int cx = g0 - 2 + (lane & 3), cy = g1 - 2 + ((lane >> 2) & 3), cz = g2 - 2 + (lane >> 4);
-  uint32_t k = 0xffffffffu, off = 0xffffffffu;
+  uint4 e = make_uint4(0xffffffffu, 0u, 0u, 0u);
   if ((unsigned)cx < (unsigned)pb && (unsigned)cy < (unsigned)pb && (unsigned)cz < (unsigned)pb)
-    k = pool.map[(scene << (pool.key_bits - 6)) + morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz)];
-  int4 h = make_int4(0, 0, 0, 0);
-  if (k != 0xffffffffu) {  // both in one round trip; the header counts only when off is valid
-    off = pool.tile_off[k];
-    h = pool.tile_hdr[k];
-  }
-  if (off == 0xffffffffu) h = make_int4(0, 0, 0, 0);
+    e = pool.mtile[(scene << (pool.key_bits - 6)) + morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz)];
+  const bool has = e.w == epoch && e.x != 0xffffffffu;  // a tile of this P2G
+  const uint32_t off = e.x;
+  const int4 h = has ? make_int4((int)(e.y & 1023u), (int)((e.y >> 10) & 1023u), (int)(e.y >> 20), (int)e.z)
+                     : make_int4(0, 0, 0, 0);
   const int ex = h.w & 255, ey = (h.w >> 8) & 255, ez = (h.w >> 16) & 255;
   const bool hit = ex > 0 && h.x < 4 * g0 + 4 && h.x + ex > 4 * g0 && h.y < 4 * g1 + 4 && h.y + ey > 4 * g1 &&
                    h.z < 4 * g2 + 4 && h.z + ez > 4 * g2;
   uint64_t mask = __builtin_amdgcn_ballot_w64(hit);
   while (mask) {
-    int4 q[4];
+    int4 q[kPullBatch];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < kPullBatch; u++) {
       q[u] = make_int4(0, 0, 0, 0);
       if (mask) {
         const int c = __builtin_ctzll(mask);
@@ -1243,7 +1241,7 @@ __device__ __forceinline__ int4 pull_tiles(const Pool& pool, uint32_t scene, int
       }
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < kPullBatch; u++) {
       acc.x = (int)((uint32_t)acc.x + (uint32_t)q[u].x);
       acc.y = (int)((uint32_t)acc.y + (uint32_t)q[u].y);
       acc.z = (int)((uint32_t)acc.z + (uint32_t)q[u].z);
@@ -1306,8 +1304,9 @@ __global__ __launch_bounds__(kThreads) void k_grid_pull(Pool pool, int use_s) {
   }
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (uint32_t j = blockIdx.x * 4 + w; j < ns + nd; j += gridDim.x * 4) {
+    // an S entry carries its block's info; a D entry (a block written through A) looks it up
     const uint32_t b = __builtin_amdgcn_readfirstlane(j < ns ? pool.slist[j] : pool.dlist[j - ns]);
-    const uint2 bi = pool.block_info[b];
+    const uint2 bi = j < ns ? pool.sinfo[j] : pool.block_info[b];
     const uint32_t scene = __builtin_amdgcn_readfirstlane(bi.x), bc = __builtin_amdgcn_readfirstlane(bi.y);
     const SceneGeo geo = uniform_geo(pool, scene);
     const int g0 = (int)(bc & 1023u), g1 = (int)((bc >> 10) & 1023u), g2 = (int)(bc >> 20);
@@ -1316,7 +1315,7 @@ __global__ __launch_bounds__(kThreads) void k_grid_pull(Pool pool, int use_s) {
     int4 acc = make_int4(0, 0, 0, 0);
     const bool written = pool.stamp[b] == epoch;
     if (written) acc = pool.A[addr];
-    if (j < ns) acc = pull_tiles(pool, scene, g0, g1, g2, lane, n0, n1, n2, acc);
+    if (j < ns) acc = pull_tiles(pool, scene, epoch, g0, g1, g2, lane, n0, n1, n2, acc);
     if (VELOCITY) {
       f3 vf;
       const float4 vo = grid_node(acc, n0, n1, n2, geo, pool.cols + 12 * kColRows * scene, vf);
@@ -1428,10 +1427,9 @@ hipError_t launch_begin_bin(const Pool& pool, hipStream_t s) {
   hipLaunchKernelGGL(k_begin_bin, dim3(1), dim3(1), 0, s, pool.ctr);
   return hipGetLastError();
 }
-hipError_t launch_bin_reset(const Pool& pool, uint32_t map_words, uint32_t occ_words32, hipStream_t s) {
-  const uint64_t n = (uint64_t)map_words + occ_words32;
-  if (n == 0 || n > 0xffffffffull) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_bin_reset, dim3(blocks_for((uint32_t)n, 256)), dim3(256), 0, s, pool, map_words, occ_words32);
+hipError_t launch_bin_reset(const Pool& pool, uint32_t occ_words32, hipStream_t s) {
+  if (occ_words32 == 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_bin_reset, dim3(blocks_for(occ_words32, 256)), dim3(256), 0, s, pool, occ_words32);
   return hipGetLastError();
 }
 hipError_t launch_keys(const Pool& pool, const float4* P, uint32_t n, uint32_t* keys, hipStream_t s) {

@@ -20,14 +20,15 @@
 //    velocity and mass after the grid update).
 //  - Binning (rebuilt at every sort): particles sorted by (scene, Morton code of their stencil base
 //    cell) form one contiguous run per 4^3-cell particle block. Each occupied block is a work item
-//    (start offset, key >> 6); map[map_base(scene) + Morton block] = item. Inside a block the order
-//    is (rank within cell, cell), so consecutive particles sit in different cells.
+//    (start offset, key >> 6). Inside a block the order is (rank within cell, cell), so
+//    consecutive particles sit in different cells.
 //  - Tiles: P2G gives every work item one workgroup that sums the 27 fixed-point contributions of
 //    its particles in an LDS tile (LDS integer atomics) sized to the bounding box of their stencils,
 //    takes that many nodes from one region of a bump-allocated arena and stores the box and the
-//    nodes there with plain stores (tile_hdr[item], tile_off[item]). The grid update of a node
-//    block pulls from the tiles of particle blocks G-2..G+1 per axis, so the tiled path needs no
-//    global atomic. A particle whose base drifted more than kDrift cells outside its block since
+//    nodes there with plain stores (tile_hdr[item], tile_off[item], and by the block's slot in
+//    mtile, stamped with the P2G epoch). The grid update of a node block pulls from the tiles of
+//    particle blocks G-2..G+1 per axis (their mtile slots), so the tiled path needs no global
+//    atomic. A particle whose base drifted more than kDrift cells outside its block since
 //    the binning, or every particle of an item whose tile does not fit its arena region, falls back
 //    to global atomics on A and stamps the node blocks it wrote.
 //  - Active node blocks: S = node blocks within reach of any work item (rebuilt at each binning);
@@ -89,9 +90,9 @@ enum Counter : int {
 struct SceneGeo {
   int32_t X, Y, Z;        // cells (= nodes) per axis
   int32_t nbx, nby, nbz;  // 4^3 node blocks per axis
-  int32_t mbits;          // bits per axis of a Morton particle-block coordinate (map has 2^(3*mbits))
+  int32_t mbits;          // bits per axis of a Morton particle-block coordinate (2^(3*mbits) slots)
   uint32_t block_base;    // first node block of the scene in the pool
-  uint32_t map_base;      // first map entry of the scene
+  uint32_t map_base;      // first particle-block slot of the scene (scene << 3 * mbits)
   float bound;            // boundary cells (mpm.js:103)
   float g[3];             // lattice gravity (mpm.js:588)
   float floor_friction;   // mpm.js:594
@@ -117,7 +118,11 @@ struct Pool {
   uint32_t* dlist;
   uint32_t* sstamp;        // S stamps (binning epoch)
   uint32_t* slist;
-  uint32_t* map;
+  uint2* sinfo;            // [nblocks] block_info of slist's entries, written beside them
+  uint4* mtile;            // [block slots] tile of the particle block at each slot (scene << (key_bits - 6)
+                           // | Morton block): (arena node, ox | oy << 10 | oz << 20, ex | ey << 8 | ez << 16,
+                           // P2G epoch that wrote it); an entry of another epoch, or arena node 0xffffffff,
+                           // has no tile
   uint32_t* wl_start;      // [max_items + 1]
   uint32_t* wl_block;      // [max_items] key >> 6 (scene bits on top)
   int4* tiles;             // [tile_nodes] arena
@@ -157,8 +162,8 @@ hipError_t launch_block_info(uint2* info, uint32_t base, uint32_t n, uint32_t sc
 hipError_t launch_begin_p2g(const Pool& pool, hipStream_t s);
 // ctr[kCtrBinEpoch]++, |S| = 0, items = 0 (start of a binning)
 hipError_t launch_begin_bin(const Pool& pool, hipStream_t s);
-// launch_begin_bin + the map set to 0xffffffff + the occupancy bitmap cleared (one launch)
-hipError_t launch_bin_reset(const Pool& pool, uint32_t map_words, uint32_t occ_words32, hipStream_t s);
+// launch_begin_bin + the occupancy bitmap cleared (one launch)
+hipError_t launch_bin_reset(const Pool& pool, uint32_t occ_words32, hipStream_t s);
 
 // Sort keys (scene << key_bits | Morton(base cell)) of the particles in their current order;
 // particles of inactive scenes get key 0xffffffff (sorted behind every live particle).
@@ -168,8 +173,8 @@ hipError_t launch_keys(const Pool& pool, const float4* P, uint32_t n, uint32_t* 
 hipError_t launch_live_keys(const Pool& pool, const float4* P, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t s);
 hipError_t launch_permute(const float4* src, float4* dst, uint32_t cap, uint32_t n, const uint32_t* perm, hipStream_t s);
 // Binning from the block-sorted keys: per-tile head counts (bin_tiles(n) + 1 words, the last one 0),
-// then, after an exclusive scan of those words (the last becomes the item count), the work items +
-// map; the within-block transposition of the sorted indices into `out`; and S.
+// then, after an exclusive scan of those words (the last becomes the item count), the work items;
+// the within-block transposition of the sorted indices into `out`; and S.
 uint32_t bin_tiles(uint32_t n);
 hipError_t launch_bin_count(const uint32_t* keys_sorted, uint32_t n, uint32_t* tile_heads, hipStream_t s);
 // scanned: tile_first holds the scanned head counts; otherwise the counts (bin_emit_scans(n) must hold)

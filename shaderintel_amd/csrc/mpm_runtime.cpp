@@ -151,7 +151,9 @@ struct mpm_pool {
   float4* V = nullptr;
   uint2* block_info = nullptr;
   uint32_t *stamp = nullptr, *dlist = nullptr, *sstamp = nullptr, *slist = nullptr;
-  uint32_t *map = nullptr, *wl_start = nullptr, *wl_block = nullptr;
+  uint2* sinfo = nullptr;
+  uint4* mtile = nullptr;
+  uint32_t *wl_start = nullptr, *wl_block = nullptr;
   int4 *tiles = nullptr, *tile_hdr = nullptr;
   uint32_t* tile_off = nullptr;
   float *mats = nullptr, *cols = nullptr;
@@ -237,7 +239,8 @@ Pool view(const mpm_pool* p) {
   v.dlist = p->dlist;
   v.sstamp = p->sstamp;
   v.slist = p->slist;
-  v.map = p->map;
+  v.sinfo = p->sinfo;
+  v.mtile = p->mtile;
   v.wl_start = p->wl_start;
   v.wl_block = p->wl_block;
   v.tiles = p->tiles;
@@ -272,9 +275,10 @@ void free_pool(mpm_pool* p) {
     if (t.rec) hipFree(t.rec);
   for (hipEvent_t e : p->ev_pool) hipEventDestroy(e);
   void* ptrs[] = {p->P[0],   p->P[1],     p->A,        p->V,        p->block_info, p->stamp,   p->dlist,
-                  p->sstamp, p->slist,    p->map,      p->wl_start, p->wl_block,    p->tiles,   p->tile_hdr, p->tile_off,
+                  p->sstamp, p->slist,    p->wl_start, p->wl_block,    p->tiles,   p->tile_hdr, p->tile_off,
                   p->mats,   p->cols,     p->scenes_d, p->ctr, p->arena_ctr,      p->occ,      p->keys[0],     p->keys[1], p->vals[0],
-                  p->vals[1], p->order,   p->flags,    p->sort_scratch, p->staging, p->export_buf, p->law};
+                  p->vals[1], p->order,   p->flags,    p->sort_scratch, p->staging, p->export_buf, p->law,
+                  p->sinfo,   p->mtile};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (p->stream) hipStreamDestroy(p->stream);
@@ -282,7 +286,7 @@ void free_pool(mpm_pool* p) {
 }
 
 // Device bytes a pool of this shape allocates (used for admission against an HBM budget).
-// 64-bit words of one scene's particle-block occupancy bitmap (mps = pb^3 map entries).
+// 64-bit words of one scene's particle-block occupancy bitmap (mps = pb^3 block slots).
 uint32_t occ_words(uint32_t mps) {
   uint32_t pb = 1;
   while (pb * pb * pb < mps) pb *= 2;
@@ -306,7 +310,8 @@ size_t pool_bytes(uint32_t cap, uint32_t max_scenes, uint32_t max_blocks, int ma
   b += 2 * (size_t)cap * 128;                            // P[2]
   b += (size_t)max_blocks * 64 * (16 + 16);              // A, V
   b += (size_t)max_blocks * 4 * 6;                       // block_info, stamp, dlist, sstamp, slist
-  b += (size_t)mps * max_scenes * 4;                     // map
+  b += (size_t)max_blocks * 8;                           // sinfo
+  b += (size_t)mps * max_scenes * 16;                    // mtile
   b += (size_t)occ_words(mps) * max_scenes * 8;          // occupancy bitmap
   b += (items + 1) * 4 * 2;                              // wl_start, wl_block
   b += arena * sizeof(int4) + items * 20;               // tile arena, headers, offsets
@@ -417,7 +422,7 @@ mpm_status apply_pending_reorder(mpm_pool* p) {
 }
 
 // Sort keys, then reorder the particles now (MPM._sort, mpm.js:614-621), at this substep's G2P, or
-// not at all (the sorted order is kept as a permutation), and rebuild work items, map and S.
+// not at all (the sorted order is kept as a permutation), and rebuild work items and S.
 mpm_status repack(mpm_pool* p);
 
 mpm_status do_bin(mpm_pool* p, int mode) {
@@ -441,9 +446,8 @@ mpm_status do_bin(mpm_pool* p, int mode) {
   // by the transposition
   HIP_TRY(radix_sort_pairs(p->keys[0], p->vals[0], p->keys[1], p->vals[1], p->count, 6, (int)p->sort_bits,
                            p->sort_scratch, p->stream, &rk, &rv, true, true));
-  // a new binning epoch, S and the items reset, the map and the occupancy bitmap cleared
-  HIP_TRY(launch_bin_reset(pv, p->map_per_scene * p->max_scenes, 2 * occ_words(p->map_per_scene) * p->max_scenes,
-                           p->stream));
+  // a new binning epoch, S and the items reset, the occupancy bitmap cleared
+  HIP_TRY(launch_bin_reset(pv, 2 * occ_words(p->map_per_scene) * p->max_scenes, p->stream));
   {  // work items: heads per tile, scanned, then numbered (the sort's scratch is free again)
     const uint32_t tiles = bin_tiles(p->count);
     uint32_t* tile_first = p->sort_scratch;
@@ -558,7 +562,7 @@ mpm_status mark(mpm_pool* p) {
 // writes the sort keys. presort_next (with keys_next): the next substep follows in the same call, and
 // its P2G may be done by this G2P too, over the current work items, ahead of its sort (P2G's sums
 // do not depend on the particle order or on the items they are gathered by); that substep then
-// runs its grid update before the sort rebuilds the map and S.
+// runs its grid update before the sort rebuilds the work items and S.
 mpm_status do_g2p(mpm_pool* p, float vmax, bool fuse_next, bool keys_next, bool presort_next) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (p->profiling == 2) {
@@ -817,7 +821,8 @@ mpm_status create_pool(mpm_ctx* c, uint32_t cap, uint32_t max_scenes, uint32_t m
   if (e == hipSuccess) e = dmalloc(&p->dlist, max_blocks);
   if (e == hipSuccess) e = dmalloc(&p->sstamp, max_blocks);
   if (e == hipSuccess) e = dmalloc(&p->slist, max_blocks);
-  if (e == hipSuccess) e = dmalloc(&p->map, (size_t)p->map_per_scene * max_scenes);
+  if (e == hipSuccess) e = dmalloc(&p->sinfo, max_blocks);
+  if (e == hipSuccess) e = dmalloc(&p->mtile, (size_t)p->map_per_scene * max_scenes);
   if (e == hipSuccess) e = dmalloc(&p->occ, (size_t)occ_words(p->map_per_scene) * max_scenes);
   if (e == hipSuccess) e = dmalloc(&p->wl_start, (size_t)p->max_items + 1);
   if (e == hipSuccess) e = dmalloc(&p->wl_block, p->max_items);
@@ -846,6 +851,8 @@ mpm_status create_pool(mpm_ctx* c, uint32_t cap, uint32_t max_scenes, uint32_t m
   if (e == hipSuccess) e = hipMemsetAsync(p->block_info, 0xff, (size_t)max_blocks * 8, p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->stamp, 0xff, (size_t)max_blocks * 4, p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->sstamp, 0xff, (size_t)max_blocks * 4, p->stream);
+  if (e == hipSuccess)  // epoch 0xffffffff: no P2G has written the entry
+    e = hipMemsetAsync(p->mtile, 0xff, (size_t)p->map_per_scene * max_scenes * sizeof(uint4), p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->ctr, 0, kCounters * 4, p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->arena_ctr, 0, (size_t)kArenaCtrWords * 4, p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->mats, 0, (size_t)max_scenes * kMatRows * 48, p->stream);
