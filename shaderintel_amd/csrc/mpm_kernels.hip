@@ -718,22 +718,13 @@ __global__ __launch_bounds__(kThreads) void k_bin_emit(Pool pool, const uint32_t
 // pos(r, c) = S(r) + #{c' < c : n_c' > r} with S(r) = sum_c' min(n_c', r) over the 64 cells c' of
 // the block with counts n_c'. One wavefront per item, lane c owning cell c: it counts the cells,
 // tabulates S(r) and the mask of cells with n_c' > r for r < kRankTable (a cell holds ~8
-// particles; S(r + 1) = S(r) + popcount(mask(r))), then ranks the item in index order (ballots find
-// the lanes of equal cell) so a position is one table read and a popcount.
+// particles; S(r + 1) = S(r) + popcount(mask(r))), then ranks the item in index order (per-cell
+// lane masks in LDS find the lanes of equal cell) so a position is one table read and a popcount.
 // out[start + pos] = in[start + j].
 constexpr int kRankTable = 128;
 constexpr int kBinWaves = kThreads / 64;
 constexpr int kCellCache = 1024;  // cells of an item's first 1024 particles kept in LDS by the count pass
-__device__ __forceinline__ uint64_t cell_peers(uint32_t c, uint64_t valid) {
-  uint64_t peers = valid;
-#pragma unroll
-  for (int b = 0; b < 6; b++) {
-    const bool bit = (c >> b) & 1u;
-    const uint64_t m = __ballot(bit);
-    peers &= bit ? m : ~m;
-  }
-  return peers;
-}
+constexpr int kTransposeBatch = 8;  // steps of 64 particles whose loads are issued together
 __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uint32_t* __restrict__ keys,
                                                             const uint32_t* __restrict__ in,
                                                             uint32_t* __restrict__ out) {
@@ -742,6 +733,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uin
   __shared__ uint32_t srank[kBinWaves][kRankTable];
   __shared__ uint64_t mrank[kBinWaves][kRankTable];
   __shared__ uint8_t wcell[kBinWaves][kCellCache];
+  __shared__ unsigned long long wmask[kBinWaves][64];  // lanes of one step per cell; zero between steps
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -751,16 +743,26 @@ __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uin
   // with another XCD's but at the ends
   const uint32_t per = (items + 7u) / 8u, xcd = blockIdx.x % 8u, wgs = gridDim.x / 8u;
   const uint32_t kend = min(items, (xcd + 1u) * per);
+  wmask[w][lane] = 0ull;
   for (uint32_t k = xcd * per + (blockIdx.x / 8u) * kBinWaves + w; k < kend; k += wgs * kBinWaves) {
     const uint32_t start = __builtin_amdgcn_readfirstlane(pool.wl_start[k]);
     const uint32_t cnt = __builtin_amdgcn_readfirstlane(pool.wl_start[k + 1]) - start;
     wrun[w][lane] = 0;
-    for (uint32_t j0 = 0; j0 < cnt; j0 += 64) {  // the cell counts (LDS atomics; their order does not matter)
-      const uint32_t j = j0 + lane;
-      if (j < cnt) {
-        const uint32_t c = keys[start + j] & 63u;
-        atomicAdd(&wrun[w][c], 1u);
-        if (j < (uint32_t)kCellCache) wcell[w][j] = (uint8_t)c;
+    // the cell counts (LDS atomics; their order does not matter), kTransposeBatch steps of loads at once
+    for (uint32_t j0 = 0; j0 < cnt; j0 += 64 * kTransposeBatch) {
+      uint32_t cb[kTransposeBatch];
+#pragma unroll
+      for (int u = 0; u < kTransposeBatch; u++) {
+        const uint32_t j = j0 + 64 * u + lane;
+        cb[u] = j < cnt ? keys[start + j] & 63u : 64u;
+      }
+#pragma unroll
+      for (int u = 0; u < kTransposeBatch; u++) {
+        const uint32_t j = j0 + 64 * u + lane;
+        if (cb[u] < 64u) {
+          atomicAdd(&wrun[w][cb[u]], 1u);
+          if (j < (uint32_t)kCellCache) wcell[w][j] = (uint8_t)cb[u];
+        }
       }
     }
     const uint32_t nc = wrun[w][lane];  // particles in cell `lane`
@@ -779,30 +781,52 @@ __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uin
       }
       sr += (uint32_t)__popcll(m);
     }
-    for (uint32_t j0 = 0; j0 < cnt; j0 += 64) {
-      const uint32_t j = j0 + lane;
-      const bool act = j < cnt;
-      const uint32_t v = act ? in[start + j] : 0u;
-      const uint32_t c = !act ? 0u : (j < (uint32_t)kCellCache ? (uint32_t)wcell[w][j] : keys[start + j] & 63u);
-      // rank within the cell: the count of its earlier steps (wrun) plus the lanes of this step
-      // below this one with the same cell (ballots over the cell's 6 bits; no atomic decides an order)
-      const uint64_t peers = cell_peers(c, __ballot(act));
-      const uint32_t r = act ? wrun[w][c] + (uint32_t)__popcll(peers & below) : 0u;
-      if (act) {
-        uint32_t pos;
-        if (r < (uint32_t)kRankTable) {
-          pos = srank[w][r] + (uint32_t)__builtin_popcountll(mrank[w][r] & ((1ull << c) - 1ull));
-        } else {
-          pos = 0;
-          for (uint32_t q = 0; q < 64; q++) {
-            const uint32_t nq = wcnt[w][q];
-            pos += min(nq, r) + ((q < c && nq > r) ? 1u : 0u);
-          }
-        }
-        out[start + pos] = v;
+    for (uint32_t j0 = 0; j0 < cnt; j0 += 64 * kTransposeBatch) {
+      uint32_t vb[kTransposeBatch], kb[kTransposeBatch];  // this batch's values (and keys past the cache)
+#pragma unroll
+      for (int u = 0; u < kTransposeBatch; u++) {
+        const uint32_t j = j0 + 64 * u + lane;
+        vb[u] = j < cnt ? in[start + j] : 0u;
+        kb[u] = j < cnt && j >= (uint32_t)kCellCache ? keys[start + j] & 63u : 0u;
       }
-      // every lane has read wrun[w][c]; the first lane of each cell advances it
-      if (act && (peers & below) == 0) wrun[w][c] += (uint32_t)__popcll(peers);
+#pragma unroll
+      for (int u = 0; u < kTransposeBatch; u++) {
+        if (j0 + 64 * u >= cnt) break;  // uniform
+        const uint32_t j = j0 + 64 * u + lane;
+        const bool act = j < cnt;
+        const uint32_t v = vb[u];
+        const uint32_t c = !act ? 0u : (j < (uint32_t)kCellCache ? (uint32_t)wcell[w][j] : kb[u]);
+        // rank within the cell: the count of its earlier steps (wrun) plus the lanes of this step
+        // below this one with the same cell. The lanes of a cell find each other by OR-ing their bits
+        // into the cell's mask and reading it back (ORs commute: no atomic decides an order; the
+        // wavefront's LDS operations complete in program order, and the fences keep the compiler to it)
+        if (act) __hip_atomic_fetch_or(&wmask[w][c], 1ull << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const uint64_t peers = act ? __hip_atomic_load(&wmask[w][c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) : 0ull;
+        const uint32_t r = act ? wrun[w][c] + (uint32_t)__popcll(peers & below) : 0u;
+        if (act) {
+          uint32_t pos;
+          if (r < (uint32_t)kRankTable) {
+            pos = srank[w][r] + (uint32_t)__builtin_popcountll(mrank[w][r] & ((1ull << c) - 1ull));
+          } else {
+            pos = 0;
+            for (uint32_t q = 0; q < 64; q++) {
+              const uint32_t nq = wcnt[w][q];
+              pos += min(nq, r) + ((q < c && nq > r) ? 1u : 0u);
+            }
+          }
+          out[start + pos] = v;
+        }
+        // every lane has read wrun[w][c] and the mask; the first lane of each cell advances the one and
+        // clears the other
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (act && (peers & below) == 0) {
+          wrun[w][c] += (uint32_t)__popcll(peers);
+          __hip_atomic_store(&wmask[w][c], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        }
+      }
     }
   }
 }
