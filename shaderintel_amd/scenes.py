@@ -10,6 +10,7 @@ given in cells as [lo, hi). Seeds follow MPM.add's default (count + 1, mpm.js:21
   falling at 2 cells/s: 1,048,576 particles.
 - ``mixed256``: 256^3 grid, four 64^3-cell blocks at y in [3,67): water, jelly, snow and
   Drucker-Prager sand (friction 30 deg), Gaussian velocity noise 0.1 cells/s: 8,388,608 particles.
+- ``mixed256-water`` / ``-jelly`` / ``-snow`` / ``-sand``: the same four blocks, all of one material.
 """
 from __future__ import annotations
 
@@ -37,6 +38,11 @@ SCENES = {
                    (SAND30, (128, 3, 128), (192, 67, 192), (0, 0, 0), 0.1)],
     },
 }
+
+# The 8M scene with all four blocks of one material (same boxes, noise and seeds): the per-material
+# cost of the substep (the snow and sand SVD against water's and jelly's closed forms).
+for _mat, _key in (("water", "water"), ("jelly", "jelly"), ("snow", "snow"), (SAND30, "sand")):
+    SCENES["mixed256-" + _key] = {"grid": 256, "blocks": [(_mat,) + b[1:] for b in SCENES["mixed256"]["blocks"]]}
 
 SUBSTEPS = 8
 DX = 1.0 / 64.0

@@ -41,7 +41,8 @@ def test_percentiles():
 def test_traffic_record(scene, particles):
     with open(os.path.join(ROOT, "profiles", f"traffic_{scene}.json")) as f:
         t = json.load(f)
-    assert t["workload"] == scene and t["kernel"] == "k_g2p_p2g"
+    assert t["workload"] == scene and t["kernel"] == "k_g2p_p2g" and t["instance"].startswith("k_g2p_p2g<")
+    assert t["limiter"]["unit"] == "VALU" and 0 < t["limiter"]["valu_busy"] <= 1
     assert t["algo_bytes"] == bench.ALGO_BYTES["mpm g2p_p2g"] * particles
     assert t["traffic_bytes"] == pytest.approx(t["read_bytes"] + t["write_bytes"])
     # the fused kernel writes each record once (128 B per particle) and its tiles; its reads are
