@@ -58,6 +58,16 @@ def build_library() -> str:
     return LIB
 
 
+def build_checks() -> str:
+    """The GPU math check library the tests call (tests/gpu_math_check.hip)."""
+    out_dir = os.path.join(ROOT, "tests", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "libgpu_math_check.so")
+    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{CSRC}", os.path.join(ROOT, "tests", "gpu_math_check.hip"), "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_addon() -> str | None:
     """Build the N-API addon if node headers are present (returns None otherwise)."""
     inc = "/usr/include/node"

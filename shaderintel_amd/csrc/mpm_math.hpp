@@ -160,6 +160,19 @@ MPM_HD float rsqrt_1to2(float x) {
   return y;
 }
 
+// sqrt(x) for x >= 1 (or +inf), correctly rounded as sqrtf: the hardware square root (within 1 ulp)
+// moved to the neighbour its residual points to. It is the compiler's own sequence for sqrtf without
+// the rescaling of small inputs and the select of +-0 and +inf, which x >= 1 does not need (+inf
+// passes through: both residuals are NaN). Checked bit for bit against sqrtf over every float in
+// [1, 4), [2^62, 2^64) and [2^126, 2^128) and at +inf (tests/gpu_math_check.hip).
+__device__ __forceinline__ float sqrt_ge1(float x) {
+  const float y = __builtin_amdgcn_sqrtf(x);
+  const float dn = __uint_as_float(__float_as_uint(y) - 1u), up = __uint_as_float(__float_as_uint(y) + 1u);
+  const float rdn = fmaf(-dn, y, x), rup = fmaf(-up, y, x);
+  const float r = rdn <= 0.0f ? dn : y;
+  return rup > 0.0f ? up : r;
+}
+
 // ---- 3x3 symmetric eigen-decomposition and SVD (cyclic Jacobi, 4 sweeps) ----
 MPM_HD float& el(m3& m, int c, int r) { return r == 0 ? m.c[c].x : (r == 1 ? m.c[c].y : m.c[c].z); }
 MPM_HD float el(const m3& m, int c, int r) { return r == 0 ? m.c[c].x : (r == 1 ? m.c[c].y : m.c[c].z); }
@@ -168,12 +181,12 @@ MPM_HD float el(const m3& m, int c, int r) { return r == 0 ? m.c[c].x : (r == 1 
 // Q change, each as the two-term sum the full 3x3 products would form (mul above: the first nonzero
 // product rounded, the second fused onto it; their other terms are multiples of zero).
 template <int P, int Q>
-MPM_HD void jacobi(m3& A, m3& V) {
+__device__ __forceinline__ void jacobi(m3& A, m3& V) {
   float apq = el(A, Q, P);
   if (fabsf(apq) < 1e-30f) return;
   float theta = (el(A, Q, Q) - el(A, P, P)) / (2.0f * apq);
   float sgn = theta >= 0.0f ? 1.0f : -1.0f;
-  float t = sgn / (fabsf(theta) + sqrtf(theta * theta + 1.0f));
+  float t = sgn / (fabsf(theta) + sqrt_ge1(theta * theta + 1.0f));  // theta^2 + 1 >= 1
   float c = rsqrt_1to2(t * t + 1.0f);  // inverseSqrt (linalg.js:20); |t| <= 1
   float s = t * c;
   float ms = -s;
@@ -194,7 +207,7 @@ MPM_HD void jacobi(m3& A, m3& V) {
   }
 }
 
-MPM_HD void svd3(const m3& F, m3& U, f3& s, m3& V) {
+__device__ __forceinline__ void svd3(const m3& F, m3& U, f3& s, m3& V) {
   m3 A = mul(tr(F), F);
   V = ident();
 #pragma unroll  // the 12 rotations unrolled: no loop-carried register moves

@@ -1,0 +1,40 @@
+// GPU check of the kernels' short square root (sqrt_ge1, shaderintel_amd/csrc/mpm_math.hpp) against
+// sqrtf, bit for bit, over every float of a range of bit patterns. Built as a small shared library by
+// the package build; tests/test_gpu_math.py calls it.
+#include <hip/hip_runtime.h>
+
+#include "mpm_math.hpp"
+
+namespace {
+__global__ void k_check_sqrt_ge1(uint32_t first, uint32_t count, unsigned long long* bad, uint32_t* first_bad) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t bits = first + i;
+  const float x = __uint_as_float(bits);
+  if (__float_as_uint(mpm::sqrt_ge1(x)) != __float_as_uint(sqrtf(x))) {
+    atomicAdd(bad, 1ull);
+    atomicMin(first_bad, bits);
+  }
+}
+}  // namespace
+
+// Mismatches of sqrt_ge1 against sqrtf over the bit patterns [first, first + count), and the first
+// mismatching pattern (0xffffffff if none). Returns a hipError_t code.
+extern "C" int check_sqrt_ge1(uint32_t first, uint32_t count, unsigned long long* mismatches, uint32_t* first_bad) {
+  unsigned long long* d_bad = nullptr;
+  uint32_t* d_first = nullptr;
+  hipError_t e = hipMalloc(&d_bad, 8);
+  if (e == hipSuccess) e = hipMalloc(&d_first, 4);
+  if (e == hipSuccess) e = hipMemset(d_bad, 0, 8);
+  if (e == hipSuccess) e = hipMemset(d_first, 0xff, 4);
+  if (e == hipSuccess && count) {
+    hipLaunchKernelGGL(k_check_sqrt_ge1, dim3((count + 255u) / 256u), dim3(256), 0, 0, first, count, d_bad, d_first);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(mismatches, d_bad, 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(first_bad, d_first, 4, hipMemcpyDeviceToHost);
+  if (d_bad) hipFree(d_bad);
+  if (d_first) hipFree(d_first);
+  return (int)e;
+}
