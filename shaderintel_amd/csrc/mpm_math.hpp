@@ -173,6 +173,18 @@ __device__ __forceinline__ float sqrt_ge1(float x) {
   return rup > 0.0f ? up : r;
 }
 
+// 1 / d for d >= 1, correctly rounded as 1.0f / d: the hardware reciprocal and one Newton step
+// (r1 = fma(fma(-d, r0, 1), r0, r0)) below 2^126, where the result is normal; IEEE division above
+// (denormal results, +inf). Checked bit for bit against 1.0f / d over every float in [1, 2), [2^20,
+// 2^21), [2^60, 2^61), [2^125, 2^128) and at +inf (tests/gpu_math_check.hip).
+__device__ __forceinline__ float rcp_ge1(float d) {
+  if (d < 0x1p126f) {
+    const float r0 = __builtin_amdgcn_rcpf(d);
+    return fmaf(fmaf(-d, r0, 1.0f), r0, r0);
+  }
+  return 1.0f / d;
+}
+
 // ---- 3x3 symmetric eigen-decomposition and SVD (cyclic Jacobi, 4 sweeps) ----
 MPM_HD float& el(m3& m, int c, int r) { return r == 0 ? m.c[c].x : (r == 1 ? m.c[c].y : m.c[c].z); }
 MPM_HD float el(const m3& m, int c, int r) { return r == 0 ? m.c[c].x : (r == 1 ? m.c[c].y : m.c[c].z); }
@@ -186,7 +198,8 @@ __device__ __forceinline__ void jacobi(m3& A, m3& V) {
   if (fabsf(apq) < 1e-30f) return;
   float theta = (el(A, Q, Q) - el(A, P, P)) / (2.0f * apq);
   float sgn = theta >= 0.0f ? 1.0f : -1.0f;
-  float t = sgn / (fabsf(theta) + sqrt_ge1(theta * theta + 1.0f));  // theta^2 + 1 >= 1
+  // t = sgn / d with d = |theta| + sqrt(theta^2 + 1) >= 1; (+-1) / d = +-(1 / d) exactly
+  float t = sgn * rcp_ge1(fabsf(theta) + sqrt_ge1(theta * theta + 1.0f));
   float c = rsqrt_1to2(t * t + 1.0f);  // inverseSqrt (linalg.js:20); |t| <= 1
   float s = t * c;
   float ms = -s;

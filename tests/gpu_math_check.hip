@@ -1,6 +1,7 @@
-// GPU check of the kernels' short square root (sqrt_ge1, shaderintel_amd/csrc/mpm_math.hpp) against
-// sqrtf, bit for bit, over every float of a range of bit patterns. Built as a small shared library by
-// the package build; tests/test_gpu_math.py calls it.
+// GPU check of the kernels' short square root and reciprocal (sqrt_ge1, rcp_ge1,
+// shaderintel_amd/csrc/mpm_math.hpp) against the IEEE operations, bit for bit, over every float of a
+// range of bit patterns. Built as a small shared library by the package build; tests/test_gpu_math.py
+// calls it.
 #include <hip/hip_runtime.h>
 
 #include "mpm_math.hpp"
@@ -16,11 +17,23 @@ __global__ void k_check_sqrt_ge1(uint32_t first, uint32_t count, unsigned long l
     atomicMin(first_bad, bits);
   }
 }
+__global__ void k_check_rcp_ge1(uint32_t first, uint32_t count, unsigned long long* bad, uint32_t* first_bad) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t bits = first + i;
+  const float d = __uint_as_float(bits);
+  if (__float_as_uint(mpm::rcp_ge1(d)) != __float_as_uint(1.0f / d)) {
+    atomicAdd(bad, 1ull);
+    atomicMin(first_bad, bits);
+  }
+}
 }  // namespace
 
-// Mismatches of sqrt_ge1 against sqrtf over the bit patterns [first, first + count), and the first
+// Mismatches against the IEEE operation over the bit patterns [first, first + count) of
+// sqrt_ge1 (which = 0, against sqrtf) or rcp_ge1 (which = 1, against 1.0f / d), and the first
 // mismatching pattern (0xffffffff if none). Returns a hipError_t code.
-extern "C" int check_sqrt_ge1(uint32_t first, uint32_t count, unsigned long long* mismatches, uint32_t* first_bad) {
+extern "C" int check_short_math(int which, uint32_t first, uint32_t count, unsigned long long* mismatches,
+                                uint32_t* first_bad) {
   unsigned long long* d_bad = nullptr;
   uint32_t* d_first = nullptr;
   hipError_t e = hipMalloc(&d_bad, 8);
@@ -28,7 +41,11 @@ extern "C" int check_sqrt_ge1(uint32_t first, uint32_t count, unsigned long long
   if (e == hipSuccess) e = hipMemset(d_bad, 0, 8);
   if (e == hipSuccess) e = hipMemset(d_first, 0xff, 4);
   if (e == hipSuccess && count) {
-    hipLaunchKernelGGL(k_check_sqrt_ge1, dim3((count + 255u) / 256u), dim3(256), 0, 0, first, count, d_bad, d_first);
+    const dim3 grid((count + 255u) / 256u), block(256);
+    if (which == 0)
+      hipLaunchKernelGGL(k_check_sqrt_ge1, grid, block, 0, 0, first, count, d_bad, d_first);
+    else
+      hipLaunchKernelGGL(k_check_rcp_ge1, grid, block, 0, 0, first, count, d_bad, d_first);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipDeviceSynchronize();
