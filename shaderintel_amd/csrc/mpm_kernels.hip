@@ -721,7 +721,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_emit(Pool pool, const uint32_t
 // particles; S(r + 1) = S(r) + popcount(mask(r))), then ranks the item in index order (per-cell
 // lane masks in LDS find the lanes of equal cell) so a position is one table read and a popcount.
 // out[start + pos] = in[start + j].
-constexpr int kRankTable = 128;
+constexpr int kRankTable = 256;
 constexpr int kBinWaves = kThreads / 64;
 constexpr int kCellCache = 1024;  // cells of an item's first 1024 particles kept in LDS by the count pass
 constexpr int kTransposeBatch = 8;  // steps of 64 particles whose loads are issued together
