@@ -564,8 +564,15 @@ inline uint32_t particle_key(const float* p) {
 
 // MPM._sort (mpm.js:614-621). Particles end up grouped by 4^3-cell block in Morton order, which is
 // the reference's order (its key is the block's Morton code); inside a block they are ordered by
-// (rank within their base cell, base cell), where the rank counts earlier particles of the same
-// cell in the input order. Consecutive particles therefore sit in different cells of the block.
+// (rank within their base cell, base cell numbered x fastest: cx + 4 cy + 16 cz), where the rank
+// counts earlier particles of the same cell in the input order. Consecutive particles therefore sit
+// in different cells of the block. (The reference's sort is not stable, so it fixes no order inside
+// a block; this one is the GPU's.)
+uint32_t cell_in_block(uint32_t key) {  // Morton bits x0 y0 z0 x1 y1 z1 -> cx + 4 cy + 16 cz
+  const uint32_t cx = (key & 1u) | ((key >> 2) & 2u), cy = ((key >> 1) & 1u) | ((key >> 3) & 2u);
+  const uint32_t cz = ((key >> 2) & 1u) | ((key >> 4) & 2u);
+  return cx + 4u * cy + 16u * cz;
+}
 void sort_particles(Scene& S) {
   const uint32_t n = S.count;
   std::vector<uint64_t> kv(n);
@@ -580,7 +587,7 @@ void sort_particles(Scene& S) {
     uint32_t rank = 0;
     for (uint32_t j = b0; j < b1; j++) {
       rank = (j > b0 && (kv[j] >> 32) == (kv[j - 1] >> 32)) ? rank + 1 : 0;
-      loc.push_back({((uint64_t)rank << 6) | ((kv[j] >> 32) & 63u), j});
+      loc.push_back({((uint64_t)rank << 6) | cell_in_block((uint32_t)(kv[j] >> 32)), j});
     }
     std::sort(loc.begin(), loc.end());
     for (uint32_t j = b0; j < b1; j++) order[j] = (uint32_t)(kv[loc[j - b0].second] & 0xffffffffu);

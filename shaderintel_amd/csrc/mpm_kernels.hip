@@ -713,14 +713,24 @@ __global__ __launch_bounds__(kThreads) void k_bin_emit(Pool pool, const uint32_t
 }
 
 // Within-block transposition: the particles of item k arrive grouped by block and otherwise in
-// their previous order; they leave ordered by (rank within cell, cell), where a particle's rank
-// within its cell counts the earlier particles of the same cell (a stable order). Output position
-// pos(r, c) = S(r) + #{c' < c : n_c' > r} with S(r) = sum_c' min(n_c', r) over the 64 cells c' of
-// the block with counts n_c'. One wavefront per item, lane c owning cell c: it counts the cells,
-// tabulates S(r) and the mask of cells with n_c' > r for r < kRankTable (a cell holds ~8
-// particles; S(r + 1) = S(r) + popcount(mask(r))), then ranks the item in index order (per-cell
-// lane masks in LDS find the lanes of equal cell) so a position is one table read and a popcount.
+// their previous order; they leave ordered by (rank within cell, cell_in_block), where a particle's
+// rank within its cell counts the earlier particles of the same cell (a stable order). Output
+// position pos(r, c) = S(r) + #{c' < c : n_c' > r} with S(r) = sum_c' min(n_c', r) over the 64 cells
+// c' of the block with counts n_c'. One wavefront per item, lane c owning cell c: it counts the
+// cells, tabulates S(r) and the mask of cells with n_c' > r for r < kRankTable (a cell holds ~8
+// particles, up to ~200 where a material lands), then ranks the item in index order (per-cell lane
+// masks in LDS find the lanes of equal cell) so a position is one table read and a popcount.
 // out[start + pos] = in[start + j].
+// A particle's cell inside its 4^3-cell block, numbered x fastest (cx + 4 cy + 16 cz) from the low 6
+// bits of its key (the cell's Morton code, bits x0 y0 z0 x1 y1 z1). Ordering a block's rank layers by
+// this number puts the 16 cells of one z layer in each quarter of a wavefront: the quarter that one
+// LDS cycle of a 96-bit velocity-box read serves then reads 16 distinct banks (fixed box strides 12
+// and 96), where Morton order put a 4 x 2 x 2 group there and read each bank twice.
+__device__ __forceinline__ uint32_t cell_in_block(uint32_t key) {
+  const uint32_t m = key & 63u;
+  return (m & 1u) | ((m >> 2) & 2u) | (((m >> 1) & 1u) << 2) | (((m >> 4) & 1u) << 3) | (((m >> 2) & 1u) << 4) |
+         (((m >> 5) & 1u) << 5);
+}
 constexpr int kRankTable = 256;
 constexpr int kBinWaves = kThreads / 64;
 constexpr int kCellCache = 1024;  // cells of an item's first 1024 particles kept in LDS by the count pass
@@ -754,7 +764,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uin
 #pragma unroll
       for (int u = 0; u < kTransposeBatch; u++) {
         const uint32_t j = j0 + 64 * u + lane;
-        cb[u] = j < cnt ? keys[start + j] & 63u : 64u;
+        cb[u] = j < cnt ? cell_in_block(keys[start + j]) : 64u;
       }
 #pragma unroll
       for (int u = 0; u < kTransposeBatch; u++) {
@@ -787,7 +797,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uin
       for (int u = 0; u < kTransposeBatch; u++) {
         const uint32_t j = j0 + 64 * u + lane;
         vb[u] = j < cnt ? in[start + j] : 0u;
-        kb[u] = j < cnt && j >= (uint32_t)kCellCache ? keys[start + j] & 63u : 0u;
+        kb[u] = j < cnt && j >= (uint32_t)kCellCache ? cell_in_block(keys[start + j]) : 0u;
       }
 #pragma unroll
       for (int u = 0; u < kTransposeBatch; u++) {

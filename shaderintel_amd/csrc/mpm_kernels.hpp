@@ -20,8 +20,9 @@
 //    velocity and mass after the grid update).
 //  - Binning (rebuilt at every sort): particles sorted by (scene, Morton code of their stencil base
 //    cell) form one contiguous run per 4^3-cell particle block. Each occupied block is a work item
-//    (start offset, key >> 6). Inside a block the order is (rank within cell, cell), so
-//    consecutive particles sit in different cells.
+//    (start offset, key >> 6). Inside a block the order is (rank within cell, cell numbered x
+//    fastest), so consecutive particles sit in different cells and a rank layer's z layers fill
+//    the quarters of a wavefront.
 //  - Tiles: P2G gives every work item one workgroup that sums the 27 fixed-point contributions of
 //    its particles in an LDS tile (LDS integer atomics) sized to the bounding box of their stencils,
 //    takes that many nodes from one region of a bump-allocated arena and stores the box and the

@@ -171,12 +171,14 @@ def test_sort_matches_oracle(ctx):
     want = sc.read(len(raw))
     assert np.array_equal(bits(got), bits(want)), _first_diff(got, want)
     # the reference's invariant (checkSorted on its 4^3-block Morton keys, mpm.js:567-568): block keys
-    # are non-decreasing; inside a block the order is (rank in cell, cell)
+    # are non-decreasing; inside a block the order is (rank in cell, cell numbered x fastest)
     keys = sc.keys(len(raw)).astype(np.int64)
     assert np.all(np.diff(keys >> 6) >= 0)
     blocks = keys >> 6
+    m = keys & 63  # the cell's Morton bits x0 y0 z0 x1 y1 z1
+    lin = ((m & 1) | ((m >> 2) & 2)) + 4 * (((m >> 1) & 1) | ((m >> 3) & 2)) + 16 * (((m >> 2) & 1) | ((m >> 4) & 2))
     for b in np.unique(blocks)[:50]:
-        cells = keys[blocks == b] & 63
+        cells = lin[blocks == b]
         ranks = np.array([np.sum(cells[:i] == c) for i, c in enumerate(cells)])
         order_key = ranks * 64 + cells
         assert np.all(np.diff(order_key) > 0)
