@@ -218,6 +218,36 @@ def test_server_completes_requests_and_rejects_oversize(ctx):
     req = srv.latencies("request_s")
     wait = srv.latencies("wait_s")
     assert len(req) == 12 and np.all(req >= wait) and np.all(wait >= 0)
+    # scene-step requests: one per step of each completed session (lifetimes 1 + k % 4), each served
+    # after it was issued; after a drain nothing is pending, so the censored series is the same
+    steps = srv.latencies("scene_step_s")
+    assert st["scene_steps"] == len(steps) == sum(1 + k % 4 for k in range(12))
+    assert np.all(steps > 0)
+    assert np.array_equal(np.sort(srv.latencies("scene_step_age_s")), np.sort(steps))
     srv.destroy()
     assert pool.info()["scenes"] == 0
     pool.destroy()
+
+
+def test_server_graph_after_keeps_results(ctx):
+    """With captured sort periods replayed once the resident scenes are unchanged for 2 steps, served
+    scenes equal the same sessions served eagerly (bit for bit)."""
+    _say("[gpu] serving loop: graph_after 2 against eager")
+    from shaderintel_amd.mpm import Server
+    built = _build_specs()[:2]
+    out = {}
+    for ga in (0, 2):
+        pool = ScenePool(ctx, sum(len(b[2]) for b in built) + 1000, 4, 2 * sum(node_blocks(b[0]) for b in built), 64)
+        tids = [pool.add_template(d, t, r, g) for d, t, r, g in built]
+        srv = Server(pool, substeps=4, graph_after=ga)
+        res = [np.zeros((len(built[i][2]), 32), np.float32) for i in range(2)]
+        tickets = [srv.submit(tids[0], 9, result=res[0]), srv.submit(tids[1], 6, result=res[1])]
+        for t in tickets:
+            assert srv.wait(t, 60000)[0] == "done"
+        srv.stop(drain=True)
+        assert srv.stats()["error"] == 0
+        srv.destroy()
+        pool.destroy()
+        out[ga] = res
+    for a, b in zip(out[0], out[2]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
