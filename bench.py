@@ -378,6 +378,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--scene", default="mixed256")
     ap.add_argument("--cpu-substeps", type=int, default=10)
+    ap.add_argument("--time-every", type=int, default=7,
+                    help="time every N-th G2P launch of the timed region with its own dispatch events (1: all)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", action="store_true", help="time each tuning knob setting and exit")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched comparison")
@@ -435,8 +437,10 @@ def main() -> int:
                               "particle_substeps_per_s": round(n * args.steps / dt, 1)}), flush=True)
         return 0
     # The timed region. G2P (the dominant kernel) is timed by its own dispatches: the launch records
-    # its start and end, so no packet is added to the stream and the substeps run as unprofiled.
-    sim.set_profiling(2)
+    # its start and end, so no packet is added to the stream. Every --time-every-th launch is timed
+    # (7, prime to the sort period of 4, samples every position in it); a timed launch starts and ends
+    # a few microseconds later than an untimed one, so the others run as unprofiled.
+    sim.set_profiling(2, every=args.time_every)
     sim.sync()
     t0 = time.perf_counter()
     run(args.steps)
@@ -479,7 +483,7 @@ def main() -> int:
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom.replace(" ", "_"),
                 "algo_bytes_per_launch": ALGO_BYTES[dom] * n,
                 "launches": {k: {"ms_total": round(v[0], 3), "launches": v[1]} for k, v in ktimes.items()},
-                "avg_launch_ms": round(per_launch[dom], 4)}
+                "avg_launch_ms": round(per_launch[dom], 4), "timed_every": args.time_every}
     # HBM bytes per launch of the dominant kernel, from the committed rocprofv3 counter passes
     # (FETCH_SIZE / WRITE_SIZE scaled by a copy kernel of known size) on the same workload
     tpath = os.path.join(ROOT, "profiles", f"traffic_{args.scene}.json")

@@ -180,6 +180,10 @@ mpm_status mpm_phase_times(mpm_sim* sim, float ms[4]);
  * launches since profiling was enabled, split into {G2P alone, G2P fused with the next substep's
  * P2G}. */
 mpm_status mpm_kernel_times(mpm_sim* sim, float ms[2], uint32_t launches[2]);
+/* Profiling mode 2 times only G2P launches 0, every, 2 every, ... counted from when profiling was
+ * enabled (default every = 1: all of them); the launches in between run without events. No
+ * counterpart in the reference. */
+mpm_status mpm_set_profiling_every(mpm_sim* sim, uint32_t every);
 
 /* Tuning knobs (no counterpart in the reference; results do not depend on them — every variant
  * computes the same integer grid sums and the same per-particle float sequence). */

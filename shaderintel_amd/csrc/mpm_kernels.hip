@@ -320,13 +320,14 @@ __device__ __forceinline__ void g2p_finish(const float* __restrict__ mats, const
     const float4 f6 = src[(size_t)6 * cap + si], f7 = src[(size_t)7 * cap + si];
     F = mul(ident() + C, mat(mk(f6.x, f6.y, f6.z), mk(f6.w, f7.x, f7.y), mk(f7.z, f7.w, s5.w)));
     const float mu0 = m0[2], la0 = m0[3];
+    // snow and sand share one SVD of F (one copy of its code; a wavefront holding both runs it once)
+    m3 U, Vm;
+    f3 sv;
+    if (kind != 1u) svd3(F, U, sv, Vm);
     if (kind == 1u) {
       const float J = fmaxf(det3(F), 0.05f);
       tau = mu0 * (mul(F, tr(F)) - ident()) + (la0 * log_cw(J)) * ident();
     } else if (kind == 2u) {
-      m3 U, Vm;
-      f3 sv;
-      svd3(F, U, sv, Vm);
       const float clo = 1.0f - m0[6], chi = 1.0f + m0[7];
       const f3 s = mk(clampf(sv.x, clo, chi), clampf(sv.y, clo, chi), clampf(sv.z, clo, chi));
       const float Jold = (sv.x * sv.y) * sv.z;
@@ -339,9 +340,6 @@ __device__ __forceinline__ void g2p_finish(const float* __restrict__ mats, const
       const float J = Jnew;
       tau = mul((2.0f * mu) * (F - R), tr(F)) + ((la * J) * (J - 1.0f)) * ident();
     } else {
-      m3 U, Vm;
-      f3 sv;
-      svd3(F, U, sv, Vm);
       f3 e = mk(log_cw(fmaxf(fabsf(sv.x), 1e-4f)), log_cw(fmaxf(fabsf(sv.y), 1e-4f)), log_cw(fmaxf(fabsf(sv.z), 1e-4f)));
       const float trc = (e.x + e.y) + e.z;
       const f3 dev = e - mk(trc / 3.0f, trc / 3.0f, trc / 3.0f);

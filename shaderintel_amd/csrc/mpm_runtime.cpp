@@ -201,8 +201,10 @@ struct mpm_pool {
   hipGraphExec_t period_graph[2][2][2] = {};
   uint64_t graph_key = 0;                                // (count, scenes, vmax) the graphs were captured for
   // profiling 1: five events per profiled substep (sort, p2g, grid, g2p start; end);
-  // profiling 2: two events per G2P launch, recorded by the dispatch itself (start, end)
+  // profiling 2: two events per timed G2P launch, recorded by the dispatch itself (start, end)
   int profiling = 0;
+  uint32_t prof_every = 1;  // profiling 2 times G2P launches 0, every, 2 every, ... since it was enabled
+  uint32_t prof_seq = 0;    // G2P launches since profiling 2 was enabled
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   float phase_ms[4] = {0, 0, 0, 0};
@@ -565,7 +567,7 @@ mpm_status mark(mpm_pool* p) {
 // runs its grid update before the sort rebuilds the work items and S.
 mpm_status do_g2p(mpm_pool* p, float vmax, bool fuse_next, bool keys_next, bool presort_next) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (p->profiling == 2) {
+  if (p->profiling == 2 && p->prof_seq++ % p->prof_every == 0) {
     ST_TRY(next_event(p, &e0));
     ST_TRY(next_event(p, &e1));
   }
@@ -1500,12 +1502,21 @@ mpm_status mpm_set_profiling(mpm_sim* s, int32_t enable) {
   mpm_pool* p = s->pool;
   if (enable < 0 || enable > 2) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_profiling: mode %d not in 0..2", enable);
   p->profiling = enable;
+  p->prof_seq = 0;
   p->ev_used = 0;
   p->ev_fused.clear();
   p->ev_presort.clear();
   for (float& m : p->phase_ms) m = 0.0f;
   p->kernel_ms[0] = p->kernel_ms[1] = 0.0f;
   p->kernel_launches[0] = p->kernel_launches[1] = 0;
+  return MPM_OK;
+}
+
+mpm_status mpm_set_profiling_every(mpm_sim* s, uint32_t every) {
+  if (!s) return fail(MPM_ERR_INVALID_ARGUMENT, "simulation is NULL");
+  if (every == 0) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_profiling_every: every = 0");
+  s->pool->prof_every = every;
+  s->pool->prof_seq = 0;
   return MPM_OK;
 }
 

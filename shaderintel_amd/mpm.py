@@ -165,6 +165,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "mpm_read_grid": (i32, [vp, vp]),
         "mpm_sync": (i32, [vp]),
         "mpm_set_profiling": (i32, [vp, i32]),
+        "mpm_set_profiling_every": (i32, [vp, u32]),
         "mpm_kernel_times": (i32, [vp, vp, vp]),
         "mpm_set_tuning": (i32, [vp, i32, i32]),
         "mpm_phase_times": (i32, [vp, vp]),
@@ -725,10 +726,12 @@ class MPM:
         for k, v in knobs.items():
             _check(load_library().mpm_set_tuning(self._h, self.TUNE[k], int(v)), "MPM.set_tuning")
 
-    def set_profiling(self, on) -> None:
+    def set_profiling(self, on, every: int = 1) -> None:
         """True / 1: events between the phases of every substep; 2: G2P launches only, timed by
-        their own dispatch (no extra packets on the stream); False / 0: off."""
+        their own dispatch (no extra packets on the stream), launches 0, every, 2 every, ... (the
+        others run without events); False / 0: off."""
         _check(load_library().mpm_set_profiling(self._h, int(on)))
+        _check(load_library().mpm_set_profiling_every(self._h, int(every)))
 
     def kernel_times(self) -> dict:
         """Profiling mode 2: {"g2p": (ms, launches), "g2p_p2g": (ms, launches)} (G2P alone, fused)."""
