@@ -1127,8 +1127,8 @@ __global__ __launch_bounds__(kFuseThreadsOf[SHAPE]) __attribute__((amdgpu_waves_
     const uint32_t wb = pool.wl_block[k];
     const bool has_box = pool.tile_off[k] != 0xffffffffu;
     const int4 h = pool.tile_hdr[k];  // meaningful when has_box; loaded beside tile_off
-    // this thread's first record, loaded while the velocity box is staged (later ones at the top of
-    // each pass: a record held one pass ahead takes the registers of a fourth wave per SIMD)
+    // this thread's first record, loaded while the velocity box is staged (later ones during the
+    // previous pass's P2G)
     uint32_t li = tid, si = 0;
     G2PIn in = {};
     if (li < cnt) {
@@ -1159,10 +1159,6 @@ __global__ __launch_bounds__(kFuseThreadsOf[SHAPE]) __attribute__((amdgpu_waves_
     int lo0 = INT_MAX, lo1 = INT_MAX, lo2 = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hi2 = INT_MIN;
     for (; li < cnt; li += kFuseThreads) {
       const uint32_t j = start + li;
-      if (li != (uint32_t)tid) {
-        si = perm ? perm[j] : j;
-        g2p_fetch(src, si, pool.cap, in);
-      }
       g2p_base(in);
       const bool inbox = nodes > 0 && (unsigned)(in.bx - ox) <= (unsigned)(ex - 3) &&
                          (unsigned)(in.by - oy) <= (unsigned)(ey - 3) && (unsigned)(in.bz - oz) <= (unsigned)(ez - 3);
@@ -1183,6 +1179,12 @@ __global__ __launch_bounds__(kFuseThreadsOf[SHAPE]) __attribute__((amdgpu_waves_
       float4 r[8];
       g2p_finish(pool.mats, geo, src, si, dst, sorted_out ? j : si, pool.cap, vmax, in, v, B, r);
       if (keys_out) keys_out[sorted_out ? j : si] = sort_key(pool, wb >> mshift, geo.active, r[0]);
+      // this thread's record of the next pass, loaded while this one's P2G runs (in and si are free now;
+      // in place, its slot is another particle's, which only this thread reads or writes)
+      if (li + kFuseThreads < cnt) {
+        si = perm ? perm[j + kFuseThreads] : j + kFuseThreads;
+        g2p_fetch(src, si, pool.cap, in);
+      }
       // the next substep's P2G of the new record
       const int b0 = (int)floorf(r[0].x - 0.5f), b1 = (int)floorf(r[0].y - 0.5f), b2 = (int)floorf(r[0].z - 0.5f);
       const bool inwin = in_window(b0, b1, b2, wx, wy, wz);
