@@ -60,6 +60,42 @@ MPM_HD bool in_window(int b0, int b1, int b2, int wx, int wy, int wz) {
          (unsigned)(b2 - wz) < (unsigned)(4 + 2 * kDrift);
 }
 
+// Tile storage order. A tile box (origin o, extents e per axis) is cut at node-block boundaries
+// (multiples of 4) into segments per axis; the chunks (z segment, y segment, x segment) are stored one
+// after another in that order, each x fastest, so the part of the tile one node block holds is one
+// contiguous run of at most 64 nodes (the grid update reads it as whole lines).
+// tile_segment: the segment of node g (o <= g < o + e) along one axis: its start in box coordinates,
+// its width, and g's place in it.
+MPM_HD void tile_segment(int o, int e, int g, int& start, int& width, int& at) {
+  const int b = g & ~3, lo = b > o ? b : o, hi = b + 4 < o + e ? b + 4 : o + e;
+  start = lo - o;
+  width = hi - lo;
+  at = g - lo;
+}
+// The chunk of node block (b0, b1, b2) in a box: its first node (offset) and widths, or false when the
+// box does not reach the block.
+MPM_HD bool tile_chunk(int ox, int oy, int oz, int ex, int ey, int ez, int b0, int b1, int b2, uint32_t& offset, int& wx,
+                       int& wy, int& wz, int& lo0, int& lo1, int& lo2) {
+  lo0 = 4 * b0 > ox ? 4 * b0 : ox;
+  lo1 = 4 * b1 > oy ? 4 * b1 : oy;
+  lo2 = 4 * b2 > oz ? 4 * b2 : oz;
+  wx = (4 * b0 + 4 < ox + ex ? 4 * b0 + 4 : ox + ex) - lo0;
+  wy = (4 * b1 + 4 < oy + ey ? 4 * b1 + 4 : oy + ey) - lo1;
+  wz = (4 * b2 + 4 < oz + ez ? 4 * b2 + 4 : oz + ez) - lo2;
+  if (wx <= 0 || wy <= 0 || wz <= 0) return false;
+  offset = (uint32_t)((lo2 - oz) * ey * ex + wz * ((lo1 - oy) * ex + wy * (lo0 - ox)));
+  return true;
+}
+// Storage index of box node (lx, ly, lz) (box coordinates) in a tile: a bijection of the box's nodes
+// onto [0, ex ey ez).
+MPM_HD uint32_t tile_index(int ox, int oy, int oz, int ex, int ey, int ez, int lx, int ly, int lz) {
+  int sx, wx, px, sy, wy, py, sz, wz, pz;
+  tile_segment(ox, ex, ox + lx, sx, wx, px);
+  tile_segment(oy, ey, oy + ly, sy, wy, py);
+  tile_segment(oz, ez, oz + lz, sz, wz, pz);
+  return (uint32_t)(sz * ey * ex + wz * (sy * ex + wy * sx) + px + wx * (py + wy * pz));
+}
+
 // Arena offset of a tile of `pnodes` nodes for which region `region` (rsize nodes) returned o from
 // its bump counter, or 0xffffffff when it does not fit (the tile then goes to A with atomics). A
 // tile box never exceeds the window tile (pnodes <= kTileNodes, by the window test); the bound is

@@ -991,7 +991,7 @@ __device__ __forceinline__ void store_item_tile(const Pool& pool, uint32_t k, ui
     tile[2 * kTileNodes + w] = 0;
     tile[3 * kTileNodes + w] = 0;
     if (toff != 0xffffffffu) {
-      pool.tiles[toff + (uint32_t)t] = q;
+      pool.tiles[toff + tile_index(px, py, pz, fx, fy, fz, lx, ly, lz)] = q;
     } else if (q.x | q.y | q.z | q.w) {  // no arena room: add the node to A
       const int gx = px + lx, gy = py + ly, gz = pz + lz;
       stamp_fallback(pool, geo.block_base + (uint32_t)(gx >> 2) + (uint32_t)geo.nbx * ((uint32_t)(gy >> 2) +
@@ -1239,9 +1239,10 @@ __global__ void k_write_words(Words64 src, uint32_t* dst, uint32_t n) {
 constexpr int kPullBatch = 8;  // tile loads a grid-update wavefront keeps in flight
 // Sum of the P2G result at scene-local node (n0, n1, n2) of node block (g0, g1, g2): the tiles of
 // particle blocks G-2..G+1 per axis that cover the node. Lane L looks up candidate particle block L
-// (its mtile slot: one load for the whole wave); the candidates whose tile box
-// overlaps the node block are then visited with wave-uniform tile indices, up to kPullBatch tile
-// loads in flight at a time. Integer sums, so the visiting order does not matter.
+// (its mtile slot: one load for the whole wave) and the run of its tile that holds the node block;
+// the candidates whose tile box overlaps the node block are then visited with wave-uniform tile
+// indices, up to kPullBatch tile loads (one contiguous run each) in flight at a time. Integer sums,
+// so the visiting order does not matter.
 __device__ __forceinline__ int4 pull_tiles(const Pool& pool, uint32_t scene, uint32_t epoch, int g0, int g1, int g2,
                                            int lane, int n0, int n1, int n2, int4 acc) {
   const int pb = 1 << (pool.key_bits / 3 - 2);  // particle blocks per axis a scene's slots address
@@ -1250,12 +1251,20 @@ __device__ __forceinline__ int4 pull_tiles(const Pool& pool, uint32_t scene, uin
   if ((unsigned)cx < (unsigned)pb && (unsigned)cy < (unsigned)pb && (unsigned)cz < (unsigned)pb)
     e = pool.mtile[(scene << (pool.key_bits - 6)) + morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz)];
   const bool has = e.w == epoch && e.x != 0xffffffffu;  // a tile of this P2G
-  const uint32_t off = e.x;
-  const int4 h = has ? make_int4((int)(e.y & 1023u), (int)((e.y >> 10) & 1023u), (int)(e.y >> 20), (int)e.z)
-                     : make_int4(0, 0, 0, 0);
-  const int ex = h.w & 255, ey = (h.w >> 8) & 255, ez = (h.w >> 16) & 255;
-  const bool hit = ex > 0 && h.x < 4 * g0 + 4 && h.x + ex > 4 * g0 && h.y < 4 * g1 + 4 && h.y + ey > 4 * g1 &&
-                   h.z < 4 * g2 + 4 && h.z + ez > 4 * g2;
+  // this lane's candidate tile: the chunk it holds for the wave's node block (tile_chunk), as the
+  // run's first node in the arena and its corner and widths packed into one word each
+  uint32_t run = 0, corner = 0, widths = 0;
+  bool hit = false;
+  if (has) {
+    const int ex = (int)(e.z & 255u), ey = (int)((e.z >> 8) & 255u), ez = (int)((e.z >> 16) & 255u);
+    uint32_t co;
+    int wx, wy, wz, lo0, lo1, lo2;
+    hit = tile_chunk((int)(e.y & 1023u), (int)((e.y >> 10) & 1023u), (int)(e.y >> 20), ex, ey, ez, g0, g1, g2, co, wx,
+                     wy, wz, lo0, lo1, lo2);
+    run = e.x + co;
+    corner = (uint32_t)lo0 | ((uint32_t)lo1 << 10) | ((uint32_t)lo2 << 20);
+    widths = (uint32_t)wx | ((uint32_t)wy << 4) | ((uint32_t)wz << 8);
+  }
   uint64_t mask = __builtin_amdgcn_ballot_w64(hit);
   while (mask) {
     int4 q[kPullBatch];
@@ -1265,13 +1274,13 @@ __device__ __forceinline__ int4 pull_tiles(const Pool& pool, uint32_t scene, uin
       if (mask) {
         const int c = __builtin_ctzll(mask);
         mask &= mask - 1;
-        const uint32_t oc = (uint32_t)__builtin_amdgcn_readlane((int)off, c);
-        const int hx = __builtin_amdgcn_readlane(h.x, c), hy = __builtin_amdgcn_readlane(h.y, c),
-                  hz = __builtin_amdgcn_readlane(h.z, c), hw = __builtin_amdgcn_readlane(h.w, c);
-        const int tx = hw & 255, ty = (hw >> 8) & 255, tz = (hw >> 16) & 255;
-        const int lx = n0 - hx, ly = n1 - hy, lz = n2 - hz;
-        if ((unsigned)lx < (unsigned)tx && (unsigned)ly < (unsigned)ty && (unsigned)lz < (unsigned)tz)
-          q[u] = pool.tiles[(size_t)oc + (uint32_t)((lz * ty + ly) * tx + lx)];
+        const uint32_t rc = (uint32_t)__builtin_amdgcn_readlane((int)run, c);
+        const uint32_t cc = (uint32_t)__builtin_amdgcn_readlane((int)corner, c);
+        const uint32_t wc = (uint32_t)__builtin_amdgcn_readlane((int)widths, c);
+        const int wx = (int)(wc & 15u), wy = (int)((wc >> 4) & 15u), wz = (int)(wc >> 8);
+        const int lx = n0 - (int)(cc & 1023u), ly = n1 - (int)((cc >> 10) & 1023u), lz = n2 - (int)(cc >> 20);
+        if ((unsigned)lx < (unsigned)wx && (unsigned)ly < (unsigned)wy && (unsigned)lz < (unsigned)wz)
+          q[u] = pool.tiles[(size_t)rc + (uint32_t)(lx + wx * (ly + wy * lz))];
       }
     }
 #pragma unroll

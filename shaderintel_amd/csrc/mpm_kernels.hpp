@@ -27,11 +27,12 @@
 //    its particles in an LDS tile (LDS integer atomics) sized to the bounding box of their stencils,
 //    takes that many nodes from one region of a bump-allocated arena and stores the box and the
 //    nodes there with plain stores (tile_hdr[item], tile_off[item], and by the block's slot in
-//    mtile, stamped with the P2G epoch). The grid update of a node block pulls from the tiles of
-//    particle blocks G-2..G+1 per axis (their mtile slots), so the tiled path needs no global
-//    atomic. A particle whose base drifted more than kDrift cells outside its block since
-//    the binning, or every particle of an item whose tile does not fit its arena region, falls back
-//    to global atomics on A and stamps the node blocks it wrote.
+//    mtile, stamped with the P2G epoch), node block by node block (tile_index, item_sched.hpp: the
+//    part of a box that one node block holds is one contiguous run). The grid update of a node
+//    block pulls from the tiles of particle blocks G-2..G+1 per axis (their mtile slots), so the
+//    tiled path needs no global atomic. A particle whose base drifted more than kDrift cells
+//    outside its block since the binning, or every particle of an item whose tile does not fit its
+//    arena region, falls back to global atomics on A and stamps the node blocks it wrote.
 //  - Active node blocks: S = node blocks within reach of any work item (rebuilt at each binning);
 //    D = node blocks written through A in the current P2G. The grid update visits S and D and
 //    returns A to zero where it was written.

@@ -10,6 +10,7 @@
 // before all of its waves had read it: waves 1-3 then work on the next item, so the box spans two
 // windows. It counts the boxes that break the window bound and the arena offsets the unguarded check
 // (o <= rsize - pnodes, without pnodes <= kTileNodes) would have given past the arena's end.
+// The tile storage order (tile_index, tile_chunk) is checked over boxes of every extent up to 12.
 // Prints one summary line; exit status 0 = the loop's bounds hold.
 #include <algorithm>
 #include <climits>
@@ -104,6 +105,41 @@ int main(int argc, char** argv) {
     }
     boxes[k] = bx;
   }
+
+  // the tile storage order: tile_index is a bijection of every box's nodes onto [0, nodes), and each
+  // node block's chunk (tile_chunk, as the grid update reads it) holds exactly that block's nodes, in
+  // one run at the chunk's offset
+  for (int ox = 0; ox < 8; ox++)
+    for (int ex = 1; ex <= 12; ex++)
+      for (int ey = 1; ey <= 12; ey += 5)
+        for (int ez = 1; ez <= 12; ez += 3) {
+          const int oy = (ox * 3 + 1) % 8, oz = (ox * 5 + 2) % 8, nn = ex * ey * ez;
+          std::vector<uint8_t> seen((size_t)nn, 0);
+          for (int lz = 0; lz < ez; lz++)
+            for (int ly = 0; ly < ey; ly++)
+              for (int lx = 0; lx < ex; lx++) {
+                const uint32_t t = tile_index(ox, oy, oz, ex, ey, ez, lx, ly, lz);
+                CHECK(t < (uint32_t)nn && !seen[t]);
+                seen[t] = 1;
+                const int gx = ox + lx, gy = oy + ly, gz = oz + lz;
+                uint32_t off;
+                int wx, wy, wz, l0, l1, l2;
+                CHECK(tile_chunk(ox, oy, oz, ex, ey, ez, gx >> 2, gy >> 2, gz >> 2, off, wx, wy, wz, l0, l1, l2));
+                CHECK(wx * wy * wz <= 64 && gx - l0 < wx && gy - l1 < wy && gz - l2 < wz);
+                CHECK(off + (uint32_t)((gx - l0) + wx * ((gy - l1) + wy * (gz - l2))) == t);
+              }
+          for (int bz = 0; bz < 8; bz++)
+            for (int by = 0; by < 8; by++)
+              for (int bx = 0; bx < 8; bx++) {
+                uint32_t off;
+                int wx, wy, wz, l0, l1, l2;
+                const bool any = tile_chunk(ox, oy, oz, ex, ey, ez, bx, by, bz, off, wx, wy, wz, l0, l1, l2);
+                const bool overlaps = 4 * bx < ox + ex && 4 * bx + 4 > ox && 4 * by < oy + ey && 4 * by + 4 > oy &&
+                                      4 * bz < oz + ez && 4 * bz + 4 > oz;
+                CHECK(any == overlaps);
+                if (any) CHECK(off + (uint32_t)(wx * wy * wz) <= (uint32_t)nn);
+              }
+        }
 
   std::mt19937_64 rng(12345);
   const uint32_t grids[] = {8, 64, 256, 2048, 2056, 4096};
