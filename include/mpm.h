@@ -197,6 +197,10 @@ mpm_status mpm_set_profiling_every(mpm_sim* sim, uint32_t every);
                               * MPM_TUNE_G2P_ITEMS); 0: separate launches */
 #define MPM_TUNE_FUSE_SHAPE 7 /* workgroup shape of the fused G2P + P2G: 0 = 192 threads at 3 waves per
                                * SIMD, 1 = 256 threads at 4 (default) */
+#define MPM_TUNE_DRIFT 8      /* P2G's tile shape: the cells a particle's stencil base may drift outside
+                               * its work item's 4^3-cell block before it takes the global-atomic path:
+                               * 2 (a 10^3-node LDS tile, default) or 3 (12^3; no particle slower than
+                               * 1 cell per substep leaves it between sorts) */
 mpm_status mpm_set_tuning(mpm_sim* sim, int32_t knob, int32_t value);
 
 /* ---- scene pools: many independent simulations advanced by the same launches ----
@@ -285,10 +289,10 @@ mpm_status mpm_pool_read_grid_law(mpm_pool* pool, mpm_grid_law* out, int32_t res
 
 /* Times the kernel shapes for trial_substeps substeps each (rounded up to whole sort periods) with HIP
  * events and keeps the fastest: first the fused G2P + P2G's workgroup shape (MPM_TUNE_FUSE_SHAPE, 2
- * trials), then the occupancy of the G2P without a fused P2G and of the binned P2G (12 trials).
- * *g2p_waves and *p2g_waves receive the chosen occupancies (mpm_pool_tuning reads every knob). The
- * trials are real steps of the pool with the vmax of its last step (every variant gives identical
- * results), so the pool advances by 14 trials. */
+ * trials), then P2G's tile shape (MPM_TUNE_DRIFT, 2 trials), then the occupancy of the G2P without a
+ * fused P2G and of the binned P2G (12 trials). *g2p_waves and *p2g_waves receive the chosen
+ * occupancies (mpm_pool_tuning reads every knob). The trials are real steps of the pool with the vmax
+ * of its last step (every variant gives identical results), so the pool advances by 16 trials. */
 mpm_status mpm_pool_autotune(mpm_pool* pool, uint32_t trial_substeps, int32_t* g2p_waves, int32_t* p2g_waves);
 /* The current value of a tuning knob. */
 mpm_status mpm_pool_tuning(mpm_pool* pool, int32_t knob, int32_t* value);

@@ -53,11 +53,12 @@ MPM_HD uint32_t compact3(uint32_t x) {
   return x;
 }
 
-// A stencil base (b0, b1, b2) is in the window of the work item whose lowest in-window base is
-// (wx, wy, wz) when it lies within kDrift cells of the item's 4^3-cell block on every axis.
+// A stencil base (b0, b1, b2) is in the drift-D window of the work item whose lowest in-window base
+// is (wx, wy, wz) when it lies within D cells of the item's 4^3-cell block on every axis.
+template <int D = kDrift>
 MPM_HD bool in_window(int b0, int b1, int b2, int wx, int wy, int wz) {
-  return (unsigned)(b0 - wx) < (unsigned)(4 + 2 * kDrift) && (unsigned)(b1 - wy) < (unsigned)(4 + 2 * kDrift) &&
-         (unsigned)(b2 - wz) < (unsigned)(4 + 2 * kDrift);
+  return (unsigned)(b0 - wx) < (unsigned)(4 + 2 * D) && (unsigned)(b1 - wy) < (unsigned)(4 + 2 * D) &&
+         (unsigned)(b2 - wz) < (unsigned)(4 + 2 * D);
 }
 
 // Tile storage order. A tile box (origin o, extents e per axis) is cut at node-block boundaries

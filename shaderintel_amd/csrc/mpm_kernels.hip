@@ -949,17 +949,17 @@ struct ItemCursor {
 // to it (an add past the end takes nothing, so a region's counter ends at most one tile per
 // workgroup past its size). Every thread of the workgroup calls it; s_item is read after the
 // caller's next barrier.
-template <int THREADS>
+template <int THREADS, int D>
 __device__ __forceinline__ void store_item_tile(const Pool& pool, uint32_t k, uint32_t wb, const SceneGeo& geo, int* tile,
                                                 const int* bbox, int wx, int wy, int wz, uint32_t epoch,
                                                 uint32_t bin_epoch, uint32_t& s_off, ItemCursor& cur,
                                                 uint32_t& s_item) {
-  constexpr int S = kTileSpan;
+  constexpr int S = Window<D>::kSpan, N = Window<D>::kNodes;
   const int tid = threadIdx.x;
   const bool any = bbox[0] != INT_MAX;
   const int px = bbox[0], py = bbox[1], pz = bbox[2];
   const int fx = any ? bbox[3] - px + 3 : 0, fy = any ? bbox[4] - py + 3 : 0, fz = any ? bbox[5] - pz + 3 : 0;
-  const int pnodes = fx * fy * fz;  // <= kTileNodes: inside the window tile
+  const int pnodes = fx * fy * fz;  // <= N: inside the window tile
   if (tid == 0) {  // arena space for the tile, from this workgroup's region
     const uint32_t region = blockIdx.x % kArenaRegions, rsize = pool.tile_nodes / kArenaRegions;
     const bool add = pnodes > 0 && !cur.full;
@@ -985,11 +985,11 @@ __device__ __forceinline__ void store_item_tile(const Pool& pool, uint32_t k, ui
     int lx, ly, lz;
     box_coords(t, fx, fy, dx, dxy, lx, ly, lz);
     const int w = ((pz + lz - wz) * S + (py + ly - wy)) * S + (px + lx - wx);
-    const int4 q = make_int4(tile[w], tile[kTileNodes + w], tile[2 * kTileNodes + w], tile[3 * kTileNodes + w]);
+    const int4 q = make_int4(tile[w], tile[N + w], tile[2 * N + w], tile[3 * N + w]);
     tile[w] = 0;
-    tile[kTileNodes + w] = 0;
-    tile[2 * kTileNodes + w] = 0;
-    tile[3 * kTileNodes + w] = 0;
+    tile[N + w] = 0;
+    tile[2 * N + w] = 0;
+    tile[3 * N + w] = 0;
     if (toff != 0xffffffffu) {
       pool.tiles[toff + tile_index(px, py, pz, fx, fy, fz, lx, ly, lz)] = q;
     } else if (q.x | q.y | q.z | q.w) {  // no arena room: add the node to A
@@ -1014,10 +1014,11 @@ __device__ __forceinline__ void store_item_tile(const Pool& pool, uint32_t k, ui
 // Inside a block the sort orders particles by (rank in cell, cell), so the 64 lanes of one
 // instruction read consecutive records (coalesced) that sit in different cells (their LDS atomics
 // hit different nodes).
-template <int WAVES>
+template <int WAVES, int D>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_p2g_binned(
     Pool pool, const float4* __restrict__ P, const uint32_t* __restrict__ perm) {
-  __shared__ int tile[4 * kTileNodes];
+  constexpr int S = Window<D>::kSpan, N = Window<D>::kNodes;  // window tile: S^3 nodes from the lowest in-window base
+  __shared__ int tile[4 * N];
   __shared__ int bbox[6];
   __shared__ uint32_t s_item, s_off;
   const int tid = threadIdx.x;
@@ -1025,8 +1026,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   const uint32_t epoch = pool.ctr[kCtrEpoch], bin_epoch = pool.ctr[kCtrBinEpoch];
   const uint32_t mshift = pool.key_bits - 6;
   ItemCursor cur{pool.arena_ctr + kArenaRegions * kArenaStride, items, blockIdx.x % kWorkSplits, 0, false};
-  constexpr int S = kTileSpan;  // window tile: S^3 nodes from the lowest in-window base
-  for (int t = tid; t < kTileNodes; t += kThreads)
+  for (int t = tid; t < N; t += kThreads)
     reinterpret_cast<int4*>(tile)[t] = make_int4(0, 0, 0, 0);  // the 4 planes; each item leaves them zero
   if (tid == 0) s_item = first_item(cur.items);
   __syncthreads();
@@ -1041,14 +1041,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
     if (li < cnt) load_p2g_record(P, pool.cap, perm ? perm[start + li] : start + li, r);
     const SceneGeo geo = uniform_geo(pool, wb >> mshift);
     const uint32_t mb = wb & ((1u << mshift) - 1u);
-    const int wx = 4 * (int)compact3(mb) - kDrift, wy = 4 * (int)compact3(mb >> 1) - kDrift,
-              wz = 4 * (int)compact3(mb >> 2) - kDrift;  // lowest in-window base
+    const int wx = 4 * (int)compact3(mb) - D, wy = 4 * (int)compact3(mb >> 1) - D,
+              wz = 4 * (int)compact3(mb >> 2) - D;  // lowest in-window base
     if (tid < 3) { bbox[tid] = INT_MAX; bbox[3 + tid] = INT_MIN; }
     __syncthreads();
     int lo0 = INT_MAX, lo1 = INT_MAX, lo2 = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hi2 = INT_MIN;
     for (; li < cnt; li += kThreads) {
       const int b0 = (int)floorf(r[0].x - 0.5f), b1 = (int)floorf(r[0].y - 0.5f), b2 = (int)floorf(r[0].z - 0.5f);
-      const bool inwin = in_window(b0, b1, b2, wx, wy, wz);
+      const bool inwin = in_window<D>(b0, b1, b2, wx, wy, wz);
       if (inwin) {
         lo0 = min(lo0, b0); lo1 = min(lo1, b1); lo2 = min(lo2, b2);
         hi0 = max(hi0, b0); hi1 = max(hi1, b1); hi2 = max(hi2, b2);
@@ -1056,9 +1056,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
         p2g_particle(r, pool.mats, geo.vol4, [&](int gx, int gy, int gz, int q0, int q1, int q2, int q3) {
           const int t = tb + ((gz - b2) * S + (gy - b1)) * S + (gx - b0);
           atomicAdd(&tile[t], q0);
-          atomicAdd(&tile[kTileNodes + t], q1);
-          atomicAdd(&tile[2 * kTileNodes + t], q2);
-          atomicAdd(&tile[3 * kTileNodes + t], q3);
+          atomicAdd(&tile[N + t], q1);
+          atomicAdd(&tile[2 * N + t], q2);
+          atomicAdd(&tile[3 * N + t], q3);
         });
       } else {
         fallback_particle(pool, r, epoch, true, bin_epoch);
@@ -1075,7 +1075,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
       atomicMax(&bbox[3], hi0); atomicMax(&bbox[4], hi1); atomicMax(&bbox[5], hi2);
     }
     __syncthreads();
-    store_item_tile<kThreads>(pool, k, wb, geo, tile, bbox, wx, wy, wz, epoch, bin_epoch, s_off, cur, s_item);
+    store_item_tile<kThreads, D>(pool, k, wb, geo, tile, bbox, wx, wy, wz, epoch, bin_epoch, s_off, cur, s_item);
     __syncthreads();  // the tile is read out and zero again
   }
 }
@@ -1099,12 +1099,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
 constexpr int kFuseBoxNodes = 768;
 constexpr int kBoxRow = 12, kBoxPlane = 96;
 static_assert(kBoxPlane * 8 == kFuseBoxNodes, "the fixed-stride box holds 8 planes");
-template <int SHAPE, int kFuseThreads = kFuseThreadsOf[SHAPE]>
+template <int SHAPE, int D, int kFuseThreads = kFuseThreadsOf[SHAPE]>
 __global__ __launch_bounds__(kFuseThreadsOf[SHAPE]) __attribute__((amdgpu_waves_per_eu(kFuseWavesOf[SHAPE], 8))) void k_g2p_p2g(
     Pool pool, const float4* src, float4* dst, const uint32_t* __restrict__ perm, int sorted_out, float vmax,
     uint32_t* __restrict__ keys_out) {
+  constexpr int S = Window<D>::kSpan, N = Window<D>::kNodes;  // window tile: S^3 nodes from the lowest in-window base
   __shared__ float4 vbox[kFuseBoxNodes];
-  __shared__ int tile[4 * kTileNodes];
+  __shared__ int tile[4 * N];
   __shared__ int bbox[6];
   __shared__ uint32_t s_item, s_off;
   const int tid = threadIdx.x;
@@ -1115,8 +1116,7 @@ __global__ __launch_bounds__(kFuseThreadsOf[SHAPE]) __attribute__((amdgpu_waves_
   // the next substep's G2P, which has no launch_begin_p2g of its own before it)
   ItemCursor cur{pool.arena_ctr + kArenaRegions * kArenaStride, items, blockIdx.x % kWorkSplits, 0, false};
   const float4* __restrict__ V = pool.V;
-  constexpr int S = kTileSpan;  // window tile: S^3 nodes from the lowest in-window base
-  for (int t = tid; t < kTileNodes; t += kFuseThreads)
+  for (int t = tid; t < N; t += kFuseThreads)
     reinterpret_cast<int4*>(tile)[t] = make_int4(0, 0, 0, 0);  // the 4 planes; each item leaves them zero
   if (tid == 0) s_item = first_item(cur.items);
   __syncthreads();
@@ -1137,8 +1137,8 @@ __global__ __launch_bounds__(kFuseThreadsOf[SHAPE]) __attribute__((amdgpu_waves_
     }
     const SceneGeo geo = uniform_geo(pool, wb >> mshift);
     const uint32_t mb = wb & ((1u << mshift) - 1u);
-    const int wx = 4 * (int)compact3(mb) - kDrift, wy = 4 * (int)compact3(mb >> 1) - kDrift,
-              wz = 4 * (int)compact3(mb >> 2) - kDrift;  // lowest in-window base
+    const int wx = 4 * (int)compact3(mb) - D, wy = 4 * (int)compact3(mb >> 1) - D,
+              wz = 4 * (int)compact3(mb >> 2) - D;  // lowest in-window base
     int ox = 0, oy = 0, oz = 0, ex = 0, ey = 0, ez = 0;
     if (has_box) {
       ox = h.x; oy = h.y; oz = h.z;
@@ -1187,7 +1187,7 @@ __global__ __launch_bounds__(kFuseThreadsOf[SHAPE]) __attribute__((amdgpu_waves_
       }
       // the next substep's P2G of the new record
       const int b0 = (int)floorf(r[0].x - 0.5f), b1 = (int)floorf(r[0].y - 0.5f), b2 = (int)floorf(r[0].z - 0.5f);
-      const bool inwin = in_window(b0, b1, b2, wx, wy, wz);
+      const bool inwin = in_window<D>(b0, b1, b2, wx, wy, wz);
       if (inwin) {
         lo0 = min(lo0, b0); lo1 = min(lo1, b1); lo2 = min(lo2, b2);
         hi0 = max(hi0, b0); hi1 = max(hi1, b1); hi2 = max(hi2, b2);
@@ -1195,9 +1195,9 @@ __global__ __launch_bounds__(kFuseThreadsOf[SHAPE]) __attribute__((amdgpu_waves_
         auto to_tile = [&](int gx, int gy, int gz, int q0, int q1, int q2, int q3) {
           const int t = tb + ((gz - b2) * S + (gy - b1)) * S + (gx - b0);
           atomicAdd(&tile[t], q0);
-          atomicAdd(&tile[kTileNodes + t], q1);
-          atomicAdd(&tile[2 * kTileNodes + t], q2);
-          atomicAdd(&tile[3 * kTileNodes + t], q3);
+          atomicAdd(&tile[N + t], q1);
+          atomicAdd(&tile[2 * N + t], q2);
+          atomicAdd(&tile[3 * N + t], q3);
         };
         // one wave-uniform choice: the clamp-free form when no active lane's particle can reach the clamp
         if (__builtin_amdgcn_ballot_w64(!p2g_unclamped(r, pool.mats, geo.vol4)) == 0)
@@ -1215,7 +1215,7 @@ __global__ __launch_bounds__(kFuseThreadsOf[SHAPE]) __attribute__((amdgpu_waves_
       atomicMax(&bbox[3], hi0); atomicMax(&bbox[4], hi1); atomicMax(&bbox[5], hi2);
     }
     __syncthreads();
-    store_item_tile<kFuseThreads>(pool, k, wb, geo, tile, bbox, wx, wy, wz, epoch, bin_epoch, s_off, cur, s_item);
+    store_item_tile<kFuseThreads, D>(pool, k, wb, geo, tile, bbox, wx, wy, wz, epoch, bin_epoch, s_off, cur, s_item);
     __syncthreads();  // the tile is read out and zero again
   }
 }
@@ -1517,13 +1517,19 @@ hipError_t launch_bin_dilate(const Pool& pool, uint32_t max_scenes, hipStream_t 
                      dim3(kThreads), 0, s, pool, max_scenes);
   return hipGetLastError();
 }
-hipError_t launch_p2g_binned(const Pool& pool, const float4* P, const uint32_t* perm, int waves, hipStream_t s) {
+template <int D>
+void launch_p2g_binned_d(const Pool& pool, const float4* P, const uint32_t* perm, int waves, hipStream_t s) {
   const dim3 grid((persistent_grid(pool.max_items, 1, 2048) + 7u) / 8u * 8u);  // a multiple of 8 (first_item)
   switch (waves) {
-    case 5: hipLaunchKernelGGL(k_p2g_binned<5>, grid, dim3(kThreads), 0, s, pool, P, perm); break;
-    case 6: hipLaunchKernelGGL(k_p2g_binned<6>, grid, dim3(kThreads), 0, s, pool, P, perm); break;
-    default: hipLaunchKernelGGL(k_p2g_binned<4>, grid, dim3(kThreads), 0, s, pool, P, perm); break;
+    case 5: hipLaunchKernelGGL((k_p2g_binned<5, D>), grid, dim3(kThreads), 0, s, pool, P, perm); break;
+    case 6: hipLaunchKernelGGL((k_p2g_binned<6, D>), grid, dim3(kThreads), 0, s, pool, P, perm); break;
+    default: hipLaunchKernelGGL((k_p2g_binned<4, D>), grid, dim3(kThreads), 0, s, pool, P, perm); break;
   }
+}
+hipError_t launch_p2g_binned(const Pool& pool, const float4* P, const uint32_t* perm, int waves, int drift,
+                             hipStream_t s) {
+  if (drift == 2) launch_p2g_binned_d<2>(pool, P, perm, waves, s);
+  else launch_p2g_binned_d<3>(pool, P, perm, waves, s);
   return hipGetLastError();
 }
 hipError_t launch_p2g_global(const Pool& pool, const float4* P, uint32_t n, hipStream_t s) {
@@ -1589,22 +1595,28 @@ hipError_t launch_g2p_items(const Pool& pool, const float4* src, float4* dst, co
   }
   return hipGetLastError();
 }
-hipError_t launch_g2p_p2g(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, bool sorted_out,
-                          float vmax, int shape, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop,
-                          uint32_t* keys_out) {
-  if (sorted_out ? (!perm || src == dst) : src != dst) return hipErrorInvalidValue;
+template <int D>
+void launch_g2p_p2g_d(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, int so, float vmax,
+                      int shape, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop, uint32_t* keys_out) {
   const dim3 grid((persistent_grid(pool.max_items, 1, 2048) + 7u) / 8u * 8u);  // a multiple of 8 (first_item)
-  const int so = sorted_out ? 1 : 0;
   switch (shape) {
     case 0:
-      MPM_G2P_LAUNCH_T(k_g2p_p2g<0>, grid, kFuseThreadsOf[0], s, ev_start, ev_stop, pool, src, dst, perm, so, vmax,
+      MPM_G2P_LAUNCH_T((k_g2p_p2g<0, D>), grid, kFuseThreadsOf[0], s, ev_start, ev_stop, pool, src, dst, perm, so, vmax,
                        keys_out);
       break;
     default:
-      MPM_G2P_LAUNCH_T(k_g2p_p2g<1>, grid, kFuseThreadsOf[1], s, ev_start, ev_stop, pool, src, dst, perm, so, vmax,
+      MPM_G2P_LAUNCH_T((k_g2p_p2g<1, D>), grid, kFuseThreadsOf[1], s, ev_start, ev_stop, pool, src, dst, perm, so, vmax,
                        keys_out);
       break;
   }
+}
+hipError_t launch_g2p_p2g(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, bool sorted_out,
+                          float vmax, int shape, int drift, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop,
+                          uint32_t* keys_out) {
+  if (sorted_out ? (!perm || src == dst) : src != dst) return hipErrorInvalidValue;
+  const int so = sorted_out ? 1 : 0;
+  if (drift == 2) launch_g2p_p2g_d<2>(pool, src, dst, perm, so, vmax, shape, s, ev_start, ev_stop, keys_out);
+  else launch_g2p_p2g_d<3>(pool, src, dst, perm, so, vmax, shape, s, ev_start, ev_stop, keys_out);
   return hipGetLastError();
 }
 hipError_t launch_g2p(const Pool& pool, float4* P, uint32_t n, float vmax, int waves, hipStream_t s, hipEvent_t ev_start,

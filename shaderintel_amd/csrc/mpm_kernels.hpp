@@ -44,9 +44,19 @@
 
 namespace mpm {
 
-constexpr int kDrift = 3;                                      // cells a base may drift outside its block
+constexpr int kDrift = 3;                                      // cells a base may drift outside its block, at most
 constexpr int kTileSpan = 4 + 2 + 2 * kDrift;                  // nodes per axis a work item can touch (12)
 constexpr int kTileNodes = kTileSpan * kTileSpan * kTileSpan;  // 1728 nodes = 27 KiB of int4
+// The window of a work item with drift D (2 or 3; MPM_TUNE_DRIFT): stencil bases within D cells of
+// its 4^3-cell block, an LDS tile of kSpan^3 nodes (10^3 = 16 KiB or 12^3 = 27 KiB of int4). A particle
+// that moves at most 1 cell per substep stays in the drift-3 window for the 3 substeps between
+// sorts; one that leaves its window takes P2G's global-atomic path (the same sums).
+template <int D>
+struct Window {
+  static_assert(D >= 2 && D <= kDrift, "drift 2 or 3");
+  static constexpr int kSpan = 4 + 2 + 2 * D;
+  static constexpr int kNodes = kSpan * kSpan * kSpan;
+};
 // Float f of the reference's 32-float record (x, v, c0, c1, c2, f0, f1, f2 as vec4f, mpm.js:54-63)
 // is component kRecSlot[f] & 3 of stream kRecSlot[f] >> 2.
 constexpr uint8_t kRecSlot[32] = {
@@ -191,7 +201,9 @@ hipError_t launch_bin_dilate(const Pool& pool, uint32_t max_scenes, hipStream_t 
 
 // P2G over the work items into tiles (+ fallback atomics on A). perm: sorted position -> particle
 // (nullptr: identity).
-hipError_t launch_p2g_binned(const Pool& pool, const float4* P, const uint32_t* perm, int waves, hipStream_t s);
+// drift: the work items' window (Window<drift>, 2 or 3).
+hipError_t launch_p2g_binned(const Pool& pool, const float4* P, const uint32_t* perm, int waves, int drift,
+                             hipStream_t s);
 // P2G with one global atomic per contribution (the reference's forceGlobal path), any order.
 hipError_t launch_p2g_global(const Pool& pool, const float4* P, uint32_t n, hipStream_t s);
 // Writes n (<= 64) 32-bit words to dst from a kernel argument (ordered with the stream's work).
@@ -216,9 +228,10 @@ hipError_t launch_g2p_items(const Pool& pool, const float4* src, float4* dst, co
 // records are scattered into tiles (+ fallback atomics on A) without being read back. Needs
 // launch_begin_p2g after the grid update; leaves the grid state of a binned P2G. keys_out: as
 // launch_g2p_items (the sort keys of the new records).
-// shape: 0 = 192-thread workgroups at 3 waves per SIMD, 1 = 256 threads at 4 (result-neutral).
+// shape: 0 = 192-thread workgroups at 3 waves per SIMD, 1 = 256 threads at 4; drift: the window
+// (Window<drift>, 2 or 3). Both result-neutral.
 hipError_t launch_g2p_p2g(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, bool sorted_out,
-                          float vmax, int shape, hipStream_t s, hipEvent_t ev_start = nullptr,
+                          float vmax, int shape, int drift, hipStream_t s, hipEvent_t ev_start = nullptr,
                           hipEvent_t ev_stop = nullptr, uint32_t* keys_out = nullptr);
 // G2P that also applies a pending reorder: slot i of dst takes particle perm[i] of src.
 hipError_t launch_g2p_perm(const Pool& pool, const float4* src, float4* dst, const uint32_t* perm, uint32_t n, float vmax,

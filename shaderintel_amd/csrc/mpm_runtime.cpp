@@ -184,6 +184,7 @@ struct mpm_pool {
   bool a_dirty_s = false, a_dirty_d = false;  // A holds data over S / over D
   int g2p_waves = 3, p2g_waves = 4;
   int fuse_shape = 1;        // fused G2P + P2G workgroup shape (launch_g2p_p2g)
+  int drift = 2;             // the work items' window (Window<drift>) in the binned and fused P2G
   float last_vmax = 0.9f;    // vmax of the last step (the autotuner's trials step with it)
   bool g2p_items = true;     // G2P by work item (LDS velocity boxes) after a binned P2G
   bool tiles_fresh = false;  // the tiles and work items are those of this substep's binned P2G
@@ -524,7 +525,7 @@ mpm_status do_p2g(mpm_pool* p, bool force_global) {
     p->grid_state = kGridA;
     p->tiles_fresh = false;
   } else {
-    HIP_TRY(launch_p2g_binned(pv, p->P[p->cur], p->perm, p->p2g_waves, p->stream));
+    HIP_TRY(launch_p2g_binned(pv, p->P[p->cur], p->perm, p->p2g_waves, p->drift, p->stream));
     p->grid_state = kGridTiles;
     p->tiles_fresh = true;
   }
@@ -580,7 +581,7 @@ mpm_status do_g2p(mpm_pool* p, float vmax, bool fuse_next, bool keys_next, bool 
     HIP_TRY(launch_begin_p2g(view(p), p->stream));  // the next P2G's epoch, arena and work counters
     const bool sorted_out = p->reorder_in_g2p;
     HIP_TRY(launch_g2p_p2g(view(p), p->P[p->cur], p->P[p->cur ^ (sorted_out ? 1 : 0)], p->perm, sorted_out, vmax,
-                           p->fuse_shape, p->stream, e0, e1, presort ? p->keys[0] : nullptr));
+                           p->fuse_shape, p->drift, p->stream, e0, e1, presort ? p->keys[0] : nullptr));
     p->keys_ahead = presort;
     p->p2g_presort = presort;
     if (sorted_out) {
@@ -1738,9 +1739,9 @@ mpm_status mpm_pool_autotune(mpm_pool* p, uint32_t trial_substeps, int32_t* g2p_
   };
   // align to a sort boundary so every trial covers the same phases
   while (st == MPM_OK && p->step_count % p->sort_every != 0) st = pool_step(p, 1, vmax);
-  // the fused G2P + P2G (3 substeps of 4) first: its workgroup shape; then the occupancy of the G2P
-  // without a fused P2G (a call's last substep) and of the binned P2G (a call's first), each tried
-  // with the others at their best so far
+  // the fused G2P + P2G (3 substeps of 4) first: its workgroup shape; then the work items' window
+  // (P2G's tile: drift 2 or 3); then the occupancy of the G2P without a fused P2G (a call's last
+  // substep) and of the binned P2G (a call's first), each tried with the others at their best so far
   float best = 1e30f;
   int bf = p->fuse_shape;
   for (int f = 0; f < 2 && st == MPM_OK; f++) {
@@ -1750,6 +1751,15 @@ mpm_status mpm_pool_autotune(mpm_pool* p, uint32_t trial_substeps, int32_t* g2p_
     if (st == MPM_OK && ms < best) { best = ms; bf = f; }
   }
   p->fuse_shape = bf;
+  best = 1e30f;
+  int bd = p->drift;
+  for (int d = 2; d <= 3 && st == MPM_OK; d++) {
+    p->drift = d;
+    float ms = 0;
+    st = time_trial(&ms);
+    if (st == MPM_OK && ms < best) { best = ms; bd = d; }
+  }
+  p->drift = bd;
   best = 1e30f;
   int bg = p->g2p_waves, bp = p->p2g_waves;
   for (int g = 3; g <= 6 && st == MPM_OK; g++)
@@ -1780,6 +1790,7 @@ mpm_status mpm_pool_tuning(mpm_pool* p, int32_t knob, int32_t* value) {
     case MPM_TUNE_G2P_ITEMS: *value = p->g2p_items ? 1 : 0; return MPM_OK;
     case MPM_TUNE_FUSE: *value = p->fuse ? 1 : 0; return MPM_OK;
     case MPM_TUNE_FUSE_SHAPE: *value = p->fuse_shape; return MPM_OK;
+    case MPM_TUNE_DRIFT: *value = p->drift; return MPM_OK;
     default: return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_tuning: unknown knob %d", knob);
   }
 }
@@ -1880,6 +1891,11 @@ mpm_status mpm_pool_set_tuning(mpm_pool* p, int32_t knob, int32_t value) {
     case MPM_TUNE_FUSE_SHAPE:
       if (value < 0 || value > 1) break;
       p->fuse_shape = value;
+      drop_graphs(p);
+      return MPM_OK;
+    case MPM_TUNE_DRIFT:
+      if (value < 2 || value > 3) break;
+      p->drift = value;
       drop_graphs(p);
       return MPM_OK;
     default:

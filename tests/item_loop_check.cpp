@@ -5,7 +5,8 @@
 // For several grid sizes the workgroups take their items (first_item_of, then next_item_of on shared
 // counters) in a random interleaving; every item must be taken exactly once. Each item's P2G is then
 // followed through its window: every in-window stencil's LDS tile index, the tile box (pnodes <=
-// kTileNodes, 8-bit extents), the tile store's indices and the arena offsets.
+// kTileNodes, 8-bit extents), the tile store's indices and the arena offsets; and the same LDS
+// bounds in the narrower drift-2 window (a 10^3-node tile).
 // A second pass replays the loop as if a workgroup's item slot were overwritten with its next item
 // before all of its waves had read it: waves 1-3 then work on the next item, so the box spans two
 // windows. It counts the boxes that break the window bound and the arena offsets the unguarded check
@@ -104,6 +105,31 @@ int main(int argc, char** argv) {
       for (int a = 0; a < 3; a++) CHECK(bx.extent(a) <= 255 && bx.lo[a] - w[a] >= 0 && bx.hi[a] + 2 - w[a] < S);
     }
     boxes[k] = bx;
+  }
+  // the narrower window (drift 2, a 10^3-node tile): the same bounds for its in-window particles
+  uint64_t fallbacks2 = 0;
+  {
+    constexpr int D = 2, S2 = Window<D>::kSpan, N2 = Window<D>::kNodes;
+    for (uint32_t k = 0; k < items; k++) {
+      const uint32_t mb = block[k];
+      const int w[3] = {4 * (int)compact3(mb) - D, 4 * (int)compact3(mb >> 1) - D, 4 * (int)compact3(mb >> 2) - D};
+      Box bx;
+      for (uint32_t j = start[k]; j < start[k + 1]; j++) {
+        int b[3];
+        base(j, b);
+        if (!in_window<D>(b[0], b[1], b[2], w[0], w[1], w[2])) {
+          fallbacks2++;
+          continue;
+        }
+        bx.add(b);
+        const long tb = ((long)(b[2] - w[2]) * S2 + (b[1] - w[1])) * S2 + (b[0] - w[0]);
+        CHECK(tb >= 0 && tb + (2L * S2 + 2) * S2 + 2 < N2);
+      }
+      if (bx.any()) {
+        CHECK(bx.nodes() <= N2);
+        for (int a = 0; a < 3; a++) CHECK(bx.lo[a] - w[a] >= 0 && bx.hi[a] + 2 - w[a] < S2);
+      }
+    }
   }
 
   // the tile storage order: tile_index is a bijection of every box's nodes onto [0, nodes), and each
@@ -225,10 +251,11 @@ int main(int argc, char** argv) {
     for (uint32_t k = 0; k < items; k++) CHECK(taken[k] == 1);
   }
   std::printf(
-      "ok items=%u particles=%u in_window=%llu fallback=%llu arena_nodes=%llu | overwritten-slot replay: %llu item "
+      "ok items=%u particles=%u in_window=%llu fallback=%llu (drift 2: %llu) arena_nodes=%llu | overwritten-slot replay: %llu item "
       "pairs, %llu boxes over %d nodes, %llu with an extent over 255, %llu arena ends past the arena (worst by %ld "
       "nodes)\n",
-      items, n, (unsigned long long)inwin_total, (unsigned long long)fallbacks, (unsigned long long)arena,
+      items, n, (unsigned long long)inwin_total, (unsigned long long)fallbacks, (unsigned long long)fallbacks2,
+      (unsigned long long)arena,
       (unsigned long long)mixed_pairs, (unsigned long long)mixed_over, kTileNodes, (unsigned long long)mixed_ext,
       (unsigned long long)mixed_oob, worst_end);
   return 0;

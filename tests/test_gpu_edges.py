@@ -118,16 +118,18 @@ def test_empty_simulation(ctx):
     sim.destroy()
 
 
-@pytest.mark.parametrize("items,fuse,shape", [(1, 1, 1), (1, 1, 0), (1, 0, 1), (0, 0, 1), (0, 1, 1)])
-def test_kernel_variants_give_the_same_bits(ctx, items, fuse, shape):
+@pytest.mark.parametrize("items,fuse,shape,drift", [(1, 1, 1, 2), (1, 1, 1, 3), (1, 1, 0, 2), (1, 1, 0, 3),
+                                                    (1, 0, 1, 3), (0, 0, 1, 2), (0, 1, 1, 2)])
+def test_kernel_variants_give_the_same_bits(ctx, items, fuse, shape, drift):
     """G2P by work item with LDS velocity boxes (g2p_items) and G2P fused with the next substep's P2G
-    (fuse), on and off, and both workgroup shapes of the fused kernel (fuse_shape: 256 threads at 4
-    waves per SIMD, 192 at 3): 14 substeps in one call (3 sorts, 9 fused transitions when both are
-    on) of four materials, then 2 more calls of 3 substeps (fusion never crosses a call),
-    bit-identical to the oracle in every combination (fuse needs g2p_items; (0, 1) runs unfused)."""
-    print(f"[gpu] kernel variants g2p_items={items} fuse={fuse} fuse_shape={shape}", flush=True)
+    (fuse), on and off, both workgroup shapes of the fused kernel (fuse_shape: 256 threads at 4
+    waves per SIMD, 192 at 3) and both P2G tile shapes (drift 2: a 10^3-node window, 3: 12^3): 14
+    substeps in one call (3 sorts, 9 fused transitions when both are on) of four materials, then 2
+    more calls of 3 substeps (fusion never crosses a call), bit-identical to the oracle in every
+    combination (fuse needs g2p_items; (0, 1) runs unfused)."""
+    print(f"[gpu] kernel variants g2p_items={items} fuse={fuse} fuse_shape={shape} drift={drift}", flush=True)
     sim = MPM(ctx, grid=40, maxParticles=120000, substeps=8)
-    sim.set_tuning(g2p_items=items, fuse=fuse, fuse_shape=shape)
+    sim.set_tuning(g2p_items=items, fuse=fuse, fuse_shape=shape, drift=drift)
     sim.add({"box": [[0.1, 0.1, 0.1], [0.45, 0.45, 0.45]]}, material="water", velocity=[0.8, -0.5, 0.3], jitter=1.0)
     sim.add({"box": [[0.55, 0.1, 0.1], [0.9, 0.45, 0.45]]}, material="jelly", velocity=[-0.6, 0.2, 0.4], jitter=1.0)
     sim.add({"box": [[0.1, 0.1, 0.55], [0.45, 0.45, 0.9]]}, material="snow", velocity=[0.3, -1.0, -0.2], jitter=1.0)

@@ -248,11 +248,14 @@ def test_errors(ctx):
         sim.step()
 
 
-def test_fast_particles_use_fallback_and_stay_exact(ctx):
+@pytest.mark.parametrize("drift", [2, 3])
+def test_fast_particles_use_fallback_and_stay_exact(ctx, drift):
     """Particles fast enough to leave their block's window between sorts (sortEvery 8, |v| near the
-    CFL clamp) take the global-atomic fallback; the result stays identical to the oracle."""
-    _say("[gpu] fallback path: fast particles, sortEvery 8")
+    CFL clamp) take the global-atomic fallback; the result stays identical to the oracle with either
+    window (drift 2: a 10^3-node tile, 3: 12^3)."""
+    _say(f"[gpu] fallback path: fast particles, sortEvery 8, drift {drift}")
     sim = MPM(ctx, grid=48, maxParticles=60000, substeps=8, sortEvery=8, gravity=[0, 0, 0])
+    sim.set_tuning(drift=drift)
     sim.add({"box": [[0.2, 0.2, 0.2], [0.45, 0.45, 0.45]]}, material="water", velocity=[20.0, 15.0, -10.0], jitter=1.0)
     sim.add({"box": [[0.55, 0.5, 0.5], [0.8, 0.75, 0.75]]}, material="jelly", velocity=[-25.0, -5.0, 12.0], jitter=1.0)
     rec = sim.read_particles()

@@ -6,9 +6,10 @@ The positions are the scene's binned ones moved by up to 2.7 cells per axis (thr
 0.9-cell speed limit: the farthest a particle gets before the next sort), and 1% of them by up to 40
 cells (they leave their item's window and take the global-atomic path). Checked: every item is taken
 exactly once at six grid sizes in random interleavings; every in-window stencil's LDS tile index,
-every tile box (<= 12^3 nodes, 8-bit extents) and every arena offset stays in bounds. The tile
-storage order (node block by node block) is a bijection of every box's nodes, and each node block's
-chunk, as the grid update reads it, holds exactly that block's nodes.
+every tile box (<= 12^3 nodes, 8-bit extents) and every arena offset stays in bounds, and the LDS
+indices and boxes of the narrower drift-2 window (10^3 nodes) too. The tile storage order (node
+block by node block) is a bijection of every box's nodes, and each node block's chunk, as the grid
+update reads it, holds exactly that block's nodes.
 
 The replay of an item slot overwritten before all of a workgroup's waves have read it (a race a
 kernel that fetches its next item early can have) shows what such a race does on this item list:
