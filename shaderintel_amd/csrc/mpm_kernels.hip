@@ -17,9 +17,9 @@ namespace mpm {
 namespace {
 
 constexpr int kThreads = 256;
-// Shapes of the fused G2P + P2G (40 KB of LDS per workgroup, so 4 workgroups per CU): 256 threads
-// at the 4-waves-per-SIMD register budget (128 VGPRs; every SIMD holds 4 waves), or 192 threads at
-// 3 waves per SIMD (room for 168 VGPRs).
+// Shapes of the fused G2P + P2G (28 KB of LDS per workgroup with the drift-2 window, 39 KB with
+// drift 3): 256 threads at the 4-waves-per-SIMD register budget (128 VGPRs; every SIMD holds 4
+// waves, 4 workgroups per CU), or 192 threads at 3 waves per SIMD (room for 168 VGPRs).
 constexpr int kFuseShapes = 2;
 constexpr int kFuseThreadsOf[kFuseShapes] = {192, 256};
 constexpr int kFuseWavesOf[kFuseShapes] = {3, 4};
