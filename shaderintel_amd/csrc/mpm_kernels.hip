@@ -237,10 +237,23 @@ struct G2PIn {
   int bx, by, bz;
 };
 
+// Particle records stream through G2P: each is read once and written once per substep, so their
+// loads and stores are non-temporal (they do not displace the velocity boxes and tiles in the caches).
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 load_stream(const float4* p) {
+  const nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void store_stream(float4* p, float4 v) {
+  nt_f4 w;
+  w.x = v.x; w.y = v.y; w.z = v.z; w.w = v.w;
+  __builtin_nontemporal_store(w, reinterpret_cast<nt_f4*>(p));
+}
+
 // g2p_load = g2p_fetch (the two loads) + g2p_base (what follows from them).
 __device__ __forceinline__ void g2p_fetch(const float4* src, uint32_t si, uint32_t cap, G2PIn& in) {
-  in.x4 = src[si];
-  in.s5 = src[(size_t)5 * cap + si];
+  in.x4 = load_stream(src + si);
+  in.s5 = load_stream(src + (size_t)5 * cap + si);
 }
 __device__ __forceinline__ void g2p_base(G2PIn& in) {
   in.xp = mk(in.x4.x, in.x4.y, in.x4.z);
@@ -317,7 +330,7 @@ __device__ __forceinline__ void g2p_finish(const float* __restrict__ mats, const
     tau = (m0[2] * fminf(J - 1.0f, 0.02f)) * ident() + (m0[4] * J) * (C + tr(C));
     F = ident();
   } else {
-    const float4 f6 = src[(size_t)6 * cap + si], f7 = src[(size_t)7 * cap + si];
+    const float4 f6 = load_stream(src + (size_t)6 * cap + si), f7 = load_stream(src + (size_t)7 * cap + si);
     F = mul(ident() + C, mat(mk(f6.x, f6.y, f6.z), mk(f6.w, f7.x, f7.y), mk(f7.z, f7.w, s5.w)));
     const float mu0 = m0[2], la0 = m0[3];
     // snow and sand share one SVD of F (one copy of its code; a wavefront holding both runs it once)
@@ -366,7 +379,7 @@ __device__ __forceinline__ void g2p_finish(const float* __restrict__ mats, const
   r[6] = make_float4(F.c[0].x, F.c[0].y, F.c[0].z, F.c[1].x);
   r[7] = make_float4(F.c[1].y, F.c[1].z, F.c[2].x, F.c[2].y);
 #pragma unroll
-  for (int k = 0; k < 8; k++) dst[(size_t)k * cap + di] = r[k];
+  for (int k = 0; k < 8; k++) store_stream(dst + (size_t)k * cap + di, r[k]);
 }
 
 template <class Gather>
