@@ -751,6 +751,17 @@ mpm_status pool_step(mpm_pool* p, uint32_t substeps, float vmax) {
   p->last_vmax = vmax;
   if (p->needs_repack) ST_TRY(repack(p));
   if (p->count == 0) return MPM_OK;  // mpm.js:584
+  // A call that starts with a sort while the previous sort's work items still cover the records in
+  // their order does that substep's P2G over those items first (as a G2P before a sort does within a
+  // call), with the sort keys beside it; the substep then runs its grid update before the sort. The
+  // P2G reads the records in order instead of through the new permutation; its sums do not depend
+  // on the order or on the items.
+  if (p->fuse && p->g2p_items && !p->hook && p->profiling != 1 && p->bin_valid && !p->perm && !p->reorder_in_g2p &&
+      p->step_count % p->sort_every == 0) {
+    ST_TRY(do_p2g(p, false));
+    HIP_TRY(launch_keys(view(p), p->P[p->cur], p->count, p->keys[0], p->stream));
+    p->p2g_ahead = p->keys_ahead = p->p2g_presort = true;
+  }
   uint32_t k = 0;
   while (k < substeps) {
     const bool at_sort = p->step_count % p->sort_every == 0;

@@ -286,3 +286,39 @@ def test_p2g_after_write_without_sort(ctx):
     sc.p2g()
     assert np.array_equal(sim.read_grid(), sc.grid())
     sim.destroy()
+
+
+@pytest.mark.parametrize("sort_every", [4, 3])
+def test_split_calls_match_one_run(ctx, sort_every):
+    """Substeps split over calls of 1-5 (calls that start on a sort, mid-period and right after one)
+    give the oracle's bits of one uninterrupted run; so does a call that starts on a sort after the
+    particles were rewritten in place (their bins are then stale: P2G's fallback takes the movers)."""
+    _say(f"[gpu] split calls vs one oracle run (sort every {sort_every})")
+    sim = MPM(ctx, grid=32, maxParticles=40000, substeps=8, sortEvery=sort_every)
+    sim.add({"box": [[0.15, 0.15, 0.15], [0.45, 0.5, 0.45]]}, material="snow", velocity=[0.3, -0.2, 0.1], noise=0.2)
+    sim.add({"box": [[0.55, 0.15, 0.55], [0.85, 0.45, 0.85]]}, material=scenes.SAND30, velocity=[-0.2, 0.1, 0.0])
+    raw = sim.read_particles()
+    sc = _mirror(sim, raw, sort_every)
+    g = sim.lattice_gravity()
+    from shaderintel_amd.mpm import load_library, _ptr
+    L = load_library()
+    n = len(raw)
+    for calls in ([1, 3, 4, 5, 2, 1], [4, 4, 8]):
+        for k in calls:
+            assert L.mpm_step(sim._h, k, _ptr(g), 0.4, 0.9) == 0
+        sc.step(sum(calls), g)
+        got, want = sim.read_particles(), sc.read(n)
+        assert np.array_equal(bits(got), bits(want)), f"calls {calls}: {_first_diff(got, want)}"
+    # rewrite in place (same order): shift every particle by up to 3 cells, then step from a sort
+    moved = got.copy()
+    moved[:, 0:3] += (np.random.default_rng(5).random((n, 3)) - 0.5) * 6.0
+    moved[:, 0:3] = np.clip(moved[:, 0:3], 3.0, 28.0)
+    sim.write_particles(moved)
+    sc.write(moved)
+    assert L.mpm_step(sim._h, 2 * sort_every, _ptr(g), 0.4, 0.9) == 0
+    sc.step(2 * sort_every, g)
+    got, want = sim.read_particles(), sc.read(n)
+    assert np.array_equal(bits(got), bits(want)), f"after rewrite: {_first_diff(got, want)}"
+    gg, gw = sim.read_grid(), sc.grid()
+    assert np.array_equal(gg, gw), _first_diff(gg, gw)
+    sim.destroy()
