@@ -1029,7 +1029,7 @@ __device__ __forceinline__ void store_item_tile(const Pool& pool, uint32_t k, ui
 // hit different nodes).
 template <int WAVES, int D>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_p2g_binned(
-    Pool pool, const float4* __restrict__ P, const uint32_t* __restrict__ perm) {
+    Pool pool, const float4* __restrict__ P, const uint32_t* __restrict__ perm, uint32_t* __restrict__ keys_out) {
   constexpr int S = Window<D>::kSpan, N = Window<D>::kNodes;  // window tile: S^3 nodes from the lowest in-window base
   __shared__ int tile[4 * N];
   __shared__ int bbox[6];
@@ -1049,9 +1049,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
     const uint32_t start = pool.wl_start[k], cnt = pool.wl_start[k + 1] - start;
     const uint32_t wb = pool.wl_block[k];
     // this thread's first record
-    uint32_t li = tid;
+    uint32_t li = tid, si = 0;
     float4 r[kP2GStreams] = {};
-    if (li < cnt) load_p2g_record(P, pool.cap, perm ? perm[start + li] : start + li, r);
+    if (li < cnt) {
+      si = perm ? perm[start + li] : start + li;
+      load_p2g_record(P, pool.cap, si, r);
+    }
     const SceneGeo geo = uniform_geo(pool, wb >> mshift);
     const uint32_t mb = wb & ((1u << mshift) - 1u);
     const int wx = 4 * (int)compact3(mb) - D, wy = 4 * (int)compact3(mb >> 1) - D,
@@ -1060,6 +1063,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
     __syncthreads();
     int lo0 = INT_MAX, lo1 = INT_MAX, lo2 = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hi2 = INT_MIN;
     for (; li < cnt; li += kThreads) {
+      if (keys_out) keys_out[si] = sort_key(pool, wb >> mshift, geo.active, r[0]);
       const int b0 = (int)floorf(r[0].x - 0.5f), b1 = (int)floorf(r[0].y - 0.5f), b2 = (int)floorf(r[0].z - 0.5f);
       const bool inwin = in_window<D>(b0, b1, b2, wx, wy, wz);
       if (inwin) {
@@ -1078,7 +1082,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
       }
       if (li + kThreads < cnt) {  // the next record of this thread
         const uint32_t jn = start + li + kThreads;
-        load_p2g_record(P, pool.cap, perm ? perm[jn] : jn, r);
+        si = perm ? perm[jn] : jn;
+        load_p2g_record(P, pool.cap, si, r);
       }
     }
     lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
@@ -1531,18 +1536,19 @@ hipError_t launch_bin_dilate(const Pool& pool, uint32_t max_scenes, hipStream_t 
   return hipGetLastError();
 }
 template <int D>
-void launch_p2g_binned_d(const Pool& pool, const float4* P, const uint32_t* perm, int waves, hipStream_t s) {
+void launch_p2g_binned_d(const Pool& pool, const float4* P, const uint32_t* perm, int waves, hipStream_t s,
+                         uint32_t* keys_out) {
   const dim3 grid((persistent_grid(pool.max_items, 1, 2048) + 7u) / 8u * 8u);  // a multiple of 8 (first_item)
   switch (waves) {
-    case 5: hipLaunchKernelGGL((k_p2g_binned<5, D>), grid, dim3(kThreads), 0, s, pool, P, perm); break;
-    case 6: hipLaunchKernelGGL((k_p2g_binned<6, D>), grid, dim3(kThreads), 0, s, pool, P, perm); break;
-    default: hipLaunchKernelGGL((k_p2g_binned<4, D>), grid, dim3(kThreads), 0, s, pool, P, perm); break;
+    case 5: hipLaunchKernelGGL((k_p2g_binned<5, D>), grid, dim3(kThreads), 0, s, pool, P, perm, keys_out); break;
+    case 6: hipLaunchKernelGGL((k_p2g_binned<6, D>), grid, dim3(kThreads), 0, s, pool, P, perm, keys_out); break;
+    default: hipLaunchKernelGGL((k_p2g_binned<4, D>), grid, dim3(kThreads), 0, s, pool, P, perm, keys_out); break;
   }
 }
 hipError_t launch_p2g_binned(const Pool& pool, const float4* P, const uint32_t* perm, int waves, int drift,
-                             hipStream_t s) {
-  if (drift == 2) launch_p2g_binned_d<2>(pool, P, perm, waves, s);
-  else launch_p2g_binned_d<3>(pool, P, perm, waves, s);
+                             hipStream_t s, uint32_t* keys_out) {
+  if (drift == 2) launch_p2g_binned_d<2>(pool, P, perm, waves, s, keys_out);
+  else launch_p2g_binned_d<3>(pool, P, perm, waves, s, keys_out);
   return hipGetLastError();
 }
 hipError_t launch_p2g_global(const Pool& pool, const float4* P, uint32_t n, hipStream_t s) {

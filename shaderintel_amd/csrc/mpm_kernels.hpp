@@ -201,9 +201,10 @@ hipError_t launch_bin_dilate(const Pool& pool, uint32_t max_scenes, hipStream_t 
 
 // P2G over the work items into tiles (+ fallback atomics on A). perm: sorted position -> particle
 // (nullptr: identity).
-// drift: the work items' window (Window<drift>, 2 or 3).
+// drift: the work items' window (Window<drift>, 2 or 3). keys_out (optional): the sort key of each
+// record at its slot (as launch_keys computes them).
 hipError_t launch_p2g_binned(const Pool& pool, const float4* P, const uint32_t* perm, int waves, int drift,
-                             hipStream_t s);
+                             hipStream_t s, uint32_t* keys_out = nullptr);
 // P2G with one global atomic per contribution (the reference's forceGlobal path), any order.
 hipError_t launch_p2g_global(const Pool& pool, const float4* P, uint32_t n, hipStream_t s);
 // Writes n (<= 64) 32-bit words to dst from a kernel argument (ordered with the stream's work).

@@ -509,7 +509,8 @@ mpm_status repack(mpm_pool* p) {
 
 // P2G (mpm.js:607-612). Binned: into tiles (+ A for stragglers), grid state kGridTiles. Global: every
 // contribution a global atomic on A, grid state kGridA.
-mpm_status do_p2g(mpm_pool* p, bool force_global) {
+// keys_out (binned only): the sort keys of the records, written by the P2G.
+mpm_status do_p2g(mpm_pool* p, bool force_global, uint32_t* keys_out = nullptr) {
   if (p->p2g_ahead && !force_global) {  // done by the fused G2P of the previous substep
     p->p2g_ahead = false;
     return MPM_OK;
@@ -525,7 +526,7 @@ mpm_status do_p2g(mpm_pool* p, bool force_global) {
     p->grid_state = kGridA;
     p->tiles_fresh = false;
   } else {
-    HIP_TRY(launch_p2g_binned(pv, p->P[p->cur], p->perm, p->p2g_waves, p->drift, p->stream));
+    HIP_TRY(launch_p2g_binned(pv, p->P[p->cur], p->perm, p->p2g_waves, p->drift, p->stream, keys_out));
     p->grid_state = kGridTiles;
     p->tiles_fresh = true;
   }
@@ -753,13 +754,12 @@ mpm_status pool_step(mpm_pool* p, uint32_t substeps, float vmax) {
   if (p->count == 0) return MPM_OK;  // mpm.js:584
   // A call that starts with a sort while the previous sort's work items still cover the records in
   // their order does that substep's P2G over those items first (as a G2P before a sort does within a
-  // call), with the sort keys beside it; the substep then runs its grid update before the sort. The
+  // call), which also writes the sort keys; the substep then runs its grid update before the sort. The
   // P2G reads the records in order instead of through the new permutation; its sums do not depend
   // on the order or on the items.
   if (p->fuse && p->g2p_items && !p->hook && p->profiling != 1 && p->bin_valid && !p->perm && !p->reorder_in_g2p &&
       p->step_count % p->sort_every == 0) {
-    ST_TRY(do_p2g(p, false));
-    HIP_TRY(launch_keys(view(p), p->P[p->cur], p->count, p->keys[0], p->stream));
+    ST_TRY(do_p2g(p, false, p->keys[0]));
     p->p2g_ahead = p->keys_ahead = p->p2g_presort = true;
   }
   uint32_t k = 0;
