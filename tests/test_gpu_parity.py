@@ -290,9 +290,10 @@ def test_p2g_after_write_without_sort(ctx):
 
 @pytest.mark.parametrize("sort_every", [4, 3])
 def test_split_calls_match_one_run(ctx, sort_every):
-    """Substeps split over calls of 1-5 (calls that start on a sort, mid-period and right after one)
-    give the oracle's bits of one uninterrupted run; so does a call that starts on a sort after the
-    particles were rewritten in place (their bins are then stale: P2G's fallback takes the movers)."""
+    """Substeps split over calls of 1-5 (calls that start on a sort, which open with the sort's P2G
+    over the previous work items, mid-period and right after one) give the oracle's bits of one
+    uninterrupted run; so does a call that starts on a sort after the particles were rewritten (the
+    write drops the work items, so that call sorts first)."""
     _say(f"[gpu] split calls vs one oracle run (sort every {sort_every})")
     sim = MPM(ctx, grid=32, maxParticles=40000, substeps=8, sortEvery=sort_every)
     sim.add({"box": [[0.15, 0.15, 0.15], [0.45, 0.5, 0.45]]}, material="snow", velocity=[0.3, -0.2, 0.1], noise=0.2)
@@ -309,7 +310,7 @@ def test_split_calls_match_one_run(ctx, sort_every):
         sc.step(sum(calls), g)
         got, want = sim.read_particles(), sc.read(n)
         assert np.array_equal(bits(got), bits(want)), f"calls {calls}: {_first_diff(got, want)}"
-    # rewrite in place (same order): shift every particle by up to 3 cells, then step from a sort
+    # rewrite (same order): shift every particle by up to 3 cells, then step from a sort
     moved = got.copy()
     moved[:, 0:3] += (np.random.default_rng(5).random((n, 3)) - 0.5) * 6.0
     moved[:, 0:3] = np.clip(moved[:, 0:3], 3.0, 28.0)
