@@ -191,11 +191,16 @@ MPM_HD float el(const m3& m, int c, int r) { return r == 0 ? m.c[c].x : (r == 1 
 
 // One cyclic-Jacobi rotation in the (P, Q) plane: A <- J^T A J, V <- V J. Only rows / columns P and
 // Q change, each as the two-term sum the full 3x3 products would form (mul above: the first nonzero
-// product rounded, the second fused onto it; their other terms are multiples of zero).
+// product rounded, the second fused onto it; their other terms are multiples of zero). A lane whose
+// |a_pq| < 1e-30 keeps A and V as they are (the rotation is skipped). Every lane computes the
+// rotation (a skipping lane's values, possibly inf or NaN, are discarded), and only a wavefront with
+// a skipping lane selects the old values back: a branch around the rotation would end in a join
+// where the new values are copied into the old ones' registers (10 moves per rotation).
 template <int P, int Q>
 __device__ __forceinline__ void jacobi(m3& A, m3& V) {
+  const m3 A0 = A, V0 = V;
   float apq = el(A, Q, P);
-  if (fabsf(apq) < 1e-30f) return;
+  const bool skip = fabsf(apq) < 1e-30f;
   float theta = (el(A, Q, Q) - el(A, P, P)) / (2.0f * apq);
   float sgn = theta >= 0.0f ? 1.0f : -1.0f;
   // t = sgn / d with d = |theta| + sqrt(theta^2 + 1) >= 1; (+-1) / d = +-(1 / d) exactly
@@ -217,6 +222,13 @@ __device__ __forceinline__ void jacobi(m3& A, m3& V) {
     const float vp = el(V, P, i), vq = el(V, Q, i);
     el(V, P, i) = fmaf(vq, ms, vp * c);
     el(V, Q, i) = fmaf(vq, c, vp * s);
+  }
+  if (__builtin_amdgcn_ballot_w64(skip) != 0) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      A.c[k] = mk(skip ? A0.c[k].x : A.c[k].x, skip ? A0.c[k].y : A.c[k].y, skip ? A0.c[k].z : A.c[k].z);
+      V.c[k] = mk(skip ? V0.c[k].x : V.c[k].x, skip ? V0.c[k].y : V.c[k].y, skip ? V0.c[k].z : V.c[k].z);
+    }
   }
 }
 
