@@ -126,7 +126,8 @@ mpm_status mpm_read_grid(mpm_sim* sim, int32_t* words);
 mpm_status mpm_read_points(mpm_sim* sim, float* pts, float* vel, uint32_t offset, uint32_t n, const double origin[3],
                            double dx, double vscale);
 /* Device pointers of the 8 particle streams S0..S7 (capacity float4 entries each), for consumers on
- * the same GPU (e.g. an interop renderer). Float f of a reference record (mpm.js:54-63) is component
+ * the same GPU that read them (e.g. an interop renderer; changes go through mpm_write_particles, since
+ * a P2G carried from the last step call, MPM_TUNE_CARRY, does not see writes through these pointers). Float f of a reference record (mpm.js:54-63) is component
  * MPM_REC_SLOT[f] & 3 of stream MPM_REC_SLOT[f] >> 2: S0 = (x.xyz, material), S1 = (v.xyz, tau01),
  * S2..S4 = (C column k, tau_kk), S5 = (tau02, tau12, J, F22), S6 = (F00, F10, F20, F01),
  * S7 = (F11, F21, F02, F12) with Frc = row r of column c (the reference's f0..f2 are F's columns).
@@ -201,6 +202,11 @@ mpm_status mpm_set_profiling_every(mpm_sim* sim, uint32_t every);
                                * its work item's 4^3-cell block before it takes the global-atomic path:
                                * 2 (a 10^3-node LDS tile, default) or 3 (12^3; no particle slower than
                                * 1 cell per substep leaves it between sorts) */
+#define MPM_TUNE_CARRY 9      /* 1 (default): a step call that follows another with only reads (or
+                               * collider changes) between them ends with a G2P fused with the next
+                               * substep's P2G, which the next step call takes up instead of running
+                               * its own; any other call in between discards it. 0: every call opens
+                               * with its own P2G and ends with a plain G2P */
 mpm_status mpm_set_tuning(mpm_sim* sim, int32_t knob, int32_t value);
 
 /* ---- scene pools: many independent simulations advanced by the same launches ----

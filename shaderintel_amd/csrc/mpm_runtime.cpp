@@ -189,10 +189,18 @@ struct mpm_pool {
   bool g2p_items = true;     // G2P by work item (LDS velocity boxes) after a binned P2G
   bool tiles_fresh = false;  // the tiles and work items are those of this substep's binned P2G
   bool fuse = true;          // G2P fused with the next substep's P2G when no sort comes between them
-  bool p2g_ahead = false;    // the next substep's P2G is done (by a fused G2P of this mpm_step call)
+  bool p2g_ahead = false;    // the next substep's P2G is done (by a fused G2P of this mpm_step call, or
+                             // of the previous one when it is carried)
   bool p2g_presort = false;  // ... before that substep's sort, over the items of the sort before it
   bool keys_ahead = false;   // keys[0] holds the sort keys of the current particles (written by the
                              // G2P before a sort in the same mpm_step call)
+  // Work carried across calls: the last G2P of a step call also did the next substep's P2G (and,
+  // before a sort, its keys). Kept only while nothing but reads and further steps follow; any call
+  // that changes the pool discards it (drop_carry). carry_on: the knob; clean: no such call since the
+  // last step call (a call that follows another this way carries its own last P2G forward).
+  bool carry = false;
+  bool carry_on = true;
+  bool clean = false;
   bool use_graphs = false;
   mpm_grid_hook hook = nullptr;  // called after every grid update (not with graphs)
   void* hook_user = nullptr;
@@ -427,6 +435,19 @@ mpm_status apply_pending_reorder(mpm_pool* p) {
 // Sort keys, then reorder the particles now (MPM._sort, mpm.js:614-621), at this substep's G2P, or
 // not at all (the sorted order is kept as a permutation), and rebuild work items and S.
 mpm_status repack(mpm_pool* p);
+
+// Discards the P2G carried from the last step call (its tiles, A and keys are left to the next P2G,
+// which starts from a clean arena and clears A), and marks the pool changed since that call. Every
+// entry point that changes particles, materials, scenes, the step count, the sort period, the hook,
+// the grid or the kernel variants calls it first; reads and step calls do not.
+void drop_carry(mpm_pool* p) {
+  p->clean = false;
+  if (!p->carry) return;
+  p->carry = false;
+  p->p2g_ahead = p->keys_ahead = p->p2g_presort = false;
+  p->tiles_fresh = false;
+  p->grid_state = kGridVelocity;  // V still holds the last grid update's velocities
+}
 
 mpm_status do_bin(mpm_pool* p, int mode) {
   if (p->needs_repack) {  // binning assumes every particle belongs to a live scene
@@ -748,7 +769,13 @@ mpm_status period_graph(mpm_pool* p, float vmax, bool exit_ahead, hipGraphExec_t
 }
 
 mpm_status pool_step(mpm_pool* p, uint32_t substeps, float vmax) {
-  p->p2g_ahead = p->keys_ahead = p->p2g_presort = false;  // only ever set for the next substep of the same call
+  if (substeps == 0) return MPM_OK;
+  // work ahead is only kept within one call, or carried from the previous call (see carry)
+  if (!p->carry) p->p2g_ahead = p->keys_ahead = p->p2g_presort = false;
+  p->carry = false;
+  // a call that follows another with nothing but reads between them carries its last P2G forward
+  const bool carry_out = p->clean && p->carry_on && p->fuse && !p->hook && p->profiling != 1;
+  p->clean = false;
   p->last_vmax = vmax;
   if (p->needs_repack) ST_TRY(repack(p));
   if (p->count == 0) return MPM_OK;  // mpm.js:584
@@ -757,8 +784,8 @@ mpm_status pool_step(mpm_pool* p, uint32_t substeps, float vmax) {
   // call), which also writes the sort keys; the substep then runs its grid update before the sort. The
   // P2G reads the records in order instead of through the new permutation; its sums do not depend
   // on the order or on the items.
-  if (p->fuse && p->g2p_items && !p->hook && p->profiling != 1 && p->bin_valid && !p->perm && !p->reorder_in_g2p &&
-      p->step_count % p->sort_every == 0) {
+  if (!p->p2g_ahead && p->fuse && p->g2p_items && !p->hook && p->profiling != 1 && p->bin_valid && !p->perm &&
+      !p->reorder_in_g2p && p->step_count % p->sort_every == 0) {
     ST_TRY(do_p2g(p, false, p->keys[0]));
     p->p2g_ahead = p->keys_ahead = p->p2g_presort = true;
   }
@@ -767,8 +794,9 @@ mpm_status pool_step(mpm_pool* p, uint32_t substeps, float vmax) {
     const bool at_sort = p->step_count % p->sort_every == 0;
     // a sort period from a captured graph
     if (p->use_graphs && !p->profiling && !p->hook && at_sort && substeps - k >= p->sort_every) {
-      // the period's last G2P does the next sort's P2G when another substep follows in this call
-      const bool exit_ahead = substeps - k > p->sort_every;
+      // the period's last G2P does the next sort's P2G when another substep follows in this call (or
+      // the next call's, carried)
+      const bool exit_ahead = substeps - k > p->sort_every || carry_out;
       if (!p->p2g_presort) {  // (otherwise A holds part of the P2G done ahead, and its keys are set)
         ST_TRY(clean_a(p));
         ST_TRY(apply_pending_reorder(p));
@@ -791,11 +819,13 @@ mpm_status pool_step(mpm_pool* p, uint32_t substeps, float vmax) {
       p->step_count += p->sort_every;
       k += p->sort_every;
     } else {
-      const bool more = k + 1 < substeps;
+      const bool more = k + 1 < substeps || carry_out;
       ST_TRY(substep(p, vmax, k, more, more));
       k++;
     }
   }
+  p->carry = carry_out && p->p2g_ahead;
+  p->clean = true;
   return MPM_OK;
 }
 
@@ -1158,6 +1188,7 @@ void mpm_sim_destroy(mpm_sim* s) {
 
 mpm_status mpm_set_materials(mpm_sim* s, const float* tbl, uint32_t n) {
   CHECK_SIM(s);
+  drop_carry(s->pool);
   if (n > 8) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_materials: at most 8 materials (got %u)", n);
   if (n && !tbl) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_materials: table is NULL");
   for (uint32_t k = 0; k < n; k++) {
@@ -1183,6 +1214,7 @@ mpm_status mpm_set_colliders(mpm_sim* s, const float* tbl, uint32_t n) {
 
 mpm_status mpm_write_particles(mpm_sim* s, const float* rec, uint32_t offset, uint32_t n) {
   CHECK_SIM(s);
+  drop_carry(s->pool);
   if (s->pool->mid_substep) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_write_particles: inside a substep begun with mpm_substep_begin");
   if (n == 0) return MPM_OK;
   if (!rec) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_write_particles: records NULL");
@@ -1205,6 +1237,7 @@ mpm_status mpm_read_particles(mpm_sim* s, float* rec, uint32_t offset, uint32_t 
 
 mpm_status mpm_set_count(mpm_sim* s, uint32_t n) {
   CHECK_SIM(s);
+  drop_carry(s->pool);
   if (s->pool->mid_substep) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_count: inside a substep begun with mpm_substep_begin");
   mpm_pool* p = s->pool;
   if (n > p->cap) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_count: %u exceeds capacity %u", n, p->cap);
@@ -1226,12 +1259,14 @@ mpm_status mpm_get_count(mpm_sim* s, uint32_t* n) {
 
 mpm_status mpm_set_step_count(mpm_sim* s, uint64_t n) {
   if (!s) return fail(MPM_ERR_INVALID_ARGUMENT, "simulation is NULL");
+  drop_carry(s->pool);
   s->pool->step_count = n;
   return MPM_OK;
 }
 
 mpm_status mpm_set_sort_every(mpm_sim* s, uint32_t n) {
   CHECK_SIM(s);
+  drop_carry(s->pool);
   if (n == 0) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_sort_every: 0");
   if (s->pool->mid_substep) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_sort_every: inside a substep begun with mpm_substep_begin");
   if (s->pool->sort_every != n) {
@@ -1258,6 +1293,7 @@ mpm_status mpm_step(mpm_sim* s, uint32_t substeps, const float g[3], float ff, f
 
 mpm_status mpm_p2g(mpm_sim* s, int32_t force_global) {
   CHECK_SIM(s);
+  drop_carry(s->pool);
   if (s->pool->mid_substep) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_p2g: inside a substep begun with mpm_substep_begin");
   mpm_pool* p = s->pool;
   ST_TRY(do_p2g(p, force_global != 0));
@@ -1278,12 +1314,14 @@ mpm_status mpm_p2g_to(mpm_sim* s, int32_t force_global, int32_t* words) {
 
 mpm_status mpm_sort(mpm_sim* s) {
   CHECK_SIM(s);
+  drop_carry(s->pool);
   if (s->pool->mid_substep) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_sort: inside a substep begun with mpm_substep_begin");
   return do_bin(s->pool, kBinReorder);
 }
 
 mpm_status mpm_rescale(mpm_sim* s, float k) {
   CHECK_SIM(s);
+  drop_carry(s->pool);
   if (s->pool->mid_substep) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_rescale: inside a substep begun with mpm_substep_begin");
   mpm_pool* p = s->pool;
   ST_TRY(apply_pending_reorder(p));
@@ -1293,6 +1331,7 @@ mpm_status mpm_rescale(mpm_sim* s, float k) {
 
 mpm_status mpm_set_grid_hook(mpm_sim* s, mpm_grid_hook hook, void* user) {
   CHECK_SIM(s);
+  drop_carry(s->pool);
   s->pool->hook = hook;
   s->pool->hook_user = user;
   drop_graphs(s->pool);
@@ -1301,6 +1340,7 @@ mpm_status mpm_set_grid_hook(mpm_sim* s, mpm_grid_hook hook, void* user) {
 
 mpm_status mpm_substep_begin(mpm_sim* s, const float g[3], float ff) {
   CHECK_SIM(s);
+  drop_carry(s->pool);
   if (!g) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_substep_begin: gravity is NULL");
   mpm_pool* p = s->pool;
   if (p->mid_substep) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_substep_begin: the previous substep was not ended");
@@ -1315,6 +1355,7 @@ mpm_status mpm_substep_begin(mpm_sim* s, const float g[3], float ff) {
 
 mpm_status mpm_substep_end(mpm_sim* s, float vmax) {
   CHECK_SIM(s);
+  drop_carry(s->pool);
   mpm_pool* p = s->pool;
   if (!p->mid_substep) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_substep_end: no substep was begun");
   p->mid_substep = false;
@@ -1323,6 +1364,7 @@ mpm_status mpm_substep_end(mpm_sim* s, float vmax) {
 
 mpm_status mpm_write_grid(mpm_sim* s, const int32_t* words) {
   CHECK_SIM(s);
+  drop_carry(s->pool);
   if (!words) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_write_grid: words NULL");
   mpm_pool* p = s->pool;
   if (p->grid_state != kGridVelocity)
@@ -1432,7 +1474,7 @@ mpm_status mpm_read_grid_begin(mpm_sim* s, mpm_read** out) {
   mpm_pool* p = s->pool;
   const SceneGeo& geo = p->scenes[0].geo;
   const size_t nodes = (size_t)geo.X * geo.Y * geo.Z;
-  const int state = p->grid_state;
+  const int state = p->carry ? kGridVelocity : p->grid_state;  // a carried P2G leaves V as it was
   return begin_read(p, nodes * 16, out, [&](void* stg) {
     if (state != kGridA && state != kGridVelocity)
       return launch_fill_u32(static_cast<uint32_t*>(stg), 0u, (uint32_t)(nodes * 4), p->stream);
@@ -1482,7 +1524,8 @@ mpm_status mpm_read_grid(mpm_sim* s, int32_t* words) {
   mpm_pool* p = s->pool;
   const SceneGeo& geo = p->scenes[0].geo;
   const size_t nodes = (size_t)geo.X * geo.Y * geo.Z;
-  if (p->grid_state != kGridA && p->grid_state != kGridVelocity) {
+  const int state = p->carry ? kGridVelocity : p->grid_state;  // a carried P2G leaves V as it was
+  if (state != kGridA && state != kGridVelocity) {
     std::memset(words, 0, nodes * 16);
     return MPM_OK;
   }
@@ -1492,7 +1535,7 @@ mpm_status mpm_read_grid(mpm_sim* s, int32_t* words) {
     HIP_TRY(dmalloc(&p->export_buf, nodes));
     p->export_nodes = nodes;
   }
-  HIP_TRY(launch_grid_export(view(p), geo, p->grid_state == kGridA ? 0 : 1, p->export_buf, p->stream));
+  HIP_TRY(launch_grid_export(view(p), geo, state == kGridA ? 0 : 1, p->export_buf, p->stream));
   HIP_TRY(hipStreamSynchronize(p->stream));
   HIP_TRY(hipMemcpy(words, p->export_buf, nodes * 16, hipMemcpyDeviceToHost));
   return MPM_OK;
@@ -1513,6 +1556,7 @@ mpm_status mpm_set_profiling(mpm_sim* s, int32_t enable) {
   if (!s) return fail(MPM_ERR_INVALID_ARGUMENT, "simulation is NULL");
   mpm_pool* p = s->pool;
   if (enable < 0 || enable > 2) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_profiling: mode %d not in 0..2", enable);
+  drop_carry(p);
   p->profiling = enable;
   p->prof_seq = 0;
   p->ev_used = 0;
@@ -1600,6 +1644,7 @@ void mpm_pool_destroy(mpm_pool* p) { free_pool(p); }
 
 mpm_status mpm_pool_add_scene(mpm_pool* p, const mpm_scene_desc* d, uint32_t* scene) {
   CHECK_POOL(p);
+  drop_carry(p);
   if (!d || !scene) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_add_scene: NULL argument");
   if (d->count && !d->records) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_add_scene: records NULL");
   ST_TRY(check_materials(d->materials, d->num_materials, "mpm_pool_add_scene"));
@@ -1609,6 +1654,7 @@ mpm_status mpm_pool_add_scene(mpm_pool* p, const mpm_scene_desc* d, uint32_t* sc
 
 mpm_status mpm_pool_add_template(mpm_pool* p, const mpm_scene_desc* d, uint32_t* tmpl) {
   CHECK_POOL(p);
+  drop_carry(p);
   if (!d || !tmpl) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_add_template: NULL argument");
   if (d->count && !d->records) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_add_template: records NULL");
   ST_TRY(check_materials(d->materials, d->num_materials, "mpm_pool_add_template"));
@@ -1641,6 +1687,7 @@ mpm_status mpm_pool_add_template(mpm_pool* p, const mpm_scene_desc* d, uint32_t*
 
 mpm_status mpm_pool_remove_template(mpm_pool* p, uint32_t tmpl) {
   CHECK_POOL(p);
+  drop_carry(p);
   if (tmpl >= p->templates.size() || !p->templates[tmpl].used)
     return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_remove_template: no template %u", tmpl);
   HIP_TRY(hipStreamSynchronize(p->stream));
@@ -1651,6 +1698,7 @@ mpm_status mpm_pool_remove_template(mpm_pool* p, uint32_t tmpl) {
 
 mpm_status mpm_pool_add_scene_from_template(mpm_pool* p, uint32_t tmpl, uint32_t* scene) {
   CHECK_POOL(p);
+  drop_carry(p);
   if (!scene) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_add_scene_from_template: NULL argument");
   if (tmpl >= p->templates.size() || !p->templates[tmpl].used)
     return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_add_scene_from_template: no template %u", tmpl);
@@ -1660,6 +1708,7 @@ mpm_status mpm_pool_add_scene_from_template(mpm_pool* p, uint32_t tmpl, uint32_t
 
 mpm_status mpm_pool_remove_scene(mpm_pool* p, uint32_t scene) {
   CHECK_POOL(p);
+  drop_carry(p);
   if (scene >= p->scenes.size() || !p->scenes[scene].used || !p->scenes[scene].geo.active)
     return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_remove_scene: no scene %u", scene);
   p->scenes[scene].geo.active = 0;
@@ -1728,13 +1777,16 @@ mpm_status mpm_pool_use_graphs(mpm_pool* p, int32_t enable) {
 
 mpm_status mpm_pool_autotune(mpm_pool* p, uint32_t trial_substeps, int32_t* g2p_waves, int32_t* p2g_waves) {
   CHECK_POOL(p);
+  drop_carry(p);
   if (trial_substeps == 0) return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_autotune: trial_substeps = 0");
   const uint32_t trial = (trial_substeps + p->sort_every - 1) / p->sort_every * p->sort_every;
   const bool graphs = p->use_graphs;
   const int prof = p->profiling;
   const float vmax = p->last_vmax;  // the trials step as the caller does
+  const bool carry_on = p->carry_on;
   p->use_graphs = false;
   p->profiling = 0;
+  p->carry_on = false;  // every trial opens with its own P2G and ends with its own G2P
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
@@ -1787,6 +1839,7 @@ mpm_status mpm_pool_autotune(mpm_pool* p, uint32_t trial_substeps, int32_t* g2p_
   p->p2g_waves = bp;
   p->use_graphs = graphs;
   p->profiling = prof;
+  p->carry_on = carry_on;
   drop_graphs(p);
   if (g2p_waves) *g2p_waves = bg;
   if (p2g_waves) *p2g_waves = bp;
@@ -1802,6 +1855,7 @@ mpm_status mpm_pool_tuning(mpm_pool* p, int32_t knob, int32_t* value) {
     case MPM_TUNE_FUSE: *value = p->fuse ? 1 : 0; return MPM_OK;
     case MPM_TUNE_FUSE_SHAPE: *value = p->fuse_shape; return MPM_OK;
     case MPM_TUNE_DRIFT: *value = p->drift; return MPM_OK;
+    case MPM_TUNE_CARRY: *value = p->carry_on ? 1 : 0; return MPM_OK;
     default: return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_pool_tuning: unknown knob %d", knob);
   }
 }
@@ -1878,6 +1932,7 @@ mpm_status mpm_diagnostics_of(mpm_sim* s, mpm_diagnostics* out) {
 
 mpm_status mpm_pool_set_tuning(mpm_pool* p, int32_t knob, int32_t value) {
   if (!p) return fail(MPM_ERR_INVALID_ARGUMENT, "pool is NULL");
+  drop_carry(p);
   switch (knob) {
     case MPM_TUNE_G2P_WAVES:
       if (value < 3 || value > 6) break;
@@ -1908,6 +1963,10 @@ mpm_status mpm_pool_set_tuning(mpm_pool* p, int32_t knob, int32_t value) {
       if (value < 2 || value > 3) break;
       p->drift = value;
       drop_graphs(p);
+      return MPM_OK;
+    case MPM_TUNE_CARRY:
+      if (value < 0 || value > 1) break;
+      p->carry_on = value != 0;
       return MPM_OK;
     default:
       return fail(MPM_ERR_INVALID_ARGUMENT, "mpm_set_tuning: unknown knob %d", knob);

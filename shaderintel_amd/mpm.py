@@ -719,10 +719,10 @@ class MPM:
         self._alive("MPM.read_grid_law")
         return grid_law(self._h, reset=reset)
 
-    TUNE = {"g2p_waves": 2, "p2g_waves": 4, "g2p_items": 5, "fuse": 6, "fuse_shape": 7, "drift": 8}
+    TUNE = {"g2p_waves": 2, "p2g_waves": 4, "g2p_items": 5, "fuse": 6, "fuse_shape": 7, "drift": 8, "carry": 9}
 
     def set_tuning(self, **knobs) -> None:
-        """Kernel variant knobs (result-neutral): g2p_waves, p2g_waves, g2p_items, fuse, fuse_shape, drift."""
+        """Kernel variant knobs (result-neutral): g2p_waves, p2g_waves, g2p_items, fuse, fuse_shape, drift, carry."""
         for k, v in knobs.items():
             _check(load_library().mpm_set_tuning(self._h, self.TUNE[k], int(v)), "MPM.set_tuning")
 

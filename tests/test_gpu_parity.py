@@ -323,3 +323,91 @@ def test_split_calls_match_one_run(ctx, sort_every):
     gg, gw = sim.read_grid(), sc.grid()
     assert np.array_equal(gg, gw), _first_diff(gg, gw)
     sim.destroy()
+
+
+def _three_materials(ctx, grid=32):
+    sim = MPM(ctx, grid=grid, maxParticles=60000, substeps=8, sortEvery=4)
+    sim.add({"box": [[0.12, 0.15, 0.12], [0.42, 0.45, 0.42]]}, material="snow", velocity=[0.3, -0.2, 0.1], noise=0.2)
+    sim.add({"box": [[0.55, 0.15, 0.55], [0.85, 0.4, 0.85]]}, material=scenes.SAND30, velocity=[-0.2, 0.1, 0.0])
+    sim.add({"box": [[0.15, 0.55, 0.55], [0.45, 0.8, 0.85]]}, material="jelly", velocity=[0.0, -0.3, 0.0])
+    return sim
+
+
+@pytest.mark.parametrize("carry", [1, 0])
+def test_frames_with_reads_between_match_one_run(ctx, carry):
+    """The JavaScript host's loop: calls of 8 substeps with particle and grid reads between them. With
+    MPM_TUNE_CARRY on, a call that follows another with only reads between them ends with its G2P
+    fused with the next substep's P2G, and the next call takes that P2G up (the launch counts show
+    it); particles and grid (the last grid update's velocities) equal the oracle's after every call,
+    with the carry on and off."""
+    _say(f"[gpu] frames with reads between them vs oracle (carry {carry})")
+    sim = _three_materials(ctx)
+    sim.set_tuning(carry=carry)
+    raw = sim.read_particles()
+    sc = _mirror(sim, raw)
+    g = sim.lattice_gravity()
+    from shaderintel_amd.mpm import load_library, _ptr
+    L = load_library()
+    n = len(raw)
+    sim.set_profiling(2)
+    for frame in range(4):
+        assert L.mpm_step(sim._h, 8, _ptr(g), 0.4, 0.9) == 0
+        sc.step(8, g)
+        got, want = sim.read_particles(), sc.read(n)
+        assert np.array_equal(bits(got), bits(want)), f"frame {frame}: {_first_diff(got, want)}"
+        gg, gw = sim.read_grid(), sc.grid()
+        assert np.array_equal(gg, gw), f"frame {frame} grid: {_first_diff(gg, gw)}"
+    kt = sim.kernel_times()
+    # the first call after set_profiling ends with a plain G2P; with the carry, the next three end
+    # fused and open without a P2G of their own
+    assert (kt["g2p"][1], kt["g2p_p2g"][1]) == ((1, 31) if carry else (4, 28)), kt
+    sim.destroy()
+
+
+def test_changes_between_calls_discard_the_carried_p2g(ctx):
+    """Between calls that carry a P2G forward: a rewrite of the particles, a new material table, a
+    retime, a new step count, an explicit sort and an explicit P2G. Each discards the carried P2G
+    (otherwise the next call would start from sums of the old state); after each, the next call's
+    particles equal the oracle's doing the same."""
+    _say("[gpu] changes between calls vs oracle")
+    sim = _three_materials(ctx)
+    raw = sim.read_particles()
+    sc = _mirror(sim, raw)
+    g = sim.lattice_gravity()
+    from shaderintel_amd.mpm import load_library, _ptr
+    L = load_library()
+    n = len(raw)
+    rng = np.random.default_rng(3)
+
+    def frame(label):
+        for _ in range(2):  # the second call carries its last P2G forward
+            assert L.mpm_step(sim._h, 8, _ptr(g), 0.4, 0.9) == 0
+            sc.step(8, g)
+        got, want = sim.read_particles(), sc.read(n)
+        assert np.array_equal(bits(got), bits(want)), f"after {label}: {_first_diff(got, want)}"
+        return got
+
+    cur = frame("start")
+    moved = cur.copy()
+    moved[:, 4:7] += (rng.random((n, 3)) - 0.5).astype(np.float32) * 0.2  # velocities: P2G's momentum
+    sim.write_particles(moved)
+    sc.write(moved)
+    frame("a rewrite")
+    tbl = sim.material_table()
+    tbl[:3, 1] *= 1.25  # particle masses: P2G's mass and momentum
+    assert L.mpm_set_materials(sim._h, _ptr(tbl), 8) == 0
+    sc.set_materials(tbl)
+    frame("a new material table")
+    assert L.mpm_rescale(sim._h, 0.8) == 0
+    sc.rescale(0.8)
+    frame("a retime")
+    sim.stepCount = sim.stepCount + 3
+    sc.step_count = sc.step_count + 3
+    frame("a new step count")
+    sim._sort()
+    sc.sort()
+    frame("a sort")
+    sim.p2g()
+    sc.p2g()
+    frame("an explicit P2G")
+    sim.destroy()
