@@ -335,23 +335,25 @@ def _three_materials(ctx, grid=32):
 
 @pytest.mark.parametrize("carry", [1, 0])
 def test_frames_with_reads_between_match_one_run(ctx, carry):
-    """The JavaScript host's loop: calls of 8 substeps with particle and grid reads between them. With
-    MPM_TUNE_CARRY on, a call that follows another with only reads between them ends with its G2P
+    """The JavaScript host's loop: frames of 8 substeps (MPM.step, which uploads the colliders first)
+    with particle and grid reads between them. With MPM_TUNE_CARRY on, a call that follows another
+    with only reads (and collider uploads) between them ends with its G2P
     fused with the next substep's P2G, and the next call takes that P2G up (the launch counts show
     it); particles and grid (the last grid update's velocities) equal the oracle's after every call,
     with the carry on and off."""
     _say(f"[gpu] frames with reads between them vs oracle (carry {carry})")
     sim = _three_materials(ctx)
+    sim.setColliders([{"type": "sphere", "center": [0.5, 0.2, 0.5], "radius": 0.1, "friction": 0.4,
+                       "velocity": [0.1, 0.0, 0.0]}])
     sim.set_tuning(carry=carry)
     raw = sim.read_particles()
     sc = _mirror(sim, raw)
+    sc.set_colliders(sim.collider_table(), 1)
     g = sim.lattice_gravity()
-    from shaderintel_amd.mpm import load_library, _ptr
-    L = load_library()
     n = len(raw)
     sim.set_profiling(2)
     for frame in range(4):
-        assert L.mpm_step(sim._h, 8, _ptr(g), 0.4, 0.9) == 0
+        sim.step()  # uploads the collider table (kept: colliders act in the grid update), 8 substeps
         sc.step(8, g)
         got, want = sim.read_particles(), sc.read(n)
         assert np.array_equal(bits(got), bits(want)), f"frame {frame}: {_first_diff(got, want)}"
