@@ -262,7 +262,9 @@ typedef struct mpm_diagnostics {
   uint32_t work_items;          /* occupied 4x4x4-cell particle blocks of the last binning */
   uint32_t s_blocks;            /* node blocks the tiles of those items can reach */
   uint32_t d_blocks;            /* further node blocks written by the fallback path in the last P2G */
-  uint32_t tile_nodes_used;     /* tile arena nodes taken in the last P2G (may exceed capacity: overflow) */
+  uint32_t tile_nodes_used;     /* tile arena nodes taken in the last P2G: a slot of (6 + 2 drift)^3
+                                 * per work item when they all fit, else what the tiles took (may
+                                 * exceed capacity: overflow) */
   uint32_t tile_nodes_capacity;
   uint32_t particles;
 } mpm_diagnostics;

@@ -957,22 +957,47 @@ struct ItemCursor {
 // when the arena has no room, the box's non-zero nodes are added to A with atomics instead (and
 // their node blocks stamped), which gives the same sums. The nodes read are cleared, which leaves
 // the whole tile zero for the next item (the stencils of the in-window particles lie in the box).
-// Thread 0 takes the arena space and the workgroup's next item (into s_item) with two atomics
-// issued together: one round trip. Once an add finds the region full, the workgroup stops adding
-// to it (an add past the end takes nothing, so a region's counter ends at most one tile per
-// workgroup past its size). Every thread of the workgroup calls it; s_item is read after the
-// caller's next barrier.
+// slots (when the launch's items times N nodes fit the arena): item k's tile goes to its own slot
+// at k N, which every thread knows, so the store starts at once (no barrier after an arena atomic)
+// while thread 0 takes the workgroup's next item (into s_item). Otherwise thread 0 takes the arena
+// space and the next item with two atomics issued together (one round trip) before the store. Once
+// an add finds the region full, the workgroup stops adding to it (an add past the end takes nothing,
+// so a region's counter ends at most one tile per workgroup past its size). Every thread of the
+// workgroup calls it; s_item is read after the caller's next barrier.
 template <int THREADS, int D>
 __device__ __forceinline__ void store_item_tile(const Pool& pool, uint32_t k, uint32_t wb, const SceneGeo& geo, int* tile,
                                                 const int* bbox, int wx, int wy, int wz, uint32_t epoch,
                                                 uint32_t bin_epoch, uint32_t& s_off, ItemCursor& cur,
-                                                uint32_t& s_item) {
+                                                uint32_t& s_item, bool slots) {
   constexpr int S = Window<D>::kSpan, N = Window<D>::kNodes;
   const int tid = threadIdx.x;
   const bool any = bbox[0] != INT_MAX;
   const int px = bbox[0], py = bbox[1], pz = bbox[2];
   const int fx = any ? bbox[3] - px + 3 : 0, fy = any ? bbox[4] - py + 3 : 0, fz = any ? bbox[5] - pz + 3 : 0;
   const int pnodes = fx * fy * fz;  // <= N: inside the window tile
+  if (slots) {
+    const uint32_t toff = pnodes > 0 ? k * (uint32_t)N : 0xffffffffu;
+    if (tid == 0) {
+      s_item = next_item(cur.wctr, cur.items, cur.range, cur.spent);
+      pool.tile_off[k] = toff;
+      const int ext = fx | (fy << 8) | (fz << 16);
+      if (pnodes > 0) pool.tile_hdr[k] = make_int4(px, py, pz, ext);
+      pool.mtile[wb] = make_uint4(toff, (uint32_t)px | ((uint32_t)py << 10) | ((uint32_t)pz << 20), (uint32_t)ext, epoch);
+    }
+    const BoxDiv dx = box_div((uint32_t)fx), dxy = box_div((uint32_t)(fx * fy));
+    for (int t = tid; t < pnodes; t += THREADS) {
+      int lx, ly, lz;
+      box_coords(t, fx, fy, dx, dxy, lx, ly, lz);
+      const int w = ((pz + lz - wz) * S + (py + ly - wy)) * S + (px + lx - wx);
+      const int4 q = make_int4(tile[w], tile[N + w], tile[2 * N + w], tile[3 * N + w]);
+      tile[w] = 0;
+      tile[N + w] = 0;
+      tile[2 * N + w] = 0;
+      tile[3 * N + w] = 0;
+      pool.tiles[toff + tile_index(px, py, pz, fx, fy, fz, lx, ly, lz)] = q;
+    }
+    return;
+  }
   if (tid == 0) {  // arena space for the tile, from this workgroup's region
     const uint32_t region = blockIdx.x % kArenaRegions, rsize = pool.tile_nodes / kArenaRegions;
     const bool add = pnodes > 0 && !cur.full;
@@ -1038,6 +1063,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   const uint32_t items = pool.ctr[kCtrItems];
   const uint32_t epoch = pool.ctr[kCtrEpoch], bin_epoch = pool.ctr[kCtrBinEpoch];
   const uint32_t mshift = pool.key_bits - 6;
+  const bool slots = (uint64_t)items * N <= pool.tile_nodes;  // every item's tile in its own slot
   ItemCursor cur{pool.arena_ctr + kArenaRegions * kArenaStride, items, blockIdx.x % kWorkSplits, 0, false};
   for (int t = tid; t < N; t += kThreads)
     reinterpret_cast<int4*>(tile)[t] = make_int4(0, 0, 0, 0);  // the 4 planes; each item leaves them zero
@@ -1093,7 +1119,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
       atomicMax(&bbox[3], hi0); atomicMax(&bbox[4], hi1); atomicMax(&bbox[5], hi2);
     }
     __syncthreads();
-    store_item_tile<kThreads, D>(pool, k, wb, geo, tile, bbox, wx, wy, wz, epoch, bin_epoch, s_off, cur, s_item);
+    store_item_tile<kThreads, D>(pool, k, wb, geo, tile, bbox, wx, wy, wz, epoch, bin_epoch, s_off, cur, s_item, slots);
     __syncthreads();  // the tile is read out and zero again
   }
 }
@@ -1130,6 +1156,7 @@ __global__ __launch_bounds__(kFuseThreadsOf[SHAPE]) __attribute__((amdgpu_waves_
   const uint32_t items = pool.ctr[kCtrItems];
   const uint32_t epoch = pool.ctr[kCtrEpoch], bin_epoch = pool.ctr[kCtrBinEpoch];
   const uint32_t mshift = pool.key_bits - 6;
+  const bool slots = (uint64_t)items * N <= pool.tile_nodes;  // every item's tile in its own slot
   // P2G's work counters (the launch_begin_p2g before this kernel cleared them; G2P's stay fresh for
   // the next substep's G2P, which has no launch_begin_p2g of its own before it)
   ItemCursor cur{pool.arena_ctr + kArenaRegions * kArenaStride, items, blockIdx.x % kWorkSplits, 0, false};
@@ -1233,7 +1260,8 @@ __global__ __launch_bounds__(kFuseThreadsOf[SHAPE]) __attribute__((amdgpu_waves_
       atomicMax(&bbox[3], hi0); atomicMax(&bbox[4], hi1); atomicMax(&bbox[5], hi2);
     }
     __syncthreads();
-    store_item_tile<kFuseThreads, D>(pool, k, wb, geo, tile, bbox, wx, wy, wz, epoch, bin_epoch, s_off, cur, s_item);
+    store_item_tile<kFuseThreads, D>(pool, k, wb, geo, tile, bbox, wx, wy, wz, epoch, bin_epoch, s_off, cur, s_item,
+                                     slots);
     __syncthreads();  // the tile is read out and zero again
   }
 }

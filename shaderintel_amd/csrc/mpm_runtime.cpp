@@ -1916,6 +1916,9 @@ mpm_status mpm_pool_diagnostics(mpm_pool* p, mpm_diagnostics* out) {
   HIP_TRY(hipMemcpy(a.data(), p->arena_ctr, a.size() * 4, hipMemcpyDeviceToHost));
   uint64_t used = 0;
   for (int r = 0; r < kArenaRegions; r++) used += a[(size_t)r * kArenaStride];
+  // the P2G kernels give every work item a slot of the window tile's size when all slots fit
+  const uint64_t slots = (uint64_t)c[kCtrItems] * (p->drift == 2 ? Window<2>::kNodes : Window<3>::kNodes);
+  if (slots <= p->tile_nodes) used = slots;
   out->work_items = c[kCtrItems];
   out->s_blocks = c[kCtrS];
   out->d_blocks = c[kCtrD];

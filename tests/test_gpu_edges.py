@@ -1,13 +1,13 @@
 """Edge cases of the substep against the oracle, bit for bit: all three collider kinds (moving,
 with friction), retiming (rescaleK), grids whose edges are not multiples of 4 and not cubic,
-1 and 3 particles per cell axis, a sort every substep, a full particle buffer, and an empty
-simulation."""
+1 and 3 particles per cell axis, a sort every substep, a full particle buffer, an empty
+simulation, and particles so sparse that the work items' tiles take arena space by atomics."""
 import numpy as np
 import pytest
 
 import oracle_mpm as O
 from helpers import bits
-from shaderintel_amd.mpm import MPM, load_library, _ptr
+from shaderintel_amd.mpm import MPM, diagnostics, load_library, _ptr
 
 pytestmark = pytest.mark.gpu
 
@@ -79,6 +79,32 @@ def test_odd_grids_and_ppc(ctx, size, grid, ppc):
     sim.step()
     sc.step(16, sim.lattice_gravity())
     _same(sim.read_particles(), sc.read(len(rec)), "particles")
+    _same(sim.read_grid(), sc.grid(), "grid")
+    sim.destroy()
+
+
+def test_sparse_items_take_arena_space_by_atomics(ctx):
+    """10,000 particles scattered over a 128^3 grid, most alone in their 4^3-cell block: about 8,600
+    work items, whose window-tile slots (1,000 nodes each) do not fit the 7.1M-node arena, so the P2G
+    kernels take each tile's space from the arena's counters instead of a slot per item. Particles
+    and grid equal the oracle's after two frames (with the slots, every other test's path)."""
+    print("[gpu] sparse items: tiles placed by arena atomics", flush=True)
+    sim = MPM(ctx, grid=128, maxParticles=12000, substeps=8)
+    sim.add({"box": [[0.4, 0.4, 0.4], [0.45, 0.45, 0.45]]}, material="snow", velocity=[0.2, -0.1, 0.3], noise=0.2)
+    base = sim.read_particles()
+    n = 10000
+    rng = np.random.default_rng(21)
+    rec = base[rng.integers(0, len(base), n)].copy()
+    rec[:, 0:3] = rng.uniform(4.0, 123.0, (n, 3)).astype(np.float32)
+    sim.write_particles(rec)
+    assert load_library().mpm_set_count(sim._h, n) == 0
+    sc = _oracle_of(sim, rec)
+    sim.step()
+    sim.step()
+    sc.step(16, sim.lattice_gravity())
+    d = diagnostics(sim._h)
+    assert d["work_items"] * 1000 > d["tile_nodes_capacity"] >= d["tile_nodes_used"] > 0, d
+    _same(sim.read_particles(), sc.read(n), "particles")
     _same(sim.read_grid(), sc.grid(), "grid")
     sim.destroy()
 
