@@ -958,9 +958,9 @@ struct ItemCursor {
 // their node blocks stamped), which gives the same sums. The nodes read are cleared, which leaves
 // the whole tile zero for the next item (the stencils of the in-window particles lie in the box).
 // slots (when the launch's items times N nodes fit the arena): item k's tile goes to its own slot
-// at k N, which every thread knows, so the store starts at once (no barrier after an arena atomic)
-// while thread 0 takes the workgroup's next item (into s_item). Otherwise thread 0 takes the arena
-// space and the next item with two atomics issued together (one round trip) before the store. Once
+// at k N, which every thread knows, so the store starts at once (no barrier after an arena atomic;
+// the caller took the workgroup's next item before the bounding box). Otherwise thread 0 takes the
+// arena space and the next item with two atomics issued together (one round trip) before the store. Once
 // an add finds the region full, the workgroup stops adding to it (an add past the end takes nothing,
 // so a region's counter ends at most one tile per workgroup past its size). Every thread of the
 // workgroup calls it; s_item is read after the caller's next barrier.
@@ -978,7 +978,6 @@ __device__ __forceinline__ void store_item_tile(const Pool& pool, uint32_t k, ui
   if (slots) {
     const uint32_t toff = pnodes > 0 ? k * (uint32_t)N : 0xffffffffu;
     if (tid == 0) {
-      s_item = next_item(cur.wctr, cur.items, cur.range, cur.spent);
       pool.tile_off[k] = toff;
       const int ext = fx | (fy << 8) | (fz << 16);
       if (pnodes > 0) pool.tile_hdr[k] = make_int4(px, py, pz, ext);
@@ -1112,6 +1111,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
         load_p2g_record(P, pool.cap, si, r);
       }
     }
+    // with slots, the next item is taken here (every thread read k before the first barrier), its round
+    // trip hidden by the bounding box and the tile store
+    if (slots && tid == 0) s_item = next_item(cur.wctr, cur.items, cur.range, cur.spent);
     lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
     hi0 = wave_max(hi0); hi1 = wave_max(hi1); hi2 = wave_max(hi2);
     if ((tid & 63) == 0) {
@@ -1253,6 +1255,9 @@ __global__ __launch_bounds__(kFuseThreadsOf[SHAPE]) __attribute__((amdgpu_waves_
         fallback_particle(pool, r, epoch, true, bin_epoch);
       }
     }
+    // with slots, the next item is taken here (every thread read k before the box barrier), its round
+    // trip hidden by the bounding box and the tile store
+    if (slots && tid == 0) s_item = next_item(cur.wctr, cur.items, cur.range, cur.spent);
     lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
     hi0 = wave_max(hi0); hi1 = wave_max(hi1); hi2 = wave_max(hi2);
     if ((tid & 63) == 0) {
