@@ -4,7 +4,8 @@
   175-193) on the P2G grid;
 - the 64-scene batched mix from master seed 1 (edges 32-128, 1e4-1e6 particles, water / jelly / snow /
   sand): one scene pool with captured sort periods, 8 substeps in two calls, each scene bit for bit
-  against the oracle running it alone."""
+  against the oracle running it alone;
+- a pool stepped call by call with reads, an added and a removed scene between the calls."""
 from __future__ import annotations
 
 import numpy as np
@@ -72,4 +73,47 @@ def test_batched_mix_matches_each_scene_alone(ctx):
         got, want = pool.read_scene(sid), sc.read(len(rec))
         assert np.array_equal(bits(got), bits(want)), f"scene {sid} ({dims[0]}^3, {len(rec)}): {_first_diff(got, want)}"
         del sc
+    pool.destroy()
+
+
+def test_pool_calls_with_reads_and_scene_changes_between(ctx):
+    """A pool stepped in calls of 4 substeps, with its scenes read between them (which keeps the P2G
+    the previous call carried forward) and a scene added and another removed between two of them
+    (which discard it): every scene equals the oracle running it alone, after every call."""
+    built = [scenes.host_blocks(edge, blocks) for edge, blocks in scenes.batched_blocks(3, 6)]
+    total = sum(len(b[2]) for b in built)
+    print(f"[gpu] pool calls with reads and scene changes between: 6 scenes, {total} particles", flush=True)
+    pool = ScenePool(ctx, total, 8, sum(node_blocks(b[0]) for b in built), max(b[0][0] for b in built))
+    live = {}
+    for dims, tbl, rec, g in built[:4]:
+        sid = pool.add_scene(dims, tbl, rec, g)
+        sc = O.Scene(dims, len(rec), 2, 4)
+        sc.set_materials(tbl)
+        sc.write(rec)
+        sc.set_count(len(rec))
+        live[sid] = (sc, g, len(rec))
+
+    def call(label):
+        pool.step(4)
+        for sid, (sc, g, n) in live.items():
+            sc.step(4, g)
+            got, want = pool.read_scene(sid), sc.read(n)
+            assert np.array_equal(bits(got), bits(want)), f"{label}, scene {sid}: {_first_diff(got, want)}"
+
+    for k in range(3):
+        call(f"call {k}")
+    dims, tbl, rec, g = built[4]
+    sid = pool.add_scene(dims, tbl, rec, g)
+    sc = O.Scene(dims, len(rec), 2, 4)
+    sc.set_materials(tbl)
+    sc.write(rec)
+    sc.set_count(len(rec))
+    live[sid] = (sc, g, len(rec))
+    call("after an added scene")
+    call("the call after it")
+    gone = next(iter(live))
+    pool.remove_scene(gone)
+    del live[gone]
+    call("after a removed scene")
+    call("the call after it")
     pool.destroy()
