@@ -852,8 +852,9 @@ __global__ __launch_bounds__(kThreads) void k_bin_transpose(Pool pool, const uin
   }
 }
 
-// S: the node blocks [B-1, B+2]^3 of every work item B (everything its tile can reach), i.e. node
-// block b when a particle block in [b-2, b+1]^3 is occupied. One thread per 64-block word of a row
+// S: the node blocks [B-1, B+D-1]^3 of every work item B (everything its drift-D tile can reach:
+// nodes [4B-D, 4B+6+D)), i.e. node block b when a particle block in [b-D+1, b+1]^3 is occupied (D =
+// 2 or 3). One thread per 64-block word of a row
 // (by, bz) of a scene's node-block grid: the row's word ORs the x-dilated occupancy words
 // (O | O<<1 | O<<2 | O>>1) of the 16 rows (by-2..by+1, bz-2..bz+1); its set bits inside the grid are
 // appended to slist (one atomic per wave) and stamped.
@@ -862,7 +863,7 @@ __device__ __forceinline__ unsigned long long occ_word(const Pool& pool, uint32_
   if (by < 0 || bz < 0 || by >= (int)pb || bz >= (int)pb || w < 0 || w >= (int)W) return 0ull;
   return pool.occ[((size_t)scene * pb * pb + (size_t)bz * pb + (size_t)by) * W + (size_t)w];
 }
-__global__ __launch_bounds__(kThreads) void k_bin_dilate(Pool pool, uint32_t max_scenes) {
+__global__ __launch_bounds__(kThreads) void k_bin_dilate(Pool pool, uint32_t max_scenes, int drift) {
   const uint32_t pb = 1u << (pool.key_bits / 3 - 2), W = occ_row_words(pb);
   const uint32_t epoch = pool.ctr[kCtrBinEpoch];
   const int lane = threadIdx.x & 63;
@@ -881,12 +882,14 @@ __global__ __launch_bounds__(kThreads) void k_bin_dilate(Pool pool, uint32_t max
       nbx = geo.nbx; nby = geo.nby;
       base = geo.block_base;
       if (geo.active && by < geo.nby && bz < geo.nbz && 64 * w < geo.nbx) {
-        for (int dz = -2; dz <= 1; dz++)
-          for (int dy = -2; dy <= 1; dy++) {
+        const int r = drift - 1;  // particle blocks b-r .. b+1 reach node block b
+        for (int dz = -r; dz <= 1; dz++)
+          for (int dy = -r; dy <= 1; dy++) {
             const unsigned long long o = occ_word(pool, scene, pb, W, by + dy, bz + dz, w);
             const unsigned long long lo = occ_word(pool, scene, pb, W, by + dy, bz + dz, w - 1);
             const unsigned long long hi = occ_word(pool, scene, pb, W, by + dy, bz + dz, w + 1);
-            d |= o | (o << 1) | (lo >> 63) | (o << 2) | (lo >> 62) | (o >> 1) | (hi << 63);
+            d |= o | (o << 1) | (lo >> 63) | (o >> 1) | (hi << 63);
+            if (r == 2) d |= (o << 2) | (lo >> 62);
           }
         const int valid = geo.nbx - 64 * w;  // bits of this word inside the grid
         if (valid < 64) d &= (1ull << valid) - 1ull;
@@ -1294,12 +1297,18 @@ constexpr int kPullBatch = 8;  // tile loads a grid-update wavefront keeps in fl
 // the candidates whose tile box overlaps the node block are then visited with wave-uniform tile
 // indices, up to kPullBatch tile loads (one contiguous run each) in flight at a time. Integer sums,
 // so the visiting order does not matter.
+template <int D>
 __device__ __forceinline__ int4 pull_tiles(const Pool& pool, uint32_t scene, uint32_t epoch, int g0, int g1, int g2,
                                            int lane, int n0, int n1, int n2, int4 acc) {
   const int pb = 1 << (pool.key_bits / 3 - 2);  // particle blocks per axis a scene's slots address
-  const int cx = g0 - 2 + (lane & 3), cy = g1 - 2 + ((lane >> 2) & 3), cz = g2 - 2 + (lane >> 4);
+  // lane L's candidate: the particle blocks whose drift-D tiles can reach node block g, [g-D+1, g+1]
+  // per axis (4^3 = 64 candidates for D = 3, 3^3 = 27 for D = 2; the other lanes have none)
+  constexpr int kSide = D + 1;
+  const int cx = g0 - (D - 1) + lane % kSide, cy = g1 - (D - 1) + (lane / kSide) % kSide,
+            cz = g2 - (D - 1) + lane / (kSide * kSide);
   uint4 e = make_uint4(0xffffffffu, 0u, 0u, 0u);
-  if ((unsigned)cx < (unsigned)pb && (unsigned)cy < (unsigned)pb && (unsigned)cz < (unsigned)pb)
+  if (lane < kSide * kSide * kSide && (unsigned)cx < (unsigned)pb && (unsigned)cy < (unsigned)pb &&
+      (unsigned)cz < (unsigned)pb)
     e = pool.mtile[(scene << (pool.key_bits - 6)) + morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz)];
   const bool has = e.w == epoch && e.x != 0xffffffffu;  // a tile of this P2G
   // this lane's candidate tile: the chunk it holds for the wave's node block (tile_chunk), as the
@@ -1387,13 +1396,12 @@ __device__ __forceinline__ void law_block(unsigned long long* __restrict__ law, 
 
 // Grid update (VELOCITY = true, mpm.js:428-470) or tile resolve into A (VELOCITY = false) over S
 // (when use_s) then D, one wavefront per node block.
-template <bool VELOCITY>
+template <bool VELOCITY, int D>
 __global__ __launch_bounds__(kThreads) void k_grid_pull(Pool pool, int use_s) {
   const uint32_t ns = use_s ? pool.ctr[kCtrS] : 0u, nd = pool.ctr[kCtrD];
   const uint32_t epoch = pool.ctr[kCtrEpoch];
-  if (VELOCITY && blockIdx.x == 0 && threadIdx.x == 0) {  // label V: the P2G and binning epochs it comes from
+  if (VELOCITY && blockIdx.x == 0 && threadIdx.x == 0) {  // label V: the P2G epoch it comes from
     pool.ctr[kCtrVEpoch] = epoch;
-    pool.ctr[kCtrVBinEpoch] = use_s ? pool.ctr[kCtrBinEpoch] : 0xffffffffu;
     if (pool.law) atomicAdd(pool.law + 14, 1ull);  // one more grid update in the bookkeeping
   }
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1409,11 +1417,12 @@ __global__ __launch_bounds__(kThreads) void k_grid_pull(Pool pool, int use_s) {
     int4 acc = make_int4(0, 0, 0, 0);
     const bool written = pool.stamp[b] == epoch;
     if (written) acc = pool.A[addr];
-    if (j < ns) acc = pull_tiles(pool, scene, epoch, g0, g1, g2, lane, n0, n1, n2, acc);
+    if (j < ns) acc = pull_tiles<D>(pool, scene, epoch, g0, g1, g2, lane, n0, n1, n2, acc);
     if (VELOCITY) {
       f3 vf;
       const float4 vo = grid_node(acc, n0, n1, n2, geo, pool.cols + 12 * kColRows * scene, vf);
       pool.V[addr] = vo;
+      if (lane == 0) pool.vstamp[b] = epoch;  // V of this block is this grid update's (read by exports)
       if (written) pool.A[addr] = make_int4(0, 0, 0, 0);
       if (pool.law) law_block(pool.law, acc, vo, vf, geo.g, lane);
     } else {
@@ -1441,7 +1450,7 @@ __global__ void k_grid_export(Pool pool, SceneGeo geo, int mode, int4* __restric
   const uint32_t b = addr >> 6;
   if (mode == 0) {
     out[i] = pool.A[addr];
-  } else if (pool.sstamp[b] == pool.ctr[kCtrVBinEpoch] || pool.stamp[b] == pool.ctr[kCtrVEpoch]) {
+  } else if (pool.vstamp[b] == pool.ctr[kCtrVEpoch]) {  // written by the last grid update
     const float4 v = pool.V[addr];
     out[i] = make_int4(__float_as_int(v.x), __float_as_int(v.y), __float_as_int(v.z), __float_as_int(v.w));
   } else {
@@ -1561,11 +1570,11 @@ hipError_t launch_bin_transpose(const Pool& pool, const uint32_t* keys, const ui
   hipLaunchKernelGGL(k_bin_transpose, dim3(grid), dim3(kThreads), 0, s, pool, keys, in, out);
   return hipGetLastError();
 }
-hipError_t launch_bin_dilate(const Pool& pool, uint32_t max_scenes, hipStream_t s) {
+hipError_t launch_bin_dilate(const Pool& pool, uint32_t max_scenes, int drift, hipStream_t s) {
   const uint32_t pb = 1u << (pool.key_bits / 3 - 2);
   const uint64_t total = (uint64_t)max_scenes * pb * pb * occ_row_words(pb);
   hipLaunchKernelGGL(k_bin_dilate, dim3((unsigned)std::min<uint64_t>((total + kThreads - 1) / kThreads, 8192)),
-                     dim3(kThreads), 0, s, pool, max_scenes);
+                     dim3(kThreads), 0, s, pool, max_scenes, drift);
   return hipGetLastError();
 }
 template <int D>
@@ -1596,13 +1605,16 @@ hipError_t launch_write_words(uint32_t* dst, const uint32_t* src, uint32_t n, hi
   hipLaunchKernelGGL(k_write_words, dim3(1), dim3(64), 0, s, w, dst, n);
   return hipGetLastError();
 }
-hipError_t launch_grid_update(const Pool& pool, bool use_s, hipStream_t s) {
-  hipLaunchKernelGGL(k_grid_pull<true>, dim3(persistent_grid(2 * pool.nblocks, 4, 8192)), dim3(kThreads), 0, s, pool,
-                     use_s ? 1 : 0);
+hipError_t launch_grid_update(const Pool& pool, bool use_s, int drift, hipStream_t s) {
+  const dim3 grid(persistent_grid(2 * pool.nblocks, 4, 8192));
+  if (drift == 2) hipLaunchKernelGGL((k_grid_pull<true, 2>), grid, dim3(kThreads), 0, s, pool, use_s ? 1 : 0);
+  else hipLaunchKernelGGL((k_grid_pull<true, 3>), grid, dim3(kThreads), 0, s, pool, use_s ? 1 : 0);
   return hipGetLastError();
 }
-hipError_t launch_resolve_tiles(const Pool& pool, hipStream_t s) {
-  hipLaunchKernelGGL(k_grid_pull<false>, dim3(persistent_grid(2 * pool.nblocks, 4, 8192)), dim3(kThreads), 0, s, pool, 1);
+hipError_t launch_resolve_tiles(const Pool& pool, int drift, hipStream_t s) {
+  const dim3 grid(persistent_grid(2 * pool.nblocks, 4, 8192));
+  if (drift == 2) hipLaunchKernelGGL((k_grid_pull<false, 2>), grid, dim3(kThreads), 0, s, pool, 1);
+  else hipLaunchKernelGGL((k_grid_pull<false, 3>), grid, dim3(kThreads), 0, s, pool, 1);
   return hipGetLastError();
 }
 hipError_t launch_clear_a(const Pool& pool, bool use_s, hipStream_t s) {

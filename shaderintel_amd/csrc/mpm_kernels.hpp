@@ -94,7 +94,7 @@ enum Counter : int {
   kCtrS = 4,         // |S|
   kCtrWork = 5,      // unused (P2G's work counters are in arena_ctr)
   kCtrVEpoch = 6,    // P2G epoch whose grid update wrote V
-  kCtrVBinEpoch = 7, // binning epoch of that grid update (0xffffffff: S not used)
+  kCtrVBinEpoch = 7, // unused (a grid update stamps the blocks it writes, vstamp)
   kCounters = 8
 };
 
@@ -129,6 +129,7 @@ struct Pool {
   uint32_t* stamp;         // D stamps (P2G epoch)
   uint32_t* dlist;
   uint32_t* sstamp;        // S stamps (binning epoch)
+  uint32_t* vstamp;        // V stamps: the P2G epoch of the grid update that last wrote the block's V
   uint32_t* slist;
   uint2* sinfo;            // [nblocks] block_info of slist's entries, written beside them
   uint4* mtile;            // [block slots] tile of the particle block at each slot (scene << (key_bits - 6)
@@ -196,8 +197,9 @@ hipError_t launch_bin_emit(const Pool& pool, const uint32_t* keys_sorted, uint32
 hipError_t launch_bin_transpose(const Pool& pool, const uint32_t* keys_sorted, const uint32_t* in, uint32_t* out,
                                 hipStream_t s);
 // S from the occupancy bitmap: each node block of a scene's grid with an occupied particle block
-// within [-2, +1] per axis, appended to slist and stamped with the binning epoch.
-hipError_t launch_bin_dilate(const Pool& pool, uint32_t max_scenes, hipStream_t s);
+// within [-(drift - 1), +1] per axis (the blocks whose drift-window tiles can reach it), appended to
+// slist and stamped with the binning epoch.
+hipError_t launch_bin_dilate(const Pool& pool, uint32_t max_scenes, int drift, hipStream_t s);
 
 // P2G over the work items into tiles (+ fallback atomics on A). perm: sorted position -> particle
 // (nullptr: identity).
@@ -209,10 +211,11 @@ hipError_t launch_p2g_binned(const Pool& pool, const float4* P, const uint32_t* 
 hipError_t launch_p2g_global(const Pool& pool, const float4* P, uint32_t n, hipStream_t s);
 // Writes n (<= 64) 32-bit words to dst from a kernel argument (ordered with the stream's work).
 hipError_t launch_write_words(uint32_t* dst, const uint32_t* src, uint32_t n, hipStream_t s);
-// Grid update over S and D (tiles + A -> V, A back to zero). use_s = false: only D holds data.
-hipError_t launch_grid_update(const Pool& pool, bool use_s, hipStream_t s);
+// Grid update over S and D (tiles + A -> V, A back to zero). use_s = false: only D holds data. drift:
+// the window of the P2G that made the tiles (its tiles' reach), as for S.
+hipError_t launch_grid_update(const Pool& pool, bool use_s, int drift, hipStream_t s);
 // Folds the tiles into A over S (A then holds the whole P2G result, for reads).
-hipError_t launch_resolve_tiles(const Pool& pool, hipStream_t s);
+hipError_t launch_resolve_tiles(const Pool& pool, int drift, hipStream_t s);
 // Zeroes A over S (use_s) and D.
 hipError_t launch_clear_a(const Pool& pool, bool use_s, hipStream_t s);
 // G2P; ev_start / ev_stop (optional) are recorded by the dispatch itself at the kernel's start / end.

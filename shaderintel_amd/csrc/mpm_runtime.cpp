@@ -150,7 +150,7 @@ struct mpm_pool {
   int4* A = nullptr;
   float4* V = nullptr;
   uint2* block_info = nullptr;
-  uint32_t *stamp = nullptr, *dlist = nullptr, *sstamp = nullptr, *slist = nullptr;
+  uint32_t *stamp = nullptr, *dlist = nullptr, *sstamp = nullptr, *slist = nullptr, *vstamp = nullptr;
   uint2* sinfo = nullptr;
   uint4* mtile = nullptr;
   uint32_t *wl_start = nullptr, *wl_block = nullptr;
@@ -249,6 +249,7 @@ Pool view(const mpm_pool* p) {
   v.stamp = p->stamp;
   v.dlist = p->dlist;
   v.sstamp = p->sstamp;
+  v.vstamp = p->vstamp;
   v.slist = p->slist;
   v.sinfo = p->sinfo;
   v.mtile = p->mtile;
@@ -289,7 +290,7 @@ void free_pool(mpm_pool* p) {
                   p->sstamp, p->slist,    p->wl_start, p->wl_block,    p->tiles,   p->tile_hdr, p->tile_off,
                   p->mats,   p->cols,     p->scenes_d, p->ctr, p->arena_ctr,      p->occ,      p->keys[0],     p->keys[1], p->vals[0],
                   p->vals[1], p->order,   p->flags,    p->sort_scratch, p->staging, p->export_buf, p->law,
-                  p->sinfo,   p->mtile};
+                  p->sinfo,   p->mtile,   p->vstamp};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (p->stream) hipStreamDestroy(p->stream);
@@ -320,7 +321,7 @@ size_t pool_bytes(uint32_t cap, uint32_t max_scenes, uint32_t max_blocks, int ma
   size_t b = 0;
   b += 2 * (size_t)cap * 128;                            // P[2]
   b += (size_t)max_blocks * 64 * (16 + 16);              // A, V
-  b += (size_t)max_blocks * 4 * 6;                       // block_info, stamp, dlist, sstamp, slist
+  b += (size_t)max_blocks * 4 * 7;                       // block_info, stamp, dlist, sstamp, slist, vstamp
   b += (size_t)max_blocks * 8;                           // sinfo
   b += (size_t)mps * max_scenes * 16;                    // mtile
   b += (size_t)occ_words(mps) * max_scenes * 8;          // occupancy bitmap
@@ -489,7 +490,7 @@ mpm_status do_bin(mpm_pool* p, int mode) {
     p->perm = p->order;
     p->reorder_in_g2p = mode == kBinReorderInG2P;
   }
-  HIP_TRY(launch_bin_dilate(pv, p->max_scenes, p->stream));
+  HIP_TRY(launch_bin_dilate(pv, p->max_scenes, p->drift, p->stream));
   p->bin_valid = true;
   return MPM_OK;
 }
@@ -557,7 +558,7 @@ mpm_status do_p2g(mpm_pool* p, bool force_global, uint32_t* keys_out = nullptr) 
 
 // Grid update (mpm.js:593-595) from tiles + A into V; A returns to zero.
 mpm_status do_grid(mpm_pool* p) {
-  HIP_TRY(launch_grid_update(view(p), p->grid_state == kGridTiles, p->stream));
+  HIP_TRY(launch_grid_update(view(p), p->grid_state == kGridTiles, p->drift, p->stream));
   p->a_dirty_d = false;
   p->grid_state = kGridVelocity;
   return MPM_OK;
@@ -864,6 +865,7 @@ mpm_status create_pool(mpm_ctx* c, uint32_t cap, uint32_t max_scenes, uint32_t m
   if (e == hipSuccess) e = dmalloc(&p->stamp, max_blocks);
   if (e == hipSuccess) e = dmalloc(&p->dlist, max_blocks);
   if (e == hipSuccess) e = dmalloc(&p->sstamp, max_blocks);
+  if (e == hipSuccess) e = dmalloc(&p->vstamp, max_blocks);
   if (e == hipSuccess) e = dmalloc(&p->slist, max_blocks);
   if (e == hipSuccess) e = dmalloc(&p->sinfo, max_blocks);
   if (e == hipSuccess) e = dmalloc(&p->mtile, (size_t)p->map_per_scene * max_scenes);
@@ -895,6 +897,7 @@ mpm_status create_pool(mpm_ctx* c, uint32_t cap, uint32_t max_scenes, uint32_t m
   if (e == hipSuccess) e = hipMemsetAsync(p->block_info, 0xff, (size_t)max_blocks * 8, p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->stamp, 0xff, (size_t)max_blocks * 4, p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->sstamp, 0xff, (size_t)max_blocks * 4, p->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(p->vstamp, 0xff, (size_t)max_blocks * 4, p->stream);
   if (e == hipSuccess)  // epoch 0xffffffff: no P2G has written the entry
     e = hipMemsetAsync(p->mtile, 0xff, (size_t)p->map_per_scene * max_scenes * sizeof(uint4), p->stream);
   if (e == hipSuccess) e = hipMemsetAsync(p->ctr, 0, kCounters * 4, p->stream);
@@ -954,6 +957,7 @@ mpm_status add_scene(mpm_pool* p, const int32_t dims[3], int32_t ppc_axis, float
   HIP_TRY(launch_block_info(p->block_info, geo.block_base, sc.nblocks, s, (uint32_t)geo.nbx, (uint32_t)geo.nby, p->stream));
   HIP_TRY(launch_fill_u32(p->stamp + geo.block_base, 0xffffffffu, sc.nblocks, p->stream));
   HIP_TRY(launch_fill_u32(p->sstamp + geo.block_base, 0xffffffffu, sc.nblocks, p->stream));
+  HIP_TRY(launch_fill_u32(p->vstamp + geo.block_base, 0xffffffffu, sc.nblocks, p->stream));
   *id = s;
   return MPM_OK;
 }
@@ -1298,7 +1302,7 @@ mpm_status mpm_p2g(mpm_sim* s, int32_t force_global) {
   mpm_pool* p = s->pool;
   ST_TRY(do_p2g(p, force_global != 0));
   if (p->grid_state == kGridTiles) {  // fold the tiles into A so the grid reads as the i32 P2G result
-    HIP_TRY(launch_resolve_tiles(view(p), p->stream));
+    HIP_TRY(launch_resolve_tiles(view(p), p->drift, p->stream));
     p->grid_state = kGridA;
     p->a_dirty_s = true;
   }
@@ -1817,11 +1821,13 @@ mpm_status mpm_pool_autotune(mpm_pool* p, uint32_t trial_substeps, int32_t* g2p_
   best = 1e30f;
   int bd = p->drift;
   for (int d = 2; d <= 3 && st == MPM_OK; d++) {
+    if (p->drift != d) p->bin_valid = false;  // S and the tiles' reach depend on the window
     p->drift = d;
     float ms = 0;
     st = time_trial(&ms);
     if (st == MPM_OK && ms < best) { best = ms; bd = d; }
   }
+  if (p->drift != bd) p->bin_valid = false;
   p->drift = bd;
   best = 1e30f;
   int bg = p->g2p_waves, bp = p->p2g_waves;
@@ -1964,6 +1970,7 @@ mpm_status mpm_pool_set_tuning(mpm_pool* p, int32_t knob, int32_t value) {
       return MPM_OK;
     case MPM_TUNE_DRIFT:
       if (value < 2 || value > 3) break;
+      if (p->drift != value) p->bin_valid = false;  // S and the tiles' reach depend on the window
       p->drift = value;
       drop_graphs(p);
       return MPM_OK;

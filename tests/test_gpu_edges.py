@@ -109,6 +109,28 @@ def test_sparse_items_take_arena_space_by_atomics(ctx):
     sim.destroy()
 
 
+def test_window_changes_between_calls(ctx):
+    """The P2G window (MPM_TUNE_DRIFT) switched between calls, mid sort period: S and the grid update's
+    candidate tiles follow the window (a drift-3 tile reaches one node block further than a drift-2
+    one), so a switch rebuilds the work items and S before the next P2G. Particles and grid equal the
+    oracle's after every call."""
+    print("[gpu] window changes between calls", flush=True)
+    sim = MPM(ctx, grid=40, maxParticles=120000, substeps=8)
+    sim.add({"box": [[0.1, 0.1, 0.1], [0.45, 0.45, 0.45]]}, material="water", velocity=[0.8, -0.5, 0.3], jitter=1.0)
+    sim.add({"box": [[0.55, 0.2, 0.55], [0.9, 0.55, 0.9]]}, material="sand", velocity=[-0.2, -2.0, 0.1], jitter=1.0)
+    rec = sim.read_particles()
+    sc = _oracle_of(sim, rec)
+    g = sim.lattice_gravity()
+    L = load_library()
+    for k, drift in ((6, 2), (5, 3), (7, 2), (3, 3)):
+        sim.set_tuning(drift=drift)
+        assert L.mpm_step(sim._h, k, _ptr(g), 0.4, 0.9) == 0
+        sc.step(k, g)
+        _same(sim.read_particles(), sc.read(len(rec)), f"particles after {k} substeps at drift {drift}")
+        _same(sim.read_grid(), sc.grid(), f"grid after {k} substeps at drift {drift}")
+    sim.destroy()
+
+
 def test_sort_every_substep_and_full_buffer(ctx):
     print("[gpu] sortEvery 1, buffer exactly full", flush=True)
     probe = MPM(ctx, grid=32, maxParticles=100000, substeps=8)
@@ -151,8 +173,8 @@ def test_kernel_variants_give_the_same_bits(ctx, items, fuse, shape, drift):
     (fuse), on and off, both workgroup shapes of the fused kernel (fuse_shape: 256 threads at 4
     waves per SIMD, 192 at 3) and both P2G tile shapes (drift 2: a 10^3-node window, 3: 12^3): 14
     substeps in one call (3 sorts, 9 fused transitions when both are on) of four materials, then 2
-    more calls of 3 substeps (fusion never crosses a call), bit-identical to the oracle in every
-    combination (fuse needs g2p_items; (0, 1) runs unfused)."""
+    more calls of 3 substeps (each takes up the P2G the call before carried forward, with fusion on),
+    bit-identical to the oracle in every combination (fuse needs g2p_items; (0, 1) runs unfused)."""
     print(f"[gpu] kernel variants g2p_items={items} fuse={fuse} fuse_shape={shape} drift={drift}", flush=True)
     sim = MPM(ctx, grid=40, maxParticles=120000, substeps=8)
     sim.set_tuning(g2p_items=items, fuse=fuse, fuse_shape=shape, drift=drift)

@@ -292,8 +292,9 @@ def test_p2g_after_write_without_sort(ctx):
 def test_split_calls_match_one_run(ctx, sort_every):
     """Substeps split over calls of 1-5 (calls that start on a sort, which open with the sort's P2G
     over the previous work items, mid-period and right after one) give the oracle's bits of one
-    uninterrupted run; so does a call that starts on a sort after the particles were rewritten (the
-    write drops the work items, so that call sorts first)."""
+    uninterrupted run, particles and the grid after every call (also after a call whose last substep
+    sorts, whose grid update ran before the sort); so does a call that starts on a sort after the
+    particles were rewritten (the write drops the work items, so that call sorts first)."""
     _say(f"[gpu] split calls vs one oracle run (sort every {sort_every})")
     sim = MPM(ctx, grid=32, maxParticles=40000, substeps=8, sortEvery=sort_every)
     sim.add({"box": [[0.15, 0.15, 0.15], [0.45, 0.5, 0.45]]}, material="snow", velocity=[0.3, -0.2, 0.1], noise=0.2)
@@ -307,7 +308,11 @@ def test_split_calls_match_one_run(ctx, sort_every):
     for calls in ([1, 3, 4, 5, 2, 1], [4, 4, 8]):
         for k in calls:
             assert L.mpm_step(sim._h, k, _ptr(g), 0.4, 0.9) == 0
-        sc.step(sum(calls), g)
+            sc.step(k, g)
+            # the grid after each call: a call whose last substep sorts ran that substep's grid update
+            # before the sort, over the previous work items' S
+            gg, gw = sim.read_grid(), sc.grid()
+            assert np.array_equal(gg, gw), f"grid after a call of {k} in {calls}: {_first_diff(gg, gw)}"
         got, want = sim.read_particles(), sc.read(n)
         assert np.array_equal(bits(got), bits(want)), f"calls {calls}: {_first_diff(got, want)}"
     # rewrite (same order): shift every particle by up to 3 cells, then step from a sort
