@@ -418,3 +418,79 @@ def test_changes_between_calls_discard_the_carried_p2g(ctx):
     sc.p2g()
     frame("an explicit P2G")
     sim.destroy()
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_random_call_sequences_match_the_oracle(ctx, seed):
+    """A seeded sequence of 40 calls mixing steps of 1-9 substeps with everything a caller may do
+    between them: particle and grid reads, particle rewrites, a new material table, a retime, a new
+    step count or sort period, an explicit sort or P2G, and result-neutral kernel variants (window,
+    workgroup shape, fusion, items, the carried P2G). Particles equal the oracle's after every call,
+    and the grid after every read."""
+    _say(f"[gpu] random call sequence {seed} vs oracle")
+    sim = _three_materials(ctx)
+    raw = sim.read_particles()
+    sc = _mirror(sim, raw)
+    g = sim.lattice_gravity()
+    from shaderintel_amd.mpm import load_library, _ptr
+    L = load_library()
+    n = len(raw)
+    rng = np.random.default_rng(seed)
+    ops = ["step"] * 10 + ["grid", "write", "mats", "retime", "count", "every", "sort", "p2g", "tune", "tune"]
+    log = []
+    for _ in range(40):
+        op = ops[rng.integers(len(ops))]
+        if op == "step":
+            k = int(rng.integers(1, 10))
+            assert L.mpm_step(sim._h, k, _ptr(g), 0.4, 0.9) == 0
+            sc.step(k, g)
+            log.append(f"step {k}")
+        elif op == "grid":
+            gg, gw = sim.read_grid(), sc.grid()
+            assert np.array_equal(gg, gw), f"grid after {log[-4:]}: {_first_diff(gg, gw)}"
+            log.append("grid")
+        elif op == "write":
+            rec = sim.read_particles()
+            rec[:, 4:7] += ((rng.random((n, 3)) - 0.5) * 0.1).astype(np.float32)
+            sim.write_particles(rec)
+            sc.write(rec)
+            log.append("write")
+        elif op == "mats":
+            tbl = sim.material_table()
+            tbl[:3, 1] *= np.float32(rng.uniform(0.8, 1.25))
+            assert L.mpm_set_materials(sim._h, _ptr(tbl), 8) == 0
+            sc.set_materials(tbl)
+            log.append("mats")
+        elif op == "retime":
+            k = float(np.float32(rng.uniform(0.8, 1.2)))
+            assert L.mpm_rescale(sim._h, k) == 0
+            sc.rescale(k)
+            log.append("retime")
+        elif op == "count":
+            c = sim.stepCount + int(rng.integers(1, 4))
+            sim.stepCount = c
+            sc.step_count = c
+            log.append(f"stepCount {c}")
+        elif op == "every":
+            e = int(rng.integers(1, 6))
+            assert L.mpm_set_sort_every(sim._h, e) == 0
+            sc.set_sort_every(e)
+            log.append(f"sortEvery {e}")
+        elif op == "sort":
+            sim._sort()
+            sc.sort()
+            log.append("sort")
+        elif op == "p2g":
+            sim.p2g()
+            sc.p2g()
+            gg, gw = sim.read_grid(), sc.grid()
+            assert np.array_equal(gg, gw), f"P2G grid after {log[-4:]}: {_first_diff(gg, gw)}"
+            log.append("p2g")
+        else:
+            knob = ["drift", "fuse_shape", "fuse", "g2p_items", "carry"][rng.integers(5)]
+            v = int(rng.integers(2, 4)) if knob == "drift" else int(rng.integers(0, 2))
+            sim.set_tuning(**{knob: v})
+            log.append(f"{knob}={v}")
+        got, want = sim.read_particles(), sc.read(n)
+        assert np.array_equal(bits(got), bits(want)), f"after {log[-4:]}: {_first_diff(got, want)}"
+    sim.destroy()
