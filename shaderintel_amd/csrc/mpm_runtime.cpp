@@ -796,8 +796,9 @@ mpm_status pool_step(mpm_pool* p, uint32_t substeps, float vmax) {
     // a sort period from a captured graph
     if (p->use_graphs && !p->profiling && !p->hook && at_sort && substeps - k >= p->sort_every) {
       // the period's last G2P does the next sort's P2G when another substep follows in this call (or
-      // the next call's, carried)
-      const bool exit_ahead = substeps - k > p->sort_every || carry_out;
+      // the next call's, carried), if G2P fuses at all (with MPM_TUNE_FUSE or MPM_TUNE_G2P_ITEMS off it
+      // does not, and the next substep runs its own P2G)
+      const bool exit_ahead = (substeps - k > p->sort_every || carry_out) && p->fuse && p->g2p_items;
       if (!p->p2g_presort) {  // (otherwise A holds part of the P2G done ahead, and its keys are set)
         ST_TRY(clean_a(p));
         ST_TRY(apply_pending_reorder(p));

@@ -117,3 +117,97 @@ def test_pool_calls_with_reads_and_scene_changes_between(ctx):
     call("after a removed scene")
     call("the call after it")
     pool.destroy()
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_random_pool_sequences_match_each_scene_alone(ctx, seed):
+    """A seeded sequence of 30 pool calls: steps of 1-8 substeps, scene reads, scenes added and
+    removed, captured sort periods switched on and off, and result-neutral kernel variants (window,
+    workgroup shape, fusion, items, the carried P2G). After every step every live scene equals the oracle
+    running it alone (a scene added mid-period sorts on the pool's step count, so its oracle starts
+    from that count)."""
+    pool_scenes = sorted((scenes.host_blocks(edge, blocks) for edge, blocks in scenes.batched_blocks(seed, 64)),
+                         key=lambda b: len(b[2]))[:8]
+    total = sum(len(b[2]) for b in pool_scenes)
+    print(f"[gpu] random pool sequence {seed}: up to 8 scenes, {total} particles", flush=True)
+    # twice the node blocks of all eight scenes: room for a scene however the free runs are cut up
+    pool = ScenePool(ctx, total, 8, 2 * sum(node_blocks(b[0]) for b in pool_scenes), max(b[0][0] for b in pool_scenes))
+    rng = np.random.default_rng(seed)
+    live, spare = {}, list(range(len(pool_scenes)))
+    done = [0]  # substeps the pool has run: its step count, which sets when every scene sorts
+
+    def add():
+        i = spare.pop(int(rng.integers(len(spare))))
+        dims, tbl, rec, g = pool_scenes[i]
+        sid = pool.add_scene(dims, tbl, rec, g)
+        sc = O.Scene(dims, len(rec), 2, 4)
+        sc.set_materials(tbl)
+        sc.write(rec)
+        sc.set_count(len(rec))
+        sc.step_count = done[0]  # a scene joining mid-period sorts when the pool does
+        live[sid] = (sc, g, len(rec), i)
+        return f"add {i} as {sid}"
+
+    log = [add() for _ in range(4)]
+    ops = ["step"] * 8 + ["read", "add", "remove", "graphs", "tune"]
+    for _ in range(30):
+        op = ops[rng.integers(len(ops))]
+        if op == "step" and live:
+            k = int(rng.integers(1, 9))
+            pool.step(k)
+            done[0] += k
+            log.append(f"step {k}")
+            for sid, (sc, g, n, _i) in live.items():
+                sc.step(k, g)
+                got, want = pool.read_scene(sid), sc.read(n)
+                assert np.array_equal(bits(got), bits(want)), f"after {log}, scene {sid}: {_first_diff(got, want)}"
+        elif op == "read" and live:
+            sid = list(live)[int(rng.integers(len(live)))]
+            pool.read_scene(sid)
+            log.append(f"read {sid}")
+        elif op == "add" and spare:
+            log.append(add())
+        elif op == "remove" and len(live) > 1:
+            sid = list(live)[int(rng.integers(len(live)))]
+            pool.remove_scene(sid)
+            spare.append(live.pop(sid)[3])
+            log.append(f"remove {sid}")
+        elif op == "graphs":
+            on = bool(rng.integers(0, 2))
+            pool.use_graphs(on)
+            log.append(f"graphs {on}")
+        elif op == "tune":
+            knob = ["drift", "fuse_shape", "fuse", "g2p_items", "carry"][rng.integers(5)]
+            v = int(rng.integers(2, 4)) if knob == "drift" else int(rng.integers(0, 2))
+            pool.set_tuning(**{knob: v})
+            log.append(f"{knob}={v}")
+    pool.destroy()
+
+
+@pytest.mark.parametrize("knob", ["fuse", "g2p_items"])
+def test_pool_captured_periods_without_fusion(ctx, knob):
+    """Captured sort periods with G2P fusion off (MPM_TUNE_FUSE or MPM_TUNE_G2P_ITEMS 0) in a call
+    that goes on after the period: the period's last G2P then does not do the next sort's P2G, and
+    the substep after the period runs its own (an earlier version took that P2G as done). Every scene
+    equals the oracle running it alone after each call."""
+    built = sorted((scenes.host_blocks(e, b) for e, b in scenes.batched_blocks(19, 64)), key=lambda b: len(b[2]))[:4]
+    total = sum(len(b[2]) for b in built)
+    print(f"[gpu] captured periods with {knob} = 0: 4 scenes, {total} particles", flush=True)
+    pool = ScenePool(ctx, total, 4, sum(node_blocks(b[0]) for b in built), max(b[0][0] for b in built))
+    pool.set_tuning(**{knob: 0})
+    pool.use_graphs(True)
+    live = {}
+    for dims, tbl, rec, g in built:
+        sid = pool.add_scene(dims, tbl, rec, g)
+        sc = O.Scene(dims, len(rec), 2, 4)
+        sc.set_materials(tbl)
+        sc.write(rec)
+        sc.set_count(len(rec))
+        live[sid] = (sc, g, len(rec))
+    for k in (6, 9, 4, 5):
+        pool.step(k)
+        for sid, (sc, g, n) in live.items():
+            sc.step(k, g)
+            got, want = pool.read_scene(sid), sc.read(n)
+            assert np.array_equal(bits(got), bits(want)), f"call of {k}, scene {sid}: {_first_diff(got, want)}"
+    pool.destroy()
