@@ -424,8 +424,9 @@ def test_changes_between_calls_discard_the_carried_p2g(ctx):
 def test_random_call_sequences_match_the_oracle(ctx, seed):
     """A seeded sequence of 40 calls mixing steps of 1-9 substeps with everything a caller may do
     between them: particle and grid reads, particle rewrites, a new material table, a retime, a new
-    step count or sort period, an explicit sort or P2G, and result-neutral kernel variants (window,
-    workgroup shape, fusion, items, the carried P2G). Particles equal the oracle's after every call,
+    step count or sort period, an explicit sort or P2G, a substep split at its grid update with the
+    grid edited in between, and result-neutral kernel variants (window, workgroup shape, fusion,
+    items, the carried P2G). Particles equal the oracle's after every call,
     and the grid after every read."""
     _say(f"[gpu] random call sequence {seed} vs oracle")
     sim = _three_materials(ctx)
@@ -436,8 +437,9 @@ def test_random_call_sequences_match_the_oracle(ctx, seed):
     L = load_library()
     n = len(raw)
     rng = np.random.default_rng(seed)
-    ops = ["step"] * 10 + ["grid", "write", "mats", "retime", "count", "every", "sort", "p2g", "tune", "tune"]
+    ops = ["step"] * 10 + ["grid", "write", "mats", "retime", "count", "every", "sort", "p2g", "tune", "tune", "split"]
     log = []
+    every = [4]
     for _ in range(40):
         op = ops[rng.integers(len(ops))]
         if op == "step":
@@ -475,11 +477,29 @@ def test_random_call_sequences_match_the_oracle(ctx, seed):
             e = int(rng.integers(1, 6))
             assert L.mpm_set_sort_every(sim._h, e) == 0
             sc.set_sort_every(e)
+            every[0] = e
             log.append(f"sortEvery {e}")
         elif op == "sort":
             sim._sort()
             sc.sort()
             log.append("sort")
+        elif op == "split":  # a substep split at its grid update, the grid edited in between
+            sim.substep_begin()
+            words = sim.read_grid()
+            if sc.step_count % every[0] == 0:
+                sc.sort()
+            sc.p2g()
+            sc.grid_update(g, 0.4)
+            assert np.array_equal(words, sc.grid()), f"split-substep grid after {log[-4:]}"
+            v = words.view(np.float32).reshape(-1, 4).copy()
+            v[::3, :3] *= np.float32(0.5)
+            edited = v.view(np.int32)
+            sim.write_grid(edited)
+            sim.substep_end()
+            sc.write_grid(edited)
+            sc.g2p(0.9)
+            sc.step_count = sc.step_count + 1
+            log.append("split")
         elif op == "p2g":
             sim.p2g()
             sc.p2g()
