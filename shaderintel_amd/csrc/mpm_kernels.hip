@@ -863,17 +863,22 @@ __device__ __forceinline__ unsigned long long occ_word(const Pool& pool, uint32_
   if (by < 0 || bz < 0 || by >= (int)pb || bz >= (int)pb || w < 0 || w >= (int)W) return 0ull;
   return pool.occ[((size_t)scene * pb * pb + (size_t)bz * pb + (size_t)by) * W + (size_t)w];
 }
+// kDilateSplit threads per word: each computes the word (the same loads) and writes the S entries
+// of its 64 / kDilateSplit bits, so a dense row's 64 entries are written by several threads.
+constexpr int kDilateSplit = 8;
 __global__ __launch_bounds__(kThreads) void k_bin_dilate(Pool pool, uint32_t max_scenes, int drift) {
   const uint32_t pb = 1u << (pool.key_bits / 3 - 2), W = occ_row_words(pb);
   const uint32_t epoch = pool.ctr[kCtrBinEpoch];
   const int lane = threadIdx.x & 63;
-  const uint64_t total = (uint64_t)max_scenes * pb * pb * W;
+  const uint64_t total = (uint64_t)max_scenes * pb * pb * W * kDilateSplit;
   for (uint64_t t0 = (uint64_t)blockIdx.x * kThreads; t0 < total; t0 += (uint64_t)gridDim.x * kThreads) {
-    const uint64_t t = t0 + threadIdx.x;  // the loop bound is uniform over the block
+    const uint64_t ts = t0 + threadIdx.x;  // the loop bound is uniform over the block
+    const uint64_t t = ts / kDilateSplit;  // the word
+    const int part = (int)(ts % kDilateSplit);
     unsigned long long d = 0;
     uint32_t scene = 0, base = 0;
     int w = 0, by = 0, bz = 0, nbx = 0, nby = 0;
-    if (t < total) {
+    if (ts < total) {
       w = (int)(t % W);
       by = (int)((t / W) % pb);
       bz = (int)((t / ((uint64_t)W * pb)) % pb);
@@ -893,6 +898,8 @@ __global__ __launch_bounds__(kThreads) void k_bin_dilate(Pool pool, uint32_t max
           }
         const int valid = geo.nbx - 64 * w;  // bits of this word inside the grid
         if (valid < 64) d &= (1ull << valid) - 1ull;
+        constexpr int kBits = 64 / kDilateSplit;  // this thread's bits of the word
+        d &= ((1ull << kBits) - 1ull) << (kBits * part);
       }
     }
     // wave prefix of the counts, one atomic per wave
@@ -1572,7 +1579,7 @@ hipError_t launch_bin_transpose(const Pool& pool, const uint32_t* keys, const ui
 }
 hipError_t launch_bin_dilate(const Pool& pool, uint32_t max_scenes, int drift, hipStream_t s) {
   const uint32_t pb = 1u << (pool.key_bits / 3 - 2);
-  const uint64_t total = (uint64_t)max_scenes * pb * pb * occ_row_words(pb);
+  const uint64_t total = (uint64_t)max_scenes * pb * pb * occ_row_words(pb) * kDilateSplit;
   hipLaunchKernelGGL(k_bin_dilate, dim3((unsigned)std::min<uint64_t>((total + kThreads - 1) / kThreads, 8192)),
                      dim3(kThreads), 0, s, pool, max_scenes, drift);
   return hipGetLastError();
