@@ -1438,11 +1438,14 @@ __global__ __launch_bounds__(kThreads) void k_grid_pull(Pool pool, int use_s) {
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_clear_a(Pool pool, int use_s) {
-  const uint32_t ns = use_s ? pool.ctr[kCtrS] : 0u, nd = pool.ctr[kCtrD];
+// s_stamped: an S block is cleared only if the last P2G wrote it through A (stamp = its epoch): a
+// binned P2G's fallback writes to blocks of S go to A without a D entry.
+__global__ __launch_bounds__(kThreads) void k_clear_a(Pool pool, int use_s, int s_stamped) {
+  const uint32_t ns = use_s ? pool.ctr[kCtrS] : 0u, nd = pool.ctr[kCtrD], epoch = pool.ctr[kCtrEpoch];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (uint32_t j = blockIdx.x * 4 + w; j < ns + nd; j += gridDim.x * 4) {
     const uint32_t b = j < ns ? pool.slist[j] : pool.dlist[j - ns];
+    if (j < ns && s_stamped && pool.stamp[b] != epoch) continue;
     pool.A[(b << 6) | (uint32_t)lane] = make_int4(0, 0, 0, 0);
   }
 }
@@ -1624,9 +1627,9 @@ hipError_t launch_resolve_tiles(const Pool& pool, int drift, hipStream_t s) {
   else hipLaunchKernelGGL((k_grid_pull<false, 3>), grid, dim3(kThreads), 0, s, pool, 1);
   return hipGetLastError();
 }
-hipError_t launch_clear_a(const Pool& pool, bool use_s, hipStream_t s) {
+hipError_t launch_clear_a(const Pool& pool, bool use_s, bool s_stamped, hipStream_t s) {
   hipLaunchKernelGGL(k_clear_a, dim3(persistent_grid(2 * pool.nblocks, 4, 8192)), dim3(kThreads), 0, s, pool,
-                     use_s ? 1 : 0);
+                     use_s ? 1 : 0, s_stamped ? 1 : 0);
   return hipGetLastError();
 }
 // G2P launches; with ev_start / ev_stop (either may be null) the dispatch itself records them

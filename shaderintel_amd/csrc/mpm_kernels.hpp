@@ -216,8 +216,8 @@ hipError_t launch_write_words(uint32_t* dst, const uint32_t* src, uint32_t n, hi
 hipError_t launch_grid_update(const Pool& pool, bool use_s, int drift, hipStream_t s);
 // Folds the tiles into A over S (A then holds the whole P2G result, for reads).
 hipError_t launch_resolve_tiles(const Pool& pool, int drift, hipStream_t s);
-// Zeroes A over S (use_s) and D.
-hipError_t launch_clear_a(const Pool& pool, bool use_s, hipStream_t s);
+// Zeroes A over S (use_s; with s_stamped only the S blocks the last P2G wrote through A) and D.
+hipError_t launch_clear_a(const Pool& pool, bool use_s, bool s_stamped, hipStream_t s);
 // G2P; ev_start / ev_stop (optional) are recorded by the dispatch itself at the kernel's start / end.
 hipError_t launch_g2p(const Pool& pool, float4* P, uint32_t n, float vmax, int waves, hipStream_t s,
                       hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);

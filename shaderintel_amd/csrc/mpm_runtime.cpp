@@ -416,9 +416,13 @@ void recompute_offsets(mpm_pool* p) {
   layout_offsets(p);
 }
 
-// Return A to all-zero where the last P2G left data (before S or D change).
+// Return A to all-zero where the last P2G left data (before S or D change, and before the next P2G's
+// epoch): a P2G's writes through A are in the blocks of D and, for a binned P2G, in blocks of S that
+// carry its epoch's stamp (S blocks are not listed in D); after a fold of the tiles (a_dirty_s) A
+// holds data over all of S. A P2G whose grid update never runs (a carried P2G dropped by a change
+// between calls, a P2G read out by mpm_p2g) leaves such data behind.
 mpm_status clean_a(mpm_pool* p) {
-  if (p->a_dirty_s || p->a_dirty_d) HIP_TRY(launch_clear_a(view(p), p->a_dirty_s, p->stream));
+  if (p->a_dirty_s || p->a_dirty_d) HIP_TRY(launch_clear_a(view(p), true, !p->a_dirty_s, p->stream));
   p->a_dirty_d = p->a_dirty_s = false;
   return MPM_OK;
 }

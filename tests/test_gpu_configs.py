@@ -122,8 +122,9 @@ def test_pool_calls_with_reads_and_scene_changes_between(ctx):
 @pytest.mark.parametrize("seed", [1, 2])
 def test_random_pool_sequences_match_each_scene_alone(ctx, seed):
     """A seeded sequence of 30 pool calls: steps of 1-8 substeps, scene reads, scenes added and
-    removed, captured sort periods switched on and off, and result-neutral kernel variants (window,
-    workgroup shape, fusion, items, the carried P2G). After every step every live scene equals the oracle
+    removed (host records or device templates), captured sort periods switched on and off, the
+    autotuner's trials, and result-neutral kernel variants (window, workgroup shape, occupancy,
+    fusion, items, the carried P2G). After every step every live scene equals the oracle
     running it alone (a scene added mid-period sorts on the pool's step count, so its oracle starts
     from that count)."""
     pool_scenes = sorted((scenes.host_blocks(edge, blocks) for edge, blocks in scenes.batched_blocks(seed, 64)),
@@ -135,21 +136,23 @@ def test_random_pool_sequences_match_each_scene_alone(ctx, seed):
     rng = np.random.default_rng(seed)
     live, spare = {}, list(range(len(pool_scenes)))
     done = [0]  # substeps the pool has run: its step count, which sets when every scene sorts
+    templates = [pool.add_template(*b) for b in pool_scenes]  # device copies for admission
 
     def add():
         i = spare.pop(int(rng.integers(len(spare))))
         dims, tbl, rec, g = pool_scenes[i]
-        sid = pool.add_scene(dims, tbl, rec, g)
+        from_tmpl = bool(rng.integers(0, 2))
+        sid = pool.add_from_template(templates[i]) if from_tmpl else pool.add_scene(dims, tbl, rec, g)
         sc = O.Scene(dims, len(rec), 2, 4)
         sc.set_materials(tbl)
         sc.write(rec)
         sc.set_count(len(rec))
         sc.step_count = done[0]  # a scene joining mid-period sorts when the pool does
         live[sid] = (sc, g, len(rec), i)
-        return f"add {i} as {sid}"
+        return f"add {i} as {sid}" + (" (template)" if from_tmpl else "")
 
     log = [add() for _ in range(4)]
-    ops = ["step"] * 8 + ["read", "add", "remove", "graphs", "tune"]
+    ops = ["step"] * 8 + ["read", "add", "remove", "graphs", "tune", "autotune"]
     for _ in range(30):
         op = ops[rng.integers(len(ops))]
         if op == "step" and live:
@@ -172,13 +175,24 @@ def test_random_pool_sequences_match_each_scene_alone(ctx, seed):
             pool.remove_scene(sid)
             spare.append(live.pop(sid)[3])
             log.append(f"remove {sid}")
+        elif op == "autotune" and live:
+            # the trials advance the pool: to the next sort boundary, then 16 trials of 4 substeps
+            k = (-done[0]) % 4 + 16 * 4
+            t = pool.autotune(4)
+            done[0] += k
+            log.append(f"autotune ({k} substeps) -> {t}")
+            for sid, (sc, g, n, _i) in live.items():
+                sc.step(k, g)
+                got, want = pool.read_scene(sid), sc.read(n)
+                assert np.array_equal(bits(got), bits(want)), f"after {log}, scene {sid}: {_first_diff(got, want)}"
         elif op == "graphs":
             on = bool(rng.integers(0, 2))
             pool.use_graphs(on)
             log.append(f"graphs {on}")
         elif op == "tune":
-            knob = ["drift", "fuse_shape", "fuse", "g2p_items", "carry"][rng.integers(5)]
-            v = int(rng.integers(2, 4)) if knob == "drift" else int(rng.integers(0, 2))
+            knob = ["drift", "fuse_shape", "fuse", "g2p_items", "carry", "g2p_waves", "p2g_waves"][rng.integers(7)]
+            lo, hi = {"drift": (2, 3), "g2p_waves": (3, 6), "p2g_waves": (4, 6)}.get(knob, (0, 1))
+            v = int(rng.integers(lo, hi + 1))
             pool.set_tuning(**{knob: v})
             log.append(f"{knob}={v}")
     pool.destroy()

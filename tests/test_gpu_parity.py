@@ -371,6 +371,40 @@ def test_frames_with_reads_between_match_one_run(ctx, carry):
     sim.destroy()
 
 
+def test_dropped_carried_p2g_leaves_no_sums_in_a(ctx):
+    """Particles moving together at 0.88 cells per substep leave their work items' windows (drift 2) within a sort
+    period, so the P2G a call carries forward at a sort boundary sends many of them through A, also
+    into blocks of S (which it stamps but does not list in D). A change between the calls drops that
+    P2G; the next call's P2G must find none of its sums left in A (an earlier version cleared only
+    the D blocks, and the next grid update over a restamped block counted them twice): the next
+    call's grid bookkeeping (mass, momentum) and particles equal the oracle's."""
+    _say("[gpu] dropped carried P2G with fast particles vs oracle")
+    sim = _three_materials(ctx)
+    sim.set_tuning(drift=2)
+    raw = sim.read_particles()
+    n = len(raw)
+    raw[:, 4:7] = np.array([0.8, -0.3, 0.2], np.float32)  # one velocity: the grid keeps it
+    sim.write_particles(raw)
+    sc = _mirror(sim, raw)
+    g = sim.lattice_gravity()
+    from shaderintel_amd.mpm import load_library, _ptr
+    L = load_library()
+    for _ in range(2):  # the second call carries its last P2G (the next sort's) forward
+        assert L.mpm_step(sim._h, 4, _ptr(g), 0.4, 0.9) == 0
+        sc.step(4, g)
+    for label, change in (("a knob set", lambda: sim.set_tuning(carry=1)), ("a drift change", lambda: sim.set_tuning(drift=3))):
+        sim.set_grid_law(True)
+        sc.set_grid_law(True)
+        change()  # drops the carried P2G
+        for _ in range(2):
+            assert L.mpm_step(sim._h, 4, _ptr(g), 0.4, 0.9) == 0
+            sc.step(4, g)
+        assert sim.read_grid_law()["raw"] == [int(x) for x in sc.read_grid_law()[:14].astype(np.int64)], label
+        got, want = sim.read_particles(), sc.read(n)
+        assert np.array_equal(bits(got), bits(want)), f"after {label}: {_first_diff(got, want)}"
+    sim.destroy()
+
+
 def test_changes_between_calls_discard_the_carried_p2g(ctx):
     """Between calls that carry a P2G forward: a rewrite of the particles, a new material table, a
     retime, a new step count, an explicit sort and an explicit P2G. Each discards the carried P2G
