@@ -119,16 +119,20 @@ def test_pool_calls_with_reads_and_scene_changes_between(ctx):
     pool.destroy()
 
 
-@pytest.mark.parametrize("seed", [1, 2])
-def test_random_pool_sequences_match_each_scene_alone(ctx, seed):
+@pytest.mark.parametrize("seed,fast", [(1, False), (2, False), (3, True)])
+def test_random_pool_sequences_match_each_scene_alone(ctx, seed, fast):
     """A seeded sequence of 30 pool calls: steps of 1-8 substeps, scene reads, scenes added and
     removed (host records or device templates), captured sort periods switched on and off, the
     autotuner's trials, and result-neutral kernel variants (window, workgroup shape, occupancy,
     fusion, items, the carried P2G). After every step every live scene equals the oracle
     running it alone (a scene added mid-period sorts on the pool's step count, so its oracle starts
-    from that count)."""
+    from that count). fast: every particle starts at 0.88 cells per substep in one direction (P2G's
+    fallback within each sort period)."""
     pool_scenes = sorted((scenes.host_blocks(edge, blocks) for edge, blocks in scenes.batched_blocks(seed, 64)),
                          key=lambda b: len(b[2]))[:8]
+    if fast:
+        for _d, _t, rec, _g in pool_scenes:
+            rec[:, 4:7] = np.array([0.8, -0.3, 0.2], np.float32)
     total = sum(len(b[2]) for b in pool_scenes)
     print(f"[gpu] random pool sequence {seed}: up to 8 scenes, {total} particles", flush=True)
     # twice the node blocks of all eight scenes: room for a scene however the free runs are cut up
